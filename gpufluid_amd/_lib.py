@@ -1,0 +1,144 @@
+"""ctypes binding of libgpufluid_hip.so (include/gpufluid_hip.h).
+
+The library is the only compute path: if it is missing or fails to load,
+every entry point raises instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes
+import threading
+from ctypes import POINTER, c_double, c_float, c_int, c_int32, c_int64, c_uint8, c_void_p
+from pathlib import Path
+
+from .blocks import BlockError
+
+LIB_PATH = Path(__file__).resolve().with_name("libgpufluid_hip.so")
+
+GF_TRANSFER_FLIP = 0
+GF_TRANSFER_APIC = 1
+GF_PCG_MIXED = 0
+GF_PCG_FP64 = 1
+TILE = 8
+TILE_CELLS = 512
+
+_lock = threading.Lock()
+_lib = None
+
+
+class GfScene(ctypes.Structure):
+    """Mirror of gf_scene_t (include/gpufluid_hip.h)."""
+
+    _fields_ = [
+        ("nx", c_int32), ("ny", c_int32), ("nz", c_int32),
+        ("tnx", c_int32), ("tny", c_int32), ("tnz", c_int32),
+        ("dx", c_float), ("dom", c_float * 3),
+        ("n", c_int32), ("cap", c_int32),
+        ("pos", c_void_p), ("vel", c_void_p), ("affine_c", c_void_p), ("pid", c_void_p),
+        ("pos_b", c_void_p), ("vel_b", c_void_p), ("affine_c_b", c_void_p), ("pid_b", c_void_p),
+        ("keys", c_void_p), ("tmp", c_void_p), ("counts", c_void_p), ("cell_start", c_void_p),
+        ("scan_part", c_void_p),
+        ("tile_occ", c_void_p), ("tile_map", c_void_p), ("active", c_void_p),
+        ("n_active", c_void_p), ("nbr", c_void_p),
+        ("marker", c_void_p),
+        ("u", c_void_p), ("v", c_void_p), ("w", c_void_p),
+        ("us", c_void_p), ("vs", c_void_p), ("ws", c_void_p),
+        ("solid_u", c_void_p), ("solid_v", c_void_p), ("solid_w", c_void_p),
+        ("flags", c_void_p), ("b", c_void_p),
+        ("x32", c_void_p), ("r32", c_void_p), ("p32", c_void_p), ("ap32", c_void_p),
+        ("x64", c_void_p), ("r64", c_void_p), ("p64", c_void_p), ("ap64", c_void_p),
+        ("pc", c_void_p), ("ptmp", c_void_p),
+        ("partials", c_void_p), ("scalars", c_void_p),
+    ]
+
+
+class GfSizes(ctypes.Structure):
+    _fields_ = [("cells", c_int64), ("ntiles", c_int64), ("compact", c_int64),
+                ("scan_parts", c_int64), ("partials", c_int64), ("scalars", c_int64)]
+
+
+V, I, F, D = c_void_p, c_int, c_float, c_double
+PI, PD = POINTER(c_int), POINTER(c_double)
+SP = POINTER(GfScene)
+
+# name -> argtypes (all return int)
+SIGNATURES = {
+    "gf_version": [],
+    "gf_init": [I, PI],
+    "gf_scene_sizes": [I, I, I, POINTER(GfSizes)],
+    "gf_scene_sort_sync": [SP, I, PI, V],
+    "gf_scene_p2g": [SP, I, F, F, F, V],
+    "gf_scene_p2g_raw": [SP, I, V, V, V, V],
+    "gf_scene_jacobi": [SP, I, I, V],
+    "gf_scene_pcg_sync": [SP, I, I, D, I, I, I, PI, PD, PI, V],
+    "gf_scene_g2p": [SP, V, I, F, F, F, V],
+    "gf_scene_compact_to_dense": [SP, V, V, V],
+    "gf_scene_unsort": [SP, V, I, V, V],
+    "gf_k3_clear_grid": [V, V, V, V, V, V, V, I, I, I, V],
+    "gf_k3_mark_fluid": [V, I, V, F, I, I, I, V],
+    "gf_k3_normalize": [V, V, V, V, V, V, V, V, V, I, I, I, V],
+    "gf_k3_add_gravity": [V, I, I, I, F, F, V],
+    "gf_k3_enforce_solid_bc": [V, V, V, V, V, V, V, I, I, I, V],
+    "gf_k3_compute_divergence": [V, V, V, V, V, I, I, I, F, F, F, V],
+    "gf_k3_jacobi_pressure": [V, V, V, V, I, I, I, V],
+    "gf_k3_apply_A": [V, V, V, V, I, I, I, V],
+    "gf_k3_compute_diag": [V, V, I, I, I, V],
+    "gf_k3_apply_invM": [V, V, V, V, V, I, I, I, V],
+    "gf_k3_dot_fluid": [V, V, V, V, V, I, I, I, V],
+    "gf_k3_axpy_devscalar": [V, F, V, V, V, V, V, I, I, I, V],
+    "gf_k3_scalar_op": [I, V, V, V, F, V, V],
+    "gf_k3_subtract_pressure_grad": [V, V, V, V, V, I, I, I, F, F, F, V],
+    "gf_k3_g2p_and_advect": [V, V, V, V, V, V, V, V, I, F, F, I, I, I, F, POINTER(c_float), V],
+    "gf_k3_g2p_apic_advect": [V, V, V, V, V, V, I, F, F, I, I, I, POINTER(c_float), V],
+    "gf_k3_vmax_sq_reduce": [V, I, V, V],
+}
+
+
+def lib() -> ctypes.CDLL:
+    """Load the HIP library once; raise if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is None:
+            if not LIB_PATH.exists():
+                raise RuntimeError(
+                    f"{LIB_PATH.name} is not built: run `python -m gpufluid_amd.build` "
+                    "(the HIP library is the only compute path)")
+            h = ctypes.CDLL(str(LIB_PATH))
+            for name, args in SIGNATURES.items():
+                fn = getattr(h, name)
+                fn.argtypes = args
+                fn.restype = c_int
+            h.gf_last_error.argtypes = []
+            h.gf_last_error.restype = ctypes.c_char_p
+            _lib = h
+    return _lib
+
+
+def last_error() -> str:
+    return lib().gf_last_error().decode(errors="replace")
+
+
+def call(block_id: str, name: str, *args) -> None:
+    """Invoke an entry point; map a nonzero return to BlockError(block_id)."""
+    rc = getattr(lib(), name)(*args)
+    if rc != 0:
+        raise BlockError(block_id, f"{name} failed ({rc}): {last_error()}")
+
+
+def ptr(t) -> int | None:
+    """Device pointer of a torch tensor (None -> NULL)."""
+    if t is None:
+        return None
+    return t.data_ptr()
+
+
+def stream_handle(device=None) -> int:
+    import torch
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def sizes(nx: int, ny: int, nz: int) -> GfSizes:
+    s = GfSizes()
+    call("F3.2", "gf_scene_sizes", nx, ny, nz, ctypes.byref(s))
+    return s

@@ -1,0 +1,625 @@
+// Dense-grid replacements of the reference's individual S2 kernels, with the
+// reference argument meaning (sub-dense offsets fixed at 0).  They serve
+// callers that launch one block at a time (the S2 block interface); the
+// substep itself runs the fused tile kernels of gf_transfer.hip and
+// gf_pressure.hip.
+#include "gf_common.h"
+
+namespace {
+
+__device__ __forceinline__ bool idx3(int I, int J, int K, int& i, int& j, int& k) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)I * J * K) return false;
+    k = (int)(t % K);
+    j = (int)((t / K) % J);
+    i = (int)(t / ((long long)K * J));
+    return true;
+}
+
+inline dim3 grid_for(long long n) { return dim3((unsigned)((n + GF_WG - 1) / GF_WG)); }
+
+// solver3d.py:41-65
+__global__ void k_clear_grid(float* u, float* v, float* w, float* uw, float* vw, float* ww,
+                             int32_t* marker, int nx, int ny, int nz) {
+    int i, j, k;
+    if (!idx3(nx + 1, ny + 1, nz + 1, i, j, k)) return;
+    if (i < nx + 1 && j < ny && k < nz) {
+        size_t f = ((size_t)i * ny + j) * nz + k;
+        u[f] = 0.0f; uw[f] = 0.0f;
+    }
+    if (i < nx && j < ny + 1 && k < nz) {
+        size_t f = ((size_t)i * (ny + 1) + j) * nz + k;
+        v[f] = 0.0f; vw[f] = 0.0f;
+    }
+    if (i < nx && j < ny && k < nz + 1) {
+        size_t f = ((size_t)i * ny + j) * (nz + 1) + k;
+        w[f] = 0.0f; ww[f] = 0.0f;
+    }
+    if (i < nx && j < ny && k < nz) {
+        size_t c = ((size_t)i * ny + j) * nz + k;
+        if (marker[c] != 2) marker[c] = 0;
+    }
+}
+
+// solver3d.py:96-101
+__global__ void k_mark_fluid(const float* pos, int n, int32_t* marker, float dx, int nx, int ny,
+                             int nz) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    int ci = gf::bin_axis(pos[3 * p + 0], dx, nx);
+    int cj = gf::bin_axis(pos[3 * p + 1], dx, ny);
+    int ck = gf::bin_axis(pos[3 * p + 2], dx, nz);
+    size_t c = ((size_t)ci * ny + cj) * nz + ck;
+    if (marker[c] != 2) marker[c] = 1;
+}
+
+// solver3d.py:121-142
+__global__ void k_normalize(float* u, float* v, float* w, const float* uw, const float* vw,
+                            const float* ww, float* us, float* vs, float* ws, int nx, int ny,
+                            int nz) {
+    int i, j, k;
+    if (!idx3(nx + 1, ny + 1, nz + 1, i, j, k)) return;
+    if (i < nx + 1 && j < ny && k < nz) {
+        size_t f = ((size_t)i * ny + j) * nz + k;
+        float val = uw[f] > 1.0e-8f ? u[f] / uw[f] : 0.0f;
+        u[f] = val; us[f] = val;
+    }
+    if (i < nx && j < ny + 1 && k < nz) {
+        size_t f = ((size_t)i * (ny + 1) + j) * nz + k;
+        float val = vw[f] > 1.0e-8f ? v[f] / vw[f] : 0.0f;
+        v[f] = val; vs[f] = val;
+    }
+    if (i < nx && j < ny && k < nz + 1) {
+        size_t f = ((size_t)i * ny + j) * (nz + 1) + k;
+        float val = ww[f] > 1.0e-8f ? w[f] / ww[f] : 0.0f;
+        w[f] = val; ws[f] = val;
+    }
+}
+
+// solver3d.py:149-156
+__global__ void k_add_gravity(float* v, long long n, float g, float dt) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = v[i] + g * dt;
+}
+
+// solver3d.py:758-799
+__global__ void k_enforce_solid_bc(float* u, float* v, float* w, const int32_t* marker,
+                                   const float* su, const float* sv, const float* sw, int nx,
+                                   int ny, int nz) {
+    int i, j, k;
+    if (!idx3(nx + 1, ny + 1, nz + 1, i, j, k)) return;
+    if (i <= nx && j < ny && k < nz) {
+        bool left = (i == 0) || (i > 0 && marker[((size_t)(i - 1) * ny + j) * nz + k] == 2);
+        bool right = (i == nx) || (i < nx && marker[((size_t)i * ny + j) * nz + k] == 2);
+        if (left || right) {
+            size_t f = ((size_t)i * ny + j) * nz + k;
+            u[f] = su ? su[f] : 0.0f;
+        }
+    }
+    if (i < nx && j <= ny && k < nz) {
+        bool bot = (j == 0) || (j > 0 && marker[((size_t)i * ny + j - 1) * nz + k] == 2);
+        bool top = (j == ny) || (j < ny && marker[((size_t)i * ny + j) * nz + k] == 2);
+        if (bot || top) {
+            size_t f = ((size_t)i * (ny + 1) + j) * nz + k;
+            v[f] = sv ? sv[f] : 0.0f;
+        }
+    }
+    if (i < nx && j < ny && k <= nz) {
+        bool back = (k == 0) || (k > 0 && marker[((size_t)i * ny + j) * nz + k - 1] == 2);
+        bool front = (k == nz) || (k < nz && marker[((size_t)i * ny + j) * nz + k] == 2);
+        if (back || front) {
+            size_t f = ((size_t)i * ny + j) * (nz + 1) + k;
+            w[f] = sw ? sw[f] : 0.0f;
+        }
+    }
+}
+
+// solver3d.py:846-871
+__global__ void k_divergence(const float* u, const float* v, const float* w, float* div,
+                             const int32_t* marker, int nx, int ny, int nz, float dx, float dt,
+                             float rho) {
+    int i, j, k;
+    if (!idx3(nx, ny, nz, i, j, k)) return;
+    size_t c = ((size_t)i * ny + j) * nz + k;
+    if (marker[c] != 1) { div[c] = 0.0f; return; }
+    float u0 = u[((size_t)i * ny + j) * nz + k], u1 = u[((size_t)(i + 1) * ny + j) * nz + k];
+    float v0 = v[((size_t)i * (ny + 1) + j) * nz + k], v1 = v[((size_t)i * (ny + 1) + j + 1) * nz + k];
+    float w0 = w[((size_t)i * ny + j) * (nz + 1) + k], w1 = w[((size_t)i * ny + j) * (nz + 1) + k + 1];
+    float d = u1 - u0 + v1 - v0 + w1 - w0;
+    div[c] = d * (rho * dx / dt);
+}
+
+// Neighbour pass shared by the Jacobi sweep and A.x (solver3d.py:878-938, 1743-1797).
+__device__ __forceinline__ void stencil(const float* x, const int32_t* marker, int i, int j,
+                                        int k, int nx, int ny, int nz, float& sum_nb,
+                                        float& diag) {
+    const size_t c = ((size_t)i * ny + j) * nz + k;
+    const size_t sx = (size_t)ny * nz, sy = nz;
+    sum_nb = 0.0f; diag = 0.0f;
+    int m;
+    if (i > 0) { m = marker[c - sx]; if (m != 2) { diag += 1.0f; if (m == 1) sum_nb += x[c - sx]; } }
+    if (i < nx - 1) { m = marker[c + sx]; if (m != 2) { diag += 1.0f; if (m == 1) sum_nb += x[c + sx]; } }
+    if (j > 0) { m = marker[c - sy]; if (m != 2) { diag += 1.0f; if (m == 1) sum_nb += x[c - sy]; } }
+    if (j < ny - 1) { m = marker[c + sy]; if (m != 2) { diag += 1.0f; if (m == 1) sum_nb += x[c + sy]; } }
+    if (k > 0) { m = marker[c - 1]; if (m != 2) { diag += 1.0f; if (m == 1) sum_nb += x[c - 1]; } }
+    if (k < nz - 1) { m = marker[c + 1]; if (m != 2) { diag += 1.0f; if (m == 1) sum_nb += x[c + 1]; } }
+}
+
+__global__ void k_jacobi_dense(const float* p_in, float* p_out, const float* div,
+                               const int32_t* marker, int nx, int ny, int nz) {
+    int i, j, k;
+    if (!idx3(nx, ny, nz, i, j, k)) return;
+    size_t c = ((size_t)i * ny + j) * nz + k;
+    if (marker[c] != 1) { p_out[c] = 0.0f; return; }
+    float s, d;
+    stencil(p_in, marker, i, j, k, nx, ny, nz, s, d);
+    if (d < 0.5f) { p_out[c] = 0.0f; return; }
+    p_out[c] = (s - div[c]) / d;
+}
+
+__global__ void k_apply_A_dense(const float* x, float* y, const int32_t* marker,
+                                const int32_t* done, int nx, int ny, int nz) {
+    if (done && done[0] != 0) return;
+    int i, j, k;
+    if (!idx3(nx, ny, nz, i, j, k)) return;
+    size_t c = ((size_t)i * ny + j) * nz + k;
+    if (marker[c] != 1) { y[c] = 0.0f; return; }
+    float s, d;
+    stencil(x, marker, i, j, k, nx, ny, nz, s, d);
+    y[c] = d * x[c] - s;
+}
+
+__global__ void k_compute_diag_dense(const int32_t* marker, float* diag, int nx, int ny, int nz) {
+    int i, j, k;
+    if (!idx3(nx, ny, nz, i, j, k)) return;
+    size_t c = ((size_t)i * ny + j) * nz + k;
+    if (marker[c] != 1) { diag[c] = 1.0f; return; }
+    const size_t sx = (size_t)ny * nz, sy = nz;
+    float d = 0.0f;
+    if (i > 0 && marker[c - sx] != 2) d += 1.0f;
+    if (i < nx - 1 && marker[c + sx] != 2) d += 1.0f;
+    if (j > 0 && marker[c - sy] != 2) d += 1.0f;
+    if (j < ny - 1 && marker[c + sy] != 2) d += 1.0f;
+    if (k > 0 && marker[c - 1] != 2) d += 1.0f;
+    if (k < nz - 1 && marker[c + 1] != 2) d += 1.0f;
+    diag[c] = fmaxf(d, 1.0f);
+}
+
+__global__ void k_apply_invM_dense(const float* r, const float* diag, float* z,
+                                   const int32_t* marker, const int32_t* done, long long n) {
+    if (done && done[0] != 0) return;
+    long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    z[c] = marker[c] != 1 ? 0.0f : r[c] / diag[c];
+}
+
+__global__ __launch_bounds__(GF_WG) void k_dot_dense(const float* a, const float* b, float* out,
+                                                     const int32_t* marker, const int32_t* done,
+                                                     long long n) {
+    __shared__ double s_w[GF_WG / GF_WAVE];
+    if (done && done[0] != 0) return;
+    long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double v = 0.0;
+    if (c < n && marker[c] == 1) v = (double)(a[c] * b[c]);
+    v = gf::wave_sum(v);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < GF_WG / GF_WAVE; ++i) t += s_w[i];
+        if (t != 0.0) atomicAdd(out, (float)t);
+    }
+}
+
+__global__ void k_axpy_devscalar_dense(const float* a_dev, float sign, const float* x,
+                                       const float* y_in, float* y_out, const int32_t* marker,
+                                       const int32_t* done, long long n) {
+    if (done && done[0] != 0) return;
+    long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    if (marker[c] != 1) { y_out[c] = 0.0f; return; }
+    float a = a_dev[0] * sign;
+    y_out[c] = y_in[c] + a * x[c];
+}
+
+__global__ void k_scalar_op(int op, float* a, float* b, float* c, float tol, int32_t* done) {
+    switch (op) {
+        case 0:  // k3_zero_scalar(s, done)
+            if (done && done[0] != 0) return;
+            a[0] = 0.0f;
+            break;
+        case 1: {  // k3_div_scalar(numer, denom, out, done)
+            if (done && done[0] != 0) return;
+            float d = b[0];
+            c[0] = fabsf(d) < 1.0e-30f ? 0.0f : a[0] / d;
+            break;
+        }
+        case 2:  // k3_copy_scalar(src, dst, done)
+            if (done && done[0] != 0) return;
+            c[0] = a[0];
+            break;
+        case 3:  // k3_set_tol_sq(r0_norm2, tol, tol_sq)
+            c[0] = tol * tol * a[0];
+            break;
+        case 4:  // k3_check_converged(r_norm2, tol_sq, done)
+            if (a[0] < b[0]) done[0] = 1;
+            break;
+        case 5:  // k3_zero_int_scalar(s)
+            done[0] = 0;
+            break;
+    }
+}
+
+// solver3d.py:1364-1407
+__global__ void k_subtract_grad(float* u, float* v, float* w, const float* p,
+                                const int32_t* marker, int nx, int ny, int nz, float dx,
+                                float dt, float rho) {
+    int i, j, k;
+    if (!idx3(nx + 1, ny + 1, nz + 1, i, j, k)) return;
+    const float scale = dt / (rho * dx);
+    auto M = [&](int a, int b2, int c) { return marker[((size_t)a * ny + b2) * nz + c]; };
+    auto P = [&](int a, int b2, int c) { return p[((size_t)a * ny + b2) * nz + c]; };
+    if (i >= 1 && i <= nx - 1 && j < ny && k < nz) {
+        int ml = M(i - 1, j, k), mr = M(i, j, k);
+        if ((ml == 1 || mr == 1) && ml != 2 && mr != 2) {
+            float pl = ml == 1 ? P(i - 1, j, k) : 0.0f;
+            float pr = mr == 1 ? P(i, j, k) : 0.0f;
+            size_t f = ((size_t)i * ny + j) * nz + k;
+            u[f] = u[f] - scale * (pr - pl);
+        }
+    }
+    if (j >= 1 && j <= ny - 1 && i < nx && k < nz) {
+        int mb = M(i, j - 1, k), mt = M(i, j, k);
+        if ((mb == 1 || mt == 1) && mb != 2 && mt != 2) {
+            float pb = mb == 1 ? P(i, j - 1, k) : 0.0f;
+            float pt = mt == 1 ? P(i, j, k) : 0.0f;
+            size_t f = ((size_t)i * (ny + 1) + j) * nz + k;
+            v[f] = v[f] - scale * (pt - pb);
+        }
+    }
+    if (k >= 1 && k <= nz - 1 && i < nx && j < ny) {
+        int mba = M(i, j, k - 1), mfr = M(i, j, k);
+        if ((mba == 1 || mfr == 1) && mba != 2 && mfr != 2) {
+            float pba = mba == 1 ? P(i, j, k - 1) : 0.0f;
+            float pfr = mfr == 1 ? P(i, j, k) : 0.0f;
+            size_t f = ((size_t)i * ny + j) * (nz + 1) + k;
+            w[f] = w[f] - scale * (pfr - pba);
+        }
+    }
+}
+
+// gridmath.py:30-65 on a dense face array of shape (nmx, nmy, nmz).
+__device__ float sample3(const float* f, float fx, float fy, float fz, int nmx, int nmy, int nmz) {
+    fx = fminf(fmaxf(fx, 0.0f), (float)nmx);
+    fy = fminf(fmaxf(fy, 0.0f), (float)nmy);
+    fz = fminf(fmaxf(fz, 0.0f), (float)nmz);
+    int i0 = (int)floorf(fx), j0 = (int)floorf(fy), k0 = (int)floorf(fz);
+    if (i0 >= nmx) i0 = nmx - 1;
+    if (j0 >= nmy) j0 = nmy - 1;
+    if (k0 >= nmz) k0 = nmz - 1;
+    if (i0 < 0) i0 = 0;
+    if (j0 < 0) j0 = 0;
+    if (k0 < 0) k0 = 0;
+    int i1 = i0 + 1, j1 = j0 + 1, k1 = k0 + 1;
+    if (i1 >= nmx) i1 = nmx - 1;
+    if (j1 >= nmy) j1 = nmy - 1;
+    if (k1 >= nmz) k1 = nmz - 1;
+    const float sx = fminf(fmaxf(fx - (float)i0, 0.0f), 1.0f);
+    const float sy = fminf(fmaxf(fy - (float)j0, 0.0f), 1.0f);
+    const float sz = fminf(fmaxf(fz - (float)k0, 0.0f), 1.0f);
+    auto F = [&](int a, int b, int c) { return f[((size_t)a * nmy + b) * nmz + c]; };
+    const float c000 = F(i0, j0, k0), c100 = F(i1, j0, k0);
+    const float c010 = F(i0, j1, k0), c110 = F(i1, j1, k0);
+    const float c001 = F(i0, j0, k1), c101 = F(i1, j0, k1);
+    const float c011 = F(i0, j1, k1), c111 = F(i1, j1, k1);
+    const float c00 = c000 * (1.0f - sx) + c100 * sx;
+    const float c10 = c010 * (1.0f - sx) + c110 * sx;
+    const float c01 = c001 * (1.0f - sx) + c101 * sx;
+    const float c11 = c011 * (1.0f - sx) + c111 * sx;
+    const float c0 = c00 * (1.0f - sy) + c10 * sy;
+    const float c1 = c01 * (1.0f - sy) + c11 * sy;
+    return c0 * (1.0f - sz) + c1 * sz;
+}
+
+// solver3d.py:1414-1463
+__global__ void k_g2p_advect_dense(float* pos, float* vel, const float* u, const float* v,
+                                   const float* w, const float* us, const float* vs,
+                                   const float* ws, int n, float dx, float dt, int nx, int ny,
+                                   int nz, float fb, float dmx, float dmy, float dmz) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const float px = pos[3 * p], py = pos[3 * p + 1], pz = pos[3 * p + 2];
+    const float lx = px / dx, ly = py / dx, lz = pz / dx;
+    const float pic_u = sample3(u, lx, ly - 0.5f, lz - 0.5f, nx + 1, ny, nz);
+    const float old_u = sample3(us, lx, ly - 0.5f, lz - 0.5f, nx + 1, ny, nz);
+    const float pic_v = sample3(v, lx - 0.5f, ly, lz - 0.5f, nx, ny + 1, nz);
+    const float old_v = sample3(vs, lx - 0.5f, ly, lz - 0.5f, nx, ny + 1, nz);
+    const float pic_w = sample3(w, lx - 0.5f, ly - 0.5f, lz, nx, ny, nz + 1);
+    const float old_w = sample3(ws, lx - 0.5f, ly - 0.5f, lz, nx, ny, nz + 1);
+    const float fu = vel[3 * p] + (pic_u - old_u);
+    const float fv = vel[3 * p + 1] + (pic_v - old_v);
+    const float fw = vel[3 * p + 2] + (pic_w - old_w);
+    const float nu = fb * fu + (1.0f - fb) * pic_u;
+    const float nv = fb * fv + (1.0f - fb) * pic_v;
+    const float nw = fb * fw + (1.0f - fb) * pic_w;
+    vel[3 * p] = nu; vel[3 * p + 1] = nv; vel[3 * p + 2] = nw;
+    const float eps = dx * 1.001f;
+    pos[3 * p] = fminf(fmaxf(px + nu * dt, eps), dmx - eps);
+    pos[3 * p + 1] = fminf(fmaxf(py + nv * dt, eps), dmy - eps);
+    pos[3 * p + 2] = fminf(fmaxf(pz + nw * dt, eps), dmz - eps);
+}
+
+// solver3d.py:1591-1732
+template <int COMP>
+__device__ void apic_dense(const float* f, float px, float py, float pz, float dx,
+                           float inv_dx2_3, int nx, int ny, int nz, float& vout, float* crow) {
+    float fx = px / dx, fy = py / dx, fz = pz / dx;
+    if (COMP != 0) fx = fx - 0.5f;
+    if (COMP != 1) fy = fy - 0.5f;
+    if (COMP != 2) fz = fz - 0.5f;
+    const int i0 = (int)floorf(fx), j0 = (int)floorf(fy), k0 = (int)floorf(fz);
+    const float sx = fx - (float)i0, sy = fy - (float)j0, sz = fz - (float)k0;
+    const int imax = COMP == 0 ? nx : nx - 1, jmax = COMP == 1 ? ny : ny - 1,
+              kmax = COMP == 2 ? nz : nz - 1;
+    const int nfj = ny + (COMP == 1), nfk = nz + (COMP == 2);
+    float acc = 0.0f, c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
+    for (int di = 0; di < 2; ++di)
+        for (int dj = 0; dj < 2; ++dj)
+            for (int dk = 0; dk < 2; ++dk) {
+                int ii = i0 + di, jj = j0 + dj, kk = k0 + dk;
+                if (ii < 0) ii = 0;
+                if (jj < 0) jj = 0;
+                if (kk < 0) kk = 0;
+                if (ii > imax) ii = imax;
+                if (jj > jmax) jj = jmax;
+                if (kk > kmax) kk = kmax;
+                const float wt = (di ? sx : 1.0f - sx) * (dj ? sy : 1.0f - sy) * (dk ? sz : 1.0f - sz);
+                const float val = f[((size_t)ii * nfj + jj) * nfk + kk];
+                acc += wt * val;
+                const float ox = (COMP == 0 ? (float)ii : (float)ii + 0.5f) * dx - px;
+                const float oy = (COMP == 1 ? (float)jj : (float)jj + 0.5f) * dx - py;
+                const float oz = (COMP == 2 ? (float)kk : (float)kk + 0.5f) * dx - pz;
+                const float fu = wt * val * inv_dx2_3;
+                c0 += fu * ox; c1 += fu * oy; c2 += fu * oz;
+            }
+    vout = acc; crow[0] = c0; crow[1] = c1; crow[2] = c2;
+}
+
+__global__ void k_g2p_apic_dense(float* pos, float* vel, float* C, const float* u, const float* v,
+                                 const float* w, int n, float dx, float dt, int nx, int ny,
+                                 int nz, float dmx, float dmy, float dmz) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const float px = pos[3 * p], py = pos[3 * p + 1], pz = pos[3 * p + 2];
+    const float inv = 3.0f / (dx * dx);
+    float nu, nv, nw, cm[9];
+    apic_dense<0>(u, px, py, pz, dx, inv, nx, ny, nz, nu, cm);
+    apic_dense<1>(v, px, py, pz, dx, inv, nx, ny, nz, nv, cm + 3);
+    apic_dense<2>(w, px, py, pz, dx, inv, nx, ny, nz, nw, cm + 6);
+    vel[3 * p] = nu; vel[3 * p + 1] = nv; vel[3 * p + 2] = nw;
+    for (int c = 0; c < 9; ++c) C[9 * p + c] = cm[c];
+    const float eps = dx * 1.001f;
+    pos[3 * p] = fminf(fmaxf(px + nu * dt, eps), dmx - eps);
+    pos[3 * p + 1] = fminf(fmaxf(py + nv * dt, eps), dmy - eps);
+    pos[3 * p + 2] = fminf(fmaxf(pz + nw * dt, eps), dmz - eps);
+}
+
+// solver3d.py:2241-2252 (non-negative floats order like their bit patterns)
+__global__ __launch_bounds__(GF_WG) void k_vmax_sq(const float* vel, int n, float* out) {
+    __shared__ float s_w[GF_WG / GF_WAVE];
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    float s = 0.0f;
+    if (p < n) {
+        float a = vel[3 * p], b = vel[3 * p + 1], c = vel[3 * p + 2];
+        s = a * a + b * b + c * c;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s = fmaxf(s, __shfl_xor(s, off, 64));
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float m = 0.0f;
+        for (int i = 0; i < GF_WG / GF_WAVE; ++i) m = fmaxf(m, s_w[i]);
+        atomicMax((unsigned int*)out, __float_as_uint(m));
+    }
+}
+
+}  // namespace
+
+#define GRID_ARGS_OK(who)                                                         \
+    GF_REQUIRE(nx >= 1 && ny >= 1 && nz >= 1, who ": grid %dx%dx%d", nx, ny, nz)
+
+extern "C" int gf_k3_clear_grid(float* u, float* v, float* w, float* uw, float* vw, float* ww,
+                                int32_t* marker, int nx, int ny, int nz, void* stream) {
+    GRID_ARGS_OK("gf_k3_clear_grid");
+    GF_REQUIRE(u && v && w && uw && vw && ww && marker, "gf_k3_clear_grid: null buffer");
+    hipLaunchKernelGGL(k_clear_grid, grid_for((long long)(nx + 1) * (ny + 1) * (nz + 1)),
+                       dim3(GF_WG), 0, (hipStream_t)stream, u, v, w, uw, vw, ww, marker, nx, ny, nz);
+    GF_LAUNCH_CHECK("k_clear_grid");
+    return 0;
+}
+
+extern "C" int gf_k3_mark_fluid(const float* pos, int n, int32_t* marker, float dx, int nx,
+                                int ny, int nz, void* stream) {
+    GRID_ARGS_OK("gf_k3_mark_fluid");
+    GF_REQUIRE(n >= 0 && (n == 0 || pos) && marker, "gf_k3_mark_fluid: bad buffers");
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_mark_fluid, grid_for(n), dim3(GF_WG), 0, (hipStream_t)stream, pos, n,
+                       marker, dx, nx, ny, nz);
+    GF_LAUNCH_CHECK("k_mark_fluid");
+    return 0;
+}
+
+extern "C" int gf_k3_normalize(float* u, float* v, float* w, const float* uw, const float* vw,
+                               const float* ww, float* us, float* vs, float* ws, int nx, int ny,
+                               int nz, void* stream) {
+    GRID_ARGS_OK("gf_k3_normalize");
+    GF_REQUIRE(u && v && w && uw && vw && ww && us && vs && ws, "gf_k3_normalize: null buffer");
+    hipLaunchKernelGGL(k_normalize, grid_for((long long)(nx + 1) * (ny + 1) * (nz + 1)),
+                       dim3(GF_WG), 0, (hipStream_t)stream, u, v, w, uw, vw, ww, us, vs, ws, nx,
+                       ny, nz);
+    GF_LAUNCH_CHECK("k_normalize");
+    return 0;
+}
+
+extern "C" int gf_k3_add_gravity(float* v, int nx, int ny, int nz, float g, float dt,
+                                 void* stream) {
+    GRID_ARGS_OK("gf_k3_add_gravity");
+    GF_REQUIRE(v, "gf_k3_add_gravity: null buffer");
+    long long n = (long long)nx * (ny + 1) * nz;
+    hipLaunchKernelGGL(k_add_gravity, grid_for(n), dim3(GF_WG), 0, (hipStream_t)stream, v, n, g, dt);
+    GF_LAUNCH_CHECK("k_add_gravity");
+    return 0;
+}
+
+extern "C" int gf_k3_enforce_solid_bc(float* u, float* v, float* w, const int32_t* marker,
+                                      const float* solid_u, const float* solid_v,
+                                      const float* solid_w, int nx, int ny, int nz,
+                                      void* stream) {
+    GRID_ARGS_OK("gf_k3_enforce_solid_bc");
+    GF_REQUIRE(u && v && w && marker, "gf_k3_enforce_solid_bc: null buffer");
+    hipLaunchKernelGGL(k_enforce_solid_bc, grid_for((long long)(nx + 1) * (ny + 1) * (nz + 1)),
+                       dim3(GF_WG), 0, (hipStream_t)stream, u, v, w, marker, solid_u, solid_v,
+                       solid_w, nx, ny, nz);
+    GF_LAUNCH_CHECK("k_enforce_solid_bc");
+    return 0;
+}
+
+extern "C" int gf_k3_compute_divergence(const float* u, const float* v, const float* w,
+                                        float* div, const int32_t* marker, int nx, int ny, int nz,
+                                        float dx, float dt, float rho, void* stream) {
+    GRID_ARGS_OK("gf_k3_compute_divergence");
+    GF_REQUIRE(u && v && w && div && marker, "gf_k3_compute_divergence: null buffer");
+    hipLaunchKernelGGL(k_divergence, grid_for((long long)nx * ny * nz), dim3(GF_WG), 0,
+                       (hipStream_t)stream, u, v, w, div, marker, nx, ny, nz, dx, dt, rho);
+    GF_LAUNCH_CHECK("k_divergence");
+    return 0;
+}
+
+extern "C" int gf_k3_jacobi_pressure(const float* p_in, float* p_out, const float* div,
+                                     const int32_t* marker, int nx, int ny, int nz,
+                                     void* stream) {
+    GRID_ARGS_OK("gf_k3_jacobi_pressure");
+    GF_REQUIRE(p_in && p_out && div && marker && p_in != p_out,
+               "gf_k3_jacobi_pressure: bad buffers");
+    hipLaunchKernelGGL(k_jacobi_dense, grid_for((long long)nx * ny * nz), dim3(GF_WG), 0,
+                       (hipStream_t)stream, p_in, p_out, div, marker, nx, ny, nz);
+    GF_LAUNCH_CHECK("k_jacobi_dense");
+    return 0;
+}
+
+extern "C" int gf_k3_apply_A(const float* x, float* y, const int32_t* marker,
+                             const int32_t* done, int nx, int ny, int nz, void* stream) {
+    GRID_ARGS_OK("gf_k3_apply_A");
+    GF_REQUIRE(x && y && marker && x != y, "gf_k3_apply_A: bad buffers");
+    hipLaunchKernelGGL(k_apply_A_dense, grid_for((long long)nx * ny * nz), dim3(GF_WG), 0,
+                       (hipStream_t)stream, x, y, marker, done, nx, ny, nz);
+    GF_LAUNCH_CHECK("k_apply_A_dense");
+    return 0;
+}
+
+extern "C" int gf_k3_compute_diag(const int32_t* marker, float* diag, int nx, int ny, int nz,
+                                  void* stream) {
+    GRID_ARGS_OK("gf_k3_compute_diag");
+    GF_REQUIRE(marker && diag, "gf_k3_compute_diag: null buffer");
+    hipLaunchKernelGGL(k_compute_diag_dense, grid_for((long long)nx * ny * nz), dim3(GF_WG), 0,
+                       (hipStream_t)stream, marker, diag, nx, ny, nz);
+    GF_LAUNCH_CHECK("k_compute_diag_dense");
+    return 0;
+}
+
+extern "C" int gf_k3_apply_invM(const float* r, const float* diag, float* z,
+                                const int32_t* marker, const int32_t* done, int nx, int ny,
+                                int nz, void* stream) {
+    GRID_ARGS_OK("gf_k3_apply_invM");
+    GF_REQUIRE(r && diag && z && marker, "gf_k3_apply_invM: null buffer");
+    long long n = (long long)nx * ny * nz;
+    hipLaunchKernelGGL(k_apply_invM_dense, grid_for(n), dim3(GF_WG), 0, (hipStream_t)stream, r,
+                       diag, z, marker, done, n);
+    GF_LAUNCH_CHECK("k_apply_invM_dense");
+    return 0;
+}
+
+extern "C" int gf_k3_dot_fluid(const float* a, const float* b, float* out, const int32_t* marker,
+                               const int32_t* done, int nx, int ny, int nz, void* stream) {
+    GRID_ARGS_OK("gf_k3_dot_fluid");
+    GF_REQUIRE(a && b && out && marker, "gf_k3_dot_fluid: null buffer");
+    long long n = (long long)nx * ny * nz;
+    hipLaunchKernelGGL(k_dot_dense, grid_for(n), dim3(GF_WG), 0, (hipStream_t)stream, a, b, out,
+                       marker, done, n);
+    GF_LAUNCH_CHECK("k_dot_dense");
+    return 0;
+}
+
+extern "C" int gf_k3_axpy_devscalar(const float* a_dev, float sign, const float* x,
+                                    const float* y_in, float* y_out, const int32_t* marker,
+                                    const int32_t* done, int nx, int ny, int nz, void* stream) {
+    GRID_ARGS_OK("gf_k3_axpy_devscalar");
+    GF_REQUIRE(a_dev && x && y_in && y_out && marker, "gf_k3_axpy_devscalar: null buffer");
+    long long n = (long long)nx * ny * nz;
+    hipLaunchKernelGGL(k_axpy_devscalar_dense, grid_for(n), dim3(GF_WG), 0, (hipStream_t)stream,
+                       a_dev, sign, x, y_in, y_out, marker, done, n);
+    GF_LAUNCH_CHECK("k_axpy_devscalar_dense");
+    return 0;
+}
+
+extern "C" int gf_k3_scalar_op(int op, float* a, float* b, float* c, float tol, int32_t* done,
+                               void* stream) {
+    GF_REQUIRE(op >= 0 && op <= 5, "gf_k3_scalar_op: op %d", op);
+    const bool need_a = op <= 4, need_b = op == 1 || op == 4, need_c = op == 1 || op == 2 || op == 3;
+    const bool need_done = op == 4 || op == 5;
+    GF_REQUIRE((!need_a || a) && (!need_b || b) && (!need_c || c) && (!need_done || done),
+               "gf_k3_scalar_op: op %d missing a buffer", op);
+    hipLaunchKernelGGL(k_scalar_op, dim3(1), dim3(1), 0, (hipStream_t)stream, op, a, b, c, tol, done);
+    GF_LAUNCH_CHECK("k_scalar_op");
+    return 0;
+}
+
+extern "C" int gf_k3_subtract_pressure_grad(float* u, float* v, float* w, const float* p,
+                                            const int32_t* marker, int nx, int ny, int nz,
+                                            float dx, float dt, float rho, void* stream) {
+    GRID_ARGS_OK("gf_k3_subtract_pressure_grad");
+    GF_REQUIRE(u && v && w && p && marker, "gf_k3_subtract_pressure_grad: null buffer");
+    hipLaunchKernelGGL(k_subtract_grad, grid_for((long long)(nx + 1) * (ny + 1) * (nz + 1)),
+                       dim3(GF_WG), 0, (hipStream_t)stream, u, v, w, p, marker, nx, ny, nz, dx,
+                       dt, rho);
+    GF_LAUNCH_CHECK("k_subtract_grad");
+    return 0;
+}
+
+extern "C" int gf_k3_g2p_and_advect(float* pos, float* vel, const float* u, const float* v,
+                                    const float* w, const float* us, const float* vs,
+                                    const float* ws, int n, float dx, float dt, int nx, int ny,
+                                    int nz, float flip_blend, const float* dom, void* stream) {
+    GRID_ARGS_OK("gf_k3_g2p_and_advect");
+    GF_REQUIRE(n >= 0 && dom, "gf_k3_g2p_and_advect: bad arguments");
+    if (n == 0) return 0;
+    GF_REQUIRE(pos && vel && u && v && w && us && vs && ws, "gf_k3_g2p_and_advect: null buffer");
+    hipLaunchKernelGGL(k_g2p_advect_dense, grid_for(n), dim3(GF_WG), 0, (hipStream_t)stream, pos,
+                       vel, u, v, w, us, vs, ws, n, dx, dt, nx, ny, nz, flip_blend, dom[0],
+                       dom[1], dom[2]);
+    GF_LAUNCH_CHECK("k_g2p_advect_dense");
+    return 0;
+}
+
+extern "C" int gf_k3_g2p_apic_advect(float* pos, float* vel, float* affine_c, const float* u,
+                                     const float* v, const float* w, int n, float dx, float dt,
+                                     int nx, int ny, int nz, const float* dom, void* stream) {
+    GRID_ARGS_OK("gf_k3_g2p_apic_advect");
+    GF_REQUIRE(n >= 0 && dom, "gf_k3_g2p_apic_advect: bad arguments");
+    if (n == 0) return 0;
+    GF_REQUIRE(pos && vel && affine_c && u && v && w, "gf_k3_g2p_apic_advect: null buffer");
+    hipLaunchKernelGGL(k_g2p_apic_dense, grid_for(n), dim3(GF_WG), 0, (hipStream_t)stream, pos,
+                       vel, affine_c, u, v, w, n, dx, dt, nx, ny, nz, dom[0], dom[1], dom[2]);
+    GF_LAUNCH_CHECK("k_g2p_apic_dense");
+    return 0;
+}
+
+extern "C" int gf_k3_vmax_sq_reduce(const float* vel, int n, float* out, void* stream) {
+    GF_REQUIRE(n >= 0 && out, "gf_k3_vmax_sq_reduce: bad arguments");
+    if (n == 0) return 0;
+    GF_REQUIRE(vel, "gf_k3_vmax_sq_reduce: null buffer");
+    hipLaunchKernelGGL(k_vmax_sq, grid_for(n), dim3(GF_WG), 0, (hipStream_t)stream, vel, n, out);
+    GF_LAUNCH_CHECK("k_vmax_sq");
+    return 0;
+}

@@ -1,0 +1,139 @@
+// Shared device helpers for the gfx950 FLIP substep kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "../../include/gpufluid_hip.h"
+
+#define GF_WAVE 64
+#define GF_WG 256
+
+namespace gf {
+
+// Host-side error recording (gf_api.hip).
+int set_error(int code, const char* fmt, ...);
+int check_launch(const char* what);
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) {
+    return v < lo ? lo : (v > hi ? hi : v);
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
+// contiguous range of work items so that neighbouring tiles (which share halo
+// data) land in the same L2.
+__device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
+    const int nx = 8;
+    if (nblocks < nx * 2) return bid;
+    int xcd = bid % nx, local = bid / nx;
+    int per = nblocks / nx, rem = nblocks % nx;
+    int start = xcd * per + (xcd < rem ? xcd : rem);
+    return start + local;
+}
+
+// Cell binning of solver3d.py:96-99: clamp(int(p/dx), 0, n-1) with an IEEE
+// FP32 divide and truncation.  The quotient is clamped to [-1, n] first so the
+// conversion is always in range (NaN maps to cell 0 as the saturating
+// conversion does); in-range quotients are unaffected.
+__device__ __forceinline__ int bin_axis(float p, float dx, int n) {
+    float q = p / dx;
+    q = fminf(fmaxf(q, -1.0f), (float)n);
+    int c = (int)q;
+    return clampi(c, 0, n - 1);
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Workgroup-wide exclusive scan of one int per thread (blockDim.x <= 1024,
+// multiple of 64). Returns the exclusive prefix; *total gets the sum.
+__device__ __forceinline__ int wg_exclusive_scan(int v, int* s_wave, int* total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nwaves = blockDim.x >> 6;
+    int incl = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        int t = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += t;
+    }
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int i = 0; i < nwaves; ++i) { int t = s_wave[i]; s_wave[i] = acc; acc += t; }
+        s_wave[nwaves] = acc;
+    }
+    __syncthreads();
+    int res = s_wave[wave] + incl - v;
+    *total = s_wave[nwaves];
+    __syncthreads();
+    return res;
+}
+
+// Deterministic two-level FP64 reduction of NV values per thread over the whole
+// grid.  Every workgroup writes its partial sums to partials[k*nblocks + bid];
+// the last workgroup to arrive sums all partials in index order (so the result
+// does not depend on arrival order), stores the totals in tot[] and returns
+// true (uniformly across that workgroup); all other workgroups return false.
+template <int NV>
+__device__ bool wg_reduce_last(const double (&v)[NV], double* partials, int nblocks,
+                               unsigned int* counter, double (&tot)[NV]) {
+    __shared__ double s_red[NV][GF_WG / GF_WAVE];
+    __shared__ int s_last;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nwaves = blockDim.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        double t = wave_sum(v[k]);
+        if (lane == 0) s_red[k][wave] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            double a = 0.0;
+            for (int i = 0; i < nwaves; ++i) a += s_red[k][i];
+            partials[(size_t)k * nblocks + blockIdx.x] = a;
+        }
+        __threadfence();
+        unsigned int ticket = atomicAdd(counter, 1u);
+        s_last = (ticket == (unsigned int)(nblocks - 1));
+    }
+    __syncthreads();
+    if (!s_last) return false;
+    __threadfence();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        double a = 0.0;
+        for (int i = threadIdx.x; i < nblocks; i += blockDim.x)
+            a += __builtin_nontemporal_load(&partials[(size_t)k * nblocks + i]);
+        a = wave_sum(a);
+        __syncthreads();
+        if (lane == 0) s_red[k][wave] = a;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        double a = 0.0;
+        for (int i = 0; i < nwaves; ++i) a += s_red[k][i];
+        tot[k] = a;
+    }
+    if (threadIdx.x == 0) *counter = 0u;
+    return true;
+}
+
+}  // namespace gf
+
+// Launch helpers (host).
+#define GF_LAUNCH_CHECK(what)                          \
+    do {                                               \
+        int _rc = gf::check_launch(what);              \
+        if (_rc) return _rc;                           \
+    } while (0)
+
+#define GF_REQUIRE(cond, ...)                                   \
+    do {                                                        \
+        if (!(cond)) return gf::set_error(GF_EINVAL, __VA_ARGS__); \
+    } while (0)

@@ -1,0 +1,295 @@
+// Cell binning, stable counting sort of particles by flat cell key, and the
+// per-substep tile bookkeeping (marker, compact flags, tile neighbour table).
+//
+// Binning follows solver3d.py:96-99 (clamp(int(p/dx),0,n-1), FP32 divide);
+// the flat key i*ny*nz + j*nz + k and the stable in-cell order follow
+// sim/reseed.py:72-75 and 114 (np.argsort(kind="stable")).  Only integer
+// atomics are used; the in-cell order is fixed afterwards by ranking each
+// particle among its cell's members by original index, so the permutation is
+// deterministic and equal to a stable sort.
+#include "gf_common.h"
+
+namespace {
+
+constexpr int SCAN_ITEMS = 16;
+constexpr int SCAN_CHUNK = GF_WG * SCAN_ITEMS;  // 4096
+
+__global__ void k_fill_i32(int32_t* a, int n, int32_t v) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+__global__ __launch_bounds__(GF_WG) void k_bin(int n, const float* __restrict__ pos,
+                                               float dx, int nx, int ny, int nz,
+                                               const int32_t* __restrict__ marker,
+                                               int32_t* __restrict__ keys,
+                                               int32_t* __restrict__ counts,
+                                               int32_t* __restrict__ tile_occ, int tny,
+                                               int tnz) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    float x = pos[3 * p + 0], y = pos[3 * p + 1], z = pos[3 * p + 2];
+    int ci = gf::bin_axis(x, dx, nx);
+    int cj = gf::bin_axis(y, dx, ny);
+    int ck = gf::bin_axis(z, dx, nz);
+    int key = (ci * ny + cj) * nz + ck;
+    keys[p] = key;
+    atomicAdd(&counts[key], 1);
+    if (marker[key] != 2) tile_occ[((ci >> 3) * tny + (cj >> 3)) * tnz + (ck >> 3)] = 1;
+}
+
+__global__ __launch_bounds__(GF_WG) void k_scan_reduce(const int32_t* __restrict__ in, int n,
+                                                       int32_t* __restrict__ part) {
+    __shared__ int s_wave[GF_WG / GF_WAVE + 1];
+    int base = blockIdx.x * SCAN_CHUNK;
+    int acc = 0;
+#pragma unroll
+    for (int it = 0; it < SCAN_ITEMS; ++it) {
+        int i = base + it * GF_WG + threadIdx.x;
+        if (i < n) acc += in[i];
+    }
+    int total;
+    gf::wg_exclusive_scan(acc, s_wave, &total);
+    if (threadIdx.x == 0) part[blockIdx.x] = total;
+}
+
+// Exclusive scan of part[0..nb) in place; part[nb] = total.
+__global__ __launch_bounds__(1024) void k_scan_part(int32_t* part, int nb) {
+    __shared__ int s_wave[1024 / GF_WAVE + 1];
+    int carry = 0;
+    for (int base = 0; base < nb; base += blockDim.x) {
+        int i = base + threadIdx.x;
+        int v = i < nb ? part[i] : 0;
+        int total;
+        int ex = gf::wg_exclusive_scan(v, s_wave, &total);
+        if (i < nb) part[i] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0) part[nb] = carry;
+}
+
+__global__ __launch_bounds__(GF_WG) void k_scan_apply(const int32_t* __restrict__ in, int n,
+                                                      const int32_t* __restrict__ part,
+                                                      int nb, int32_t* __restrict__ out) {
+    __shared__ int s_items[SCAN_CHUNK];
+    __shared__ int s_wave[GF_WG / GF_WAVE + 1];
+    int base = blockIdx.x * SCAN_CHUNK;
+#pragma unroll
+    for (int it = 0; it < SCAN_ITEMS; ++it) {
+        int l = it * GF_WG + threadIdx.x;
+        int i = base + l;
+        s_items[l] = i < n ? in[i] : 0;
+    }
+    __syncthreads();
+    int loc[SCAN_ITEMS];
+    int acc = 0;
+#pragma unroll
+    for (int it = 0; it < SCAN_ITEMS; ++it) {
+        loc[it] = acc;
+        acc += s_items[threadIdx.x * SCAN_ITEMS + it];
+    }
+    int total;
+    int ex = gf::wg_exclusive_scan(acc, s_wave, &total);
+    int off = part[blockIdx.x] + ex;
+#pragma unroll
+    for (int it = 0; it < SCAN_ITEMS; ++it) s_items[threadIdx.x * SCAN_ITEMS + it] = off + loc[it];
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < SCAN_ITEMS; ++it) {
+        int l = it * GF_WG + threadIdx.x;
+        int i = base + l;
+        if (i < n) out[i] = s_items[l];
+    }
+    if (blockIdx.x == nb - 1 && threadIdx.x == 0) out[n] = part[nb];
+}
+
+__global__ __launch_bounds__(GF_WG) void k_slot(int n, const int32_t* __restrict__ keys,
+                                                int32_t* __restrict__ counts,
+                                                const int32_t* __restrict__ cell_start,
+                                                int32_t* __restrict__ tmp) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    int key = keys[p];
+    int old = atomicSub(&counts[key], 1);
+    tmp[cell_start[key] + old - 1] = p;
+}
+
+// Rank each particle among the members of its cell by original index (a stable
+// order), then gather its state into the sorted arrays.
+__global__ __launch_bounds__(GF_WG) void k_rank(
+    int n, const int32_t* __restrict__ keys, const int32_t* __restrict__ cell_start,
+    const int32_t* __restrict__ tmp, const float* __restrict__ pos,
+    const float* __restrict__ vel, const float* __restrict__ C,
+    const int32_t* __restrict__ pid, float* __restrict__ pos_o, float* __restrict__ vel_o,
+    float* __restrict__ C_o, int32_t* __restrict__ pid_o) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    int key = keys[p];
+    int s = cell_start[key], e = cell_start[key + 1];
+    int rank = 0;
+    for (int q = s; q < e; ++q) rank += (tmp[q] < p);
+    int d = s + rank;
+    pos_o[3 * d + 0] = pos[3 * p + 0];
+    pos_o[3 * d + 1] = pos[3 * p + 1];
+    pos_o[3 * d + 2] = pos[3 * p + 2];
+    vel_o[3 * d + 0] = vel[3 * p + 0];
+    vel_o[3 * d + 1] = vel[3 * p + 1];
+    vel_o[3 * d + 2] = vel[3 * p + 2];
+    pid_o[d] = pid[p];
+    if (C) {
+#pragma unroll
+        for (int c = 0; c < 9; ++c) C_o[9 * d + c] = C[9 * p + c];
+    }
+}
+
+// One workgroup: compact slot per occupied tile (in tile-id order).
+__global__ __launch_bounds__(1024) void k_tile_scan(const int32_t* __restrict__ occ,
+                                                    int ntiles,
+                                                    int32_t* __restrict__ tile_map,
+                                                    int32_t* __restrict__ active,
+                                                    int32_t* __restrict__ n_active) {
+    __shared__ int s_wave[1024 / GF_WAVE + 1];
+    int carry = 0;
+    for (int base = 0; base < ntiles; base += blockDim.x) {
+        int t = base + threadIdx.x;
+        int o = (t < ntiles && occ[t]) ? 1 : 0;
+        int total;
+        int ex = gf::wg_exclusive_scan(o, s_wave, &total);
+        if (t < ntiles) {
+            tile_map[t] = o ? carry + ex : -1;
+            if (o) active[carry + ex] = t;
+        }
+        carry += total;
+    }
+    if (threadIdx.x == 0) n_active[0] = carry;
+}
+
+// Per tile: new marker (solid kept, fluid where a particle sits), compact
+// flags (fluid bit + non-solid neighbour count) and the neighbour slot table.
+__global__ __launch_bounds__(GF_WG) void k_tile_setup(
+    int nx, int ny, int nz, int tnx, int tny, int tnz, const int32_t* __restrict__ cell_start,
+    int32_t* marker, const int32_t* __restrict__ tile_map, uint8_t* __restrict__ flags,
+    int32_t* __restrict__ nbr) {
+    const int ntiles = tnx * tny * tnz;
+    const int t = gf::xcd_remap(blockIdx.x, ntiles);
+    const int tk = t % tnz, tj = (t / tnz) % tny, ti = t / (tnz * tny);
+    const int slot = tile_map[t];
+    const int i0 = ti * 8, j0 = tj * 8, k0 = tk * 8;
+    const int lj = (threadIdx.x >> 3) & 7, lk = threadIdx.x & 7;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int li = (threadIdx.x >> 6) + 4 * h;
+        const int i = i0 + li, j = j0 + lj, k = k0 + lk;
+        const int l = (li * 8 + lj) * 8 + lk;
+        uint8_t f = 0;
+        if (i < nx && j < ny && k < nz) {
+            const int c = (i * ny + j) * nz + k;
+            const int mold = marker[c];
+            const int cnt = cell_start[c + 1] - cell_start[c];
+            const int m = (mold == 2) ? 2 : (cnt > 0 ? 1 : 0);
+            if (m != mold) marker[c] = m;
+            if (m == 1) {
+                int d = 0;
+                if (i > 0 && marker[c - ny * nz] != 2) ++d;
+                if (i < nx - 1 && marker[c + ny * nz] != 2) ++d;
+                if (j > 0 && marker[c - nz] != 2) ++d;
+                if (j < ny - 1 && marker[c + nz] != 2) ++d;
+                if (k > 0 && marker[c - 1] != 2) ++d;
+                if (k < nz - 1 && marker[c + 1] != 2) ++d;
+                f = (uint8_t)(8 | d);
+            }
+        }
+        if (slot >= 0) flags[(size_t)slot * GF_TILE_CELLS + l] = f;
+    }
+    if (slot >= 0 && threadIdx.x < 6) {
+        int d = threadIdx.x;
+        int a = ti, b = tj, c = tk;
+        if (d == 0) a -= 1; else if (d == 1) a += 1;
+        else if (d == 2) b -= 1; else if (d == 3) b += 1;
+        else if (d == 4) c -= 1; else c += 1;
+        int ns = -1;
+        if (a >= 0 && a < tnx && b >= 0 && b < tny && c >= 0 && c < tnz)
+            ns = tile_map[(a * tny + b) * tnz + c];
+        nbr[(size_t)slot * 6 + d] = ns;
+    }
+}
+
+int scan_exclusive(const int32_t* in, int n, int32_t* out, int32_t* part, hipStream_t st) {
+    int nb = (n + SCAN_CHUNK - 1) / SCAN_CHUNK;
+    if (nb < 1) nb = 1;
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(GF_WG), 0, st, in, n, part);
+    hipLaunchKernelGGL(k_scan_part, dim3(1), dim3(1024), 0, st, part, nb);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(GF_WG), 0, st, in, n, part, nb, out);
+    return gf::check_launch("scan");
+}
+
+}  // namespace
+
+namespace gf {
+int scene_check(const gf_scene_t* s, const char* who) {
+    if (!s) return set_error(GF_EINVAL, "%s: null scene", who);
+    if (s->nx < 1 || s->ny < 1 || s->nz < 1)
+        return set_error(GF_EINVAL, "%s: grid %dx%dx%d", who, s->nx, s->ny, s->nz);
+    if (s->tnx != (s->nx + 7) / 8 || s->tny != (s->ny + 7) / 8 || s->tnz != (s->nz + 7) / 8)
+        return set_error(GF_ESTATE, "%s: tile counts do not match the grid", who);
+    if (s->n < 0 || s->n > s->cap)
+        return set_error(GF_ESTATE, "%s: n=%d outside [0, cap=%d]", who, s->n, s->cap);
+    if (!(s->dx > 0.0f)) return set_error(GF_EINVAL, "%s: dx must be > 0", who);
+    return 0;
+}
+}  // namespace gf
+
+extern "C" int gf_scene_sort_sync(gf_scene_t* s, int with_affine, int* n_active_host,
+                                  void* stream) {
+    int rc = gf::scene_check(s, "gf_scene_sort_sync");
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    GF_REQUIRE(s->keys && s->tmp && s->counts && s->cell_start && s->scan_part,
+               "gf_scene_sort_sync: sort workspace not set");
+    GF_REQUIRE(s->tile_occ && s->tile_map && s->active && s->n_active && s->nbr && s->flags,
+               "gf_scene_sort_sync: tile workspace not set");
+    GF_REQUIRE(s->marker, "gf_scene_sort_sync: marker not set");
+    if (s->n > 0)
+        GF_REQUIRE(s->pos && s->vel && s->pid && s->pos_b && s->vel_b && s->pid_b,
+                   "gf_scene_sort_sync: particle buffers not set");
+    if (with_affine && s->n > 0)
+        GF_REQUIRE(s->affine_c && s->affine_c_b, "gf_scene_sort_sync: affine buffers not set");
+    const int C = s->nx * s->ny * s->nz;
+    const int ntiles = s->tnx * s->tny * s->tnz;
+    hipLaunchKernelGGL(k_fill_i32, dim3((ntiles + GF_WG - 1) / GF_WG), dim3(GF_WG), 0, st,
+                       s->tile_occ, ntiles, 0);
+    const int npb = (s->n + GF_WG - 1) / GF_WG;
+    if (s->n > 0)
+        hipLaunchKernelGGL(k_bin, dim3(npb), dim3(GF_WG), 0, st, s->n, s->pos, s->dx, s->nx,
+                           s->ny, s->nz, s->marker, s->keys, s->counts, s->tile_occ, s->tny,
+                           s->tnz);
+    GF_LAUNCH_CHECK("k_bin");
+    rc = scan_exclusive(s->counts, C, s->cell_start, s->scan_part, st);
+    if (rc) return rc;
+    if (s->n > 0) {
+        hipLaunchKernelGGL(k_slot, dim3(npb), dim3(GF_WG), 0, st, s->n, s->keys, s->counts,
+                           s->cell_start, s->tmp);
+        hipLaunchKernelGGL(k_rank, dim3(npb), dim3(GF_WG), 0, st, s->n, s->keys,
+                           s->cell_start, s->tmp, s->pos, s->vel,
+                           with_affine ? s->affine_c : nullptr, s->pid, s->pos_b, s->vel_b,
+                           with_affine ? s->affine_c_b : nullptr, s->pid_b);
+        GF_LAUNCH_CHECK("k_slot/k_rank");
+    }
+    hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, st, s->tile_occ, ntiles,
+                       s->tile_map, s->active, s->n_active);
+    hipLaunchKernelGGL(k_tile_setup, dim3(ntiles), dim3(GF_WG), 0, st, s->nx, s->ny, s->nz,
+                       s->tnx, s->tny, s->tnz, s->cell_start, s->marker, s->tile_map,
+                       s->flags, s->nbr);
+    GF_LAUNCH_CHECK("k_tile_scan/k_tile_setup");
+    if (n_active_host) {
+        int32_t na = 0;
+        hipError_t e = hipMemcpyAsync(&na, s->n_active, sizeof(int32_t),
+                                      hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess)
+            return gf::set_error((int)e, "gf_scene_sort_sync: readback: %s",
+                                 hipGetErrorString(e));
+        *n_active_host = na;
+    }
+    return 0;
+}

@@ -1,0 +1,679 @@
+// Particle <-> grid transfers over 8^3 tiles of cell-sorted particles.
+//
+// P2G: one workgroup per tile owns the MAC faces of the tile's cells (plus the
+// +1 plane needed for the tile's divergence), reads the particles of the tile
+// and of its 1-cell halo (contiguous k-rows of the cell-sorted arrays), and
+// accumulates the trilinear contributions in LDS.  No global float atomics:
+// every face is finalised by exactly one workgroup, so normalize, gravity,
+// the solid boundary condition and the divergence are fused into the same
+// kernel.  Arithmetic follows solver3d.py:72-156 (FLIP), 1475-1588 (APIC),
+// 758-799 (BC), 846-871 (divergence) and primitives/gridmath.py:71-95.
+//
+// G2P: one workgroup per tile stages the faces its particles can sample (and
+// the cell pressures around them) in LDS, applies the pressure-gradient
+// subtract and the BC there (solver3d.py:1364-1407, 758-799), writes the
+// tile's own faces back, and gathers for its particles
+// (solver3d.py:1414-1463 with primitives/gridmath.py:30-65, or 1591-1732).
+#include "gf_common.h"
+
+namespace {
+
+struct P2GParams {
+    const float* pos; const float* vel; const float* C;
+    const int32_t* cell_start; const int32_t* marker; const int32_t* tile_map;
+    float* u; float* v; float* w;
+    float* us; float* vs; float* ws;
+    float* uw; float* vw; float* ww;
+    const float* su; const float* sv; const float* sw;
+    float* b;
+    int nx, ny, nz, tnx, tny, tnz;
+    float dx, g, dt, rho;
+};
+
+// Face region a tile finalises in P2G: component c spans 9 entries along its
+// own axis (the +1 plane) and 8 along the others.
+template <int COMP> struct P2GDims;
+template <> struct P2GDims<0> { static constexpr int DI = 9, DJ = 8, DK = 8; };
+template <> struct P2GDims<1> { static constexpr int DI = 8, DJ = 9, DK = 8; };
+template <> struct P2GDims<2> { static constexpr int DI = 8, DJ = 8, DK = 9; };
+
+constexpr int P2G_REGION = 576;
+
+// Trilinear scatter of one velocity component into the tile's LDS region
+// (gridmath.py:71-95; out-of-grid corners dropped; corners outside the tile's
+// region are finalised by another tile and skipped here).
+template <int COMP, bool APIC>
+__device__ __forceinline__ void scatter_comp(float* s_val, float* s_wt, float px, float py,
+                                             float pz, float qx, float qy, float qz,
+                                             float vp, const float* crow, int i0t,
+                                             int j0t, int k0t, int nfi, int nfj,
+                                             int nfk, float dx) {
+    using D = P2GDims<COMP>;
+    float cx = qx, cy = qy, cz = qz;
+    if (COMP != 0) cx = qx - 0.5f;
+    if (COMP != 1) cy = qy - 0.5f;
+    if (COMP != 2) cz = qz - 0.5f;
+    const int i0 = (int)floorf(cx), j0 = (int)floorf(cy), k0 = (int)floorf(cz);
+    const float sx = cx - (float)i0, sy = cy - (float)j0, sz = cz - (float)k0;
+#pragma unroll
+    for (int di = 0; di < 2; ++di) {
+#pragma unroll
+        for (int dj = 0; dj < 2; ++dj) {
+#pragma unroll
+            for (int dk = 0; dk < 2; ++dk) {
+                const int ii = i0 + di, jj = j0 + dj, kk = k0 + dk;
+                if (ii < 0 || ii >= nfi || jj < 0 || jj >= nfj || kk < 0 || kk >= nfk) continue;
+                const int li = ii - i0t, lj = jj - j0t, lk = kk - k0t;
+                if ((unsigned)li >= (unsigned)D::DI || (unsigned)lj >= (unsigned)D::DJ ||
+                    (unsigned)lk >= (unsigned)D::DK)
+                    continue;
+                const float wxi = di ? sx : 1.0f - sx;
+                const float wyi = dj ? sy : 1.0f - sy;
+                const float wzi = dk ? sz : 1.0f - sz;
+                const float wt = wxi * wyi * wzi;
+                float val = vp;
+                if (APIC) {
+                    const float ox = (COMP == 0 ? (float)ii : (float)ii + 0.5f) * dx - px;
+                    const float oy = (COMP == 1 ? (float)jj : (float)jj + 0.5f) * dx - py;
+                    const float oz = (COMP == 2 ? (float)kk : (float)kk + 0.5f) * dx - pz;
+                    val = vp + crow[0] * ox + crow[1] * oy + crow[2] * oz;
+                }
+                const int idx = (li * D::DJ + lj) * D::DK + lk;
+                atomicAdd(&s_val[idx], val * wt);
+                atomicAdd(&s_wt[idx], wt);
+            }
+        }
+    }
+}
+
+// Flattened iteration over k-row segments of the cell-sorted particle arrays.
+struct RowSet {
+    int nrows, total;
+};
+
+__device__ __forceinline__ int row_find(const int* s_pref, int nrows, int t) {
+    int lo = 0, hi = nrows - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (s_pref[mid] <= t) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Rows (ci, cj) for ci in [ilo,ihi), cj in [jlo,jhi); each row covers cells
+// k in [klo,khi).  Thread r < nrows computes row r; returns the flattened count.
+__device__ __forceinline__ RowSet build_rows(const int32_t* __restrict__ cell_start, int ny,
+                                             int nz, int ilo, int ihi, int jlo, int jhi,
+                                             int klo, int khi, int* s_start, int* s_pref,
+                                             int* s_wave) {
+    const int nrj = jhi - jlo;
+    const int nrows = (ihi > ilo && jhi > jlo && khi > klo) ? (ihi - ilo) * nrj : 0;
+    int len = 0;
+    if ((int)threadIdx.x < nrows) {
+        const int ci = ilo + threadIdx.x / nrj, cj = jlo + threadIdx.x % nrj;
+        const int base = (ci * ny + cj) * nz;
+        const int s = cell_start[base + klo];
+        const int e = cell_start[base + khi];
+        s_start[threadIdx.x] = s;
+        len = e - s;
+    }
+    int total;
+    int ex = gf::wg_exclusive_scan(len, s_wave, &total);
+    if ((int)threadIdx.x < nrows) s_pref[threadIdx.x] = ex;
+    __syncthreads();
+    RowSet r;
+    r.nrows = nrows;
+    r.total = total;
+    return r;
+}
+
+template <bool APIC, bool RAW>
+__global__ __launch_bounds__(GF_WG) void k_p2g_tile(P2GParams a) {
+    __shared__ float s_val[3][P2G_REGION];
+    __shared__ float s_wt[3][P2G_REGION];
+    __shared__ int8_t s_mk[10][10][10];
+    __shared__ int s_start[128], s_pref[128];
+    __shared__ int s_wave[GF_WG / GF_WAVE + 1];
+
+    const int ntiles = a.tnx * a.tny * a.tnz;
+    const int t = gf::xcd_remap(blockIdx.x, ntiles);
+    const int tk = t % a.tnz, tj = (t / a.tnz) % a.tny, ti = t / (a.tnz * a.tny);
+    const int i0 = ti * 8, j0 = tj * 8, k0 = tk * 8;
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+
+    for (int i = threadIdx.x; i < 3 * P2G_REGION; i += GF_WG) {
+        (&s_val[0][0])[i] = 0.0f;
+        (&s_wt[0][0])[i] = 0.0f;
+    }
+    if (!RAW) {
+        for (int i = threadIdx.x; i < 1000; i += GF_WG) {
+            const int ca = i / 100, cb = (i / 10) % 10, cc = i % 10;
+            const int ci = i0 - 1 + ca, cj = j0 - 1 + cb, ck = k0 - 1 + cc;
+            int8_t m = 0;
+            if (ci >= 0 && ci < nx && cj >= 0 && cj < ny && ck >= 0 && ck < nz)
+                m = (int8_t)a.marker[(ci * ny + cj) * nz + ck];
+            s_mk[ca][cb][cc] = m;
+        }
+    }
+    __syncthreads();
+
+    const RowSet rs = build_rows(a.cell_start, ny, nz, max(i0 - 1, 0), min(i0 + 9, nx),
+                                 max(j0 - 1, 0), min(j0 + 9, ny), max(k0 - 1, 0),
+                                 min(k0 + 9, nz), s_start, s_pref, s_wave);
+    const float dx = a.dx;
+    for (int q = threadIdx.x; q < rs.total; q += GF_WG) {
+        const int r = row_find(s_pref, rs.nrows, q);
+        const int pidx = s_start[r] + (q - s_pref[r]);
+        const float px = a.pos[3 * pidx + 0], py = a.pos[3 * pidx + 1], pz = a.pos[3 * pidx + 2];
+        const float vx = a.vel[3 * pidx + 0], vy = a.vel[3 * pidx + 1], vz = a.vel[3 * pidx + 2];
+        float cm[9];
+        if (APIC) {
+#pragma unroll
+            for (int c = 0; c < 9; ++c) cm[c] = a.C[9 * pidx + c];
+        }
+        const float qx = px / dx, qy = py / dx, qz = pz / dx;
+        scatter_comp<0, APIC>(s_val[0], s_wt[0], px, py, pz, qx, qy, qz, vx, cm + 0, i0, j0,
+                              k0, nx + 1, ny, nz, dx);
+        scatter_comp<1, APIC>(s_val[1], s_wt[1], px, py, pz, qx, qy, qz, vy, cm + 3, i0, j0,
+                              k0, nx, ny + 1, nz, dx);
+        scatter_comp<2, APIC>(s_val[2], s_wt[2], px, py, pz, qx, qy, qz, vz, cm + 6, i0, j0,
+                              k0, nx, ny, nz + 1, dx);
+    }
+    __syncthreads();
+
+    // Finalise the region: normalize (121-142), gravity (149-156), BC (758-799).
+    const float gdt = a.g * a.dt;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const int DI = c == 0 ? 9 : 8, DJ = c == 1 ? 9 : 8, DK = c == 2 ? 9 : 8;
+        const int fni = nx + (c == 0), fnj = ny + (c == 1), fnk = nz + (c == 2);
+        float* gf_ = c == 0 ? a.u : (c == 1 ? a.v : a.w);
+        for (int idx = threadIdx.x; idx < P2G_REGION; idx += GF_WG) {
+            const int lk = idx % DK, lj = (idx / DK) % DJ, li = idx / (DK * DJ);
+            const int ii = i0 + li, jj = j0 + lj, kk = k0 + lk;
+            if (ii >= fni || jj >= fnj || kk >= fnk) continue;
+            const bool owned = (c == 0 ? (li < 8 || ii == nx) : true) &&
+                               (c == 1 ? (lj < 8 || jj == ny) : true) &&
+                               (c == 2 ? (lk < 8 || kk == nz) : true);
+            const size_t f = ((size_t)ii * fnj + jj) * fnk + kk;
+            if (RAW) {
+                if (owned) {
+                    float* gw = c == 0 ? a.uw : (c == 1 ? a.vw : a.ww);
+                    gf_[f] += s_val[c][idx];
+                    gw[f] += s_wt[c][idx];
+                }
+                continue;
+            }
+            const float wt = s_wt[c][idx];
+            float val = wt > 1.0e-8f ? s_val[c][idx] / wt : 0.0f;
+            const float saved = val;
+            if (c == 1) val = val + gdt;
+            // Solid BC: cells on both sides of the face (local cell coords +1).
+            const int ca = li + 1, cb = lj + 1, cc = lk + 1;
+            bool lo_side, hi_side;
+            const float* solid;
+            if (c == 0) {
+                lo_side = ii == 0 || s_mk[ca - 1][cb][cc] == 2;
+                hi_side = ii == nx || s_mk[ca][cb][cc] == 2;
+                solid = a.su;
+            } else if (c == 1) {
+                lo_side = jj == 0 || s_mk[ca][cb - 1][cc] == 2;
+                hi_side = jj == ny || s_mk[ca][cb][cc] == 2;
+                solid = a.sv;
+            } else {
+                lo_side = kk == 0 || s_mk[ca][cb][cc - 1] == 2;
+                hi_side = kk == nz || s_mk[ca][cb][cc] == 2;
+                solid = a.sw;
+            }
+            if (lo_side || hi_side) val = solid ? solid[f] : 0.0f;
+            s_val[c][idx] = val;
+            if (owned) {
+                float* gs = c == 0 ? a.us : (c == 1 ? a.vs : a.ws);
+                gf_[f] = val;
+                gs[f] = saved;
+            }
+        }
+    }
+    if (RAW) return;
+    __syncthreads();
+
+    // Divergence of the tile's cells (846-871) into the compact rhs.
+    const int slot = a.tile_map[t];
+    if (slot < 0) return;
+    const float scale = a.rho * a.dx / a.dt;
+    const int lj = (threadIdx.x >> 3) & 7, lk = threadIdx.x & 7;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int li = (threadIdx.x >> 6) + 4 * h;
+        const int l = (li * 8 + lj) * 8 + lk;
+        float dv = 0.0f;
+        if (i0 + li < nx && j0 + lj < ny && k0 + lk < nz && s_mk[li + 1][lj + 1][lk + 1] == 1) {
+            const float u0 = s_val[0][(li * 8 + lj) * 8 + lk];
+            const float u1 = s_val[0][((li + 1) * 8 + lj) * 8 + lk];
+            const float v0 = s_val[1][(li * 9 + lj) * 8 + lk];
+            const float v1 = s_val[1][(li * 9 + lj + 1) * 8 + lk];
+            const float w0 = s_val[2][(li * 8 + lj) * 9 + lk];
+            const float w1 = s_val[2][(li * 8 + lj) * 9 + lk + 1];
+            const float d = u1 - u0 + v1 - v0 + w1 - w0;
+            dv = d * scale;
+        }
+        a.b[(size_t)slot * GF_TILE_CELLS + l] = dv;
+    }
+}
+
+// ---------------------------------------------------------------- G2P ----
+
+struct G2PParams {
+    float* pos; float* vel; float* C;
+    const int32_t* cell_start; const int32_t* marker; const int32_t* tile_map;
+    float* u; float* v; float* w;
+    const float* us; const float* vs; const float* ws;
+    const float* su; const float* sv; const float* sw;
+    const float* pc;
+    int nx, ny, nz, tnx, tny, tnz;
+    float dx, dt, rho, flip_blend;
+    float domx, domy, domz;
+};
+
+// Face region a tile samples: component c spans [o, o+9) on its own axis with
+// origin at the tile corner and [o-1, o+9) (10 entries) on the other axes.
+constexpr int G2P_REGION = 900;
+
+template <int COMP>
+__device__ __forceinline__ int g2p_idx(int ii, int jj, int kk, int i0, int j0, int k0) {
+    int li = ii - (COMP == 0 ? i0 : i0 - 1);
+    int lj = jj - (COMP == 1 ? j0 : j0 - 1);
+    int lk = kk - (COMP == 2 ? k0 : k0 - 1);
+    const int DI = COMP == 0 ? 9 : 10, DJ = COMP == 1 ? 9 : 10, DK = COMP == 2 ? 9 : 10;
+    li = gf::clampi(li, 0, DI - 1);
+    lj = gf::clampi(lj, 0, DJ - 1);
+    lk = gf::clampi(lk, 0, DK - 1);
+    return (li * DJ + lj) * DK + lk;
+}
+
+// sample3 of gridmath.py:30-65 on a staged region.
+template <int COMP>
+__device__ __forceinline__ float sample_region(const float* s, float fx, float fy, float fz,
+                                               int nmx, int nmy, int nmz, int i0t, int j0t,
+                                               int k0t) {
+    fx = fminf(fmaxf(fx, 0.0f), (float)nmx);
+    fy = fminf(fmaxf(fy, 0.0f), (float)nmy);
+    fz = fminf(fmaxf(fz, 0.0f), (float)nmz);
+    int i0 = (int)floorf(fx), j0 = (int)floorf(fy), k0 = (int)floorf(fz);
+    if (i0 >= nmx) i0 = nmx - 1;
+    if (j0 >= nmy) j0 = nmy - 1;
+    if (k0 >= nmz) k0 = nmz - 1;
+    if (i0 < 0) i0 = 0;
+    if (j0 < 0) j0 = 0;
+    if (k0 < 0) k0 = 0;
+    int i1 = i0 + 1, j1 = j0 + 1, k1 = k0 + 1;
+    if (i1 >= nmx) i1 = nmx - 1;
+    if (j1 >= nmy) j1 = nmy - 1;
+    if (k1 >= nmz) k1 = nmz - 1;
+    const float sx = fminf(fmaxf(fx - (float)i0, 0.0f), 1.0f);
+    const float sy = fminf(fmaxf(fy - (float)j0, 0.0f), 1.0f);
+    const float sz = fminf(fmaxf(fz - (float)k0, 0.0f), 1.0f);
+    const float c000 = s[g2p_idx<COMP>(i0, j0, k0, i0t, j0t, k0t)];
+    const float c100 = s[g2p_idx<COMP>(i1, j0, k0, i0t, j0t, k0t)];
+    const float c010 = s[g2p_idx<COMP>(i0, j1, k0, i0t, j0t, k0t)];
+    const float c110 = s[g2p_idx<COMP>(i1, j1, k0, i0t, j0t, k0t)];
+    const float c001 = s[g2p_idx<COMP>(i0, j0, k1, i0t, j0t, k0t)];
+    const float c101 = s[g2p_idx<COMP>(i1, j0, k1, i0t, j0t, k0t)];
+    const float c011 = s[g2p_idx<COMP>(i0, j1, k1, i0t, j0t, k0t)];
+    const float c111 = s[g2p_idx<COMP>(i1, j1, k1, i0t, j0t, k0t)];
+    const float c00 = c000 * (1.0f - sx) + c100 * sx;
+    const float c10 = c010 * (1.0f - sx) + c110 * sx;
+    const float c01 = c001 * (1.0f - sx) + c101 * sx;
+    const float c11 = c011 * (1.0f - sx) + c111 * sx;
+    const float c0 = c00 * (1.0f - sy) + c10 * sy;
+    const float c1 = c01 * (1.0f - sy) + c11 * sy;
+    return c0 * (1.0f - sz) + c1 * sz;
+}
+
+// APIC gather of one component (solver3d.py:1620-1716): clamped corner
+// indices, unclamped weights, affine row from world-space face offsets.
+template <int COMP>
+__device__ __forceinline__ void apic_gather(const float* s, float px, float py, float pz,
+                                            float qx, float qy, float qz, int nx, int ny,
+                                            int nz, float dx, float inv_dx2_3, int i0t,
+                                            int j0t, int k0t, float& vout, float* crow) {
+    float fx = qx, fy = qy, fz = qz;
+    if (COMP != 0) fx = qx - 0.5f;
+    if (COMP != 1) fy = qy - 0.5f;
+    if (COMP != 2) fz = qz - 0.5f;
+    const int i0 = (int)floorf(fx), j0 = (int)floorf(fy), k0 = (int)floorf(fz);
+    const float sx = fx - (float)i0, sy = fy - (float)j0, sz = fz - (float)k0;
+    const int imax = COMP == 0 ? nx : nx - 1;
+    const int jmax = COMP == 1 ? ny : ny - 1;
+    const int kmax = COMP == 2 ? nz : nz - 1;
+    float acc = 0.0f, c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
+#pragma unroll
+    for (int di = 0; di < 2; ++di) {
+#pragma unroll
+        for (int dj = 0; dj < 2; ++dj) {
+#pragma unroll
+            for (int dk = 0; dk < 2; ++dk) {
+                int ii = i0 + di, jj = j0 + dj, kk = k0 + dk;
+                if (ii < 0) ii = 0;
+                if (jj < 0) jj = 0;
+                if (kk < 0) kk = 0;
+                if (ii > imax) ii = imax;
+                if (jj > jmax) jj = jmax;
+                if (kk > kmax) kk = kmax;
+                const float wxi = di ? sx : 1.0f - sx;
+                const float wyi = dj ? sy : 1.0f - sy;
+                const float wzi = dk ? sz : 1.0f - sz;
+                const float wt = wxi * wyi * wzi;
+                const float val = s[g2p_idx<COMP>(ii, jj, kk, i0t, j0t, k0t)];
+                acc += wt * val;
+                const float ox = (COMP == 0 ? (float)ii : (float)ii + 0.5f) * dx - px;
+                const float oy = (COMP == 1 ? (float)jj : (float)jj + 0.5f) * dx - py;
+                const float oz = (COMP == 2 ? (float)kk : (float)kk + 0.5f) * dx - pz;
+                const float fu = wt * val * inv_dx2_3;
+                c0 += fu * ox;
+                c1 += fu * oy;
+                c2 += fu * oz;
+            }
+        }
+    }
+    vout = acc;
+    crow[0] = c0;
+    crow[1] = c1;
+    crow[2] = c2;
+}
+
+template <bool APIC>
+__global__ __launch_bounds__(GF_WG) void k_g2p_tile(G2PParams a) {
+    __shared__ float s_fin[3][G2P_REGION];
+    __shared__ float s_sav[APIC ? 1 : 3][APIC ? 1 : G2P_REGION];
+    __shared__ float s_p[10][10][10];
+    __shared__ int8_t s_mk[10][10][10];
+    __shared__ int s_start[64], s_pref[64];
+    __shared__ int s_wave[GF_WG / GF_WAVE + 1];
+
+    const int ntiles = a.tnx * a.tny * a.tnz;
+    const int t = gf::xcd_remap(blockIdx.x, ntiles);
+    const int tk = t % a.tnz, tj = (t / a.tnz) % a.tny, ti = t / (a.tnz * a.tny);
+    const int i0 = ti * 8, j0 = tj * 8, k0 = tk * 8;
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+
+    const RowSet rs = build_rows(a.cell_start, ny, nz, i0, min(i0 + 8, nx), j0,
+                                 min(j0 + 8, ny), k0, min(k0 + 8, nz), s_start, s_pref,
+                                 s_wave);
+    // Owned faces can change only next to a fluid cell of this tile or of its
+    // lower neighbour tiles.
+    bool near_fluid = a.tile_map[t] >= 0 || (ti > 0 && a.tile_map[t - a.tny * a.tnz] >= 0) ||
+                      (tj > 0 && a.tile_map[t - a.tnz] >= 0) ||
+                      (tk > 0 && a.tile_map[t - 1] >= 0);
+    if (rs.total == 0 && !near_fluid) return;
+
+    for (int i = threadIdx.x; i < 1000; i += GF_WG) {
+        const int ca = i / 100, cb = (i / 10) % 10, cc = i % 10;
+        const int ci = i0 - 1 + ca, cj = j0 - 1 + cb, ck = k0 - 1 + cc;
+        int8_t m = 0;
+        float pv = 0.0f;
+        if (ci >= 0 && ci < nx && cj >= 0 && cj < ny && ck >= 0 && ck < nz) {
+            m = (int8_t)a.marker[(ci * ny + cj) * nz + ck];
+            if (m == 1) {
+                const int tt = ((ci >> 3) * a.tny + (cj >> 3)) * a.tnz + (ck >> 3);
+                const int slot = a.tile_map[tt];
+                if (slot >= 0)
+                    pv = a.pc[(size_t)slot * GF_TILE_CELLS +
+                              (((ci & 7) * 8 + (cj & 7)) * 8 + (ck & 7))];
+            }
+        }
+        s_mk[ca][cb][cc] = m;
+        s_p[ca][cb][cc] = pv;
+    }
+    __syncthreads();
+
+    const float scale = a.dt / (a.rho * a.dx);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const int DI = c == 0 ? 9 : 10, DJ = c == 1 ? 9 : 10, DK = c == 2 ? 9 : 10;
+        const int oi = c == 0 ? i0 : i0 - 1, oj = c == 1 ? j0 : j0 - 1,
+                  ok = c == 2 ? k0 : k0 - 1;
+        const int fni = nx + (c == 0), fnj = ny + (c == 1), fnk = nz + (c == 2);
+        float* gfld = c == 0 ? a.u : (c == 1 ? a.v : a.w);
+        const float* gsav = c == 0 ? a.us : (c == 1 ? a.vs : a.ws);
+        const float* solid = c == 0 ? a.su : (c == 1 ? a.sv : a.sw);
+        for (int idx = threadIdx.x; idx < G2P_REGION; idx += GF_WG) {
+            const int lk = idx % DK, lj = (idx / DK) % DJ, li = idx / (DK * DJ);
+            const int ii = oi + li, jj = oj + lj, kk = ok + lk;
+            float val = 0.0f, sav = 0.0f;
+            if (ii >= 0 && ii < fni && jj >= 0 && jj < fnj && kk >= 0 && kk < fnk) {
+                const size_t f = ((size_t)ii * fnj + jj) * fnk + kk;
+                const float orig = gfld[f];
+                val = orig;
+                if (!APIC) sav = gsav[f];
+                // Cell coords of the face's two sides in the staged cell region.
+                const int ca = ii - (i0 - 1), cb = jj - (j0 - 1), cc = kk - (k0 - 1);
+                int la, lb, lc;  // low-side cell
+                int gi_lo, n_ax, g_ax;
+                if (c == 0) { la = ca - 1; lb = cb; lc = cc; g_ax = ii; n_ax = nx; }
+                else if (c == 1) { la = ca; lb = cb - 1; lc = cc; g_ax = jj; n_ax = ny; }
+                else { la = ca; lb = cb; lc = cc - 1; g_ax = kk; n_ax = nz; }
+                (void)gi_lo;
+                // ca..cc lie in [0,10) for every staged face except the +1
+                // planes of the last tile along an axis, which the grid bounds
+                // exclude from the gradient (g_ax == n_ax) anyway.
+                const bool hi_in = ca < 10 && cb < 10 && cc < 10;
+                const bool lo_in = la >= 0 && lb >= 0 && lc >= 0 && la < 10 && lb < 10 && lc < 10;
+                const int mlo = lo_in ? s_mk[la][lb][lc] : 0;
+                const int mhi = hi_in ? s_mk[ca][cb][cc] : 0;
+                bool changed = false;
+                if (g_ax >= 1 && g_ax <= n_ax - 1) {
+                    if ((mlo == 1 || mhi == 1) && mlo != 2 && mhi != 2) {
+                        const float plo = mlo == 1 ? s_p[la][lb][lc] : 0.0f;
+                        const float phi = mhi == 1 ? s_p[ca][cb][cc] : 0.0f;
+                        val = val - scale * (phi - plo);
+                        changed = true;
+                    }
+                }
+                const bool lo_side = g_ax == 0 || mlo == 2;
+                const bool hi_side = g_ax == n_ax || mhi == 2;
+                if (lo_side || hi_side) {
+                    const float sv = solid ? solid[f] : 0.0f;
+                    if (sv != val) { val = sv; changed = true; }
+                }
+                const bool owned = (c == 0 ? (li < 8 || ii == nx) : (li >= 1 && li < 9)) &&
+                                   (c == 1 ? (lj < 8 || jj == ny) : (lj >= 1 && lj < 9)) &&
+                                   (c == 2 ? (lk < 8 || kk == nz) : (lk >= 1 && lk < 9));
+                if (owned && changed) gfld[f] = val;
+            }
+            s_fin[c][idx] = val;
+            if (!APIC) s_sav[c][idx] = sav;
+        }
+    }
+    __syncthreads();
+
+    const float dx = a.dx, dt = a.dt;
+    const float eps = dx * 1.001f;
+    for (int q = threadIdx.x; q < rs.total; q += GF_WG) {
+        const int r = row_find(s_pref, rs.nrows, q);
+        const int pidx = s_start[r] + (q - s_pref[r]);
+        const float px = a.pos[3 * pidx + 0], py = a.pos[3 * pidx + 1], pz = a.pos[3 * pidx + 2];
+        const float lx = px / dx, ly = py / dx, lz = pz / dx;
+        float nu, nv, nw;
+        if (!APIC) {
+            const float pic_u = sample_region<0>(s_fin[0], lx, ly - 0.5f, lz - 0.5f, nx + 1, ny, nz, i0, j0, k0);
+            const float old_u = sample_region<0>(s_sav[0], lx, ly - 0.5f, lz - 0.5f, nx + 1, ny, nz, i0, j0, k0);
+            const float pic_v = sample_region<1>(s_fin[1], lx - 0.5f, ly, lz - 0.5f, nx, ny + 1, nz, i0, j0, k0);
+            const float old_v = sample_region<1>(s_sav[1], lx - 0.5f, ly, lz - 0.5f, nx, ny + 1, nz, i0, j0, k0);
+            const float pic_w = sample_region<2>(s_fin[2], lx - 0.5f, ly - 0.5f, lz, nx, ny, nz + 1, i0, j0, k0);
+            const float old_w = sample_region<2>(s_sav[2], lx - 0.5f, ly - 0.5f, lz, nx, ny, nz + 1, i0, j0, k0);
+            const float ovx = a.vel[3 * pidx + 0], ovy = a.vel[3 * pidx + 1], ovz = a.vel[3 * pidx + 2];
+            const float flip_u = ovx + (pic_u - old_u);
+            const float flip_v = ovy + (pic_v - old_v);
+            const float flip_w = ovz + (pic_w - old_w);
+            const float fb = a.flip_blend;
+            nu = fb * flip_u + (1.0f - fb) * pic_u;
+            nv = fb * flip_v + (1.0f - fb) * pic_v;
+            nw = fb * flip_w + (1.0f - fb) * pic_w;
+        } else {
+            const float inv_dx2_3 = 3.0f / (dx * dx);
+            float cm[9];
+            apic_gather<0>(s_fin[0], px, py, pz, lx, ly, lz, nx, ny, nz, dx, inv_dx2_3, i0, j0, k0, nu, cm + 0);
+            apic_gather<1>(s_fin[1], px, py, pz, lx, ly, lz, nx, ny, nz, dx, inv_dx2_3, i0, j0, k0, nv, cm + 3);
+            apic_gather<2>(s_fin[2], px, py, pz, lx, ly, lz, nx, ny, nz, dx, inv_dx2_3, i0, j0, k0, nw, cm + 6);
+#pragma unroll
+            for (int c = 0; c < 9; ++c) a.C[9 * pidx + c] = cm[c];
+        }
+        a.vel[3 * pidx + 0] = nu;
+        a.vel[3 * pidx + 1] = nv;
+        a.vel[3 * pidx + 2] = nw;
+        float npx = px + nu * dt, npy = py + nv * dt, npz = pz + nw * dt;
+        npx = fminf(fmaxf(npx, eps), a.domx - eps);
+        npy = fminf(fmaxf(npy, eps), a.domy - eps);
+        npz = fminf(fmaxf(npz, eps), a.domz - eps);
+        a.pos[3 * pidx + 0] = npx;
+        a.pos[3 * pidx + 1] = npy;
+        a.pos[3 * pidx + 2] = npz;
+    }
+}
+
+__global__ __launch_bounds__(GF_WG) void k_compact_to_dense(const float* __restrict__ src,
+                                                            const int32_t* __restrict__ tile_map,
+                                                            float* __restrict__ dst, int nx,
+                                                            int ny, int nz, int tny, int tnz) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nx * ny * nz) return;
+    const int k = c % nz, j = (c / nz) % ny, i = c / (nz * ny);
+    const int slot = tile_map[((i >> 3) * tny + (j >> 3)) * tnz + (k >> 3)];
+    dst[c] = slot >= 0 ? src[(size_t)slot * GF_TILE_CELLS + (((i & 7) * 8 + (j & 7)) * 8 + (k & 7))]
+                       : 0.0f;
+}
+
+__global__ __launch_bounds__(GF_WG) void k_unsort(const float* __restrict__ in,
+                                                  const int32_t* __restrict__ pid, int n,
+                                                  int width, float* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int d = pid[i];
+    for (int c = 0; c < width; ++c) out[(size_t)d * width + c] = in[(size_t)i * width + c];
+}
+
+P2GParams p2g_params(const gf_scene_t* s, float g, float dt, float rho) {
+    P2GParams a;
+    a.pos = s->pos; a.vel = s->vel; a.C = s->affine_c;
+    a.cell_start = s->cell_start; a.marker = s->marker; a.tile_map = s->tile_map;
+    a.u = s->u; a.v = s->v; a.w = s->w; a.us = s->us; a.vs = s->vs; a.ws = s->ws;
+    a.uw = nullptr; a.vw = nullptr; a.ww = nullptr;
+    a.su = s->solid_u; a.sv = s->solid_v; a.sw = s->solid_w;
+    a.b = s->b;
+    a.nx = s->nx; a.ny = s->ny; a.nz = s->nz; a.tnx = s->tnx; a.tny = s->tny; a.tnz = s->tnz;
+    a.dx = s->dx; a.g = g; a.dt = dt; a.rho = rho;
+    return a;
+}
+
+}  // namespace
+
+namespace gf {
+int scene_check(const gf_scene_t* s, const char* who);
+}
+
+extern "C" int gf_scene_p2g(const gf_scene_t* s, int transfer_mode, float gravity, float dt,
+                            float rho, void* stream) {
+    int rc = gf::scene_check(s, "gf_scene_p2g");
+    if (rc) return rc;
+    GF_REQUIRE(transfer_mode == GF_TRANSFER_FLIP || transfer_mode == GF_TRANSFER_APIC,
+               "gf_scene_p2g: transfer_mode %d", transfer_mode);
+    GF_REQUIRE(s->u && s->v && s->w && s->us && s->vs && s->ws && s->b && s->marker &&
+                   s->cell_start && s->tile_map,
+               "gf_scene_p2g: grid buffers not set");
+    GF_REQUIRE(dt != 0.0f, "gf_scene_p2g: dt must be nonzero");
+    if (s->n > 0) GF_REQUIRE(s->pos && s->vel, "gf_scene_p2g: particles not set");
+    if (transfer_mode == GF_TRANSFER_APIC && s->n > 0)
+        GF_REQUIRE(s->affine_c, "gf_scene_p2g: APIC needs affine_c");
+    P2GParams a = p2g_params(s, gravity, dt, rho);
+    const int ntiles = s->tnx * s->tny * s->tnz;
+    hipStream_t st = (hipStream_t)stream;
+    if (transfer_mode == GF_TRANSFER_APIC)
+        hipLaunchKernelGGL((k_p2g_tile<true, false>), dim3(ntiles), dim3(GF_WG), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_p2g_tile<false, false>), dim3(ntiles), dim3(GF_WG), 0, st, a);
+    GF_LAUNCH_CHECK("k_p2g_tile");
+    return 0;
+}
+
+extern "C" int gf_scene_p2g_raw(const gf_scene_t* s, int transfer_mode, float* uw, float* vw,
+                                float* ww, void* stream) {
+    int rc = gf::scene_check(s, "gf_scene_p2g_raw");
+    if (rc) return rc;
+    GF_REQUIRE(transfer_mode == GF_TRANSFER_FLIP || transfer_mode == GF_TRANSFER_APIC,
+               "gf_scene_p2g_raw: transfer_mode %d", transfer_mode);
+    GF_REQUIRE(s->u && s->v && s->w && uw && vw && ww && s->cell_start,
+               "gf_scene_p2g_raw: grid buffers not set");
+    if (s->n > 0) GF_REQUIRE(s->pos && s->vel, "gf_scene_p2g_raw: particles not set");
+    if (transfer_mode == GF_TRANSFER_APIC && s->n > 0)
+        GF_REQUIRE(s->affine_c, "gf_scene_p2g_raw: APIC needs affine_c");
+    P2GParams a = p2g_params(s, 0.0f, 1.0f, 1.0f);
+    a.uw = uw; a.vw = vw; a.ww = ww;
+    const int ntiles = s->tnx * s->tny * s->tnz;
+    hipStream_t st = (hipStream_t)stream;
+    if (transfer_mode == GF_TRANSFER_APIC)
+        hipLaunchKernelGGL((k_p2g_tile<true, true>), dim3(ntiles), dim3(GF_WG), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_p2g_tile<false, true>), dim3(ntiles), dim3(GF_WG), 0, st, a);
+    GF_LAUNCH_CHECK("k_p2g_tile(raw)");
+    return 0;
+}
+
+extern "C" int gf_scene_g2p(const gf_scene_t* s, const float* pressure, int transfer_mode,
+                            float dt, float rho, float flip_blend, void* stream) {
+    int rc = gf::scene_check(s, "gf_scene_g2p");
+    if (rc) return rc;
+    GF_REQUIRE(transfer_mode == GF_TRANSFER_FLIP || transfer_mode == GF_TRANSFER_APIC,
+               "gf_scene_g2p: transfer_mode %d", transfer_mode);
+    GF_REQUIRE(s->u && s->v && s->w && s->marker && s->cell_start && s->tile_map && pressure,
+               "gf_scene_g2p: grid buffers not set");
+    GF_REQUIRE(rho != 0.0f, "gf_scene_g2p: rho must be nonzero");
+    if (transfer_mode == GF_TRANSFER_FLIP)
+        GF_REQUIRE(s->us && s->vs && s->ws, "gf_scene_g2p: FLIP needs us/vs/ws");
+    if (s->n > 0) GF_REQUIRE(s->pos && s->vel, "gf_scene_g2p: particles not set");
+    if (transfer_mode == GF_TRANSFER_APIC && s->n > 0)
+        GF_REQUIRE(s->affine_c, "gf_scene_g2p: APIC needs affine_c");
+    G2PParams a;
+    a.pos = s->pos; a.vel = s->vel; a.C = s->affine_c;
+    a.cell_start = s->cell_start; a.marker = s->marker; a.tile_map = s->tile_map;
+    a.u = s->u; a.v = s->v; a.w = s->w; a.us = s->us; a.vs = s->vs; a.ws = s->ws;
+    a.su = s->solid_u; a.sv = s->solid_v; a.sw = s->solid_w;
+    a.pc = pressure;
+    a.nx = s->nx; a.ny = s->ny; a.nz = s->nz; a.tnx = s->tnx; a.tny = s->tny; a.tnz = s->tnz;
+    a.dx = s->dx; a.dt = dt; a.rho = rho; a.flip_blend = flip_blend;
+    a.domx = s->dom[0]; a.domy = s->dom[1]; a.domz = s->dom[2];
+    const int ntiles = s->tnx * s->tny * s->tnz;
+    hipStream_t st = (hipStream_t)stream;
+    if (transfer_mode == GF_TRANSFER_APIC)
+        hipLaunchKernelGGL(k_g2p_tile<true>, dim3(ntiles), dim3(GF_WG), 0, st, a);
+    else
+        hipLaunchKernelGGL(k_g2p_tile<false>, dim3(ntiles), dim3(GF_WG), 0, st, a);
+    GF_LAUNCH_CHECK("k_g2p_tile");
+    return 0;
+}
+
+extern "C" int gf_scene_compact_to_dense(const gf_scene_t* s, const float* compact,
+                                         float* dense, void* stream) {
+    int rc = gf::scene_check(s, "gf_scene_compact_to_dense");
+    if (rc) return rc;
+    GF_REQUIRE(compact && dense && s->tile_map, "gf_scene_compact_to_dense: null buffer");
+    const int C = s->nx * s->ny * s->nz;
+    hipLaunchKernelGGL(k_compact_to_dense, dim3((C + GF_WG - 1) / GF_WG), dim3(GF_WG), 0,
+                       (hipStream_t)stream, compact, s->tile_map, dense, s->nx, s->ny, s->nz,
+                       s->tny, s->tnz);
+    GF_LAUNCH_CHECK("k_compact_to_dense");
+    return 0;
+}
+
+extern "C" int gf_scene_unsort(const gf_scene_t* s, const float* in, int width, float* out,
+                               void* stream) {
+    int rc = gf::scene_check(s, "gf_scene_unsort");
+    if (rc) return rc;
+    GF_REQUIRE(width >= 1 && width <= 16, "gf_scene_unsort: width %d", width);
+    if (s->n == 0) return 0;
+    GF_REQUIRE(in && out && s->pid, "gf_scene_unsort: null buffer");
+    hipLaunchKernelGGL(k_unsort, dim3((s->n + GF_WG - 1) / GF_WG), dim3(GF_WG), 0,
+                       (hipStream_t)stream, in, s->pid, s->n, width, out);
+    GF_LAUNCH_CHECK("k_unsort");
+    return 0;
+}

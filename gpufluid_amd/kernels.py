@@ -1,0 +1,267 @@
+"""S2 block interface: the reference's kernels as Python callables.
+
+Each function replaces the Warp kernel of the same name
+(src/gpufluid/solvers/solver3d.py) and is registered under the same block id.
+Arguments keep the reference order and meaning with ``wp.launch``'s ``dim``
+and ``device`` dropped: arrays are DeviceArray or torch tensors on the HIP
+device, scalars are Python numbers (float -> FP32, int -> int32), and the
+trailing sub-dense offsets ``(off_x, off_y, off_z)`` must be 0.  Work is
+enqueued on torch's current stream; nothing is returned (outputs are written
+in place), as with ``wp.launch``.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _lib
+from .blocks import BlockError, block
+from .device import DeviceArray
+from .scene import SceneBuffers
+
+
+def _t(a):
+    if a is None:
+        return None
+    t = a.t if isinstance(a, DeviceArray) else a
+    if not isinstance(t, torch.Tensor) or t.device.type != "cuda":
+        raise TypeError("block kernels take HIP device arrays (DeviceArray or torch tensor)")
+    if not t.is_contiguous():
+        raise ValueError("block kernels need C-contiguous arrays")
+    return t
+
+
+def _p(a):
+    return _lib.ptr(_t(a))
+
+
+def _offsets(block_id, *offs):
+    if any(int(o) != 0 for o in offs):
+        raise BlockError(block_id, "sub-dense offsets are not supported (dense storage only)")
+
+
+def _st(a):
+    return _lib.stream_handle(_t(a).device)
+
+
+def _cells(marker):
+    m = _t(marker)
+    if m.dim() != 3 or m.dtype != torch.int32:
+        raise ValueError("marker must be an int32 (nx, ny, nz) array")
+    return tuple(int(x) for x in m.shape)
+
+
+@block("S2.4", "clear grid: zero faces/weights, non-solid marker -> air")
+def k3_clear_grid(u, v, w, uw, vw, ww, marker, off_x=0, off_y=0, off_z=0):
+    _offsets("S2.4", off_x, off_y, off_z)
+    nx, ny, nz = _cells(marker)
+    _lib.call("S2.4", "gf_k3_clear_grid", _p(u), _p(v), _p(w), _p(uw), _p(vw), _p(ww),
+              _p(marker), nx, ny, nz, _st(marker))
+
+
+_P2G_SCENES: dict = {}
+
+
+def _p2g_block(block_id, pos, vel, affine, u, v, w, uw, vw, ww, marker, dx, nx, ny, nz):
+    m = _t(marker)
+    shape = _cells(marker)
+    if (nx, ny, nz) != shape:
+        raise BlockError(block_id, f"grid ({nx},{ny},{nz}) does not match marker {shape}")
+    pos_t = _t(pos).reshape(-1, 3)
+    n = pos_t.shape[0]
+    dev = pos_t.device
+    stream = _st(pos)
+    # Stable counting sort into a scratch scene, raw LDS accumulation, then the
+    # reference's marker side effect on the caller's marker.
+    key = (shape, float(dx), dev)
+    sc = _P2G_SCENES.get(key)
+    if sc is None:
+        import numpy as np
+        dom = tuple(float(np.float32(k * dx)) for k in shape)
+        sc = SceneBuffers(nx, ny, nz, dx, dom, dev)
+        sc.scratch_marker = torch.zeros(shape, dtype=torch.int32, device=dev)
+        _P2G_SCENES.clear()
+        _P2G_SCENES[key] = sc
+    apic = affine is not None
+    sc.ensure_particles(n, apic)
+    live = sc.live(n)
+    if n:
+        live["pos"].copy_(pos_t)
+        live["vel"].copy_(_t(vel).reshape(n, 3))
+        live["pid"].copy_(torch.arange(n, dtype=torch.int32, device=dev))
+        if apic:
+            live["affine"].copy_(_t(affine).reshape(n, 9))
+    sc.scratch_marker.copy_(torch.where(m == 2, m, torch.zeros_like(m)))
+    dummy = torch.empty(0, device=dev)
+    fields = {"marker": sc.scratch_marker, "u": _t(u), "v": _t(v), "w": _t(w),
+              "us": dummy, "vs": dummy, "ws": dummy}
+    s = sc.descriptor(fields, n, apic)
+    _lib.call(block_id, "gf_scene_sort_sync", ctypes.byref(s), int(apic), None, stream)
+    if n:
+        sc.swap()
+    s = sc.descriptor(fields, n, apic)
+    mode = _lib.GF_TRANSFER_APIC if apic else _lib.GF_TRANSFER_FLIP
+    _lib.call(block_id, "gf_scene_p2g_raw", ctypes.byref(s), mode, _p(uw), _p(vw), _p(ww), stream)
+    _lib.call(block_id, "gf_k3_mark_fluid", _lib.ptr(pos_t.contiguous()), n, _p(marker),
+              float(dx), nx, ny, nz, stream)
+
+
+@block("S2.1", "P2G: scatter particle velocities -> MAC faces (trilinear)")
+def k3_p2g(pos, vel, u, v, w, uw, vw, ww, marker, dx, nx, ny, nz, off_x=0, off_y=0, off_z=0):
+    """Accumulates into u,v,w and uw,vw,ww (+=) and marks particle cells
+    fluid (solver3d.py:72-114)."""
+    _offsets("S2.1", off_x, off_y, off_z)
+    _p2g_block("S2.1", pos, vel, None, u, v, w, uw, vw, ww, marker, dx, nx, ny, nz)
+
+
+@block("S2.12", "APIC P2G: scatter (vel + affine*offset) per face")
+def k3_p2g_apic(pos, vel, affine_C, u, v, w, uw, vw, ww, marker, dx, nx, ny, nz,
+                off_x=0, off_y=0, off_z=0):
+    """solver3d.py:1475-1588"""
+    _offsets("S2.12", off_x, off_y, off_z)
+    _p2g_block("S2.12", pos, vel, affine_C, u, v, w, uw, vw, ww, marker, dx, nx, ny, nz)
+
+
+@block("S2.2", "Normalize MAC faces and save pre-pressure copy")
+def k3_normalize(u, v, w, uw, vw, ww, us, vs, ws):
+    nx, ny, nz = int(_t(v).shape[0]), int(_t(u).shape[1]), int(_t(u).shape[2])
+    _lib.call("S2.2", "gf_k3_normalize", _p(u), _p(v), _p(w), _p(uw), _p(vw), _p(ww), _p(us),
+              _p(vs), _p(ws), nx, ny, nz, _st(u))
+
+
+@block("S2.3", "Add gravity g*dt to v faces")
+def k3_add_gravity(v, g, dt):
+    t = _t(v)
+    nx, ny1, nz = (int(x) for x in t.shape)
+    _lib.call("S2.3", "gf_k3_add_gravity", _p(v), nx, ny1 - 1, nz, float(g), float(dt), _st(v))
+
+
+@block("S2.4", "Solid boundary condition: faces = obstacle velocity")
+def k3_enforce_solid_bc(u, v, w, marker, solid_u, solid_v, solid_w, nx, ny, nz,
+                        off_x=0, off_y=0, off_z=0):
+    _offsets("S2.4", off_x, off_y, off_z)
+    _lib.call("S2.4", "gf_k3_enforce_solid_bc", _p(u), _p(v), _p(w), _p(marker), _p(solid_u),
+              _p(solid_v), _p(solid_w), int(nx), int(ny), int(nz), _st(u))
+
+
+@block("S2.5", "Cell-wise divergence x rho*dx/dt (Poisson RHS)")
+def k3_compute_divergence(u, v, w, div, marker, dx, dt, rho, off_x=0, off_y=0, off_z=0):
+    _offsets("S2.5", off_x, off_y, off_z)
+    nx, ny, nz = _cells(marker)
+    _lib.call("S2.5", "gf_k3_compute_divergence", _p(u), _p(v), _p(w), _p(div), _p(marker), nx,
+              ny, nz, float(dx), float(dt), float(rho), _st(u))
+
+
+@block("S2.6.1", "Jacobi pressure iteration (one sweep)")
+def k3_jacobi_pressure(p_in, p_out, div, marker, off_x=0, off_y=0, off_z=0):
+    _offsets("S2.6.1", off_x, off_y, off_z)
+    nx, ny, nz = _cells(marker)
+    _lib.call("S2.6.1", "gf_k3_jacobi_pressure", _p(p_in), _p(p_out), _p(div), _p(marker), nx,
+              ny, nz, _st(p_in))
+
+
+@block("S2.6.3", "PCG: apply Laplacian operator A to x")
+def k3_apply_A(x, y, marker, done, off_x=0, off_y=0, off_z=0):
+    _offsets("S2.6.3", off_x, off_y, off_z)
+    nx, ny, nz = _cells(marker)
+    _lib.call("S2.6.3", "gf_k3_apply_A", _p(x), _p(y), _p(marker), _p(done), nx, ny, nz, _st(x))
+
+
+@block("S2.6.3", "PCG: Jacobi preconditioner diagonal")
+def k3_compute_diag(marker, diag, off_x=0, off_y=0, off_z=0):
+    _offsets("S2.6.3", off_x, off_y, off_z)
+    nx, ny, nz = _cells(marker)
+    _lib.call("S2.6.3", "gf_k3_compute_diag", _p(marker), _p(diag), nx, ny, nz, _st(marker))
+
+
+@block("S2.6.3", "PCG: z = r / diag on fluid cells")
+def k3_apply_invM(r, diag, z, marker, done, off_x=0, off_y=0, off_z=0):
+    _offsets("S2.6.3", off_x, off_y, off_z)
+    nx, ny, nz = _cells(marker)
+    _lib.call("S2.6.3", "gf_k3_apply_invM", _p(r), _p(diag), _p(z), _p(marker), _p(done), nx,
+              ny, nz, _st(r))
+
+
+@block("S2.6.3", "PCG: fluid-cell dot product accumulated into out[0]")
+def k3_dot_fluid(a, b, out, marker, done, off_x=0, off_y=0, off_z=0):
+    _offsets("S2.6.3", off_x, off_y, off_z)
+    nx, ny, nz = _cells(marker)
+    _lib.call("S2.6.3", "gf_k3_dot_fluid", _p(a), _p(b), _p(out), _p(marker), _p(done), nx, ny,
+              nz, _st(a))
+
+
+@block("S2.6.3", "GPU-resident PCG axpy with a device scalar")
+def k3_axpy_devscalar(a_dev, sign, x, y_in, y_out, marker, done, off_x=0, off_y=0, off_z=0):
+    _offsets("S2.6.3", off_x, off_y, off_z)
+    nx, ny, nz = _cells(marker)
+    _lib.call("S2.6.3", "gf_k3_axpy_devscalar", _p(a_dev), float(sign), _p(x), _p(y_in),
+              _p(y_out), _p(marker), _p(done), nx, ny, nz, _st(x))
+
+
+def _scalar(op, a=None, b=None, c=None, tol=0.0, done=None):
+    ref = next(x for x in (a, b, c, done) if x is not None)
+    _lib.call("S2.6.3", "gf_k3_scalar_op", op, _p(a), _p(b), _p(c), float(tol), _p(done), _st(ref))
+
+
+def k3_zero_scalar(s, done):
+    _scalar(0, a=s, done=done)
+
+
+def k3_div_scalar(numer, denom, out, done):
+    _scalar(1, a=numer, b=denom, c=out, done=done)
+
+
+def k3_copy_scalar(src, dst, done):
+    _scalar(2, a=src, c=dst, done=done)
+
+
+def k3_set_tol_sq(r0_norm2, tol, tol_sq):
+    _scalar(3, a=r0_norm2, c=tol_sq, tol=tol)
+
+
+@block("S2.6.3", "PCG on-device convergence check (stop flag)")
+def k3_check_converged(r_norm2, tol_sq, done):
+    _scalar(4, a=r_norm2, b=tol_sq, done=done)
+
+
+def k3_zero_int_scalar(s):
+    _scalar(5, done=s)
+
+
+@block("S2.7", "Subtract pressure gradient from MAC faces")
+def k3_subtract_pressure_grad(u, v, w, p, marker, dx, dt, rho, off_x=0, off_y=0, off_z=0):
+    _offsets("S2.7", off_x, off_y, off_z)
+    nx, ny, nz = _cells(marker)
+    _lib.call("S2.7", "gf_k3_subtract_pressure_grad", _p(u), _p(v), _p(w), _p(p), _p(marker),
+              nx, ny, nz, float(dx), float(dt), float(rho), _st(u))
+
+
+def _dom(dom):
+    vals = [float(x) for x in (dom if not hasattr(dom, "numpy") else dom.numpy())]
+    return (ctypes.c_float * 3)(*vals)
+
+
+@block("S2.8", "G2P sample + FLIP/PIC blend (fused with S2.9)")
+def k3_g2p_and_advect(pos, vel, u, v, w, us, vs, ws, dx, dt, nx, ny, nz, flip_blend, dom,
+                      off_x=0, off_y=0, off_z=0):
+    _offsets("S2.8", off_x, off_y, off_z)
+    n = int(_t(pos).reshape(-1, 3).shape[0])
+    _lib.call("S2.8", "gf_k3_g2p_and_advect", _p(pos), _p(vel), _p(u), _p(v), _p(w), _p(us),
+              _p(vs), _p(ws), n, float(dx), float(dt), int(nx), int(ny), int(nz),
+              float(flip_blend), _dom(dom), _st(pos))
+
+
+@block("S2.12", "APIC G2P + advect (recovers C, no FLIP/PIC blend)")
+def k3_g2p_apic_advect(pos, vel, affine_C, u, v, w, dx, dt, nx, ny, nz, dom,
+                       off_x=0, off_y=0, off_z=0):
+    _offsets("S2.12", off_x, off_y, off_z)
+    n = int(_t(pos).reshape(-1, 3).shape[0])
+    _lib.call("S2.12", "gf_k3_g2p_apic_advect", _p(pos), _p(vel), _p(affine_C), _p(u), _p(v),
+              _p(w), n, float(dx), float(dt), int(nx), int(ny), int(nz), _dom(dom), _st(pos))
+
+
+@block("S2.10.GPU", "GPU reduction: max |v|^2 (CFL helper)")
+def k3_vmax_sq_reduce(vel, out_max_sq):
+    n = int(_t(vel).reshape(-1, 3).shape[0])
+    _lib.call("S2.10.GPU", "gf_k3_vmax_sq_reduce", _p(vel), n, _p(out_max_sq), _st(vel))

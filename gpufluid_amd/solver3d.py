@@ -1,0 +1,429 @@
+"""FlipSolver3D: the reference solver's Python surface over the gfx950 kernels.
+
+Drop-in for src/gpufluid/solvers/solver3d.py:2346-4222 (``FlipSolver3D``) on
+the substep hot path: constructor arguments, ``seed_box``, ``step``,
+``step_cfl``, ``get_particles``, ``has_diverged``, ``add_solid_from_sdf``,
+``cell_centers_np`` and the grid/particle attributes keep the reference's
+names, defaults and meaning.  Each substep runs four stages of hand-written
+HIP kernels (libgpufluid_hip.so):
+
+  clear       gf_scene_sort_sync   bin + stable counting sort of particles by
+                                   cell, marker, active 8^3 tiles
+  p2g         gf_scene_p2g         P2G + normalize + gravity + BC + divergence
+  pressure    gf_scene_jacobi | gf_scene_pcg_sync   on compacted active tiles
+  g2p_advect  gf_scene_g2p         gradient subtract + BC + G2P + advection
+
+Differences from the reference that callers can observe (all deliberate):
+  * particles are stored in cell-sorted order; ``pos``/``vel``/``affine_C``
+    and ``get_particles()`` return them in the original particle order;
+  * PCG solves the Jacobi-consistent system A p = -div by default
+    (``pcg_sign="physical"``); the reference PCG returns -p because it starts
+    from r = +div (solver3d.py:2962-2963) — ``pcg_sign="reference"``
+    reproduces that literally;
+  * PCG checks convergence every iteration on the device (the host reads the
+    flag every ``pcg_check_every`` iterations) and reports the exact count;
+    the tolerance is exposed (``pcg_tol``, ``step(pressure_tol=...)``);
+  * dot products are FP64 and deterministic; ``pcg_precision="mixed"`` adds
+    FP64 residual replacement so tolerances down to 1e-8 are reachable.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+from .blocks import BlockError, block
+from .device import DeviceArray, as_tensor, default_device
+from .profiling import StepProfiler
+from .scene import SceneBuffers
+from .seeding import seed_box_positions
+
+
+class FlipDivergenceError(RuntimeError):
+    """Raised by frame loops when ``has_diverged()`` is true (solver3d.py:2317-2343)."""
+
+    def __init__(self, frame: int, last_dumped_frame: int, requested_frames: int,
+                 detail: str = "") -> None:
+        super().__init__(detail or f"FLIP solver diverged at frame {frame} "
+                         f"(last dumped: {last_dumped_frame}/{requested_frames})")
+        self.frame = int(frame)
+        self.last_dumped_frame = int(last_dumped_frame)
+        self.requested_frames = int(requested_frames)
+
+
+@block("S2.10", "Host CFL substep count from a particle velocity array")
+def cfl_substep_count(vel_np: np.ndarray, dx: float, target_dt: float, cfl: float = 0.5,
+                      max_substeps: int = 32) -> int:
+    """ceil(target_dt / (cfl*dx/|v|max)) clamped to [1, max_substeps]
+    (solver3d.py:2218-2238)."""
+    if vel_np.size == 0:
+        return 1
+    vmax = float(np.linalg.norm(vel_np, axis=1).max())
+    if vmax < 1e-6:
+        return 1
+    n = max(1, int(np.ceil(target_dt / (cfl * dx / vmax))))
+    return min(n, max_substeps)
+
+
+_TRANSFER = {"flip": _lib.GF_TRANSFER_FLIP, "pic": _lib.GF_TRANSFER_FLIP,
+             "apic": _lib.GF_TRANSFER_APIC}
+
+
+@block("F3.2", "FLIP/PIC/APIC solver on MI355X: MAC grid state and substep pipeline")
+class FlipSolver3D:
+    """3D FLIP/PIC/APIC solver holding all state on the HIP device.
+
+    Parameters follow solver3d.py:2358-2371.  viscosity, surface tension and
+    sub-dense storage are not part of this build and raise
+    NotImplementedError when requested.  Extra keyword-only parameters:
+    pcg_precision ("mixed" | "fp64"), pcg_sign ("physical" | "reference"),
+    pcg_tol (reference default 1e-4), pcg_check_every (host readback cadence).
+    """
+
+    def __init__(self, nx: int = 48, ny: int = 48, nz: int = 48, dx: float | None = None,
+                 gravity: float = -9.81, flip_blend: float = 0.95, rho: float = 1.0,
+                 viscosity: float = 0.0, viscosity_iters: int = 12,
+                 surface_tension: float = 0.0, csf_smoothing_passes: int = 2,
+                 transfer_mode: str = "flip", device=None, enable_timing: bool = False,
+                 enable_cuda_graphs: bool = False, enable_sub_dense: bool = False,
+                 sub_rebuild_every: int = 8, sub_dilation: int = 4, *,
+                 pcg_precision: str = "mixed", pcg_sign: str = "physical",
+                 pcg_tol: float = 1e-4, pcg_check_every: int = 8):
+        if viscosity > 0.0:
+            raise NotImplementedError("viscosity (S2.13) is not part of the MI355X substep build")
+        if surface_tension > 0.0:
+            raise NotImplementedError("surface tension (S2.14) is not part of the MI355X substep build")
+        if enable_sub_dense:
+            raise NotImplementedError("sub-dense storage (F3.7) is not part of this build; "
+                                      "pressure storage is already compacted to active tiles")
+        if transfer_mode not in _TRANSFER:
+            raise ValueError(f"unknown transfer_mode: {transfer_mode!r}")
+        if pcg_precision not in ("mixed", "fp64"):
+            raise ValueError(f"unknown pcg_precision: {pcg_precision!r}")
+        if pcg_sign not in ("physical", "reference"):
+            raise ValueError(f"unknown pcg_sign: {pcg_sign!r}")
+        self.device = torch.device(device) if device is not None else default_device()
+        if self.device.type != "cuda":
+            raise RuntimeError("FlipSolver3D runs on the HIP device only")
+        _lib.lib()  # fail now if the HIP library is missing
+        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
+        self.dx = dx if dx is not None else 1.0 / max(nx, ny, nz)
+        self.dom = tuple(float(np.float32(n * self.dx)) for n in (self.nx, self.ny, self.nz))
+        self.gravity = gravity
+        self.flip_blend = flip_blend
+        self.rho = rho
+        self.viscosity = viscosity
+        self.viscosity_iters = viscosity_iters
+        self.surface_tension = surface_tension
+        self.csf_smoothing_passes = max(1, int(csf_smoothing_passes))
+        self.transfer_mode = transfer_mode
+        self.pcg_precision = pcg_precision
+        self.pcg_sign = pcg_sign
+        self.pcg_tol = float(pcg_tol)
+        self.pcg_check_every = int(pcg_check_every)
+        self._prof = StepProfiler(enabled=enable_timing, device=self.device)
+        self._enable_cuda_graphs = bool(enable_cuda_graphs)
+        self._cuda_graph = None
+        self._cuda_graph_key = None
+        self._cuda_graph_hits = 0
+        self._cuda_graph_misses = 0
+
+        dev = self.device
+        f32 = dict(dtype=torch.float32, device=dev)
+        shp_u, shp_v, shp_w = (nx + 1, ny, nz), (nx, ny + 1, nz), (nx, ny, nz + 1)
+        self._f = {
+            "u": torch.zeros(shp_u, **f32), "v": torch.zeros(shp_v, **f32),
+            "w": torch.zeros(shp_w, **f32),
+            "us": torch.zeros(shp_u, **f32), "vs": torch.zeros(shp_v, **f32),
+            "ws": torch.zeros(shp_w, **f32),
+            "solid_u": torch.zeros(shp_u, **f32), "solid_v": torch.zeros(shp_v, **f32),
+            "solid_w": torch.zeros(shp_w, **f32),
+        }
+        m = np.zeros((nx, ny, nz), dtype=np.int32)
+        m[0, :, :] = 2; m[-1, :, :] = 2
+        m[:, 0, :] = 2; m[:, -1, :] = 2
+        m[:, :, 0] = 2; m[:, :, -1] = 2
+        self._marker_host = m
+        self._wall_marker = m.copy()
+        self._f["marker"] = torch.as_tensor(m, device=dev).contiguous()
+        self._uvw_weights = None
+        self._sc = SceneBuffers(self.nx, self.ny, self.nz, self.dx, self.dom, dev)
+        self.n_particles = 0
+        self.last_pressure_iters = 0
+        self.last_pressure_residual = None
+        self.last_pressure_replacements = 0
+        self._cache = {}
+        self._pcg_done = DeviceArray(torch.zeros(1, dtype=torch.int32, device=dev))
+        self._pcg_tol_sq = DeviceArray(torch.zeros(1, dtype=torch.float32, device=dev))
+        self._static_obstacle_sdfs: list = []
+
+    # ------------------------------------------------------------ attributes
+    def _arr(self, name):
+        return DeviceArray(self._f[name])
+
+    u = property(lambda self: self._arr("u"))
+    v = property(lambda self: self._arr("v"))
+    w = property(lambda self: self._arr("w"))
+    us = property(lambda self: self._arr("us"))
+    vs = property(lambda self: self._arr("vs"))
+    ws = property(lambda self: self._arr("ws"))
+    solid_u = property(lambda self: self._arr("solid_u"))
+    solid_v = property(lambda self: self._arr("solid_v"))
+    solid_w = property(lambda self: self._arr("solid_w"))
+    marker = property(lambda self: self._arr("marker"))
+
+    def _weights(self):
+        """uw/vw/ww: scratch of the block-level P2G (the fused substep keeps
+        the weights in LDS and never writes them)."""
+        if self._uvw_weights is None:
+            f = self._f
+            self._uvw_weights = tuple(torch.zeros_like(f[k]) for k in ("u", "v", "w"))
+        return self._uvw_weights
+
+    uw = property(lambda self: DeviceArray(self._weights()[0]))
+    vw = property(lambda self: DeviceArray(self._weights()[1]))
+    ww = property(lambda self: DeviceArray(self._weights()[2]))
+
+    def _dense(self, key, compact):
+        t = self._cache.get(key)
+        if t is None:
+            t = torch.zeros((self.nx, self.ny, self.nz), dtype=torch.float32, device=self.device)
+            s = self._descriptor()
+            _lib.call("F3.3", "gf_scene_compact_to_dense", ctypes.byref(s), _lib.ptr(compact),
+                      _lib.ptr(t), self._stream())
+            self._cache[key] = t
+        return DeviceArray(t)
+
+    @property
+    def p(self):
+        """Pressure of the last substep as a dense (nx,ny,nz) array."""
+        return self._dense("p", self._sc.pc)
+
+    @property
+    def p_tmp(self):
+        return self._dense("p_tmp", self._sc.ptmp)
+
+    @property
+    def div(self):
+        """Divergence (Poisson right-hand side) of the last substep, dense."""
+        return self._dense("div", self._sc.b)
+
+    def _live(self, key):
+        if self.n_particles == 0 and self._sc.bufs[self._sc.cur] is None:
+            return None
+        return self._sc.live(self.n_particles)[key]
+
+    _pos = property(lambda self: self._live("pos"))
+    _vel = property(lambda self: self._live("vel"))
+    _pid = property(lambda self: self._live("pid"))
+    _affine = property(lambda self: self._live("affine"))
+
+    def _unsorted(self, key, t, width):
+        if t is None:
+            return None
+        out = self._cache.get(key)
+        if out is None:
+            out = torch.empty((self.n_particles, width), dtype=torch.float32, device=self.device)
+            s = self._descriptor()
+            _lib.call("F3.3", "gf_scene_unsort", ctypes.byref(s), _lib.ptr(t), width,
+                      _lib.ptr(out), self._stream())
+            self._cache[key] = out
+        return out
+
+    @property
+    def pos(self):
+        t = self._unsorted("pos", self._pos, 3)
+        return None if t is None else DeviceArray(t, vec=1)
+
+    @property
+    def vel(self):
+        t = self._unsorted("vel", self._vel, 3)
+        return None if t is None else DeviceArray(t, vec=1)
+
+    @property
+    def affine_C(self):
+        t = self._unsorted("affine", self._affine, 9)
+        return None if t is None else DeviceArray(t.view(-1, 3, 3), vec=2)
+
+    # ------------------------------------------------------------- particles
+    def set_particles(self, pos, vel=None, affine=None) -> None:
+        """Replace the particle set (positions (n,3), velocities (n,3) and,
+        for APIC, affine matrices (n,3,3); missing ones are zero)."""
+        dev = self.device
+        p = as_tensor(pos, torch.float32, dev).reshape(-1, 3)
+        n = p.shape[0]
+        sc = self._sc
+        apic = self.transfer_mode == "apic" or affine is not None
+        sc.ensure_particles(n, apic)
+        self.n_particles = n
+        self._cache.clear()
+        if n == 0:
+            return
+        live = sc.live(n)
+        live["pos"].copy_(p)
+        if vel is None:
+            live["vel"].zero_()
+        else:
+            live["vel"].copy_(as_tensor(vel, torch.float32, dev).reshape(n, 3))
+        live["pid"].copy_(torch.arange(n, dtype=torch.int32, device=dev))
+        if live["affine"] is not None:
+            if affine is None:
+                live["affine"].zero_()
+            else:
+                live["affine"].copy_(as_tensor(affine, torch.float32, dev).reshape(n, 9))
+
+    @block("D4.5.1", "Box seeder (uniform jittered, ppc particles per cell)")
+    def seed_box(self, lo, hi, ppc: int = 8, color=None, temperature=None):
+        """Seed [lo, hi] as FlipSolver3D.seed_box (solver3d.py:2826-2910);
+        a second call replaces the particle set (per-particle colour and
+        temperature attributes are not part of this build)."""
+        if color is not None or temperature is not None:
+            raise NotImplementedError("per-particle colour/temperature (S2.15/S2.18) are not "
+                                      "part of the MI355X substep build")
+        self.set_particles(seed_box_positions(lo, hi, ppc, self.dx))
+
+    def get_particles(self):
+        if self._pos is None:
+            return None, None
+        return self.pos.numpy(), self.vel.numpy()
+
+    def has_diverged(self) -> bool:
+        """NaN probe on particle positions (solver3d.py:2642-2656)."""
+        if self._pos is None or self.n_particles == 0:
+            return False
+        return bool(torch.isnan(self._pos).any().item())
+
+    @block("D4.4", "Inject SDF obstacle into marker grid")
+    def add_solid_from_sdf(self, sdf, padding: float = 0.0):
+        m = self._marker_host
+        m[np.asarray(sdf) <= padding] = 2
+        self._f["marker"].copy_(torch.as_tensor(m, device=self.device))
+        self._static_obstacle_sdfs.append(np.asarray(sdf, dtype=np.float32))
+
+    def cell_centers_np(self):
+        i = (np.arange(self.nx) + 0.5) * self.dx
+        j = (np.arange(self.ny) + 0.5) * self.dx
+        k = (np.arange(self.nz) + 0.5) * self.dx
+        X, Y, Z = np.meshgrid(i, j, k, indexing="ij")
+        return np.stack([X, Y, Z], axis=-1).astype(np.float32)
+
+    # --------------------------------------------------------------- substep
+    def _stream(self):
+        return _lib.stream_handle(self.device)
+
+    def _descriptor(self):
+        return self._sc.descriptor(self._f, self.n_particles, self.transfer_mode == "apic")
+
+    @block("F3.3", "Per-step pipeline: sort, fused P2G, pressure, fused G2P")
+    def step(self, dt: float, pressure_iters: int = 80, pressure_solver: str = "jacobi",
+             pressure_block_sparse: bool = False, *, pressure_tol: float | None = None):
+        """Advance one substep of length ``dt`` (solver3d.py:3422-3667).
+        ``pressure_iters`` is the Jacobi sweep count or the PCG iteration cap.
+        ``pressure_block_sparse`` is accepted for compatibility: the pressure
+        solve always runs on the active 8^3 tiles only."""
+        if pressure_solver not in ("jacobi", "pcg", "gsrb"):
+            raise ValueError(f"unknown pressure_solver: {pressure_solver!r}")
+        if pressure_solver == "gsrb":
+            raise NotImplementedError("GS-RB (S2.6.2) is not part of the MI355X substep build")
+        tol = self.pcg_tol if pressure_tol is None else float(pressure_tol)
+        self._step_impl(float(dt), int(pressure_iters), pressure_solver, tol)
+
+    def _step_impl(self, dt, pressure_iters, pressure_solver, tol):
+        prof = self._prof
+        sc = self._sc
+        apic = self.transfer_mode == "apic"
+        mode = _TRANSFER[self.transfer_mode]
+        flip_blend = self.flip_blend
+        n = self.n_particles
+        stream = self._stream()
+        self._cache.clear()
+        sc.ensure_particles(n, apic)
+        with prof.section("clear"):
+            s = self._descriptor()
+            na = ctypes.c_int(0)
+            try:
+                _lib.call("F3.3", "gf_scene_sort_sync", ctypes.byref(s), int(apic),
+                          ctypes.byref(na), stream)
+            except BlockError:
+                sc.reset_counts()
+                raise
+            if n > 0:
+                sc.swap()
+            sc.n_active_host = na.value
+        s = self._descriptor()
+        with prof.section("p2g"):
+            _lib.call("S2.1", "gf_scene_p2g", ctypes.byref(s), mode, self.gravity, dt, self.rho,
+                      stream)
+        prof.fused("normalize")
+        prof.fused("gravity_bc")
+        prof.fused("divergence")
+        with prof.section("pressure"):
+            if pressure_solver == "jacobi":
+                _lib.call("S2.6.1", "gf_scene_jacobi", ctypes.byref(s), na.value,
+                          pressure_iters, stream)
+                self.last_pressure_iters = pressure_iters
+                self.last_pressure_residual = None
+            else:
+                prec = _lib.GF_PCG_MIXED if self.pcg_precision == "mixed" else _lib.GF_PCG_FP64
+                if prec == _lib.GF_PCG_FP64:
+                    sc.ensure_fp64()
+                    s = self._descriptor()
+                it = ctypes.c_int(0)
+                res = ctypes.c_double(0.0)
+                nrep = ctypes.c_int(0)
+                sign = -1 if self.pcg_sign == "physical" else 1
+                _lib.call("S2.6.3", "gf_scene_pcg_sync", ctypes.byref(s), na.value, prec, tol,
+                          pressure_iters, self.pcg_check_every, sign, ctypes.byref(it),
+                          ctypes.byref(res), ctypes.byref(nrep), stream)
+                self.last_pressure_iters = it.value
+                self.last_pressure_residual = res.value
+                self.last_pressure_replacements = nrep.value
+                self._pcg_tol_sq.t.fill_(float(np.float32(tol) ** 2))
+        with prof.section("g2p_advect"):
+            _lib.call("S2.8", "gf_scene_g2p", ctypes.byref(s), _lib.ptr(sc.pc), mode, dt,
+                      self.rho, flip_blend, stream)
+        prof.fused("grad_subtract_bc")
+        prof.fused("color")
+
+    @block("F3.4", "CFL-adaptive substepping")
+    def step_cfl(self, target_dt: float, pressure_iters: int = 80, cfl: float = 0.5,
+                 max_substeps: int = 16, pressure_solver: str = "jacobi") -> int:
+        """Substep count from the device |v|max (solver3d.py:4077-4121)."""
+        n = cfl_substep_count_gpu(self, self.dx, target_dt, cfl=cfl, max_substeps=max_substeps)
+        sub_dt = target_dt / n
+        for _ in range(n):
+            self.step(sub_dt, pressure_iters=pressure_iters, pressure_solver=pressure_solver)
+        return n
+
+    # -------------------------------------------------------------- queries
+    @property
+    def n_active_tiles(self) -> int:
+        return self._sc.n_active_host
+
+    def fluid_cell_count(self) -> int:
+        return int((self._f["marker"] == 1).sum().item())
+
+
+@block("S2.10.GPU", "Device |v|max reduction for the CFL substep count")
+def cfl_substep_count_gpu(solver_or_vel, dx: float, target_dt: float, cfl: float = 0.5,
+                          max_substeps: int = 32, scratch=None) -> int:
+    """ceil(target_dt / (cfl*dx/|v|max)) with |v|max reduced on the device and
+    one float read back (solver3d.py:2255-2280)."""
+    if isinstance(solver_or_vel, FlipSolver3D):
+        vel = solver_or_vel._vel
+    else:
+        vel = solver_or_vel.t if isinstance(solver_or_vel, DeviceArray) else solver_or_vel
+    if vel is None or vel.shape[0] == 0:
+        return 1
+    out = scratch if scratch is not None else torch.zeros(1, dtype=torch.float32, device=vel.device)
+    out.zero_()
+    _lib.call("S2.10.GPU", "gf_k3_vmax_sq_reduce", _lib.ptr(vel.contiguous()), int(vel.shape[0]),
+              _lib.ptr(out), _lib.stream_handle(vel.device))
+    vmax = math.sqrt(float(out.item()))
+    if vmax < 1e-6:
+        return 1
+    n = max(1, int(np.ceil(target_dt / (cfl * dx / vmax))))
+    return min(n, max_substeps)

@@ -1,0 +1,249 @@
+/*
+ * gpufluid_hip.h — C ABI of libgpufluid_hip.so, the MI355X (gfx950) FLIP/PIC/APIC
+ * substep hot path.
+ *
+ * Conventions (every entry point):
+ *   - returns 0 on success, a negative GF_E* code on a bad argument, or a positive
+ *     hipError_t when a launch failed; gf_last_error() then holds a message
+ *     (thread-local, valid until the next failing call on the same thread);
+ *   - pointers are device pointers to C-order contiguous buffers owned by the
+ *     caller, except where a parameter is documented as host memory;
+ *   - `stream` is a hipStream_t passed as void* (NULL = the null stream);
+ *   - no entry point allocates device memory; entry points whose name ends in
+ *     `_sync` synchronise `stream` (they read scalars back to the host), every
+ *     other entry point only enqueues work on `stream`.
+ *
+ * Grid layout follows the reference MAC grid (src/gpufluid/solvers/solver3d.py:1-5):
+ *   u (nx+1,ny,nz), v (nx,ny+1,nz), w (nx,ny,nz+1), cell arrays (nx,ny,nz), k fastest;
+ *   marker int32: 0 air, 1 fluid, 2 solid.  Particle arrays are (n,3) FP32 (vec3)
+ *   and (n,9) FP32 for the APIC affine matrix (row-major mat33).
+ *
+ * Scene entry points (gf_scene_*) run the fused, tile-based substep on a
+ * gf_scene_t descriptor whose buffers the caller allocates with the sizes given
+ * by gf_scene_sizes().  Block entry points (gf_k3_*) are one-to-one replacements
+ * of the reference's dense Warp kernels and keep their argument meaning.
+ */
+#ifndef GPUFLUID_HIP_H
+#define GPUFLUID_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ------------------------------------------------------ */
+#define GF_OK 0
+#define GF_EINVAL (-1)   /* bad argument (null pointer, size, mode) */
+#define GF_ERANGE (-2)   /* a size exceeds a documented limit */
+#define GF_ESTATE (-3)   /* descriptor inconsistent with the call */
+
+/* ---- library ---------------------------------------------------------- */
+
+/* ABI version (major*10000 + minor*100 + patch). No reference counterpart. */
+int gf_version(void);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* gf_last_error(void);
+
+/* Select `device` for the calling thread and report its compute-unit count
+ * (host int* cus, may be NULL). No reference counterpart
+ * (replaces wp.init/device selection, src/gpufluid/primitives/runtime.py:13-24). */
+int gf_init(int device, int* cus);
+
+/* ---- scene descriptor -------------------------------------------------- */
+
+/* Tile edge of the compacted pressure storage and of the P2G/G2P work units. */
+#define GF_TILE 8
+#define GF_TILE_CELLS 512
+
+/* Transfer modes (FlipSolver3D.transfer_mode, solver3d.py:2395). */
+#define GF_TRANSFER_FLIP 0   /* FLIP/PIC blend by flip_blend, solver3d.py:1414-1463 */
+#define GF_TRANSFER_APIC 1   /* APIC, solver3d.py:1475-1732 */
+
+/* Pressure precision of gf_scene_pcg_sync. */
+#define GF_PCG_MIXED 0       /* FP32 vectors, FP64 dots, FP64 residual replacement */
+#define GF_PCG_FP64 1        /* plain FP64 PCG */
+
+typedef struct gf_scene {
+    /* grid */
+    int32_t nx, ny, nz;          /* cells */
+    int32_t tnx, tny, tnz;       /* tiles per axis = ceil(n/8) */
+    float dx;                    /* cell size as FP32 */
+    float dom[3];                /* FP32(n*dx) per axis */
+    /* particles (stored in cell-sorted order; pid maps slot -> particle id) */
+    int32_t n;                   /* particle count */
+    int32_t cap;                 /* allocated particle slots (>= n) */
+    float* pos;  float* vel;  float* affine_c;  int32_t* pid;    /* current */
+    float* pos_b; float* vel_b; float* affine_c_b; int32_t* pid_b; /* sort target */
+    /* sort workspace */
+    int32_t* keys;               /* (cap) flat cell key per particle */
+    int32_t* tmp;                /* (cap) unstable in-cell slots */
+    int32_t* counts;             /* (C) per-cell counts; zero between calls */
+    int32_t* cell_start;         /* (C+1) exclusive prefix of counts */
+    int32_t* scan_part;          /* (gf_scene_sizes.scan_parts) */
+    /* tiles */
+    int32_t* tile_occ;           /* (ntiles) 1 if the tile holds a fluid cell */
+    int32_t* tile_map;           /* (ntiles) compact slot or -1 */
+    int32_t* active;             /* (ntiles) tile id per compact slot */
+    int32_t* n_active;           /* (4) device counter; [0] = active tiles */
+    int32_t* nbr;                /* (ntiles*6) neighbour slots -x,+x,-y,+y,-z,+z or -1 */
+    /* dense MAC grid */
+    int32_t* marker;             /* (nx,ny,nz) */
+    float* u; float* v; float* w;        /* velocities */
+    float* us; float* vs; float* ws;     /* saved pre-pressure velocities */
+    const float* solid_u; const float* solid_v; const float* solid_w; /* NULL = 0 */
+    /* compact pressure storage, (ntiles*512) entries, slot-major */
+    uint8_t* flags;              /* bit3 fluid, bits0-2 non-solid neighbour count */
+    float* b;                    /* divergence (reference scaling) */
+    float* x32; float* r32; float* p32; float* ap32;  /* FP32 PCG vectors */
+    double* x64; double* r64; double* p64; double* ap64; /* FP64 vectors */
+    float* pc;  float* ptmp;     /* pressure result / Jacobi ping-pong */
+    double* partials;            /* (2*ntiles + 64) per-workgroup partial sums */
+    double* scalars;             /* (64) solver scalars, see gf_scene_pcg_scalars */
+} gf_scene_t;
+
+typedef struct gf_scene_size_info {
+    int64_t cells;        /* C = nx*ny*nz */
+    int64_t ntiles;       /* tnx*tny*tnz */
+    int64_t compact;      /* ntiles*512 */
+    int64_t scan_parts;   /* entries of scan_part */
+    int64_t partials;     /* entries of partials */
+    int64_t scalars;      /* entries of scalars */
+} gf_scene_size_info;
+
+/* Buffer sizes for a grid (host out-pointer). */
+int gf_scene_sizes(int nx, int ny, int nz, gf_scene_size_info* out);
+
+/* Bin particles to cells and sort them stably by flat cell key into the *_b
+ * arrays; build per-cell ranges, the marker (walls/obstacles kept), the tile
+ * map, the compact flags and the tile neighbour table. Writes the active tile
+ * count to n_active[0] and also returns it in *n_active_host (sync).
+ * Follows solver3d.py:41-65 (marker clear), 96-101 (cell binning/marking),
+ * sim/reseed.py:72-75,114 (flat key, stable in-cell order), 953-969 (8^3 tiles). */
+int gf_scene_sort_sync(gf_scene_t* s, int with_affine, int* n_active_host, void* stream);
+
+/* Fused P2G + normalize + gravity + solid BC + divergence over 8^3 tiles
+ * (sorted particles, LDS accumulation, no global float atomics).
+ * Replaces k3_clear_grid/k3_p2g|k3_p2g_apic/k3_normalize/k3_add_gravity/
+ * k3_enforce_solid_bc/k3_compute_divergence (solver3d.py:41-156, 758-799,
+ * 846-871, 1475-1588) as launched by _step_impl (solver3d.py:3490-3559).
+ * Writes u,v,w (post BC), us,vs,ws (post normalize), b (compact div). */
+int gf_scene_p2g(const gf_scene_t* s, int transfer_mode, float gravity, float dt,
+                 float rho, void* stream);
+
+/* Raw P2G accumulation into u,v,w (+=) and uw,vw,ww (+=) from the sorted
+ * particles: the reference block k3_p2g / k3_p2g_apic (solver3d.py:72-114,
+ * 1475-1588) for callers that launch P2G alone. */
+int gf_scene_p2g_raw(const gf_scene_t* s, int transfer_mode, float* uw, float* vw,
+                     float* ww, void* stream);
+
+/* Jacobi pressure iterations on the compact active tiles, p = 0 initially;
+ * result in s->pc. Follows k3_jacobi_pressure (solver3d.py:878-938) and its
+ * driver (solver3d.py:3570-3600). */
+int gf_scene_jacobi(const gf_scene_t* s, int n_active, int iters, void* stream);
+
+/* Diagonal-preconditioned CG for A p = rhs_sign * div on the compact tiles
+ * (A = the 7-point operator of k3_apply_A, solver3d.py:1743-1797; Jacobi
+ * preconditioner of k3_compute_diag, 1803-1823; driver 2917-3043).
+ * precision GF_PCG_MIXED or GF_PCG_FP64; stops when ||r||_2 < tol*||b||_2
+ * (mixed: on the FP64 true residual) or after max_iter iterations; the host
+ * reads the scalars every `check_every` iterations. rhs_sign = -1 solves the
+ * Jacobi-consistent pressure, +1 reproduces the reference PCG's sign.
+ * Result in s->pc.  Host outputs (may be NULL): iterations, true relative
+ * residual ||b-Ap||/||b|| (FP64; mixed mode only, else the iterated one),
+ * number of FP64 residual replacements. */
+int gf_scene_pcg_sync(const gf_scene_t* s, int n_active, int precision, double tol,
+                      int max_iter, int check_every, int rhs_sign, int* iters,
+                      double* rel_residual, int* replacements, void* stream);
+
+/* Fused pressure-gradient subtract + solid BC + G2P + advection over 8^3
+ * tiles: k3_subtract_pressure_grad (solver3d.py:1364-1407), k3_enforce_solid_bc
+ * (758-799), k3_g2p_and_advect (1414-1463) or k3_g2p_apic_advect (1591-1732).
+ * `pressure` is a compact pressure vector (s->pc). Updates u,v,w and the
+ * sorted particle arrays in place. */
+int gf_scene_g2p(const gf_scene_t* s, const float* pressure, int transfer_mode,
+                 float dt, float rho, float flip_blend, void* stream);
+
+/* Scatter a compact cell vector (e.g. s->pc, s->b) into a dense (nx,ny,nz)
+ * array; cells of inactive tiles get 0. No reference counterpart. */
+int gf_scene_compact_to_dense(const gf_scene_t* s, const float* compact, float* dense,
+                              void* stream);
+
+/* Reorder a sorted per-slot particle array (n, width floats) back to particle
+ * id order: out[pid[i]] = in[i]. No reference counterpart. */
+int gf_scene_unsort(const gf_scene_t* s, const float* in, int width, float* out,
+                    void* stream);
+
+/* ---- block entry points: reference kernels, dense arrays -------------- */
+/* Each replaces the named Warp kernel with the same argument meaning; the
+ * reference's trailing (off_x,off_y,off_z) sub-dense offsets are not taken
+ * (dense storage only). nx,ny,nz are cell counts. */
+
+/* k3_clear_grid, solver3d.py:41-65 */
+int gf_k3_clear_grid(float* u, float* v, float* w, float* uw, float* vw, float* ww,
+                     int32_t* marker, int nx, int ny, int nz, void* stream);
+/* marker[cell(p)] = 1 unless solid, solver3d.py:96-101 */
+int gf_k3_mark_fluid(const float* pos, int n, int32_t* marker, float dx, int nx, int ny,
+                     int nz, void* stream);
+/* k3_normalize, solver3d.py:121-142 */
+int gf_k3_normalize(float* u, float* v, float* w, const float* uw, const float* vw,
+                    const float* ww, float* us, float* vs, float* ws, int nx, int ny,
+                    int nz, void* stream);
+/* k3_add_gravity, solver3d.py:149-156 (v has (nx,ny+1,nz) entries) */
+int gf_k3_add_gravity(float* v, int nx, int ny, int nz, float g, float dt, void* stream);
+/* k3_enforce_solid_bc, solver3d.py:758-799 (solid_* may be NULL = 0) */
+int gf_k3_enforce_solid_bc(float* u, float* v, float* w, const int32_t* marker,
+                           const float* solid_u, const float* solid_v,
+                           const float* solid_w, int nx, int ny, int nz, void* stream);
+/* k3_compute_divergence, solver3d.py:846-871 */
+int gf_k3_compute_divergence(const float* u, const float* v, const float* w, float* div,
+                             const int32_t* marker, int nx, int ny, int nz, float dx,
+                             float dt, float rho, void* stream);
+/* k3_jacobi_pressure, solver3d.py:878-938 */
+int gf_k3_jacobi_pressure(const float* p_in, float* p_out, const float* div,
+                          const int32_t* marker, int nx, int ny, int nz, void* stream);
+/* k3_apply_A, solver3d.py:1743-1797 (done: device int[1], may be NULL) */
+int gf_k3_apply_A(const float* x, float* y, const int32_t* marker, const int32_t* done,
+                  int nx, int ny, int nz, void* stream);
+/* k3_compute_diag, solver3d.py:1803-1823 */
+int gf_k3_compute_diag(const int32_t* marker, float* diag, int nx, int ny, int nz,
+                       void* stream);
+/* k3_apply_invM, solver3d.py:1826-1840 */
+int gf_k3_apply_invM(const float* r, const float* diag, float* z, const int32_t* marker,
+                     const int32_t* done, int nx, int ny, int nz, void* stream);
+/* k3_dot_fluid, solver3d.py:1859-1871: out[0] += sum over fluid a*b (FP64
+ * workgroup partials, one FP32 atomic per workgroup) */
+int gf_k3_dot_fluid(const float* a, const float* b, float* out, const int32_t* marker,
+                    const int32_t* done, int nx, int ny, int nz, void* stream);
+/* k3_axpy_devscalar, solver3d.py:1988-2009 */
+int gf_k3_axpy_devscalar(const float* a_dev, float sign, const float* x,
+                         const float* y_in, float* y_out, const int32_t* marker,
+                         const int32_t* done, int nx, int ny, int nz, void* stream);
+/* one-thread scalar kernels, solver3d.py:1874-1939:
+ * op 0 k3_zero_scalar(s=a), 1 k3_div_scalar(out=c, numer=a, denom=b),
+ * 2 k3_copy_scalar(src=a, dst=c), 3 k3_set_tol_sq(r0=a, tol, out=c),
+ * 4 k3_check_converged(r=a, tol_sq=b, done), 5 k3_zero_int_scalar(done) */
+int gf_k3_scalar_op(int op, float* a, float* b, float* c, float tol, int32_t* done,
+                    void* stream);
+/* k3_subtract_pressure_grad, solver3d.py:1364-1407 */
+int gf_k3_subtract_pressure_grad(float* u, float* v, float* w, const float* p,
+                                 const int32_t* marker, int nx, int ny, int nz, float dx,
+                                 float dt, float rho, void* stream);
+/* k3_g2p_and_advect, solver3d.py:1414-1463 (dom: host float[3]) */
+int gf_k3_g2p_and_advect(float* pos, float* vel, const float* u, const float* v,
+                         const float* w, const float* us, const float* vs, const float* ws,
+                         int n, float dx, float dt, int nx, int ny, int nz,
+                         float flip_blend, const float* dom, void* stream);
+/* k3_g2p_apic_advect, solver3d.py:1591-1732 (dom: host float[3]) */
+int gf_k3_g2p_apic_advect(float* pos, float* vel, float* affine_c, const float* u,
+                          const float* v, const float* w, int n, float dx, float dt,
+                          int nx, int ny, int nz, const float* dom, void* stream);
+/* k3_vmax_sq_reduce, solver3d.py:2241-2252: out[0] = max(out[0], max |v|^2) */
+int gf_k3_vmax_sq_reduce(const float* vel, int n, float* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GPUFLUID_HIP_H */

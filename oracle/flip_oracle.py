@@ -1,0 +1,604 @@
+"""CPU restatement of the reference FLIP/PIC/APIC substep — TEST INFRASTRUCTURE.
+
+This module is the checker, never the product: only ``tests/``,
+``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg may import
+it.  It restates, in numpy (FP64 by default, or FP32-faithful), the arithmetic
+of the reference's Warp kernels in ``src/gpufluid/solvers/solver3d.py`` and
+``src/gpufluid/primitives/gridmath.py`` (ArkBee/GPUFLUID, gpufluid 0.0.2);
+each function cites the lines it follows.
+
+How each part is pinned (tests/test_oracle_*.py):
+  * cell binning / flat key / stable order: closed form clamp(trunc(fp32(p)/
+    fp32(dx)),0,n-1) on boundary vectors, brute-force per-particle loop, and
+    ``np.argsort(kind="stable")`` (reference sim/reseed.py:75,114);
+  * marker: set algebra walls U solid U {cells with particles};
+  * P2G + normalize: pure-Python per-particle loop on tiny grids, partition of
+    unity of the trilinear weights, uniform-velocity invariance;
+  * gravity / BC / divergence / gradient: closed forms on hand-built fields;
+  * Jacobi: fixed point equals the scipy.sparse solve of the assembled
+    7-point system; 1-cell and 1-D tridiagonal textbook cases;
+  * PCG (reference restatement): converges to the spsolve solution with the
+    reference's sign (x = -p); iteration counts on tiny cases;
+  * sample3 / G2P: trilinear interpolation reproduces affine fields exactly;
+    PIC == FLIP with flip_blend 0; clamp box;
+  * APIC: affine velocity field is transferred exactly (P2G->G2P round trip).
+Functions without such a check say "parity unpinned" in their docstring.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+try:  # scipy is only needed for the assembled-matrix checks
+    import scipy.sparse as _sp
+    import scipy.sparse.linalg as _spla
+except Exception:  # pragma: no cover
+    _sp = None
+    _spla = None
+
+AIR, FLUID, SOLID = 0, 1, 2
+
+
+# ----------------------------------------------------------------- grid set-up
+def wall_marker(nx: int, ny: int, nz: int) -> np.ndarray:
+    """One-cell solid shell (solver3d.py:2475-2482)."""
+    m = np.zeros((nx, ny, nz), dtype=np.int32)
+    m[0, :, :] = SOLID
+    m[-1, :, :] = SOLID
+    m[:, 0, :] = SOLID
+    m[:, -1, :] = SOLID
+    m[:, :, 0] = SOLID
+    m[:, :, -1] = SOLID
+    return m
+
+
+def domain(n: int, dx: float) -> np.float32:
+    """Domain extent as the kernels see it: FP32 of the FP64 product n*dx
+    (solver3d.py:2382 builds wp.vec3(n*dx, ...))."""
+    return np.float32(n * dx)
+
+
+# --------------------------------------------------------------- cell binning
+def cell_index(pos: np.ndarray, dx: float, shape) -> tuple:
+    """clamp(int(p/dx), 0, n-1) per axis with an FP32 divide and truncation
+    toward zero (solver3d.py:96-99, 1502-1507).  NaN maps to cell 0."""
+    p = np.asarray(pos, dtype=np.float32)
+    q = p / np.float32(dx)
+    out = []
+    for ax in range(3):
+        n = shape[ax]
+        qa = np.nan_to_num(q[:, ax], nan=0.0, posinf=float(n), neginf=-1.0)
+        qa = np.clip(qa, -1.0, float(n))
+        c = np.trunc(qa).astype(np.int64)
+        out.append(np.clip(c, 0, n - 1))
+    return tuple(out)
+
+
+def flat_key(ci, cj, ck, shape) -> np.ndarray:
+    """Flat C-order cell key i*ny*nz + j*nz + k (sim/reseed.py:72-75)."""
+    _, ny, nz = shape
+    return (ci * ny + cj) * nz + ck
+
+
+def stable_order(key: np.ndarray) -> np.ndarray:
+    """Stable counting-sort permutation by key (sim/reseed.py:114)."""
+    return np.argsort(key, kind="stable")
+
+
+def update_marker(marker: np.ndarray, pos: np.ndarray, dx: float) -> np.ndarray:
+    """Clear non-solid cells, then mark cells holding a particle fluid unless
+    solid (solver3d.py:62-64 then 100-101)."""
+    m = marker.copy()
+    m[m != SOLID] = AIR
+    if len(pos):
+        ci, cj, ck = cell_index(pos, dx, m.shape)
+        hit = np.zeros(m.shape, dtype=bool)
+        hit[ci, cj, ck] = True
+        m[hit & (m != SOLID)] = FLUID
+    return m
+
+
+# ------------------------------------------------------------------------ P2G
+def _face_shapes(shape):
+    nx, ny, nz = shape
+    return (nx + 1, ny, nz), (nx, ny + 1, nz), (nx, ny, nz + 1)
+
+
+def _corners(c):
+    """floor and fractional parts of face-space coordinates (gridmath.py:79-80)."""
+    i0 = np.floor(c).astype(np.int64)
+    return i0, c - i0
+
+
+def p2g(pos, vel, marker, dx, dtype=np.float64, affine=None):
+    """Raw P2G scatter: returns (u, v, w, uw, vw, ww).  FLIP/PIC follows
+    k3_p2g (solver3d.py:72-114) with scatter_face (gridmath.py:71-95):
+    corners outside the face array are dropped.  With ``affine`` (n,3,3) it
+    follows k3_p2g_apic (solver3d.py:1475-1588): each corner receives
+    v + C[row]·(x_face - x_p) with world-space face positions."""
+    shapes = _face_shapes(marker.shape)
+    fields = [np.zeros(s, dtype=dtype) for s in shapes]
+    weights = [np.zeros(s, dtype=dtype) for s in shapes]
+    if len(pos) == 0:
+        return (*fields, *weights)
+    if dtype == np.float32:
+        p = np.asarray(pos, dtype=np.float32)
+        q = p / np.float32(dx)
+        half = np.float32(0.5)
+        dxc = np.float32(dx)
+    else:
+        p = np.asarray(pos, dtype=np.float32).astype(np.float64)
+        dxc = float(np.float32(dx))
+        q = p / dxc
+        half = 0.5
+    vv = np.asarray(vel).astype(dtype)
+    C = None if affine is None else np.asarray(affine).reshape(-1, 3, 3).astype(dtype)
+    for comp in range(3):
+        c = q.copy()
+        for ax in range(3):
+            if ax != comp:
+                c[:, ax] = q[:, ax] - half
+        i0, frac = _corners(c)
+        shp = shapes[comp]
+        for di in (0, 1):
+            for dj in (0, 1):
+                for dk in (0, 1):
+                    ii, jj, kk = i0[:, 0] + di, i0[:, 1] + dj, i0[:, 2] + dk
+                    ok = ((ii >= 0) & (ii < shp[0]) & (jj >= 0) & (jj < shp[1])
+                          & (kk >= 0) & (kk < shp[2]))
+                    wx = frac[:, 0] if di else 1 - frac[:, 0]
+                    wy = frac[:, 1] if dj else 1 - frac[:, 1]
+                    wz = frac[:, 2] if dk else 1 - frac[:, 2]
+                    wt = (wx * wy * wz).astype(dtype)
+                    val = vv[:, comp]
+                    if C is not None:
+                        fpos = []
+                        for ax, idx in enumerate((ii, jj, kk)):
+                            off = 0.0 if ax == comp else 0.5
+                            fpos.append((idx.astype(dtype) + dtype(off)) * dxc - p[:, ax])
+                        val = (val + C[:, comp, 0] * fpos[0] + C[:, comp, 1] * fpos[1]
+                               + C[:, comp, 2] * fpos[2])
+                    np.add.at(fields[comp], (ii[ok], jj[ok], kk[ok]), (val * wt)[ok])
+                    np.add.at(weights[comp], (ii[ok], jj[ok], kk[ok]), wt[ok])
+    return (*fields, *weights)
+
+
+def p2g_loop(pos, vel, marker, dx):
+    """Pure-Python per-particle restatement of scatter_face
+    (gridmath.py:71-95) in FP64, for tiny cases; used to pin ``p2g``."""
+    shapes = _face_shapes(marker.shape)
+    fields = [np.zeros(s) for s in shapes]
+    weights = [np.zeros(s) for s in shapes]
+    dxc = float(np.float32(dx))
+    for n in range(len(pos)):
+        q = [float(np.float32(pos[n][a])) / dxc for a in range(3)]
+        for comp in range(3):
+            c = [q[a] - (0.0 if a == comp else 0.5) for a in range(3)]
+            i0 = [int(np.floor(x)) for x in c]
+            s = [c[a] - i0[a] for a in range(3)]
+            for di in (0, 1):
+                for dj in (0, 1):
+                    for dk in (0, 1):
+                        ii, jj, kk = i0[0] + di, i0[1] + dj, i0[2] + dk
+                        shp = shapes[comp]
+                        if 0 <= ii < shp[0] and 0 <= jj < shp[1] and 0 <= kk < shp[2]:
+                            w = ((s[0] if di else 1 - s[0]) * (s[1] if dj else 1 - s[1])
+                                 * (s[2] if dk else 1 - s[2]))
+                            fields[comp][ii, jj, kk] += float(vel[n][comp]) * w
+                            weights[comp][ii, jj, kk] += w
+    return (*fields, *weights)
+
+
+def normalize(f, wgt):
+    """f/w where w > 1e-8 else 0 (solver3d.py:121-142); returns (value, saved copy)."""
+    out = np.where(wgt > 1e-8, f / np.where(wgt > 1e-8, wgt, 1), 0).astype(f.dtype)
+    return out, out.copy()
+
+
+def add_gravity(v, g, dt):
+    """v += g*dt on every v face (solver3d.py:149-156); g*dt in the field's precision."""
+    return (v + v.dtype.type(g) * v.dtype.type(dt)).astype(v.dtype)
+
+
+def enforce_solid_bc(u, v, w, marker, solid_u=None, solid_v=None, solid_w=None):
+    """Faces on the domain edge or next to a solid cell take the obstacle
+    velocity (zero when none is given) (solver3d.py:758-799)."""
+    u, v, w = u.copy(), v.copy(), w.copy()
+    solid = marker == SOLID
+    for comp, f, sf in ((0, u, solid_u), (1, v, solid_v), (2, w, solid_w)):
+        n = marker.shape[comp]
+        mask = np.zeros(f.shape, dtype=bool)
+        lo = [slice(None)] * 3
+        hi = [slice(None)] * 3
+        lo[comp] = 0
+        hi[comp] = n
+        mask[tuple(lo)] = True
+        mask[tuple(hi)] = True
+        left = [slice(None)] * 3
+        left[comp] = slice(1, n + 1)
+        mask[tuple(left)] |= solid               # cell below the face is solid
+        right = [slice(None)] * 3
+        right[comp] = slice(0, n)
+        mask[tuple(right)] |= solid              # cell above the face is solid
+        f[mask] = 0 if sf is None else sf[mask]
+    return u, v, w
+
+
+def divergence(u, v, w, marker, dx, dt, rho):
+    """(Δu+Δv+Δw)·ρ·dx/dt on fluid cells, else 0 (solver3d.py:846-871);
+    summed in the reference's left-to-right order."""
+    t = u.dtype.type
+    d = (u[1:] - u[:-1]) + v[:, 1:] - v[:, :-1] + w[:, :, 1:] - w[:, :, :-1]
+    scale = t(rho) * t(dx) / t(dt)
+    return np.where(marker == FLUID, d * scale, 0).astype(u.dtype)
+
+
+# ------------------------------------------------------------------ pressure
+_DIRS = ((0, -1), (0, 1), (1, -1), (1, 1), (2, -1), (2, 1))
+
+
+def _shift(a, ax, d, fill):
+    """b[c] = a[c + d along ax]; out-of-grid entries = fill."""
+    out = np.full_like(a, fill)
+    src = [slice(None)] * 3
+    dst = [slice(None)] * 3
+    if d < 0:
+        src[ax] = slice(0, -1)
+        dst[ax] = slice(1, None)
+    else:
+        src[ax] = slice(1, None)
+        dst[ax] = slice(0, -1)
+    out[tuple(dst)] = a[tuple(src)]
+    return out
+
+
+def neighbour_count(marker):
+    """Number of in-grid non-solid neighbours (the diagonal of A,
+    solver3d.py:1767-1796, 1815-1821)."""
+    d = np.zeros(marker.shape, dtype=np.int32)
+    for ax, s in _DIRS:
+        nb = _shift(marker, ax, s, SOLID)
+        d += (nb != SOLID)
+    return d
+
+
+def _sum_fluid_nb(x, marker):
+    s = np.zeros_like(x)
+    for ax, d in _DIRS:                       # -x,+x,-y,+y,-z,+z as in the reference
+        nbm = _shift(marker, ax, d, SOLID)
+        nbx = _shift(x, ax, d, 0)
+        s = s + np.where(nbm == FLUID, nbx, 0).astype(x.dtype)
+    return s
+
+
+def jacobi_sweep(p_in, div, marker):
+    """One Jacobi sweep p_out = (Σ_fluid-nb p_in − div)/diag on fluid cells,
+    0 elsewhere or when diag < 0.5 (solver3d.py:878-938)."""
+    diag = neighbour_count(marker).astype(p_in.dtype)
+    s = _sum_fluid_nb(p_in, marker)
+    ok = (marker == FLUID) & (diag >= 0.5)
+    return np.where(ok, (s - div) / np.where(ok, diag, 1), 0).astype(p_in.dtype)
+
+
+def jacobi(div, marker, iters, dtype=np.float32):
+    """Driver of solver3d.py:3570-3600: p = 0, then `iters` sweeps."""
+    p = np.zeros(marker.shape, dtype=dtype)
+    d = div.astype(dtype)
+    for _ in range(iters):
+        p = jacobi_sweep(p, d, marker)
+    return p
+
+
+def apply_A(x, marker):
+    """A x = diag·x − Σ_fluid-nb x on fluid cells, 0 elsewhere (solver3d.py:1743-1797)."""
+    diag = neighbour_count(marker).astype(x.dtype)
+    return np.where(marker == FLUID, diag * x - _sum_fluid_nb(x, marker), 0).astype(x.dtype)
+
+
+def precond_diag(marker):
+    """max(diag, 1) on fluid cells, 1 elsewhere (solver3d.py:1803-1823)."""
+    d = neighbour_count(marker).astype(np.float64)
+    return np.where(marker == FLUID, np.maximum(d, 1.0), 1.0)
+
+
+def poisson_matrix(marker):
+    """Assembled sparse 7-point matrix over the fluid cells (row order = C
+    order of the fluid cells) and the index map; built from index arithmetic,
+    independently of the stencil code above."""
+    if _sp is None:
+        raise RuntimeError("scipy is required for poisson_matrix")
+    fluid = np.argwhere(marker == FLUID)
+    index = -np.ones(marker.shape, dtype=np.int64)
+    index[tuple(fluid.T)] = np.arange(len(fluid))
+    rows, cols, vals = [], [], []
+    shape = np.array(marker.shape)
+    diag = np.zeros(len(fluid))
+    for off in ((-1, 0, 0), (1, 0, 0), (0, -1, 0), (0, 1, 0), (0, 0, -1), (0, 0, 1)):
+        nb = fluid + np.array(off)
+        inside = np.all((nb >= 0) & (nb < shape), axis=1)
+        nbc = np.where(inside[:, None], nb, 0)
+        m = marker[tuple(nbc.T)]
+        nonsolid = inside & (m != SOLID)
+        diag += nonsolid
+        isf = inside & (m == FLUID)
+        rows.append(np.nonzero(isf)[0])
+        cols.append(index[tuple(nbc[isf].T)])
+        vals.append(-np.ones(isf.sum()))
+    rows.append(np.arange(len(fluid)))
+    cols.append(np.arange(len(fluid)))
+    vals.append(diag)
+    A = _sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                       shape=(len(fluid), len(fluid)))
+    return A, fluid, index
+
+
+def solve_exact(div, marker):
+    """FP64 direct solve of A p = −div over the fluid cells (the Jacobi fixed
+    point, physically signed); p = 0 off fluid."""
+    A, fluid, _ = poisson_matrix(marker)
+    p = np.zeros(marker.shape)
+    if len(fluid):
+        rhs = -div[tuple(fluid.T)].astype(np.float64)
+        p[tuple(fluid.T)] = _spla.spsolve(A.tocsc(), rhs)
+    return p
+
+
+def pcg_reference(div, marker, max_iter, tol=1e-4, check_every=10, dtype=np.float32):
+    """Literal restatement of FlipSolver3D._pressure_pcg (solver3d.py:2917-3043):
+    x0 = 0, r = b = +div (so x = −p, the reference's sign), Jacobi
+    preconditioner, α = 0 when |pAp| < 1e-30, β = 0 when |rz| < 1e-30,
+    stop when r·r < tol²·r0·r0 evaluated every `check_every` iterations and at
+    the last one.  Returns (x, iterations)."""
+    t = np.dtype(dtype).type
+    fl = marker == FLUID
+    diag = precond_diag(marker).astype(dtype)
+    x = np.zeros(marker.shape, dtype=dtype)
+    r = div.astype(dtype).copy()
+    z = np.where(fl, r / diag, 0).astype(dtype)
+    p = z.copy()
+
+    def dot(a, b):
+        return t(np.sum((a * b)[fl], dtype=np.float64))
+
+    rz_old = dot(r, z)
+    r0 = dot(r, r)
+    if r0 < 1e-30:
+        return x, 0
+    tol_sq = t(tol) ** 2 * r0
+    for it in range(max_iter):
+        Ap = apply_A(p, marker)
+        pAp = dot(p, Ap)
+        alpha = t(0) if abs(pAp) < 1e-30 else t(rz_old / pAp)
+        x = np.where(fl, x + alpha * p, 0).astype(dtype)
+        r = np.where(fl, r + (-alpha) * Ap, 0).astype(dtype)
+        if (it + 1) % check_every == 0 or it == max_iter - 1:
+            if dot(r, r) < tol_sq:
+                return x, it + 1
+        z = np.where(fl, r / diag, 0).astype(dtype)
+        rz_new = dot(r, z)
+        beta = t(0) if abs(rz_old) < 1e-30 else t(rz_new / rz_old)
+        p = np.where(fl, z + beta * p, 0).astype(dtype)
+        rz_old = rz_new
+    return x, max_iter
+
+
+def residual_norm(p, div, marker):
+    """‖−div − A p‖₂ / ‖div‖₂ over fluid cells in FP64 (relative residual of
+    the physically signed system)."""
+    fl = marker == FLUID
+    b = -div.astype(np.float64)
+    r = (b - apply_A(p.astype(np.float64), marker))[fl]
+    nb = np.linalg.norm(b[fl])
+    return float(np.linalg.norm(r) / nb) if nb > 0 else 0.0
+
+
+def subtract_pressure_grad(u, v, w, p, marker, dx, dt, rho):
+    """u −= dt/(ρdx)·(p_r − p_l) on interior faces with ≥1 fluid and no solid
+    neighbour; a non-fluid side reads p = 0 (solver3d.py:1364-1407)."""
+    t = u.dtype.type
+    scale = t(dt) / (t(rho) * t(dx))
+    out = []
+    for comp, f in ((0, u, ), (1, v), (2, w)):
+        f = f.copy()
+        n = marker.shape[comp]
+        lo = [slice(None)] * 3
+        hi = [slice(None)] * 3
+        lo[comp] = slice(0, n - 1)
+        hi[comp] = slice(1, n)
+        ml, mr = marker[tuple(lo)], marker[tuple(hi)]
+        pl = np.where(ml == FLUID, p[tuple(lo)], 0).astype(f.dtype)
+        pr = np.where(mr == FLUID, p[tuple(hi)], 0).astype(f.dtype)
+        act = ((ml == FLUID) | (mr == FLUID)) & (ml != SOLID) & (mr != SOLID)
+        face = [slice(None)] * 3
+        face[comp] = slice(1, n)
+        sub = f[tuple(face)]
+        f[tuple(face)] = np.where(act, sub - scale * (pr - pl), sub).astype(f.dtype)
+        out.append(f)
+    return tuple(out)
+
+
+# ------------------------------------------------------------------------ G2P
+def sample3(f, fx, fy, fz):
+    """Trilinear sample with coordinate and index clamping
+    (gridmath.py:30-65); vectorised over points."""
+    nmx, nmy, nmz = f.shape
+    t = f.dtype.type
+    fx = np.clip(fx, 0, nmx).astype(f.dtype)
+    fy = np.clip(fy, 0, nmy).astype(f.dtype)
+    fz = np.clip(fz, 0, nmz).astype(f.dtype)
+    i0 = np.clip(np.floor(fx).astype(np.int64), 0, nmx - 1)
+    j0 = np.clip(np.floor(fy).astype(np.int64), 0, nmy - 1)
+    k0 = np.clip(np.floor(fz).astype(np.int64), 0, nmz - 1)
+    i1 = np.minimum(i0 + 1, nmx - 1)
+    j1 = np.minimum(j0 + 1, nmy - 1)
+    k1 = np.minimum(k0 + 1, nmz - 1)
+    sx = np.clip(fx - i0, 0, 1).astype(f.dtype)
+    sy = np.clip(fy - j0, 0, 1).astype(f.dtype)
+    sz = np.clip(fz - k0, 0, 1).astype(f.dtype)
+    one = t(1)
+    c00 = f[i0, j0, k0] * (one - sx) + f[i1, j0, k0] * sx
+    c10 = f[i0, j1, k0] * (one - sx) + f[i1, j1, k0] * sx
+    c01 = f[i0, j0, k1] * (one - sx) + f[i1, j0, k1] * sx
+    c11 = f[i0, j1, k1] * (one - sx) + f[i1, j1, k1] * sx
+    c0 = c00 * (one - sy) + c10 * sy
+    c1 = c01 * (one - sy) + c11 * sy
+    return c0 * (one - sz) + c1 * sz
+
+
+def _lcoords(pos, dx, dtype):
+    if dtype == np.float32:
+        q = np.asarray(pos, dtype=np.float32) / np.float32(dx)
+        return q, np.float32(0.5)
+    return np.asarray(pos, dtype=np.float32).astype(np.float64) / float(np.float32(dx)), 0.5
+
+
+def g2p_advect(pos, vel, u, v, w, us, vs, ws, dx, dt, flip_blend, dom):
+    """FLIP/PIC gather + advection + clamp to [1.001dx, dom−1.001dx]
+    (solver3d.py:1414-1463).  Returns (pos, vel) in the fields' precision."""
+    dtype = u.dtype.type
+    q, h = _lcoords(pos, dx, dtype)
+    lx, ly, lz = q[:, 0], q[:, 1], q[:, 2]
+    pic = np.stack([sample3(u, lx, ly - h, lz - h), sample3(v, lx - h, ly, lz - h),
+                    sample3(w, lx - h, ly - h, lz)], axis=1)
+    old = np.stack([sample3(us, lx, ly - h, lz - h), sample3(vs, lx - h, ly, lz - h),
+                    sample3(ws, lx - h, ly - h, lz)], axis=1)
+    fb = dtype(flip_blend)
+    flip = np.asarray(vel).astype(dtype) + (pic - old)
+    nvel = (fb * flip + (dtype(1) - fb) * pic).astype(dtype)
+    p = np.asarray(pos, dtype=np.float32).astype(dtype)
+    npos = p + nvel * dtype(dt)
+    eps = dtype(np.float32(dx) * np.float32(1.001))
+    for ax in range(3):
+        npos[:, ax] = np.clip(npos[:, ax], eps, dtype(dom[ax]) - eps)
+    return npos, nvel
+
+
+def g2p_apic_advect(pos, u, v, w, dx, dt, dom):
+    """APIC gather with clamped corner indices, affine matrix rows
+    Σ w·u·(x_face − x_p)·3/dx², then advection + clamp (solver3d.py:1591-1732).
+    Returns (pos, vel, C (n,3,3))."""
+    dtype = u.dtype.type
+    q, h = _lcoords(pos, dx, dtype)
+    p = np.asarray(pos, dtype=np.float32).astype(dtype)
+    dxc = dtype(np.float32(dx))
+    inv = dtype(3.0) / (dxc * dxc)
+    n = len(p)
+    vel = np.zeros((n, 3), dtype=dtype)
+    C = np.zeros((n, 3, 3), dtype=dtype)
+    for comp, f in enumerate((u, v, w)):
+        c = q.copy()
+        for ax in range(3):
+            if ax != comp:
+                c[:, ax] = q[:, ax] - h
+        i0, frac = _corners(c)
+        hi = [f.shape[0] - 1, f.shape[1] - 1, f.shape[2] - 1]
+        for di in (0, 1):
+            for dj in (0, 1):
+                for dk in (0, 1):
+                    idx = [np.clip(i0[:, 0] + di, 0, hi[0]), np.clip(i0[:, 1] + dj, 0, hi[1]),
+                           np.clip(i0[:, 2] + dk, 0, hi[2])]
+                    wx = frac[:, 0] if di else 1 - frac[:, 0]
+                    wy = frac[:, 1] if dj else 1 - frac[:, 1]
+                    wz = frac[:, 2] if dk else 1 - frac[:, 2]
+                    wt = (wx * wy * wz).astype(dtype)
+                    val = f[idx[0], idx[1], idx[2]]
+                    vel[:, comp] += wt * val
+                    fu = wt * val * inv
+                    for ax in range(3):
+                        off = 0.0 if ax == comp else 0.5
+                        o = (idx[ax].astype(dtype) + dtype(off)) * dxc - p[:, ax]
+                        C[:, comp, ax] += fu * o
+    npos = p + vel * dtype(dt)
+    eps = dtype(np.float32(dx) * np.float32(1.001))
+    for ax in range(3):
+        npos[:, ax] = np.clip(npos[:, ax], eps, dtype(dom[ax]) - eps)
+    return npos, vel, C
+
+
+# -------------------------------------------------------------------- substep
+def substep(state: dict, dt: float, pressure="pcg", iters=80, tol=1e-4, dtype=np.float64,
+            gravity=-9.81, flip_blend=0.95, rho=1.0, transfer="flip", pcg_sign="physical"):
+    """One substep in the stage order of _step_impl (solver3d.py:3475-3667):
+    clear → P2G → normalize → gravity → BC → divergence → pressure → grad → BC
+    → G2P + advect.  ``state`` holds pos, vel, marker, dx and (APIC) affine.
+    pressure: "jacobi" (fixed iterations), "pcg" (reference restatement;
+    pcg_sign "physical" negates its result, "reference" keeps it) or
+    "exact" (spsolve).  Returns a new state plus the grid fields."""
+    pos, vel, marker, dx = state["pos"], state["vel"], state["marker"], state["dx"]
+    nx, ny, nz = marker.shape
+    dom = [domain(n, dx) for n in (nx, ny, nz)]
+    m = update_marker(marker, pos, dx)
+    aff = state.get("affine") if transfer == "apic" else None
+    if transfer == "apic" and aff is None:
+        aff = np.zeros((len(pos), 3, 3), dtype=np.float32)
+    u, v, w, uw, vw, ww = p2g(pos, vel, m, dx, dtype=dtype, affine=aff)
+    u, us = normalize(u, uw)
+    v, vs = normalize(v, vw)
+    w, ws = normalize(w, ww)
+    v = add_gravity(v, gravity, dt)
+    u, v, w = enforce_solid_bc(u, v, w, m)
+    div = divergence(u, v, w, m, dx, dt, rho)
+    info = {}
+    if pressure == "jacobi":
+        p = jacobi(div, m, iters, dtype=dtype)
+    elif pressure == "pcg":
+        x, it = pcg_reference(div, m, iters, tol=tol, dtype=dtype)
+        p = -x if pcg_sign == "physical" else x
+        info["iters"] = it
+    elif pressure == "exact":
+        p = solve_exact(div, m).astype(dtype)
+    else:
+        raise ValueError(f"unknown pressure_solver: {pressure!r}")
+    u2, v2, w2 = subtract_pressure_grad(u, v, w, p, m, dx, dt, rho)
+    u2, v2, w2 = enforce_solid_bc(u2, v2, w2, m)
+    out = dict(state)
+    out["marker"] = m
+    if transfer == "apic":
+        npos, nvel, C = g2p_apic_advect(pos, u2, v2, w2, dx, dt, dom)
+        out["affine"] = C
+    else:
+        npos, nvel = g2p_advect(pos, vel, u2, v2, w2, us, vs, ws, dx, dt, flip_blend, dom)
+    out["pos"], out["vel"] = npos, nvel
+    grid = dict(u=u2, v=v2, w=w2, us=us, vs=vs, ws=ws, div=div, p=p, marker=m, info=info)
+    return out, grid
+
+
+# -------------------------------------------------------------------- seeding
+def seed_box_reference(lo, hi, ppc, dx):
+    """Literal FlipSolver3D.seed_box positions (solver3d.py:2826-2910): cells
+    [int(lo/dx), int(hi/dx)) per axis, per_axis = round(ppc^(1/3)), jitter
+    0.3·(U−0.5)/per_axis from default_rng(0) drawn x,y,z per particle."""
+    i0, i1 = int(lo[0] / dx), int(hi[0] / dx)
+    j0, j1 = int(lo[1] / dx), int(hi[1] / dx)
+    k0, k1 = int(lo[2] / dx), int(hi[2] / dx)
+    per = int(round(ppc ** (1.0 / 3.0)))
+    rng = np.random.default_rng(0)
+    out = []
+    for i in range(i0, i1):
+        for j in range(j0, j1):
+            for k in range(k0, k1):
+                for sx in range(per):
+                    for sy in range(per):
+                        for sz in range(per):
+                            px = (i + (sx + 0.5 + 0.3 * (rng.random() - 0.5)) / per) * dx
+                            py = (j + (sy + 0.5 + 0.3 * (rng.random() - 0.5)) / per) * dx
+                            pz = (k + (sz + 0.5 + 0.3 * (rng.random() - 0.5)) / per) * dx
+                            out.append([px, py, pz])
+    return np.array(out, dtype=np.float32).reshape(-1, 3)
+
+
+def seed_cells_loop(cells, dx, seed, per_axis=2, jitter=0.25):
+    """Pure-Python form of the configuration seeding recipe (used to pin the
+    vectorised seeder in gpufluid_amd.seeding): per cell in C order, per
+    sub-position (sx,sy,sz) in C order, one U(−jitter, jitter)·dx draw per axis
+    in x,y,z order from default_rng(seed)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for (i, j, k) in cells:
+        for sx in range(per_axis):
+            for sy in range(per_axis):
+                for sz in range(per_axis):
+                    jx, jy, jz = rng.random(3)
+                    out.append([(i + (sx + 0.5) / per_axis + (jx - 0.5) * 2 * jitter) * dx,
+                                (j + (sy + 0.5) / per_axis + (jy - 0.5) * 2 * jitter) * dx,
+                                (k + (sz + 0.5) / per_axis + (jz - 0.5) * 2 * jitter) * dx])
+    return np.array(out, dtype=np.float32).reshape(-1, 3)
