@@ -1,0 +1,233 @@
+"""Substep throughput on one MI355X: 256^3 pool + sphere (P2G + PCG + G2P).
+
+python bench.py [--steps K] [--warmup W] [--config c3|c2|c1] [--no-cpu-baseline]
+
+One step = one full substep on ~40.2 M particles: stable counting sort by cell,
+fused P2G (+normalize, gravity, BC, divergence), mixed-precision PCG to a 1e-8
+relative residual (cap 500 iterations), fused gradient subtract + G2P + advection.
+Particles start resident in HBM.  Prints one JSON line (rank 0, N = 1).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import subprocess
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+if str(ROOT) not in sys.path:
+    sys.path.insert(0, str(ROOT))
+
+METRIC = "Mparticle-substeps/s at 256³ (P2G+PCG+G2P), % HBM peak; adjoint/fwd time"
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic HBM bytes per unit (fluid cells N_f, particles N_p, active-tile
+# cells N_a) of each kernel's compulsory inputs and outputs.
+BYTES = {
+    "sort": lambda np_, nf, na: 68 * np_ + 12 * na,
+    "p2g": lambda np_, nf, na: 24 * np_ + 29 * nf,
+    "g2p": lambda np_, nf, na: 48 * np_ + 41 * nf,
+    "pcg_spmv": lambda np_, nf, na: 9 * nf,
+    "pcg_update": lambda np_, nf, na: 24 * nf,
+    "pcg_dir": lambda np_, nf, na: 13 * nf,
+    "pcg_replace": lambda np_, nf, na: 17 * nf,
+}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def build_scene(name):
+    from gpufluid_amd import seeding
+    return {"c1": seeding.config_c1, "c2": seeding.config_c2, "c3": seeding.config_c3}[name]()
+
+
+def cpu_baseline(scene, max_seconds=30.0):
+    """Time the C restatement of the reference algorithm (oracle/) on the
+    host cores: one substep of the same scene (or fewer particles when the
+    build is missing) and report Mparticle-substeps/s."""
+    exe = ROOT / "oracle" / "_build" / "flip_cpu"
+    if not exe.exists():
+        return None
+    import numpy as np
+    import tempfile
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
+    with tempfile.TemporaryDirectory() as td:
+        pfile = Path(td) / "particles.f32"
+        np.concatenate([scene.pos, scene.vel], axis=1).astype(np.float32).tofile(pfile)
+        n = scene.shape[0]
+        p = scene.params
+        cmd = [str(exe), str(n), str(scene.dx), str(p.get("dt", 1 / 240)), str(p.get("tol", 1e-8)),
+               str(p.get("max_iter", 500)), "1", str(pfile), str(len(scene.pos))]
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+        r = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=600)
+    if r.returncode != 0:
+        log(f"cpu baseline failed: {r.stderr[-400:]}")
+        return None
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    out["cores"] = threads
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="c3")
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--check-every", type=int, default=8)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64-compare", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: every box here has one MI355X and the "
+              "substep path is single-device (see DESIGN.md for how it would shard).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from gpufluid_amd import FlipSolver3D, _lib
+
+    t0 = time.time()
+    scene = build_scene(args.config)
+    p = scene.params
+    dt = p.get("dt", 1.0 / 240)
+    tol = p.get("tol", 1e-8)
+    max_iter = p.get("max_iter", 500)
+    log(f"scene {scene.name}: {scene.shape} grid, {scene.n_particles} particles "
+        f"(seeded in {time.time() - t0:.1f}s)")
+    s = FlipSolver3D(*scene.shape, dx=scene.dx, flip_blend=p.get("flip_blend", 0.95),
+                     transfer_mode=p.get("transfer", "flip"), enable_timing=True,
+                     pcg_precision=args.precision, pcg_check_every=args.check_every,
+                     device="cuda:0")
+    s.set_particles(scene.pos, scene.vel)
+    s.pcg_kernel_timing = True
+
+    def one_step():
+        s.step(dt, pressure_iters=max_iter, pressure_solver="pcg", pressure_tol=tol)
+
+    for i in range(args.warmup):
+        one_step()
+        log(f"warmup {i}: PCG {s.last_pressure_iters} it, residual {s.last_pressure_residual:.2e}")
+    s._prof.reset()
+    stats = []
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record()
+    w0 = time.perf_counter()
+    for i in range(args.steps):
+        one_step()
+        stats.append(dict(iters=s.last_pressure_iters, res=s.last_pressure_residual,
+                          nrep=s.last_pressure_replacements, kms=s.last_pcg_kernel_ms,
+                          na=s.n_active_tiles, nf=s._sc_fluid_cells))
+    end.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    total_ms = start.elapsed_time(end)
+    ms_step = total_ms / args.steps
+    npart = s.n_particles
+    value = npart * args.steps / (total_ms / 1e3) / 1e6
+    log(f"timed {args.steps} steps: {ms_step:.3f} ms/step (wall {wall * 1e3 / args.steps:.3f}), "
+        f"{value:.1f} Mparticle-substeps/s")
+
+    sec = s._prof.summarize()
+    nf = float(np.mean([st["nf"] for st in stats]))
+    na = float(np.mean([st["na"] for st in stats])) * 512
+    iters = [st["iters"] for st in stats]
+    kms = np.sum([st["kms"] for st in stats], axis=0)  # spmv, update, dir, replace (ms)
+    n_it = sum(iters)
+    n_rep = sum(st["nrep"] for st in stats)
+    kern = {
+        "pcg_spmv": (kms[0], n_it), "pcg_update": (kms[1], n_it), "pcg_dir": (kms[2], n_it),
+        "pcg_replace": (kms[3], n_rep),
+        "sort": (sec.get("clear", 0.0), args.steps), "p2g": (sec.get("p2g", 0.0), args.steps),
+        "g2p": (sec.get("g2p_advect", 0.0), args.steps),
+    }
+    stages = {}
+    for k, (ms, calls) in kern.items():
+        if calls <= 0 or ms <= 0:
+            continue
+        avg_ms = ms / calls
+        b = BYTES[k](npart, nf, na)
+        stages[k] = {"avg_ms": round(avg_ms, 5), "launches": int(calls),
+                     "alg_bytes": int(b), "GBps": round(b / (avg_ms * 1e-3) / 1e9, 1),
+                     "frac_hbm": round(b / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    pcg_ms = float(kms[:3].sum() + kms[3])
+    pcg_bytes = n_it * (9 + 24 + 13) * nf + n_rep * 17 * nf
+    dom = max(("pcg_spmv", "pcg_update", "pcg_dir"), key=lambda k: kern[k][0])
+    d = stages[dom]
+    roofline = {"bound": "hbm", "achieved": d["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": d["frac_hbm"], "traffic": None, "kernel": dom,
+                "alg_bytes_per_launch": d["alg_bytes"], "avg_launch_ms": d["avg_ms"],
+                "pcg_total_GBps": round(pcg_bytes / (pcg_ms * 1e-3) / 1e9, 1),
+                "pcg_total_frac": round(pcg_bytes / (pcg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+    fp64 = None
+    if not args.no_fp64_compare:
+        fp64 = compare_fp64(s, tol, max_iter, args.check_every)
+        log(f"pcg mixed vs fp64: {fp64}")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        c = cpu_baseline(scene)
+        if c is not None:
+            cpu = {"value": c["mps"], "unit": "Mparticle-substeps/s", "cores": c["cores"],
+                   "kind": "port", "sample": c.get("sample", "1 substep of the same scene"),
+                   "pcg_iters": c.get("iters"), "seconds": c.get("seconds")}
+            log(f"cpu baseline: {cpu}")
+
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": "Mparticle-substeps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 (f64 dots + residual replacement)",
+        "data": "synthetic",
+        "config": {"workload": scene.name, "grid": list(scene.shape), "particles": npart,
+                   "transfer": p.get("transfer", "flip"), "dt": dt, "pcg_tol": tol,
+                   "pcg_max_iter": max_iter, "pcg_precision": args.precision,
+                   "pcg_iters": iters, "pcg_replacements": n_rep,
+                   "fluid_cells": int(nf), "active_tiles": int(na // 512)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "stages": stages,
+        "pcg_vs_fp64": fp64,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def compare_fp64(s, tol, max_iter, check_every):
+    """Solve the last substep's pressure system again with the mixed and the
+    plain FP64 PCG on the same right-hand side; time both with HIP events."""
+    import torch
+    from gpufluid_amd import _lib
+    sc = s._sc
+    sc.ensure_fp64()
+    res = {}
+    for name, prec in (("mixed", _lib.GF_PCG_MIXED), ("fp64", _lib.GF_PCG_FP64)):
+        d = s._descriptor()
+        it, rr, nrep = ctypes.c_int(), ctypes.c_double(), ctypes.c_int()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        a.record()
+        _lib.call("S2.6.3", "gf_scene_pcg_sync", ctypes.byref(d), sc.n_active_host, prec, tol,
+                  max_iter, check_every, -1, ctypes.byref(it), ctypes.byref(rr),
+                  ctypes.byref(nrep), None, s._stream())
+        b.record()
+        torch.cuda.synchronize()
+        res[name] = {"ms": round(a.elapsed_time(b), 3), "iters": it.value,
+                     "rel_residual": rr.value}
+    res["speedup"] = round(res["fp64"]["ms"] / res["mixed"]["ms"], 3)
+    return res
+
+
+if __name__ == "__main__":
+    main()

@@ -20,6 +20,7 @@
 // whenever ||r|| has dropped by 10x since the last replacement and before
 // convergence is accepted (reliable updates, Sleijpen & van der Vorst 1996).
 #include "gf_common.h"
+#include <string.h>
 
 namespace {
 
@@ -388,7 +389,8 @@ extern "C" int gf_scene_jacobi(const gf_scene_t* s, int n_active, int iters, voi
 
 extern "C" int gf_scene_pcg_sync(const gf_scene_t* s, int n_active, int precision, double tol,
                                  int max_iter, int check_every, int rhs_sign, int* iters,
-                                 double* rel_residual, int* replacements, void* stream) {
+                                 double* rel_residual, int* replacements, double* kernel_ms,
+                                 void* stream) {
     int rc = gf::scene_check(s, "gf_scene_pcg_sync");
     if (rc) return rc;
     const int ntiles = s->tnx * s->tny * s->tnz;
@@ -430,31 +432,79 @@ extern "C" int gf_scene_pcg_sync(const gf_scene_t* s, int n_active, int precisio
         hipLaunchKernelGGL((k_pcg_setup<double, false>), grid, block, 0, st, a, tol2, delta2, max_iter);
     GF_LAUNCH_CHECK("k_pcg_setup");
 
+    // Optional per-kernel timing of the iterations that did work: events
+    // around every launch of a chunk, read back at the chunk's host sync.
+    const bool timed = kernel_ms != nullptr;
+    hipEvent_t ev[4 * 64 + 2];
+    int nev = 0;
+    if (timed) {
+        if (check_every > 64) check_every = 64;
+        for (int i = 0; i < 4; ++i) kernel_ms[i] = 0.0;
+        nev = 4 * check_every + 2;
+        for (int i = 0; i < nev; ++i) {
+            e = hipEventCreate(&ev[i]);
+            if (e != hipSuccess) {
+                for (int j = 0; j < i; ++j) hipEventDestroy(ev[j]);
+                return gf::set_error((int)e, "gf_scene_pcg_sync: event: %s", hipGetErrorString(e));
+            }
+        }
+    }
     PcgState hs;
-    int launched = 0;
+    memset(&hs, 0, sizeof(hs));
+    int launched = 0, iters_before = 0, nupd_before = 0;
+    rc = 0;
     for (;;) {
         const int chunk = check_every;
         for (int j = 0; j < chunk; ++j) {
-            if (mixed) {
-                hipLaunchKernelGGL(k_pcg_spmv<float>, grid, block, 0, st, a);
-                hipLaunchKernelGGL((k_pcg_update<float, true>), grid, block, 0, st, a);
-                hipLaunchKernelGGL(k_pcg_dir<float>, grid, block, 0, st, a);
-            } else {
-                hipLaunchKernelGGL(k_pcg_spmv<double>, grid, block, 0, st, a);
-                hipLaunchKernelGGL((k_pcg_update<double, false>), grid, block, 0, st, a);
-                hipLaunchKernelGGL(k_pcg_dir<double>, grid, block, 0, st, a);
-            }
+            if (timed) hipEventRecord(ev[4 * j], st);
+            if (mixed) hipLaunchKernelGGL(k_pcg_spmv<float>, grid, block, 0, st, a);
+            else hipLaunchKernelGGL(k_pcg_spmv<double>, grid, block, 0, st, a);
+            if (timed) hipEventRecord(ev[4 * j + 1], st);
+            if (mixed) hipLaunchKernelGGL((k_pcg_update<float, true>), grid, block, 0, st, a);
+            else hipLaunchKernelGGL((k_pcg_update<double, false>), grid, block, 0, st, a);
+            if (timed) hipEventRecord(ev[4 * j + 2], st);
+            if (mixed) hipLaunchKernelGGL(k_pcg_dir<float>, grid, block, 0, st, a);
+            else hipLaunchKernelGGL(k_pcg_dir<double>, grid, block, 0, st, a);
+            if (timed) hipEventRecord(ev[4 * j + 3], st);
         }
         launched += chunk;
-        if (mixed) hipLaunchKernelGGL(k_pcg_replace, grid, block, 0, st, a);
-        GF_LAUNCH_CHECK("pcg iteration");
+        if (mixed) {
+            if (timed) hipEventRecord(ev[4 * chunk], st);
+            hipLaunchKernelGGL(k_pcg_replace, grid, block, 0, st, a);
+            if (timed) hipEventRecord(ev[4 * chunk + 1], st);
+        }
+        rc = gf::check_launch("pcg iteration");
+        if (rc) break;
         e = hipMemcpyAsync(&hs, a.st, sizeof(PcgState), hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
-        if (e != hipSuccess)
-            return gf::set_error((int)e, "gf_scene_pcg_sync: readback: %s", hipGetErrorString(e));
+        if (e != hipSuccess) {
+            rc = gf::set_error((int)e, "gf_scene_pcg_sync: readback: %s", hipGetErrorString(e));
+            break;
+        }
+        if (timed) {
+            const int real = hs.iters - iters_before;
+            for (int j = 0; j < real && j < chunk; ++j) {
+                float t0 = 0.f, t1 = 0.f, t2 = 0.f;
+                hipEventElapsedTime(&t0, ev[4 * j], ev[4 * j + 1]);
+                hipEventElapsedTime(&t1, ev[4 * j + 1], ev[4 * j + 2]);
+                hipEventElapsedTime(&t2, ev[4 * j + 2], ev[4 * j + 3]);
+                kernel_ms[0] += t0;
+                kernel_ms[1] += t1;
+                kernel_ms[2] += t2;
+            }
+            if (mixed && hs.nupd > nupd_before) {
+                float t3 = 0.f;
+                hipEventElapsedTime(&t3, ev[4 * chunk], ev[4 * chunk + 1]);
+                kernel_ms[3] += t3;
+            }
+        }
+        iters_before = hs.iters;
+        nupd_before = hs.nupd;
         if (hs.done) break;
         if (hs.iters >= max_iter || launched > max_iter + 2 * chunk) break;
     }
+    for (int i = 0; i < nev; ++i) hipEventDestroy(ev[i]);
+    if (rc) return rc;
     if (mixed)
         hipLaunchKernelGGL(k_pcg_final<true>, grid, block, 0, st, a);
     else

@@ -161,7 +161,10 @@ __global__ __launch_bounds__(1024) void k_tile_scan(const int32_t* __restrict__ 
         }
         carry += total;
     }
-    if (threadIdx.x == 0) n_active[0] = carry;
+    if (threadIdx.x == 0) {
+        n_active[0] = carry;
+        n_active[1] = 0;  // fluid cells, counted by k_tile_setup
+    }
 }
 
 // Per tile: new marker (solid kept, fluid where a particle sits), compact
@@ -169,13 +172,15 @@ __global__ __launch_bounds__(1024) void k_tile_scan(const int32_t* __restrict__ 
 __global__ __launch_bounds__(GF_WG) void k_tile_setup(
     int nx, int ny, int nz, int tnx, int tny, int tnz, const int32_t* __restrict__ cell_start,
     int32_t* marker, const int32_t* __restrict__ tile_map, uint8_t* __restrict__ flags,
-    int32_t* __restrict__ nbr) {
+    int32_t* __restrict__ nbr, int32_t* __restrict__ n_active) {
+    __shared__ int s_wave[GF_WG / GF_WAVE + 1];
     const int ntiles = tnx * tny * tnz;
     const int t = gf::xcd_remap(blockIdx.x, ntiles);
     const int tk = t % tnz, tj = (t / tnz) % tny, ti = t / (tnz * tny);
     const int slot = tile_map[t];
     const int i0 = ti * 8, j0 = tj * 8, k0 = tk * 8;
     const int lj = (threadIdx.x >> 3) & 7, lk = threadIdx.x & 7;
+    int nfluid = 0;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         const int li = (threadIdx.x >> 6) + 4 * h;
@@ -197,10 +202,14 @@ __global__ __launch_bounds__(GF_WG) void k_tile_setup(
                 if (k > 0 && marker[c - 1] != 2) ++d;
                 if (k < nz - 1 && marker[c + 1] != 2) ++d;
                 f = (uint8_t)(8 | d);
+                ++nfluid;
             }
         }
         if (slot >= 0) flags[(size_t)slot * GF_TILE_CELLS + l] = f;
     }
+    int total;
+    gf::wg_exclusive_scan(nfluid, s_wave, &total);
+    if (threadIdx.x == 0 && total > 0) atomicAdd(&n_active[1], total);
     if (slot >= 0 && threadIdx.x < 6) {
         int d = threadIdx.x;
         int a = ti, b = tj, c = tk;
@@ -279,17 +288,18 @@ extern "C" int gf_scene_sort_sync(gf_scene_t* s, int with_affine, int* n_active_
                        s->tile_map, s->active, s->n_active);
     hipLaunchKernelGGL(k_tile_setup, dim3(ntiles), dim3(GF_WG), 0, st, s->nx, s->ny, s->nz,
                        s->tnx, s->tny, s->tnz, s->cell_start, s->marker, s->tile_map,
-                       s->flags, s->nbr);
+                       s->flags, s->nbr, s->n_active);
     GF_LAUNCH_CHECK("k_tile_scan/k_tile_setup");
     if (n_active_host) {
-        int32_t na = 0;
-        hipError_t e = hipMemcpyAsync(&na, s->n_active, sizeof(int32_t),
+        int32_t na[2] = {0, 0};
+        hipError_t e = hipMemcpyAsync(na, s->n_active, 2 * sizeof(int32_t),
                                       hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
         if (e != hipSuccess)
             return gf::set_error((int)e, "gf_scene_sort_sync: readback: %s",
                                  hipGetErrorString(e));
-        *n_active_host = na;
+        n_active_host[0] = na[0];
+        n_active_host[1] = na[1];
     }
     return 0;
 }

@@ -152,9 +152,12 @@ class FlipSolver3D:
         self._uvw_weights = None
         self._sc = SceneBuffers(self.nx, self.ny, self.nz, self.dx, self.dom, dev)
         self.n_particles = 0
+        self._sc_fluid_cells = 0
         self.last_pressure_iters = 0
         self.last_pressure_residual = None
         self.last_pressure_replacements = 0
+        self.pcg_kernel_timing = False      # per-kernel HIP-event timing of PCG launches
+        self.last_pcg_kernel_ms = None      # (spmv, update, direction, replacement) ms
         self._cache = {}
         self._pcg_done = DeviceArray(torch.zeros(1, dtype=torch.int32, device=dev))
         self._pcg_tol_sq = DeviceArray(torch.zeros(1, dtype=torch.float32, device=dev))
@@ -343,16 +346,17 @@ class FlipSolver3D:
         sc.ensure_particles(n, apic)
         with prof.section("clear"):
             s = self._descriptor()
-            na = ctypes.c_int(0)
+            na = (ctypes.c_int * 2)()
             try:
                 _lib.call("F3.3", "gf_scene_sort_sync", ctypes.byref(s), int(apic),
-                          ctypes.byref(na), stream)
+                          na, stream)
             except BlockError:
                 sc.reset_counts()
                 raise
             if n > 0:
                 sc.swap()
-            sc.n_active_host = na.value
+            sc.n_active_host = na[0]
+            self._sc_fluid_cells = na[1]
         s = self._descriptor()
         with prof.section("p2g"):
             _lib.call("S2.1", "gf_scene_p2g", ctypes.byref(s), mode, self.gravity, dt, self.rho,
@@ -362,7 +366,7 @@ class FlipSolver3D:
         prof.fused("divergence")
         with prof.section("pressure"):
             if pressure_solver == "jacobi":
-                _lib.call("S2.6.1", "gf_scene_jacobi", ctypes.byref(s), na.value,
+                _lib.call("S2.6.1", "gf_scene_jacobi", ctypes.byref(s), na[0],
                           pressure_iters, stream)
                 self.last_pressure_iters = pressure_iters
                 self.last_pressure_residual = None
@@ -375,9 +379,12 @@ class FlipSolver3D:
                 res = ctypes.c_double(0.0)
                 nrep = ctypes.c_int(0)
                 sign = -1 if self.pcg_sign == "physical" else 1
-                _lib.call("S2.6.3", "gf_scene_pcg_sync", ctypes.byref(s), na.value, prec, tol,
+                kms = (ctypes.c_double * 4)() if self.pcg_kernel_timing else None
+                _lib.call("S2.6.3", "gf_scene_pcg_sync", ctypes.byref(s), na[0], prec, tol,
                           pressure_iters, self.pcg_check_every, sign, ctypes.byref(it),
-                          ctypes.byref(res), ctypes.byref(nrep), stream)
+                          ctypes.byref(res), ctypes.byref(nrep), kms, stream)
+                if kms is not None:
+                    self.last_pcg_kernel_ms = tuple(kms)
                 self.last_pressure_iters = it.value
                 self.last_pressure_residual = res.value
                 self.last_pressure_replacements = nrep.value

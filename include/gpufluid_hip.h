@@ -87,7 +87,7 @@ typedef struct gf_scene {
     int32_t* tile_occ;           /* (ntiles) 1 if the tile holds a fluid cell */
     int32_t* tile_map;           /* (ntiles) compact slot or -1 */
     int32_t* active;             /* (ntiles) tile id per compact slot */
-    int32_t* n_active;           /* (4) device counter; [0] = active tiles */
+    int32_t* n_active;           /* (4) [0] active tiles, [1] fluid cells */
     int32_t* nbr;                /* (ntiles*6) neighbour slots -x,+x,-y,+y,-z,+z or -1 */
     /* dense MAC grid */
     int32_t* marker;             /* (nx,ny,nz) */
@@ -119,7 +119,8 @@ int gf_scene_sizes(int nx, int ny, int nz, gf_scene_size_info* out);
 /* Bin particles to cells and sort them stably by flat cell key into the *_b
  * arrays; build per-cell ranges, the marker (walls/obstacles kept), the tile
  * map, the compact flags and the tile neighbour table. Writes the active tile
- * count to n_active[0] and also returns it in *n_active_host (sync).
+ * count to n_active[0] and the fluid cell count to n_active[1], and returns
+ * both in the host array n_active_host[2] (sync; NULL skips the readback).
  * Follows solver3d.py:41-65 (marker clear), 96-101 (cell binning/marking),
  * sim/reseed.py:72-75,114 (flat key, stable in-cell order), 953-969 (8^3 tiles). */
 int gf_scene_sort_sync(gf_scene_t* s, int with_affine, int* n_active_host, void* stream);
@@ -153,10 +154,13 @@ int gf_scene_jacobi(const gf_scene_t* s, int n_active, int iters, void* stream);
  * Jacobi-consistent pressure, +1 reproduces the reference PCG's sign.
  * Result in s->pc.  Host outputs (may be NULL): iterations, true relative
  * residual ||b-Ap||/||b|| (FP64; mixed mode only, else the iterated one),
- * number of FP64 residual replacements. */
+ * number of FP64 residual replacements, and kernel_ms[4]: summed GPU time of
+ * the spmv, update, direction and residual-replacement kernels over the
+ * launches that did work (HIP events on `stream`; NULL disables timing). */
 int gf_scene_pcg_sync(const gf_scene_t* s, int n_active, int precision, double tol,
                       int max_iter, int check_every, int rhs_sign, int* iters,
-                      double* rel_residual, int* replacements, void* stream);
+                      double* rel_residual, int* replacements, double* kernel_ms,
+                      void* stream);
 
 /* Fused pressure-gradient subtract + solid BC + G2P + advection over 8^3
  * tiles: k3_subtract_pressure_grad (solver3d.py:1364-1407), k3_enforce_solid_bc

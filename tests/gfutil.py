@@ -1,0 +1,67 @@
+"""Shared helpers for the parity tests (not a test module)."""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from oracle import flip_oracle as fo
+
+
+def rel_err(a, b) -> float:
+    """max|a-b| / max|b| (the reference tests' pair-parity measure,
+    tests/test_b5_a_pcg_graph.py:75-78), with a floor on the scale."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    scale = max(float(np.abs(b).max()) if b.size else 0.0, 1e-12)
+    return float(np.abs(a - b).max()) / scale if a.size else 0.0
+
+
+def dam_scene(n=16, seed=0, x_max=0.4, y_max=0.6, vel_sigma=0.0, shape=None):
+    """Seeded dam-break particles for an (n,n,n) or given grid with dx = 1/max(shape)."""
+    from gpufluid_amd.seeding import fluid_cells, seed_cells
+    shape = shape or (n, n, n)
+    dx = 1.0 / max(shape)
+    cells = fluid_cells(shape, dx, lambda x, y, z: (x < x_max) & (y < y_max))
+    rng = np.random.default_rng(seed)
+    pos = seed_cells(cells, dx, rng)
+    vel = (rng.normal(0.0, vel_sigma, pos.shape) if vel_sigma else np.zeros_like(pos))
+    return shape, dx, pos, vel.astype(np.float32)
+
+
+def make_solver(shape, dx, pos, vel, **kw):
+    from gpufluid_amd import FlipSolver3D
+    s = FlipSolver3D(*shape, dx=dx, **kw)
+    s.set_particles(pos, vel)
+    return s
+
+
+def run_sort(s):
+    """Run only the binning/sort stage on a solver; returns n_active."""
+    from gpufluid_amd import _lib
+    sc = s._sc
+    apic = s.transfer_mode == "apic"
+    sc.ensure_particles(s.n_particles, apic)
+    d = s._descriptor()
+    na = (ctypes.c_int * 2)()
+    _lib.call("F3.3", "gf_scene_sort_sync", ctypes.byref(d), int(apic), na,
+              s._stream())
+    if s.n_particles:
+        sc.swap()
+    sc.n_active_host = na[0]
+    s._cache.clear()
+    return na[0]
+
+
+def run_p2g(s, dt, gravity=None):
+    from gpufluid_amd import _lib
+    d = s._descriptor()
+    mode = _lib.GF_TRANSFER_APIC if s.transfer_mode == "apic" else _lib.GF_TRANSFER_FLIP
+    _lib.call("S2.1", "gf_scene_p2g", ctypes.byref(d), mode,
+              s.gravity if gravity is None else gravity, dt, s.rho, s._stream())
+    s._cache.clear()
+
+
+def oracle_state(shape, dx, pos, vel, marker=None):
+    m = fo.wall_marker(*shape) if marker is None else marker.copy()
+    return {"pos": pos.copy(), "vel": vel.copy(), "marker": m, "dx": dx}
