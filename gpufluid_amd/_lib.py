@@ -70,7 +70,7 @@ SIGNATURES = {
     "gf_scene_p2g_raw": [SP, I, V, V, V, V],
     "gf_scene_jacobi": [SP, I, I, V],
     "gf_scene_pcg_sync": [SP, I, I, D, I, I, I, PI, PD, PI, PD, V],
-    "gf_scene_g2p": [SP, V, I, F, F, F, V],
+    "gf_scene_g2p": [SP, V, I, F, F, F, F, V],
     "gf_scene_compact_to_dense": [SP, V, V, V],
     "gf_scene_unsort": [SP, V, I, V, V],
     "gf_k3_clear_grid": [V, V, V, V, V, V, V, I, I, I, V],

@@ -271,7 +271,7 @@ struct G2PParams {
     const float* su; const float* sv; const float* sw;
     const float* pc;
     int nx, ny, nz, tnx, tny, tnz;
-    float dx, dt, rho, flip_blend;
+    float dx, dt, rho, flip_blend, gdt;
     float domx, domy, domz;
 };
 
@@ -443,17 +443,18 @@ __global__ __launch_bounds__(GF_WG) void k_g2p_tile(G2PParams a) {
             float val = 0.0f, sav = 0.0f;
             if (ii >= 0 && ii < fni && jj >= 0 && jj < fnj && kk >= 0 && kk < fnk) {
                 const size_t f = ((size_t)ii * fnj + jj) * fnk + kk;
-                const float orig = gfld[f];
-                val = orig;
-                if (!APIC) sav = gsav[f];
+                // Pre-projection face value rebuilt from the saved copy exactly
+                // as P2G produced it (us [+ g*dt on v], then the BC below), so
+                // the halo never reads u while owners overwrite it.
+                sav = gsav[f];
+                val = c == 1 ? sav + a.gdt : sav;
                 // Cell coords of the face's two sides in the staged cell region.
                 const int ca = ii - (i0 - 1), cb = jj - (j0 - 1), cc = kk - (k0 - 1);
                 int la, lb, lc;  // low-side cell
-                int gi_lo, n_ax, g_ax;
+                int n_ax, g_ax;
                 if (c == 0) { la = ca - 1; lb = cb; lc = cc; g_ax = ii; n_ax = nx; }
                 else if (c == 1) { la = ca; lb = cb - 1; lc = cc; g_ax = jj; n_ax = ny; }
                 else { la = ca; lb = cb; lc = cc - 1; g_ax = kk; n_ax = nz; }
-                (void)gi_lo;
                 // ca..cc lie in [0,10) for every staged face except the +1
                 // planes of the last tile along an axis, which the grid bounds
                 // exclude from the gradient (g_ax == n_ax) anyway.
@@ -461,6 +462,13 @@ __global__ __launch_bounds__(GF_WG) void k_g2p_tile(G2PParams a) {
                 const bool lo_in = la >= 0 && lb >= 0 && lc >= 0 && la < 10 && lb < 10 && lc < 10;
                 const int mlo = lo_in ? s_mk[la][lb][lc] : 0;
                 const int mhi = hi_in ? s_mk[ca][cb][cc] : 0;
+                const bool lo_side = g_ax == 0 || mlo == 2;
+                const bool hi_side = g_ax == n_ax || mhi == 2;
+                float sv = 0.0f;
+                if (lo_side || hi_side) {
+                    sv = solid ? solid[f] : 0.0f;
+                    val = sv;
+                }
                 bool changed = false;
                 if (g_ax >= 1 && g_ax <= n_ax - 1) {
                     if ((mlo == 1 || mhi == 1) && mlo != 2 && mhi != 2) {
@@ -470,12 +478,7 @@ __global__ __launch_bounds__(GF_WG) void k_g2p_tile(G2PParams a) {
                         changed = true;
                     }
                 }
-                const bool lo_side = g_ax == 0 || mlo == 2;
-                const bool hi_side = g_ax == n_ax || mhi == 2;
-                if (lo_side || hi_side) {
-                    const float sv = solid ? solid[f] : 0.0f;
-                    if (sv != val) { val = sv; changed = true; }
-                }
+                if (lo_side || hi_side) val = sv;
                 const bool owned = (c == 0 ? (li < 8 || ii == nx) : (li >= 1 && li < 9)) &&
                                    (c == 1 ? (lj < 8 || jj == ny) : (lj >= 1 && lj < 9)) &&
                                    (c == 2 ? (lk < 8 || kk == nz) : (lk >= 1 && lk < 9));
@@ -620,7 +623,7 @@ extern "C" int gf_scene_p2g_raw(const gf_scene_t* s, int transfer_mode, float* u
 }
 
 extern "C" int gf_scene_g2p(const gf_scene_t* s, const float* pressure, int transfer_mode,
-                            float dt, float rho, float flip_blend, void* stream) {
+                            float gravity, float dt, float rho, float flip_blend, void* stream) {
     int rc = gf::scene_check(s, "gf_scene_g2p");
     if (rc) return rc;
     GF_REQUIRE(transfer_mode == GF_TRANSFER_FLIP || transfer_mode == GF_TRANSFER_APIC,
@@ -628,8 +631,7 @@ extern "C" int gf_scene_g2p(const gf_scene_t* s, const float* pressure, int tran
     GF_REQUIRE(s->u && s->v && s->w && s->marker && s->cell_start && s->tile_map && pressure,
                "gf_scene_g2p: grid buffers not set");
     GF_REQUIRE(rho != 0.0f, "gf_scene_g2p: rho must be nonzero");
-    if (transfer_mode == GF_TRANSFER_FLIP)
-        GF_REQUIRE(s->us && s->vs && s->ws, "gf_scene_g2p: FLIP needs us/vs/ws");
+    GF_REQUIRE(s->us && s->vs && s->ws, "gf_scene_g2p: us/vs/ws not set");
     if (s->n > 0) GF_REQUIRE(s->pos && s->vel, "gf_scene_g2p: particles not set");
     if (transfer_mode == GF_TRANSFER_APIC && s->n > 0)
         GF_REQUIRE(s->affine_c, "gf_scene_g2p: APIC needs affine_c");
@@ -640,7 +642,7 @@ extern "C" int gf_scene_g2p(const gf_scene_t* s, const float* pressure, int tran
     a.su = s->solid_u; a.sv = s->solid_v; a.sw = s->solid_w;
     a.pc = pressure;
     a.nx = s->nx; a.ny = s->ny; a.nz = s->nz; a.tnx = s->tnx; a.tny = s->tny; a.tnz = s->tnz;
-    a.dx = s->dx; a.dt = dt; a.rho = rho; a.flip_blend = flip_blend;
+    a.dx = s->dx; a.dt = dt; a.rho = rho; a.flip_blend = flip_blend; a.gdt = gravity * dt;
     a.domx = s->dom[0]; a.domy = s->dom[1]; a.domz = s->dom[2];
     const int ntiles = s->tnx * s->tny * s->tnz;
     hipStream_t st = (hipStream_t)stream;

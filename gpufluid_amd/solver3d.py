@@ -390,8 +390,8 @@ class FlipSolver3D:
                 self.last_pressure_replacements = nrep.value
                 self._pcg_tol_sq.t.fill_(float(np.float32(tol) ** 2))
         with prof.section("g2p_advect"):
-            _lib.call("S2.8", "gf_scene_g2p", ctypes.byref(s), _lib.ptr(sc.pc), mode, dt,
-                      self.rho, flip_blend, stream)
+            _lib.call("S2.8", "gf_scene_g2p", ctypes.byref(s), _lib.ptr(sc.pc), mode,
+                      self.gravity, dt, self.rho, flip_blend, stream)
         prof.fused("grad_subtract_bc")
         prof.fused("color")
 

@@ -165,10 +165,13 @@ int gf_scene_pcg_sync(const gf_scene_t* s, int n_active, int precision, double t
 /* Fused pressure-gradient subtract + solid BC + G2P + advection over 8^3
  * tiles: k3_subtract_pressure_grad (solver3d.py:1364-1407), k3_enforce_solid_bc
  * (758-799), k3_g2p_and_advect (1414-1463) or k3_g2p_apic_advect (1591-1732).
- * `pressure` is a compact pressure vector (s->pc). Updates u,v,w and the
- * sorted particle arrays in place. */
+ * `pressure` is a compact pressure vector (s->pc). The pre-projection face
+ * values are rebuilt from us,vs,ws exactly as gf_scene_p2g produced them
+ * (us [+ gravity*dt on v], then the BC), so this call must follow
+ * gf_scene_p2g with the same gravity and dt. Updates u,v,w (faces next to a
+ * fluid cell) and the sorted particle arrays in place. */
 int gf_scene_g2p(const gf_scene_t* s, const float* pressure, int transfer_mode,
-                 float dt, float rho, float flip_blend, void* stream);
+                 float gravity, float dt, float rho, float flip_blend, void* stream);
 
 /* Scatter a compact cell vector (e.g. s->pc, s->b) into a dense (nx,ny,nz)
  * array; cells of inactive tiles get 0. No reference counterpart. */
