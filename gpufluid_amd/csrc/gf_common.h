@@ -72,58 +72,6 @@ __device__ __forceinline__ int wg_exclusive_scan(int v, int* s_wave, int* total)
     return res;
 }
 
-// Deterministic two-level FP64 reduction of NV values per thread over the whole
-// grid.  Every workgroup writes its partial sums to partials[k*nblocks + bid];
-// the last workgroup to arrive sums all partials in index order (so the result
-// does not depend on arrival order), stores the totals in tot[] and returns
-// true (uniformly across that workgroup); all other workgroups return false.
-template <int NV>
-__device__ bool wg_reduce_last(const double (&v)[NV], double* partials, int nblocks,
-                               unsigned int* counter, double (&tot)[NV]) {
-    __shared__ double s_red[NV][GF_WG / GF_WAVE];
-    __shared__ int s_last;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int nwaves = blockDim.x >> 6;
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-        double t = wave_sum(v[k]);
-        if (lane == 0) s_red[k][wave] = t;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; ++k) {
-            double a = 0.0;
-            for (int i = 0; i < nwaves; ++i) a += s_red[k][i];
-            partials[(size_t)k * nblocks + blockIdx.x] = a;
-        }
-        __threadfence();
-        unsigned int ticket = atomicAdd(counter, 1u);
-        s_last = (ticket == (unsigned int)(nblocks - 1));
-    }
-    __syncthreads();
-    if (!s_last) return false;
-    __threadfence();
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-        double a = 0.0;
-        for (int i = threadIdx.x; i < nblocks; i += blockDim.x)
-            a += __builtin_nontemporal_load(&partials[(size_t)k * nblocks + i]);
-        a = wave_sum(a);
-        __syncthreads();
-        if (lane == 0) s_red[k][wave] = a;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-        double a = 0.0;
-        for (int i = 0; i < nwaves; ++i) a += s_red[k][i];
-        tot[k] = a;
-    }
-    if (threadIdx.x == 0) *counter = 0u;
-    return true;
-}
-
 }  // namespace gf
 
 // Launch helpers (host).

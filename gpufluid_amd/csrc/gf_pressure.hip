@@ -24,14 +24,19 @@
 
 namespace {
 
-struct PcgState {
-    double rz, pAp, alpha, beta, rr, bb, rr_upd, tol2, delta2, rr_true;
-    int done, converged, upd, merge, iters, max_iter, maxed, nupd;
-    int k3go, pad0;
-    unsigned int counter;
-    int pad[5];
+// Solver scalars.  Kernels form a fixed launch sequence numbered L; launch L
+// reads entry (L-1)&1 and writes entry L&1 (workgroup 0 only), so no kernel
+// reads a field that it also writes.
+struct PcgLog {
+    double rz, rr, bb, alpha, beta, rr_upd, rr_true, pap;
+    int iters, done, converged, maxed, upd, merge, nupd, pending;
+    double pad[4];
 };
-static_assert(sizeof(PcgState) <= 64 * sizeof(double), "state must fit in scalars");
+static_assert(sizeof(PcgLog) == 16 * sizeof(double), "log entry is 16 doubles");
+
+// Per-workgroup partial sums, PCG_GMAX entries per kind.
+enum { P_PAP = 0, P_RZ, P_RR, P_BB, P_RRT, P_RZT, P_KINDS };
+constexpr int PCG_GMAX = 1024;
 
 struct PcgArgs {
     const uint8_t* flags;
@@ -40,10 +45,13 @@ struct PcgArgs {
     float* x32; float* r32; float* p32; float* ap32;
     double* x64; double* r64; double* p64; double* ap64;
     float* pc;
-    double* partials;
-    PcgState* st;
-    int nact;
+    double* part;        // [P_KINDS][PCG_GMAX]
+    PcgLog* log;         // [2]
+    int nact;            // active tiles
+    int G;               // workgroups; workgroup b owns tiles [b*nact/G, (b+1)*nact/G)
     float rhs_sign;
+    double tol2, delta2;
+    int max_iter;
 };
 
 __device__ __forceinline__ int cell_l(int h) {
@@ -51,29 +59,56 @@ __device__ __forceinline__ int cell_l(int h) {
     return (li * 8 + lj) * 8 + lk;
 }
 
+// Tile range of this workgroup; consecutive logical workgroups sit on the same
+// XCD so that neighbouring tiles share an L2.
+__device__ __forceinline__ void tile_range(const PcgArgs& a, int& t0, int& t1, int& b) {
+    b = gf::xcd_remap(blockIdx.x, a.G);
+    t0 = (int)(((long long)b * a.nact) / a.G);
+    t1 = (int)(((long long)(b + 1) * a.nact) / a.G);
+}
+
 // Sum of the six neighbour values in the order -x,+x,-y,+y,-z,+z of the
 // reference stencil; neighbours in inactive tiles read as 0.
 template <typename T, typename F>
-__device__ __forceinline__ T nb_sum(F get, size_t base, int l, const int* s_nbr) {
+__device__ __forceinline__ T nb_sum(F get, size_t base, int l, const int32_t* nb) {
     const int li = l >> 6, lj = (l >> 3) & 7, lk = l & 7;
     T s = (T)0;
-    s += li > 0 ? get(base + l - 64) : (s_nbr[0] >= 0 ? get((size_t)s_nbr[0] * 512 + l + 448) : (T)0);
-    s += li < 7 ? get(base + l + 64) : (s_nbr[1] >= 0 ? get((size_t)s_nbr[1] * 512 + l - 448) : (T)0);
-    s += lj > 0 ? get(base + l - 8) : (s_nbr[2] >= 0 ? get((size_t)s_nbr[2] * 512 + l + 56) : (T)0);
-    s += lj < 7 ? get(base + l + 8) : (s_nbr[3] >= 0 ? get((size_t)s_nbr[3] * 512 + l - 56) : (T)0);
-    s += lk > 0 ? get(base + l - 1) : (s_nbr[4] >= 0 ? get((size_t)s_nbr[4] * 512 + l + 7) : (T)0);
-    s += lk < 7 ? get(base + l + 1) : (s_nbr[5] >= 0 ? get((size_t)s_nbr[5] * 512 + l - 7) : (T)0);
+    s += li > 0 ? get(base + l - 64) : (nb[0] >= 0 ? get((size_t)nb[0] * 512 + l + 448) : (T)0);
+    s += li < 7 ? get(base + l + 64) : (nb[1] >= 0 ? get((size_t)nb[1] * 512 + l - 448) : (T)0);
+    s += lj > 0 ? get(base + l - 8) : (nb[2] >= 0 ? get((size_t)nb[2] * 512 + l + 56) : (T)0);
+    s += lj < 7 ? get(base + l + 8) : (nb[3] >= 0 ? get((size_t)nb[3] * 512 + l - 56) : (T)0);
+    s += lk > 0 ? get(base + l - 1) : (nb[4] >= 0 ? get((size_t)nb[4] * 512 + l + 7) : (T)0);
+    s += lk < 7 ? get(base + l + 1) : (nb[5] >= 0 ? get((size_t)nb[5] * 512 + l - 7) : (T)0);
     return s;
 }
 
-__device__ __forceinline__ void load_nbr(const int32_t* nbr, int slot, int* s_nbr) {
-    if (threadIdx.x < 6) s_nbr[threadIdx.x] = nbr[(size_t)slot * 6 + threadIdx.x];
+// Workgroup sum of one value per thread, in a fixed order; result in all threads.
+__device__ __forceinline__ double wg_sum(double v, double* s_red) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    v = gf::wave_sum(v);
     __syncthreads();
+    if (lane == 0) s_red[wave] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int i = 0; i < GF_WG / GF_WAVE; ++i) t += s_red[i];
+    return t;
 }
 
-template <typename T> __device__ __forceinline__ T* vx(const PcgArgs& a);
-template <> __device__ __forceinline__ float* vx<float>(const PcgArgs& a) { return a.x32; }
-template <> __device__ __forceinline__ double* vx<double>(const PcgArgs& a) { return a.x64; }
+// Sum of the G partials of one kind (deterministic, identical in every
+// workgroup).  Written by the previous kernel, so visible at this launch.
+__device__ __forceinline__ double sum_partials(const double* part, int kind, int G,
+                                               double* s_red) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < G; i += GF_WG) v += part[kind * PCG_GMAX + i];
+    return wg_sum(v, s_red);
+}
+
+__device__ __forceinline__ void put_partial(double* part, int kind, int b, double v,
+                                            double* s_red) {
+    const double t = wg_sum(v, s_red);
+    if (threadIdx.x == 0) part[kind * PCG_GMAX + b] = t;
+}
+
 template <typename T> __device__ __forceinline__ T* vr(const PcgArgs& a);
 template <> __device__ __forceinline__ float* vr<float>(const PcgArgs& a) { return a.r32; }
 template <> __device__ __forceinline__ double* vr<double>(const PcgArgs& a) { return a.r64; }
@@ -84,256 +119,281 @@ template <typename T> __device__ __forceinline__ T* vap(const PcgArgs& a);
 template <> __device__ __forceinline__ float* vap<float>(const PcgArgs& a) { return a.ap32; }
 template <> __device__ __forceinline__ double* vap<double>(const PcgArgs& a) { return a.ap64; }
 
-// x = 0, r = rhs_sign*b, p = z = r/M, rz = r.z, bb = r.r
+enum { PEND_NONE = 0, PEND_SETUP = 1, PEND_REPLACE = 2 };
+
+// L = 0: x = 0, r = rhs_sign*b, p = z = r/M; partials r.z and r.r.
 template <typename T, bool MIXED>
-__global__ __launch_bounds__(GF_WG) void k_pcg_setup(PcgArgs a, double tol2, double delta2,
-                                                     int max_iter) {
-    const int slot = gf::xcd_remap(blockIdx.x, a.nact);
-    const size_t base = (size_t)slot * GF_TILE_CELLS;
+__global__ __launch_bounds__(GF_WG) void k_pcg_setup(PcgArgs a) {
+    __shared__ double s_red[GF_WG / GF_WAVE];
+    int t0, t1, b;
+    tile_range(a, t0, t1, b);
     T* r = vr<T>(a);
     T* p = vp<T>(a);
-    double v[2] = {0.0, 0.0};
+    double rz = 0.0, rr = 0.0;
+    for (int t = t0; t < t1; ++t) {
+        const size_t base = (size_t)t * GF_TILE_CELLS;
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const size_t idx = base + cell_l(h);
-        const uint8_t f = a.flags[idx];
-        T rv = (T)0, z = (T)0;
-        if (f & 8) {
-            rv = (T)(a.rhs_sign * a.b[idx]);
-            const int d = f & 7;
-            z = rv / (T)(d > 1 ? d : 1);
+        for (int h = 0; h < 2; ++h) {
+            const size_t idx = base + cell_l(h);
+            const uint8_t f = a.flags[idx];
+            T rv = (T)0, z = (T)0;
+            if (f & 8) {
+                rv = (T)(a.rhs_sign * a.b[idx]);
+                const int d = f & 7;
+                z = rv / (T)(d > 1 ? d : 1);
+            }
+            r[idx] = rv;
+            p[idx] = z;
+            a.x64[idx] = 0.0;
+            if (MIXED) a.x32[idx] = 0.0f;
+            rz += (double)rv * (double)z;
+            rr += (double)rv * (double)rv;
         }
-        r[idx] = rv;
-        p[idx] = z;
-        a.x64[idx] = 0.0;
-        if (MIXED) a.x32[idx] = 0.0f;
-        v[0] += (double)rv * (double)z;
-        v[1] += (double)rv * (double)rv;
     }
-    double tot[2];
-    if (gf::wg_reduce_last<2>(v, a.partials, gridDim.x, &a.st->counter, tot) && threadIdx.x == 0) {
-        PcgState* st = a.st;
-        st->rz = tot[0];
-        st->bb = tot[1];
-        st->rr = tot[1];
-        st->rr_upd = tot[1];
-        st->rr_true = tot[1];
-        st->tol2 = tol2;
-        st->delta2 = delta2;
-        st->max_iter = max_iter;
-        st->iters = 0;
-        st->maxed = 0;
-        st->upd = 0;
-        st->merge = 0;
-        st->nupd = 0;
-        st->k3go = 0;
-        const int zero = tot[1] < 1e-30;
-        st->done = zero || max_iter <= 0;
-        st->converged = zero;
+    put_partial(a.part, P_RZ, b, rz, s_red);
+    put_partial(a.part, P_BB, b, rr, s_red);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        PcgLog e = {};
+        e.pending = PEND_SETUP;
+        a.log[0] = e;
     }
 }
 
-// Ap = A p, pAp -> alpha
+// One-workgroup finalisation after the setup or a residual replacement.
+__global__ __launch_bounds__(GF_WG) void k_pcg_fin(PcgArgs a, int L) {
+    __shared__ double s_red[GF_WG / GF_WAVE];
+    PcgLog e = a.log[(L - 1) & 1];
+    if (e.pending == PEND_SETUP) {
+        const double rz = sum_partials(a.part, P_RZ, a.G, s_red);
+        const double bb = sum_partials(a.part, P_BB, a.G, s_red);
+        e.rz = rz; e.bb = bb; e.rr = bb; e.rr_upd = bb; e.rr_true = bb;
+        e.iters = 0; e.maxed = 0; e.upd = 0; e.merge = 0; e.nupd = 0;
+        const int zero = bb < 1e-30;
+        e.converged = zero;
+        e.done = zero || a.max_iter <= 0;
+    } else if (e.pending == PEND_REPLACE) {
+        const double rrt = sum_partials(a.part, P_RRT, a.G, s_red);
+        const double rz = sum_partials(a.part, P_RZT, a.G, s_red);
+        e.rr_true = rrt; e.rr = rrt; e.rr_upd = rrt; e.rz = rz;
+        e.upd = 0; e.merge = 1; e.nupd += 1;
+        if (rrt < a.tol2 * e.bb) { e.converged = 1; e.done = 1; }
+        else if (e.iters >= a.max_iter) { e.maxed = 1; e.done = 1; }
+        else e.done = 0;
+    }
+    e.pending = PEND_NONE;
+    if (threadIdx.x == 0) a.log[L & 1] = e;
+}
+
+__device__ __forceinline__ bool pass_through(const PcgArgs& a, const PcgLog& e, int L) {
+    if (!e.done) return false;
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.log[L & 1] = e;
+    return true;
+}
+
+// Ap = A p; partial p.Ap.
 template <typename T>
-__global__ __launch_bounds__(GF_WG) void k_pcg_spmv(PcgArgs a) {
-    __shared__ int s_nbr[6];
-    if (a.st->done) return;
-    const int slot = gf::xcd_remap(blockIdx.x, a.nact);
-    const size_t base = (size_t)slot * GF_TILE_CELLS;
-    load_nbr(a.nbr, slot, s_nbr);
+__global__ __launch_bounds__(GF_WG) void k_pcg_spmv(PcgArgs a, int L) {
+    __shared__ double s_red[GF_WG / GF_WAVE];
+    const PcgLog e = a.log[(L - 1) & 1];
+    if (pass_through(a, e, L)) return;
+    int t0, t1, b;
+    tile_range(a, t0, t1, b);
     const T* __restrict__ p = vp<T>(a);
     T* __restrict__ ap = vap<T>(a);
     auto get = [&](size_t i) -> T { return p[i]; };
-    double v[1] = {0.0};
+    double pap = 0.0;
+    for (int t = t0; t < t1; ++t) {
+        const size_t base = (size_t)t * GF_TILE_CELLS;
+        const int32_t* nb = a.nbr + (size_t)t * 6;
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const int l = cell_l(h);
-        const size_t idx = base + l;
-        const uint8_t f = a.flags[idx];
-        if (f & 8) {
-            const T xc = p[idx];
-            const T s = nb_sum<T>(get, base, l, s_nbr);
-            const T y = (T)(f & 7) * xc - s;
-            ap[idx] = y;
-            v[0] += (double)xc * (double)y;
+        for (int h = 0; h < 2; ++h) {
+            const int l = cell_l(h);
+            const size_t idx = base + l;
+            const uint8_t f = a.flags[idx];
+            if (f & 8) {
+                const T xc = p[idx];
+                const T s = nb_sum<T>(get, base, l, nb);
+                const T y = (T)(f & 7) * xc - s;
+                ap[idx] = y;
+                pap += (double)xc * (double)y;
+            }
         }
     }
-    double tot[1];
-    if (gf::wg_reduce_last<1>(v, a.partials, gridDim.x, &a.st->counter, tot) && threadIdx.x == 0) {
-        PcgState* st = a.st;
-        st->pAp = tot[0];
-        st->alpha = fabs(tot[0]) < 1e-30 ? 0.0 : st->rz / tot[0];
-    }
+    put_partial(a.part, P_PAP, b, pap, s_red);
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.log[L & 1] = e;
 }
 
-// x += alpha p, r -= alpha Ap; r.z, r.r -> beta, stop / replacement flags
+// alpha = rz / pAp; x += alpha p, r -= alpha Ap; partials r.z, r.r.
 template <typename T, bool MIXED>
-__global__ __launch_bounds__(GF_WG) void k_pcg_update(PcgArgs a) {
-    PcgState* st = a.st;
-    if (st->done) return;
-    const int slot = gf::xcd_remap(blockIdx.x, a.nact);
-    const size_t base = (size_t)slot * GF_TILE_CELLS;
-    const double alpha = st->alpha;
+__global__ __launch_bounds__(GF_WG) void k_pcg_update(PcgArgs a, int L) {
+    __shared__ double s_red[GF_WG / GF_WAVE];
+    PcgLog e = a.log[(L - 1) & 1];
+    if (pass_through(a, e, L)) return;
+    const double pap = sum_partials(a.part, P_PAP, a.G, s_red);
+    const double alpha = fabs(pap) < 1e-30 ? 0.0 : e.rz / pap;
+    int t0, t1, b;
+    tile_range(a, t0, t1, b);
     const T af = (T)alpha;
-    const int merge = MIXED ? st->merge : 0;
+    const int merge = MIXED ? e.merge : 0;
     T* __restrict__ r = vr<T>(a);
     const T* __restrict__ p = vp<T>(a);
     const T* __restrict__ ap = vap<T>(a);
-    double v[2] = {0.0, 0.0};
+    double rz = 0.0, rr = 0.0;
+    for (int t = t0; t < t1; ++t) {
+        const size_t base = (size_t)t * GF_TILE_CELLS;
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const size_t idx = base + cell_l(h);
-        const uint8_t f = a.flags[idx];
-        if (f & 8) {
-            const T pv = p[idx];
-            if (MIXED) {
-                float xo = a.x32[idx];
-                if (merge) {
-                    a.x64[idx] += (double)xo;
-                    xo = 0.0f;
+        for (int h = 0; h < 2; ++h) {
+            const size_t idx = base + cell_l(h);
+            const uint8_t f = a.flags[idx];
+            if (f & 8) {
+                const T pv = p[idx];
+                if (MIXED) {
+                    float xo = a.x32[idx];
+                    if (merge) {
+                        a.x64[idx] += (double)xo;
+                        xo = 0.0f;
+                    }
+                    a.x32[idx] = xo + (float)af * (float)pv;
+                } else {
+                    a.x64[idx] += (double)af * (double)pv;
                 }
-                a.x32[idx] = xo + (float)af * (float)pv;
-            } else {
-                a.x64[idx] += (double)af * (double)pv;
+                const T rv = r[idx] - af * ap[idx];
+                r[idx] = rv;
+                const int d = f & 7;
+                const T z = rv / (T)(d > 1 ? d : 1);
+                rz += (double)rv * (double)z;
+                rr += (double)rv * (double)rv;
             }
-            const T rv = r[idx] - af * ap[idx];
-            r[idx] = rv;
-            const int d = f & 7;
-            const T z = rv / (T)(d > 1 ? d : 1);
-            v[0] += (double)rv * (double)z;
-            v[1] += (double)rv * (double)rv;
         }
     }
-    double tot[2];
-    if (gf::wg_reduce_last<2>(v, a.partials, gridDim.x, &st->counter, tot) && threadIdx.x == 0) {
-        const double rz_old = st->rz;
-        st->beta = fabs(rz_old) < 1e-30 ? 0.0 : tot[0] / rz_old;
-        st->rz = tot[0];
-        st->rr = tot[1];
-        st->iters += 1;
-        st->merge = 0;
-        st->k3go = 1;
-        if (tot[1] < st->tol2 * st->bb) {
-            st->done = 1;
-            if (!MIXED) { st->converged = 1; st->rr_true = tot[1]; }
-        } else if (st->iters >= st->max_iter) {
-            st->done = 1;
-            st->maxed = 1;
-            if (!MIXED) st->rr_true = tot[1];
-        } else if (MIXED && tot[1] < st->delta2 * st->rr_upd) {
-            st->upd = 1;
-        }
+    put_partial(a.part, P_RZ, b, rz, s_red);
+    put_partial(a.part, P_RR, b, rr, s_red);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        e.alpha = alpha;
+        e.pap = pap;
+        e.merge = 0;
+        a.log[L & 1] = e;
     }
 }
 
-// p = z + beta p (runs once after each executed update kernel)
-template <typename T>
-__global__ __launch_bounds__(GF_WG) void k_pcg_dir(PcgArgs a) {
-    __shared__ int s_last;
-    PcgState* st = a.st;
-    if (!st->k3go) return;
-    const int slot = gf::xcd_remap(blockIdx.x, a.nact);
-    const size_t base = (size_t)slot * GF_TILE_CELLS;
-    const T bf = (T)st->beta;
+// beta = rz_new / rz; p = z + beta p; convergence / replacement decisions.
+template <typename T, bool MIXED>
+__global__ __launch_bounds__(GF_WG) void k_pcg_dir(PcgArgs a, int L) {
+    __shared__ double s_red[GF_WG / GF_WAVE];
+    PcgLog e = a.log[(L - 1) & 1];
+    if (pass_through(a, e, L)) return;
+    const double rz_new = sum_partials(a.part, P_RZ, a.G, s_red);
+    const double rr = sum_partials(a.part, P_RR, a.G, s_red);
+    const double beta = fabs(e.rz) < 1e-30 ? 0.0 : rz_new / e.rz;
+    int t0, t1, b;
+    tile_range(a, t0, t1, b);
+    const T bf = (T)beta;
     const T* __restrict__ r = vr<T>(a);
     T* __restrict__ p = vp<T>(a);
+    for (int t = t0; t < t1; ++t) {
+        const size_t base = (size_t)t * GF_TILE_CELLS;
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const size_t idx = base + cell_l(h);
-        const uint8_t f = a.flags[idx];
-        if (f & 8) {
-            const int d = f & 7;
-            const T z = r[idx] / (T)(d > 1 ? d : 1);
-            p[idx] = z + bf * p[idx];
+        for (int h = 0; h < 2; ++h) {
+            const size_t idx = base + cell_l(h);
+            const uint8_t f = a.flags[idx];
+            if (f & 8) {
+                const int d = f & 7;
+                const T z = r[idx] / (T)(d > 1 ? d : 1);
+                p[idx] = z + bf * p[idx];
+            }
         }
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        unsigned int ticket = atomicAdd(&st->counter, 1u);
-        s_last = ticket == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (s_last && threadIdx.x == 0) {
-        st->k3go = 0;
-        st->counter = 0u;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        e.beta = beta;
+        e.rz = rz_new;
+        e.rr = rr;
+        e.iters += 1;
+        if (rr < a.tol2 * e.bb) {
+            e.done = 1;
+            if (!MIXED) { e.converged = 1; e.rr_true = rr; }
+        } else if (e.iters >= a.max_iter) {
+            e.done = 1;
+            e.maxed = 1;
+            if (!MIXED) e.rr_true = rr;
+        } else if (MIXED && rr < a.delta2 * e.rr_upd) {
+            e.upd = 1;
+        }
+        a.log[L & 1] = e;
     }
 }
 
-// Residual replacement: r = rhs - A (x64 + x32) in FP64 (mixed mode only).
-__global__ __launch_bounds__(GF_WG) void k_pcg_replace(PcgArgs a) {
-    __shared__ int s_nbr[6];
-    PcgState* st = a.st;
-    if (!(st->upd || (st->done && !st->converged))) return;
-    const int slot = gf::xcd_remap(blockIdx.x, a.nact);
-    const size_t base = (size_t)slot * GF_TILE_CELLS;
-    load_nbr(a.nbr, slot, s_nbr);
+// Residual replacement (mixed mode): r = rhs - A (x64 + x32) in FP64 when a
+// replacement is due or convergence is to be confirmed; partials of the true
+// r.r and of r.z.  Finalised by k_pcg_fin.
+__global__ __launch_bounds__(GF_WG) void k_pcg_replace(PcgArgs a, int L) {
+    __shared__ double s_red[GF_WG / GF_WAVE];
+    PcgLog e = a.log[(L - 1) & 1];
+    const bool run = e.upd || (e.done && !e.converged);
+    if (!run) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) a.log[L & 1] = e;
+        return;
+    }
+    int t0, t1, b;
+    tile_range(a, t0, t1, b);
     const double* __restrict__ x64 = a.x64;
     const float* __restrict__ x32 = a.x32;
     auto get = [&](size_t i) -> double { return x64[i] + (double)x32[i]; };
-    double v[2] = {0.0, 0.0};
+    double rrt = 0.0, rz = 0.0;
+    for (int t = t0; t < t1; ++t) {
+        const size_t base = (size_t)t * GF_TILE_CELLS;
+        const int32_t* nb = a.nbr + (size_t)t * 6;
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const int l = cell_l(h);
-        const size_t idx = base + l;
-        const uint8_t f = a.flags[idx];
-        if (f & 8) {
-            const int d = f & 7;
-            const double ax = (double)d * get(idx) - nb_sum<double>(get, base, l, s_nbr);
-            const double rv = (double)(a.rhs_sign * a.b[idx]) - ax;
-            const float r32 = (float)rv;
-            a.r32[idx] = r32;
-            const float z = r32 / (float)(d > 1 ? d : 1);
-            v[0] += rv * rv;
-            v[1] += (double)r32 * (double)z;
+        for (int h = 0; h < 2; ++h) {
+            const int l = cell_l(h);
+            const size_t idx = base + l;
+            const uint8_t f = a.flags[idx];
+            if (f & 8) {
+                const int d = f & 7;
+                const double ax = (double)d * get(idx) - nb_sum<double>(get, base, l, nb);
+                const double rv = (double)(a.rhs_sign * a.b[idx]) - ax;
+                const float r32 = (float)rv;
+                a.r32[idx] = r32;
+                const float z = r32 / (float)(d > 1 ? d : 1);
+                rrt += rv * rv;
+                rz += (double)r32 * (double)z;
+            }
         }
     }
-    double tot[2];
-    if (gf::wg_reduce_last<2>(v, a.partials, gridDim.x, &st->counter, tot) && threadIdx.x == 0) {
-        st->rr_true = tot[0];
-        st->rr = tot[0];
-        st->rr_upd = tot[0];
-        st->rz = tot[1];
-        st->upd = 0;
-        st->merge = 1;
-        st->nupd += 1;
-        if (tot[0] < st->tol2 * st->bb) {
-            st->converged = 1;
-            st->done = 1;
-        } else if (st->iters >= st->max_iter) {
-            st->maxed = 1;
-            st->done = 1;
-        } else {
-            st->done = 0;
-        }
+    put_partial(a.part, P_RRT, b, rrt, s_red);
+    put_partial(a.part, P_RZT, b, rz, s_red);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        e.pending = PEND_REPLACE;
+        a.log[L & 1] = e;
     }
 }
 
 // pc = FP32 result (x64 [+ x32]); non-fluid cells 0.
 template <bool MIXED>
 __global__ __launch_bounds__(GF_WG) void k_pcg_final(PcgArgs a) {
-    const int slot = gf::xcd_remap(blockIdx.x, a.nact);
-    const size_t base = (size_t)slot * GF_TILE_CELLS;
+    int t0, t1, b;
+    tile_range(a, t0, t1, b);
+    for (int t = t0; t < t1; ++t) {
+        const size_t base = (size_t)t * GF_TILE_CELLS;
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const size_t idx = base + cell_l(h);
-        const uint8_t f = a.flags[idx];
-        float out = 0.0f;
-        if (f & 8) out = MIXED ? (float)(a.x64[idx] + (double)a.x32[idx]) : (float)a.x64[idx];
-        a.pc[idx] = out;
+        for (int h = 0; h < 2; ++h) {
+            const size_t idx = base + cell_l(h);
+            const uint8_t f = a.flags[idx];
+            float out = 0.0f;
+            if (f & 8) out = MIXED ? (float)(a.x64[idx] + (double)a.x32[idx]) : (float)a.x64[idx];
+            a.pc[idx] = out;
+        }
     }
 }
 
-// Jacobi sweep (solver3d.py:878-938) on compact tiles.
+// Jacobi sweep (solver3d.py:878-938) on compact tiles, one workgroup per tile.
 __global__ __launch_bounds__(GF_WG) void k_jacobi(const uint8_t* __restrict__ flags,
                                                   const int32_t* __restrict__ nbr,
                                                   const float* __restrict__ b,
                                                   const float* __restrict__ pin,
                                                   float* __restrict__ pout, int nact) {
-    __shared__ int s_nbr[6];
     const int slot = gf::xcd_remap(blockIdx.x, nact);
     const size_t base = (size_t)slot * GF_TILE_CELLS;
-    load_nbr(nbr, slot, s_nbr);
+    const int32_t* nb = nbr + (size_t)slot * 6;
     auto get = [&](size_t i) -> float { return pin[i]; };
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
@@ -344,7 +404,7 @@ __global__ __launch_bounds__(GF_WG) void k_jacobi(const uint8_t* __restrict__ fl
             const int d = f & 7;
             float out = 0.0f;
             if (d > 0) {
-                const float s = nb_sum<float>(get, base, l, s_nbr);
+                const float s = nb_sum<float>(get, base, l, nb);
                 out = (s - b[idx]) / (float)d;
             }
             pout[idx] = out;
@@ -406,9 +466,11 @@ extern "C" int gf_scene_pcg_sync(const gf_scene_t* s, int n_active, int precisio
     else
         GF_REQUIRE(s->r64 && s->p64 && s->ap64, "gf_scene_pcg_sync: FP64 vectors not set");
     if (check_every < 1) check_every = 1;
+    if (check_every > 64) check_every = 64;
     if (iters) *iters = 0;
     if (rel_residual) *rel_residual = 0.0;
     if (replacements) *replacements = 0;
+    if (kernel_ms) for (int i = 0; i < 4; ++i) kernel_ms[i] = 0.0;
     hipStream_t st = (hipStream_t)stream;
     if (n_active == 0) return 0;
 
@@ -416,30 +478,31 @@ extern "C" int gf_scene_pcg_sync(const gf_scene_t* s, int n_active, int precisio
     a.flags = s->flags; a.nbr = s->nbr; a.b = s->b;
     a.x32 = s->x32; a.r32 = s->r32; a.p32 = s->p32; a.ap32 = s->ap32;
     a.x64 = s->x64; a.r64 = s->r64; a.p64 = s->p64; a.ap64 = s->ap64;
-    a.pc = s->pc; a.partials = s->partials; a.st = (PcgState*)s->scalars;
-    a.nact = n_active; a.rhs_sign = (float)rhs_sign;
-    const dim3 grid(n_active), block(GF_WG);
+    a.pc = s->pc; a.part = s->partials; a.log = (PcgLog*)s->scalars;
+    a.nact = n_active;
+    a.G = n_active < PCG_GMAX ? n_active : PCG_GMAX;
+    a.rhs_sign = (float)rhs_sign;
+    a.tol2 = tol * tol;
+    a.delta2 = 0.01;
+    a.max_iter = max_iter;
+    const dim3 grid(a.G), block(GF_WG);
     const bool mixed = precision == GF_PCG_MIXED;
-    const double tol2 = tol * tol, delta2 = 0.01;
 
-    // The reduction counter must start at zero; clear it once per solve.
-    hipError_t e = hipMemsetAsync(&a.st->counter, 0, sizeof(unsigned int), st);
-    if (e != hipSuccess)
-        return gf::set_error((int)e, "gf_scene_pcg_sync: counter reset: %s", hipGetErrorString(e));
     if (mixed)
-        hipLaunchKernelGGL((k_pcg_setup<float, true>), grid, block, 0, st, a, tol2, delta2, max_iter);
+        hipLaunchKernelGGL((k_pcg_setup<float, true>), grid, block, 0, st, a);
     else
-        hipLaunchKernelGGL((k_pcg_setup<double, false>), grid, block, 0, st, a, tol2, delta2, max_iter);
-    GF_LAUNCH_CHECK("k_pcg_setup");
+        hipLaunchKernelGGL((k_pcg_setup<double, false>), grid, block, 0, st, a);
+    int L = 1;
+    hipLaunchKernelGGL(k_pcg_fin, dim3(1), block, 0, st, a, L);
+    GF_LAUNCH_CHECK("k_pcg_setup/fin");
 
     // Optional per-kernel timing of the iterations that did work: events
     // around every launch of a chunk, read back at the chunk's host sync.
     const bool timed = kernel_ms != nullptr;
     hipEvent_t ev[4 * 64 + 2];
     int nev = 0;
+    hipError_t e;
     if (timed) {
-        if (check_every > 64) check_every = 64;
-        for (int i = 0; i < 4; ++i) kernel_ms[i] = 0.0;
         nev = 4 * check_every + 2;
         for (int i = 0; i < nev; ++i) {
             e = hipEventCreate(&ev[i]);
@@ -449,33 +512,41 @@ extern "C" int gf_scene_pcg_sync(const gf_scene_t* s, int n_active, int precisio
             }
         }
     }
-    PcgState hs;
+    PcgLog hs;
     memset(&hs, 0, sizeof(hs));
-    int launched = 0, iters_before = 0, nupd_before = 0;
-    rc = 0;
-    for (;;) {
+    e = hipMemcpyAsync(&hs, &a.log[L & 1], sizeof(PcgLog), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) rc = gf::set_error((int)e, "gf_scene_pcg_sync: readback: %s", hipGetErrorString(e));
+    int iters_before = 0, nupd_before = 0, launched = 0;
+    while (!rc && !hs.done) {
         const int chunk = check_every;
         for (int j = 0; j < chunk; ++j) {
             if (timed) hipEventRecord(ev[4 * j], st);
-            if (mixed) hipLaunchKernelGGL(k_pcg_spmv<float>, grid, block, 0, st, a);
-            else hipLaunchKernelGGL(k_pcg_spmv<double>, grid, block, 0, st, a);
+            ++L;
+            if (mixed) hipLaunchKernelGGL(k_pcg_spmv<float>, grid, block, 0, st, a, L);
+            else hipLaunchKernelGGL(k_pcg_spmv<double>, grid, block, 0, st, a, L);
             if (timed) hipEventRecord(ev[4 * j + 1], st);
-            if (mixed) hipLaunchKernelGGL((k_pcg_update<float, true>), grid, block, 0, st, a);
-            else hipLaunchKernelGGL((k_pcg_update<double, false>), grid, block, 0, st, a);
+            ++L;
+            if (mixed) hipLaunchKernelGGL((k_pcg_update<float, true>), grid, block, 0, st, a, L);
+            else hipLaunchKernelGGL((k_pcg_update<double, false>), grid, block, 0, st, a, L);
             if (timed) hipEventRecord(ev[4 * j + 2], st);
-            if (mixed) hipLaunchKernelGGL(k_pcg_dir<float>, grid, block, 0, st, a);
-            else hipLaunchKernelGGL(k_pcg_dir<double>, grid, block, 0, st, a);
+            ++L;
+            if (mixed) hipLaunchKernelGGL((k_pcg_dir<float, true>), grid, block, 0, st, a, L);
+            else hipLaunchKernelGGL((k_pcg_dir<double, false>), grid, block, 0, st, a, L);
             if (timed) hipEventRecord(ev[4 * j + 3], st);
         }
         launched += chunk;
         if (mixed) {
             if (timed) hipEventRecord(ev[4 * chunk], st);
-            hipLaunchKernelGGL(k_pcg_replace, grid, block, 0, st, a);
+            ++L;
+            hipLaunchKernelGGL(k_pcg_replace, grid, block, 0, st, a, L);
+            ++L;
+            hipLaunchKernelGGL(k_pcg_fin, dim3(1), block, 0, st, a, L);
             if (timed) hipEventRecord(ev[4 * chunk + 1], st);
         }
         rc = gf::check_launch("pcg iteration");
         if (rc) break;
-        e = hipMemcpyAsync(&hs, a.st, sizeof(PcgState), hipMemcpyDeviceToHost, st);
+        e = hipMemcpyAsync(&hs, &a.log[L & 1], sizeof(PcgLog), hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
         if (e != hipSuccess) {
             rc = gf::set_error((int)e, "gf_scene_pcg_sync: readback: %s", hipGetErrorString(e));
@@ -500,7 +571,6 @@ extern "C" int gf_scene_pcg_sync(const gf_scene_t* s, int n_active, int precisio
         }
         iters_before = hs.iters;
         nupd_before = hs.nupd;
-        if (hs.done) break;
         if (hs.iters >= max_iter || launched > max_iter + 2 * chunk) break;
     }
     for (int i = 0; i < nev; ++i) hipEventDestroy(ev[i]);

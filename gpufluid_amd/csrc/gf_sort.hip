@@ -19,6 +19,27 @@ __global__ void k_fill_i32(int32_t* a, int n, int32_t v) {
     if (i < n) a[i] = v;
 }
 
+// Runs of equal keys among the active lanes of a wave (particles arrive
+// nearly cell-sorted, so a cell's members are usually neighbours): the first
+// lane of a run gets len = run length (others 0) and every lane gets `lead`,
+// the lane index of its run's first lane.  Lanes past the end of the array
+// must not call this (the whole wave does at the array tail only when n is a
+// multiple of 64; otherwise the guard below keeps exited lanes out).
+__device__ __forceinline__ void run_of(int key, int& lead, int& len) {
+    const int lane = threadIdx.x & 63;
+    const int prev = __shfl_up(key, 1, 64);
+    const bool first = lane == 0 || prev != key;
+    const unsigned long long leaders = __ballot(first);
+    const unsigned long long active = __ballot(true);
+    // highest leader at or below this lane
+    const unsigned long long below = leaders & ((lane == 63) ? ~0ull : ((2ull << lane) - 1));
+    lead = 63 - __clzll(below);
+    // next leader above this lane (or one past the last active lane)
+    const unsigned long long above = leaders & ~((lane == 63) ? ~0ull : ((2ull << lane) - 1));
+    const int end = above ? __ffsll(above) - 1 : 64 - __clzll(active);
+    len = first ? end - lane : 0;
+}
+
 __global__ __launch_bounds__(GF_WG) void k_bin(int n, const float* __restrict__ pos,
                                                float dx, int nx, int ny, int nz,
                                                const int32_t* __restrict__ marker,
@@ -34,8 +55,12 @@ __global__ __launch_bounds__(GF_WG) void k_bin(int n, const float* __restrict__ 
     int ck = gf::bin_axis(z, dx, nz);
     int key = (ci * ny + cj) * nz + ck;
     keys[p] = key;
-    atomicAdd(&counts[key], 1);
-    if (marker[key] != 2) tile_occ[((ci >> 3) * tny + (cj >> 3)) * tnz + (ck >> 3)] = 1;
+    int lead, len;
+    run_of(key, lead, len);
+    if (len > 0) {
+        atomicAdd(&counts[key], len);
+        if (marker[key] != 2) tile_occ[((ci >> 3) * tny + (cj >> 3)) * tnz + (ck >> 3)] = 1;
+    }
 }
 
 __global__ __launch_bounds__(GF_WG) void k_scan_reduce(const int32_t* __restrict__ in, int n,
@@ -110,8 +135,13 @@ __global__ __launch_bounds__(GF_WG) void k_slot(int n, const int32_t* __restrict
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     int key = keys[p];
-    int old = atomicSub(&counts[key], 1);
-    tmp[cell_start[key] + old - 1] = p;
+    int lead, len;
+    run_of(key, lead, len);
+    int old = 0;
+    if (len > 0) old = atomicSub(&counts[key], len);
+    old = __shfl(old, lead, 64);
+    const int lane = threadIdx.x & 63;
+    tmp[cell_start[key] + old - 1 - (lane - lead)] = p;
 }
 
 // Rank each particle among the members of its cell by original index (a stable

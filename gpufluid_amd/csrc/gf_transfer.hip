@@ -132,8 +132,6 @@ __global__ __launch_bounds__(GF_WG) void k_p2g_tile(P2GParams a) {
     __shared__ float s_val[3][P2G_REGION];
     __shared__ float s_wt[3][P2G_REGION];
     __shared__ int8_t s_mk[10][10][10];
-    __shared__ int s_start[128], s_pref[128];
-    __shared__ int s_wave[GF_WG / GF_WAVE + 1];
 
     const int ntiles = a.tnx * a.tny * a.tnz;
     const int t = gf::xcd_remap(blockIdx.x, ntiles);
@@ -157,27 +155,33 @@ __global__ __launch_bounds__(GF_WG) void k_p2g_tile(P2GParams a) {
     }
     __syncthreads();
 
-    const RowSet rs = build_rows(a.cell_start, ny, nz, max(i0 - 1, 0), min(i0 + 9, nx),
-                                 max(j0 - 1, 0), min(j0 + 9, ny), max(k0 - 1, 0),
-                                 min(k0 + 9, nz), s_start, s_pref, s_wave);
+    // One lane per cell of the 1-cell-haloed tile: particles of one cell (which
+    // share their scatter targets) go through the same lane, so the 64 lanes of
+    // an LDS atomic hit distinct faces except where neighbouring cells meet.
+    const int ilo = max(i0 - 1, 0), jlo = max(j0 - 1, 0), klo = max(k0 - 1, 0);
+    const int ni = min(i0 + 9, nx) - ilo, nj = min(j0 + 9, ny) - jlo, nk = min(k0 + 9, nz) - klo;
+    const int ncell = ni * nj * nk;
     const float dx = a.dx;
-    for (int q = threadIdx.x; q < rs.total; q += GF_WG) {
-        const int r = row_find(s_pref, rs.nrows, q);
-        const int pidx = s_start[r] + (q - s_pref[r]);
-        const float px = a.pos[3 * pidx + 0], py = a.pos[3 * pidx + 1], pz = a.pos[3 * pidx + 2];
-        const float vx = a.vel[3 * pidx + 0], vy = a.vel[3 * pidx + 1], vz = a.vel[3 * pidx + 2];
-        float cm[9];
-        if (APIC) {
+    for (int c = threadIdx.x; c < ncell; c += GF_WG) {
+        const int ck = klo + c % nk, cj = jlo + (c / nk) % nj, ci = ilo + c / (nk * nj);
+        const int key = (ci * ny + cj) * nz + ck;
+        const int pe = a.cell_start[key + 1];
+        for (int pidx = a.cell_start[key]; pidx < pe; ++pidx) {
+            const float px = a.pos[3 * pidx + 0], py = a.pos[3 * pidx + 1], pz = a.pos[3 * pidx + 2];
+            const float vx = a.vel[3 * pidx + 0], vy = a.vel[3 * pidx + 1], vz = a.vel[3 * pidx + 2];
+            float cm[9];
+            if (APIC) {
 #pragma unroll
-            for (int c = 0; c < 9; ++c) cm[c] = a.C[9 * pidx + c];
+                for (int q = 0; q < 9; ++q) cm[q] = a.C[9 * pidx + q];
+            }
+            const float qx = px / dx, qy = py / dx, qz = pz / dx;
+            scatter_comp<0, APIC>(s_val[0], s_wt[0], px, py, pz, qx, qy, qz, vx, cm + 0, i0, j0,
+                                  k0, nx + 1, ny, nz, dx);
+            scatter_comp<1, APIC>(s_val[1], s_wt[1], px, py, pz, qx, qy, qz, vy, cm + 3, i0, j0,
+                                  k0, nx, ny + 1, nz, dx);
+            scatter_comp<2, APIC>(s_val[2], s_wt[2], px, py, pz, qx, qy, qz, vz, cm + 6, i0, j0,
+                                  k0, nx, ny, nz + 1, dx);
         }
-        const float qx = px / dx, qy = py / dx, qz = pz / dx;
-        scatter_comp<0, APIC>(s_val[0], s_wt[0], px, py, pz, qx, qy, qz, vx, cm + 0, i0, j0,
-                              k0, nx + 1, ny, nz, dx);
-        scatter_comp<1, APIC>(s_val[1], s_wt[1], px, py, pz, qx, qy, qz, vy, cm + 3, i0, j0,
-                              k0, nx, ny + 1, nz, dx);
-        scatter_comp<2, APIC>(s_val[2], s_wt[2], px, py, pz, qx, qy, qz, vz, cm + 6, i0, j0,
-                              k0, nx, ny, nz + 1, dx);
     }
     __syncthreads();
 

@@ -100,8 +100,8 @@ typedef struct gf_scene {
     float* x32; float* r32; float* p32; float* ap32;  /* FP32 PCG vectors */
     double* x64; double* r64; double* p64; double* ap64; /* FP64 vectors */
     float* pc;  float* ptmp;     /* pressure result / Jacobi ping-pong */
-    double* partials;            /* (2*ntiles + 64) per-workgroup partial sums */
-    double* scalars;             /* (64) solver scalars, see gf_scene_pcg_scalars */
+    double* partials;            /* (gf_scene_sizes.partials) per-workgroup partial sums */
+    double* scalars;             /* (gf_scene_sizes.scalars) PCG solver log */
 } gf_scene_t;
 
 typedef struct gf_scene_size_info {
