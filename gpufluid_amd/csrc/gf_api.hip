@@ -58,7 +58,7 @@ extern "C" int gf_scene_sizes(int nx, int ny, int nz, gf_scene_size_info* out) {
     out->ntiles = ntiles;
     out->compact = ntiles * GF_TILE_CELLS;
     out->scan_parts = (cells + 1 + 4095) / 4096 + 1;
-    out->partials = 8 * 1024;  // per-workgroup partial sums of the PCG reductions
+    out->partials = 8 * 2048;  // per-workgroup partial sums of the PCG reductions
     out->scalars = 64;         // two 16-double solver log entries, plus spare
     return 0;
 }

@@ -36,7 +36,7 @@ static_assert(sizeof(PcgLog) == 16 * sizeof(double), "log entry is 16 doubles");
 
 // Per-workgroup partial sums, PCG_GMAX entries per kind.
 enum { P_PAP = 0, P_RZ, P_RR, P_BB, P_RRT, P_RZT, P_KINDS };
-constexpr int PCG_GMAX = 1024;
+constexpr int PCG_GMAX = 2048;
 
 struct PcgArgs {
     const uint8_t* flags;
@@ -54,9 +54,96 @@ struct PcgArgs {
     int max_iter;
 };
 
-__device__ __forceinline__ int cell_l(int h) {
-    const int li = (threadIdx.x >> 6) + 4 * h, lj = (threadIdx.x >> 3) & 7, lk = threadIdx.x & 7;
-    return (li * 8 + lj) * 8 + lk;
+// ---- quad addressing ----------------------------------------------------
+// A workgroup processes two tiles per pass; thread -> (tile half, quad q),
+// quad q covers cells (li, lj, 4*kh .. 4*kh+3) of its tile, i.e. one aligned
+// 16-byte (FP32) / 32-byte (FP64) vector.
+struct Quad {
+    int li, lj, kh, l0;
+};
+
+__device__ __forceinline__ Quad quad_of() {
+    const int q = threadIdx.x & 127;
+    Quad Q;
+    Q.li = q >> 4;
+    Q.lj = (q >> 1) & 7;
+    Q.kh = q & 1;
+    Q.l0 = (Q.li * 8 + Q.lj) * 8 + Q.kh * 4;
+    return Q;
+}
+
+template <typename T> struct V4;
+template <> struct V4<float> { using t = float4; };
+template <> struct V4<double> { using t = double4; };
+
+template <typename T>
+__device__ __forceinline__ void ld4(const T* p, T (&o)[4]) {
+    const typename V4<T>::t v = *reinterpret_cast<const typename V4<T>::t*>(p);
+    o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+}
+
+template <typename T>
+__device__ __forceinline__ void st4(T* p, const T (&o)[4]) {
+    typename V4<T>::t v;
+    v.x = o[0]; v.y = o[1]; v.z = o[2]; v.w = o[3];
+    *reinterpret_cast<typename V4<T>::t*>(p) = v;
+}
+
+__device__ __forceinline__ uint32_t ld_flags(const uint8_t* f, size_t idx) {
+    return *reinterpret_cast<const uint32_t*>(f + idx);
+}
+__device__ __forceinline__ int fl_fluid(uint32_t fw, int e) { return (fw >> (8 * e + 3)) & 1; }
+__device__ __forceinline__ int fl_diag(uint32_t fw, int e) { return (fw >> (8 * e)) & 7; }
+
+// Neighbour values of a quad: the four -x/+x/-y/+y vectors and the two z
+// values that lie outside the quad (-z of its first cell, +z of its last).
+// Neighbours in inactive tiles read as 0; every load uses a valid address.
+template <typename T>
+struct Nb4 {
+    T xm[4], xp[4], ym[4], yp[4];
+    T zm0, zp3;
+};
+
+template <typename T>
+__device__ __forceinline__ void load_nb4(const T* __restrict__ v, size_t base, const Quad& Q,
+                                         const int32_t* nb, Nb4<T>& o) {
+    const size_t own = base + Q.l0;
+    bool ok;
+    size_t a;
+    ok = Q.li > 0 || nb[0] >= 0;
+    a = Q.li > 0 ? own - 64 : (nb[0] >= 0 ? (size_t)nb[0] * 512 + Q.l0 + 448 : own);
+    ld4(v + a, o.xm);
+    if (!ok) for (int e = 0; e < 4; ++e) o.xm[e] = (T)0;
+    ok = Q.li < 7 || nb[1] >= 0;
+    a = Q.li < 7 ? own + 64 : (nb[1] >= 0 ? (size_t)nb[1] * 512 + Q.l0 - 448 : own);
+    ld4(v + a, o.xp);
+    if (!ok) for (int e = 0; e < 4; ++e) o.xp[e] = (T)0;
+    ok = Q.lj > 0 || nb[2] >= 0;
+    a = Q.lj > 0 ? own - 8 : (nb[2] >= 0 ? (size_t)nb[2] * 512 + Q.l0 + 56 : own);
+    ld4(v + a, o.ym);
+    if (!ok) for (int e = 0; e < 4; ++e) o.ym[e] = (T)0;
+    ok = Q.lj < 7 || nb[3] >= 0;
+    a = Q.lj < 7 ? own + 8 : (nb[3] >= 0 ? (size_t)nb[3] * 512 + Q.l0 - 56 : own);
+    ld4(v + a, o.yp);
+    if (!ok) for (int e = 0; e < 4; ++e) o.yp[e] = (T)0;
+    ok = Q.kh == 1 || nb[4] >= 0;
+    a = Q.kh == 1 ? own - 1 : (nb[4] >= 0 ? (size_t)nb[4] * 512 + Q.l0 + 7 : own);
+    o.zm0 = ok ? v[a] : (T)0;
+    ok = Q.kh == 0 || nb[5] >= 0;
+    a = Q.kh == 0 ? own + 4 : (nb[5] >= 0 ? (size_t)nb[5] * 512 + Q.l0 - 4 : own);
+    o.zp3 = ok ? v[a] : (T)0;
+}
+
+// Neighbour sum of quad element e in the reference order -x,+x,-y,+y,-z,+z.
+template <typename T>
+__device__ __forceinline__ T nb_sum4(const Nb4<T>& o, const T (&c)[4], int e) {
+    T s = o.xm[e];
+    s += o.xp[e];
+    s += o.ym[e];
+    s += o.yp[e];
+    s += e == 0 ? o.zm0 : c[e - 1];
+    s += e == 3 ? o.zp3 : c[e + 1];
+    return s;
 }
 
 // Tile range of this workgroup; consecutive logical workgroups sit on the same
@@ -65,21 +152,6 @@ __device__ __forceinline__ void tile_range(const PcgArgs& a, int& t0, int& t1, i
     b = gf::xcd_remap(blockIdx.x, a.G);
     t0 = (int)(((long long)b * a.nact) / a.G);
     t1 = (int)(((long long)(b + 1) * a.nact) / a.G);
-}
-
-// Sum of the six neighbour values in the order -x,+x,-y,+y,-z,+z of the
-// reference stencil; neighbours in inactive tiles read as 0.
-template <typename T, typename F>
-__device__ __forceinline__ T nb_sum(F get, size_t base, int l, const int32_t* nb) {
-    const int li = l >> 6, lj = (l >> 3) & 7, lk = l & 7;
-    T s = (T)0;
-    s += li > 0 ? get(base + l - 64) : (nb[0] >= 0 ? get((size_t)nb[0] * 512 + l + 448) : (T)0);
-    s += li < 7 ? get(base + l + 64) : (nb[1] >= 0 ? get((size_t)nb[1] * 512 + l - 448) : (T)0);
-    s += lj > 0 ? get(base + l - 8) : (nb[2] >= 0 ? get((size_t)nb[2] * 512 + l + 56) : (T)0);
-    s += lj < 7 ? get(base + l + 8) : (nb[3] >= 0 ? get((size_t)nb[3] * 512 + l - 56) : (T)0);
-    s += lk > 0 ? get(base + l - 1) : (nb[4] >= 0 ? get((size_t)nb[4] * 512 + l + 7) : (T)0);
-    s += lk < 7 ? get(base + l + 1) : (nb[5] >= 0 ? get((size_t)nb[5] * 512 + l - 7) : (T)0);
-    return s;
 }
 
 // Workgroup sum of one value per thread, in a fixed order; result in all threads.
@@ -121,34 +193,42 @@ template <> __device__ __forceinline__ double* vap<double>(const PcgArgs& a) { r
 
 enum { PEND_NONE = 0, PEND_SETUP = 1, PEND_REPLACE = 2 };
 
+#define FOR_QUADS(a)                                                        \
+    int t0_, t1_, b;                                                        \
+    tile_range(a, t0_, t1_, b);                                             \
+    const Quad Q = quad_of();                                               \
+    for (int tb_ = t0_; tb_ < t1_; tb_ += 2)                                \
+        if (const int t = tb_ + (threadIdx.x >> 7); t < t1_)
+
 // L = 0: x = 0, r = rhs_sign*b, p = z = r/M; partials r.z and r.r.
 template <typename T, bool MIXED>
 __global__ __launch_bounds__(GF_WG) void k_pcg_setup(PcgArgs a) {
     __shared__ double s_red[GF_WG / GF_WAVE];
-    int t0, t1, b;
-    tile_range(a, t0, t1, b);
     T* r = vr<T>(a);
     T* p = vp<T>(a);
     double rz = 0.0, rr = 0.0;
-    for (int t = t0; t < t1; ++t) {
-        const size_t base = (size_t)t * GF_TILE_CELLS;
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const size_t idx = base + cell_l(h);
-            const uint8_t f = a.flags[idx];
-            T rv = (T)0, z = (T)0;
-            if (f & 8) {
-                rv = (T)(a.rhs_sign * a.b[idx]);
-                const int d = f & 7;
-                z = rv / (T)(d > 1 ? d : 1);
-            }
-            r[idx] = rv;
-            p[idx] = z;
-            a.x64[idx] = 0.0;
-            if (MIXED) a.x32[idx] = 0.0f;
-            rz += (double)rv * (double)z;
-            rr += (double)rv * (double)rv;
+    FOR_QUADS(a) {
+        const size_t idx = (size_t)t * GF_TILE_CELLS + Q.l0;
+        const uint32_t fw = ld_flags(a.flags, idx);
+        float bv[4];
+        ld4(a.b + idx, bv);
+        T rv[4], z[4], zero[4] = {(T)0, (T)0, (T)0, (T)0};
+        double zd[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int e = 0; e < 4; ++e) {
+            rv[e] = fl_fluid(fw, e) ? (T)(a.rhs_sign * bv[e]) : (T)0;
+            const int d = fl_diag(fw, e);
+            z[e] = rv[e] / (T)(d > 1 ? d : 1);
+            rz += (double)rv[e] * (double)z[e];
+            rr += (double)rv[e] * (double)rv[e];
         }
+        st4(r + idx, rv);
+        st4(p + idx, z);
+        st4(a.x64 + idx, zd);
+        if (MIXED) {
+            float zf[4] = {0.f, 0.f, 0.f, 0.f};
+            st4(a.x32 + idx, zf);
+        }
+        (void)zero;
     }
     put_partial(a.part, P_RZ, b, rz, s_red);
     put_partial(a.part, P_BB, b, rr, s_red);
@@ -196,81 +276,84 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_spmv(PcgArgs a, int L) {
     __shared__ double s_red[GF_WG / GF_WAVE];
     const PcgLog e = a.log[(L - 1) & 1];
     if (pass_through(a, e, L)) return;
-    int t0, t1, b;
-    tile_range(a, t0, t1, b);
     const T* __restrict__ p = vp<T>(a);
     T* __restrict__ ap = vap<T>(a);
-    auto get = [&](size_t i) -> T { return p[i]; };
     double pap = 0.0;
-    for (int t = t0; t < t1; ++t) {
+    FOR_QUADS(a) {
         const size_t base = (size_t)t * GF_TILE_CELLS;
-        const int32_t* nb = a.nbr + (size_t)t * 6;
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int l = cell_l(h);
-            const size_t idx = base + l;
-            const uint8_t f = a.flags[idx];
-            if (f & 8) {
-                const T xc = p[idx];
-                const T s = nb_sum<T>(get, base, l, nb);
-                const T y = (T)(f & 7) * xc - s;
-                ap[idx] = y;
-                pap += (double)xc * (double)y;
-            }
+        const size_t idx = base + Q.l0;
+        const uint32_t fw = ld_flags(a.flags, idx);
+        T c[4], y[4];
+        ld4(p + idx, c);
+        Nb4<T> o;
+        load_nb4(p, base, Q, a.nbr + (size_t)t * 6, o);
+        for (int k = 0; k < 4; ++k) {
+            const T s = nb_sum4(o, c, k);
+            y[k] = fl_fluid(fw, k) ? (T)fl_diag(fw, k) * c[k] - s : (T)0;
+            pap += (double)c[k] * (double)y[k];
         }
+        st4(ap + idx, y);
     }
     put_partial(a.part, P_PAP, b, pap, s_red);
     if (blockIdx.x == 0 && threadIdx.x == 0) a.log[L & 1] = e;
 }
 
 // alpha = rz / pAp; x += alpha p, r -= alpha Ap; partials r.z, r.r.
+// Non-fluid entries of p, Ap, r and x are 0 and stay 0.
 template <typename T, bool MIXED>
 __global__ __launch_bounds__(GF_WG) void k_pcg_update(PcgArgs a, int L) {
     __shared__ double s_red[GF_WG / GF_WAVE];
     PcgLog e = a.log[(L - 1) & 1];
     if (pass_through(a, e, L)) return;
-    const double pap = sum_partials(a.part, P_PAP, a.G, s_red);
-    const double alpha = fabs(pap) < 1e-30 ? 0.0 : e.rz / pap;
-    int t0, t1, b;
-    tile_range(a, t0, t1, b);
+    const double pap_t = sum_partials(a.part, P_PAP, a.G, s_red);
+    const double alpha = fabs(pap_t) < 1e-30 ? 0.0 : e.rz / pap_t;
     const T af = (T)alpha;
     const int merge = MIXED ? e.merge : 0;
     T* __restrict__ r = vr<T>(a);
     const T* __restrict__ p = vp<T>(a);
     const T* __restrict__ ap = vap<T>(a);
     double rz = 0.0, rr = 0.0;
-    for (int t = t0; t < t1; ++t) {
-        const size_t base = (size_t)t * GF_TILE_CELLS;
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const size_t idx = base + cell_l(h);
-            const uint8_t f = a.flags[idx];
-            if (f & 8) {
-                const T pv = p[idx];
-                if (MIXED) {
-                    float xo = a.x32[idx];
-                    if (merge) {
-                        a.x64[idx] += (double)xo;
-                        xo = 0.0f;
-                    }
-                    a.x32[idx] = xo + (float)af * (float)pv;
-                } else {
-                    a.x64[idx] += (double)af * (double)pv;
+    FOR_QUADS(a) {
+        const size_t idx = (size_t)t * GF_TILE_CELLS + Q.l0;
+        const uint32_t fw = ld_flags(a.flags, idx);
+        T pv[4], apv[4], rv[4];
+        ld4(p + idx, pv);
+        ld4(ap + idx, apv);
+        ld4(r + idx, rv);
+        if (MIXED) {
+            float xo[4];
+            ld4(a.x32 + idx, xo);
+            if (merge) {
+                double xd[4];
+                ld4(a.x64 + idx, xd);
+                for (int k = 0; k < 4; ++k) {
+                    xd[k] += (double)xo[k];
+                    xo[k] = 0.0f;
                 }
-                const T rv = r[idx] - af * ap[idx];
-                r[idx] = rv;
-                const int d = f & 7;
-                const T z = rv / (T)(d > 1 ? d : 1);
-                rz += (double)rv * (double)z;
-                rr += (double)rv * (double)rv;
+                st4(a.x64 + idx, xd);
             }
+            for (int k = 0; k < 4; ++k) xo[k] = xo[k] + (float)af * (float)pv[k];
+            st4(a.x32 + idx, xo);
+        } else {
+            double xd[4];
+            ld4(a.x64 + idx, xd);
+            for (int k = 0; k < 4; ++k) xd[k] += (double)af * (double)pv[k];
+            st4(a.x64 + idx, xd);
         }
+        for (int k = 0; k < 4; ++k) {
+            rv[k] = rv[k] - af * apv[k];
+            const int d = fl_diag(fw, k);
+            const T z = rv[k] / (T)(d > 1 ? d : 1);
+            rz += (double)rv[k] * (double)z;
+            rr += (double)rv[k] * (double)rv[k];
+        }
+        st4(r + idx, rv);
     }
     put_partial(a.part, P_RZ, b, rz, s_red);
     put_partial(a.part, P_RR, b, rr, s_red);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         e.alpha = alpha;
-        e.pap = pap;
+        e.pap = pap_t;
         e.merge = 0;
         a.log[L & 1] = e;
     }
@@ -285,24 +368,23 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_dir(PcgArgs a, int L) {
     const double rz_new = sum_partials(a.part, P_RZ, a.G, s_red);
     const double rr = sum_partials(a.part, P_RR, a.G, s_red);
     const double beta = fabs(e.rz) < 1e-30 ? 0.0 : rz_new / e.rz;
-    int t0, t1, b;
-    tile_range(a, t0, t1, b);
     const T bf = (T)beta;
     const T* __restrict__ r = vr<T>(a);
     T* __restrict__ p = vp<T>(a);
-    for (int t = t0; t < t1; ++t) {
-        const size_t base = (size_t)t * GF_TILE_CELLS;
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const size_t idx = base + cell_l(h);
-            const uint8_t f = a.flags[idx];
-            if (f & 8) {
-                const int d = f & 7;
-                const T z = r[idx] / (T)(d > 1 ? d : 1);
-                p[idx] = z + bf * p[idx];
-            }
+    FOR_QUADS(a) {
+        const size_t idx = (size_t)t * GF_TILE_CELLS + Q.l0;
+        const uint32_t fw = ld_flags(a.flags, idx);
+        T rv[4], pv[4];
+        ld4(r + idx, rv);
+        ld4(p + idx, pv);
+        for (int k = 0; k < 4; ++k) {
+            const int d = fl_diag(fw, k);
+            const T z = rv[k] / (T)(d > 1 ? d : 1);
+            pv[k] = z + bf * pv[k];
         }
+        st4(p + idx, pv);
     }
+    (void)b;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         e.beta = beta;
         e.rz = rz_new;
@@ -333,31 +415,46 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_replace(PcgArgs a, int L) {
         if (blockIdx.x == 0 && threadIdx.x == 0) a.log[L & 1] = e;
         return;
     }
-    int t0, t1, b;
-    tile_range(a, t0, t1, b);
-    const double* __restrict__ x64 = a.x64;
-    const float* __restrict__ x32 = a.x32;
-    auto get = [&](size_t i) -> double { return x64[i] + (double)x32[i]; };
     double rrt = 0.0, rz = 0.0;
-    for (int t = t0; t < t1; ++t) {
+    FOR_QUADS(a) {
         const size_t base = (size_t)t * GF_TILE_CELLS;
+        const size_t idx = base + Q.l0;
+        const uint32_t fw = ld_flags(a.flags, idx);
         const int32_t* nb = a.nbr + (size_t)t * 6;
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int l = cell_l(h);
-            const size_t idx = base + l;
-            const uint8_t f = a.flags[idx];
-            if (f & 8) {
-                const int d = f & 7;
-                const double ax = (double)d * get(idx) - nb_sum<double>(get, base, l, nb);
-                const double rv = (double)(a.rhs_sign * a.b[idx]) - ax;
-                const float r32 = (float)rv;
-                a.r32[idx] = r32;
-                const float z = r32 / (float)(d > 1 ? d : 1);
-                rrt += rv * rv;
-                rz += (double)r32 * (double)z;
-            }
+        double cd[4];
+        float cf[4], bv[4];
+        ld4(a.x64 + idx, cd);
+        ld4(a.x32 + idx, cf);
+        ld4(a.b + idx, bv);
+        Nb4<double> od;
+        Nb4<float> of;
+        load_nb4(a.x64, base, Q, nb, od);
+        load_nb4(a.x32, base, Q, nb, of);
+        double c[4];
+        for (int k = 0; k < 4; ++k) {
+            c[k] = cd[k] + (double)cf[k];
+            od.xm[k] += (double)of.xm[k];
+            od.xp[k] += (double)of.xp[k];
+            od.ym[k] += (double)of.ym[k];
+            od.yp[k] += (double)of.yp[k];
         }
+        od.zm0 += (double)of.zm0;
+        od.zp3 += (double)of.zp3;
+        float r32[4];
+        for (int k = 0; k < 4; ++k) {
+            float rk = 0.0f;
+            if (fl_fluid(fw, k)) {
+                const int d = fl_diag(fw, k);
+                const double ax = (double)d * c[k] - nb_sum4(od, c, k);
+                const double rv = (double)(a.rhs_sign * bv[k]) - ax;
+                rk = (float)rv;
+                const float z = rk / (float)(d > 1 ? d : 1);
+                rrt += rv * rv;
+                rz += (double)rk * (double)z;
+            }
+            r32[k] = rk;
+        }
+        st4(a.r32 + idx, r32);
     }
     put_partial(a.part, P_RRT, b, rrt, s_red);
     put_partial(a.part, P_RZT, b, rz, s_red);
@@ -367,49 +464,53 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_replace(PcgArgs a, int L) {
     }
 }
 
-// pc = FP32 result (x64 [+ x32]); non-fluid cells 0.
+// pc = FP32 result (x64 [+ x32]); non-fluid cells hold 0 in both.
 template <bool MIXED>
 __global__ __launch_bounds__(GF_WG) void k_pcg_final(PcgArgs a) {
-    int t0, t1, b;
-    tile_range(a, t0, t1, b);
-    for (int t = t0; t < t1; ++t) {
-        const size_t base = (size_t)t * GF_TILE_CELLS;
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const size_t idx = base + cell_l(h);
-            const uint8_t f = a.flags[idx];
-            float out = 0.0f;
-            if (f & 8) out = MIXED ? (float)(a.x64[idx] + (double)a.x32[idx]) : (float)a.x64[idx];
-            a.pc[idx] = out;
+    FOR_QUADS(a) {
+        const size_t idx = (size_t)t * GF_TILE_CELLS + Q.l0;
+        double xd[4];
+        ld4(a.x64 + idx, xd);
+        float out[4];
+        if (MIXED) {
+            float xf[4];
+            ld4(a.x32 + idx, xf);
+            for (int k = 0; k < 4; ++k) out[k] = (float)(xd[k] + (double)xf[k]);
+        } else {
+            for (int k = 0; k < 4; ++k) out[k] = (float)xd[k];
         }
+        st4(a.pc + idx, out);
     }
+    (void)b;
 }
 
-// Jacobi sweep (solver3d.py:878-938) on compact tiles, one workgroup per tile.
+// Jacobi sweep (solver3d.py:878-938) on compact tiles: two tiles per
+// workgroup, p_out = (sum of fluid-neighbour p_in - div) / diag on fluid cells
+// with diag > 0, else 0 (non-fluid p_in entries are 0, so the neighbour sum in
+// the reference order equals the reference's fluid-only sum).
 __global__ __launch_bounds__(GF_WG) void k_jacobi(const uint8_t* __restrict__ flags,
                                                   const int32_t* __restrict__ nbr,
                                                   const float* __restrict__ b,
                                                   const float* __restrict__ pin,
                                                   float* __restrict__ pout, int nact) {
-    const int slot = gf::xcd_remap(blockIdx.x, nact);
-    const size_t base = (size_t)slot * GF_TILE_CELLS;
-    const int32_t* nb = nbr + (size_t)slot * 6;
-    auto get = [&](size_t i) -> float { return pin[i]; };
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const int l = cell_l(h);
-        const size_t idx = base + l;
-        const uint8_t f = flags[idx];
-        if (f & 8) {
-            const int d = f & 7;
-            float out = 0.0f;
-            if (d > 0) {
-                const float s = nb_sum<float>(get, base, l, nb);
-                out = (s - b[idx]) / (float)d;
-            }
-            pout[idx] = out;
-        }
+    const int t = 2 * gf::xcd_remap(blockIdx.x, gridDim.x) + (threadIdx.x >> 7);
+    if (t >= nact) return;
+    const Quad Q = quad_of();
+    const size_t base = (size_t)t * GF_TILE_CELLS;
+    const size_t idx = base + Q.l0;
+    const uint32_t fw = ld_flags(flags, idx);
+    float c[4], bv[4], out[4];
+    ld4(pin + idx, c);
+    ld4(b + idx, bv);
+    Nb4<float> o;
+    load_nb4(pin, base, Q, nbr + (size_t)t * 6, o);
+    for (int k = 0; k < 4; ++k) {
+        const int d = fl_diag(fw, k);
+        float v = 0.0f;
+        if (fl_fluid(fw, k) && d > 0) v = (nb_sum4(o, c, k) - bv[k]) / (float)d;
+        out[k] = v;
     }
+    st4(pout + idx, out);
 }
 
 __global__ void k_fill_f32(float* a, size_t n, float v) {
@@ -440,8 +541,8 @@ extern "C" int gf_scene_jacobi(const gf_scene_t* s, int n_active, int iters, voi
         const bool to_pc = ((iters - 1 - k) % 2) == 0;
         float* out = to_pc ? s->pc : s->ptmp;
         const float* in = to_pc ? s->ptmp : s->pc;
-        hipLaunchKernelGGL(k_jacobi, dim3(n_active), dim3(GF_WG), 0, st, s->flags, s->nbr, s->b,
-                           in, out, n_active);
+        hipLaunchKernelGGL(k_jacobi, dim3((n_active + 1) / 2), dim3(GF_WG), 0, st, s->flags,
+                           s->nbr, s->b, in, out, n_active);
     }
     GF_LAUNCH_CHECK("k_jacobi");
     return 0;
@@ -480,7 +581,8 @@ extern "C" int gf_scene_pcg_sync(const gf_scene_t* s, int n_active, int precisio
     a.x64 = s->x64; a.r64 = s->r64; a.p64 = s->p64; a.ap64 = s->ap64;
     a.pc = s->pc; a.part = s->partials; a.log = (PcgLog*)s->scalars;
     a.nact = n_active;
-    a.G = n_active < PCG_GMAX ? n_active : PCG_GMAX;
+    const int pairs = (n_active + 1) / 2;
+    a.G = pairs < PCG_GMAX ? pairs : PCG_GMAX;
     a.rhs_sign = (float)rhs_sign;
     a.tol2 = tol * tol;
     a.delta2 = 0.01;
