@@ -39,71 +39,152 @@ template <> struct P2GDims<2> { static constexpr int DI = 8, DJ = 8, DK = 9; };
 
 constexpr int P2G_REGION = 576;
 
-// Gather form of the trilinear scatter of gridmath.py:71-95: the face of
-// component COMP at (fi,fj,fk) receives w*val from every particle whose
-// corner set contains it.  The particle's corner origin and fractions are
-// computed exactly as the scatter computes them (floor of the face-space
-// coordinate, s = c - floor(c), w = (1-s or s) per axis, multiplied x*y*z),
-// so each contribution is bit-identical to the reference's atomic_add term.
-struct FaceAcc {
-    float val, wt;
-};
-
+// Trilinear scatter of one velocity component into the tile's LDS region
+// with LDS atomics (gridmath.py:71-95; out-of-grid corners dropped; corners
+// outside the tile's region are finalised by another tile and skipped).
+// Used only for particles whose corners fall outside their cell's stencil
+// (particles outside the domain).
 template <int COMP, bool APIC>
-__device__ __forceinline__ void add_contrib(FaceAcc& acc, int fi, int fj, int fk, float px,
-                                            float py, float pz, float qx, float qy, float qz,
-                                            float vp, const float* crow, float dx) {
-    const float cx = COMP == 0 ? qx : qx - 0.5f;
-    const float cy = COMP == 1 ? qy : qy - 0.5f;
-    const float cz = COMP == 2 ? qz : qz - 0.5f;
-    const float fx0 = floorf(cx), fy0 = floorf(cy), fz0 = floorf(cz);
-    const int di = fi - (int)fx0, dj = fj - (int)fy0, dk = fk - (int)fz0;
-    if ((unsigned)di > 1u || (unsigned)dj > 1u || (unsigned)dk > 1u) return;
-    const float sx = cx - fx0, sy = cy - fy0, sz = cz - fz0;
-    const float wxi = di ? sx : 1.0f - sx;
-    const float wyi = dj ? sy : 1.0f - sy;
-    const float wzi = dk ? sz : 1.0f - sz;
-    const float wt = wxi * wyi * wzi;
-    float val = vp;
-    if (APIC) {
-        const float ox = (COMP == 0 ? (float)fi : (float)fi + 0.5f) * dx - px;
-        const float oy = (COMP == 1 ? (float)fj : (float)fj + 0.5f) * dx - py;
-        const float oz = (COMP == 2 ? (float)fk : (float)fk + 0.5f) * dx - pz;
-        val = vp + crow[0] * ox + crow[1] * oy + crow[2] * oz;
-    }
-    acc.val += val * wt;
-    acc.wt += wt;
-}
-
-// Loops over the particles of cells [ci-1,ci+1] x [cj-1,cj+1] x [ck-1,ck+1]
-// (clamped to the grid; each (i,j) pair is one contiguous k-row range of the
-// cell-sorted arrays) and calls f(px,py,pz,qx,qy,qz,vel,C) for each.
-template <bool APIC, typename F>
-__device__ __forceinline__ void for_particles_near(const P2GParams& a, int ci, int cj, int ck,
-                                                   F f) {
-    const int ny = a.ny, nz = a.nz;
-    const int ilo = max(ci - 1, 0), ihi = min(ci + 1, a.nx - 1);
-    const int jlo = max(cj - 1, 0), jhi = min(cj + 1, ny - 1);
-    const int klo = max(ck - 1, 0), khi = min(ck + 1, nz - 1);
-    const float dx = a.dx;
-    for (int i = ilo; i <= ihi; ++i) {
-        for (int j = jlo; j <= jhi; ++j) {
-            const int base = (i * ny + j) * nz;
-            const int pe = a.cell_start[base + khi + 1];
-            for (int p = a.cell_start[base + klo]; p < pe; ++p) {
-                const float px = a.pos[3 * p + 0], py = a.pos[3 * p + 1], pz = a.pos[3 * p + 2];
-                float v3[3], cm[9];
-                v3[0] = a.vel[3 * p + 0];
-                v3[1] = a.vel[3 * p + 1];
-                v3[2] = a.vel[3 * p + 2];
-                if (APIC) {
+__device__ __forceinline__ void scatter_comp(float* s_val, float* s_wt, float px, float py,
+                                             float pz, float qx, float qy, float qz,
+                                             float vp, const float* crow, int i0t,
+                                             int j0t, int k0t, int nfi, int nfj,
+                                             int nfk, float dx) {
+    using D = P2GDims<COMP>;
+    float cx = qx, cy = qy, cz = qz;
+    if (COMP != 0) cx = qx - 0.5f;
+    if (COMP != 1) cy = qy - 0.5f;
+    if (COMP != 2) cz = qz - 0.5f;
+    const int i0 = (int)floorf(cx), j0 = (int)floorf(cy), k0 = (int)floorf(cz);
+    const float sx = cx - (float)i0, sy = cy - (float)j0, sz = cz - (float)k0;
 #pragma unroll
-                    for (int q = 0; q < 9; ++q) cm[q] = a.C[9 * p + q];
+    for (int di = 0; di < 2; ++di) {
+#pragma unroll
+        for (int dj = 0; dj < 2; ++dj) {
+#pragma unroll
+            for (int dk = 0; dk < 2; ++dk) {
+                const int ii = i0 + di, jj = j0 + dj, kk = k0 + dk;
+                if (ii < 0 || ii >= nfi || jj < 0 || jj >= nfj || kk < 0 || kk >= nfk) continue;
+                const int li = ii - i0t, lj = jj - j0t, lk = kk - k0t;
+                if ((unsigned)li >= (unsigned)D::DI || (unsigned)lj >= (unsigned)D::DJ ||
+                    (unsigned)lk >= (unsigned)D::DK)
+                    continue;
+                const float wxi = di ? sx : 1.0f - sx;
+                const float wyi = dj ? sy : 1.0f - sy;
+                const float wzi = dk ? sz : 1.0f - sz;
+                const float wt = wxi * wyi * wzi;
+                float val = vp;
+                if (APIC) {
+                    const float ox = (COMP == 0 ? (float)ii : (float)ii + 0.5f) * dx - px;
+                    const float oy = (COMP == 1 ? (float)jj : (float)jj + 0.5f) * dx - py;
+                    const float oz = (COMP == 2 ? (float)kk : (float)kk + 0.5f) * dx - pz;
+                    val = vp + crow[0] * ox + crow[1] * oy + crow[2] * oz;
                 }
-                f(px, py, pz, px / dx, py / dx, pz / dx, v3, cm);
+                const int idx = (li * D::DJ + lj) * D::DK + lk;
+                atomicAdd(&s_val[idx], val * wt);
+                atomicAdd(&s_wt[idx], wt);
             }
         }
     }
+}
+
+// Per-cell stencil accumulator of one velocity component: the faces that
+// particles of cell (ci,cj,ck) can reach are [ci, ci+1] along the
+// component's own axis and [c-1, c+1] along the other two.
+template <int COMP>
+struct CellAcc {
+    static constexpr int NX = COMP == 0 ? 2 : 3, NY = COMP == 1 ? 2 : 3, NZ = COMP == 2 ? 2 : 3;
+    float val[NX][NY][NZ];
+    float wt[NX][NY][NZ];
+};
+
+// Corner weights of one axis expanded onto the cell stencil: entries the
+// particle does not touch get weight 0, touched entries get exactly the
+// reference factor (1-s or s).  Returns false when the particle's corner
+// origin lies outside the stencil.
+template <int N>
+__device__ __forceinline__ bool axis_weights(float c, int cell, float (&w)[N], float& s, int& org) {
+    const float f0 = floorf(c);
+    const int i0 = (int)f0;
+    s = c - f0;
+    const int o = i0 - (N == 2 ? cell : cell - 1);
+    org = N == 2 ? cell : cell - 1;
+    if (N == 2) {
+        w[0] = 1.0f - s;
+        w[1] = s;
+        return o == 0;
+    }
+    w[0] = o == 0 ? 1.0f - s : 0.0f;
+    w[1] = o == 0 ? s : 1.0f - s;
+    w[N - 1] = o == 1 ? s : 0.0f;
+    return (unsigned)o <= 1u;
+}
+
+template <int COMP, bool APIC>
+__device__ __forceinline__ bool accumulate(CellAcc<COMP>& A, int ci, int cj, int ck, float px,
+                                           float py, float pz, float qx, float qy, float qz,
+                                           float vp, const float* crow, float dx) {
+    using C = CellAcc<COMP>;
+    float wx[C::NX], wy[C::NY], wz[C::NZ], sx, sy, sz;
+    int ox0, oy0, oz0;
+    bool ok = axis_weights<C::NX>(COMP == 0 ? qx : qx - 0.5f, ci, wx, sx, ox0);
+    ok &= axis_weights<C::NY>(COMP == 1 ? qy : qy - 0.5f, cj, wy, sy, oy0);
+    ok &= axis_weights<C::NZ>(COMP == 2 ? qz : qz - 0.5f, ck, wz, sz, oz0);
+    if (!ok) return false;
+    float offx[C::NX], offy[C::NY], offz[C::NZ];
+    if (APIC) {
+#pragma unroll
+        for (int a = 0; a < C::NX; ++a)
+            offx[a] = (COMP == 0 ? (float)(ox0 + a) : (float)(ox0 + a) + 0.5f) * dx - px;
+#pragma unroll
+        for (int b = 0; b < C::NY; ++b)
+            offy[b] = (COMP == 1 ? (float)(oy0 + b) : (float)(oy0 + b) + 0.5f) * dx - py;
+#pragma unroll
+        for (int c = 0; c < C::NZ; ++c)
+            offz[c] = (COMP == 2 ? (float)(oz0 + c) : (float)(oz0 + c) + 0.5f) * dx - pz;
+    }
+#pragma unroll
+    for (int a = 0; a < C::NX; ++a)
+#pragma unroll
+        for (int b = 0; b < C::NY; ++b)
+#pragma unroll
+            for (int c = 0; c < C::NZ; ++c) {
+                const float w = wx[a] * wy[b] * wz[c];
+                float val = vp;
+                if (APIC) val = vp + crow[0] * offx[a] + crow[1] * offy[b] + crow[2] * offz[c];
+                A.val[a][b][c] += val * w;
+                A.wt[a][b][c] += w;
+            }
+    return true;
+}
+
+// Adds a cell's stencil into the tile's LDS face region (plain read-modify-
+// write; the caller guarantees no concurrent writer of the same entries).
+template <int COMP>
+__device__ __forceinline__ void flush(const CellAcc<COMP>& A, float* s_val, float* s_wt, int ci,
+                                      int cj, int ck, int i0t, int j0t, int k0t, int nx, int ny,
+                                      int nz) {
+    using C = CellAcc<COMP>;
+    using D = P2GDims<COMP>;
+    const int nfi = nx + (COMP == 0), nfj = ny + (COMP == 1), nfk = nz + (COMP == 2);
+    const int fi0 = COMP == 0 ? ci : ci - 1, fj0 = COMP == 1 ? cj : cj - 1,
+              fk0 = COMP == 2 ? ck : ck - 1;
+#pragma unroll
+    for (int a = 0; a < C::NX; ++a)
+#pragma unroll
+        for (int b = 0; b < C::NY; ++b)
+#pragma unroll
+            for (int c = 0; c < C::NZ; ++c) {
+                const int fi = fi0 + a, fj = fj0 + b, fk = fk0 + c;
+                const int li = fi - i0t, lj = fj - j0t, lk = fk - k0t;
+                if (fi < 0 || fi >= nfi || fj < 0 || fj >= nfj || fk < 0 || fk >= nfk) continue;
+                if ((unsigned)li >= (unsigned)D::DI || (unsigned)lj >= (unsigned)D::DJ ||
+                    (unsigned)lk >= (unsigned)D::DK)
+                    continue;
+                const int idx = (li * D::DJ + lj) * D::DK + lk;
+                s_val[idx] += A.val[a][b][c];
+                s_wt[idx] += A.wt[a][b][c];
+            }
 }
 
 // Flattened iteration over k-row segments of the cell-sorted particle arrays.
@@ -175,63 +256,72 @@ __global__ __launch_bounds__(GF_WG) void k_p2g_tile(P2GParams a) {
     }
     __syncthreads();
 
-    // Main pass: thread -> cell (i,j,k) of the tile, gathering its u, v and w
-    // faces (i,j,k) in one sweep over the 27 neighbouring cells' particles.
+    // Lane per cell of the 1-cell-haloed tile: the lane runs through its
+    // cell's particles (contiguous in the sorted arrays) and accumulates the
+    // cell's face stencil in registers; stencils are then added to the LDS
+    // region in 27 phases by cell class (ci%3, cj%3, ck%3) -- cells of one
+    // class have disjoint stencils, so no atomics and a fixed summation order.
+    const int ilo = max(i0 - 1, 0), jlo = max(j0 - 1, 0), klo = max(k0 - 1, 0);
+    const int ni = min(i0 + 9, nx) - ilo, nj = min(j0 + 9, ny) - jlo, nk = min(k0 + 9, nz) - klo;
+    const int ncell = ni * nj * nk;
     const float dx = a.dx;
-#pragma unroll 1
-    for (int h = 0; h < 2; ++h) {
-        const int li = (threadIdx.x >> 6) + 4 * h, lj = (threadIdx.x >> 3) & 7, lk = threadIdx.x & 7;
-        const int i = i0 + li, j = j0 + lj, k = k0 + lk;
-        if (i >= nx || j >= ny || k >= nz) continue;
-        FaceAcc fu = {0.0f, 0.0f}, fv = {0.0f, 0.0f}, fw = {0.0f, 0.0f};
-        for_particles_near<APIC>(a, i, j, k, [&](float px, float py, float pz, float qx, float qy,
-                                                 float qz, const float* v3, const float* cm) {
-            add_contrib<0, APIC>(fu, i, j, k, px, py, pz, qx, qy, qz, v3[0], cm + 0, dx);
-            add_contrib<1, APIC>(fv, i, j, k, px, py, pz, qx, qy, qz, v3[1], cm + 3, dx);
-            add_contrib<2, APIC>(fw, i, j, k, px, py, pz, qx, qy, qz, v3[2], cm + 6, dx);
-        });
-        s_val[0][(li * 8 + lj) * 8 + lk] = fu.val;
-        s_wt[0][(li * 8 + lj) * 8 + lk] = fu.wt;
-        s_val[1][(li * 9 + lj) * 8 + lk] = fv.val;
-        s_wt[1][(li * 9 + lj) * 8 + lk] = fv.wt;
-        s_val[2][(li * 8 + lj) * 9 + lk] = fw.val;
-        s_wt[2][(li * 8 + lj) * 9 + lk] = fw.wt;
-    }
-    // Closing planes: u at i = min(i0+8, nx), v at j = min(j0+8, ny), w at
-    // k = min(k0+8, nz), for the tile's extent on the other two axes.
-    if (threadIdx.x < 192) {
-        const int c = threadIdx.x >> 6, la = (threadIdx.x >> 3) & 7, lb = threadIdx.x & 7;
-        int fi, fj, fk, idx;
-        bool ok;
-        if (c == 0) {
-            fi = min(i0 + 8, nx); fj = j0 + la; fk = k0 + lb;
-            ok = fj < ny && fk < nz;
-            idx = ((fi - i0) * 8 + la) * 8 + lb;
-        } else if (c == 1) {
-            fi = i0 + la; fj = min(j0 + 8, ny); fk = k0 + lb;
-            ok = fi < nx && fk < nz;
-            idx = (la * 9 + (fj - j0)) * 8 + lb;
-        } else {
-            fi = i0 + la; fj = j0 + lb; fk = min(k0 + 8, nz);
-            ok = fi < nx && fj < ny;
-            idx = (la * 8 + lb) * 9 + (fk - k0);
+    for (int base = 0; base < ncell; base += GF_WG) {
+        const int c = base + threadIdx.x;
+        int ci = 0, cj = 0, ck = 0, cls = -1;
+        CellAcc<0> Au;
+        CellAcc<1> Av;
+        CellAcc<2> Aw;
+#pragma unroll
+        for (int q = 0; q < 18; ++q) {
+            (&Au.val[0][0][0])[q] = 0.0f; (&Au.wt[0][0][0])[q] = 0.0f;
+            (&Av.val[0][0][0])[q] = 0.0f; (&Av.wt[0][0][0])[q] = 0.0f;
+            (&Aw.val[0][0][0])[q] = 0.0f; (&Aw.wt[0][0][0])[q] = 0.0f;
         }
-        if (ok) {
-            FaceAcc fa = {0.0f, 0.0f};
-            // cells touching the face: the two cells on either side of it
-            const int ci = c == 0 ? fi - 1 : fi, cj = c == 1 ? fj - 1 : fj, ck = c == 2 ? fk - 1 : fk;
-            for_particles_near<APIC>(a, ci, cj, ck, [&](float px, float py, float pz, float qx,
-                                                        float qy, float qz, const float* v3,
-                                                        const float* cm) {
-                if (c == 0) add_contrib<0, APIC>(fa, fi, fj, fk, px, py, pz, qx, qy, qz, v3[0], cm + 0, dx);
-                else if (c == 1) add_contrib<1, APIC>(fa, fi, fj, fk, px, py, pz, qx, qy, qz, v3[1], cm + 3, dx);
-                else add_contrib<2, APIC>(fa, fi, fj, fk, px, py, pz, qx, qy, qz, v3[2], cm + 6, dx);
-            });
-            s_val[c][idx] = fa.val;
-            s_wt[c][idx] = fa.wt;
+        if (c < ncell) {
+            ck = klo + c % nk;
+            cj = jlo + (c / nk) % nj;
+            ci = ilo + c / (nk * nj);
+            cls = (ci % 3) * 9 + (cj % 3) * 3 + ck % 3;
+            const int key = (ci * ny + cj) * nz + ck;
+            const int pe = a.cell_start[key + 1];
+            int pidx = a.cell_start[key];
+            // the next particle's loads are issued before the current one's math
+            float npx = 0.f, npy = 0.f, npz = 0.f, nvx = 0.f, nvy = 0.f, nvz = 0.f;
+            if (pidx < pe) {
+                npx = a.pos[3 * pidx + 0]; npy = a.pos[3 * pidx + 1]; npz = a.pos[3 * pidx + 2];
+                nvx = a.vel[3 * pidx + 0]; nvy = a.vel[3 * pidx + 1]; nvz = a.vel[3 * pidx + 2];
+            }
+            for (; pidx < pe; ++pidx) {
+                const float px = npx, py = npy, pz = npz, vx = nvx, vy = nvy, vz = nvz;
+                if (pidx + 1 < pe) {
+                    const int nxt = pidx + 1;
+                    npx = a.pos[3 * nxt + 0]; npy = a.pos[3 * nxt + 1]; npz = a.pos[3 * nxt + 2];
+                    nvx = a.vel[3 * nxt + 0]; nvy = a.vel[3 * nxt + 1]; nvz = a.vel[3 * nxt + 2];
+                }
+                float cm[9];                const float qx = px / dx, qy = py / dx, qz = pz / dx;
+                // A component whose corners leave the cell stencil (only for
+                // particles outside the domain) is scattered with LDS atomics.
+                if (!accumulate<0, APIC>(Au, ci, cj, ck, px, py, pz, qx, qy, qz, vx, cm + 0, dx))
+                    scatter_comp<0, APIC>(s_val[0], s_wt[0], px, py, pz, qx, qy, qz, vx, cm + 0,
+                                          i0, j0, k0, nx + 1, ny, nz, dx);
+                if (!accumulate<1, APIC>(Av, ci, cj, ck, px, py, pz, qx, qy, qz, vy, cm + 3, dx))
+                    scatter_comp<1, APIC>(s_val[1], s_wt[1], px, py, pz, qx, qy, qz, vy, cm + 3,
+                                          i0, j0, k0, nx, ny + 1, nz, dx);
+                if (!accumulate<2, APIC>(Aw, ci, cj, ck, px, py, pz, qx, qy, qz, vz, cm + 6, dx))
+                    scatter_comp<2, APIC>(s_val[2], s_wt[2], px, py, pz, qx, qy, qz, vz, cm + 6,
+                                          i0, j0, k0, nx, ny, nz + 1, dx);
+            }
         }
+        for (int ph = 0; ph < 27; ++ph) {
+            __syncthreads();
+            if (cls == ph) {
+                flush<0>(Au, s_val[0], s_wt[0], ci, cj, ck, i0, j0, k0, nx, ny, nz);
+                flush<1>(Av, s_val[1], s_wt[1], ci, cj, ck, i0, j0, k0, nx, ny, nz);
+                flush<2>(Aw, s_val[2], s_wt[2], ci, cj, ck, i0, j0, k0, nx, ny, nz);
+            }
+        }
+        __syncthreads();
     }
-    __syncthreads();
 
     // Finalise the region: normalize (121-142), gravity (149-156), BC (758-799).
     const float gdt = a.g * a.dt;
