@@ -298,7 +298,12 @@ __global__ __launch_bounds__(GF_WG) void k_p2g_tile(P2GParams a) {
                     npx = a.pos[3 * nxt + 0]; npy = a.pos[3 * nxt + 1]; npz = a.pos[3 * nxt + 2];
                     nvx = a.vel[3 * nxt + 0]; nvy = a.vel[3 * nxt + 1]; nvz = a.vel[3 * nxt + 2];
                 }
-                float cm[9];                const float qx = px / dx, qy = py / dx, qz = pz / dx;
+                float cm[9];
+                if (APIC) {
+#pragma unroll
+                    for (int q = 0; q < 9; ++q) cm[q] = a.C[9 * pidx + q];
+                }
+                const float qx = px / dx, qy = py / dx, qz = pz / dx;
                 // A component whose corners leave the cell stencil (only for
                 // particles outside the domain) is scattered with LDS atomics.
                 if (!accumulate<0, APIC>(Au, ci, cj, ck, px, py, pz, qx, qy, qz, vx, cm + 0, dx))
