@@ -169,6 +169,23 @@ __device__ __forceinline__ void flush(const CellAcc<COMP>& A, float* s_val, floa
     const int nfi = nx + (COMP == 0), nfj = ny + (COMP == 1), nfk = nz + (COMP == 2);
     const int fi0 = COMP == 0 ? ci : ci - 1, fj0 = COMP == 1 ? cj : cj - 1,
               fk0 = COMP == 2 ? ck : ck - 1;
+    const int li0 = fi0 - i0t, lj0 = fj0 - j0t, lk0 = fk0 - k0t;
+    if (li0 >= 0 && lj0 >= 0 && lk0 >= 0 && li0 + C::NX <= D::DI && lj0 + C::NY <= D::DJ &&
+        lk0 + C::NZ <= D::DK && fi0 >= 0 && fj0 >= 0 && fk0 >= 0 && fi0 + C::NX <= nfi &&
+        fj0 + C::NY <= nfj && fk0 + C::NZ <= nfk) {
+        // whole stencil inside the region and the grid
+#pragma unroll
+        for (int a = 0; a < C::NX; ++a)
+#pragma unroll
+            for (int b = 0; b < C::NY; ++b)
+#pragma unroll
+                for (int c = 0; c < C::NZ; ++c) {
+                    const int idx = ((li0 + a) * D::DJ + lj0 + b) * D::DK + lk0 + c;
+                    s_val[idx] += A.val[a][b][c];
+                    s_wt[idx] += A.wt[a][b][c];
+                }
+        return;
+    }
 #pragma unroll
     for (int a = 0; a < C::NX; ++a)
 #pragma unroll
@@ -228,11 +245,61 @@ __device__ __forceinline__ RowSet build_rows(const int32_t* __restrict__ cell_st
     return r;
 }
 
+// One component of the lane-per-cell P2G: accumulate the cell's stencil
+// over particles [p0, p1) in registers, then add it to the LDS region in the
+// flush phases r_lo..r_hi (phase = cell class; cls < 0 for empty cells).
+template <int COMP, bool APIC>
+__device__ __forceinline__ void p2g_component(const P2GParams& a, float* s_val, float* s_wt,
+                                              int ci, int cj, int ck, int p0, int p1, int cls,
+                                              int r_lo, int r_hi, int i0, int j0, int k0) {
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const float dx = a.dx;
+    CellAcc<COMP> A;
+#pragma unroll
+    for (int q = 0; q < 18; ++q) {
+        (&A.val[0][0][0])[q] = 0.0f;
+        (&A.wt[0][0][0])[q] = 0.0f;
+    }
+    // the next particle's loads are issued before the current one's math
+    float npx = 0.f, npy = 0.f, npz = 0.f, nv = 0.f;
+    if (p0 < p1) {
+        npx = a.pos[3 * p0 + 0]; npy = a.pos[3 * p0 + 1]; npz = a.pos[3 * p0 + 2];
+        nv = a.vel[3 * p0 + COMP];
+    }
+    for (int p = p0; p < p1; ++p) {
+        const float px = npx, py = npy, pz = npz, vp = nv;
+        if (p + 1 < p1) {
+            npx = a.pos[3 * p + 3]; npy = a.pos[3 * p + 4]; npz = a.pos[3 * p + 5];
+            nv = a.vel[3 * p + 3 + COMP];
+        }
+        float crow[3];
+        if (APIC) {
+            crow[0] = a.C[9 * p + 3 * COMP + 0];
+            crow[1] = a.C[9 * p + 3 * COMP + 1];
+            crow[2] = a.C[9 * p + 3 * COMP + 2];
+        }
+        const float qx = px / dx, qy = py / dx, qz = pz / dx;
+        // corners outside the cell stencil (particles outside the domain) are
+        // scattered with LDS atomics
+        if (!accumulate<COMP, APIC>(A, ci, cj, ck, px, py, pz, qx, qy, qz, vp, crow, dx))
+            scatter_comp<COMP, APIC>(s_val, s_wt, px, py, pz, qx, qy, qz, vp, crow, i0, j0, k0,
+                                     nx + (COMP == 0), ny + (COMP == 1), nz + (COMP == 2), dx);
+    }
+    for (int ph = r_lo; ph <= r_hi; ++ph) {
+        __syncthreads();
+        if (cls == ph) flush<COMP>(A, s_val, s_wt, ci, cj, ck, i0, j0, k0, nx, ny, nz);
+    }
+    __syncthreads();
+}
+
 template <bool APIC, bool RAW>
 __global__ __launch_bounds__(GF_WG) void k_p2g_tile(P2GParams a) {
     __shared__ float s_val[3][P2G_REGION];
     __shared__ float s_wt[3][P2G_REGION];
     __shared__ int8_t s_mk[10][10][10];
+    __shared__ int s_cls[28];
+    __shared__ int s_start[128], s_pref[128];
+    __shared__ int s_wave[GF_WG / GF_WAVE + 1];
 
     const int ntiles = a.tnx * a.tny * a.tnz;
     const int t = gf::xcd_remap(blockIdx.x, ntiles);
@@ -265,67 +332,46 @@ __global__ __launch_bounds__(GF_WG) void k_p2g_tile(P2GParams a) {
     const int ni = min(i0 + 9, nx) - ilo, nj = min(j0 + 9, ny) - jlo, nk = min(k0 + 9, nz) - klo;
     const int ncell = ni * nj * nk;
     const float dx = a.dx;
-    for (int base = 0; base < ncell; base += GF_WG) {
-        const int c = base + threadIdx.x;
-        int ci = 0, cj = 0, ck = 0, cls = -1;
-        CellAcc<0> Au;
-        CellAcc<1> Av;
-        CellAcc<2> Aw;
-#pragma unroll
-        for (int q = 0; q < 18; ++q) {
-            (&Au.val[0][0][0])[q] = 0.0f; (&Au.wt[0][0][0])[q] = 0.0f;
-            (&Av.val[0][0][0])[q] = 0.0f; (&Av.wt[0][0][0])[q] = 0.0f;
-            (&Aw.val[0][0][0])[q] = 0.0f; (&Aw.wt[0][0][0])[q] = 0.0f;
+    // Cells are enumerated class-major (class = relative coordinates mod 3),
+    // so one flush phase involves one or two waves.  s_cls[r] = first linear
+    // index of class r.
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int r = 0; r < 27; ++r) {
+            const int ri = r / 9, rj = (r / 3) % 3, rk = r % 3;
+            s_cls[r] = acc;
+            acc += max(0, (ni - ri + 2) / 3) * max(0, (nj - rj + 2) / 3) * max(0, (nk - rk + 2) / 3);
         }
+        s_cls[27] = acc;
+    }
+    // A tile whose haloed region holds no particle has nothing to scatter.
+    const RowSet rs = build_rows(a.cell_start, ny, nz, ilo, ilo + ni, jlo, jlo + nj, klo, klo + nk,
+                                 s_start, s_pref, s_wave);
+    const int nrounds = rs.total > 0 ? (ncell + GF_WG - 1) / GF_WG : 0;
+    for (int round = 0; round < nrounds; ++round) {
+        const int c = round * GF_WG + threadIdx.x;
+        int ci = 0, cj = 0, ck = 0, cls = -1, p0 = 0, p1 = 0;
         if (c < ncell) {
-            ck = klo + c % nk;
-            cj = jlo + (c / nk) % nj;
-            ci = ilo + c / (nk * nj);
-            cls = (ci % 3) * 9 + (cj % 3) * 3 + ck % 3;
+            int r = 0;
+            while (r < 26 && s_cls[r + 1] <= c) ++r;
+            const int ri = r / 9, rj = (r / 3) % 3, rk = r % 3;
+            const int mj = max(0, (nj - rj + 2) / 3), mk = max(0, (nk - rk + 2) / 3);
+            const int loc = c - s_cls[r];
+            ck = klo + rk + 3 * (loc % mk);
+            cj = jlo + rj + 3 * ((loc / mk) % mj);
+            ci = ilo + ri + 3 * (loc / (mk * mj));
             const int key = (ci * ny + cj) * nz + ck;
-            const int pe = a.cell_start[key + 1];
-            int pidx = a.cell_start[key];
-            // the next particle's loads are issued before the current one's math
-            float npx = 0.f, npy = 0.f, npz = 0.f, nvx = 0.f, nvy = 0.f, nvz = 0.f;
-            if (pidx < pe) {
-                npx = a.pos[3 * pidx + 0]; npy = a.pos[3 * pidx + 1]; npz = a.pos[3 * pidx + 2];
-                nvx = a.vel[3 * pidx + 0]; nvy = a.vel[3 * pidx + 1]; nvz = a.vel[3 * pidx + 2];
-            }
-            for (; pidx < pe; ++pidx) {
-                const float px = npx, py = npy, pz = npz, vx = nvx, vy = nvy, vz = nvz;
-                if (pidx + 1 < pe) {
-                    const int nxt = pidx + 1;
-                    npx = a.pos[3 * nxt + 0]; npy = a.pos[3 * nxt + 1]; npz = a.pos[3 * nxt + 2];
-                    nvx = a.vel[3 * nxt + 0]; nvy = a.vel[3 * nxt + 1]; nvz = a.vel[3 * nxt + 2];
-                }
-                float cm[9];
-                if (APIC) {
-#pragma unroll
-                    for (int q = 0; q < 9; ++q) cm[q] = a.C[9 * pidx + q];
-                }
-                const float qx = px / dx, qy = py / dx, qz = pz / dx;
-                // A component whose corners leave the cell stencil (only for
-                // particles outside the domain) is scattered with LDS atomics.
-                if (!accumulate<0, APIC>(Au, ci, cj, ck, px, py, pz, qx, qy, qz, vx, cm + 0, dx))
-                    scatter_comp<0, APIC>(s_val[0], s_wt[0], px, py, pz, qx, qy, qz, vx, cm + 0,
-                                          i0, j0, k0, nx + 1, ny, nz, dx);
-                if (!accumulate<1, APIC>(Av, ci, cj, ck, px, py, pz, qx, qy, qz, vy, cm + 3, dx))
-                    scatter_comp<1, APIC>(s_val[1], s_wt[1], px, py, pz, qx, qy, qz, vy, cm + 3,
-                                          i0, j0, k0, nx, ny + 1, nz, dx);
-                if (!accumulate<2, APIC>(Aw, ci, cj, ck, px, py, pz, qx, qy, qz, vz, cm + 6, dx))
-                    scatter_comp<2, APIC>(s_val[2], s_wt[2], px, py, pz, qx, qy, qz, vz, cm + 6,
-                                          i0, j0, k0, nx, ny, nz + 1, dx);
-            }
+            p0 = a.cell_start[key];
+            p1 = a.cell_start[key + 1];
+            if (p0 < p1) cls = r;
         }
-        for (int ph = 0; ph < 27; ++ph) {
-            __syncthreads();
-            if (cls == ph) {
-                flush<0>(Au, s_val[0], s_wt[0], ci, cj, ck, i0, j0, k0, nx, ny, nz);
-                flush<1>(Av, s_val[1], s_wt[1], ci, cj, ck, i0, j0, k0, nx, ny, nz);
-                flush<2>(Aw, s_val[2], s_wt[2], ci, cj, ck, i0, j0, k0, nx, ny, nz);
-            }
-        }
-        __syncthreads();
+        const int last = min(ncell, (round + 1) * GF_WG) - 1;
+        int r_lo = 0, r_hi = 0;
+        while (r_lo < 26 && s_cls[r_lo + 1] <= round * GF_WG) ++r_lo;
+        while (r_hi < 26 && s_cls[r_hi + 1] <= last) ++r_hi;
+        p2g_component<0, APIC>(a, s_val[0], s_wt[0], ci, cj, ck, p0, p1, cls, r_lo, r_hi, i0, j0, k0);
+        p2g_component<1, APIC>(a, s_val[1], s_wt[1], ci, cj, ck, p0, p1, cls, r_lo, r_hi, i0, j0, k0);
+        p2g_component<2, APIC>(a, s_val[2], s_wt[2], ci, cj, ck, p0, p1, cls, r_lo, r_hi, i0, j0, k0);
     }
 
     // Finalise the region: normalize (121-142), gravity (149-156), BC (758-799).
