@@ -47,30 +47,49 @@ def build_scene(name):
     return {"c1": seeding.config_c1, "c2": seeding.config_c2, "c3": seeding.config_c3}[name]()
 
 
-def cpu_baseline(scene, max_seconds=30.0):
-    """Time the C restatement of the reference algorithm (oracle/) on the
-    host cores: one substep of the same scene (or fewer particles when the
-    build is missing) and report Mparticle-substeps/s."""
-    exe = ROOT / "oracle" / "_build" / "flip_cpu"
-    if not exe.exists():
-        return None
-    import numpy as np
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(scene):
+    """Time the C restatement of the reference algorithm (oracle/flip_cpu.c)
+    on this host's cores for one substep of the same scene, with the same PCG
+    tolerance and cap.  It is compiled for the host CPU in a temporary
+    directory (the prebuilt oracle/_build/flip_cpu is the fallback)."""
     import tempfile
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
+    import numpy as np
+    src = ROOT / "oracle" / "flip_cpu.c"
+    threads = min(16, len(os.sched_getaffinity(0)))
+    p = scene.params
     with tempfile.TemporaryDirectory() as td:
+        exe = Path(td) / "flip_cpu"
+        r = subprocess.run(["gcc", "-O3", "-march=native", "-fopenmp", "-std=c11", str(src), "-o",
+                            str(exe), "-lm"], capture_output=True, text=True)
+        if r.returncode != 0:
+            exe = ROOT / "oracle" / "_build" / "flip_cpu"
+            if not exe.exists():
+                log("cpu baseline: no compiler and no prebuilt binary")
+                return None
         pfile = Path(td) / "particles.f32"
         np.concatenate([scene.pos, scene.vel], axis=1).astype(np.float32).tofile(pfile)
         n = scene.shape[0]
-        p = scene.params
-        cmd = [str(exe), str(n), str(scene.dx), str(p.get("dt", 1 / 240)), str(p.get("tol", 1e-8)),
+        cmd = [str(exe), str(n), repr(scene.dx), repr(p.get("dt", 1 / 240)), repr(p.get("tol", 1e-8)),
                str(p.get("max_iter", 500)), "1", str(pfile), str(len(scene.pos))]
         env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+        log(f"cpu baseline: {threads} threads on {_cpu_model()}")
         r = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=600)
     if r.returncode != 0:
         log(f"cpu baseline failed: {r.stderr[-400:]}")
         return None
     out = json.loads(r.stdout.strip().splitlines()[-1])
     out["cores"] = threads
+    out["cpu"] = _cpu_model()
     return out
 
 
@@ -179,9 +198,12 @@ def main():
     if not args.no_cpu_baseline:
         c = cpu_baseline(scene)
         if c is not None:
-            cpu = {"value": c["mps"], "unit": "Mparticle-substeps/s", "cores": c["cores"],
-                   "kind": "port", "sample": c.get("sample", "1 substep of the same scene"),
-                   "pcg_iters": c.get("iters"), "seconds": c.get("seconds")}
+            cpu = {"value": round(c["mps"], 4), "unit": "Mparticle-substeps/s",
+                   "cores": c["cores"], "kind": "port", "sample": c.get("sample"),
+                   "cpu": c.get("cpu"), "pcg_iters": c.get("iters"),
+                   "seconds": c.get("seconds"),
+                   "label": "reference-algorithm CPU restatement (oracle/flip_cpu.c, FP32 "
+                            "Jacobi-PCG checked every 10 iterations, dense grid)"}
             log(f"cpu baseline: {cpu}")
 
     out = {
