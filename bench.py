@@ -31,10 +31,12 @@ BYTES = {
     "sort": lambda np_, nf, na: 68 * np_ + 12 * na,
     "p2g": lambda np_, nf, na: 24 * np_ + 29 * nf,
     "g2p": lambda np_, nf, na: 48 * np_ + 41 * nf,
-    "pcg_spmv": lambda np_, nf, na: 9 * nf,
-    "pcg_update": lambda np_, nf, na: 24 * nf,
-    "pcg_dir": lambda np_, nf, na: 13 * nf,
-    "pcg_replace": lambda np_, nf, na: 17 * nf,
+    # r, p_old, flags in; p_new, Ap out (FP32)
+    "pcg_dirspmv": lambda np_, nf, na: 17 * nf,
+    # p, Ap, r, x32, flags in; r, x32 out
+    "pcg_update": lambda np_, nf, na: 25 * nf,
+    # x64, x32, b, flags in; r out
+    "pcg_replace": lambda np_, nf, na: 21 * nf,
 }
 
 
@@ -103,6 +105,9 @@ def main():
     ap.add_argument("--check-every", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-compare", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 256 MiB device copy (3 launches) so that profiler "
+                         "byte counters can be scaled against a kernel of known traffic")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: every box here has one MI355X and the "
@@ -132,6 +137,14 @@ def main():
     def one_step():
         s.step(dt, pressure_iters=max_iter, pressure_solver="pcg", pressure_tol=tol)
 
+    if args.copy_calibration:
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy calibration: 3 x 256 MiB device-to-device copies")
+        del src, dst
     for i in range(args.warmup):
         one_step()
         log(f"warmup {i}: PCG {s.last_pressure_iters} it, residual {s.last_pressure_residual:.2e}")
@@ -161,11 +174,11 @@ def main():
     nf = float(np.mean([st["nf"] for st in stats]))
     na = float(np.mean([st["na"] for st in stats])) * 512
     iters = [st["iters"] for st in stats]
-    kms = np.sum([st["kms"] for st in stats], axis=0)  # spmv, update, dir, replace (ms)
+    kms = np.sum([st["kms"] for st in stats], axis=0)  # dirspmv, update, -, replace (ms)
     n_it = sum(iters)
     n_rep = sum(st["nrep"] for st in stats)
     kern = {
-        "pcg_spmv": (kms[0], n_it), "pcg_update": (kms[1], n_it), "pcg_dir": (kms[2], n_it),
+        "pcg_dirspmv": (kms[0], n_it), "pcg_update": (kms[1], n_it),
         "pcg_replace": (kms[3], n_rep),
         "sort": (sec.get("clear", 0.0), args.steps), "p2g": (sec.get("p2g", 0.0), args.steps),
         "g2p": (sec.get("g2p_advect", 0.0), args.steps),
@@ -179,9 +192,9 @@ def main():
         stages[k] = {"avg_ms": round(avg_ms, 5), "launches": int(calls),
                      "alg_bytes": int(b), "GBps": round(b / (avg_ms * 1e-3) / 1e9, 1),
                      "frac_hbm": round(b / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
-    pcg_ms = float(kms[:3].sum() + kms[3])
-    pcg_bytes = n_it * (9 + 24 + 13) * nf + n_rep * 17 * nf
-    dom = max(("pcg_spmv", "pcg_update", "pcg_dir"), key=lambda k: kern[k][0])
+    pcg_ms = float(kms.sum())
+    pcg_bytes = n_it * (17 + 25) * nf + n_rep * 21 * nf
+    dom = max(("pcg_dirspmv", "pcg_update"), key=lambda k: kern[k][0])
     d = stages[dom]
     roofline = {"bound": "hbm", "achieved": d["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": d["frac_hbm"], "traffic": None, "kernel": dom,

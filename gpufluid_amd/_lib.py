@@ -48,6 +48,7 @@ class GfScene(ctypes.Structure):
         ("x64", c_void_p), ("r64", c_void_p), ("p64", c_void_p), ("ap64", c_void_p),
         ("pc", c_void_p), ("ptmp", c_void_p),
         ("partials", c_void_p), ("scalars", c_void_p),
+        ("p32b", c_void_p), ("p64b", c_void_p),
     ]
 
 

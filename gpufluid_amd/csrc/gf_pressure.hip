@@ -10,10 +10,14 @@
 // PCG (solver3d.py:2917-3043 restated): Jacobi preconditioner max(diag,1)
 // (k3_compute_diag, 1803-1823), x0 = 0, stop on ||r||_2 < tol*||b||_2,
 // alpha = 0 when |pAp| < 1e-30 and beta = 0 when |rz| < 1e-30 (1883-1896).
-// Three kernels per iteration; every scalar stays on the device:
-//   spmv:   Ap = A p, p.Ap                          -> alpha
-//   update: x += alpha p, r -= alpha Ap, z = r/M    -> r.z, r.r, beta, stop flags
-//   dir:    p = z + beta p
+// Two kernels per iteration; every scalar stays on the device:
+//   dirspmv: beta = rz/rz_old, p = z + beta p (ping-pong buffers; the stencil
+//            neighbours' p is recomputed from their z and old p), Ap = A p, p.Ap
+//   update:  alpha = rz/pAp, x += alpha p, r = M z - alpha Ap, z = r/M -> r.z, r.r
+// The residual is stored preconditioned (z = r/M, M = max(diag,1)), so the
+// fused direction kernel needs no divide and no neighbour flags; the update
+// rebuilds r = M z.  The stop / replacement decisions on r.r are taken by the
+// next launch.
 // Dot products are FP64 with a deterministic two-level reduction.  In the
 // mixed-precision mode the vectors are FP32 and an FP64 iterate x64 + x32 is
 // kept; the residual is replaced by the FP64 true residual b - A(x64+x32)
@@ -28,9 +32,13 @@ namespace {
 // reads entry (L-1)&1 and writes entry L&1 (workgroup 0 only), so no kernel
 // reads a field that it also writes.
 struct PcgLog {
-    double rz, rr, bb, alpha, beta, rr_upd, rr_true, pap;
+    double rz;       // r.z of the current residual (used by the next alpha)
+    double rz_old;   // r.z used by the last alpha (denominator of the next beta)
+    double rr, bb, alpha, beta, rr_upd, rr_true, pap;
     int iters, done, converged, maxed, upd, merge, nupd, pending;
-    double pad[4];
+    int fresh;       // 1: the update kernel's r.z / r.r partials are not consumed yet
+    int pbuf;        // direction buffer holding the current p
+    double pad[2];
 };
 static_assert(sizeof(PcgLog) == 16 * sizeof(double), "log entry is 16 doubles");
 
@@ -42,8 +50,10 @@ struct PcgArgs {
     const uint8_t* flags;
     const int32_t* nbr;
     const float* b;
-    float* x32; float* r32; float* p32; float* ap32;
-    double* x64; double* r64; double* p64; double* ap64;
+    float* x32; float* r32; float* ap32;
+    double* x64; double* r64; double* ap64;
+    float* p32[2];       // ping-pong directions
+    double* p64[2];
     float* pc;
     double* part;        // [P_KINDS][PCG_GMAX]
     PcgLog* log;         // [2]
@@ -93,11 +103,51 @@ __device__ __forceinline__ uint32_t ld_flags(const uint8_t* f, size_t idx) {
     return *reinterpret_cast<const uint32_t*>(f + idx);
 }
 __device__ __forceinline__ int fl_fluid(uint32_t fw, int e) { return (fw >> (8 * e + 3)) & 1; }
+
+// v if keep, else +0, as a bit mask: a select here would let the compiler
+// sink the loads that feed v into a branch and serialise them.
+__device__ __forceinline__ float keep_if(float v, int keep) {
+    return __uint_as_float(__float_as_uint(v) & (0u - (uint32_t)keep));
+}
+__device__ __forceinline__ double keep_if(double v, int keep) {
+    return __longlong_as_double(__double_as_longlong(v) & (0ll - (long long)keep));
+}
 __device__ __forceinline__ int fl_diag(uint32_t fw, int e) { return (fw >> (8 * e)) & 7; }
 
-// Neighbour values of a quad: the four -x/+x/-y/+y vectors and the two z
-// values that lie outside the quad (-z of its first cell, +z of its last).
-// Neighbours in inactive tiles read as 0; every load uses a valid address.
+// Element offsets of a quad's neighbours: the four -x/+x/-y/+y quads and the
+// two z cells outside the quad (-z of its first cell, +z of its last).  A
+// neighbour in an inactive tile gets the quad's own offset and a cleared ok
+// bit, so every later load is unconditional and in bounds.  The tile's
+// neighbour table is read unconditionally (t is wave-uniform) and the
+// addresses are formed with selects: no load sits behind a branch.
+struct NbIdx {
+    size_t xm, xp, ym, yp, zm, zp;
+    uint32_t ok;  // bit d set: neighbour d (order -x,+x,-y,+y,-z,+z) exists
+};
+
+__device__ __forceinline__ NbIdx nb_index(const int32_t* __restrict__ nbr, int t, size_t base,
+                                          const Quad& Q) {
+    const int2* row = reinterpret_cast<const int2*>(nbr + (size_t)t * 6);
+    const int2 n01 = row[0], n23 = row[1], n45 = row[2];
+    const size_t own = base + Q.l0;
+    NbIdx o;
+    const bool bxm = Q.li > 0, bxp = Q.li < 7, bym = Q.lj > 0, byp = Q.lj < 7;
+    const bool bzm = Q.kh == 1, bzp = Q.kh == 0;
+    const bool kxm = bxm || n01.x >= 0, kxp = bxp || n01.y >= 0;
+    const bool kym = bym || n23.x >= 0, kyp = byp || n23.y >= 0;
+    const bool kzm = bzm || n45.x >= 0, kzp = bzp || n45.y >= 0;
+    o.xm = bxm ? own - 64 : (kxm ? (size_t)n01.x * 512 + Q.l0 + 448 : own);
+    o.xp = bxp ? own + 64 : (kxp ? (size_t)n01.y * 512 + Q.l0 - 448 : own);
+    o.ym = bym ? own - 8 : (kym ? (size_t)n23.x * 512 + Q.l0 + 56 : own);
+    o.yp = byp ? own + 8 : (kyp ? (size_t)n23.y * 512 + Q.l0 - 56 : own);
+    o.zm = bzm ? own - 1 : (kzm ? (size_t)n45.x * 512 + Q.l0 + 7 : own);
+    o.zp = bzp ? own + 4 : (kzp ? (size_t)n45.y * 512 + Q.l0 - 4 : own);
+    o.ok = (uint32_t)kxm | (uint32_t)kxp << 1 | (uint32_t)kym << 2 | (uint32_t)kyp << 3 |
+           (uint32_t)kzm << 4 | (uint32_t)kzp << 5;
+    return o;
+}
+
+// Neighbour values of a quad (see NbIdx); neighbours in inactive tiles read as 0.
 template <typename T>
 struct Nb4 {
     T xm[4], xp[4], ym[4], yp[4];
@@ -105,33 +155,21 @@ struct Nb4 {
 };
 
 template <typename T>
-__device__ __forceinline__ void load_nb4(const T* __restrict__ v, size_t base, const Quad& Q,
-                                         const int32_t* nb, Nb4<T>& o) {
-    const size_t own = base + Q.l0;
-    bool ok;
-    size_t a;
-    ok = Q.li > 0 || nb[0] >= 0;
-    a = Q.li > 0 ? own - 64 : (nb[0] >= 0 ? (size_t)nb[0] * 512 + Q.l0 + 448 : own);
-    ld4(v + a, o.xm);
-    if (!ok) for (int e = 0; e < 4; ++e) o.xm[e] = (T)0;
-    ok = Q.li < 7 || nb[1] >= 0;
-    a = Q.li < 7 ? own + 64 : (nb[1] >= 0 ? (size_t)nb[1] * 512 + Q.l0 - 448 : own);
-    ld4(v + a, o.xp);
-    if (!ok) for (int e = 0; e < 4; ++e) o.xp[e] = (T)0;
-    ok = Q.lj > 0 || nb[2] >= 0;
-    a = Q.lj > 0 ? own - 8 : (nb[2] >= 0 ? (size_t)nb[2] * 512 + Q.l0 + 56 : own);
-    ld4(v + a, o.ym);
-    if (!ok) for (int e = 0; e < 4; ++e) o.ym[e] = (T)0;
-    ok = Q.lj < 7 || nb[3] >= 0;
-    a = Q.lj < 7 ? own + 8 : (nb[3] >= 0 ? (size_t)nb[3] * 512 + Q.l0 - 56 : own);
-    ld4(v + a, o.yp);
-    if (!ok) for (int e = 0; e < 4; ++e) o.yp[e] = (T)0;
-    ok = Q.kh == 1 || nb[4] >= 0;
-    a = Q.kh == 1 ? own - 1 : (nb[4] >= 0 ? (size_t)nb[4] * 512 + Q.l0 + 7 : own);
-    o.zm0 = ok ? v[a] : (T)0;
-    ok = Q.kh == 0 || nb[5] >= 0;
-    a = Q.kh == 0 ? own + 4 : (nb[5] >= 0 ? (size_t)nb[5] * 512 + Q.l0 - 4 : own);
-    o.zp3 = ok ? v[a] : (T)0;
+__device__ __forceinline__ void load_nb4(const T* __restrict__ v, const NbIdx& n, Nb4<T>& o) {
+    ld4(v + n.xm, o.xm);
+    ld4(v + n.xp, o.xp);
+    ld4(v + n.ym, o.ym);
+    ld4(v + n.yp, o.yp);
+    o.zm0 = v[n.zm];
+    o.zp3 = v[n.zp];
+    for (int e = 0; e < 4; ++e) {
+        o.xm[e] = (n.ok & 1) ? o.xm[e] : (T)0;
+        o.xp[e] = (n.ok & 2) ? o.xp[e] : (T)0;
+        o.ym[e] = (n.ok & 4) ? o.ym[e] : (T)0;
+        o.yp[e] = (n.ok & 8) ? o.yp[e] : (T)0;
+    }
+    o.zm0 = (n.ok & 16) ? o.zm0 : (T)0;
+    o.zp3 = (n.ok & 32) ? o.zp3 : (T)0;
 }
 
 // Neighbour sum of quad element e in the reference order -x,+x,-y,+y,-z,+z.
@@ -184,51 +222,96 @@ __device__ __forceinline__ void put_partial(double* part, int kind, int b, doubl
 template <typename T> __device__ __forceinline__ T* vr(const PcgArgs& a);
 template <> __device__ __forceinline__ float* vr<float>(const PcgArgs& a) { return a.r32; }
 template <> __device__ __forceinline__ double* vr<double>(const PcgArgs& a) { return a.r64; }
-template <typename T> __device__ __forceinline__ T* vp(const PcgArgs& a);
-template <> __device__ __forceinline__ float* vp<float>(const PcgArgs& a) { return a.p32; }
-template <> __device__ __forceinline__ double* vp<double>(const PcgArgs& a) { return a.p64; }
+template <typename T> __device__ __forceinline__ T* vp(const PcgArgs& a, int buf);
+template <> __device__ __forceinline__ float* vp<float>(const PcgArgs& a, int buf) { return a.p32[buf]; }
+template <> __device__ __forceinline__ double* vp<double>(const PcgArgs& a, int buf) { return a.p64[buf]; }
 template <typename T> __device__ __forceinline__ T* vap(const PcgArgs& a);
 template <> __device__ __forceinline__ float* vap<float>(const PcgArgs& a) { return a.ap32; }
 template <> __device__ __forceinline__ double* vap<double>(const PcgArgs& a) { return a.ap64; }
 
 enum { PEND_NONE = 0, PEND_SETUP = 1, PEND_REPLACE = 2 };
 
+// A tile is handled by two whole waves, so t is wave-uniform: readfirstlane
+// lets the compiler keep the tile's base and neighbour table in scalar registers.
 #define FOR_QUADS(a)                                                        \
     int t0_, t1_, b;                                                        \
     tile_range(a, t0_, t1_, b);                                             \
     const Quad Q = quad_of();                                               \
     for (int tb_ = t0_; tb_ < t1_; tb_ += 2)                                \
-        if (const int t = tb_ + (threadIdx.x >> 7); t < t1_)
+        if (const int t = __builtin_amdgcn_readfirstlane(tb_ + (threadIdx.x >> 7)); t < t1_)
 
-// L = 0: x = 0, r = rhs_sign*b, p = z = r/M; partials r.z and r.r.
+template <typename T>
+__device__ __forceinline__ T zdiv(T r, uint32_t flagbyte) {
+    const int d = flagbyte & 7;
+    return r / (T)(d > 1 ? d : 1);
+}
+
+template <typename T>
+__device__ __forceinline__ T diagm(uint32_t flagbyte) {
+    const int d = flagbyte & 7;
+    return (T)(d > 1 ? d : 1);
+}
+
+// p = z + beta p_old as one explicit fma, so that a cell's value is
+// bit-identical whether its owner or a neighbour computes it.
+template <typename T>
+__device__ __forceinline__ void dir4(const T (&z)[4], const T (&po)[4], T bf, T (&pn)[4]) {
+    for (int k = 0; k < 4; ++k) pn[k] = fma(bf, po[k], z[k]);
+}
+
+// Consumes the update kernel's r.z / r.r partials (when fresh): counts the
+// finished iteration and sets the stop / replacement flags.  Identical in
+// every workgroup; workgroup 0 writes the log.
+template <bool MIXED>
+__device__ __forceinline__ void consume(const PcgArgs& a, PcgLog& e, double* s_red) {
+    if (!e.fresh) return;
+    const double rz_new = sum_partials(a.part, P_RZ, a.G, s_red);
+    const double rr = sum_partials(a.part, P_RR, a.G, s_red);
+    e.rz_old = e.rz;
+    e.rz = rz_new;
+    e.rr = rr;
+    e.fresh = 0;
+    e.iters += 1;
+    if (rr < a.tol2 * e.bb) {
+        e.done = 1;
+        if (!MIXED) { e.converged = 1; e.rr_true = rr; }
+    } else if (e.iters >= a.max_iter) {
+        e.done = 1;
+        e.maxed = 1;
+        if (!MIXED) e.rr_true = rr;
+    } else if (MIXED && rr < a.delta2 * e.rr_upd) {
+        e.upd = 1;
+    }
+}
+
+// L = 0: x = 0, r = rhs_sign*b, z = r/M stored, p (buffer 0) = 0; partials
+// r.z and r.r.
 template <typename T, bool MIXED>
 __global__ __launch_bounds__(GF_WG) void k_pcg_setup(PcgArgs a) {
     __shared__ double s_red[GF_WG / GF_WAVE];
     T* r = vr<T>(a);
-    T* p = vp<T>(a);
+    T* p = vp<T>(a, 0);
     double rz = 0.0, rr = 0.0;
     FOR_QUADS(a) {
         const size_t idx = (size_t)t * GF_TILE_CELLS + Q.l0;
         const uint32_t fw = ld_flags(a.flags, idx);
         float bv[4];
         ld4(a.b + idx, bv);
-        T rv[4], z[4], zero[4] = {(T)0, (T)0, (T)0, (T)0};
+        T zv[4], zero[4] = {(T)0, (T)0, (T)0, (T)0};
         double zd[4] = {0.0, 0.0, 0.0, 0.0};
         for (int e = 0; e < 4; ++e) {
-            rv[e] = fl_fluid(fw, e) ? (T)(a.rhs_sign * bv[e]) : (T)0;
-            const int d = fl_diag(fw, e);
-            z[e] = rv[e] / (T)(d > 1 ? d : 1);
-            rz += (double)rv[e] * (double)z[e];
-            rr += (double)rv[e] * (double)rv[e];
+            const T rv = fl_fluid(fw, e) ? (T)(a.rhs_sign * bv[e]) : (T)0;
+            zv[e] = zdiv(rv, fw >> (8 * e));
+            rz += (double)rv * (double)zv[e];
+            rr += (double)rv * (double)rv;
         }
-        st4(r + idx, rv);
-        st4(p + idx, z);
+        st4(r + idx, zv);
+        st4(p + idx, zero);
         st4(a.x64 + idx, zd);
         if (MIXED) {
             float zf[4] = {0.f, 0.f, 0.f, 0.f};
             st4(a.x32 + idx, zf);
         }
-        (void)zero;
     }
     put_partial(a.part, P_RZ, b, rz, s_red);
     put_partial(a.part, P_BB, b, rr, s_red);
@@ -246,8 +329,8 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_fin(PcgArgs a, int L) {
     if (e.pending == PEND_SETUP) {
         const double rz = sum_partials(a.part, P_RZ, a.G, s_red);
         const double bb = sum_partials(a.part, P_BB, a.G, s_red);
-        e.rz = rz; e.bb = bb; e.rr = bb; e.rr_upd = bb; e.rr_true = bb;
-        e.iters = 0; e.maxed = 0; e.upd = 0; e.merge = 0; e.nupd = 0;
+        e.rz = rz; e.rz_old = 0.0; e.bb = bb; e.rr = bb; e.rr_upd = bb; e.rr_true = bb;
+        e.iters = 0; e.maxed = 0; e.upd = 0; e.merge = 0; e.nupd = 0; e.fresh = 0; e.pbuf = 0;
         const int zero = bb < 1e-30;
         e.converged = zero;
         e.done = zero || a.max_iter <= 0;
@@ -270,36 +353,62 @@ __device__ __forceinline__ bool pass_through(const PcgArgs& a, const PcgLog& e, 
     return true;
 }
 
-// Ap = A p; partial p.Ap.
-template <typename T>
-__global__ __launch_bounds__(GF_WG) void k_pcg_spmv(PcgArgs a, int L) {
+// beta = rz / rz_old; p_new = z + beta p_old (own cells, and recomputed for
+// the stencil neighbours); Ap = A p_new; partial p.Ap.
+template <typename T, bool MIXED>
+__global__ __launch_bounds__(GF_WG) void k_pcg_dirspmv(PcgArgs a, int L) {
     __shared__ double s_red[GF_WG / GF_WAVE];
-    const PcgLog e = a.log[(L - 1) & 1];
+    PcgLog e = a.log[(L - 1) & 1];
     if (pass_through(a, e, L)) return;
-    const T* __restrict__ p = vp<T>(a);
+    consume<MIXED>(a, e, s_red);
+    if (e.done) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) a.log[L & 1] = e;
+        return;
+    }
+    const double beta = fabs(e.rz_old) < 1e-30 ? 0.0 : e.rz / e.rz_old;
+    const T bf = (T)beta;
+    const T* __restrict__ po = vp<T>(a, e.pbuf);
+    T* __restrict__ pn = vp<T>(a, e.pbuf ^ 1);
+    const T* __restrict__ z = vr<T>(a);
     T* __restrict__ ap = vap<T>(a);
     double pap = 0.0;
     FOR_QUADS(a) {
         const size_t base = (size_t)t * GF_TILE_CELLS;
         const size_t idx = base + Q.l0;
+        const NbIdx ni = nb_index(a.nbr, t, base, Q);
         const uint32_t fw = ld_flags(a.flags, idx);
-        T c[4], y[4];
-        ld4(p + idx, c);
+        T zq[4], pq[4], c[4], y[4];
+        ld4(z + idx, zq);
+        ld4(po + idx, pq);
+        Nb4<T> nz, np;
+        load_nb4(z, ni, nz);
+        load_nb4(po, ni, np);
+        dir4(zq, pq, bf, c);
         Nb4<T> o;
-        load_nb4(p, base, Q, a.nbr + (size_t)t * 6, o);
+        dir4(nz.xm, np.xm, bf, o.xm);
+        dir4(nz.xp, np.xp, bf, o.xp);
+        dir4(nz.ym, np.ym, bf, o.ym);
+        dir4(nz.yp, np.yp, bf, o.yp);
+        o.zm0 = fma(bf, np.zm0, nz.zm0);
+        o.zp3 = fma(bf, np.zp3, nz.zp3);
         for (int k = 0; k < 4; ++k) {
             const T s = nb_sum4(o, c, k);
-            y[k] = fl_fluid(fw, k) ? (T)fl_diag(fw, k) * c[k] - s : (T)0;
+            y[k] = keep_if((T)fl_diag(fw, k) * c[k] - s, fl_fluid(fw, k));
             pap += (double)c[k] * (double)y[k];
         }
+        st4(pn + idx, c);
         st4(ap + idx, y);
     }
     put_partial(a.part, P_PAP, b, pap, s_red);
-    if (blockIdx.x == 0 && threadIdx.x == 0) a.log[L & 1] = e;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        e.beta = beta;
+        e.pbuf ^= 1;
+        a.log[L & 1] = e;
+    }
 }
 
-// alpha = rz / pAp; x += alpha p, r -= alpha Ap; partials r.z, r.r.
-// Non-fluid entries of p, Ap, r and x are 0 and stay 0.
+// alpha = rz / pAp; x += alpha p, r = M z - alpha Ap, z = r/M stored;
+// partials r.z, r.r.  Non-fluid entries of p, Ap, z and x are 0 and stay 0.
 template <typename T, bool MIXED>
 __global__ __launch_bounds__(GF_WG) void k_pcg_update(PcgArgs a, int L) {
     __shared__ double s_red[GF_WG / GF_WAVE];
@@ -309,17 +418,17 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_update(PcgArgs a, int L) {
     const double alpha = fabs(pap_t) < 1e-30 ? 0.0 : e.rz / pap_t;
     const T af = (T)alpha;
     const int merge = MIXED ? e.merge : 0;
-    T* __restrict__ r = vr<T>(a);
-    const T* __restrict__ p = vp<T>(a);
+    T* __restrict__ zp = vr<T>(a);
+    const T* __restrict__ p = vp<T>(a, e.pbuf);
     const T* __restrict__ ap = vap<T>(a);
     double rz = 0.0, rr = 0.0;
     FOR_QUADS(a) {
         const size_t idx = (size_t)t * GF_TILE_CELLS + Q.l0;
         const uint32_t fw = ld_flags(a.flags, idx);
-        T pv[4], apv[4], rv[4];
+        T pv[4], apv[4], zv[4];
         ld4(p + idx, pv);
         ld4(ap + idx, apv);
-        ld4(r + idx, rv);
+        ld4(zp + idx, zv);
         if (MIXED) {
             float xo[4];
             ld4(a.x32 + idx, xo);
@@ -341,13 +450,13 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_update(PcgArgs a, int L) {
             st4(a.x64 + idx, xd);
         }
         for (int k = 0; k < 4; ++k) {
-            rv[k] = rv[k] - af * apv[k];
-            const int d = fl_diag(fw, k);
-            const T z = rv[k] / (T)(d > 1 ? d : 1);
-            rz += (double)rv[k] * (double)z;
-            rr += (double)rv[k] * (double)rv[k];
+            const T dm = diagm<T>(fw >> (8 * k));
+            const T rv = fma(-af, apv[k], dm * zv[k]);
+            zv[k] = rv / dm;
+            rz += (double)rv * (double)zv[k];
+            rr += (double)rv * (double)rv;
         }
-        st4(r + idx, rv);
+        st4(zp + idx, zv);
     }
     put_partial(a.part, P_RZ, b, rz, s_red);
     put_partial(a.part, P_RR, b, rr, s_red);
@@ -355,62 +464,21 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_update(PcgArgs a, int L) {
         e.alpha = alpha;
         e.pap = pap_t;
         e.merge = 0;
+        e.fresh = 1;
         a.log[L & 1] = e;
     }
 }
 
-// beta = rz_new / rz; p = z + beta p; convergence / replacement decisions.
-template <typename T, bool MIXED>
-__global__ __launch_bounds__(GF_WG) void k_pcg_dir(PcgArgs a, int L) {
-    __shared__ double s_red[GF_WG / GF_WAVE];
-    PcgLog e = a.log[(L - 1) & 1];
-    if (pass_through(a, e, L)) return;
-    const double rz_new = sum_partials(a.part, P_RZ, a.G, s_red);
-    const double rr = sum_partials(a.part, P_RR, a.G, s_red);
-    const double beta = fabs(e.rz) < 1e-30 ? 0.0 : rz_new / e.rz;
-    const T bf = (T)beta;
-    const T* __restrict__ r = vr<T>(a);
-    T* __restrict__ p = vp<T>(a);
-    FOR_QUADS(a) {
-        const size_t idx = (size_t)t * GF_TILE_CELLS + Q.l0;
-        const uint32_t fw = ld_flags(a.flags, idx);
-        T rv[4], pv[4];
-        ld4(r + idx, rv);
-        ld4(p + idx, pv);
-        for (int k = 0; k < 4; ++k) {
-            const int d = fl_diag(fw, k);
-            const T z = rv[k] / (T)(d > 1 ? d : 1);
-            pv[k] = z + bf * pv[k];
-        }
-        st4(p + idx, pv);
-    }
-    (void)b;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        e.beta = beta;
-        e.rz = rz_new;
-        e.rr = rr;
-        e.iters += 1;
-        if (rr < a.tol2 * e.bb) {
-            e.done = 1;
-            if (!MIXED) { e.converged = 1; e.rr_true = rr; }
-        } else if (e.iters >= a.max_iter) {
-            e.done = 1;
-            e.maxed = 1;
-            if (!MIXED) e.rr_true = rr;
-        } else if (MIXED && rr < a.delta2 * e.rr_upd) {
-            e.upd = 1;
-        }
-        a.log[L & 1] = e;
-    }
-}
-
-// Residual replacement (mixed mode): r = rhs - A (x64 + x32) in FP64 when a
-// replacement is due or convergence is to be confirmed; partials of the true
-// r.r and of r.z.  Finalised by k_pcg_fin.
+// Chunk end: consumes the last update's partials and decides; in mixed mode,
+// when a replacement is due or convergence is to be confirmed, computes the
+// FP64 true residual r = rhs - A (x64 + x32), stores its FP32 z = r/M, and
+// leaves partials of its r.r and r.z for k_pcg_fin.
+template <bool MIXED>
 __global__ __launch_bounds__(GF_WG) void k_pcg_replace(PcgArgs a, int L) {
     __shared__ double s_red[GF_WG / GF_WAVE];
     PcgLog e = a.log[(L - 1) & 1];
-    const bool run = e.upd || (e.done && !e.converged);
+    consume<MIXED>(a, e, s_red);
+    const bool run = MIXED && (e.upd || (e.done && !e.converged));
     if (!run) {
         if (blockIdx.x == 0 && threadIdx.x == 0) a.log[L & 1] = e;
         return;
@@ -420,7 +488,7 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_replace(PcgArgs a, int L) {
         const size_t base = (size_t)t * GF_TILE_CELLS;
         const size_t idx = base + Q.l0;
         const uint32_t fw = ld_flags(a.flags, idx);
-        const int32_t* nb = a.nbr + (size_t)t * 6;
+        const NbIdx ni = nb_index(a.nbr, t, base, Q);
         double cd[4];
         float cf[4], bv[4];
         ld4(a.x64 + idx, cd);
@@ -428,8 +496,8 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_replace(PcgArgs a, int L) {
         ld4(a.b + idx, bv);
         Nb4<double> od;
         Nb4<float> of;
-        load_nb4(a.x64, base, Q, nb, od);
-        load_nb4(a.x32, base, Q, nb, of);
+        load_nb4(a.x64, ni, od);
+        load_nb4(a.x32, ni, of);
         double c[4];
         for (int k = 0; k < 4; ++k) {
             c[k] = cd[k] + (double)cf[k];
@@ -440,21 +508,18 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_replace(PcgArgs a, int L) {
         }
         od.zm0 += (double)of.zm0;
         od.zp3 += (double)of.zp3;
-        float r32[4];
+        float z32[4];
         for (int k = 0; k < 4; ++k) {
-            float rk = 0.0f;
-            if (fl_fluid(fw, k)) {
-                const int d = fl_diag(fw, k);
-                const double ax = (double)d * c[k] - nb_sum4(od, c, k);
-                const double rv = (double)(a.rhs_sign * bv[k]) - ax;
-                rk = (float)rv;
-                const float z = rk / (float)(d > 1 ? d : 1);
-                rrt += rv * rv;
-                rz += (double)rk * (double)z;
-            }
-            r32[k] = rk;
+            const int d = fl_diag(fw, k), fl = fl_fluid(fw, k);
+            const double ax = (double)d * c[k] - nb_sum4(od, c, k);
+            const double rv = keep_if((double)(a.rhs_sign * bv[k]) - ax, fl);
+            const float rk = (float)rv;
+            const float zk = rk / (float)(d > 1 ? d : 1);
+            rrt += rv * rv;
+            rz += (double)rk * (double)zk;
+            z32[k] = zk;
         }
-        st4(a.r32 + idx, r32);
+        st4(a.r32 + idx, z32);
     }
     put_partial(a.part, P_RRT, b, rrt, s_red);
     put_partial(a.part, P_RZT, b, rz, s_red);
@@ -493,7 +558,8 @@ __global__ __launch_bounds__(GF_WG) void k_jacobi(const uint8_t* __restrict__ fl
                                                   const float* __restrict__ b,
                                                   const float* __restrict__ pin,
                                                   float* __restrict__ pout, int nact) {
-    const int t = 2 * gf::xcd_remap(blockIdx.x, gridDim.x) + (threadIdx.x >> 7);
+    const int t = __builtin_amdgcn_readfirstlane(2 * gf::xcd_remap(blockIdx.x, gridDim.x) +
+                                                 (threadIdx.x >> 7));
     if (t >= nact) return;
     const Quad Q = quad_of();
     const size_t base = (size_t)t * GF_TILE_CELLS;
@@ -503,12 +569,11 @@ __global__ __launch_bounds__(GF_WG) void k_jacobi(const uint8_t* __restrict__ fl
     ld4(pin + idx, c);
     ld4(b + idx, bv);
     Nb4<float> o;
-    load_nb4(pin, base, Q, nbr + (size_t)t * 6, o);
+    load_nb4(pin, nb_index(nbr, t, base, Q), o);
     for (int k = 0; k < 4; ++k) {
         const int d = fl_diag(fw, k);
-        float v = 0.0f;
-        if (fl_fluid(fw, k) && d > 0) v = (nb_sum4(o, c, k) - bv[k]) / (float)d;
-        out[k] = v;
+        const float v = (nb_sum4(o, c, k) - bv[k]) / (float)(d > 0 ? d : 1);
+        out[k] = keep_if(v, fl_fluid(fw, k) & (d > 0));
     }
     st4(pout + idx, out);
 }
@@ -563,9 +628,10 @@ extern "C" int gf_scene_pcg_sync(const gf_scene_t* s, int n_active, int precisio
     GF_REQUIRE(s->flags && s->nbr && s->b && s->pc && s->partials && s->scalars && s->x64,
                "gf_scene_pcg_sync: buffers not set");
     if (precision == GF_PCG_MIXED)
-        GF_REQUIRE(s->x32 && s->r32 && s->p32 && s->ap32, "gf_scene_pcg_sync: FP32 vectors not set");
+        GF_REQUIRE(s->x32 && s->r32 && s->p32 && s->p32b && s->ap32,
+                   "gf_scene_pcg_sync: FP32 vectors not set");
     else
-        GF_REQUIRE(s->r64 && s->p64 && s->ap64, "gf_scene_pcg_sync: FP64 vectors not set");
+        GF_REQUIRE(s->r64 && s->p64 && s->p64b && s->ap64, "gf_scene_pcg_sync: FP64 vectors not set");
     if (check_every < 1) check_every = 1;
     if (check_every > 64) check_every = 64;
     if (iters) *iters = 0;
@@ -577,8 +643,10 @@ extern "C" int gf_scene_pcg_sync(const gf_scene_t* s, int n_active, int precisio
 
     PcgArgs a;
     a.flags = s->flags; a.nbr = s->nbr; a.b = s->b;
-    a.x32 = s->x32; a.r32 = s->r32; a.p32 = s->p32; a.ap32 = s->ap32;
-    a.x64 = s->x64; a.r64 = s->r64; a.p64 = s->p64; a.ap64 = s->ap64;
+    a.x32 = s->x32; a.r32 = s->r32; a.ap32 = s->ap32;
+    a.x64 = s->x64; a.r64 = s->r64; a.ap64 = s->ap64;
+    a.p32[0] = s->p32; a.p32[1] = s->p32b;
+    a.p64[0] = s->p64; a.p64[1] = s->p64b;
     a.pc = s->pc; a.part = s->partials; a.log = (PcgLog*)s->scalars;
     a.nact = n_active;
     const int pairs = (n_active + 1) / 2;
@@ -600,12 +668,13 @@ extern "C" int gf_scene_pcg_sync(const gf_scene_t* s, int n_active, int precisio
 
     // Optional per-kernel timing of the iterations that did work: events
     // around every launch of a chunk, read back at the chunk's host sync.
+    // kernel_ms: [0] dirspmv, [1] update, [2] unused, [3] chunk-end replacement.
     const bool timed = kernel_ms != nullptr;
-    hipEvent_t ev[4 * 64 + 2];
+    hipEvent_t ev[2 * 64 + 3];
     int nev = 0;
     hipError_t e;
     if (timed) {
-        nev = 4 * check_every + 2;
+        nev = 2 * check_every + 3;
         for (int i = 0; i < nev; ++i) {
             e = hipEventCreate(&ev[i]);
             if (e != hipSuccess) {
@@ -623,29 +692,27 @@ extern "C" int gf_scene_pcg_sync(const gf_scene_t* s, int n_active, int precisio
     while (!rc && !hs.done) {
         const int chunk = check_every;
         for (int j = 0; j < chunk; ++j) {
-            if (timed) hipEventRecord(ev[4 * j], st);
+            if (timed && j == 0) hipEventRecord(ev[0], st);
             ++L;
-            if (mixed) hipLaunchKernelGGL(k_pcg_spmv<float>, grid, block, 0, st, a, L);
-            else hipLaunchKernelGGL(k_pcg_spmv<double>, grid, block, 0, st, a, L);
-            if (timed) hipEventRecord(ev[4 * j + 1], st);
+            if (mixed) hipLaunchKernelGGL((k_pcg_dirspmv<float, true>), grid, block, 0, st, a, L);
+            else hipLaunchKernelGGL((k_pcg_dirspmv<double, false>), grid, block, 0, st, a, L);
+            if (timed) hipEventRecord(ev[2 * j + 1], st);
             ++L;
             if (mixed) hipLaunchKernelGGL((k_pcg_update<float, true>), grid, block, 0, st, a, L);
             else hipLaunchKernelGGL((k_pcg_update<double, false>), grid, block, 0, st, a, L);
-            if (timed) hipEventRecord(ev[4 * j + 2], st);
-            ++L;
-            if (mixed) hipLaunchKernelGGL((k_pcg_dir<float, true>), grid, block, 0, st, a, L);
-            else hipLaunchKernelGGL((k_pcg_dir<double, false>), grid, block, 0, st, a, L);
-            if (timed) hipEventRecord(ev[4 * j + 3], st);
+            if (timed) hipEventRecord(ev[2 * j + 2], st);
         }
         launched += chunk;
+        // Chunk end: take the last update's decisions (and, in mixed mode,
+        // replace the residual when due) so the host sees a settled log.
+        ++L;
+        if (mixed) hipLaunchKernelGGL(k_pcg_replace<true>, grid, block, 0, st, a, L);
+        else hipLaunchKernelGGL(k_pcg_replace<false>, grid, block, 0, st, a, L);
         if (mixed) {
-            if (timed) hipEventRecord(ev[4 * chunk], st);
-            ++L;
-            hipLaunchKernelGGL(k_pcg_replace, grid, block, 0, st, a, L);
             ++L;
             hipLaunchKernelGGL(k_pcg_fin, dim3(1), block, 0, st, a, L);
-            if (timed) hipEventRecord(ev[4 * chunk + 1], st);
         }
+        if (timed) hipEventRecord(ev[2 * chunk + 1], st);
         rc = gf::check_launch("pcg iteration");
         if (rc) break;
         e = hipMemcpyAsync(&hs, &a.log[L & 1], sizeof(PcgLog), hipMemcpyDeviceToHost, st);
@@ -657,17 +724,15 @@ extern "C" int gf_scene_pcg_sync(const gf_scene_t* s, int n_active, int precisio
         if (timed) {
             const int real = hs.iters - iters_before;
             for (int j = 0; j < real && j < chunk; ++j) {
-                float t0 = 0.f, t1 = 0.f, t2 = 0.f;
-                hipEventElapsedTime(&t0, ev[4 * j], ev[4 * j + 1]);
-                hipEventElapsedTime(&t1, ev[4 * j + 1], ev[4 * j + 2]);
-                hipEventElapsedTime(&t2, ev[4 * j + 2], ev[4 * j + 3]);
+                float t0 = 0.f, t1 = 0.f;
+                hipEventElapsedTime(&t0, ev[2 * j], ev[2 * j + 1]);
+                hipEventElapsedTime(&t1, ev[2 * j + 1], ev[2 * j + 2]);
                 kernel_ms[0] += t0;
                 kernel_ms[1] += t1;
-                kernel_ms[2] += t2;
             }
             if (mixed && hs.nupd > nupd_before) {
                 float t3 = 0.f;
-                hipEventElapsedTime(&t3, ev[4 * chunk], ev[4 * chunk + 1]);
+                hipEventElapsedTime(&t3, ev[2 * chunk], ev[2 * chunk + 1]);
                 kernel_ms[3] += t3;
             }
         }

@@ -40,11 +40,12 @@ class SceneBuffers:
         self.x32 = torch.zeros(sz.compact, **f32)
         self.r32 = torch.zeros(sz.compact, **f32)
         self.p32 = torch.zeros(sz.compact, **f32)
+        self.p32b = torch.zeros(sz.compact, **f32)
         self.ap32 = torch.zeros(sz.compact, **f32)
         self.pc = torch.zeros(sz.compact, **f32)
         self.ptmp = torch.zeros(sz.compact, **f32)
         self.x64 = torch.zeros(sz.compact, **f64)
-        self.r64 = self.p64 = self.ap64 = None            # FP64 PCG, allocated on first use
+        self.r64 = self.p64 = self.p64b = self.ap64 = None  # FP64 PCG, allocated on first use
         self.partials = torch.zeros(sz.partials, **f64)
         self.scalars = torch.zeros(sz.scalars, **f64)
         self.cap = 0
@@ -58,6 +59,7 @@ class SceneBuffers:
             kw = dict(dtype=torch.float64, device=self.device)
             self.r64 = torch.zeros(self.compact, **kw)
             self.p64 = torch.zeros(self.compact, **kw)
+            self.p64b = torch.zeros(self.compact, **kw)
             self.ap64 = torch.zeros(self.compact, **kw)
 
     def _alloc(self, cap, affine):
@@ -131,6 +133,7 @@ class SceneBuffers:
         s.x64, s.r64, s.p64, s.ap64 = P(self.x64), P(self.r64), P(self.p64), P(self.ap64)
         s.pc, s.ptmp = P(self.pc), P(self.ptmp)
         s.partials, s.scalars = P(self.partials), P(self.scalars)
+        s.p32b, s.p64b = P(self.p32b), P(self.p64b)
         return s
 
 

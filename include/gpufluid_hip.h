@@ -97,11 +97,12 @@ typedef struct gf_scene {
     /* compact pressure storage, (ntiles*512) entries, slot-major */
     uint8_t* flags;              /* bit3 fluid, bits0-2 non-solid neighbour count */
     float* b;                    /* divergence (reference scaling) */
-    float* x32; float* r32; float* p32; float* ap32;  /* FP32 PCG vectors */
+    float* x32; float* r32; float* p32; float* ap32;  /* FP32 PCG vectors (r32/r64 hold z = r/max(diag,1)) */
     double* x64; double* r64; double* p64; double* ap64; /* FP64 vectors */
     float* pc;  float* ptmp;     /* pressure result / Jacobi ping-pong */
     double* partials;            /* (gf_scene_sizes.partials) per-workgroup partial sums */
     double* scalars;             /* (gf_scene_sizes.scalars) PCG solver log */
+    float* p32b; double* p64b;   /* second PCG direction buffer (ping-pong with p32/p64) */
 } gf_scene_t;
 
 typedef struct gf_scene_size_info {
@@ -155,8 +156,9 @@ int gf_scene_jacobi(const gf_scene_t* s, int n_active, int iters, void* stream);
  * Result in s->pc.  Host outputs (may be NULL): iterations, true relative
  * residual ||b-Ap||/||b|| (FP64; mixed mode only, else the iterated one),
  * number of FP64 residual replacements, and kernel_ms[4]: summed GPU time of
- * the spmv, update, direction and residual-replacement kernels over the
- * launches that did work (HIP events on `stream`; NULL disables timing). */
+ * the fused direction+SpMV kernel [0], the update kernel [1] and the
+ * residual-replacement kernels [3] over the launches that did work ([2] is 0;
+ * HIP events on `stream`; NULL disables timing). */
 int gf_scene_pcg_sync(const gf_scene_t* s, int n_active, int precision, double tol,
                       int max_iter, int check_every, int rhs_sign, int* iters,
                       double* rel_residual, int* replacements, double* kernel_ms,
