@@ -202,6 +202,9 @@ def main():
                 "pcg_total_GBps": round(pcg_bytes / (pcg_ms * 1e-3) / 1e9, 1),
                 "pcg_total_frac": round(pcg_bytes / (pcg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 
+    pcie = pcie_inclusive(s, npart, ms_step)
+    log(f"pcie-inclusive: {pcie}")
+
     fp64 = None
     if not args.no_fp64_compare:
         fp64 = compare_fp64(s, tol, max_iter, args.check_every)
@@ -234,8 +237,34 @@ def main():
         "cpu_baseline": cpu,
         "stages": stages,
         "pcg_vs_fp64": fp64,
+        "pcie_inclusive_value": pcie,
     }
     print(json.dumps(out), flush=True)
+
+
+def pcie_inclusive(s, npart, ms_step):
+    """Rate a caller would see if it handed pos+vel over from host memory every
+    substep: one pinned upload and one download of (N,3)+(N,3) FP32, timed
+    with HIP events, added to the device-resident step time."""
+    import torch
+    dev = s.device
+    host = torch.empty((2, npart, 3), dtype=torch.float32, pin_memory=True)
+    buf = torch.empty((2, npart, 3), dtype=torch.float32, device=dev)
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    c = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    a.record()
+    buf.copy_(host, non_blocking=True)
+    b.record()
+    host.copy_(buf, non_blocking=True)
+    c.record()
+    torch.cuda.synchronize()
+    h2d, d2h = a.elapsed_time(b), b.elapsed_time(c)
+    del host, buf
+    ms = ms_step + h2d + d2h
+    return {"value": round(npart / (ms * 1e-3) / 1e6, 2), "unit": "Mparticle-substeps/s",
+            "h2d_ms": round(h2d, 3), "d2h_ms": round(d2h, 3), "bytes_each_way": npart * 24}
 
 
 def compare_fp64(s, tol, max_iter, check_every):

@@ -21,6 +21,6 @@ def __getattr__(name):
         from . import solver3d
         return getattr(solver3d, name)
     if name == "kernels":
-        from . import kernels
-        return kernels
+        import importlib
+        return importlib.import_module(".kernels", __name__)
     raise AttributeError(name)

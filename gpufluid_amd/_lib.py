@@ -18,6 +18,7 @@ GF_TRANSFER_FLIP = 0
 GF_TRANSFER_APIC = 1
 GF_PCG_MIXED = 0
 GF_PCG_FP64 = 1
+GF_EINVAL, GF_ERANGE, GF_ESTATE = -1, -2, -3  # negative return codes of the C-ABI
 TILE = 8
 TILE_CELLS = 512
 

@@ -52,8 +52,9 @@ def _cells(marker):
     return tuple(int(x) for x in m.shape)
 
 
-@block("S2.4", "clear grid: zero faces/weights, non-solid marker -> air")
 def k3_clear_grid(u, v, w, uw, vw, ww, marker, off_x=0, off_y=0, off_z=0):
+    """k3_clear_grid (solver3d.py:41-65; not registered as a block there):
+    zero the face and weight arrays and reset non-solid marker cells to air."""
     _offsets("S2.4", off_x, off_y, off_z)
     nx, ny, nz = _cells(marker)
     _lib.call("S2.4", "gf_k3_clear_grid", _p(u), _p(v), _p(w), _p(uw), _p(vw), _p(ww),
