@@ -40,6 +40,28 @@ BYTES = {
 }
 
 
+# Measured HBM traffic per dispatch (rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled
+# by a plain-copy calibration) for this workload; the file states its commands.
+TRAFFIC_FILE = ROOT / "profiles" / "pmc_traffic_substep_pool_sphere_256.txt"
+TRAFFIC_KERNEL = {"pcg_dirspmv": "k_pcg_dirspmv<float,true>", "pcg_update": "k_pcg_update<float,true>",
+                  "pcg_replace": "k_pcg_replace<true>", "p2g": "k_p2g_tile<false,false>",
+                  "g2p": "k_g2p_tile<false>"}
+
+
+def measured_traffic(stage):
+    """HBM bytes per launch of a stage's kernel from the committed PMC summary, or None."""
+    try:
+        lines = TRAFFIC_FILE.read_text().splitlines()
+    except OSError:
+        return None
+    want = TRAFFIC_KERNEL.get(stage)
+    for line in lines:
+        f = line.split()
+        if len(f) == 5 and f[0] == want:
+            return int(f[4])
+    return None
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -197,7 +219,10 @@ def main():
     dom = max(("pcg_dirspmv", "pcg_update"), key=lambda k: kern[k][0])
     d = stages[dom]
     roofline = {"bound": "hbm", "achieved": d["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": d["frac_hbm"], "traffic": None, "kernel": dom,
+                "frac": d["frac_hbm"],
+                "traffic": measured_traffic(dom) if args.precision == "mixed" and args.config == "c3"
+                else None,
+                "traffic_source": TRAFFIC_FILE.name, "kernel": dom,
                 "alg_bytes_per_launch": d["alg_bytes"], "avg_launch_ms": d["avg_ms"],
                 "pcg_total_GBps": round(pcg_bytes / (pcg_ms * 1e-3) / 1e9, 1),
                 "pcg_total_frac": round(pcg_bytes / (pcg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
