@@ -53,6 +53,23 @@ class GfScene(ctypes.Structure):
     ]
 
 
+class GfAdjoint(ctypes.Structure):
+    """Mirror of gf_adjoint_t (include/gpufluid_hip.h)."""
+
+    _fields_ = [
+        ("nx", c_int32), ("ny", c_int32), ("nz", c_int32),
+        ("dx", c_float), ("dom", c_float * 3), ("n", c_int32),
+        ("pos", c_void_p), ("vel", c_void_p), ("pid", c_void_p),
+        ("marker", c_void_p), ("tile_map", c_void_p),
+        ("u", c_void_p), ("v", c_void_p), ("w", c_void_p),
+        ("us", c_void_p), ("vs", c_void_p), ("ws", c_void_p),
+        ("mu", c_void_p), ("mv", c_void_p), ("mw", c_void_p),
+        ("au", c_void_p), ("av", c_void_p), ("aw", c_void_p),
+        ("aus", c_void_p), ("avs", c_void_p), ("aws", c_void_p),
+        ("apos", c_void_p), ("avel", c_void_p),
+    ]
+
+
 class GfSizes(ctypes.Structure):
     _fields_ = [("cells", c_int64), ("ntiles", c_int64), ("compact", c_int64),
                 ("scan_parts", c_int64), ("partials", c_int64), ("scalars", c_int64)]
@@ -61,6 +78,7 @@ class GfSizes(ctypes.Structure):
 V, I, F, D = c_void_p, c_int, c_float, c_double
 PI, PD = POINTER(c_int), POINTER(c_double)
 SP = POINTER(GfScene)
+AP = POINTER(GfAdjoint)
 
 # name -> argtypes (all return int)
 SIGNATURES = {
@@ -92,6 +110,12 @@ SIGNATURES = {
     "gf_k3_g2p_and_advect": [V, V, V, V, V, V, V, V, I, F, F, I, I, I, F, POINTER(c_float), V],
     "gf_k3_g2p_apic_advect": [V, V, V, V, V, V, I, F, F, I, I, I, POINTER(c_float), V],
     "gf_k3_vmax_sq_reduce": [V, I, V, V],
+    "gf_adj_clear": [AP, V],
+    "gf_adj_weights": [AP, V],
+    "gf_adj_g2p": [AP, F, F, V],
+    "gf_adj_pressure_rhs": [AP, F, F, V, V],
+    "gf_adj_faces": [AP, V, F, F, V],
+    "gf_adj_p2g": [AP, V],
 }
 
 

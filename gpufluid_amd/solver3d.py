@@ -334,7 +334,10 @@ class FlipSolver3D:
         tol = self.pcg_tol if pressure_tol is None else float(pressure_tol)
         self._step_impl(float(dt), int(pressure_iters), pressure_solver, tol)
 
-    def _step_impl(self, dt, pressure_iters, pressure_solver, tol):
+    def _step_impl(self, dt, pressure_iters, pressure_solver, tol, hook=None):
+        """The substep pipeline.  ``hook(stage)``, if given, is called after
+        the sort ("sorted": particles in this substep's cell order) and at the
+        end ("done"); differentiable rollouts record forward state there."""
         prof = self._prof
         sc = self._sc
         apic = self.transfer_mode == "apic"
@@ -357,6 +360,9 @@ class FlipSolver3D:
                 sc.swap()
             sc.n_active_host = na[0]
             self._sc_fluid_cells = na[1]
+        if hook is not None:
+            hook("sorted")       # may substitute the masks and tile tables (frozen structure)
+        na[0] = sc.n_active_host
         s = self._descriptor()
         with prof.section("p2g"):
             _lib.call("S2.1", "gf_scene_p2g", ctypes.byref(s), mode, self.gravity, dt, self.rho,
@@ -394,6 +400,8 @@ class FlipSolver3D:
                       self.gravity, dt, self.rho, flip_blend, stream)
         prof.fused("grad_subtract_bc")
         prof.fused("color")
+        if hook is not None:
+            hook("done")
 
     @block("F3.4", "CFL-adaptive substepping")
     def step_cfl(self, target_dt: float, pressure_iters: int = 80, cfl: float = 0.5,

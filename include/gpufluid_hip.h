@@ -58,6 +58,11 @@ int gf_init(int device, int* cus);
 #define GF_TILE 8
 #define GF_TILE_CELLS 512
 
+/* Marker codes (solver3d.py:1-5). */
+#define GF_AIR 0
+#define GF_FLUID 1
+#define GF_SOLID 2
+
 /* Transfer modes (FlipSolver3D.transfer_mode, solver3d.py:2395). */
 #define GF_TRANSFER_FLIP 0   /* FLIP/PIC blend by flip_blend, solver3d.py:1414-1463 */
 #define GF_TRANSFER_APIC 1   /* APIC, solver3d.py:1475-1732 */
@@ -251,6 +256,53 @@ int gf_k3_g2p_apic_advect(float* pos, float* vel, float* affine_c, const float* 
                           int nx, int ny, int nz, const float* dom, void* stream);
 /* k3_vmax_sq_reduce, solver3d.py:2241-2252: out[0] = max(out[0], max |v|^2) */
 int gf_k3_vmax_sq_reduce(const float* vel, int n, float* out, void* stream);
+
+/* ---- adjoint of a FLIP/PIC substep (differentiable rollouts) ------------
+ * No counterpart in the reference (its solver is not differentiable).  A
+ * substep maps particle state (x, v) to (x', v'); these entry points apply its
+ * transpose Jacobian, with the fluid/solid masks and the cell binning held
+ * fixed.  The descriptor holds the substep's recorded forward state, in the
+ * cell-sorted particle order that gf_scene_sort_sync produced:
+ *   pos/vel/pid     the substep's inputs after the sort (pid: slot -> particle id)
+ *   marker/tile_map the substep's masks and compact tile map
+ *   u,v,w           faces after projection and BC (what G2P sampled)
+ *   us,vs,ws        normalised faces before gravity (what G2P subtracted)
+ * Work arrays (dense face shapes): mu/mv/mw weight sums, au/av/aw and
+ * aus/avs/aws adjoint accumulators.  apos/avel (n,3) are the particle
+ * adjoints indexed by particle id: the adjoint of (x', v') on entry to
+ * gf_adj_g2p, the adjoint of (x, v) after gf_adj_p2g.  Order per substep:
+ *   gf_adj_clear, gf_adj_weights, gf_adj_g2p, gf_adj_pressure_rhs,
+ *   gf_scene_pcg_sync (same flags/nbr as the forward solve, b = p-bar, same
+ *   rhs_sign), gf_adj_faces (b-bar = the PCG result), gf_adj_p2g. */
+typedef struct gf_adjoint {
+    int32_t nx, ny, nz;
+    float dx;
+    float dom[3];
+    int32_t n;
+    const float* pos; const float* vel; const int32_t* pid;
+    const int32_t* marker; const int32_t* tile_map;
+    const float* u; const float* v; const float* w;
+    const float* us; const float* vs; const float* ws;
+    float* mu; float* mv; float* mw;
+    float* au; float* av; float* aw;
+    float* aus; float* avs; float* aws;
+    float* apos; float* avel;
+} gf_adjoint_t;
+
+/* Zero the nine face work arrays. */
+int gf_adj_clear(const gf_adjoint_t* a, void* stream);
+/* Rebuild the P2G weight sums m (scatter_face weights, gridmath.py:71-95). */
+int gf_adj_weights(const gf_adjoint_t* a, void* stream);
+/* Reverse of G2P + advection + clamp (solver3d.py:1414-1463): updates
+ * apos/avel in place and scatters into au/av/aw and aus/avs/aws. */
+int gf_adj_g2p(const gf_adjoint_t* a, float flip_blend, float dt, void* stream);
+/* Reverse of the gradient subtract (solver3d.py:1364-1407): compact p-bar. */
+int gf_adj_pressure_rhs(const gf_adjoint_t* a, float dt, float rho, float* pbar, void* stream);
+/* Reverse of BC, divergence, gravity and normalize (solver3d.py:121-156,
+ * 758-799, 846-871): overwrites au/av/aw with q-bar and aus/avs/aws with m-bar. */
+int gf_adj_faces(const gf_adjoint_t* a, const float* bbar, float dt, float rho, void* stream);
+/* Reverse of the P2G scatter (solver3d.py:72-114): adds into apos/avel. */
+int gf_adj_p2g(const gf_adjoint_t* a, void* stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,212 @@
+"""FP64 PyTorch-CPU restatement of the FLIP/PIC substep, differentiable by
+torch.autograd — TEST INFRASTRUCTURE.
+
+The checker of the hand-written adjoint (gpufluid_amd/adjoint.py); only
+``tests/`` and ``bench.py``'s checks import it.  It restates the same
+arithmetic as oracle/flip_oracle.py (solver3d.py:72-156, 758-799, 846-871,
+1364-1463; gridmath.py:30-95) with torch tensors so that autograd gives the
+reference gradient of a rollout.  The pressure solve A p = -b (A: the 7-point
+matrix over fluid cells, built by ``flip_oracle.poisson_matrix``) is an
+autograd Function whose backward is a second solve with the same matrix.
+Masks and binning are computed from the FP32 positions with the oracle's
+functions and held fixed, exactly as the adjoint does.
+
+Pinned by tests/test_oracle_adjoint.py: the forward equals the numpy oracle's
+substep, and the autograd gradient equals central finite differences of this
+same FP64 forward.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import flip_oracle as fo
+
+try:
+    import scipy.sparse.linalg as _spla
+except Exception:  # pragma: no cover
+    _spla = None
+
+F64 = torch.float64
+
+
+def _solve(A, rhs):
+    """FP64 solve of the SPD system A x = rhs to 1e-13 (CG, Jacobi-preconditioned)."""
+    if rhs.size == 0:
+        return rhs.copy()
+    d = A.diagonal()
+    M = _spla.LinearOperator(A.shape, matvec=lambda y: y / d)
+    x, info = _spla.cg(A, rhs, rtol=1e-13, atol=0.0, maxiter=20000, M=M)
+    if info != 0:
+        x = _spla.spsolve(A.tocsc(), rhs)
+    return x
+
+
+class _Poisson(torch.autograd.Function):
+    """p = sign * A^-1 b on fluid cells (sign = -1: the physical pressure)."""
+
+    @staticmethod
+    def forward(ctx, b, A, sign):
+        ctx.A, ctx.sign = A, sign
+        return torch.from_numpy(sign * _solve(A, b.detach().numpy()))
+
+    @staticmethod
+    def backward(ctx, g):
+        return torch.from_numpy(ctx.sign * _solve(ctx.A, g.detach().numpy())), None, None
+
+
+def _face_masks(marker):
+    """BC mask Mb (1 = face left free by the solid BC) per component."""
+    ones = [np.ones(s) for s in fo._face_shapes(marker.shape)]
+    return [torch.from_numpy(m.astype(np.float64)) for m in fo.enforce_solid_bc(*ones, marker)]
+
+
+def _p2g(x, v, shape, dx):
+    """Raw scatter (q, m) per component; corners outside the array dropped."""
+    l = x / dx
+    out = []
+    for comp, ext in enumerate(fo._face_shapes(shape)):
+        c = l - torch.tensor([0.0 if ax == comp else 0.5 for ax in range(3)], dtype=F64)
+        i0 = torch.floor(c).detach().long()
+        f = c - i0
+        size = ext[0] * ext[1] * ext[2]
+        q = torch.zeros(size, dtype=F64)
+        m = torch.zeros(size, dtype=F64)
+        for di in (0, 1):
+            for dj in (0, 1):
+                for dk in (0, 1):
+                    ii, jj, kk = i0[:, 0] + di, i0[:, 1] + dj, i0[:, 2] + dk
+                    ok = ((ii >= 0) & (ii < ext[0]) & (jj >= 0) & (jj < ext[1]) & (kk >= 0)
+                          & (kk < ext[2]))
+                    w = ((f[:, 0] if di else 1 - f[:, 0]) * (f[:, 1] if dj else 1 - f[:, 1])
+                         * (f[:, 2] if dk else 1 - f[:, 2]))
+                    idx = ((ii * ext[1] + jj) * ext[2] + kk)[ok]
+                    q = q.index_add(0, idx, (w * v[:, comp])[ok])
+                    m = m.index_add(0, idx, w[ok])
+        out.append((q.reshape(ext), m.reshape(ext)))
+    return out
+
+
+def _sample3(f, fx, fy, fz):
+    """sample3 of gridmath.py:30-65 (coordinate, index and weight clamps)."""
+    nx, ny, nz = f.shape
+    fx = torch.clamp(fx, 0, nx)
+    fy = torch.clamp(fy, 0, ny)
+    fz = torch.clamp(fz, 0, nz)
+    i0 = torch.clamp(torch.floor(fx).detach().long(), 0, nx - 1)
+    j0 = torch.clamp(torch.floor(fy).detach().long(), 0, ny - 1)
+    k0 = torch.clamp(torch.floor(fz).detach().long(), 0, nz - 1)
+    i1 = torch.clamp(i0 + 1, max=nx - 1)
+    j1 = torch.clamp(j0 + 1, max=ny - 1)
+    k1 = torch.clamp(k0 + 1, max=nz - 1)
+    sx = torch.clamp(fx - i0, 0, 1)
+    sy = torch.clamp(fy - j0, 0, 1)
+    sz = torch.clamp(fz - k0, 0, 1)
+    flat = f.reshape(-1)
+
+    def at(i, j, k):
+        return flat[(i * ny + j) * nz + k]
+
+    c00 = at(i0, j0, k0) * (1 - sx) + at(i1, j0, k0) * sx
+    c10 = at(i0, j1, k0) * (1 - sx) + at(i1, j1, k0) * sx
+    c01 = at(i0, j0, k1) * (1 - sx) + at(i1, j0, k1) * sx
+    c11 = at(i0, j1, k1) * (1 - sx) + at(i1, j1, k1) * sx
+    c0 = c00 * (1 - sy) + c10 * sy
+    c1 = c01 * (1 - sy) + c11 * sy
+    return c0 * (1 - sz) + c1 * sz
+
+
+def substep(x, v, walls, dx, dt, gravity=-9.81, rho=1.0, flip_blend=0.95):
+    """One FLIP substep (FP64 torch): returns (x', v').  ``walls`` is the
+    static marker (solids); the fluid mask comes from the FP32 positions."""
+    shape = walls.shape
+    marker = fo.update_marker(walls, x.detach().numpy().astype(np.float32), dx)
+    fluid = torch.from_numpy((marker == fo.FLUID).astype(np.float64))
+    dxf = float(np.float32(dx))
+    (qu, mu), (qv, mv), (qw, mw) = _p2g(x, v, shape, dxf)
+
+    def norm(q, m):
+        ok = m > 1e-8
+        return torch.where(ok, q / torch.where(ok, m, torch.ones_like(m)), torch.zeros_like(q))
+
+    un, vn, wn = norm(qu, mu), norm(qv, mv), norm(qw, mw)
+    mbu, mbv, mbw = _face_masks(marker)
+    u = un * mbu
+    vv = (vn + gravity * dt) * mbv
+    w = wn * mbw
+    sd = rho * dxf / dt
+    div = ((u[1:] - u[:-1]) + (vv[:, 1:] - vv[:, :-1]) + (w[:, :, 1:] - w[:, :, :-1])) * sd * fluid
+    A, cells, _ = fo.poisson_matrix(marker)
+    p = torch.zeros(shape, dtype=F64)
+    if len(cells):
+        ci = tuple(torch.from_numpy(c) for c in cells.T)
+        pf = _Poisson.apply(div[ci], A, -1.0)
+        p = p.index_put(ci, pf)
+    sg = dt / (rho * dxf)
+    faces = []
+    for comp, f in ((0, u), (1, vv), (2, w)):
+        n = shape[comp]
+        lo = [slice(None)] * 3
+        hi = [slice(None)] * 3
+        lo[comp] = slice(0, n - 1)
+        hi[comp] = slice(1, n)
+        ml, mr = marker[tuple(lo)], marker[tuple(hi)]
+        act = torch.from_numpy((((ml == fo.FLUID) | (mr == fo.FLUID)) & (ml != fo.SOLID)
+                                & (mr != fo.SOLID)).astype(np.float64))
+        pl = p[tuple(lo)] * torch.from_numpy((ml == fo.FLUID).astype(np.float64))
+        pr = p[tuple(hi)] * torch.from_numpy((mr == fo.FLUID).astype(np.float64))
+        upd = -sg * (pr - pl) * act
+        pad = [0, 0, 0, 0, 0, 0]
+        pad[2 * (2 - comp)] = 1
+        pad[2 * (2 - comp) + 1] = 1
+        faces.append(f + torch.nn.functional.pad(upd, pad))
+    mb = (mbu, mbv, mbw)
+    u2, v2, w2 = (faces[c] * mb[c] for c in range(3))
+    l = x / dxf
+    h = 0.5
+    lx, ly, lz = l[:, 0], l[:, 1], l[:, 2]
+    pic = torch.stack([_sample3(u2, lx, ly - h, lz - h), _sample3(v2, lx - h, ly, lz - h),
+                       _sample3(w2, lx - h, ly - h, lz)], dim=1)
+    old = torch.stack([_sample3(un, lx, ly - h, lz - h), _sample3(vn, lx - h, ly, lz - h),
+                       _sample3(wn, lx - h, ly - h, lz)], dim=1)
+    nv = flip_blend * (v + (pic - old)) + (1 - flip_blend) * pic
+    eps = float(np.float32(dx) * np.float32(1.001))
+    dom = [float(np.float32(n * dx)) for n in shape]
+    xp = x + nv * dt
+    nx_ = torch.stack([torch.clamp(xp[:, a], eps, dom[a] - eps) for a in range(3)], dim=1)
+    return nx_, nv
+
+
+def rollout(x0, v0, walls, dx, dt, substeps, checkpoint=True, **kw):
+    """``substeps`` substeps from (x0, v0) (FP64 tensors); with ``checkpoint``
+    every substep is recomputed in the backward (torch.utils.checkpoint)."""
+    x, v = x0, v0
+    for _ in range(substeps):
+        if checkpoint and (x.requires_grad or v.requires_grad):
+            x, v = torch.utils.checkpoint.checkpoint(
+                lambda a, b: substep(a, b, walls, dx, dt, **kw), x, v, use_reentrant=False)
+        else:
+            x, v = substep(x, v, walls, dx, dt, **kw)
+    return x, v
+
+
+def vjp_along(states, walls, dx, dt, grad_x, grad_v=None, **kw):
+    """FP64 reverse pass along a given trajectory: ``states`` holds (x_k, v_k)
+    at the start of each substep (e.g. the GPU's, caller order); each substep
+    is re-evaluated in FP64 from that state and its vector-Jacobian product
+    applied.  Returns (dL/dx_0, dL/dv_0) as numpy arrays.  This isolates the
+    adjoint's arithmetic from the FP32/FP64 drift of the trajectory itself."""
+    gx = torch.as_tensor(np.asarray(grad_x, dtype=np.float64))
+    gv = torch.zeros_like(gx) if grad_v is None else torch.as_tensor(np.asarray(grad_v, np.float64))
+    for xk, vk in reversed(states):
+        x = torch.as_tensor(np.asarray(xk, dtype=np.float32).astype(np.float64)).requires_grad_()
+        v = torch.as_tensor(np.asarray(vk, dtype=np.float32).astype(np.float64)).requires_grad_()
+        x1, v1 = substep(x, v, walls, dx, dt, **kw)
+        gx, gv = torch.autograd.grad((x1, v1), (x, v), (gx, gv))
+    return gx.numpy(), gv.numpy()
+
+
+def com_loss(x, target):
+    """Squared distance of the centre of mass from ``target``."""
+    t = torch.as_tensor(target, dtype=F64)
+    return ((x.mean(dim=0) - t) ** 2).sum()

@@ -112,7 +112,8 @@ def _c_layout(struct_name, fields):
 
 
 @pytest.mark.parametrize("cname,pyname", [("gf_scene_t", "GfScene"),
-                                          ("gf_scene_size_info", "GfSizes")])
+                                          ("gf_scene_size_info", "GfSizes"),
+                                          ("gf_adjoint_t", "GfAdjoint")])
 def test_ctypes_struct_layout_matches_c(cname, pyname):
     from gpufluid_amd import _lib
     cls = getattr(_lib, pyname)
