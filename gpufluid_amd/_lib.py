@@ -83,6 +83,8 @@ AP = POINTER(GfAdjoint)
 # name -> argtypes (all return int)
 SIGNATURES = {
     "gf_version": [],
+    "gf_tune_set": [ctypes.c_char_p, I],
+    "gf_tune_get": [ctypes.c_char_p, PI],
     "gf_init": [I, PI],
     "gf_scene_sizes": [I, I, I, POINTER(GfSizes)],
     "gf_scene_sort_sync": [SP, I, PI, V],

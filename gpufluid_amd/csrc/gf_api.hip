@@ -1,7 +1,9 @@
 // Library-level entry points: error reporting, device init, scene sizes.
 #include "gf_common.h"
+#include <atomic>
 #include <stdarg.h>
 #include <stdio.h>
+#include <string.h>
 
 namespace gf {
 
@@ -24,7 +26,34 @@ int check_launch(const char* what) {
     return 0;
 }
 
+// Launch-shape tunables (0 = the built-in choice).  Read at every call.
+static std::atomic<int> g_tunables[GF_TUNE_COUNT];
+
+int tunable(int id) { return g_tunables[id].load(std::memory_order_relaxed); }
+
 }  // namespace gf
+
+static int tune_id(const char* key) {
+    if (!key) return -1;
+    if (!strcmp(key, "pcg_workgroups")) return gf::GF_TUNE_PCG_WORKGROUPS;
+    return -1;
+}
+
+extern "C" int gf_tune_set(const char* key, int value) {
+    const int id = tune_id(key);
+    if (id < 0) return gf::set_error(GF_EINVAL, "gf_tune_set: unknown key '%s'", key ? key : "(null)");
+    if (value < 0) return gf::set_error(GF_EINVAL, "gf_tune_set: %s = %d", key, value);
+    gf::g_tunables[id].store(value, std::memory_order_relaxed);
+    return 0;
+}
+
+extern "C" int gf_tune_get(const char* key, int* value) {
+    const int id = tune_id(key);
+    if (id < 0 || !value)
+        return gf::set_error(GF_EINVAL, "gf_tune_get: unknown key '%s'", key ? key : "(null)");
+    *value = gf::tunable(id);
+    return 0;
+}
 
 extern "C" int gf_version(void) { return 10000; }
 

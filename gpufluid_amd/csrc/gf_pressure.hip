@@ -45,6 +45,7 @@ static_assert(sizeof(PcgLog) == 16 * sizeof(double), "log entry is 16 doubles");
 // Per-workgroup partial sums, PCG_GMAX entries per kind.
 enum { P_PAP = 0, P_RZ, P_RR, P_BB, P_RRT, P_RZT, P_KINDS };
 constexpr int PCG_GMAX = 2048;
+constexpr int PCG_DEFAULT_G = 1024;
 
 struct PcgArgs {
     const uint8_t* flags;
@@ -587,6 +588,23 @@ __global__ void k_fill_f32(float* a, size_t n, float v) {
 
 namespace gf {
 int scene_check(const gf_scene_t* s, const char* who);
+
+// Two solver-log slots in pinned host memory per host thread, allocated on
+// first use and kept for the life of the thread (the PCG's only host
+// allocation; it never touches device memory).
+static PcgLog* pinned_pcg_logs() {
+    static thread_local PcgLog* logs = nullptr;
+    if (!logs) {
+        void* p = nullptr;
+        hipError_t e = hipHostMalloc(&p, 2 * sizeof(PcgLog), hipHostMallocDefault);
+        if (e != hipSuccess) {
+            set_error((int)e, "hipHostMalloc: %s", hipGetErrorString(e));
+            return nullptr;
+        }
+        logs = static_cast<PcgLog*>(p);
+    }
+    return logs;
+}
 }
 
 extern "C" int gf_scene_jacobi(const gf_scene_t* s, int n_active, int iters, void* stream) {
@@ -650,7 +668,11 @@ extern "C" int gf_scene_pcg_sync(const gf_scene_t* s, int n_active, int precisio
     a.pc = s->pc; a.part = s->partials; a.log = (PcgLog*)s->scalars;
     a.nact = n_active;
     const int pairs = (n_active + 1) / 2;
-    a.G = pairs < PCG_GMAX ? pairs : PCG_GMAX;
+    // 1024 workgroups (4 per CU) measured best on MI355X at 256^3 (5.4 M
+    // active cells); the autotuner may choose another value up to PCG_GMAX.
+    int gmax = gf::tunable(gf::GF_TUNE_PCG_WORKGROUPS);
+    gmax = gmax > 0 ? (gmax < PCG_GMAX ? gmax : PCG_GMAX) : PCG_DEFAULT_G;
+    a.G = pairs < gmax ? pairs : gmax;
     a.rhs_sign = (float)rhs_sign;
     a.tol2 = tol * tol;
     a.delta2 = 0.01;
@@ -666,43 +688,61 @@ extern "C" int gf_scene_pcg_sync(const gf_scene_t* s, int n_active, int precisio
     hipLaunchKernelGGL(k_pcg_fin, dim3(1), block, 0, st, a, L);
     GF_LAUNCH_CHECK("k_pcg_setup/fin");
 
+    // The host reads the solver log back once per chunk of check_every
+    // iterations.  Chunks are pipelined: chunk c+1 is enqueued before the host
+    // waits for chunk c's log (copied into pinned memory), so the device does
+    // not idle during the round trip; once the solve has stopped, the kernels
+    // of the chunk already in flight pass through.
     // Optional per-kernel timing of the iterations that did work: events
-    // around every launch of a chunk, read back at the chunk's host sync.
+    // around every launch of a chunk, read when the chunk's log is read.
     // kernel_ms: [0] dirspmv, [1] update, [2] unused, [3] chunk-end replacement.
+    PcgLog* hlog = gf::pinned_pcg_logs();
+    if (!hlog) return gf::set_error(GF_ESTATE, "gf_scene_pcg_sync: pinned log: %s", gf_last_error());
     const bool timed = kernel_ms != nullptr;
-    hipEvent_t ev[2 * 64 + 3];
-    int nev = 0;
-    hipError_t e;
-    if (timed) {
-        nev = 2 * check_every + 3;
-        for (int i = 0; i < nev; ++i) {
-            e = hipEventCreate(&ev[i]);
-            if (e != hipSuccess) {
-                for (int j = 0; j < i; ++j) hipEventDestroy(ev[j]);
-                return gf::set_error((int)e, "gf_scene_pcg_sync: event: %s", hipGetErrorString(e));
-            }
+    const int chunk = check_every;
+    const int nev = timed ? 2 * chunk + 2 : 0;
+    hipEvent_t ev[2][2 * 64 + 2];
+    hipEvent_t done_ev[2] = {nullptr, nullptr};
+    int created = 0;
+    hipError_t e = hipSuccess;
+    for (int sl = 0; sl < 2 && e == hipSuccess; ++sl) {
+        e = hipEventCreateWithFlags(&done_ev[sl], hipEventDisableTiming);
+        for (int i = 0; i < nev && e == hipSuccess; ++i) {
+            e = hipEventCreate(&ev[sl][i]);
+            if (e == hipSuccess) created = sl * nev + i + 1;
         }
+    }
+    auto destroy_events = [&]() {
+        for (int sl = 0; sl < 2; ++sl) {
+            if (done_ev[sl]) hipEventDestroy(done_ev[sl]);
+            for (int i = 0; i < nev && sl * nev + i < created; ++i) hipEventDestroy(ev[sl][i]);
+        }
+    };
+    if (e != hipSuccess) {
+        destroy_events();
+        return gf::set_error((int)e, "gf_scene_pcg_sync: event: %s", hipGetErrorString(e));
     }
     PcgLog hs;
     memset(&hs, 0, sizeof(hs));
-    e = hipMemcpyAsync(&hs, &a.log[L & 1], sizeof(PcgLog), hipMemcpyDeviceToHost, st);
+    e = hipMemcpyAsync(&hlog[0], &a.log[L & 1], sizeof(PcgLog), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) rc = gf::set_error((int)e, "gf_scene_pcg_sync: readback: %s", hipGetErrorString(e));
-    int iters_before = 0, nupd_before = 0, launched = 0;
-    while (!rc && !hs.done) {
-        const int chunk = check_every;
+    else hs = hlog[0];
+
+    int launched = 0, next_slot = 0, npend = 0, pend[2] = {0, 0};
+    auto enqueue_chunk = [&]() -> int {
+        const int sl = next_slot;
         for (int j = 0; j < chunk; ++j) {
-            if (timed && j == 0) hipEventRecord(ev[0], st);
+            if (timed && j == 0) hipEventRecord(ev[sl][0], st);
             ++L;
             if (mixed) hipLaunchKernelGGL((k_pcg_dirspmv<float, true>), grid, block, 0, st, a, L);
             else hipLaunchKernelGGL((k_pcg_dirspmv<double, false>), grid, block, 0, st, a, L);
-            if (timed) hipEventRecord(ev[2 * j + 1], st);
+            if (timed) hipEventRecord(ev[sl][2 * j + 1], st);
             ++L;
             if (mixed) hipLaunchKernelGGL((k_pcg_update<float, true>), grid, block, 0, st, a, L);
             else hipLaunchKernelGGL((k_pcg_update<double, false>), grid, block, 0, st, a, L);
-            if (timed) hipEventRecord(ev[2 * j + 2], st);
+            if (timed) hipEventRecord(ev[sl][2 * j + 2], st);
         }
-        launched += chunk;
         // Chunk end: take the last update's decisions (and, in mixed mode,
         // replace the residual when due) so the host sees a settled log.
         ++L;
@@ -712,35 +752,63 @@ extern "C" int gf_scene_pcg_sync(const gf_scene_t* s, int n_active, int precisio
             ++L;
             hipLaunchKernelGGL(k_pcg_fin, dim3(1), block, 0, st, a, L);
         }
-        if (timed) hipEventRecord(ev[2 * chunk + 1], st);
-        rc = gf::check_launch("pcg iteration");
-        if (rc) break;
-        e = hipMemcpyAsync(&hs, &a.log[L & 1], sizeof(PcgLog), hipMemcpyDeviceToHost, st);
-        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (timed) hipEventRecord(ev[sl][2 * chunk + 1], st);
+        int r = gf::check_launch("pcg iteration");
+        if (r) return r;
+        hipError_t ce = hipMemcpyAsync(&hlog[sl], &a.log[L & 1], sizeof(PcgLog),
+                                       hipMemcpyDeviceToHost, st);
+        if (ce == hipSuccess) ce = hipEventRecord(done_ev[sl], st);
+        if (ce != hipSuccess)
+            return gf::set_error((int)ce, "gf_scene_pcg_sync: readback: %s", hipGetErrorString(ce));
+        pend[npend++] = sl;
+        next_slot ^= 1;
+        launched += chunk;
+        return 0;
+    };
+    const int launch_cap = max_iter + 8 * chunk;
+    int iters_before = hs.iters, nupd_before = hs.nupd;
+    if (!rc && !hs.done) rc = enqueue_chunk();
+    while (!rc && npend > 0) {
+        if (npend < 2 && launched < launch_cap) {
+            rc = enqueue_chunk();  // look-ahead
+            if (rc) break;
+        }
+        const int sl = pend[0];
+        pend[0] = pend[1];
+        --npend;
+        e = hipEventSynchronize(done_ev[sl]);
         if (e != hipSuccess) {
             rc = gf::set_error((int)e, "gf_scene_pcg_sync: readback: %s", hipGetErrorString(e));
             break;
         }
+        hs = hlog[sl];
         if (timed) {
             const int real = hs.iters - iters_before;
             for (int j = 0; j < real && j < chunk; ++j) {
                 float t0 = 0.f, t1 = 0.f;
-                hipEventElapsedTime(&t0, ev[2 * j], ev[2 * j + 1]);
-                hipEventElapsedTime(&t1, ev[2 * j + 1], ev[2 * j + 2]);
+                hipEventElapsedTime(&t0, ev[sl][2 * j], ev[sl][2 * j + 1]);
+                hipEventElapsedTime(&t1, ev[sl][2 * j + 1], ev[sl][2 * j + 2]);
                 kernel_ms[0] += t0;
                 kernel_ms[1] += t1;
             }
             if (mixed && hs.nupd > nupd_before) {
                 float t3 = 0.f;
-                hipEventElapsedTime(&t3, ev[2 * chunk], ev[2 * chunk + 1]);
+                hipEventElapsedTime(&t3, ev[sl][2 * chunk], ev[sl][2 * chunk + 1]);
                 kernel_ms[3] += t3;
             }
         }
         iters_before = hs.iters;
         nupd_before = hs.nupd;
-        if (hs.iters >= max_iter || launched > max_iter + 2 * chunk) break;
+        if (hs.done) break;
     }
-    for (int i = 0; i < nev; ++i) hipEventDestroy(ev[i]);
+    // let the chunk still in flight (passing through) finish before its
+    // events and its pinned log slot are released
+    for (int q = 0; q < npend; ++q) {
+        hipError_t we = hipEventSynchronize(done_ev[pend[q]]);
+        if (we != hipSuccess && !rc)
+            rc = gf::set_error((int)we, "gf_scene_pcg_sync: drain: %s", hipGetErrorString(we));
+    }
+    destroy_events();
     if (rc) return rc;
     if (mixed)
         hipLaunchKernelGGL(k_pcg_final<true>, grid, block, 0, st, a);

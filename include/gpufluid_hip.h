@@ -52,6 +52,13 @@ const char* gf_last_error(void);
  * (replaces wp.init/device selection, src/gpufluid/primitives/runtime.py:13-24). */
 int gf_init(int device, int* cus);
 
+/* Launch-shape tunables, process-wide, read at every call (0 = built-in choice).
+ * Keys: "pcg_workgroups" — persistent-grid size of the PCG kernels (default
+ * 1024, maximum 2048).  Used by the autotuner (gpufluid_amd/autotune.py).  No
+ * reference counterpart (Warp picks its own launch shapes). */
+int gf_tune_set(const char* key, int value);
+int gf_tune_get(const char* key, int* value);
+
 /* ---- scene descriptor -------------------------------------------------- */
 
 /* Tile edge of the compacted pressure storage and of the P2G/G2P work units. */
