@@ -129,6 +129,8 @@ def main():
     ap.add_argument("--no-fp64-compare", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the differentiable-rollout leg (64^3, 40 substeps)")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batched-scenes leg (256 scenes, 20 substeps)")
     ap.add_argument("--adjoint-reference", action="store_true",
                     help="also compare the adjoint gradient with the FP64 torch-CPU reference "
                          "at full size (about a minute of CPU time)")
@@ -247,6 +249,12 @@ def main():
         adj = adjoint_leg(reference=args.adjoint_reference)
         log(f"adjoint: {adj}")
 
+    bat = None
+    if not args.no_batch:
+        torch.cuda.empty_cache()
+        bat = batch_leg()
+        log(f"batch: {bat}")
+
     cpu = None
     if not args.no_cpu_baseline:
         c = cpu_baseline(scene)
@@ -276,6 +284,7 @@ def main():
         "pcg_vs_fp64": fp64,
         "pcie_inclusive_value": pcie,
         "adjoint": adj,
+        "batched": bat,
     }
     print(json.dumps(out), flush=True)
 
@@ -381,6 +390,84 @@ def adjoint_leg(fd_dirs=16, eps=1e-3, reference=False):
         out["fp64_fd_eps1e-5_unit_dirs"] = fd64
         out["fp64_reference_seconds"] = round(time.perf_counter() - t0, 1)
     return out
+
+
+def batch_leg(substeps=20, alone_substeps=2, tol=1e-8, max_iter=500):
+    """256 independent scenes (edges uniform in {24,32,48,64,96}, fluid boxes
+    with sides U[0.2,0.7] from seed 4, 8 particles per cell, FLIP, dt 1/240,
+    mixed PCG to 1e-8 per scene): packed into one grid, PCG grid size chosen
+    by the autotuner, ``substeps`` substeps replayed as captured hipGraph pairs
+    with an autotuned PCG budget.  Compared with the same scenes run one at a
+    time through FlipSolver3D (``alone_substeps`` timed substeps each)."""
+    import ctypes as C
+    import numpy as np
+    import torch
+    from gpufluid_amd import FlipSolver3D, _lib, autotune, seeding
+    from gpufluid_amd.batch import BatchedScenes
+    scenes = seeding.config_c5()
+    data = [(sc.shape[0], sc.pos, sc.vel) for sc in scenes]
+    npart = sum(sc.n_particles for sc in scenes)
+    dt = scenes[0].params["dt"]
+    b = BatchedScenes(data, dt=dt, device="cuda:0")
+    for _ in range(2):
+        b.step(pressure_iters=max_iter, pressure_tol=tol)   # warm-up, first stats
+    s = b.s
+
+    def solve():
+        d = s._descriptor()
+        _lib.call("bench", "gf_scene_pcg_sync", C.byref(d), 0, _lib.GF_PCG_MIXED, tol, max_iter,
+                  s.pcg_check_every, -1, None, None, None, None, s._stream())
+
+    wg, tune_times = autotune.tune_pcg_workgroups(solve, key=("batch", b.pack.shape, b.n_scenes))
+    budget = autotune.graph_budget(b.last_iters, s.pcg_check_every, max_iter)
+    b.step2_graph(max_iter, tol, budget_iters=budget)      # capture + one untimed pair
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    iters = []
+    for _ in range(substeps // 2):
+        b.step2_graph(max_iter, tol, budget_iters=budget)
+        iters.append(int(max(b.last_iters_pair[0].max(), b.last_iters_pair[1].max())))
+    torch.cuda.synchronize()
+    t_graph = time.perf_counter() - t0
+    worst_res = float(b.last_residual.max())
+    # direct launches (host-synchronised PCG), same batch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(2):
+        b.step(pressure_iters=max_iter, pressure_tol=tol)
+    torch.cuda.synchronize()
+    t_direct = (time.perf_counter() - t0) / 2
+    del b, s
+    torch.cuda.empty_cache()
+    autotune.set_pcg_workgroups(0)
+    # one scene at a time
+    t_alone = 0.0
+    for sc in scenes:
+        n = sc.shape[0]
+        f = FlipSolver3D(n, n, n, device="cuda:0")
+        f.set_particles(sc.pos, sc.vel)
+        f.step(dt, pressure_iters=max_iter, pressure_solver="pcg", pressure_tol=tol)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(alone_substeps):
+            f.step(dt, pressure_iters=max_iter, pressure_solver="pcg", pressure_tol=tol)
+        torch.cuda.synchronize()
+        t_alone += time.perf_counter() - t0
+        del f
+    graph_v = npart * substeps / t_graph / 1e6
+    alone_v = npart * alone_substeps / t_alone / 1e6
+    return {"workload": f"{len(scenes)} scenes, edges 24-96, {npart} particles, FLIP, mixed PCG to 1e-8",
+            "particles": npart, "substeps_timed": substeps,
+            "batched_graph_value": round(graph_v, 2),
+            "batched_direct_value": round(npart / t_direct / 1e6, 2),
+            "one_at_a_time_value": round(alone_v, 2),
+            "one_at_a_time_sample": f"{alone_substeps} substeps per scene after one warm-up substep",
+            "unit": "Mparticle-substeps/s",
+            "speedup_vs_one_at_a_time": round(graph_v / alone_v, 2),
+            "pcg_workgroups_autotuned": wg,
+            "autotune_seconds": {str(k): round(v, 5) for k, v in tune_times.items()},
+            "graph_pcg_budget_iters": budget, "max_pcg_iters_per_substep": iters,
+            "worst_scene_rel_residual": worst_res}
 
 
 def pcie_inclusive(s, npart, ms_step):

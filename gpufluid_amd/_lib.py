@@ -50,6 +50,8 @@ class GfScene(ctypes.Structure):
         ("pc", c_void_p), ("ptmp", c_void_p),
         ("partials", c_void_p), ("scalars", c_void_p),
         ("p32b", c_void_p), ("p64b", c_void_p),
+        ("n_scenes", c_int32), ("tile_scene", c_void_p), ("tile_order", c_void_p),
+        ("scene_par", c_void_p), ("scene_slot", c_void_p), ("pcg_part", c_void_p),
     ]
 
 
@@ -72,7 +74,8 @@ class GfAdjoint(ctypes.Structure):
 
 class GfSizes(ctypes.Structure):
     _fields_ = [("cells", c_int64), ("ntiles", c_int64), ("compact", c_int64),
-                ("scan_parts", c_int64), ("partials", c_int64), ("scalars", c_int64)]
+                ("scan_parts", c_int64), ("partials", c_int64), ("scalars", c_int64),
+                ("pcg_part", c_int64)]
 
 
 V, I, F, D = c_void_p, c_int, c_float, c_double
@@ -92,6 +95,8 @@ SIGNATURES = {
     "gf_scene_p2g_raw": [SP, I, V, V, V, V],
     "gf_scene_jacobi": [SP, I, I, V],
     "gf_scene_pcg_sync": [SP, I, I, D, I, I, I, PI, PD, PI, PD, V],
+    "gf_scene_pcg_stats": [SP, PI, PD, PI, V],
+    "gf_scene_pcg_enqueue": [SP, I, I, D, I, I, I, I, I, V],
     "gf_scene_g2p": [SP, V, I, F, F, F, F, V],
     "gf_scene_compact_to_dense": [SP, V, V, V],
     "gf_scene_unsort": [SP, V, I, V, V],

@@ -88,6 +88,7 @@ extern "C" int gf_scene_sizes(int nx, int ny, int nz, gf_scene_size_info* out) {
     out->compact = ntiles * GF_TILE_CELLS;
     out->scan_parts = (cells + 1 + 4095) / 4096 + 1;
     out->partials = 8 * 2048;  // per-workgroup partial sums of the PCG reductions
-    out->scalars = 64;         // two 16-double solver log entries, plus spare
+    out->scalars = 64;         // three 16-double solver log entries (two live, one result), spare
+    out->pcg_part = 8 * 2048 + 2;  // per-workgroup scene/tile ranges + per-scene offsets
     return 0;
 }

@@ -28,9 +28,10 @@
 
 namespace {
 
-// Solver scalars.  Kernels form a fixed launch sequence numbered L; launch L
-// reads entry (L-1)&1 and writes entry L&1 (workgroup 0 only), so no kernel
-// reads a field that it also writes.
+// Solver scalars, one log per scene.  Kernels form a fixed launch sequence
+// numbered L; launch L reads entry (L-1)&1 of its scene's pair and writes
+// entry L&1 (the scene's first workgroup only), so no kernel reads a field
+// that it also writes.  Entries 2S.. hold each scene's final log.
 struct PcgLog {
     double rz;       // r.z of the current residual (used by the next alpha)
     double rz_old;   // r.z used by the last alpha (denominator of the next beta)
@@ -57,9 +58,12 @@ struct PcgArgs {
     double* p64[2];
     float* pc;
     double* part;        // [P_KINDS][PCG_GMAX]
-    PcgLog* log;         // [2]
-    int nact;            // active tiles
-    int G;               // workgroups; workgroup b owns tiles [b*nact/G, (b+1)*nact/G)
+    PcgLog* log;         // [2*S] live pairs, then [S] results
+    int32_t* part_info;  // [8*PCG_GMAX] per workgroup: scene, wb, we, t0, t1; then [S+1] wg offsets
+    const int32_t* scene_slot;  // [S+1] compact tile offsets per scene (NULL: one scene, nact tiles)
+    int nact;            // active tiles (one scene)
+    int S;               // scenes
+    int G;               // workgroups; a scene's workgroups split its tiles evenly
     float rhs_sign;
     double tol2, delta2;
     int max_iter;
@@ -185,12 +189,24 @@ __device__ __forceinline__ T nb_sum4(const Nb4<T>& o, const T (&c)[4], int e) {
     return s;
 }
 
-// Tile range of this workgroup; consecutive logical workgroups sit on the same
-// XCD so that neighbouring tiles share an L2.
-__device__ __forceinline__ void tile_range(const PcgArgs& a, int& t0, int& t1, int& b) {
-    b = gf::xcd_remap(blockIdx.x, a.G);
-    t0 = (int)(((long long)b * a.nact) / a.G);
-    t1 = (int)(((long long)(b + 1) * a.nact) / a.G);
+// This workgroup's scene, the scene's workgroup range [wb, we) and the tile
+// range [t0, t1) it owns (scene < 0: idle).  Consecutive logical workgroups
+// sit on the same XCD so that neighbouring tiles share an L2.
+struct Part {
+    int b, s, wb, we, t0, t1;
+};
+
+__device__ __forceinline__ Part part_of(const PcgArgs& a) {
+    Part P;
+    P.b = gf::xcd_remap(blockIdx.x, a.G);
+    const int4 q = *reinterpret_cast<const int4*>(a.part_info + 8 * P.b);
+    const int t1 = a.part_info[8 * P.b + 4];
+    P.s = __builtin_amdgcn_readfirstlane(q.x);
+    P.wb = __builtin_amdgcn_readfirstlane(q.y);
+    P.we = __builtin_amdgcn_readfirstlane(q.z);
+    P.t0 = __builtin_amdgcn_readfirstlane(q.w);
+    P.t1 = __builtin_amdgcn_readfirstlane(t1);
+    return P;
 }
 
 // Workgroup sum of one value per thread, in a fixed order; result in all threads.
@@ -205,12 +221,13 @@ __device__ __forceinline__ double wg_sum(double v, double* s_red) {
     return t;
 }
 
-// Sum of the G partials of one kind (deterministic, identical in every
-// workgroup).  Written by the previous kernel, so visible at this launch.
-__device__ __forceinline__ double sum_partials(const double* part, int kind, int G,
+// Sum of one kind of partial over a scene's workgroups [wb, we)
+// (deterministic, identical in each of them).  Written by the previous
+// kernel, so visible at this launch.
+__device__ __forceinline__ double sum_partials(const double* part, int kind, int wb, int we,
                                                double* s_red) {
     double v = 0.0;
-    for (int i = threadIdx.x; i < G; i += GF_WG) v += part[kind * PCG_GMAX + i];
+    for (int i = wb + threadIdx.x; i < we; i += GF_WG) v += part[kind * PCG_GMAX + i];
     return wg_sum(v, s_red);
 }
 
@@ -234,12 +251,10 @@ enum { PEND_NONE = 0, PEND_SETUP = 1, PEND_REPLACE = 2 };
 
 // A tile is handled by two whole waves, so t is wave-uniform: readfirstlane
 // lets the compiler keep the tile's base and neighbour table in scalar registers.
-#define FOR_QUADS(a)                                                        \
-    int t0_, t1_, b;                                                        \
-    tile_range(a, t0_, t1_, b);                                             \
+#define FOR_QUADS(P)                                                        \
     const Quad Q = quad_of();                                               \
-    for (int tb_ = t0_; tb_ < t1_; tb_ += 2)                                \
-        if (const int t = __builtin_amdgcn_readfirstlane(tb_ + (threadIdx.x >> 7)); t < t1_)
+    for (int tb_ = P.t0; tb_ < P.t1; tb_ += 2)                              \
+        if (const int t = __builtin_amdgcn_readfirstlane(tb_ + (threadIdx.x >> 7)); t < P.t1)
 
 template <typename T>
 __device__ __forceinline__ T zdiv(T r, uint32_t flagbyte) {
@@ -262,12 +277,13 @@ __device__ __forceinline__ void dir4(const T (&z)[4], const T (&po)[4], T bf, T 
 
 // Consumes the update kernel's r.z / r.r partials (when fresh): counts the
 // finished iteration and sets the stop / replacement flags.  Identical in
-// every workgroup; workgroup 0 writes the log.
+// every workgroup of the scene; its first workgroup writes the log.
 template <bool MIXED>
-__device__ __forceinline__ void consume(const PcgArgs& a, PcgLog& e, double* s_red) {
+__device__ __forceinline__ void consume(const PcgArgs& a, const Part& P, PcgLog& e,
+                                        double* s_red) {
     if (!e.fresh) return;
-    const double rz_new = sum_partials(a.part, P_RZ, a.G, s_red);
-    const double rr = sum_partials(a.part, P_RR, a.G, s_red);
+    const double rz_new = sum_partials(a.part, P_RZ, P.wb, P.we, s_red);
+    const double rr = sum_partials(a.part, P_RR, P.wb, P.we, s_red);
     e.rz_old = e.rz;
     e.rz = rz_new;
     e.rr = rr;
@@ -285,15 +301,75 @@ __device__ __forceinline__ void consume(const PcgArgs& a, PcgLog& e, double* s_r
     }
 }
 
+__device__ __forceinline__ PcgLog& live_log(const PcgArgs& a, int s, int L) {
+    return a.log[2 * s + (L & 1)];
+}
+
+__device__ __forceinline__ void write_log(const PcgArgs& a, const Part& P, const PcgLog& e,
+                                          int L) {
+    if (P.b == P.wb && threadIdx.x == 0) live_log(a, P.s, L) = e;
+}
+
+// Workgroup partition of a solve (one workgroup): scene s gets
+// 1 + floor((G - NE) * tiles_s / tiles) workgroups when it has active tiles
+// (NE = scenes with tiles), none otherwise; each workgroup of a scene takes an
+// even share of its tiles.  Also marks every scene's log as awaiting the setup.
+__global__ __launch_bounds__(GF_WG) void k_pcg_partition(PcgArgs a) {
+    __shared__ int s_wave[GF_WG / GF_WAVE + 1];
+    __shared__ double s_red[GF_WG / GF_WAVE];
+    int32_t* wg_off = a.part_info + 8 * PCG_GMAX;
+    long long tiles = 0, ne = 0;
+    for (int sc = threadIdx.x; sc < a.S; sc += GF_WG) {
+        const int n = a.scene_slot ? a.scene_slot[sc + 1] - a.scene_slot[sc] : a.nact;
+        tiles += n;
+        ne += n > 0;
+    }
+    tiles = (long long)wg_sum((double)tiles, s_red);  // integer sums, exact in FP64
+    ne = (long long)wg_sum((double)ne, s_red);
+    const long long spare = a.G - ne > 0 ? a.G - ne : 0;
+    int carry = 0;
+    for (int base = 0; base < a.S; base += GF_WG) {
+        const int sc = base + threadIdx.x;
+        int g = 0, n = 0, slot0 = 0;
+        if (sc < a.S) {
+            slot0 = a.scene_slot ? a.scene_slot[sc] : 0;
+            n = a.scene_slot ? a.scene_slot[sc + 1] - slot0 : a.nact;
+            g = n > 0 ? 1 + (int)((spare * n) / (tiles > 0 ? tiles : 1)) : 0;
+        }
+        int total;
+        const int off = carry + gf::wg_exclusive_scan(g, s_wave, &total);
+        if (sc < a.S) {
+            wg_off[sc] = off;
+            for (int q = 0; q < g; ++q) {
+                int32_t* w = a.part_info + 8 * (off + q);
+                w[0] = sc; w[1] = off; w[2] = off + g;
+                w[3] = slot0 + (int)(((long long)q * n) / g);
+                w[4] = slot0 + (int)(((long long)(q + 1) * n) / g);
+            }
+            PcgLog e = {};
+            e.pending = PEND_SETUP;
+            a.log[2 * sc] = e;
+        }
+        carry += total;
+    }
+    if (threadIdx.x == 0) wg_off[a.S] = carry;
+    for (int b = carry + threadIdx.x; b < a.G; b += GF_WG) {
+        int32_t* w = a.part_info + 8 * b;
+        w[0] = -1; w[1] = 0; w[2] = 0; w[3] = 0; w[4] = 0;
+    }
+}
+
 // L = 0: x = 0, r = rhs_sign*b, z = r/M stored, p (buffer 0) = 0; partials
 // r.z and r.r.
 template <typename T, bool MIXED>
 __global__ __launch_bounds__(GF_WG) void k_pcg_setup(PcgArgs a) {
     __shared__ double s_red[GF_WG / GF_WAVE];
+    const Part P = part_of(a);
+    if (P.s < 0) return;
     T* r = vr<T>(a);
     T* p = vp<T>(a, 0);
     double rz = 0.0, rr = 0.0;
-    FOR_QUADS(a) {
+    FOR_QUADS(P) {
         const size_t idx = (size_t)t * GF_TILE_CELLS + Q.l0;
         const uint32_t fw = ld_flags(a.flags, idx);
         float bv[4];
@@ -314,30 +390,29 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_setup(PcgArgs a) {
             st4(a.x32 + idx, zf);
         }
     }
-    put_partial(a.part, P_RZ, b, rz, s_red);
-    put_partial(a.part, P_BB, b, rr, s_red);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        PcgLog e = {};
-        e.pending = PEND_SETUP;
-        a.log[0] = e;
-    }
+    put_partial(a.part, P_RZ, P.b, rz, s_red);
+    put_partial(a.part, P_BB, P.b, rr, s_red);
 }
 
-// One-workgroup finalisation after the setup or a residual replacement.
+// Finalisation after the setup or a residual replacement: one workgroup per
+// scene (grid S).
 __global__ __launch_bounds__(GF_WG) void k_pcg_fin(PcgArgs a, int L) {
     __shared__ double s_red[GF_WG / GF_WAVE];
-    PcgLog e = a.log[(L - 1) & 1];
+    const int sc = blockIdx.x;
+    const int32_t* wg_off = a.part_info + 8 * PCG_GMAX;
+    const int wb = wg_off[sc], we = wg_off[sc + 1];
+    PcgLog e = live_log(a, sc, L - 1);
     if (e.pending == PEND_SETUP) {
-        const double rz = sum_partials(a.part, P_RZ, a.G, s_red);
-        const double bb = sum_partials(a.part, P_BB, a.G, s_red);
+        const double rz = sum_partials(a.part, P_RZ, wb, we, s_red);
+        const double bb = sum_partials(a.part, P_BB, wb, we, s_red);
         e.rz = rz; e.rz_old = 0.0; e.bb = bb; e.rr = bb; e.rr_upd = bb; e.rr_true = bb;
         e.iters = 0; e.maxed = 0; e.upd = 0; e.merge = 0; e.nupd = 0; e.fresh = 0; e.pbuf = 0;
         const int zero = bb < 1e-30;
         e.converged = zero;
         e.done = zero || a.max_iter <= 0;
     } else if (e.pending == PEND_REPLACE) {
-        const double rrt = sum_partials(a.part, P_RRT, a.G, s_red);
-        const double rz = sum_partials(a.part, P_RZT, a.G, s_red);
+        const double rrt = sum_partials(a.part, P_RRT, wb, we, s_red);
+        const double rz = sum_partials(a.part, P_RZT, wb, we, s_red);
         e.rr_true = rrt; e.rr = rrt; e.rr_upd = rrt; e.rz = rz;
         e.upd = 0; e.merge = 1; e.nupd += 1;
         if (rrt < a.tol2 * e.bb) { e.converged = 1; e.done = 1; }
@@ -345,12 +420,13 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_fin(PcgArgs a, int L) {
         else e.done = 0;
     }
     e.pending = PEND_NONE;
-    if (threadIdx.x == 0) a.log[L & 1] = e;
+    if (threadIdx.x == 0) live_log(a, sc, L) = e;
 }
 
-__device__ __forceinline__ bool pass_through(const PcgArgs& a, const PcgLog& e, int L) {
+__device__ __forceinline__ bool pass_through(const PcgArgs& a, const Part& P, const PcgLog& e,
+                                             int L) {
     if (!e.done) return false;
-    if (blockIdx.x == 0 && threadIdx.x == 0) a.log[L & 1] = e;
+    write_log(a, P, e, L);
     return true;
 }
 
@@ -359,11 +435,13 @@ __device__ __forceinline__ bool pass_through(const PcgArgs& a, const PcgLog& e, 
 template <typename T, bool MIXED>
 __global__ __launch_bounds__(GF_WG) void k_pcg_dirspmv(PcgArgs a, int L) {
     __shared__ double s_red[GF_WG / GF_WAVE];
-    PcgLog e = a.log[(L - 1) & 1];
-    if (pass_through(a, e, L)) return;
-    consume<MIXED>(a, e, s_red);
+    const Part P = part_of(a);
+    if (P.s < 0) return;
+    PcgLog e = live_log(a, P.s, L - 1);
+    if (pass_through(a, P, e, L)) return;
+    consume<MIXED>(a, P, e, s_red);
     if (e.done) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) a.log[L & 1] = e;
+        write_log(a, P, e, L);
         return;
     }
     const double beta = fabs(e.rz_old) < 1e-30 ? 0.0 : e.rz / e.rz_old;
@@ -373,7 +451,7 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_dirspmv(PcgArgs a, int L) {
     const T* __restrict__ z = vr<T>(a);
     T* __restrict__ ap = vap<T>(a);
     double pap = 0.0;
-    FOR_QUADS(a) {
+    FOR_QUADS(P) {
         const size_t base = (size_t)t * GF_TILE_CELLS;
         const size_t idx = base + Q.l0;
         const NbIdx ni = nb_index(a.nbr, t, base, Q);
@@ -393,19 +471,17 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_dirspmv(PcgArgs a, int L) {
         o.zm0 = fma(bf, np.zm0, nz.zm0);
         o.zp3 = fma(bf, np.zp3, nz.zp3);
         for (int k = 0; k < 4; ++k) {
-            const T s = nb_sum4(o, c, k);
-            y[k] = keep_if((T)fl_diag(fw, k) * c[k] - s, fl_fluid(fw, k));
+            const T sm = nb_sum4(o, c, k);
+            y[k] = keep_if((T)fl_diag(fw, k) * c[k] - sm, fl_fluid(fw, k));
             pap += (double)c[k] * (double)y[k];
         }
         st4(pn + idx, c);
         st4(ap + idx, y);
     }
-    put_partial(a.part, P_PAP, b, pap, s_red);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        e.beta = beta;
-        e.pbuf ^= 1;
-        a.log[L & 1] = e;
-    }
+    put_partial(a.part, P_PAP, P.b, pap, s_red);
+    e.beta = beta;
+    e.pbuf ^= 1;
+    write_log(a, P, e, L);
 }
 
 // alpha = rz / pAp; x += alpha p, r = M z - alpha Ap, z = r/M stored;
@@ -413,9 +489,11 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_dirspmv(PcgArgs a, int L) {
 template <typename T, bool MIXED>
 __global__ __launch_bounds__(GF_WG) void k_pcg_update(PcgArgs a, int L) {
     __shared__ double s_red[GF_WG / GF_WAVE];
-    PcgLog e = a.log[(L - 1) & 1];
-    if (pass_through(a, e, L)) return;
-    const double pap_t = sum_partials(a.part, P_PAP, a.G, s_red);
+    const Part P = part_of(a);
+    if (P.s < 0) return;
+    PcgLog e = live_log(a, P.s, L - 1);
+    if (pass_through(a, P, e, L)) return;
+    const double pap_t = sum_partials(a.part, P_PAP, P.wb, P.we, s_red);
     const double alpha = fabs(pap_t) < 1e-30 ? 0.0 : e.rz / pap_t;
     const T af = (T)alpha;
     const int merge = MIXED ? e.merge : 0;
@@ -423,7 +501,7 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_update(PcgArgs a, int L) {
     const T* __restrict__ p = vp<T>(a, e.pbuf);
     const T* __restrict__ ap = vap<T>(a);
     double rz = 0.0, rr = 0.0;
-    FOR_QUADS(a) {
+    FOR_QUADS(P) {
         const size_t idx = (size_t)t * GF_TILE_CELLS + Q.l0;
         const uint32_t fw = ld_flags(a.flags, idx);
         T pv[4], apv[4], zv[4];
@@ -459,15 +537,13 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_update(PcgArgs a, int L) {
         }
         st4(zp + idx, zv);
     }
-    put_partial(a.part, P_RZ, b, rz, s_red);
-    put_partial(a.part, P_RR, b, rr, s_red);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        e.alpha = alpha;
-        e.pap = pap_t;
-        e.merge = 0;
-        e.fresh = 1;
-        a.log[L & 1] = e;
-    }
+    put_partial(a.part, P_RZ, P.b, rz, s_red);
+    put_partial(a.part, P_RR, P.b, rr, s_red);
+    e.alpha = alpha;
+    e.pap = pap_t;
+    e.merge = 0;
+    e.fresh = 1;
+    write_log(a, P, e, L);
 }
 
 // Chunk end: consumes the last update's partials and decides; in mixed mode,
@@ -477,15 +553,17 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_update(PcgArgs a, int L) {
 template <bool MIXED>
 __global__ __launch_bounds__(GF_WG) void k_pcg_replace(PcgArgs a, int L) {
     __shared__ double s_red[GF_WG / GF_WAVE];
-    PcgLog e = a.log[(L - 1) & 1];
-    consume<MIXED>(a, e, s_red);
+    const Part P = part_of(a);
+    if (P.s < 0) return;
+    PcgLog e = live_log(a, P.s, L - 1);
+    consume<MIXED>(a, P, e, s_red);
     const bool run = MIXED && (e.upd || (e.done && !e.converged));
     if (!run) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) a.log[L & 1] = e;
+        write_log(a, P, e, L);
         return;
     }
     double rrt = 0.0, rz = 0.0;
-    FOR_QUADS(a) {
+    FOR_QUADS(P) {
         const size_t base = (size_t)t * GF_TILE_CELLS;
         const size_t idx = base + Q.l0;
         const uint32_t fw = ld_flags(a.flags, idx);
@@ -522,18 +600,18 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_replace(PcgArgs a, int L) {
         }
         st4(a.r32 + idx, z32);
     }
-    put_partial(a.part, P_RRT, b, rrt, s_red);
-    put_partial(a.part, P_RZT, b, rz, s_red);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        e.pending = PEND_REPLACE;
-        a.log[L & 1] = e;
-    }
+    put_partial(a.part, P_RRT, P.b, rrt, s_red);
+    put_partial(a.part, P_RZT, P.b, rz, s_red);
+    e.pending = PEND_REPLACE;
+    write_log(a, P, e, L);
 }
 
 // pc = FP32 result (x64 [+ x32]); non-fluid cells hold 0 in both.
 template <bool MIXED>
 __global__ __launch_bounds__(GF_WG) void k_pcg_final(PcgArgs a) {
-    FOR_QUADS(a) {
+    const Part P = part_of(a);
+    if (P.s < 0) return;
+    FOR_QUADS(P) {
         const size_t idx = (size_t)t * GF_TILE_CELLS + Q.l0;
         double xd[4];
         ld4(a.x64 + idx, xd);
@@ -547,7 +625,12 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_final(PcgArgs a) {
         }
         st4(a.pc + idx, out);
     }
-    (void)b;
+}
+
+// Each scene's final log into the result entries (one thread per scene).
+__global__ void k_pcg_result(PcgArgs a, int L) {
+    const int sc = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sc < a.S) a.log[2 * a.S + sc] = live_log(a, sc, L);
 }
 
 // Jacobi sweep (solver3d.py:878-938) on compact tiles: two tiles per
@@ -589,19 +672,24 @@ __global__ void k_fill_f32(float* a, size_t n, float v) {
 namespace gf {
 int scene_check(const gf_scene_t* s, const char* who);
 
-// Two solver-log slots in pinned host memory per host thread, allocated on
-// first use and kept for the life of the thread (the PCG's only host
+// Solver-log readback slots in pinned host memory per host thread, grown on
+// demand and kept for the life of the thread (the PCG's only host
 // allocation; it never touches device memory).
-static PcgLog* pinned_pcg_logs() {
+static PcgLog* pinned_pcg_logs(size_t entries) {
     static thread_local PcgLog* logs = nullptr;
-    if (!logs) {
+    static thread_local size_t cap = 0;
+    if (entries > cap) {
+        if (logs) hipHostFree(logs);
+        logs = nullptr;
+        cap = 0;
         void* p = nullptr;
-        hipError_t e = hipHostMalloc(&p, 2 * sizeof(PcgLog), hipHostMallocDefault);
+        hipError_t e = hipHostMalloc(&p, entries * sizeof(PcgLog), hipHostMallocDefault);
         if (e != hipSuccess) {
             set_error((int)e, "hipHostMalloc: %s", hipGetErrorString(e));
             return nullptr;
         }
         logs = static_cast<PcgLog*>(p);
+        cap = entries;
     }
     return logs;
 }
@@ -631,72 +719,138 @@ extern "C" int gf_scene_jacobi(const gf_scene_t* s, int n_active, int iters, voi
     return 0;
 }
 
-extern "C" int gf_scene_pcg_sync(const gf_scene_t* s, int n_active, int precision, double tol,
-                                 int max_iter, int check_every, int rhs_sign, int* iters,
-                                 double* rel_residual, int* replacements, double* kernel_ms,
-                                 void* stream) {
-    int rc = gf::scene_check(s, "gf_scene_pcg_sync");
+namespace {
+
+// Validated solver arguments shared by gf_scene_pcg_sync and gf_scene_pcg_enqueue.
+int pcg_prepare(const gf_scene_t* s, int n_active, int precision, double tol, int max_iter,
+                int rhs_sign, const char* who, PcgArgs& a) {
+    int rc = gf::scene_check(s, who);
     if (rc) return rc;
     const int ntiles = s->tnx * s->tny * s->tnz;
-    GF_REQUIRE(n_active >= 0 && n_active <= ntiles, "gf_scene_pcg_sync: n_active %d", n_active);
-    GF_REQUIRE(precision == GF_PCG_MIXED || precision == GF_PCG_FP64,
-               "gf_scene_pcg_sync: precision %d", precision);
-    GF_REQUIRE(rhs_sign == 1 || rhs_sign == -1, "gf_scene_pcg_sync: rhs_sign %d", rhs_sign);
-    GF_REQUIRE(tol >= 0.0 && max_iter >= 0, "gf_scene_pcg_sync: tol/max_iter");
-    GF_REQUIRE(s->flags && s->nbr && s->b && s->pc && s->partials && s->scalars && s->x64,
-               "gf_scene_pcg_sync: buffers not set");
+    const bool packed = s->n_scenes > 1;
+    const int S = packed ? s->n_scenes : 1;
+    GF_REQUIRE(n_active >= 0 && n_active <= ntiles, "%s: n_active %d", who, n_active);
+    GF_REQUIRE(S <= PCG_GMAX, "%s: %d scenes exceed %d", who, S, PCG_GMAX);
+    GF_REQUIRE(precision == GF_PCG_MIXED || precision == GF_PCG_FP64, "%s: precision %d", who,
+               precision);
+    GF_REQUIRE(rhs_sign == 1 || rhs_sign == -1, "%s: rhs_sign %d", who, rhs_sign);
+    GF_REQUIRE(tol >= 0.0 && max_iter >= 0, "%s: tol/max_iter", who);
+    GF_REQUIRE(s->flags && s->nbr && s->b && s->pc && s->partials && s->scalars && s->x64 &&
+                   s->pcg_part,
+               "%s: buffers not set", who);
     if (precision == GF_PCG_MIXED)
-        GF_REQUIRE(s->x32 && s->r32 && s->p32 && s->p32b && s->ap32,
-                   "gf_scene_pcg_sync: FP32 vectors not set");
+        GF_REQUIRE(s->x32 && s->r32 && s->p32 && s->p32b && s->ap32, "%s: FP32 vectors not set", who);
     else
-        GF_REQUIRE(s->r64 && s->p64 && s->p64b && s->ap64, "gf_scene_pcg_sync: FP64 vectors not set");
-    if (check_every < 1) check_every = 1;
-    if (check_every > 64) check_every = 64;
-    if (iters) *iters = 0;
-    if (rel_residual) *rel_residual = 0.0;
-    if (replacements) *replacements = 0;
-    if (kernel_ms) for (int i = 0; i < 4; ++i) kernel_ms[i] = 0.0;
-    hipStream_t st = (hipStream_t)stream;
-    if (n_active == 0) return 0;
-
-    PcgArgs a;
+        GF_REQUIRE(s->r64 && s->p64 && s->p64b && s->ap64, "%s: FP64 vectors not set", who);
     a.flags = s->flags; a.nbr = s->nbr; a.b = s->b;
     a.x32 = s->x32; a.r32 = s->r32; a.ap32 = s->ap32;
     a.x64 = s->x64; a.r64 = s->r64; a.ap64 = s->ap64;
     a.p32[0] = s->p32; a.p32[1] = s->p32b;
     a.p64[0] = s->p64; a.p64[1] = s->p64b;
     a.pc = s->pc; a.part = s->partials; a.log = (PcgLog*)s->scalars;
+    a.part_info = s->pcg_part;
+    a.scene_slot = packed ? s->scene_slot : nullptr;
     a.nact = n_active;
-    const int pairs = (n_active + 1) / 2;
+    a.S = S;
     // 1024 workgroups (4 per CU) measured best on MI355X at 256^3 (5.4 M
     // active cells); the autotuner may choose another value up to PCG_GMAX.
     int gmax = gf::tunable(gf::GF_TUNE_PCG_WORKGROUPS);
     gmax = gmax > 0 ? (gmax < PCG_GMAX ? gmax : PCG_GMAX) : PCG_DEFAULT_G;
-    a.G = pairs < gmax ? pairs : gmax;
+    if (packed) {
+        a.G = gmax > S ? gmax : S;
+    } else {
+        const int pairs = (n_active + 1) / 2;
+        a.G = pairs < gmax ? pairs : gmax;
+    }
     a.rhs_sign = (float)rhs_sign;
     a.tol2 = tol * tol;
     a.delta2 = 0.01;
     a.max_iter = max_iter;
-    const dim3 grid(a.G), block(GF_WG);
-    const bool mixed = precision == GF_PCG_MIXED;
+    return 0;
+}
 
+// Launch numbers used by one chunk of `chunk` iterations.
+int pcg_chunk_launches(int chunk, bool mixed) { return 2 * chunk + 1 + (mixed ? 1 : 0); }
+
+void pcg_launch_setup(const PcgArgs& a, bool mixed, hipStream_t st) {
+    const dim3 grid(a.G), block(GF_WG);
+    hipLaunchKernelGGL(k_pcg_partition, dim3(1), block, 0, st, a);
     if (mixed)
         hipLaunchKernelGGL((k_pcg_setup<float, true>), grid, block, 0, st, a);
     else
         hipLaunchKernelGGL((k_pcg_setup<double, false>), grid, block, 0, st, a);
-    int L = 1;
-    hipLaunchKernelGGL(k_pcg_fin, dim3(1), block, 0, st, a, L);
-    GF_LAUNCH_CHECK("k_pcg_setup/fin");
+    hipLaunchKernelGGL(k_pcg_fin, dim3(a.S), block, 0, st, a, 1);
+}
 
-    // The host reads the solver log back once per chunk of check_every
+// One chunk: `chunk` iterations, then the chunk-end decision (and residual
+// replacement when due).  L is the launch number before the chunk; ev, when
+// given, receives 2*chunk+2 events (start, after each kernel, chunk end).
+int pcg_launch_chunk(const PcgArgs& a, bool mixed, int chunk, int& L, hipStream_t st,
+                     hipEvent_t* ev) {
+    const dim3 grid(a.G), block(GF_WG);
+    for (int j = 0; j < chunk; ++j) {
+        if (ev && j == 0) hipEventRecord(ev[0], st);
+        ++L;
+        if (mixed) hipLaunchKernelGGL((k_pcg_dirspmv<float, true>), grid, block, 0, st, a, L);
+        else hipLaunchKernelGGL((k_pcg_dirspmv<double, false>), grid, block, 0, st, a, L);
+        if (ev) hipEventRecord(ev[2 * j + 1], st);
+        ++L;
+        if (mixed) hipLaunchKernelGGL((k_pcg_update<float, true>), grid, block, 0, st, a, L);
+        else hipLaunchKernelGGL((k_pcg_update<double, false>), grid, block, 0, st, a, L);
+        if (ev) hipEventRecord(ev[2 * j + 2], st);
+    }
+    ++L;
+    if (mixed) hipLaunchKernelGGL(k_pcg_replace<true>, grid, block, 0, st, a, L);
+    else hipLaunchKernelGGL(k_pcg_replace<false>, grid, block, 0, st, a, L);
+    if (mixed) {
+        ++L;
+        hipLaunchKernelGGL(k_pcg_fin, dim3(a.S), block, 0, st, a, L);
+    }
+    if (ev) hipEventRecord(ev[2 * chunk + 1], st);
+    return gf::check_launch("pcg iteration");
+}
+
+void pcg_launch_final(const PcgArgs& a, bool mixed, int L, hipStream_t st) {
+    const dim3 grid(a.G), block(GF_WG);
+    if (mixed)
+        hipLaunchKernelGGL(k_pcg_final<true>, grid, block, 0, st, a);
+    else
+        hipLaunchKernelGGL(k_pcg_final<false>, grid, block, 0, st, a);
+    hipLaunchKernelGGL(k_pcg_result, dim3((a.S + GF_WG - 1) / GF_WG), block, 0, st, a, L);
+}
+
+}  // namespace
+
+extern "C" int gf_scene_pcg_sync(const gf_scene_t* s, int n_active, int precision, double tol,
+                                 int max_iter, int check_every, int rhs_sign, int* iters,
+                                 double* rel_residual, int* replacements, double* kernel_ms,
+                                 void* stream) {
+    if (iters) *iters = 0;
+    if (rel_residual) *rel_residual = 0.0;
+    if (replacements) *replacements = 0;
+    if (kernel_ms) for (int i = 0; i < 4; ++i) kernel_ms[i] = 0.0;
+    PcgArgs a;
+    int rc = pcg_prepare(s, n_active, precision, tol, max_iter, rhs_sign, "gf_scene_pcg_sync", a);
+    if (rc) return rc;
+    if (check_every < 1) check_every = 1;
+    if (check_every > 64) check_every = 64;
+    hipStream_t st = (hipStream_t)stream;
+    const int S = a.S;
+    if (S == 1 && n_active == 0) return 0;
+    const bool mixed = precision == GF_PCG_MIXED;
+    pcg_launch_setup(a, mixed, st);
+    int L = 1;
+    GF_LAUNCH_CHECK("k_pcg_partition/setup/fin");
+
+    // The host reads the scene logs back once per chunk of check_every
     // iterations.  Chunks are pipelined: chunk c+1 is enqueued before the host
-    // waits for chunk c's log (copied into pinned memory), so the device does
-    // not idle during the round trip; once the solve has stopped, the kernels
-    // of the chunk already in flight pass through.
+    // waits for chunk c's logs (copied into pinned memory), so the device does
+    // not idle during the round trip; once every scene has stopped, the
+    // kernels of the chunk already in flight pass through.
     // Optional per-kernel timing of the iterations that did work: events
-    // around every launch of a chunk, read when the chunk's log is read.
+    // around every launch of a chunk, read when the chunk's logs are read.
     // kernel_ms: [0] dirspmv, [1] update, [2] unused, [3] chunk-end replacement.
-    PcgLog* hlog = gf::pinned_pcg_logs();
+    PcgLog* hlog = gf::pinned_pcg_logs((size_t)6 * S);  // setup + 2 chunk slots, 2S entries each
     if (!hlog) return gf::set_error(GF_ESTATE, "gf_scene_pcg_sync: pinned log: %s", gf_last_error());
     const bool timed = kernel_ms != nullptr;
     const int chunk = check_every;
@@ -722,45 +876,43 @@ extern "C" int gf_scene_pcg_sync(const gf_scene_t* s, int n_active, int precisio
         destroy_events();
         return gf::set_error((int)e, "gf_scene_pcg_sync: event: %s", hipGetErrorString(e));
     }
-    PcgLog hs;
-    memset(&hs, 0, sizeof(hs));
-    e = hipMemcpyAsync(&hlog[0], &a.log[L & 1], sizeof(PcgLog), hipMemcpyDeviceToHost, st);
+    // Aggregate of the scene logs in one slot read at launch parity L.
+    struct Agg {
+        int done, iters, nupd;
+        double worst;  // max relative true residual
+    };
+    auto aggregate = [&](const PcgLog* lg, int Lp) {
+        Agg g = {1, 0, 0, 0.0};
+        for (int sc = 0; sc < S; ++sc) {
+            const PcgLog& q = lg[2 * sc + (Lp & 1)];
+            g.done &= q.done;
+            g.iters = q.iters > g.iters ? q.iters : g.iters;
+            g.nupd += q.nupd;
+            const double r = q.bb > 0.0 ? sqrt(q.rr_true / q.bb) : 0.0;
+            g.worst = r > g.worst ? r : g.worst;
+        }
+        return g;
+    };
+    const size_t live_bytes = (size_t)2 * S * sizeof(PcgLog);
+    Agg hs = {0, 0, 0, 0.0};
+    e = hipMemcpyAsync(hlog, a.log, live_bytes, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) rc = gf::set_error((int)e, "gf_scene_pcg_sync: readback: %s", hipGetErrorString(e));
-    else hs = hlog[0];
+    else hs = aggregate(hlog, L);
 
-    int launched = 0, next_slot = 0, npend = 0, pend[2] = {0, 0};
+    int launched = 0, next_slot = 0, npend = 0, pend[2] = {0, 0}, pend_L[2] = {0, 0};
     auto enqueue_chunk = [&]() -> int {
         const int sl = next_slot;
-        for (int j = 0; j < chunk; ++j) {
-            if (timed && j == 0) hipEventRecord(ev[sl][0], st);
-            ++L;
-            if (mixed) hipLaunchKernelGGL((k_pcg_dirspmv<float, true>), grid, block, 0, st, a, L);
-            else hipLaunchKernelGGL((k_pcg_dirspmv<double, false>), grid, block, 0, st, a, L);
-            if (timed) hipEventRecord(ev[sl][2 * j + 1], st);
-            ++L;
-            if (mixed) hipLaunchKernelGGL((k_pcg_update<float, true>), grid, block, 0, st, a, L);
-            else hipLaunchKernelGGL((k_pcg_update<double, false>), grid, block, 0, st, a, L);
-            if (timed) hipEventRecord(ev[sl][2 * j + 2], st);
-        }
-        // Chunk end: take the last update's decisions (and, in mixed mode,
-        // replace the residual when due) so the host sees a settled log.
-        ++L;
-        if (mixed) hipLaunchKernelGGL(k_pcg_replace<true>, grid, block, 0, st, a, L);
-        else hipLaunchKernelGGL(k_pcg_replace<false>, grid, block, 0, st, a, L);
-        if (mixed) {
-            ++L;
-            hipLaunchKernelGGL(k_pcg_fin, dim3(1), block, 0, st, a, L);
-        }
-        if (timed) hipEventRecord(ev[sl][2 * chunk + 1], st);
-        int r = gf::check_launch("pcg iteration");
+        int r = pcg_launch_chunk(a, mixed, chunk, L, st, timed ? ev[sl] : nullptr);
         if (r) return r;
-        hipError_t ce = hipMemcpyAsync(&hlog[sl], &a.log[L & 1], sizeof(PcgLog),
+        hipError_t ce = hipMemcpyAsync(hlog + (size_t)2 * S * (1 + sl), a.log, live_bytes,
                                        hipMemcpyDeviceToHost, st);
         if (ce == hipSuccess) ce = hipEventRecord(done_ev[sl], st);
         if (ce != hipSuccess)
             return gf::set_error((int)ce, "gf_scene_pcg_sync: readback: %s", hipGetErrorString(ce));
-        pend[npend++] = sl;
+        pend[npend] = sl;
+        pend_L[npend] = L;
+        ++npend;
         next_slot ^= 1;
         launched += chunk;
         return 0;
@@ -773,15 +925,16 @@ extern "C" int gf_scene_pcg_sync(const gf_scene_t* s, int n_active, int precisio
             rc = enqueue_chunk();  // look-ahead
             if (rc) break;
         }
-        const int sl = pend[0];
+        const int sl = pend[0], Lsl = pend_L[0];
         pend[0] = pend[1];
+        pend_L[0] = pend_L[1];
         --npend;
         e = hipEventSynchronize(done_ev[sl]);
         if (e != hipSuccess) {
             rc = gf::set_error((int)e, "gf_scene_pcg_sync: readback: %s", hipGetErrorString(e));
             break;
         }
-        hs = hlog[sl];
+        hs = aggregate(hlog + (size_t)2 * S * (1 + sl), Lsl);
         if (timed) {
             const int real = hs.iters - iters_before;
             for (int j = 0; j < real && j < chunk; ++j) {
@@ -810,13 +963,56 @@ extern "C" int gf_scene_pcg_sync(const gf_scene_t* s, int n_active, int precisio
     }
     destroy_events();
     if (rc) return rc;
-    if (mixed)
-        hipLaunchKernelGGL(k_pcg_final<true>, grid, block, 0, st, a);
-    else
-        hipLaunchKernelGGL(k_pcg_final<false>, grid, block, 0, st, a);
+    pcg_launch_final(a, mixed, L, st);
     GF_LAUNCH_CHECK("k_pcg_final");
     if (iters) *iters = hs.iters;
-    if (rel_residual) *rel_residual = hs.bb > 0.0 ? sqrt(hs.rr_true / hs.bb) : 0.0;
+    if (rel_residual) *rel_residual = hs.worst;
     if (replacements) *replacements = hs.nupd;
+    return 0;
+}
+
+extern "C" int gf_scene_pcg_enqueue(const gf_scene_t* s, int n_active, int precision, double tol,
+                                    int max_iter, int check_every, int rhs_sign, int first_chunk,
+                                    int n_chunks, void* stream) {
+    PcgArgs a;
+    int rc = pcg_prepare(s, n_active, precision, tol, max_iter, rhs_sign, "gf_scene_pcg_enqueue", a);
+    if (rc) return rc;
+    GF_REQUIRE(check_every >= 1 && check_every <= 64, "gf_scene_pcg_enqueue: check_every %d",
+               check_every);
+    GF_REQUIRE(first_chunk >= 0 && n_chunks >= 0, "gf_scene_pcg_enqueue: chunks %d+%d", first_chunk,
+               n_chunks);
+    if (a.S == 1 && n_active == 0) return 0;
+    hipStream_t st = (hipStream_t)stream;
+    const bool mixed = precision == GF_PCG_MIXED;
+    if (first_chunk == 0) pcg_launch_setup(a, mixed, st);
+    int L = 1 + first_chunk * pcg_chunk_launches(check_every, mixed);
+    for (int c = 0; c < n_chunks; ++c) {
+        rc = pcg_launch_chunk(a, mixed, check_every, L, st, nullptr);
+        if (rc) return rc;
+    }
+    pcg_launch_final(a, mixed, L, st);
+    GF_LAUNCH_CHECK("gf_scene_pcg_enqueue");
+    return 0;
+}
+
+extern "C" int gf_scene_pcg_stats(const gf_scene_t* s, int* iters, double* rel_residual,
+                                  int* replacements, void* stream) {
+    int rc = gf::scene_check(s, "gf_scene_pcg_stats");
+    if (rc) return rc;
+    GF_REQUIRE(s->scalars, "gf_scene_pcg_stats: scalars not set");
+    const int S = s->n_scenes > 1 ? s->n_scenes : 1;
+    PcgLog* hlog = gf::pinned_pcg_logs((size_t)S);
+    if (!hlog) return gf::set_error(GF_ESTATE, "gf_scene_pcg_stats: pinned log: %s", gf_last_error());
+    hipStream_t st = (hipStream_t)stream;
+    hipError_t e = hipMemcpyAsync(hlog, (const PcgLog*)s->scalars + 2 * S, S * sizeof(PcgLog),
+                                  hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess)
+        return gf::set_error((int)e, "gf_scene_pcg_stats: readback: %s", hipGetErrorString(e));
+    for (int sc = 0; sc < S; ++sc) {
+        if (iters) iters[sc] = hlog[sc].iters;
+        if (rel_residual) rel_residual[sc] = hlog[sc].bb > 0.0 ? sqrt(hlog[sc].rr_true / hlog[sc].bb) : 0.0;
+        if (replacements) replacements[sc] = hlog[sc].nupd;
+    }
     return 0;
 }

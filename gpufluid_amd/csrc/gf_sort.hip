@@ -173,27 +173,41 @@ __global__ __launch_bounds__(GF_WG) void k_rank(
 }
 
 // One workgroup: compact slot per occupied tile (in tile-id order).
+// Compact slots of the occupied tiles, in tile order or, for packed scenes,
+// in the scene-major order of `order` (so that each scene's slots are
+// contiguous; scene_slot[s] = its first slot, scene_slot[n_scenes] = total).
 __global__ __launch_bounds__(1024) void k_tile_scan(const int32_t* __restrict__ occ,
                                                     int ntiles,
+                                                    const int32_t* __restrict__ order,
+                                                    const int32_t* __restrict__ tile_scene,
+                                                    int n_scenes,
+                                                    int32_t* __restrict__ scene_slot,
                                                     int32_t* __restrict__ tile_map,
                                                     int32_t* __restrict__ active,
                                                     int32_t* __restrict__ n_active) {
     __shared__ int s_wave[1024 / GF_WAVE + 1];
     int carry = 0;
     for (int base = 0; base < ntiles; base += blockDim.x) {
-        int t = base + threadIdx.x;
-        int o = (t < ntiles && occ[t]) ? 1 : 0;
+        const int pos = base + threadIdx.x;
+        const int t = pos < ntiles ? (order ? order[pos] : pos) : 0;
+        int o = (pos < ntiles && occ[t]) ? 1 : 0;
         int total;
         int ex = gf::wg_exclusive_scan(o, s_wave, &total);
-        if (t < ntiles) {
+        if (pos < ntiles) {
             tile_map[t] = o ? carry + ex : -1;
             if (o) active[carry + ex] = t;
+            if (scene_slot) {
+                const int sc = tile_scene[t];
+                const int prev = pos > 0 ? tile_scene[order[pos - 1]] : -1;
+                if (sc >= 0 && sc != prev) scene_slot[sc] = carry + ex;
+            }
         }
         carry += total;
     }
     if (threadIdx.x == 0) {
         n_active[0] = carry;
         n_active[1] = 0;  // fluid cells, counted by k_tile_setup
+        if (scene_slot) scene_slot[n_scenes] = carry;
     }
 }
 
@@ -274,6 +288,11 @@ int scene_check(const gf_scene_t* s, const char* who) {
     if (s->n < 0 || s->n > s->cap)
         return set_error(GF_ESTATE, "%s: n=%d outside [0, cap=%d]", who, s->n, s->cap);
     if (!(s->dx > 0.0f)) return set_error(GF_EINVAL, "%s: dx must be > 0", who);
+    if (s->n_scenes < 0) return set_error(GF_EINVAL, "%s: n_scenes %d", who, s->n_scenes);
+    if (s->n_scenes > 1 &&
+        (!s->tile_scene || !s->tile_order || !s->scene_par || !s->scene_slot || !s->pcg_part))
+        return set_error(GF_EINVAL, "%s: packed scenes need tile_scene, tile_order, scene_par, "
+                                    "scene_slot and pcg_part", who);
     return 0;
 }
 }  // namespace gf
@@ -314,8 +333,11 @@ extern "C" int gf_scene_sort_sync(gf_scene_t* s, int with_affine, int* n_active_
                            with_affine ? s->affine_c_b : nullptr, s->pid_b);
         GF_LAUNCH_CHECK("k_slot/k_rank");
     }
+    const bool packed = s->n_scenes > 1;
     hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, st, s->tile_occ, ntiles,
-                       s->tile_map, s->active, s->n_active);
+                       packed ? s->tile_order : nullptr, packed ? s->tile_scene : nullptr,
+                       s->n_scenes, packed ? s->scene_slot : nullptr, s->tile_map, s->active,
+                       s->n_active);
     hipLaunchKernelGGL(k_tile_setup, dim3(ntiles), dim3(GF_WG), 0, st, s->nx, s->ny, s->nz,
                        s->tnx, s->tny, s->tnz, s->cell_start, s->marker, s->tile_map,
                        s->flags, s->nbr, s->n_active);

@@ -26,6 +26,7 @@ struct P2GParams {
     float* uw; float* vw; float* ww;
     const float* su; const float* sv; const float* sw;
     float* b;
+    const int32_t* tile_scene; const float* scene_par;  // packed scenes (NULL: one scene)
     int nx, ny, nz, tnx, tny, tnz;
     float dx, g, dt, rho;
 };
@@ -375,7 +376,8 @@ __global__ __launch_bounds__(GF_WG) void k_p2g_tile(P2GParams a) {
     }
 
     // Finalise the region: normalize (121-142), gravity (149-156), BC (758-799).
-    const float gdt = a.g * a.dt;
+    // Packed scenes carry g*dt per scene (grid units differ by scene).
+    const float gdt = a.tile_scene ? a.scene_par[8 * max(a.tile_scene[t], 0)] : a.g * a.dt;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const int DI = c == 0 ? 9 : 8, DJ = c == 1 ? 9 : 8, DK = c == 2 ? 9 : 8;
@@ -463,6 +465,7 @@ struct G2PParams {
     const float* us; const float* vs; const float* ws;
     const float* su; const float* sv; const float* sw;
     const float* pc;
+    const int32_t* tile_scene; const float* scene_par;  // packed scenes (NULL: one scene)
     int nx, ny, nz, tnx, tny, tnz;
     float dx, dt, rho, flip_blend, gdt;
     float domx, domy, domz;
@@ -621,6 +624,7 @@ __global__ __launch_bounds__(GF_WG) void k_g2p_tile(G2PParams a) {
     __syncthreads();
 
     const float scale = a.dt / (a.rho * a.dx);
+    const float gdt = a.tile_scene ? a.scene_par[8 * max(a.tile_scene[t], 0)] : a.gdt;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const int DI = c == 0 ? 9 : 10, DJ = c == 1 ? 9 : 10, DK = c == 2 ? 9 : 10;
@@ -640,7 +644,7 @@ __global__ __launch_bounds__(GF_WG) void k_g2p_tile(G2PParams a) {
                 // as P2G produced it (us [+ g*dt on v], then the BC below), so
                 // the halo never reads u while owners overwrite it.
                 sav = gsav[f];
-                val = c == 1 ? sav + a.gdt : sav;
+                val = c == 1 ? sav + gdt : sav;
                 // Cell coords of the face's two sides in the staged cell region.
                 const int ca = ii - (i0 - 1), cb = jj - (j0 - 1), cc = kk - (k0 - 1);
                 int la, lb, lc;  // low-side cell
@@ -685,6 +689,14 @@ __global__ __launch_bounds__(GF_WG) void k_g2p_tile(G2PParams a) {
 
     const float dx = a.dx, dt = a.dt;
     const float eps = dx * 1.001f;
+    // clamp box: [1.001 dx, dom - 1.001 dx], or the tile's scene box when packed
+    float lox = eps, loy = eps, loz = eps;
+    float hix = a.domx - eps, hiy = a.domy - eps, hiz = a.domz - eps;
+    if (a.tile_scene) {
+        const float* sp = a.scene_par + 8 * max(a.tile_scene[t], 0);
+        lox = sp[1]; loy = sp[2]; loz = sp[3];
+        hix = sp[4]; hiy = sp[5]; hiz = sp[6];
+    }
     for (int q = threadIdx.x; q < rs.total; q += GF_WG) {
         const int r = row_find(s_pref, rs.nrows, q);
         const int pidx = s_start[r] + (q - s_pref[r]);
@@ -719,9 +731,9 @@ __global__ __launch_bounds__(GF_WG) void k_g2p_tile(G2PParams a) {
         a.vel[3 * pidx + 1] = nv;
         a.vel[3 * pidx + 2] = nw;
         float npx = px + nu * dt, npy = py + nv * dt, npz = pz + nw * dt;
-        npx = fminf(fmaxf(npx, eps), a.domx - eps);
-        npy = fminf(fmaxf(npy, eps), a.domy - eps);
-        npz = fminf(fmaxf(npz, eps), a.domz - eps);
+        npx = fminf(fmaxf(npx, lox), hix);
+        npy = fminf(fmaxf(npy, loy), hiy);
+        npz = fminf(fmaxf(npz, loz), hiz);
         a.pos[3 * pidx + 0] = npx;
         a.pos[3 * pidx + 1] = npy;
         a.pos[3 * pidx + 2] = npz;
@@ -757,6 +769,9 @@ P2GParams p2g_params(const gf_scene_t* s, float g, float dt, float rho) {
     a.uw = nullptr; a.vw = nullptr; a.ww = nullptr;
     a.su = s->solid_u; a.sv = s->solid_v; a.sw = s->solid_w;
     a.b = s->b;
+    const bool packed = s->n_scenes > 1;
+    a.tile_scene = packed ? s->tile_scene : nullptr;
+    a.scene_par = packed ? s->scene_par : nullptr;
     a.nx = s->nx; a.ny = s->ny; a.nz = s->nz; a.tnx = s->tnx; a.tny = s->tny; a.tnz = s->tnz;
     a.dx = s->dx; a.g = g; a.dt = dt; a.rho = rho;
     return a;
@@ -837,6 +852,9 @@ extern "C" int gf_scene_g2p(const gf_scene_t* s, const float* pressure, int tran
     a.nx = s->nx; a.ny = s->ny; a.nz = s->nz; a.tnx = s->tnx; a.tny = s->tny; a.tnz = s->tnz;
     a.dx = s->dx; a.dt = dt; a.rho = rho; a.flip_blend = flip_blend; a.gdt = gravity * dt;
     a.domx = s->dom[0]; a.domy = s->dom[1]; a.domz = s->dom[2];
+    const bool packed = s->n_scenes > 1;
+    a.tile_scene = packed ? s->tile_scene : nullptr;
+    a.scene_par = packed ? s->scene_par : nullptr;
     const int ntiles = s->tnx * s->tny * s->tnz;
     hipStream_t st = (hipStream_t)stream;
     if (transfer_mode == GF_TRANSFER_APIC)

@@ -48,6 +48,11 @@ class SceneBuffers:
         self.r64 = self.p64 = self.p64b = self.ap64 = None  # FP64 PCG, allocated on first use
         self.partials = torch.zeros(sz.partials, **f64)
         self.scalars = torch.zeros(sz.scalars, **f64)
+        self.pcg_part = torch.zeros(sz.pcg_part, **i32)
+        self.pcg_part_base = sz.pcg_part
+        # packed scenes (gpufluid_amd.batch): per-tile scene ids and scene tables
+        self.n_scenes = 0
+        self.tile_scene = self.tile_order = self.scene_par = self.scene_slot = None
         self.cap = 0
         self.bufs = [None, None]     # ping-pong particle storage, cap rows each
         self.cur = 0
@@ -134,7 +139,25 @@ class SceneBuffers:
         s.pc, s.ptmp = P(self.pc), P(self.ptmp)
         s.partials, s.scalars = P(self.partials), P(self.scalars)
         s.p32b, s.p64b = P(self.p32b), P(self.p64b)
+        s.pcg_part = P(self.pcg_part)
+        if self.n_scenes > 1:
+            s.n_scenes = self.n_scenes
+            s.tile_scene, s.tile_order = P(self.tile_scene), P(self.tile_order)
+            s.scene_par, s.scene_slot = P(self.scene_par), P(self.scene_slot)
         return s
+
+    def set_scenes(self, tile_scene, tile_order, scene_par):
+        """Describe packed scenes: per-tile scene id, scene-major tile order
+        and the per-scene table (g*dt, clamp lo x,y,z, clamp hi x,y,z, 0)."""
+        dev = self.device
+        n = int(scene_par.shape[0])
+        self.n_scenes = n
+        self.tile_scene = torch.as_tensor(tile_scene, dtype=torch.int32, device=dev).contiguous()
+        self.tile_order = torch.as_tensor(tile_order, dtype=torch.int32, device=dev).contiguous()
+        self.scene_par = torch.as_tensor(scene_par, dtype=torch.float32, device=dev).reshape(n, 8).contiguous()
+        self.scene_slot = torch.zeros(n + 1, dtype=torch.int32, device=dev)
+        self.pcg_part = torch.zeros(self.pcg_part_base + n, dtype=torch.int32, device=dev)
+        self.scalars = torch.zeros(48 * n + 64, dtype=torch.float64, device=dev)
 
 
 def byref(s):

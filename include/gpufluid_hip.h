@@ -115,6 +115,15 @@ typedef struct gf_scene {
     double* partials;            /* (gf_scene_sizes.partials) per-workgroup partial sums */
     double* scalars;             /* (gf_scene_sizes.scalars) PCG solver log */
     float* p32b; double* p64b;   /* second PCG direction buffer (ping-pong with p32/p64) */
+    /* batching: several independent scenes packed into this grid, each inside
+     * its own solid shell at a tile-aligned offset (n_scenes <= 1: one scene,
+     * the fields below unused).  Positions are then in grid units (dx = 1). */
+    int32_t n_scenes;
+    const int32_t* tile_scene;   /* (ntiles) scene of each tile, -1 outside every scene */
+    const int32_t* tile_order;   /* (ntiles) tile ids in scene-major order (compaction order) */
+    const float* scene_par;      /* (n_scenes*8) per scene: g*dt, clamp lo x,y,z, clamp hi x,y,z, 0 */
+    int32_t* scene_slot;         /* (n_scenes+1) first compact slot of each scene (sort output) */
+    int32_t* pcg_part;           /* (gf_scene_sizes.pcg_part + 2*n_scenes) PCG workgroup partition */
 } gf_scene_t;
 
 typedef struct gf_scene_size_info {
@@ -123,7 +132,8 @@ typedef struct gf_scene_size_info {
     int64_t compact;      /* ntiles*512 */
     int64_t scan_parts;   /* entries of scan_part */
     int64_t partials;     /* entries of partials */
-    int64_t scalars;      /* entries of scalars */
+    int64_t scalars;      /* entries of scalars (one scene; batched: 48*n_scenes + 64) */
+    int64_t pcg_part;     /* entries of pcg_part (one scene; batched: add n_scenes) */
 } gf_scene_size_info;
 
 /* Buffer sizes for a grid (host out-pointer). */
@@ -175,6 +185,26 @@ int gf_scene_pcg_sync(const gf_scene_t* s, int n_active, int precision, double t
                       int max_iter, int check_every, int rhs_sign, int* iters,
                       double* rel_residual, int* replacements, double* kernel_ms,
                       void* stream);
+
+/* The same solve without any host synchronisation (so it can be captured in
+ * a hipGraph): enqueues the setup when first_chunk == 0, then n_chunks chunks
+ * of check_every iterations (each with its chunk-end decision), then the
+ * result into s->pc.  Scenes that stop early pass through the remaining
+ * launches.  A later call with first_chunk = the chunks already enqueued
+ * resumes the same solve.  Results: gf_scene_pcg_stats.  No reference
+ * counterpart (the reference's graph path fixes the iteration count,
+ * solver3d.py:3361-3419). */
+int gf_scene_pcg_enqueue(const gf_scene_t* s, int n_active, int precision, double tol,
+                         int max_iter, int check_every, int rhs_sign, int first_chunk,
+                         int n_chunks, void* stream);
+
+/* Per-scene results of the last gf_scene_pcg_sync on this descriptor (sync):
+ * iterations, true relative residual and FP64 residual replacements, one
+ * entry per scene (max(1, n_scenes)); any output may be NULL.  For packed
+ * scenes gf_scene_pcg_sync itself reports the maximum iterations and residual
+ * and the total replacements.  No reference counterpart. */
+int gf_scene_pcg_stats(const gf_scene_t* s, int* iters, double* rel_residual, int* replacements,
+                       void* stream);
 
 /* Fused pressure-gradient subtract + solid BC + G2P + advection over 8^3
  * tiles: k3_subtract_pressure_grad (solver3d.py:1364-1407), k3_enforce_solid_bc
