@@ -31,12 +31,18 @@ BYTES = {
     "sort": lambda np_, nf, na: 68 * np_ + 12 * na,
     "p2g": lambda np_, nf, na: 24 * np_ + 29 * nf,
     "g2p": lambda np_, nf, na: 48 * np_ + 41 * nf,
-    # r, p_old, flags in; p_new, Ap out (FP32)
+    # two-kernel PCG iteration (FP32 vectors, fluid cells only):
+    # direction+SpMV: z, p_old, flags in; p_new, Ap out
     "pcg_dirspmv": lambda np_, nf, na: 17 * nf,
-    # p, Ap, r, x32, flags in; r, x32 out
+    # update: p, Ap, z, x32, flags in; z, x32 out
     "pcg_update": lambda np_, nf, na: 25 * nf,
-    # x64, x32, b, flags in; r out
+    # FP64 replacement: x64, x32, b, flags in; z out
     "pcg_replace": lambda np_, nf, na: 21 * nf,
+    # one-kernel (Chronopoulos-Gear) iteration: u, w~, s~, p, x32, flags in;
+    # u, w~, s~, p, x32 out
+    "pcg_iter": lambda np_, nf, na: 41 * nf,
+    # its replacement: x64, x32, b, p, flags in -> u, s~ out; then u, flags in -> w~ out
+    "pcg_replace_cg1": lambda np_, nf, na: 38 * nf,
 }
 
 
@@ -44,8 +50,8 @@ BYTES = {
 # by a plain-copy calibration) for this workload; the file states its commands.
 TRAFFIC_FILE = ROOT / "profiles" / "pmc_traffic_substep_pool_sphere_256.txt"
 TRAFFIC_KERNEL = {"pcg_dirspmv": "k_pcg_dirspmv<float,true>", "pcg_update": "k_pcg_update<float,true>",
-                  "pcg_replace": "k_pcg_replace<true>", "p2g": "k_p2g_tile<false,false>",
-                  "g2p": "k_g2p_tile<false>"}
+                  "pcg_replace": "k_pcg_replace<true>", "pcg_iter": "k_cg1_iter<float,true>",
+                  "p2g": "k_p2g_tile<false,false>", "g2p": "k_g2p_tile<false>"}
 
 
 def measured_traffic(stage):
@@ -203,12 +209,19 @@ def main():
     nf = float(np.mean([st["nf"] for st in stats]))
     na = float(np.mean([st["na"] for st in stats])) * 512
     iters = [st["iters"] for st in stats]
-    kms = np.sum([st["kms"] for st in stats], axis=0)  # dirspmv, update, -, replace (ms)
+    # iteration kernel (two-kernel form: direction+SpMV), update (two-kernel
+    # form only), -, replacement (ms)
+    kms = np.sum([st["kms"] for st in stats], axis=0)
     n_it = sum(iters)
     n_rep = sum(st["nrep"] for st in stats)
+    two_kernel = kms[1] > 0
+    if two_kernel:
+        pcg_kern = {"pcg_dirspmv": (kms[0], n_it), "pcg_update": (kms[1], n_it),
+                    "pcg_replace": (kms[3], n_rep)}
+    else:
+        pcg_kern = {"pcg_iter": (kms[0], n_it), "pcg_replace_cg1": (kms[3], n_rep)}
     kern = {
-        "pcg_dirspmv": (kms[0], n_it), "pcg_update": (kms[1], n_it),
-        "pcg_replace": (kms[3], n_rep),
+        **pcg_kern,
         "sort": (sec.get("clear", 0.0), args.steps), "p2g": (sec.get("p2g", 0.0), args.steps),
         "g2p": (sec.get("g2p_advect", 0.0), args.steps),
     }
@@ -222,8 +235,9 @@ def main():
                      "alg_bytes": int(b), "GBps": round(b / (avg_ms * 1e-3) / 1e9, 1),
                      "frac_hbm": round(b / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     pcg_ms = float(kms.sum())
-    pcg_bytes = n_it * (17 + 25) * nf + n_rep * 21 * nf
-    dom = max(("pcg_dirspmv", "pcg_update"), key=lambda k: kern[k][0])
+    pcg_bytes = sum(BYTES[k](npart, nf, na) * calls for k, (_, calls) in pcg_kern.items())
+    dom = max(("pcg_dirspmv", "pcg_update") if two_kernel else ("pcg_iter",),
+              key=lambda k: kern[k][0])
     d = stages[dom]
     roofline = {"bound": "hbm", "achieved": d["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": d["frac_hbm"],
@@ -395,8 +409,8 @@ def adjoint_leg(fd_dirs=16, eps=1e-3, reference=False):
 def batch_leg(substeps=20, alone_substeps=2, tol=1e-8, max_iter=500):
     """256 independent scenes (edges uniform in {24,32,48,64,96}, fluid boxes
     with sides U[0.2,0.7] from seed 4, 8 particles per cell, FLIP, dt 1/240,
-    mixed PCG to 1e-8 per scene): packed into one grid, PCG grid size chosen
-    by the autotuner, ``substeps`` substeps replayed as captured hipGraph pairs
+    mixed PCG to 1e-8 per scene): packed into one grid, PCG iteration form
+    and grid size chosen by the autotuner, ``substeps`` substeps replayed as captured hipGraph pairs
     with an autotuned PCG budget.  Compared with the same scenes run one at a
     time through FlipSolver3D (``alone_substeps`` timed substeps each)."""
     import ctypes as C
@@ -418,7 +432,7 @@ def batch_leg(substeps=20, alone_substeps=2, tol=1e-8, max_iter=500):
         _lib.call("bench", "gf_scene_pcg_sync", C.byref(d), 0, _lib.GF_PCG_MIXED, tol, max_iter,
                   s.pcg_check_every, -1, None, None, None, None, s._stream())
 
-    wg, tune_times = autotune.tune_pcg_workgroups(solve, key=("batch", b.pack.shape, b.n_scenes))
+    (variant, wg), tune_times = autotune.tune_pcg(solve, key=("batch", b.pack.shape, b.n_scenes))
     budget = autotune.graph_budget(b.last_iters, s.pcg_check_every, max_iter)
     b.step2_graph(max_iter, tol, budget_iters=budget)      # capture + one untimed pair
     torch.cuda.synchronize()
@@ -440,6 +454,7 @@ def batch_leg(substeps=20, alone_substeps=2, tol=1e-8, max_iter=500):
     del b, s
     torch.cuda.empty_cache()
     autotune.set_pcg_workgroups(0)
+    autotune.set_pcg_variant(0)
     # one scene at a time
     t_alone = 0.0
     for sc in scenes:
@@ -465,6 +480,7 @@ def batch_leg(substeps=20, alone_substeps=2, tol=1e-8, max_iter=500):
             "unit": "Mparticle-substeps/s",
             "speedup_vs_one_at_a_time": round(graph_v / alone_v, 2),
             "pcg_workgroups_autotuned": wg,
+            "pcg_variant_autotuned": {1: "two_kernel", 2: "one_kernel"}[variant],
             "autotune_seconds": {str(k): round(v, 5) for k, v in tune_times.items()},
             "graph_pcg_budget_iters": budget, "max_pcg_iters_per_substep": iters,
             "worst_scene_rel_residual": worst_res}

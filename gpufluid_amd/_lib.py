@@ -49,7 +49,7 @@ class GfScene(ctypes.Structure):
         ("x64", c_void_p), ("r64", c_void_p), ("p64", c_void_p), ("ap64", c_void_p),
         ("pc", c_void_p), ("ptmp", c_void_p),
         ("partials", c_void_p), ("scalars", c_void_p),
-        ("p32b", c_void_p), ("p64b", c_void_p),
+        ("p32b", c_void_p), ("p64b", c_void_p), ("cg32", c_void_p), ("cg64", c_void_p),
         ("n_scenes", c_int32), ("tile_scene", c_void_p), ("tile_order", c_void_p),
         ("scene_par", c_void_p), ("scene_slot", c_void_p), ("pcg_part", c_void_p),
     ]

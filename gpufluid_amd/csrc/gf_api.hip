@@ -36,6 +36,7 @@ int tunable(int id) { return g_tunables[id].load(std::memory_order_relaxed); }
 static int tune_id(const char* key) {
     if (!key) return -1;
     if (!strcmp(key, "pcg_workgroups")) return gf::GF_TUNE_PCG_WORKGROUPS;
+    if (!strcmp(key, "pcg_variant")) return gf::GF_TUNE_PCG_VARIANT;
     return -1;
 }
 
@@ -87,7 +88,7 @@ extern "C" int gf_scene_sizes(int nx, int ny, int nz, gf_scene_size_info* out) {
     out->ntiles = ntiles;
     out->compact = ntiles * GF_TILE_CELLS;
     out->scan_parts = (cells + 1 + 4095) / 4096 + 1;
-    out->partials = 8 * 2048;  // per-workgroup partial sums of the PCG reductions
+    out->partials = 2 * 8 * 2048;  // per-workgroup PCG partial sums, two launch-parity sets
     out->scalars = 64;         // three 16-double solver log entries (two live, one result), spare
     out->pcg_part = 8 * 2048 + 2;  // per-workgroup scene/tile ranges + per-scene offsets
     return 0;

@@ -14,7 +14,7 @@ int set_error(int code, const char* fmt, ...);
 int check_launch(const char* what);
 
 // Launch-shape tunables set through gf_tune_set (0 = built-in choice).
-enum { GF_TUNE_PCG_WORKGROUPS = 0, GF_TUNE_COUNT };
+enum { GF_TUNE_PCG_WORKGROUPS = 0, GF_TUNE_PCG_VARIANT, GF_TUNE_COUNT };
 int tunable(int id);
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) {

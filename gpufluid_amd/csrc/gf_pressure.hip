@@ -7,22 +7,32 @@
 // equals diag*x minus the sum over all six neighbours, with diag = number of
 // in-grid non-solid neighbours stored in the flags byte.
 //
-// PCG (solver3d.py:2917-3043 restated): Jacobi preconditioner max(diag,1)
+// PCG (solver3d.py:2917-3043 restated): Jacobi preconditioner M = max(diag,1)
 // (k3_compute_diag, 1803-1823), x0 = 0, stop on ||r||_2 < tol*||b||_2,
 // alpha = 0 when |pAp| < 1e-30 and beta = 0 when |rz| < 1e-30 (1883-1896).
-// Two kernels per iteration; every scalar stays on the device:
-//   dirspmv: beta = rz/rz_old, p = z + beta p (ping-pong buffers; the stencil
-//            neighbours' p is recomputed from their z and old p), Ap = A p, p.Ap
-//   update:  alpha = rz/pAp, x += alpha p, r = M z - alpha Ap, z = r/M -> r.z, r.r
-// The residual is stored preconditioned (z = r/M, M = max(diag,1)), so the
-// fused direction kernel needs no divide and no neighbour flags; the update
-// rebuilds r = M z.  The stop / replacement decisions on r.r are taken by the
-// next launch.
-// Dot products are FP64 with a deterministic two-level reduction.  In the
+// A kernel boundary is the only grid-wide sync here, and two forms of the
+// iteration are built:
+//  * two kernels per iteration (standard PCG, the default above
+//    PCG_CG1_MAX_TILES active tiles): a direction+SpMV kernel
+//    (p = z + beta p, Ap, p.Ap) and an update kernel (x += alpha p,
+//    r -= alpha Ap, z = M^-1 r, r.z, r.r); r is held as z = M^-1 r;
+//  * one kernel per iteration (Chronopoulos-Gear form, below that size, where
+//    the fixed cost of a launch dominates), which needs one global reduction
+//    per iteration instead of two:
+//   gamma = (r,u), delta = (Au,u)        (reduced by the previous kernel)
+//   beta = gamma/gamma_old, alpha = gamma / (delta - beta*gamma/alpha_old)
+//   p = u + beta p,  s~ = w~ + beta s~,  x += alpha p,  u -= alpha s~,
+//   w~ = M^-1 A u  (stencil: the neighbours' new u is recomputed from their
+//                   u, w~, s~, ping-ponged),  partials of the next gamma,
+//   delta and r.r
+// with u = M^-1 r, w~ = M^-1 A u, s~ = M^-1 A p stored (r = M u is never
+// stored).  In exact arithmetic the iterates equal standard PCG's.  Dot
+// products are FP64 with a deterministic two-level reduction.  In the
 // mixed-precision mode the vectors are FP32 and an FP64 iterate x64 + x32 is
-// kept; the residual is replaced by the FP64 true residual b - A(x64+x32)
-// whenever ||r|| has dropped by 10x since the last replacement and before
-// convergence is accepted (reliable updates, Sleijpen & van der Vorst 1996).
+// kept; when ||r|| has dropped 10x since the last replacement, and before
+// convergence is accepted, the residual vectors (z; or u, s~ and w~) are
+// rebuilt from the FP64 true residual b - A(x64+x32) (reliable updates,
+// Sleijpen & van der Vorst 1996).
 #include "gf_common.h"
 #include <string.h>
 
@@ -43,19 +53,30 @@ struct PcgLog {
 };
 static_assert(sizeof(PcgLog) == 16 * sizeof(double), "log entry is 16 doubles");
 
-// Per-workgroup partial sums, PCG_GMAX entries per kind.
+// Per-workgroup partial sums, PCG_GMAX entries per kind, in two sets chosen by
+// the parity of the writing launch: a kernel may read a kind that it also
+// writes (the iteration kernel consumes and produces gamma, delta, r.r), and
+// without the second set a workgroup that finishes early would overwrite its
+// partial before a late-starting workgroup of the same launch had read it.
 enum { P_PAP = 0, P_RZ, P_RR, P_BB, P_RRT, P_RZT, P_KINDS };
+constexpr int P_SET = 8;  // kinds per parity set (P_KINDS rounded up)
 constexpr int PCG_GMAX = 2048;
+enum { PCG_STD = 1, PCG_CG1 = 2 };
+// Below this many active tiles the one-kernel iteration is faster (its fixed
+// per-launch cost is paid once per iteration); above it the two-kernel form
+// streams more efficiently (measured on MI355X: 1.66 K tiles 34 vs 47 us per
+// iteration, 5.8 K tiles about equal, 13.3 K tiles 96 vs 85 us).
+constexpr int PCG_CG1_MAX_TILES = 4096;
 constexpr int PCG_DEFAULT_G = 1024;
 
 struct PcgArgs {
     const uint8_t* flags;
     const int32_t* nbr;
     const float* b;
-    float* x32; float* r32; float* ap32;
-    double* x64; double* r64; double* ap64;
-    float* p32[2];       // ping-pong directions
-    double* p64[2];
+    float* u32[2]; float* w32[2]; float* s32[2];   // ping-pong u, w~, s~
+    float* p32; float* x32;
+    double* u64[2]; double* w64[2]; double* s64[2];
+    double* p64; double* x64;
     float* pc;
     double* part;        // [P_KINDS][PCG_GMAX]
     PcgLog* log;         // [2*S] live pairs, then [S] results
@@ -63,6 +84,7 @@ struct PcgArgs {
     const int32_t* scene_slot;  // [S+1] compact tile offsets per scene (NULL: one scene, nact tiles)
     int nact;            // active tiles (one scene)
     int S;               // scenes
+    int variant;         // PCG_STD (two kernels per iteration) or PCG_CG1 (one)
     int G;               // workgroups; a scene's workgroups split its tiles evenly
     float rhs_sign;
     double tol2, delta2;
@@ -102,6 +124,26 @@ __device__ __forceinline__ void st4(T* p, const T (&o)[4]) {
     typename V4<T>::t v;
     v.x = o[0]; v.y = o[1]; v.z = o[2]; v.w = o[3];
     *reinterpret_cast<typename V4<T>::t*>(p) = v;
+}
+
+// Streamed outputs are stored non-temporally: they are read again only by
+// the next kernel (possibly on another XCD), so keeping them dirty in L2 only
+// adds a write-back at the end of the kernel.
+typedef float gf_f4 __attribute__((ext_vector_type(4)));
+typedef double gf_d2 __attribute__((ext_vector_type(2)));
+
+template <typename T>
+__device__ __forceinline__ void st4_nt(T* p, const T (&o)[4]);
+template <>
+__device__ __forceinline__ void st4_nt<float>(float* p, const float (&o)[4]) {
+    gf_f4 v = {o[0], o[1], o[2], o[3]};
+    __builtin_nontemporal_store(v, reinterpret_cast<gf_f4*>(p));
+}
+template <>
+__device__ __forceinline__ void st4_nt<double>(double* p, const double (&o)[4]) {
+    gf_d2 v0 = {o[0], o[1]}, v1 = {o[2], o[3]};
+    __builtin_nontemporal_store(v0, reinterpret_cast<gf_d2*>(p));
+    __builtin_nontemporal_store(v1, reinterpret_cast<gf_d2*>(p + 2));
 }
 
 __device__ __forceinline__ uint32_t ld_flags(const uint8_t* f, size_t idx) {
@@ -199,6 +241,14 @@ struct Part {
 __device__ __forceinline__ Part part_of(const PcgArgs& a) {
     Part P;
     P.b = gf::xcd_remap(blockIdx.x, a.G);
+    if (a.S == 1) {  // one scene: the partition is b*nact/G, no table read
+        P.s = 0;
+        P.wb = 0;
+        P.we = a.G;
+        P.t0 = (int)(((long long)P.b * a.nact) / a.G);
+        P.t1 = (int)(((long long)(P.b + 1) * a.nact) / a.G);
+        return P;
+    }
     const int4 q = *reinterpret_cast<const int4*>(a.part_info + 8 * P.b);
     const int t1 = a.part_info[8 * P.b + 4];
     P.s = __builtin_amdgcn_readfirstlane(q.x);
@@ -221,31 +271,77 @@ __device__ __forceinline__ double wg_sum(double v, double* s_red) {
     return t;
 }
 
-// Sum of one kind of partial over a scene's workgroups [wb, we)
-// (deterministic, identical in each of them).  Written by the previous
-// kernel, so visible at this launch.
-__device__ __forceinline__ double sum_partials(const double* part, int kind, int wb, int we,
-                                               double* s_red) {
-    double v = 0.0;
-    for (int i = wb + threadIdx.x; i < we; i += GF_WG) v += part[kind * PCG_GMAX + i];
-    return wg_sum(v, s_red);
+// Workgroup sums of two values per thread with one barrier pair.
+__device__ __forceinline__ void wg_sum2(double& x, double& y, double* s_red) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    x = gf::wave_sum(x);
+    y = gf::wave_sum(y);
+    __syncthreads();
+    if (lane == 0) {
+        s_red[wave] = x;
+        s_red[GF_WG / GF_WAVE + wave] = y;
+    }
+    __syncthreads();
+    double tx = 0.0, ty = 0.0;
+    for (int i = 0; i < GF_WG / GF_WAVE; ++i) {
+        tx += s_red[i];
+        ty += s_red[GF_WG / GF_WAVE + i];
+    }
+    x = tx;
+    y = ty;
 }
 
-__device__ __forceinline__ void put_partial(double* part, int kind, int b, double v,
+// This thread's share of one kind of partial over [wb, we): at most
+// PCG_GMAX / GF_WG independent loads, issued together.
+__device__ __forceinline__ double* pslot(double* part, int kind, int L) {
+    return part + ((size_t)((L & 1) * P_SET + kind)) * PCG_GMAX;
+}
+__device__ __forceinline__ const double* pslot(const double* part, int kind, int L) {
+    return part + ((size_t)((L & 1) * P_SET + kind)) * PCG_GMAX;
+}
+
+// This thread's share of one kind of partial written by launch L, over the
+// scene's workgroups [wb, we): at most PCG_GMAX / GF_WG independent loads.
+__device__ __forceinline__ double partial_share(const double* part, int kind, int L, int wb,
+                                                int we) {
+    const double* src = pslot(part, kind, L);
+    double v = 0.0;
+#pragma unroll
+    for (int q = 0; q < PCG_GMAX / GF_WG; ++q) {
+        const int i = wb + threadIdx.x + q * GF_WG;
+        v += i < we ? src[i] : 0.0;
+    }
+    return v;
+}
+
+__device__ __forceinline__ void put_partial2(double* part, int ka, int kb, int L, int b,
+                                             double va, double vb, double* s_red) {
+    wg_sum2(va, vb, s_red);
+    if (threadIdx.x == 0) {
+        pslot(part, ka, L)[b] = va;
+        pslot(part, kb, L)[b] = vb;
+    }
+}
+
+__device__ __forceinline__ void put_partial(double* part, int kind, int L, int b, double v,
                                             double* s_red) {
     const double t = wg_sum(v, s_red);
-    if (threadIdx.x == 0) part[kind * PCG_GMAX + b] = t;
+    if (threadIdx.x == 0) pslot(part, kind, L)[b] = t;
 }
 
-template <typename T> __device__ __forceinline__ T* vr(const PcgArgs& a);
-template <> __device__ __forceinline__ float* vr<float>(const PcgArgs& a) { return a.r32; }
-template <> __device__ __forceinline__ double* vr<double>(const PcgArgs& a) { return a.r64; }
-template <typename T> __device__ __forceinline__ T* vp(const PcgArgs& a, int buf);
-template <> __device__ __forceinline__ float* vp<float>(const PcgArgs& a, int buf) { return a.p32[buf]; }
-template <> __device__ __forceinline__ double* vp<double>(const PcgArgs& a, int buf) { return a.p64[buf]; }
-template <typename T> __device__ __forceinline__ T* vap(const PcgArgs& a);
-template <> __device__ __forceinline__ float* vap<float>(const PcgArgs& a) { return a.ap32; }
-template <> __device__ __forceinline__ double* vap<double>(const PcgArgs& a) { return a.ap64; }
+template <typename T> struct Vec;
+template <> struct Vec<float> {
+    static __device__ __forceinline__ float* u(const PcgArgs& a, int b) { return a.u32[b]; }
+    static __device__ __forceinline__ float* w(const PcgArgs& a, int b) { return a.w32[b]; }
+    static __device__ __forceinline__ float* s(const PcgArgs& a, int b) { return a.s32[b]; }
+    static __device__ __forceinline__ float* p(const PcgArgs& a) { return a.p32; }
+};
+template <> struct Vec<double> {
+    static __device__ __forceinline__ double* u(const PcgArgs& a, int b) { return a.u64[b]; }
+    static __device__ __forceinline__ double* w(const PcgArgs& a, int b) { return a.w64[b]; }
+    static __device__ __forceinline__ double* s(const PcgArgs& a, int b) { return a.s64[b]; }
+    static __device__ __forceinline__ double* p(const PcgArgs& a) { return a.p64; }
+};
 
 enum { PEND_NONE = 0, PEND_SETUP = 1, PEND_REPLACE = 2 };
 
@@ -275,17 +371,64 @@ __device__ __forceinline__ void dir4(const T (&z)[4], const T (&po)[4], T bf, T 
     for (int k = 0; k < 4; ++k) pn[k] = fma(bf, po[k], z[k]);
 }
 
-// Consumes the update kernel's r.z / r.r partials (when fresh): counts the
-// finished iteration and sets the stop / replacement flags.  Identical in
-// every workgroup of the scene; its first workgroup writes the log.
+// Workgroup sums of three values per thread with one barrier pair.
+__device__ __forceinline__ void wg_sum3(double& x, double& y, double& z, double* s_red) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int NW = GF_WG / GF_WAVE;
+    x = gf::wave_sum(x);
+    y = gf::wave_sum(y);
+    z = gf::wave_sum(z);
+    __syncthreads();
+    if (lane == 0) {
+        s_red[wave] = x;
+        s_red[NW + wave] = y;
+        s_red[2 * NW + wave] = z;
+    }
+    __syncthreads();
+    double tx = 0.0, ty = 0.0, tz = 0.0;
+    for (int i = 0; i < NW; ++i) {
+        tx += s_red[i];
+        ty += s_red[NW + i];
+        tz += s_red[2 * NW + i];
+    }
+    x = tx;
+    y = ty;
+    z = tz;
+}
+
+__device__ __forceinline__ void put_partial3(double* part, int ka, int kb, int kc, int L, int b,
+                                             double va, double vb, double vc, double* s_red) {
+    wg_sum3(va, vb, vc, s_red);
+    if (threadIdx.x == 0) {
+        pslot(part, ka, L)[b] = va;
+        pslot(part, kb, L)[b] = vb;
+        pslot(part, kc, L)[b] = vc;
+    }
+}
+
+__device__ __forceinline__ PcgLog& live_log(const PcgArgs& a, int s, int L) {
+    return a.log[2 * s + (L & 1)];
+}
+
+__device__ __forceinline__ void write_log(const PcgArgs& a, const Part& P, const PcgLog& e,
+                                          int L) {
+    if (P.b == P.wb && threadIdx.x == 0) live_log(a, P.s, L) = e;
+}
+
+// Consumes the last iteration's partials (when fresh): gamma = (r,u),
+// delta = (Au,u), r.r; counts the finished iteration and sets the stop /
+// replacement flags.  Identical in every workgroup of the scene.
 template <bool MIXED>
-__device__ __forceinline__ void consume(const PcgArgs& a, const Part& P, PcgLog& e,
+__device__ __forceinline__ void consume(const PcgArgs& a, const Part& P, PcgLog& e, int L,
                                         double* s_red) {
     if (!e.fresh) return;
-    const double rz_new = sum_partials(a.part, P_RZ, P.wb, P.we, s_red);
-    const double rr = sum_partials(a.part, P_RR, P.wb, P.we, s_red);
+    double gam = partial_share(a.part, P_RZ, L - 1, P.wb, P.we);
+    double del = partial_share(a.part, P_PAP, L - 1, P.wb, P.we);
+    double rr = partial_share(a.part, P_RR, L - 1, P.wb, P.we);
+    wg_sum3(gam, del, rr, s_red);
     e.rz_old = e.rz;
-    e.rz = rz_new;
+    e.rz = gam;
+    e.pap = del;
     e.rr = rr;
     e.fresh = 0;
     e.iters += 1;
@@ -301,13 +444,11 @@ __device__ __forceinline__ void consume(const PcgArgs& a, const Part& P, PcgLog&
     }
 }
 
-__device__ __forceinline__ PcgLog& live_log(const PcgArgs& a, int s, int L) {
-    return a.log[2 * s + (L & 1)];
-}
-
-__device__ __forceinline__ void write_log(const PcgArgs& a, const Part& P, const PcgLog& e,
-                                          int L) {
-    if (P.b == P.wb && threadIdx.x == 0) live_log(a, P.s, L) = e;
+__device__ __forceinline__ bool pass_through(const PcgArgs& a, const Part& P, const PcgLog& e,
+                                             int L) {
+    if (!e.done) return false;
+    write_log(a, P, e, L);
+    return true;
 }
 
 // Workgroup partition of a solve (one workgroup): scene s gets
@@ -316,7 +457,7 @@ __device__ __forceinline__ void write_log(const PcgArgs& a, const Part& P, const
 // even share of its tiles.  Also marks every scene's log as awaiting the setup.
 __global__ __launch_bounds__(GF_WG) void k_pcg_partition(PcgArgs a) {
     __shared__ int s_wave[GF_WG / GF_WAVE + 1];
-    __shared__ double s_red[GF_WG / GF_WAVE];
+    __shared__ double s_red[2 * GF_WG / GF_WAVE];
     int32_t* wg_off = a.part_info + 8 * PCG_GMAX;
     long long tiles = 0, ne = 0;
     for (int sc = threadIdx.x; sc < a.S; sc += GF_WG) {
@@ -359,15 +500,368 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_partition(PcgArgs a) {
     }
 }
 
-// L = 0: x = 0, r = rhs_sign*b, z = r/M stored, p (buffer 0) = 0; partials
-// r.z and r.r.
+// y = A v on a quad (fluid cells; 0 elsewhere) given its neighbour values.
+template <typename T>
+__device__ __forceinline__ void apply_A4(const T (&c)[4], const Nb4<T>& o, uint32_t fw,
+                                         T (&y)[4]) {
+    for (int k = 0; k < 4; ++k)
+        y[k] = keep_if((T)fl_diag(fw, k) * c[k] - nb_sum4(o, c, k), fl_fluid(fw, k));
+}
+
+// Setup, pass 1: x = 0, p = 0, s~ = 0, u = M^-1 (rhs_sign * b) on fluid cells.
 template <typename T, bool MIXED>
-__global__ __launch_bounds__(GF_WG) void k_pcg_setup(PcgArgs a) {
-    __shared__ double s_red[GF_WG / GF_WAVE];
+__global__ __launch_bounds__(GF_WG) void k_cg1_setup(PcgArgs a) {
     const Part P = part_of(a);
     if (P.s < 0) return;
-    T* r = vr<T>(a);
-    T* p = vp<T>(a, 0);
+    T* u = Vec<T>::u(a, 0);
+    T* sv = Vec<T>::s(a, 0);
+    T* p = Vec<T>::p(a);
+    FOR_QUADS(P) {
+        const size_t idx = (size_t)t * GF_TILE_CELLS + Q.l0;
+        const uint32_t fw = ld_flags(a.flags, idx);
+        float bv[4];
+        ld4(a.b + idx, bv);
+        T uv[4], zero[4] = {(T)0, (T)0, (T)0, (T)0};
+        double zd[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int e = 0; e < 4; ++e) {
+            const T rv = fl_fluid(fw, e) ? (T)(a.rhs_sign * bv[e]) : (T)0;
+            uv[e] = zdiv(rv, fw >> (8 * e));
+        }
+        st4(u + idx, uv);
+        st4(p + idx, zero);
+        st4(sv + idx, zero);
+        st4(a.x64 + idx, zd);
+        if (MIXED) {
+            float zf[4] = {0.f, 0.f, 0.f, 0.f};
+            st4(a.x32 + idx, zf);
+        }
+    }
+}
+
+// Setup, pass 2: w~ = M^-1 A u; partials gamma0 = (r,u), delta0 = (Au,u), b.b.
+template <typename T>
+__global__ __launch_bounds__(GF_WG) void k_cg1_setup2(PcgArgs a) {
+    __shared__ double s_red[3 * GF_WG / GF_WAVE];
+    const Part P = part_of(a);
+    if (P.s < 0) return;
+    const T* __restrict__ u = Vec<T>::u(a, 0);
+    T* __restrict__ w = Vec<T>::w(a, 0);
+    double gam = 0.0, del = 0.0, rr = 0.0;
+    FOR_QUADS(P) {
+        const size_t base = (size_t)t * GF_TILE_CELLS;
+        const size_t idx = base + Q.l0;
+        const uint32_t fw = ld_flags(a.flags, idx);
+        T c[4], y[4], wv[4];
+        ld4(u + idx, c);
+        Nb4<T> o;
+        load_nb4(u, nb_index(a.nbr, t, base, Q), o);
+        apply_A4(c, o, fw, y);
+        for (int k = 0; k < 4; ++k) {
+            const T dm = diagm<T>(fw >> (8 * k));
+            wv[k] = y[k] / dm;
+            const double r = (double)dm * (double)c[k];
+            gam += r * (double)c[k];
+            del += (double)y[k] * (double)c[k];
+            rr += r * r;
+        }
+        st4(w + idx, wv);
+    }
+    put_partial3(a.part, P_RZ, P_PAP, P_BB, 0, P.b, gam, del, rr, s_red);  // read by fin at L = 1
+}
+
+// Finalisation after the setup or a residual replacement: one workgroup per
+// scene (grid S).
+__global__ __launch_bounds__(GF_WG) void k_pcg_fin(PcgArgs a, int L) {
+    __shared__ double s_red[3 * GF_WG / GF_WAVE];
+    const int sc = blockIdx.x;
+    const int32_t* wg_off = a.part_info + 8 * PCG_GMAX;
+    const int wb = wg_off[sc], we = wg_off[sc + 1];
+    PcgLog e = live_log(a, sc, L - 1);
+    if (e.pending == PEND_SETUP) {  // setup2 wrote launch 0's set; fin runs at L = 1
+        double gam = partial_share(a.part, P_RZ, L - 1, wb, we);
+        double del = partial_share(a.part, P_PAP, L - 1, wb, we);
+        double bb = partial_share(a.part, P_BB, L - 1, wb, we);
+        wg_sum3(gam, del, bb, s_red);
+        e.rz = gam; e.pap = del; e.rz_old = 0.0; e.alpha = 0.0;
+        e.bb = bb; e.rr = bb; e.rr_upd = bb; e.rr_true = bb;
+        e.iters = 0; e.maxed = 0; e.upd = 0; e.merge = 0; e.nupd = 0; e.fresh = 0; e.pbuf = 0;
+        const int zero = bb < 1e-30;
+        e.converged = zero;
+        e.done = zero || a.max_iter <= 0;
+    } else if (e.pending == PEND_REPLACE) {
+        // one-kernel form: replace ran at L - 2, replace2 at L - 1;
+        // two-kernel form: replace at L - 1 (delta unused)
+        const bool cg1 = a.variant == PCG_CG1;
+        double rrt = partial_share(a.part, P_RRT, cg1 ? L - 2 : L - 1, wb, we);
+        double gam = partial_share(a.part, P_RZT, L - 1, wb, we);
+        double del = cg1 ? partial_share(a.part, P_BB, L - 1, wb, we) : 0.0;
+        wg_sum3(rrt, gam, del, s_red);
+        e.rr_true = rrt; e.rr = rrt; e.rr_upd = rrt; e.rz = gam; e.pap = del;
+        e.upd = 0; e.merge = 1; e.nupd += 1;
+        if (rrt < a.tol2 * e.bb) { e.converged = 1; e.done = 1; }
+        else if (e.iters >= a.max_iter) { e.maxed = 1; e.done = 1; }
+        else e.done = 0;
+    }
+    e.pending = PEND_NONE;
+    if (threadIdx.x == 0) live_log(a, sc, L) = e;
+}
+
+// u_new = u - alpha (w~ + beta s~) for a quad's neighbour values (the same
+// two fmas as the owner uses, so both sides agree bit for bit).
+template <typename T>
+__device__ __forceinline__ void unew4(const T (&u)[4], const T (&w)[4], const T (&sv)[4], T al,
+                                      T be, T (&o)[4]) {
+    for (int k = 0; k < 4; ++k) o[k] = fma(-al, fma(be, sv[k], w[k]), u[k]);
+}
+
+// One iteration (see the header): reads u, w~, s~ of buffer e.pbuf (own
+// cells and stencil neighbours), p and x; writes p, x in place and u, w~, s~
+// to the other buffer; partials of the next gamma, delta and r.r.
+template <typename T, bool MIXED>
+__global__ __launch_bounds__(GF_WG) void k_cg1_iter(PcgArgs a, int L) {
+    __shared__ double s_red[3 * GF_WG / GF_WAVE];
+    const Part P = part_of(a);
+    if (P.s < 0) return;
+    PcgLog e = live_log(a, P.s, L - 1);
+    if (pass_through(a, P, e, L)) return;
+    consume<MIXED>(a, P, e, L, s_red);
+    if (e.done) {
+        write_log(a, P, e, L);
+        return;
+    }
+    const double beta = (e.iters == 0 || fabs(e.rz_old) < 1e-30) ? 0.0 : e.rz / e.rz_old;
+    double den = e.pap;
+    if (beta != 0.0 && e.alpha != 0.0) den -= beta * e.rz / e.alpha;
+    const double alpha = fabs(den) < 1e-30 ? 0.0 : e.rz / den;
+    const T be = (T)beta, al = (T)alpha;
+    const int cur = e.pbuf, nxt = e.pbuf ^ 1;
+    const T* __restrict__ u = Vec<T>::u(a, cur);
+    const T* __restrict__ w = Vec<T>::w(a, cur);
+    const T* __restrict__ sv = Vec<T>::s(a, cur);
+    T* __restrict__ un = Vec<T>::u(a, nxt);
+    T* __restrict__ wn = Vec<T>::w(a, nxt);
+    T* __restrict__ sn = Vec<T>::s(a, nxt);
+    T* __restrict__ p = Vec<T>::p(a);
+    const int merge = MIXED ? e.merge : 0;
+    double gam = 0.0, del = 0.0, rr = 0.0;
+    FOR_QUADS(P) {
+        const size_t base = (size_t)t * GF_TILE_CELLS;
+        const size_t idx = base + Q.l0;
+        const NbIdx ni = nb_index(a.nbr, t, base, Q);
+        const uint32_t fw = ld_flags(a.flags, idx);
+        T uq[4], wq[4], sq[4], pq[4];
+        ld4(u + idx, uq);
+        ld4(w + idx, wq);
+        ld4(sv + idx, sq);
+        ld4(p + idx, pq);
+        Nb4<T> nu, nw, ns;
+        load_nb4(u, ni, nu);
+        load_nb4(w, ni, nw);
+        load_nb4(sv, ni, ns);
+        // own cells
+        T pn[4], snew[4], c[4];
+        for (int k = 0; k < 4; ++k) {
+            pn[k] = fma(be, pq[k], uq[k]);
+            snew[k] = fma(be, sq[k], wq[k]);
+            c[k] = fma(-al, snew[k], uq[k]);
+        }
+        if (MIXED) {
+            float xo[4];
+            ld4(a.x32 + idx, xo);
+            if (merge) {
+                double xd[4];
+                ld4(a.x64 + idx, xd);
+                for (int k = 0; k < 4; ++k) {
+                    xd[k] += (double)xo[k];
+                    xo[k] = 0.0f;
+                }
+                st4(a.x64 + idx, xd);
+            }
+            for (int k = 0; k < 4; ++k) xo[k] = xo[k] + (float)al * (float)pn[k];
+            st4_nt(a.x32 + idx, xo);
+        } else {
+            double xd[4];
+            ld4(a.x64 + idx, xd);
+            for (int k = 0; k < 4; ++k) xd[k] += (double)al * (double)pn[k];
+            st4_nt(a.x64 + idx, xd);
+        }
+        // neighbours' new u
+        Nb4<T> o;
+        unew4(nu.xm, nw.xm, ns.xm, al, be, o.xm);
+        unew4(nu.xp, nw.xp, ns.xp, al, be, o.xp);
+        unew4(nu.ym, nw.ym, ns.ym, al, be, o.ym);
+        unew4(nu.yp, nw.yp, ns.yp, al, be, o.yp);
+        o.zm0 = fma(-al, fma(be, ns.zm0, nw.zm0), nu.zm0);
+        o.zp3 = fma(-al, fma(be, ns.zp3, nw.zp3), nu.zp3);
+        T y[4], wv[4];
+        apply_A4(c, o, fw, y);
+        for (int k = 0; k < 4; ++k) {
+            const T dm = diagm<T>(fw >> (8 * k));
+            wv[k] = y[k] / dm;
+            const double r = (double)dm * (double)c[k];
+            gam += r * (double)c[k];
+            del += (double)y[k] * (double)c[k];
+            rr += r * r;
+        }
+        st4_nt(p + idx, pn);
+        st4_nt(un + idx, c);
+        st4_nt(wn + idx, wv);
+        st4_nt(sn + idx, snew);
+    }
+    put_partial3(a.part, P_RZ, P_PAP, P_RR, L, P.b, gam, del, rr, s_red);
+    e.alpha = alpha;
+    e.beta = beta;
+    e.pbuf = nxt;
+    e.merge = 0;
+    e.fresh = 1;
+    write_log(a, P, e, L);
+}
+
+// Chunk end: consumes the last iteration's partials and decides.  In mixed
+// mode, when a replacement is due or convergence is to be confirmed, pass 1
+// of the replacement: the FP64 true residual r = rhs - A (x64 + x32), its
+// FP32 u = M^-1 r, and s~ = M^-1 A p, into the current buffers; partial of the
+// true r.r.
+template <bool MIXED>
+__global__ __launch_bounds__(GF_WG) void k_cg1_replace(PcgArgs a, int L) {
+    __shared__ double s_red[3 * GF_WG / GF_WAVE];
+    const Part P = part_of(a);
+    if (P.s < 0) return;
+    PcgLog e = live_log(a, P.s, L - 1);
+    consume<MIXED>(a, P, e, L, s_red);
+    const bool run = MIXED && (e.upd || (e.done && !e.converged));
+    if (!run) {
+        write_log(a, P, e, L);
+        return;
+    }
+    float* __restrict__ u = a.u32[e.pbuf];
+    float* __restrict__ sv = a.s32[e.pbuf];
+    double rrt = 0.0;
+    FOR_QUADS(P) {
+        const size_t base = (size_t)t * GF_TILE_CELLS;
+        const size_t idx = base + Q.l0;
+        const uint32_t fw = ld_flags(a.flags, idx);
+        const NbIdx ni = nb_index(a.nbr, t, base, Q);
+        double cd[4];
+        float cf[4], bv[4], pq[4];
+        ld4(a.x64 + idx, cd);
+        ld4(a.x32 + idx, cf);
+        ld4(a.b + idx, bv);
+        ld4(a.p32 + idx, pq);
+        Nb4<double> od;
+        Nb4<float> of, op;
+        load_nb4(a.x64, ni, od);
+        load_nb4(a.x32, ni, of);
+        load_nb4(a.p32, ni, op);
+        double c[4];
+        for (int k = 0; k < 4; ++k) {
+            c[k] = cd[k] + (double)cf[k];
+            od.xm[k] += (double)of.xm[k];
+            od.xp[k] += (double)of.xp[k];
+            od.ym[k] += (double)of.ym[k];
+            od.yp[k] += (double)of.yp[k];
+        }
+        od.zm0 += (double)of.zm0;
+        od.zp3 += (double)of.zp3;
+        float ap[4], uo[4], so[4];
+        apply_A4(pq, op, fw, ap);
+        for (int k = 0; k < 4; ++k) {
+            const int d = fl_diag(fw, k), fl = fl_fluid(fw, k);
+            const double ax = (double)d * c[k] - nb_sum4(od, c, k);
+            const double rv = keep_if((double)(a.rhs_sign * bv[k]) - ax, fl);
+            const float dm = (float)(d > 1 ? d : 1);
+            uo[k] = (float)rv / dm;
+            so[k] = ap[k] / dm;
+            rrt += rv * rv;
+        }
+        st4(u + idx, uo);
+        st4(sv + idx, so);
+    }
+    put_partial(a.part, P_RRT, L, P.b, rrt, s_red);
+    e.pending = PEND_REPLACE;
+    write_log(a, P, e, L);
+}
+
+// Pass 2 of a replacement (mixed mode; passes the log through otherwise):
+// w~ = M^-1 A u from the new u; partials of gamma and delta for k_pcg_fin.
+__global__ __launch_bounds__(GF_WG) void k_cg1_replace2(PcgArgs a, int L) {
+    __shared__ double s_red[3 * GF_WG / GF_WAVE];
+    const Part P = part_of(a);
+    if (P.s < 0) return;
+    PcgLog e = live_log(a, P.s, L - 1);
+    if (e.pending != PEND_REPLACE) {
+        write_log(a, P, e, L);
+        return;
+    }
+    const float* __restrict__ u = a.u32[e.pbuf];
+    float* __restrict__ w = a.w32[e.pbuf];
+    double gam = 0.0, del = 0.0;
+    FOR_QUADS(P) {
+        const size_t base = (size_t)t * GF_TILE_CELLS;
+        const size_t idx = base + Q.l0;
+        const uint32_t fw = ld_flags(a.flags, idx);
+        float c[4], y[4], wv[4];
+        ld4(u + idx, c);
+        Nb4<float> o;
+        load_nb4(u, nb_index(a.nbr, t, base, Q), o);
+        apply_A4(c, o, fw, y);
+        for (int k = 0; k < 4; ++k) {
+            const float dm = diagm<float>(fw >> (8 * k));
+            wv[k] = y[k] / dm;
+            gam += (double)dm * (double)c[k] * (double)c[k];
+            del += (double)y[k] * (double)c[k];
+        }
+        st4(w + idx, wv);
+    }
+    put_partial2(a.part, P_RZT, P_BB, L, P.b, gam, del, s_red);
+    write_log(a, P, e, L);
+}
+
+// ---- two-kernel (standard) PCG -----------------------------------------
+//   dirspmv: beta = rz/rz_old, p = z + beta p (ping-pong; the stencil
+//            neighbours' p is recomputed from their z and old p), Ap, p.Ap
+//   update:  alpha = rz/pAp, x += alpha p, r = M z - alpha Ap, z = r/M, r.z, r.r
+// Storage: z in u[0], p ping-pong in {p, s[0]}, Ap in w[0].
+
+template <typename T>
+__device__ __forceinline__ T* std_p(const PcgArgs& a, int buf) {
+    return buf ? Vec<T>::s(a, 0) : Vec<T>::p(a);
+}
+
+// The update kernel's r.z / r.r partials (written by launch L - 1).
+template <bool MIXED>
+__device__ __forceinline__ void consume_std(const PcgArgs& a, const Part& P, PcgLog& e, int L,
+                                            double* s_red) {
+    if (!e.fresh) return;
+    double rz_new = partial_share(a.part, P_RZ, L - 1, P.wb, P.we);
+    double rr = partial_share(a.part, P_RR, L - 1, P.wb, P.we);
+    wg_sum2(rz_new, rr, s_red);
+    e.rz_old = e.rz;
+    e.rz = rz_new;
+    e.rr = rr;
+    e.fresh = 0;
+    e.iters += 1;
+    if (rr < a.tol2 * e.bb) {
+        e.done = 1;
+        if (!MIXED) { e.converged = 1; e.rr_true = rr; }
+    } else if (e.iters >= a.max_iter) {
+        e.done = 1;
+        e.maxed = 1;
+        if (!MIXED) e.rr_true = rr;
+    } else if (MIXED && rr < a.delta2 * e.rr_upd) {
+        e.upd = 1;
+    }
+}
+
+// Setup: x = 0, z = M^-1 (rhs_sign*b), p (buffer 0) = 0; partials r.z and b.b
+// (and a zero delta, so k_pcg_fin reads both forms alike).
+template <typename T, bool MIXED>
+__global__ __launch_bounds__(GF_WG) void k_pcg_setup(PcgArgs a) {
+    __shared__ double s_red[3 * GF_WG / GF_WAVE];
+    const Part P = part_of(a);
+    if (P.s < 0) return;
+    T* z = Vec<T>::u(a, 0);
+    T* p = std_p<T>(a, 0);
     double rz = 0.0, rr = 0.0;
     FOR_QUADS(P) {
         const size_t idx = (size_t)t * GF_TILE_CELLS + Q.l0;
@@ -382,7 +876,7 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_setup(PcgArgs a) {
             rz += (double)rv * (double)zv[e];
             rr += (double)rv * (double)rv;
         }
-        st4(r + idx, zv);
+        st4(z + idx, zv);
         st4(p + idx, zero);
         st4(a.x64 + idx, zd);
         if (MIXED) {
@@ -390,66 +884,27 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_setup(PcgArgs a) {
             st4(a.x32 + idx, zf);
         }
     }
-    put_partial(a.part, P_RZ, P.b, rz, s_red);
-    put_partial(a.part, P_BB, P.b, rr, s_red);
+    put_partial3(a.part, P_RZ, P_PAP, P_BB, 0, P.b, rz, 0.0, rr, s_red);  // read by fin at L = 1
 }
 
-// Finalisation after the setup or a residual replacement: one workgroup per
-// scene (grid S).
-__global__ __launch_bounds__(GF_WG) void k_pcg_fin(PcgArgs a, int L) {
-    __shared__ double s_red[GF_WG / GF_WAVE];
-    const int sc = blockIdx.x;
-    const int32_t* wg_off = a.part_info + 8 * PCG_GMAX;
-    const int wb = wg_off[sc], we = wg_off[sc + 1];
-    PcgLog e = live_log(a, sc, L - 1);
-    if (e.pending == PEND_SETUP) {
-        const double rz = sum_partials(a.part, P_RZ, wb, we, s_red);
-        const double bb = sum_partials(a.part, P_BB, wb, we, s_red);
-        e.rz = rz; e.rz_old = 0.0; e.bb = bb; e.rr = bb; e.rr_upd = bb; e.rr_true = bb;
-        e.iters = 0; e.maxed = 0; e.upd = 0; e.merge = 0; e.nupd = 0; e.fresh = 0; e.pbuf = 0;
-        const int zero = bb < 1e-30;
-        e.converged = zero;
-        e.done = zero || a.max_iter <= 0;
-    } else if (e.pending == PEND_REPLACE) {
-        const double rrt = sum_partials(a.part, P_RRT, wb, we, s_red);
-        const double rz = sum_partials(a.part, P_RZT, wb, we, s_red);
-        e.rr_true = rrt; e.rr = rrt; e.rr_upd = rrt; e.rz = rz;
-        e.upd = 0; e.merge = 1; e.nupd += 1;
-        if (rrt < a.tol2 * e.bb) { e.converged = 1; e.done = 1; }
-        else if (e.iters >= a.max_iter) { e.maxed = 1; e.done = 1; }
-        else e.done = 0;
-    }
-    e.pending = PEND_NONE;
-    if (threadIdx.x == 0) live_log(a, sc, L) = e;
-}
-
-__device__ __forceinline__ bool pass_through(const PcgArgs& a, const Part& P, const PcgLog& e,
-                                             int L) {
-    if (!e.done) return false;
-    write_log(a, P, e, L);
-    return true;
-}
-
-// beta = rz / rz_old; p_new = z + beta p_old (own cells, and recomputed for
-// the stencil neighbours); Ap = A p_new; partial p.Ap.
 template <typename T, bool MIXED>
 __global__ __launch_bounds__(GF_WG) void k_pcg_dirspmv(PcgArgs a, int L) {
-    __shared__ double s_red[GF_WG / GF_WAVE];
+    __shared__ double s_red[3 * GF_WG / GF_WAVE];
     const Part P = part_of(a);
     if (P.s < 0) return;
     PcgLog e = live_log(a, P.s, L - 1);
     if (pass_through(a, P, e, L)) return;
-    consume<MIXED>(a, P, e, s_red);
+    consume_std<MIXED>(a, P, e, L, s_red);
     if (e.done) {
         write_log(a, P, e, L);
         return;
     }
     const double beta = fabs(e.rz_old) < 1e-30 ? 0.0 : e.rz / e.rz_old;
     const T bf = (T)beta;
-    const T* __restrict__ po = vp<T>(a, e.pbuf);
-    T* __restrict__ pn = vp<T>(a, e.pbuf ^ 1);
-    const T* __restrict__ z = vr<T>(a);
-    T* __restrict__ ap = vap<T>(a);
+    const T* __restrict__ po = std_p<T>(a, e.pbuf);
+    T* __restrict__ pn = std_p<T>(a, e.pbuf ^ 1);
+    const T* __restrict__ z = Vec<T>::u(a, 0);
+    T* __restrict__ ap = Vec<T>::w(a, 0);
     double pap = 0.0;
     FOR_QUADS(P) {
         const size_t base = (size_t)t * GF_TILE_CELLS;
@@ -470,36 +925,31 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_dirspmv(PcgArgs a, int L) {
         dir4(nz.yp, np.yp, bf, o.yp);
         o.zm0 = fma(bf, np.zm0, nz.zm0);
         o.zp3 = fma(bf, np.zp3, nz.zp3);
-        for (int k = 0; k < 4; ++k) {
-            const T sm = nb_sum4(o, c, k);
-            y[k] = keep_if((T)fl_diag(fw, k) * c[k] - sm, fl_fluid(fw, k));
-            pap += (double)c[k] * (double)y[k];
-        }
-        st4(pn + idx, c);
-        st4(ap + idx, y);
+        apply_A4(c, o, fw, y);
+        for (int k = 0; k < 4; ++k) pap += (double)c[k] * (double)y[k];
+        st4_nt(pn + idx, c);
+        st4_nt(ap + idx, y);
     }
-    put_partial(a.part, P_PAP, P.b, pap, s_red);
+    put_partial(a.part, P_PAP, L, P.b, pap, s_red);
     e.beta = beta;
     e.pbuf ^= 1;
     write_log(a, P, e, L);
 }
 
-// alpha = rz / pAp; x += alpha p, r = M z - alpha Ap, z = r/M stored;
-// partials r.z, r.r.  Non-fluid entries of p, Ap, z and x are 0 and stay 0.
 template <typename T, bool MIXED>
 __global__ __launch_bounds__(GF_WG) void k_pcg_update(PcgArgs a, int L) {
-    __shared__ double s_red[GF_WG / GF_WAVE];
+    __shared__ double s_red[3 * GF_WG / GF_WAVE];
     const Part P = part_of(a);
     if (P.s < 0) return;
     PcgLog e = live_log(a, P.s, L - 1);
     if (pass_through(a, P, e, L)) return;
-    const double pap_t = sum_partials(a.part, P_PAP, P.wb, P.we, s_red);
+    const double pap_t = wg_sum(partial_share(a.part, P_PAP, L - 1, P.wb, P.we), s_red);
     const double alpha = fabs(pap_t) < 1e-30 ? 0.0 : e.rz / pap_t;
     const T af = (T)alpha;
     const int merge = MIXED ? e.merge : 0;
-    T* __restrict__ zp = vr<T>(a);
-    const T* __restrict__ p = vp<T>(a, e.pbuf);
-    const T* __restrict__ ap = vap<T>(a);
+    T* __restrict__ zp = Vec<T>::u(a, 0);
+    const T* __restrict__ p = std_p<T>(a, e.pbuf);
+    const T* __restrict__ ap = Vec<T>::w(a, 0);
     double rz = 0.0, rr = 0.0;
     FOR_QUADS(P) {
         const size_t idx = (size_t)t * GF_TILE_CELLS + Q.l0;
@@ -521,12 +971,12 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_update(PcgArgs a, int L) {
                 st4(a.x64 + idx, xd);
             }
             for (int k = 0; k < 4; ++k) xo[k] = xo[k] + (float)af * (float)pv[k];
-            st4(a.x32 + idx, xo);
+            st4_nt(a.x32 + idx, xo);
         } else {
             double xd[4];
             ld4(a.x64 + idx, xd);
             for (int k = 0; k < 4; ++k) xd[k] += (double)af * (double)pv[k];
-            st4(a.x64 + idx, xd);
+            st4_nt(a.x64 + idx, xd);
         }
         for (int k = 0; k < 4; ++k) {
             const T dm = diagm<T>(fw >> (8 * k));
@@ -535,10 +985,9 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_update(PcgArgs a, int L) {
             rz += (double)rv * (double)zv[k];
             rr += (double)rv * (double)rv;
         }
-        st4(zp + idx, zv);
+        st4_nt(zp + idx, zv);
     }
-    put_partial(a.part, P_RZ, P.b, rz, s_red);
-    put_partial(a.part, P_RR, P.b, rr, s_red);
+    put_partial2(a.part, P_RZ, P_RR, L, P.b, rz, rr, s_red);
     e.alpha = alpha;
     e.pap = pap_t;
     e.merge = 0;
@@ -546,22 +995,22 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_update(PcgArgs a, int L) {
     write_log(a, P, e, L);
 }
 
-// Chunk end: consumes the last update's partials and decides; in mixed mode,
-// when a replacement is due or convergence is to be confirmed, computes the
-// FP64 true residual r = rhs - A (x64 + x32), stores its FP32 z = r/M, and
-// leaves partials of its r.r and r.z for k_pcg_fin.
+// Chunk end of the two-kernel form: consume and decide; in mixed mode, when
+// due, z = M^-1 (rhs - A (x64 + x32)) from the FP64 true residual, with
+// partials of its r.r and r.z for k_pcg_fin.
 template <bool MIXED>
 __global__ __launch_bounds__(GF_WG) void k_pcg_replace(PcgArgs a, int L) {
-    __shared__ double s_red[GF_WG / GF_WAVE];
+    __shared__ double s_red[3 * GF_WG / GF_WAVE];
     const Part P = part_of(a);
     if (P.s < 0) return;
     PcgLog e = live_log(a, P.s, L - 1);
-    consume<MIXED>(a, P, e, s_red);
+    consume_std<MIXED>(a, P, e, L, s_red);
     const bool run = MIXED && (e.upd || (e.done && !e.converged));
     if (!run) {
         write_log(a, P, e, L);
         return;
     }
+    float* __restrict__ z = a.u32[0];
     double rrt = 0.0, rz = 0.0;
     FOR_QUADS(P) {
         const size_t base = (size_t)t * GF_TILE_CELLS;
@@ -598,10 +1047,9 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_replace(PcgArgs a, int L) {
             rz += (double)rk * (double)zk;
             z32[k] = zk;
         }
-        st4(a.r32 + idx, z32);
+        st4(z + idx, z32);
     }
-    put_partial(a.part, P_RRT, P.b, rrt, s_red);
-    put_partial(a.part, P_RZT, P.b, rz, s_red);
+    put_partial2(a.part, P_RRT, P_RZT, L, P.b, rrt, rz, s_red);
     e.pending = PEND_REPLACE;
     write_log(a, P, e, L);
 }
@@ -738,15 +1186,22 @@ int pcg_prepare(const gf_scene_t* s, int n_active, int precision, double tol, in
     GF_REQUIRE(s->flags && s->nbr && s->b && s->pc && s->partials && s->scalars && s->x64 &&
                    s->pcg_part,
                "%s: buffers not set", who);
+    // FP32 vectors serve both precisions' replacement bookkeeping only in mixed mode
     if (precision == GF_PCG_MIXED)
-        GF_REQUIRE(s->x32 && s->r32 && s->p32 && s->p32b && s->ap32, "%s: FP32 vectors not set", who);
+        GF_REQUIRE(s->x32 && s->r32 && s->p32 && s->p32b && s->ap32 && s->cg32,
+                   "%s: FP32 vectors not set", who);
     else
-        GF_REQUIRE(s->r64 && s->p64 && s->p64b && s->ap64, "%s: FP64 vectors not set", who);
+        GF_REQUIRE(s->r64 && s->p64 && s->p64b && s->ap64 && s->cg64, "%s: FP64 vectors not set", who);
+    const size_t C = (size_t)ntiles * GF_TILE_CELLS;
     a.flags = s->flags; a.nbr = s->nbr; a.b = s->b;
-    a.x32 = s->x32; a.r32 = s->r32; a.ap32 = s->ap32;
-    a.x64 = s->x64; a.r64 = s->r64; a.ap64 = s->ap64;
-    a.p32[0] = s->p32; a.p32[1] = s->p32b;
-    a.p64[0] = s->p64; a.p64[1] = s->p64b;
+    a.u32[0] = s->r32; a.u32[1] = s->cg32;
+    a.w32[0] = s->ap32; a.w32[1] = s->cg32 ? s->cg32 + C : nullptr;
+    a.s32[0] = s->p32b; a.s32[1] = s->cg32 ? s->cg32 + 2 * C : nullptr;
+    a.p32 = s->p32; a.x32 = s->x32;
+    a.u64[0] = s->r64; a.u64[1] = s->cg64;
+    a.w64[0] = s->ap64; a.w64[1] = s->cg64 ? s->cg64 + C : nullptr;
+    a.s64[0] = s->p64b; a.s64[1] = s->cg64 ? s->cg64 + 2 * C : nullptr;
+    a.p64 = s->p64; a.x64 = s->x64;
     a.pc = s->pc; a.part = s->partials; a.log = (PcgLog*)s->scalars;
     a.part_info = s->pcg_part;
     a.scene_slot = packed ? s->scene_slot : nullptr;
@@ -762,6 +1217,11 @@ int pcg_prepare(const gf_scene_t* s, int n_active, int precision, double tol, in
         const int pairs = (n_active + 1) / 2;
         a.G = pairs < gmax ? pairs : gmax;
     }
+    const int vt = gf::tunable(gf::GF_TUNE_PCG_VARIANT);
+    if (vt == PCG_STD || vt == PCG_CG1)
+        a.variant = vt;
+    else
+        a.variant = n_active < PCG_CG1_MAX_TILES ? PCG_CG1 : PCG_STD;
     a.rhs_sign = (float)rhs_sign;
     a.tol2 = tol * tol;
     a.delta2 = 0.01;
@@ -769,44 +1229,71 @@ int pcg_prepare(const gf_scene_t* s, int n_active, int precision, double tol, in
     return 0;
 }
 
-// Launch numbers used by one chunk of `chunk` iterations.
-int pcg_chunk_launches(int chunk, bool mixed) { return 2 * chunk + 1 + (mixed ? 1 : 0); }
+// Kernels per iteration and launch numbers used by one chunk of `chunk` iterations.
+int pcg_kernels_per_iter(const PcgArgs& a) { return a.variant == PCG_CG1 ? 1 : 2; }
+int pcg_chunk_launches(const PcgArgs& a, int chunk, bool mixed) {
+    return a.variant == PCG_CG1 ? chunk + 1 + (mixed ? 2 : 0) : 2 * chunk + 1 + (mixed ? 1 : 0);
+}
 
 void pcg_launch_setup(const PcgArgs& a, bool mixed, hipStream_t st) {
     const dim3 grid(a.G), block(GF_WG);
     hipLaunchKernelGGL(k_pcg_partition, dim3(1), block, 0, st, a);
-    if (mixed)
-        hipLaunchKernelGGL((k_pcg_setup<float, true>), grid, block, 0, st, a);
-    else
-        hipLaunchKernelGGL((k_pcg_setup<double, false>), grid, block, 0, st, a);
+    if (a.variant == PCG_STD) {
+        if (mixed) hipLaunchKernelGGL((k_pcg_setup<float, true>), grid, block, 0, st, a);
+        else hipLaunchKernelGGL((k_pcg_setup<double, false>), grid, block, 0, st, a);
+    } else if (mixed) {
+        hipLaunchKernelGGL((k_cg1_setup<float, true>), grid, block, 0, st, a);
+        hipLaunchKernelGGL(k_cg1_setup2<float>, grid, block, 0, st, a);
+    } else {
+        hipLaunchKernelGGL((k_cg1_setup<double, false>), grid, block, 0, st, a);
+        hipLaunchKernelGGL(k_cg1_setup2<double>, grid, block, 0, st, a);
+    }
     hipLaunchKernelGGL(k_pcg_fin, dim3(a.S), block, 0, st, a, 1);
 }
 
 // One chunk: `chunk` iterations, then the chunk-end decision (and residual
 // replacement when due).  L is the launch number before the chunk; ev, when
-// given, receives 2*chunk+2 events (start, after each kernel, chunk end).
+// given, receives k*chunk+2 events (start, after each kernel, chunk end) with
+// k = pcg_kernels_per_iter.
 int pcg_launch_chunk(const PcgArgs& a, bool mixed, int chunk, int& L, hipStream_t st,
                      hipEvent_t* ev) {
     const dim3 grid(a.G), block(GF_WG);
+    const int kpi = pcg_kernels_per_iter(a);
+    if (ev) (void)hipEventRecord(ev[0], st);
     for (int j = 0; j < chunk; ++j) {
-        if (ev && j == 0) hipEventRecord(ev[0], st);
-        ++L;
-        if (mixed) hipLaunchKernelGGL((k_pcg_dirspmv<float, true>), grid, block, 0, st, a, L);
-        else hipLaunchKernelGGL((k_pcg_dirspmv<double, false>), grid, block, 0, st, a, L);
-        if (ev) hipEventRecord(ev[2 * j + 1], st);
-        ++L;
-        if (mixed) hipLaunchKernelGGL((k_pcg_update<float, true>), grid, block, 0, st, a, L);
-        else hipLaunchKernelGGL((k_pcg_update<double, false>), grid, block, 0, st, a, L);
-        if (ev) hipEventRecord(ev[2 * j + 2], st);
+        if (a.variant == PCG_CG1) {
+            ++L;
+            if (mixed) hipLaunchKernelGGL((k_cg1_iter<float, true>), grid, block, 0, st, a, L);
+            else hipLaunchKernelGGL((k_cg1_iter<double, false>), grid, block, 0, st, a, L);
+            if (ev) (void)hipEventRecord(ev[kpi * j + 1], st);
+        } else {
+            ++L;
+            if (mixed) hipLaunchKernelGGL((k_pcg_dirspmv<float, true>), grid, block, 0, st, a, L);
+            else hipLaunchKernelGGL((k_pcg_dirspmv<double, false>), grid, block, 0, st, a, L);
+            if (ev) (void)hipEventRecord(ev[kpi * j + 1], st);
+            ++L;
+            if (mixed) hipLaunchKernelGGL((k_pcg_update<float, true>), grid, block, 0, st, a, L);
+            else hipLaunchKernelGGL((k_pcg_update<double, false>), grid, block, 0, st, a, L);
+            if (ev) (void)hipEventRecord(ev[kpi * j + 2], st);
+        }
     }
     ++L;
-    if (mixed) hipLaunchKernelGGL(k_pcg_replace<true>, grid, block, 0, st, a, L);
-    else hipLaunchKernelGGL(k_pcg_replace<false>, grid, block, 0, st, a, L);
+    if (a.variant == PCG_CG1) {
+        if (mixed) hipLaunchKernelGGL(k_cg1_replace<true>, grid, block, 0, st, a, L);
+        else hipLaunchKernelGGL(k_cg1_replace<false>, grid, block, 0, st, a, L);
+        if (mixed) {
+            ++L;
+            hipLaunchKernelGGL(k_cg1_replace2, grid, block, 0, st, a, L);
+        }
+    } else {
+        if (mixed) hipLaunchKernelGGL(k_pcg_replace<true>, grid, block, 0, st, a, L);
+        else hipLaunchKernelGGL(k_pcg_replace<false>, grid, block, 0, st, a, L);
+    }
     if (mixed) {
         ++L;
         hipLaunchKernelGGL(k_pcg_fin, dim3(a.S), block, 0, st, a, L);
     }
-    if (ev) hipEventRecord(ev[2 * chunk + 1], st);
+    if (ev) (void)hipEventRecord(ev[kpi * chunk + 1], st);
     return gf::check_launch("pcg iteration");
 }
 
@@ -849,12 +1336,14 @@ extern "C" int gf_scene_pcg_sync(const gf_scene_t* s, int n_active, int precisio
     // kernels of the chunk already in flight pass through.
     // Optional per-kernel timing of the iterations that did work: events
     // around every launch of a chunk, read when the chunk's logs are read.
-    // kernel_ms: [0] dirspmv, [1] update, [2] unused, [3] chunk-end replacement.
+    // kernel_ms: [0] iteration kernel (two-kernel form: direction+SpMV), [1] update
+    // (two-kernel form), [2] unused, [3] chunk-end replacement.
     PcgLog* hlog = gf::pinned_pcg_logs((size_t)6 * S);  // setup + 2 chunk slots, 2S entries each
     if (!hlog) return gf::set_error(GF_ESTATE, "gf_scene_pcg_sync: pinned log: %s", gf_last_error());
     const bool timed = kernel_ms != nullptr;
     const int chunk = check_every;
-    const int nev = timed ? 2 * chunk + 2 : 0;
+    const int kpi = pcg_kernels_per_iter(a);
+    const int nev = timed ? kpi * chunk + 2 : 0;
     hipEvent_t ev[2][2 * 64 + 2];
     hipEvent_t done_ev[2] = {nullptr, nullptr};
     int created = 0;
@@ -938,15 +1427,15 @@ extern "C" int gf_scene_pcg_sync(const gf_scene_t* s, int n_active, int precisio
         if (timed) {
             const int real = hs.iters - iters_before;
             for (int j = 0; j < real && j < chunk; ++j) {
-                float t0 = 0.f, t1 = 0.f;
-                hipEventElapsedTime(&t0, ev[sl][2 * j], ev[sl][2 * j + 1]);
-                hipEventElapsedTime(&t1, ev[sl][2 * j + 1], ev[sl][2 * j + 2]);
-                kernel_ms[0] += t0;
-                kernel_ms[1] += t1;
+                for (int m = 0; m < kpi; ++m) {
+                    float t0 = 0.f;
+                    (void)hipEventElapsedTime(&t0, ev[sl][kpi * j + m], ev[sl][kpi * j + m + 1]);
+                    kernel_ms[m] += t0;
+                }
             }
             if (mixed && hs.nupd > nupd_before) {
                 float t3 = 0.f;
-                hipEventElapsedTime(&t3, ev[sl][2 * chunk], ev[sl][2 * chunk + 1]);
+                (void)hipEventElapsedTime(&t3, ev[sl][kpi * chunk], ev[sl][kpi * chunk + 1]);
                 kernel_ms[3] += t3;
             }
         }
@@ -985,7 +1474,7 @@ extern "C" int gf_scene_pcg_enqueue(const gf_scene_t* s, int n_active, int preci
     hipStream_t st = (hipStream_t)stream;
     const bool mixed = precision == GF_PCG_MIXED;
     if (first_chunk == 0) pcg_launch_setup(a, mixed, st);
-    int L = 1 + first_chunk * pcg_chunk_launches(check_every, mixed);
+    int L = 1 + first_chunk * pcg_chunk_launches(a, check_every, mixed);
     for (int c = 0; c < n_chunks; ++c) {
         rc = pcg_launch_chunk(a, mixed, check_every, L, st, nullptr);
         if (rc) return rc;

@@ -41,11 +41,12 @@ class SceneBuffers:
         self.r32 = torch.zeros(sz.compact, **f32)
         self.p32 = torch.zeros(sz.compact, **f32)
         self.p32b = torch.zeros(sz.compact, **f32)
+        self.cg32 = torch.zeros(3 * sz.compact, **f32)
         self.ap32 = torch.zeros(sz.compact, **f32)
         self.pc = torch.zeros(sz.compact, **f32)
         self.ptmp = torch.zeros(sz.compact, **f32)
         self.x64 = torch.zeros(sz.compact, **f64)
-        self.r64 = self.p64 = self.p64b = self.ap64 = None  # FP64 PCG, allocated on first use
+        self.r64 = self.p64 = self.p64b = self.ap64 = self.cg64 = None  # FP64 PCG, on first use
         self.partials = torch.zeros(sz.partials, **f64)
         self.scalars = torch.zeros(sz.scalars, **f64)
         self.pcg_part = torch.zeros(sz.pcg_part, **i32)
@@ -65,6 +66,7 @@ class SceneBuffers:
             self.r64 = torch.zeros(self.compact, **kw)
             self.p64 = torch.zeros(self.compact, **kw)
             self.p64b = torch.zeros(self.compact, **kw)
+            self.cg64 = torch.zeros(3 * self.compact, **kw)
             self.ap64 = torch.zeros(self.compact, **kw)
 
     def _alloc(self, cap, affine):
@@ -139,6 +141,7 @@ class SceneBuffers:
         s.pc, s.ptmp = P(self.pc), P(self.ptmp)
         s.partials, s.scalars = P(self.partials), P(self.scalars)
         s.p32b, s.p64b = P(self.p32b), P(self.p64b)
+        s.cg32, s.cg64 = P(self.cg32), P(self.cg64)
         s.pcg_part = P(self.pcg_part)
         if self.n_scenes > 1:
             s.n_scenes = self.n_scenes

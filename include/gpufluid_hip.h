@@ -54,8 +54,11 @@ int gf_init(int device, int* cus);
 
 /* Launch-shape tunables, process-wide, read at every call (0 = built-in choice).
  * Keys: "pcg_workgroups" — persistent-grid size of the PCG kernels (default
- * 1024, maximum 2048).  Used by the autotuner (gpufluid_amd/autotune.py).  No
- * reference counterpart (Warp picks its own launch shapes). */
+ * 1024, maximum 2048); "pcg_variant" — 1: two kernels per iteration (standard
+ * PCG), 2: one kernel per iteration (Chronopoulos-Gear form), 0 (default):
+ * the one-kernel form below 4096 active tiles, else the two-kernel form.
+ * Used by the autotuner (gpufluid_amd/autotune.py).  No reference
+ * counterpart (Warp picks its own launch shapes). */
 int gf_tune_set(const char* key, int value);
 int gf_tune_get(const char* key, int* value);
 
@@ -114,7 +117,8 @@ typedef struct gf_scene {
     float* pc;  float* ptmp;     /* pressure result / Jacobi ping-pong */
     double* partials;            /* (gf_scene_sizes.partials) per-workgroup partial sums */
     double* scalars;             /* (gf_scene_sizes.scalars) PCG solver log */
-    float* p32b; double* p64b;   /* second PCG direction buffer (ping-pong with p32/p64) */
+    float* p32b; double* p64b;   /* PCG work vector (compact) */
+    float* cg32; double* cg64;   /* 3 more PCG work vectors, 3*(ntiles*512) entries each */
     /* batching: several independent scenes packed into this grid, each inside
      * its own solid shell at a tile-aligned offset (n_scenes <= 1: one scene,
      * the fields below unused).  Positions are then in grid units (dx = 1). */

@@ -42,10 +42,19 @@ def _alone(n, pos, vel, substeps):
     return p.astype(np.float64), v.astype(np.float64), iters
 
 
-def test_packed_scenes_match_scenes_run_alone():
+@pytest.fixture(params=[0, 1, 2], ids=["auto", "two_kernel", "one_kernel"])
+def pcg_variant(request):
+    from gpufluid_amd import autotune
+    autotune.set_pcg_variant(request.param)
+    yield request.param
+    autotune.set_pcg_variant(0)
+
+
+def test_packed_scenes_match_scenes_run_alone(pcg_variant):
     from gpufluid_amd.batch import BatchedScenes
     scenes = _scenes()
-    print(f"[batch] {len(scenes)} scenes, edges {[s[0] for s in scenes]}", flush=True)
+    print(f"[batch] {len(scenes)} scenes, edges {[s[0] for s in scenes]}, PCG variant {pcg_variant}",
+          flush=True)
     b = BatchedScenes(scenes, dt=DT, device="cuda:0")
     its = []
     for _ in range(3):
@@ -65,7 +74,17 @@ def test_packed_scenes_match_scenes_run_alone():
             assert abs(int(its[k][i]) - ia[k]) <= 8
 
 
-def test_graph_replay_matches_direct_launches():
+@pytest.mark.parametrize("variant", [1, 2], ids=["two_kernel", "one_kernel"])
+def test_graph_replay_matches_direct_launches(variant):
+    from gpufluid_amd import autotune
+    autotune.set_pcg_variant(variant)
+    try:
+        _graph_vs_direct()
+    finally:
+        autotune.set_pcg_variant(0)
+
+
+def _graph_vs_direct():
     from gpufluid_amd.batch import BatchedScenes
     scenes = _scenes(seed=22)
     a = BatchedScenes(scenes, dt=DT, device="cuda:0")

@@ -117,9 +117,20 @@ def test_jacobi_bit_exact_against_fp32_oracle():
     np.testing.assert_array_equal(got, want)
 
 
+@pytest.fixture(params=[1, 2], ids=["two_kernel", "one_kernel"])
+def pcg_variant(request):
+    """Both PCG iteration forms (1: direction+SpMV and update kernels,
+    2: one Chronopoulos-Gear kernel), forced through the tunable."""
+    from gpufluid_amd import autotune
+    autotune.set_pcg_variant(request.param)
+    yield request.param
+    autotune.set_pcg_variant(0)
+
+
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
-def test_pcg_reaches_1e8_and_matches_direct_solve(precision):
+def test_pcg_reaches_1e8_and_matches_direct_solve(precision, pcg_variant):
     from gpufluid_amd import _lib
+    print(f"[pcg] precision {precision} variant {pcg_variant}", flush=True)
     s = _prepared(shape=(32, 32, 32), pcg_precision=precision)
     if precision == "fp64":
         s._sc.ensure_fp64()
@@ -140,7 +151,7 @@ def test_pcg_reaches_1e8_and_matches_direct_solve(precision):
     assert fo.residual_norm(got, div, m) < 1e-5
 
 
-def test_pcg_reference_sign_is_negated():
+def test_pcg_reference_sign_is_negated(pcg_variant):
     from gpufluid_amd import _lib
     s = _prepared(shape=(16, 16, 16))
     d = s._descriptor()
