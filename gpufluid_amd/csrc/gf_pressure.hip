@@ -64,9 +64,10 @@ constexpr int PCG_GMAX = 2048;
 enum { PCG_STD = 1, PCG_CG1 = 2 };
 // Below this many active tiles the one-kernel iteration is faster (its fixed
 // per-launch cost is paid once per iteration); above it the two-kernel form
-// streams more efficiently (measured on MI355X: 1.66 K tiles 34 vs 47 us per
-// iteration, 5.8 K tiles about equal, 13.3 K tiles 96 vs 85 us).
-constexpr int PCG_CG1_MAX_TILES = 4096;
+// streams more efficiently (measured on MI355X, us per iteration, one vs two
+// kernels: 224 tiles 9.5 vs 11.6, 1.7 K tiles 16.8 vs 17.0, 3.2 K tiles 27.5
+// vs 23.3, 13.3 K tiles 81 vs 61).
+constexpr int PCG_CG1_MAX_TILES = 1024;
 constexpr int PCG_DEFAULT_G = 1024;
 
 struct PcgArgs {
@@ -329,17 +330,21 @@ __device__ __forceinline__ void put_partial(double* part, int kind, int L, int b
     if (threadIdx.x == 0) pslot(part, kind, L)[b] = t;
 }
 
+// Vector accessors.  The buffer number is chosen with a select: indexing the
+// pointer arrays of the by-value kernel argument with a run-time value makes
+// the compiler copy the argument into per-thread LDS (measured: +13 us per
+// launch at 13 K tiles).
 template <typename T> struct Vec;
 template <> struct Vec<float> {
-    static __device__ __forceinline__ float* u(const PcgArgs& a, int b) { return a.u32[b]; }
-    static __device__ __forceinline__ float* w(const PcgArgs& a, int b) { return a.w32[b]; }
-    static __device__ __forceinline__ float* s(const PcgArgs& a, int b) { return a.s32[b]; }
+    static __device__ __forceinline__ float* u(const PcgArgs& a, int b) { return b ? a.u32[1] : a.u32[0]; }
+    static __device__ __forceinline__ float* w(const PcgArgs& a, int b) { return b ? a.w32[1] : a.w32[0]; }
+    static __device__ __forceinline__ float* s(const PcgArgs& a, int b) { return b ? a.s32[1] : a.s32[0]; }
     static __device__ __forceinline__ float* p(const PcgArgs& a) { return a.p32; }
 };
 template <> struct Vec<double> {
-    static __device__ __forceinline__ double* u(const PcgArgs& a, int b) { return a.u64[b]; }
-    static __device__ __forceinline__ double* w(const PcgArgs& a, int b) { return a.w64[b]; }
-    static __device__ __forceinline__ double* s(const PcgArgs& a, int b) { return a.s64[b]; }
+    static __device__ __forceinline__ double* u(const PcgArgs& a, int b) { return b ? a.u64[1] : a.u64[0]; }
+    static __device__ __forceinline__ double* w(const PcgArgs& a, int b) { return b ? a.w64[1] : a.w64[0]; }
+    static __device__ __forceinline__ double* s(const PcgArgs& a, int b) { return b ? a.s64[1] : a.s64[0]; }
     static __device__ __forceinline__ double* p(const PcgArgs& a) { return a.p64; }
 };
 
@@ -406,13 +411,36 @@ __device__ __forceinline__ void put_partial3(double* part, int ka, int kb, int k
     }
 }
 
-__device__ __forceinline__ PcgLog& live_log(const PcgArgs& a, int s, int L) {
-    return a.log[2 * s + (L & 1)];
+// Logs are copied field by field: a whole-struct copy between global memory
+// and a local leaves part of the local in per-thread LDS.
+#define GF_LOG_FIELDS(X)                                                              \
+    X(rz) X(rz_old) X(rr) X(bb) X(alpha) X(beta) X(rr_upd) X(rr_true) X(pap) X(iters) \
+    X(done) X(converged) X(maxed) X(upd) X(merge) X(nupd) X(pending) X(fresh) X(pbuf)
+
+__device__ __forceinline__ PcgLog copy_log(const PcgLog* src) {
+    PcgLog e;
+#define GF_LOG_LD(f) e.f = src->f;
+    GF_LOG_FIELDS(GF_LOG_LD)
+#undef GF_LOG_LD
+    e.pad[0] = e.pad[1] = 0.0;
+    return e;
+}
+__device__ __forceinline__ void put_log(PcgLog* dst, const PcgLog& e) {
+#define GF_LOG_ST(f) dst->f = e.f;
+    GF_LOG_FIELDS(GF_LOG_ST)
+#undef GF_LOG_ST
+}
+
+__device__ __forceinline__ PcgLog live_log(const PcgArgs& a, int s, int L) {
+    return copy_log(a.log + 2 * s + (L & 1));
+}
+__device__ __forceinline__ void set_live_log(const PcgArgs& a, int s, int L, const PcgLog& e) {
+    put_log(a.log + 2 * s + (L & 1), e);
 }
 
 __device__ __forceinline__ void write_log(const PcgArgs& a, const Part& P, const PcgLog& e,
                                           int L) {
-    if (P.b == P.wb && threadIdx.x == 0) live_log(a, P.s, L) = e;
+    if (P.b == P.wb && threadIdx.x == 0) set_live_log(a, P.s, L, e);
 }
 
 // Consumes the last iteration's partials (when fresh): gamma = (r,u),
@@ -603,7 +631,7 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_fin(PcgArgs a, int L) {
         else e.done = 0;
     }
     e.pending = PEND_NONE;
-    if (threadIdx.x == 0) live_log(a, sc, L) = e;
+    if (threadIdx.x == 0) set_live_log(a, sc, L, e);
 }
 
 // u_new = u - alpha (w~ + beta s~) for a quad's neighbour values (the same
@@ -1078,7 +1106,7 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_final(PcgArgs a) {
 // Each scene's final log into the result entries (one thread per scene).
 __global__ void k_pcg_result(PcgArgs a, int L) {
     const int sc = blockIdx.x * blockDim.x + threadIdx.x;
-    if (sc < a.S) a.log[2 * a.S + sc] = live_log(a, sc, L);
+    if (sc < a.S) put_log(a.log + 2 * a.S + sc, live_log(a, sc, L));
 }
 
 // Jacobi sweep (solver3d.py:878-938) on compact tiles: two tiles per
