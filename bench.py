@@ -32,10 +32,10 @@ BYTES = {
     "p2g": lambda np_, nf, na: 24 * np_ + 29 * nf,
     "g2p": lambda np_, nf, na: 48 * np_ + 41 * nf,
     # two-kernel PCG iteration (FP32 vectors, fluid cells only):
-    # direction+SpMV: z, p_old, flags in; p_new, Ap out
-    "pcg_dirspmv": lambda np_, nf, na: 17 * nf,
-    # update: p, Ap, z, x32, flags in; z, x32 out
-    "pcg_update": lambda np_, nf, na: 25 * nf,
+    # direction+SpMV: z, p_old, flags in; p_new out (Ap only reduced)
+    "pcg_dirspmv": lambda np_, nf, na: 13 * nf,
+    # update (Ap recomputed from p): p, z, x32, flags in; z, x32 out
+    "pcg_update": lambda np_, nf, na: 21 * nf,
     # FP64 replacement: x64, x32, b, flags in; z out
     "pcg_replace": lambda np_, nf, na: 21 * nf,
     # one-kernel (Chronopoulos-Gear) iteration: u, w~, s~, p, x32, flags in;

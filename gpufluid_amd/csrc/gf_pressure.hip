@@ -138,13 +138,13 @@ __device__ __forceinline__ void st4_nt(T* p, const T (&o)[4]);
 template <>
 __device__ __forceinline__ void st4_nt<float>(float* p, const float (&o)[4]) {
     gf_f4 v = {o[0], o[1], o[2], o[3]};
-    __builtin_nontemporal_store(v, reinterpret_cast<gf_f4*>(p));
+    *reinterpret_cast<gf_f4*>(p) = v;
 }
 template <>
 __device__ __forceinline__ void st4_nt<double>(double* p, const double (&o)[4]) {
     gf_d2 v0 = {o[0], o[1]}, v1 = {o[2], o[3]};
-    __builtin_nontemporal_store(v0, reinterpret_cast<gf_d2*>(p));
-    __builtin_nontemporal_store(v1, reinterpret_cast<gf_d2*>(p + 2));
+    *reinterpret_cast<gf_d2*>(p) = v0;
+    *reinterpret_cast<gf_d2*>(p + 2) = v1;
 }
 
 __device__ __forceinline__ uint32_t ld_flags(const uint8_t* f, size_t idx) {
@@ -173,10 +173,17 @@ struct NbIdx {
     uint32_t ok;  // bit d set: neighbour d (order -x,+x,-y,+y,-z,+z) exists
 };
 
+// The table row is read through the constant address space: t is
+// wave-uniform and the table is never written by these kernels, so the row
+// comes in with scalar loads (measured 1-4 us less per launch at 13 K tiles
+// than per-lane loads).
+typedef const __attribute__((address_space(4))) int32_t gf_const_i32;
+
 __device__ __forceinline__ NbIdx nb_index(const int32_t* __restrict__ nbr, int t, size_t base,
                                           const Quad& Q) {
-    const int2* row = reinterpret_cast<const int2*>(nbr + (size_t)t * 6);
-    const int2 n01 = row[0], n23 = row[1], n45 = row[2];
+    gf_const_i32* row = (gf_const_i32*)(nbr + (size_t)t * 6);
+    const int2 n01 = make_int2(row[0], row[1]), n23 = make_int2(row[2], row[3]),
+               n45 = make_int2(row[4], row[5]);
     const size_t own = base + Q.l0;
     NbIdx o;
     const bool bxm = Q.li > 0, bxp = Q.li < 7, bym = Q.lj > 0, byp = Q.lj < 7;
@@ -218,6 +225,102 @@ __device__ __forceinline__ void load_nb4(const T* __restrict__ v, const NbIdx& n
     }
     o.zm0 = (n.ok & 16) ? o.zm0 : (T)0;
     o.zp3 = (n.ok & 32) ? o.zp3 : (T)0;
+}
+
+// ---- neighbours through lane exchange -----------------------------------
+// A wave holds four li-planes of one tile (lane = 16*(li&3) + 2*lj + kh), so
+// most stencil neighbours of a quad sit in another lane of the same wave:
+// -y/+y two lanes away in the same 16-lane row (DPP row shifts), -z/+z in the
+// partner lane (DPP quad permute), -x/+x sixteen lanes away (ds_bpermute).
+// Only the values across a face of the wave's block are loaded: the -x plane
+// below the wave's first plane (previous tile or the other wave's half), the
+// +x plane above its last, the -y row / +y row of the neighbouring tile, and
+// the -z / +z cell of the neighbouring tile.  Every lane issues each face
+// load, at an address that depends on (lj, kh) only for x and on (li, kh)
+// only for y, so a wave-wide load touches a few cache lines instead of the
+// full 1 KB of a quad load, and no load sits behind a branch.
+struct FaceIdx {
+    size_t xm, xp, ym, yp, zm, zp;  // element offsets of the face values
+    uint32_t ok;                    // bit d: face neighbour d exists (-x,+x,-y,+y,-z,+z)
+};
+
+__device__ __forceinline__ FaceIdx face_index(const int32_t* __restrict__ nbr, int t, size_t base,
+                                              const Quad& Q) {
+    gf_const_i32* row = (gf_const_i32*)(nbr + (size_t)t * 6);
+    const int nxm = row[0], nxp = row[1], nym = row[2], nyp = row[3], nzm = row[4], nzp = row[5];
+    const int half = Q.li >> 2;                 // wave holds planes 4*half .. 4*half+3
+    const int pos = Q.lj * 8 + Q.kh * 4;        // offset within a plane
+    const size_t own = base + Q.l0;
+    FaceIdx f;
+    // -x of plane 4*half: plane 3 of this tile (half 1) or plane 7 of the -x tile
+    f.xm = half ? base + 3 * 64 + pos : (nxm >= 0 ? (size_t)nxm * 512 + 7 * 64 + pos : own);
+    // +x of plane 4*half+3: plane 4 of this tile (half 0) or plane 0 of the +x tile
+    f.xp = half ? (nxp >= 0 ? (size_t)nxp * 512 + pos : own) : base + 4 * 64 + pos;
+    f.ym = nym >= 0 ? (size_t)nym * 512 + (Q.li * 8 + 7) * 8 + Q.kh * 4 : own;
+    f.yp = nyp >= 0 ? (size_t)nyp * 512 + (Q.li * 8) * 8 + Q.kh * 4 : own;
+    f.zm = nzm >= 0 ? (size_t)nzm * 512 + (Q.li * 8 + Q.lj) * 8 + 7 : own;
+    f.zp = nzp >= 0 ? (size_t)nzp * 512 + (Q.li * 8 + Q.lj) * 8 : own;
+    f.ok = (uint32_t)(half || nxm >= 0) | (uint32_t)(!half || nxp >= 0) << 1 |
+           (uint32_t)(nym >= 0) << 2 | (uint32_t)(nyp >= 0) << 3 | (uint32_t)(nzm >= 0) << 4 |
+           (uint32_t)(nzp >= 0) << 5;
+    return f;
+}
+
+// Face values of one vector (see FaceIdx).
+template <typename T>
+__device__ __forceinline__ void load_face(const T* __restrict__ v, const FaceIdx& f, Nb4<T>& o) {
+    ld4(v + f.xm, o.xm);
+    ld4(v + f.xp, o.xp);
+    ld4(v + f.ym, o.ym);
+    ld4(v + f.yp, o.yp);
+    o.zm0 = v[f.zm];
+    o.zp3 = v[f.zp];
+}
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (uint32_t)lo);
+}
+__device__ __forceinline__ float lane_read(float v, int byte_addr) {
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(byte_addr, __float_as_int(v)));
+}
+__device__ __forceinline__ double lane_read(double v, int byte_addr) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_ds_bpermute(byte_addr, (int)b);
+    const int hi = __builtin_amdgcn_ds_bpermute(byte_addr, (int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (uint32_t)lo);
+}
+
+enum { DPP_ROW_SHL2 = 0x102, DPP_ROW_SHR2 = 0x112, DPP_SWAP1 = 0xB1 };
+
+// Stencil neighbours of the quad c (this lane's values of the vector) from
+// the other lanes, with the face values fv (zero where ok says the
+// neighbouring tile is absent).  Identical values to load_nb4 on the same
+// vector.
+template <typename T>
+__device__ __forceinline__ void nb_from_lanes(const T (&c)[4], const Nb4<T>& fv,
+                                              const FaceIdx& f, const Quad& Q, Nb4<T>& o) {
+    const int lane = threadIdx.x & 63;
+    const int pl = Q.li & 3;
+    const int up = 4 * (lane >= 16 ? lane - 16 : lane), dn = 4 * (lane < 48 ? lane + 16 : lane);
+    for (int e = 0; e < 4; ++e) {
+        const T xm = lane_read(c[e], up), xp = lane_read(c[e], dn);
+        const T ym = dpp_mov<DPP_ROW_SHR2>(c[e]), yp = dpp_mov<DPP_ROW_SHL2>(c[e]);
+        o.xm[e] = pl == 0 ? ((f.ok & 1) ? fv.xm[e] : (T)0) : xm;
+        o.xp[e] = pl == 3 ? ((f.ok & 2) ? fv.xp[e] : (T)0) : xp;
+        o.ym[e] = Q.lj == 0 ? ((f.ok & 4) ? fv.ym[e] : (T)0) : ym;
+        o.yp[e] = Q.lj == 7 ? ((f.ok & 8) ? fv.yp[e] : (T)0) : yp;
+    }
+    const T zm = dpp_mov<DPP_SWAP1>(c[3]), zp = dpp_mov<DPP_SWAP1>(c[0]);
+    o.zm0 = Q.kh == 0 ? ((f.ok & 16) ? fv.zm0 : (T)0) : zm;
+    o.zp3 = Q.kh == 1 ? ((f.ok & 32) ? fv.zp3 : (T)0) : zp;
 }
 
 // Neighbour sum of quad element e in the reference order -x,+x,-y,+y,-z,+z.
@@ -847,9 +950,13 @@ __global__ __launch_bounds__(GF_WG) void k_cg1_replace2(PcgArgs a, int L) {
 
 // ---- two-kernel (standard) PCG -----------------------------------------
 //   dirspmv: beta = rz/rz_old, p = z + beta p (ping-pong; the stencil
-//            neighbours' p is recomputed from their z and old p), Ap, p.Ap
-//   update:  alpha = rz/pAp, x += alpha p, r = M z - alpha Ap, z = r/M, r.z, r.r
-// Storage: z in u[0], p ping-pong in {p, s[0]}, Ap in w[0].
+//            neighbours' p is recomputed from their z and old p), p.Ap
+//   update:  alpha = rz/pAp, Ap again from the stored p (bit-identical to the
+//            direction kernel's), x += alpha p, r = M z - alpha Ap, z = r/M,
+//            r.z, r.r
+// Ap is recomputed rather than stored: the stencil re-reads p through the
+// caches, while storing and reloading Ap would cost 8 bytes per cell of HBM
+// traffic per iteration.  Storage: z in u[0], p ping-pong in {p, s[0]}.
 
 template <typename T>
 __device__ __forceinline__ T* std_p(const PcgArgs& a, int buf) {
@@ -932,31 +1039,30 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_dirspmv(PcgArgs a, int L) {
     const T* __restrict__ po = std_p<T>(a, e.pbuf);
     T* __restrict__ pn = std_p<T>(a, e.pbuf ^ 1);
     const T* __restrict__ z = Vec<T>::u(a, 0);
-    T* __restrict__ ap = Vec<T>::w(a, 0);
     double pap = 0.0;
     FOR_QUADS(P) {
         const size_t base = (size_t)t * GF_TILE_CELLS;
         const size_t idx = base + Q.l0;
-        const NbIdx ni = nb_index(a.nbr, t, base, Q);
+        const FaceIdx fi = face_index(a.nbr, t, base, Q);
         const uint32_t fw = ld_flags(a.flags, idx);
         T zq[4], pq[4], c[4], y[4];
         ld4(z + idx, zq);
         ld4(po + idx, pq);
-        Nb4<T> nz, np;
-        load_nb4(z, ni, nz);
-        load_nb4(po, ni, np);
+        Nb4<T> fz, fp, fc, o;
+        load_face(z, fi, fz);
+        load_face(po, fi, fp);
         dir4(zq, pq, bf, c);
-        Nb4<T> o;
-        dir4(nz.xm, np.xm, bf, o.xm);
-        dir4(nz.xp, np.xp, bf, o.xp);
-        dir4(nz.ym, np.ym, bf, o.ym);
-        dir4(nz.yp, np.yp, bf, o.yp);
-        o.zm0 = fma(bf, np.zm0, nz.zm0);
-        o.zp3 = fma(bf, np.zp3, nz.zp3);
+        // face cells' new p from their z and old p (the owner's fma, bit for bit)
+        dir4(fz.xm, fp.xm, bf, fc.xm);
+        dir4(fz.xp, fp.xp, bf, fc.xp);
+        dir4(fz.ym, fp.ym, bf, fc.ym);
+        dir4(fz.yp, fp.yp, bf, fc.yp);
+        fc.zm0 = fma(bf, fp.zm0, fz.zm0);
+        fc.zp3 = fma(bf, fp.zp3, fz.zp3);
+        nb_from_lanes(c, fc, fi, Q, o);
         apply_A4(c, o, fw, y);
         for (int k = 0; k < 4; ++k) pap += (double)c[k] * (double)y[k];
         st4_nt(pn + idx, c);
-        st4_nt(ap + idx, y);
     }
     put_partial(a.part, P_PAP, L, P.b, pap, s_red);
     e.beta = beta;
@@ -977,15 +1083,19 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_update(PcgArgs a, int L) {
     const int merge = MIXED ? e.merge : 0;
     T* __restrict__ zp = Vec<T>::u(a, 0);
     const T* __restrict__ p = std_p<T>(a, e.pbuf);
-    const T* __restrict__ ap = Vec<T>::w(a, 0);
     double rz = 0.0, rr = 0.0;
     FOR_QUADS(P) {
-        const size_t idx = (size_t)t * GF_TILE_CELLS + Q.l0;
+        const size_t base = (size_t)t * GF_TILE_CELLS;
+        const size_t idx = base + Q.l0;
         const uint32_t fw = ld_flags(a.flags, idx);
         T pv[4], apv[4], zv[4];
         ld4(p + idx, pv);
-        ld4(ap + idx, apv);
         ld4(zp + idx, zv);
+        const FaceIdx fi = face_index(a.nbr, t, base, Q);
+        Nb4<T> fp, np;
+        load_face(p, fi, fp);
+        nb_from_lanes(pv, fp, fi, Q, np);
+        apply_A4(pv, np, fw, apv);  // the same operands and order as the direction kernel's
         if (MIXED) {
             float xo[4];
             ld4(a.x32 + idx, xo);
