@@ -65,9 +65,9 @@ enum { PCG_STD = 1, PCG_CG1 = 2 };
 // Below this many active tiles the one-kernel iteration is faster (its fixed
 // per-launch cost is paid once per iteration); above it the two-kernel form
 // streams more efficiently (measured on MI355X, us per iteration, one vs two
-// kernels: 224 tiles 9.5 vs 11.6, 1.7 K tiles 16.8 vs 17.0, 3.2 K tiles 27.5
-// vs 23.3, 13.3 K tiles 81 vs 61).
-constexpr int PCG_CG1_MAX_TILES = 1024;
+// kernels: 224 tiles 10.6 vs 15.8, 1.7 K tiles 14.5 vs 17.0, 3.2 K tiles 20.8
+// vs 21.2, 5.8 K tiles 29.9 vs 28.1, 13.3 K tiles 65 vs 55).
+constexpr int PCG_CG1_MAX_TILES = 3072;
 constexpr int PCG_DEFAULT_G = 1024;
 
 struct PcgArgs {
@@ -81,6 +81,7 @@ struct PcgArgs {
     float* pc;
     double* part;        // [P_KINDS][PCG_GMAX]
     PcgLog* log;         // [2*S] live pairs, then [S] results
+    PcgLog* hout;        // when set: the log pairs are also written here (pinned host memory)
     int32_t* part_info;  // [8*PCG_GMAX] per workgroup: scene, wb, we, t0, t1; then [S+1] wg offsets
     const int32_t* scene_slot;  // [S+1] compact tile offsets per scene (NULL: one scene, nact tiles)
     int nact;            // active tiles (one scene)
@@ -543,7 +544,10 @@ __device__ __forceinline__ void set_live_log(const PcgArgs& a, int s, int L, con
 
 __device__ __forceinline__ void write_log(const PcgArgs& a, const Part& P, const PcgLog& e,
                                           int L) {
-    if (P.b == P.wb && threadIdx.x == 0) set_live_log(a, P.s, L, e);
+    if (P.b == P.wb && threadIdx.x == 0) {
+        set_live_log(a, P.s, L, e);
+        if (a.hout) put_log(a.hout + 2 * P.s + (L & 1), e);
+    }
 }
 
 // Consumes the last iteration's partials (when fresh): gamma = (r,u),
@@ -612,22 +616,34 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_partition(PcgArgs a) {
         const int off = carry + gf::wg_exclusive_scan(g, s_wave, &total);
         if (sc < a.S) {
             wg_off[sc] = off;
-            for (int q = 0; q < g; ++q) {
-                int32_t* w = a.part_info + 8 * (off + q);
-                w[0] = sc; w[1] = off; w[2] = off + g;
-                w[3] = slot0 + (int)(((long long)q * n) / g);
-                w[4] = slot0 + (int)(((long long)(q + 1) * n) / g);
-            }
             PcgLog e = {};
             e.pending = PEND_SETUP;
-            a.log[2 * sc] = e;
+            put_log(a.log + 2 * sc, e);
         }
         carry += total;
     }
     if (threadIdx.x == 0) wg_off[a.S] = carry;
-    for (int b = carry + threadIdx.x; b < a.G; b += GF_WG) {
+    __syncthreads();
+    // one entry per workgroup, filled in parallel: its scene is the last one
+    // whose first workgroup is at or before it
+    for (int b = threadIdx.x; b < a.G; b += GF_WG) {
         int32_t* w = a.part_info + 8 * b;
-        w[0] = -1; w[1] = 0; w[2] = 0; w[3] = 0; w[4] = 0;
+        if (b >= carry) {
+            w[0] = -1; w[1] = 0; w[2] = 0; w[3] = 0; w[4] = 0;
+            continue;
+        }
+        int lo = 0, hi = a.S - 1;  // wg_off[lo] <= b < wg_off[hi + 1]
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (wg_off[mid] <= b) lo = mid;
+            else hi = mid - 1;
+        }
+        const int sc = lo, off = wg_off[sc], g = wg_off[sc + 1] - off, q = b - off;
+        const int slot0 = a.scene_slot ? a.scene_slot[sc] : 0;
+        const int n = a.scene_slot ? a.scene_slot[sc + 1] - slot0 : a.nact;
+        w[0] = sc; w[1] = off; w[2] = off + g;
+        w[3] = slot0 + (int)(((long long)q * n) / g);
+        w[4] = slot0 + (int)(((long long)(q + 1) * n) / g);
     }
 }
 
@@ -734,7 +750,10 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_fin(PcgArgs a, int L) {
         else e.done = 0;
     }
     e.pending = PEND_NONE;
-    if (threadIdx.x == 0) set_live_log(a, sc, L, e);
+    if (threadIdx.x == 0) {
+        set_live_log(a, sc, L, e);
+        if (a.hout) put_log(a.hout + 2 * sc + (L & 1), e);
+    }
 }
 
 // u_new = u - alpha (w~ + beta s~) for a quad's neighbour values (the same
@@ -1261,12 +1280,15 @@ int scene_check(const gf_scene_t* s, const char* who);
 // Solver-log readback slots in pinned host memory per host thread, grown on
 // demand and kept for the life of the thread (the PCG's only host
 // allocation; it never touches device memory).
-static PcgLog* pinned_pcg_logs(size_t entries) {
+// *dev (optional) receives the device-side address of the same memory, which
+// kernels write directly.
+static PcgLog* pinned_pcg_logs(size_t entries, PcgLog** dev = nullptr) {
     static thread_local PcgLog* logs = nullptr;
+    static thread_local PcgLog* dlogs = nullptr;
     static thread_local size_t cap = 0;
     if (entries > cap) {
         if (logs) hipHostFree(logs);
-        logs = nullptr;
+        logs = dlogs = nullptr;
         cap = 0;
         void* p = nullptr;
         hipError_t e = hipHostMalloc(&p, entries * sizeof(PcgLog), hipHostMallocDefault);
@@ -1274,9 +1296,18 @@ static PcgLog* pinned_pcg_logs(size_t entries) {
             set_error((int)e, "hipHostMalloc: %s", hipGetErrorString(e));
             return nullptr;
         }
+        void* d = nullptr;
+        e = hipHostGetDevicePointer(&d, p, 0);
+        if (e != hipSuccess) {
+            hipHostFree(p);
+            set_error((int)e, "hipHostGetDevicePointer: %s", hipGetErrorString(e));
+            return nullptr;
+        }
         logs = static_cast<PcgLog*>(p);
+        dlogs = static_cast<PcgLog*>(d);
         cap = entries;
     }
+    if (dev) *dev = dlogs;
     return logs;
 }
 }
@@ -1340,7 +1371,7 @@ int pcg_prepare(const gf_scene_t* s, int n_active, int precision, double tol, in
     a.w64[0] = s->ap64; a.w64[1] = s->cg64 ? s->cg64 + C : nullptr;
     a.s64[0] = s->p64b; a.s64[1] = s->cg64 ? s->cg64 + 2 * C : nullptr;
     a.p64 = s->p64; a.x64 = s->x64;
-    a.pc = s->pc; a.part = s->partials; a.log = (PcgLog*)s->scalars;
+    a.pc = s->pc; a.part = s->partials; a.log = (PcgLog*)s->scalars; a.hout = nullptr;
     a.part_info = s->pcg_part;
     a.scene_slot = packed ? s->scene_slot : nullptr;
     a.nact = n_active;
@@ -1394,8 +1425,12 @@ void pcg_launch_setup(const PcgArgs& a, bool mixed, hipStream_t st) {
 // given, receives k*chunk+2 events (start, after each kernel, chunk end) with
 // k = pcg_kernels_per_iter.
 int pcg_launch_chunk(const PcgArgs& a, bool mixed, int chunk, int& L, hipStream_t st,
-                     hipEvent_t* ev) {
+                     hipEvent_t* ev, PcgLog* hout = nullptr) {
     const dim3 grid(a.G), block(GF_WG);
+    // the chunk's last kernel also writes the logs to hout (host readback
+    // without a copy on the stream)
+    PcgArgs al = a;
+    al.hout = hout;
     const int kpi = pcg_kernels_per_iter(a);
     if (ev) (void)hipEventRecord(ev[0], st);
     for (int j = 0; j < chunk; ++j) {
@@ -1418,18 +1453,18 @@ int pcg_launch_chunk(const PcgArgs& a, bool mixed, int chunk, int& L, hipStream_
     ++L;
     if (a.variant == PCG_CG1) {
         if (mixed) hipLaunchKernelGGL(k_cg1_replace<true>, grid, block, 0, st, a, L);
-        else hipLaunchKernelGGL(k_cg1_replace<false>, grid, block, 0, st, a, L);
+        else hipLaunchKernelGGL(k_cg1_replace<false>, grid, block, 0, st, al, L);
         if (mixed) {
             ++L;
             hipLaunchKernelGGL(k_cg1_replace2, grid, block, 0, st, a, L);
         }
     } else {
         if (mixed) hipLaunchKernelGGL(k_pcg_replace<true>, grid, block, 0, st, a, L);
-        else hipLaunchKernelGGL(k_pcg_replace<false>, grid, block, 0, st, a, L);
+        else hipLaunchKernelGGL(k_pcg_replace<false>, grid, block, 0, st, al, L);
     }
     if (mixed) {
         ++L;
-        hipLaunchKernelGGL(k_pcg_fin, dim3(a.S), block, 0, st, a, L);
+        hipLaunchKernelGGL(k_pcg_fin, dim3(a.S), block, 0, st, al, L);
     }
     if (ev) (void)hipEventRecord(ev[kpi * chunk + 1], st);
     return gf::check_launch("pcg iteration");
@@ -1454,7 +1489,7 @@ extern "C" int gf_scene_pcg_sync(const gf_scene_t* s, int n_active, int precisio
     if (rel_residual) *rel_residual = 0.0;
     if (replacements) *replacements = 0;
     if (kernel_ms) for (int i = 0; i < 4; ++i) kernel_ms[i] = 0.0;
-    PcgArgs a;
+    PcgArgs a = {};
     int rc = pcg_prepare(s, n_active, precision, tol, max_iter, rhs_sign, "gf_scene_pcg_sync", a);
     if (rc) return rc;
     if (check_every < 1) check_every = 1;
@@ -1476,7 +1511,8 @@ extern "C" int gf_scene_pcg_sync(const gf_scene_t* s, int n_active, int precisio
     // around every launch of a chunk, read when the chunk's logs are read.
     // kernel_ms: [0] iteration kernel (two-kernel form: direction+SpMV), [1] update
     // (two-kernel form), [2] unused, [3] chunk-end replacement.
-    PcgLog* hlog = gf::pinned_pcg_logs((size_t)6 * S);  // setup + 2 chunk slots, 2S entries each
+    PcgLog* dlog = nullptr;  // device-side address of hlog
+    PcgLog* hlog = gf::pinned_pcg_logs((size_t)6 * S, &dlog);  // setup + 2 chunk slots, 2S entries each
     if (!hlog) return gf::set_error(GF_ESTATE, "gf_scene_pcg_sync: pinned log: %s", gf_last_error());
     const bool timed = kernel_ms != nullptr;
     const int chunk = check_every;
@@ -1530,11 +1566,10 @@ extern "C" int gf_scene_pcg_sync(const gf_scene_t* s, int n_active, int precisio
     int launched = 0, next_slot = 0, npend = 0, pend[2] = {0, 0}, pend_L[2] = {0, 0};
     auto enqueue_chunk = [&]() -> int {
         const int sl = next_slot;
-        int r = pcg_launch_chunk(a, mixed, chunk, L, st, timed ? ev[sl] : nullptr);
+        int r = pcg_launch_chunk(a, mixed, chunk, L, st, timed ? ev[sl] : nullptr,
+                                 dlog + (size_t)2 * S * (1 + sl));
         if (r) return r;
-        hipError_t ce = hipMemcpyAsync(hlog + (size_t)2 * S * (1 + sl), a.log, live_bytes,
-                                       hipMemcpyDeviceToHost, st);
-        if (ce == hipSuccess) ce = hipEventRecord(done_ev[sl], st);
+        hipError_t ce = hipEventRecord(done_ev[sl], st);
         if (ce != hipSuccess)
             return gf::set_error((int)ce, "gf_scene_pcg_sync: readback: %s", hipGetErrorString(ce));
         pend[npend] = sl;
@@ -1601,7 +1636,7 @@ extern "C" int gf_scene_pcg_sync(const gf_scene_t* s, int n_active, int precisio
 extern "C" int gf_scene_pcg_enqueue(const gf_scene_t* s, int n_active, int precision, double tol,
                                     int max_iter, int check_every, int rhs_sign, int first_chunk,
                                     int n_chunks, void* stream) {
-    PcgArgs a;
+    PcgArgs a = {};
     int rc = pcg_prepare(s, n_active, precision, tol, max_iter, rhs_sign, "gf_scene_pcg_enqueue", a);
     if (rc) return rc;
     GF_REQUIRE(check_every >= 1 && check_every <= 64, "gf_scene_pcg_enqueue: check_every %d",
