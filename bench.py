@@ -137,9 +137,6 @@ def main():
                     help="skip the differentiable-rollout leg (64^3, 40 substeps)")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched-scenes leg (256 scenes, 20 substeps)")
-    ap.add_argument("--adjoint-reference", action="store_true",
-                    help="also compare the adjoint gradient with the FP64 torch-CPU reference "
-                         "at full size (about a minute of CPU time)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (3 launches) so that profiler "
                          "byte counters can be scaled against a kernel of known traffic")
@@ -260,7 +257,7 @@ def main():
     if not args.no_adjoint:
         del s
         torch.cuda.empty_cache()
-        adj = adjoint_leg(reference=args.adjoint_reference)
+        adj = adjoint_leg()
         log(f"adjoint: {adj}")
 
     bat = None
@@ -303,14 +300,14 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def adjoint_leg(fd_dirs=16, eps=1e-3, reference=False):
+def adjoint_leg():
     """Differentiable rollout on the 64^3 box (x in [0.2,0.5], y < 0.5, 292,144
     particles, initial velocities N(0, 0.5^2) from seed 3): 40 FLIP substeps at
     dt = 1/240 with mixed PCG to 1e-8 and checkpoints every 10 substeps.  Loss =
     |centre of mass - (0.7,0.3,0.5)|^2.  Reports forward and reverse wall time
-    (the reverse includes the segment recomputation), central differences of the
-    GPU forward on ``fd_dirs`` random directions, and optionally the FP64
-    torch-CPU reference gradient."""
+    (the reverse includes the segment recomputation).  The gradient itself is
+    checked against FP64 references by tests/test_gpu_adjoint.py (16^3) and
+    tests/adjoint_full_check.py (this workload)."""
     import numpy as np
     import torch
     from gpufluid_amd import FlipSolver3D, seeding
@@ -324,19 +321,12 @@ def adjoint_leg(fd_dirs=16, eps=1e-3, reference=False):
                        pressure_iters=500, pressure_tol=1e-8)
     p0 = torch.as_tensor(sc.pos, device="cuda:0")
     v0 = torch.as_tensor(sc.vel, device="cuda:0")
-
-    def loss_of(x):
-        return float(((x.double().cpu().mean(0) - target) ** 2).sum())
-
     roll.run(p0, v0)                      # warm-up (first launches, allocations)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     x = roll.run(p0, v0)
     torch.cuda.synchronize()
     fwd = time.perf_counter() - t0
-    roll.trace_masks = True
-    roll.run(p0, v0, structure="record")  # the discrete structure the adjoint holds fixed
-    frozen, base_trace = roll.structure, list(roll.mask_trace)
     npart = x.shape[0]
     com = x.double().mean(0).cpu()
     gx = (2.0 * (com - target) / npart).float().to("cuda:0").expand(npart, 3).contiguous()
@@ -345,65 +335,14 @@ def adjoint_leg(fd_dirs=16, eps=1e-3, reference=False):
     _, gvel = roll.backward(gx)
     torch.cuda.synchronize()
     bwd = time.perf_counter() - t0
-    g = gvel.double().cpu().numpy()
-    rng = np.random.default_rng(12345)
-    errs, crossed = [], 0
-    for _ in range(fd_dirs):
-        d = rng.normal(size=sc.vel.shape).astype(np.float32)
-        dd = torch.as_tensor(d, device="cuda:0")
-        lp = loss_of(roll.run(p0, v0 + eps * dd, structure=frozen))
-        crossed += roll.mask_trace != base_trace
-        lm = loss_of(roll.run(p0, v0 - eps * dd, structure=frozen))
-        crossed += roll.mask_trace != base_trace
-        an = float((g * d.astype(np.float64)).sum())
-        errs.append(abs((lp - lm) / (2 * eps) - an) / max(abs(an), 1e-30))
-    out = {"workload": "rollout_box_64 (292,144 particles, 40 FLIP substeps, checkpoint every 10)",
-           "forward_ms": round(fwd * 1e3, 2), "reverse_ms": round(bwd * 1e3, 2),
-           "adjoint_over_forward": round(bwd / fwd, 3),
-           "forward_pcg_iters_mean": float(np.mean(roll.forward_iters)),
-           "adjoint_pcg_iters_mean": float(np.mean(roll.adjoint_iters)),
-           "fd_eps": eps, "fd_directions": fd_dirs, "fd_structure": "frozen to the unperturbed run",
-           "fd_runs_that_would_change_a_mask": f"{crossed} of {2 * fd_dirs}",
-           "fd_note": "FP32 differences at this size cross discontinuities of the forward that frozen "
-                      "masks do not remove; --adjoint-reference adds FP64 checks at full size",
-           "fd_rel_err_max": float(np.max(errs)), "fd_rel_err_median": float(np.median(errs))}
-    if reference:
-        from oracle import flip_oracle as fo
-        from oracle import torch_rollout as tr
-        walls = fo.wall_marker(n, n, n)
-        t0 = time.perf_counter()
-        # FP64 reverse pass along the GPU's own trajectory (adjoint arithmetic only)
-        roll.run(p0, v0, keep_states=True)
-        states = [(a.cpu().numpy(), b.cpu().numpy()) for a, b in roll.states]
-        _, gv_same = tr.vjp_along(states, walls, sc.dx, p["dt"], gx.double().cpu().numpy())
-        out["fp64_vjp_same_trajectory_rel_err"] = float(np.linalg.norm(g - gv_same)
-                                                        / np.linalg.norm(gv_same))
-        # FP64 autograd of an independent FP64 rollout (adds FP32/FP64 trajectory drift)
-        xr0 = torch.from_numpy(sc.pos.astype(np.float64))
-        vr0 = torch.from_numpy(sc.vel.astype(np.float64)).requires_grad_()
-        xr, _ = tr.rollout(xr0, vr0, walls, sc.dx, p["dt"], p["substeps"])
-        tr.com_loss(xr, p["target"]).backward()
-        gr = vr0.grad.numpy()
-        out["fp64_rollout_rel_err"] = float(np.linalg.norm(g - gr) / np.linalg.norm(gr))
-        # FP64 central differences on unit-norm directions: at eps 1e-5 no particle
-        # crosses a mask or face-support change, and FP64 round-off stays far below
-        # the signal, so this pins the FP64 gradient at full size
-        fd64 = []
-        rng64 = np.random.default_rng(777)
-        for _ in range(2):
-            d = rng64.normal(size=sc.vel.shape)
-            d /= np.linalg.norm(d)
-            lv = []
-            for sgn in (1.0, -1.0):
-                with torch.no_grad():
-                    xs, _ = tr.rollout(xr0, torch.from_numpy(sc.vel.astype(np.float64) + sgn * 1e-5 * d),
-                                       walls, sc.dx, p["dt"], p["substeps"], checkpoint=False)
-                lv.append(float(tr.com_loss(xs, p["target"])))
-            fd = (lv[0] - lv[1]) / 2e-5
-            fd64.append({"fd": fd, "fp64_grad": float((gr * d).sum()), "gpu_grad": float((g * d).sum())})
-        out["fp64_fd_eps1e-5_unit_dirs"] = fd64
-        out["fp64_reference_seconds"] = round(time.perf_counter() - t0, 1)
-    return out
+    return {"workload": "rollout_box_64 (292,144 particles, 40 FLIP substeps, checkpoint every 10)",
+            "forward_ms": round(fwd * 1e3, 2), "reverse_ms": round(bwd * 1e3, 2),
+            "adjoint_over_forward": round(bwd / fwd, 3),
+            "forward_pcg_iters_mean": float(np.mean(roll.forward_iters)),
+            "adjoint_pcg_iters_mean": float(np.mean(roll.adjoint_iters)),
+            "grad_norm": float(gvel.double().norm()),
+            "gradient_checks": "tests/test_gpu_adjoint.py (16^3, FP64 reference and central "
+                               "differences); tests/adjoint_full_check.py (this workload, FP64)"}
 
 
 def batch_leg(substeps=20, alone_substeps=2, tol=1e-8, max_iter=500):
