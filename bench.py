@@ -207,16 +207,18 @@ def main():
     na = float(np.mean([st["na"] for st in stats])) * 512
     iters = [st["iters"] for st in stats]
     # iteration kernel (two-kernel form: direction+SpMV), update (two-kernel
-    # form only), -, replacement (ms)
+    # form only): ms summed over the sampled iterations (one per PCG chunk);
+    # number of sampled iterations; replacements: ms summed over all
     kms = np.sum([st["kms"] for st in stats], axis=0)
     n_it = sum(iters)
+    n_sampled = int(kms[2])
     n_rep = sum(st["nrep"] for st in stats)
     two_kernel = kms[1] > 0
     if two_kernel:
-        pcg_kern = {"pcg_dirspmv": (kms[0], n_it), "pcg_update": (kms[1], n_it),
+        pcg_kern = {"pcg_dirspmv": (kms[0], n_sampled), "pcg_update": (kms[1], n_sampled),
                     "pcg_replace": (kms[3], n_rep)}
     else:
-        pcg_kern = {"pcg_iter": (kms[0], n_it), "pcg_replace_cg1": (kms[3], n_rep)}
+        pcg_kern = {"pcg_iter": (kms[0], n_sampled), "pcg_replace_cg1": (kms[3], n_rep)}
     kern = {
         **pcg_kern,
         "sort": (sec.get("clear", 0.0), args.steps), "p2g": (sec.get("p2g", 0.0), args.steps),
@@ -231,8 +233,11 @@ def main():
         stages[k] = {"avg_ms": round(avg_ms, 5), "launches": int(calls),
                      "alg_bytes": int(b), "GBps": round(b / (avg_ms * 1e-3) / 1e9, 1),
                      "frac_hbm": round(b / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
-    pcg_ms = float(kms.sum())
-    pcg_bytes = sum(BYTES[k](npart, nf, na) * calls for k, (_, calls) in pcg_kern.items())
+    # whole PCG (iterations at their sampled averages, plus the replacements)
+    it_keys = [k for k in pcg_kern if k in ("pcg_dirspmv", "pcg_update", "pcg_iter")]
+    rep_key = "pcg_replace" if two_kernel else "pcg_replace_cg1"
+    pcg_ms = sum(stages[k]["avg_ms"] for k in it_keys if k in stages) * n_it + float(kms[3])
+    pcg_bytes = sum(BYTES[k](npart, nf, na) for k in it_keys) * n_it + BYTES[rep_key](npart, nf, na) * n_rep
     dom = max(("pcg_dirspmv", "pcg_update") if two_kernel else ("pcg_iter",),
               key=lambda k: kern[k][0])
     d = stages[dom]
@@ -242,6 +247,9 @@ def main():
                 else None,
                 "traffic_source": TRAFFIC_FILE.name, "kernel": dom,
                 "alg_bytes_per_launch": d["alg_bytes"], "avg_launch_ms": d["avg_ms"],
+                "timed_launches": d["launches"],
+                "timing": "HIP events on the kernel's dispatch packets (hipExtLaunchKernel), "
+                          "one iteration per PCG chunk sampled across the timed steps",
                 "pcg_total_GBps": round(pcg_bytes / (pcg_ms * 1e-3) / 1e9, 1),
                 "pcg_total_frac": round(pcg_bytes / (pcg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 

@@ -34,6 +34,7 @@
 // rebuilt from the FP64 true residual b - A(x64+x32) (reliable updates,
 // Sleijpen & van der Vorst 1996).
 #include "gf_common.h"
+#include <hip/hip_ext.h>
 #include <string.h>
 
 namespace {
@@ -1421,52 +1422,75 @@ void pcg_launch_setup(const PcgArgs& a, bool mixed, hipStream_t st) {
 }
 
 // One chunk: `chunk` iterations, then the chunk-end decision (and residual
-// replacement when due).  L is the launch number before the chunk; ev, when
-// given, receives k*chunk+2 events (start, after each kernel, chunk end) with
-// k = pcg_kernels_per_iter.
+// replacement when due).  L is the launch number before the chunk.  ev, when
+// given, holds 2*k+2 events (k = pcg_kernels_per_iter): the start and stop of
+// each kernel of the chunk's middle iteration (chunk/2), then of the chunk-end
+// replacement, taken from the kernels' own dispatch packets
+// (hipExtLaunchKernel), so they time the kernels' execution and not the gaps
+// between launches.  One iteration per chunk is a sample that keeps the
+// timing's own cost out of the other launches.
 int pcg_launch_chunk(const PcgArgs& a, bool mixed, int chunk, int& L, hipStream_t st,
                      hipEvent_t* ev, PcgLog* hout = nullptr) {
     const dim3 grid(a.G), block(GF_WG);
+    const int kpi = pcg_kernels_per_iter(a);
     // the chunk's last kernel also writes the logs to hout (host readback
     // without a copy on the stream)
     PcgArgs al = a;
     al.hout = hout;
-    const int kpi = pcg_kernels_per_iter(a);
-    if (ev) (void)hipEventRecord(ev[0], st);
+    auto e0 = [&](int i) { return ev ? ev[2 * i] : nullptr; };
+    auto e1 = [&](int i) { return ev ? ev[2 * i + 1] : nullptr; };
+    hipEvent_t* const ev_all = ev;
+    // timed launches carry their events on the dispatch packet; untimed ones
+    // (also those recorded into a graph) are plain launches
+#define GF_TLAUNCH(K, E0, E1, ...)                                                  \
+    do {                                                                            \
+        if (ev) hipExtLaunchKernelGGL(K, grid, block, 0, st, E0, E1, 0, __VA_ARGS__); \
+        else hipLaunchKernelGGL(K, grid, block, 0, st, __VA_ARGS__);                \
+    } while (0)
     for (int j = 0; j < chunk; ++j) {
+        const int i = 0;
+        ev = j == chunk / 2 ? ev_all : nullptr;
         if (a.variant == PCG_CG1) {
             ++L;
-            if (mixed) hipLaunchKernelGGL((k_cg1_iter<float, true>), grid, block, 0, st, a, L);
-            else hipLaunchKernelGGL((k_cg1_iter<double, false>), grid, block, 0, st, a, L);
-            if (ev) (void)hipEventRecord(ev[kpi * j + 1], st);
+            if (mixed)
+                GF_TLAUNCH((k_cg1_iter<float, true>), e0(i), e1(i), a, L);
+            else
+                GF_TLAUNCH((k_cg1_iter<double, false>), e0(i), e1(i), a, L);
         } else {
             ++L;
-            if (mixed) hipLaunchKernelGGL((k_pcg_dirspmv<float, true>), grid, block, 0, st, a, L);
-            else hipLaunchKernelGGL((k_pcg_dirspmv<double, false>), grid, block, 0, st, a, L);
-            if (ev) (void)hipEventRecord(ev[kpi * j + 1], st);
+            if (mixed)
+                GF_TLAUNCH((k_pcg_dirspmv<float, true>), e0(i), e1(i), a, L);
+            else
+                GF_TLAUNCH((k_pcg_dirspmv<double, false>), e0(i), e1(i), a, L);
             ++L;
-            if (mixed) hipLaunchKernelGGL((k_pcg_update<float, true>), grid, block, 0, st, a, L);
-            else hipLaunchKernelGGL((k_pcg_update<double, false>), grid, block, 0, st, a, L);
-            if (ev) (void)hipEventRecord(ev[kpi * j + 2], st);
+            if (mixed)
+                GF_TLAUNCH((k_pcg_update<float, true>), e0(i + 1), e1(i + 1), a, L);
+            else
+                GF_TLAUNCH((k_pcg_update<double, false>), e0(i + 1), e1(i + 1), a, L);
         }
     }
+    const int ic = kpi;
+    ev = ev_all;
     ++L;
     if (a.variant == PCG_CG1) {
-        if (mixed) hipLaunchKernelGGL(k_cg1_replace<true>, grid, block, 0, st, a, L);
-        else hipLaunchKernelGGL(k_cg1_replace<false>, grid, block, 0, st, al, L);
         if (mixed) {
+            GF_TLAUNCH(k_cg1_replace<true>, e0(ic), nullptr, a, L);
             ++L;
-            hipLaunchKernelGGL(k_cg1_replace2, grid, block, 0, st, a, L);
+            GF_TLAUNCH(k_cg1_replace2, nullptr, e1(ic), a, L);
+        } else {
+            GF_TLAUNCH(k_cg1_replace<false>, e0(ic), e1(ic), al, L);
         }
     } else {
-        if (mixed) hipLaunchKernelGGL(k_pcg_replace<true>, grid, block, 0, st, a, L);
-        else hipLaunchKernelGGL(k_pcg_replace<false>, grid, block, 0, st, al, L);
+        if (mixed)
+            GF_TLAUNCH(k_pcg_replace<true>, e0(ic), e1(ic), a, L);
+        else
+            GF_TLAUNCH(k_pcg_replace<false>, e0(ic), e1(ic), al, L);
     }
     if (mixed) {
         ++L;
         hipLaunchKernelGGL(k_pcg_fin, dim3(a.S), block, 0, st, al, L);
     }
-    if (ev) (void)hipEventRecord(ev[kpi * chunk + 1], st);
+#undef GF_TLAUNCH
     return gf::check_launch("pcg iteration");
 }
 
@@ -1510,15 +1534,17 @@ extern "C" int gf_scene_pcg_sync(const gf_scene_t* s, int n_active, int precisio
     // Optional per-kernel timing of the iterations that did work: events
     // around every launch of a chunk, read when the chunk's logs are read.
     // kernel_ms: [0] iteration kernel (two-kernel form: direction+SpMV), [1] update
-    // (two-kernel form), [2] unused, [3] chunk-end replacement.
+    // (two-kernel form), summed over the sampled iterations (the middle one of
+    // each chunk, when it did work), [2] the number of sampled iterations, [3]
+    // chunk-end replacements (every one that ran).
     PcgLog* dlog = nullptr;  // device-side address of hlog
     PcgLog* hlog = gf::pinned_pcg_logs((size_t)6 * S, &dlog);  // setup + 2 chunk slots, 2S entries each
     if (!hlog) return gf::set_error(GF_ESTATE, "gf_scene_pcg_sync: pinned log: %s", gf_last_error());
     const bool timed = kernel_ms != nullptr;
     const int chunk = check_every;
     const int kpi = pcg_kernels_per_iter(a);
-    const int nev = timed ? kpi * chunk + 2 : 0;
-    hipEvent_t ev[2][2 * 64 + 2];
+    const int nev = timed ? 2 * kpi + 2 : 0;
+    hipEvent_t ev[2][2 * 2 + 2];
     hipEvent_t done_ev[2] = {nullptr, nullptr};
     int created = 0;
     hipError_t e = hipSuccess;
@@ -1599,16 +1625,18 @@ extern "C" int gf_scene_pcg_sync(const gf_scene_t* s, int n_active, int precisio
         hs = aggregate(hlog + (size_t)2 * S * (1 + sl), Lsl);
         if (timed) {
             const int real = hs.iters - iters_before;
-            for (int j = 0; j < real && j < chunk; ++j) {
+            if (real > chunk / 2) {  // the sampled iteration did work
                 for (int m = 0; m < kpi; ++m) {
                     float t0 = 0.f;
-                    (void)hipEventElapsedTime(&t0, ev[sl][kpi * j + m], ev[sl][kpi * j + m + 1]);
+                    (void)hipEventElapsedTime(&t0, ev[sl][2 * m], ev[sl][2 * m + 1]);
                     kernel_ms[m] += t0;
                 }
+                kernel_ms[2] += 1.0;
             }
             if (mixed && hs.nupd > nupd_before) {
+                const int i = kpi;
                 float t3 = 0.f;
-                (void)hipEventElapsedTime(&t3, ev[sl][kpi * chunk], ev[sl][kpi * chunk + 1]);
+                (void)hipEventElapsedTime(&t3, ev[sl][2 * i], ev[sl][2 * i + 1]);
                 kernel_ms[3] += t3;
             }
         }

@@ -157,7 +157,9 @@ class FlipSolver3D:
         self.last_pressure_residual = None
         self.last_pressure_replacements = 0
         self.pcg_kernel_timing = False      # per-kernel HIP-event timing of PCG launches
-        self.last_pcg_kernel_ms = None      # (dirspmv, update, unused, replacement) ms
+        # (dirspmv, update: ms summed over sampled iterations; sampled iterations;
+        #  replacements: ms summed), see gf_scene_pcg_sync
+        self.last_pcg_kernel_ms = None
         self._cache = {}
         self._pcg_done = DeviceArray(torch.zeros(1, dtype=torch.int32, device=dev))
         self._pcg_tol_sq = DeviceArray(torch.zeros(1, dtype=torch.float32, device=dev))

@@ -181,10 +181,12 @@ int gf_scene_jacobi(const gf_scene_t* s, int n_active, int iters, void* stream);
  * Jacobi-consistent pressure, +1 reproduces the reference PCG's sign.
  * Result in s->pc.  Host outputs (may be NULL): iterations, true relative
  * residual ||b-Ap||/||b|| (FP64; mixed mode only, else the iterated one),
- * number of FP64 residual replacements, and kernel_ms[4]: summed GPU time of
- * the fused direction+SpMV kernel [0], the update kernel [1] and the
- * residual-replacement kernels [3] over the launches that did work ([2] is 0;
- * HIP events on `stream`; NULL disables timing). */
+ * number of FP64 residual replacements, and kernel_ms[4]: GPU execution time
+ * (HIP events on the launches' dispatch packets on `stream`) of the
+ * direction+SpMV (or one-kernel iteration) kernel [0] and the update kernel
+ * [1], summed over a sample of one iteration per check_every-chunk, [2] the
+ * number of sampled iterations, and [3] the summed time of the residual
+ * replacements that ran; NULL disables timing. */
 int gf_scene_pcg_sync(const gf_scene_t* s, int n_active, int precision, double tol,
                       int max_iter, int check_every, int rhs_sign, int* iters,
                       double* rel_residual, int* replacements, double* kernel_ms,
