@@ -129,25 +129,10 @@ __device__ __forceinline__ void st4(T* p, const T (&o)[4]) {
     *reinterpret_cast<typename V4<T>::t*>(p) = v;
 }
 
-// Streamed outputs are stored non-temporally: they are read again only by
-// the next kernel (possibly on another XCD), so keeping them dirty in L2 only
-// adds a write-back at the end of the kernel.
-typedef float gf_f4 __attribute__((ext_vector_type(4)));
-typedef double gf_d2 __attribute__((ext_vector_type(2)));
-
-template <typename T>
-__device__ __forceinline__ void st4_nt(T* p, const T (&o)[4]);
-template <>
-__device__ __forceinline__ void st4_nt<float>(float* p, const float (&o)[4]) {
-    gf_f4 v = {o[0], o[1], o[2], o[3]};
-    *reinterpret_cast<gf_f4*>(p) = v;
-}
-template <>
-__device__ __forceinline__ void st4_nt<double>(double* p, const double (&o)[4]) {
-    gf_d2 v0 = {o[0], o[1]}, v1 = {o[2], o[3]};
-    *reinterpret_cast<gf_d2*>(p) = v0;
-    *reinterpret_cast<gf_d2*>(p + 2) = v1;
-}
+// Outputs use plain stores.  Non-temporal stores measured the same for the
+// two-kernel PCG and slower for the one-kernel form at 3-6 K tiles (31 vs 39
+// us per iteration at 5.8 K tiles), whose vectors are re-read from the
+// last-level cache by the next launch.
 
 __device__ __forceinline__ uint32_t ld_flags(const uint8_t* f, size_t idx) {
     return *reinterpret_cast<const uint32_t*>(f + idx);
@@ -829,12 +814,12 @@ __global__ __launch_bounds__(GF_WG) void k_cg1_iter(PcgArgs a, int L) {
                 st4(a.x64 + idx, xd);
             }
             for (int k = 0; k < 4; ++k) xo[k] = xo[k] + (float)al * (float)pn[k];
-            st4_nt(a.x32 + idx, xo);
+            st4(a.x32 + idx, xo);
         } else {
             double xd[4];
             ld4(a.x64 + idx, xd);
             for (int k = 0; k < 4; ++k) xd[k] += (double)al * (double)pn[k];
-            st4_nt(a.x64 + idx, xd);
+            st4(a.x64 + idx, xd);
         }
         // neighbours' new u
         Nb4<T> o;
@@ -854,10 +839,10 @@ __global__ __launch_bounds__(GF_WG) void k_cg1_iter(PcgArgs a, int L) {
             del += (double)y[k] * (double)c[k];
             rr += r * r;
         }
-        st4_nt(p + idx, pn);
-        st4_nt(un + idx, c);
-        st4_nt(wn + idx, wv);
-        st4_nt(sn + idx, snew);
+        st4(p + idx, pn);
+        st4(un + idx, c);
+        st4(wn + idx, wv);
+        st4(sn + idx, snew);
     }
     put_partial3(a.part, P_RZ, P_PAP, P_RR, L, P.b, gam, del, rr, s_red);
     e.alpha = alpha;
@@ -1082,7 +1067,7 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_dirspmv(PcgArgs a, int L) {
         nb_from_lanes(c, fc, fi, Q, o);
         apply_A4(c, o, fw, y);
         for (int k = 0; k < 4; ++k) pap += (double)c[k] * (double)y[k];
-        st4_nt(pn + idx, c);
+        st4(pn + idx, c);
     }
     put_partial(a.part, P_PAP, L, P.b, pap, s_red);
     e.beta = beta;
@@ -1129,12 +1114,12 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_update(PcgArgs a, int L) {
                 st4(a.x64 + idx, xd);
             }
             for (int k = 0; k < 4; ++k) xo[k] = xo[k] + (float)af * (float)pv[k];
-            st4_nt(a.x32 + idx, xo);
+            st4(a.x32 + idx, xo);
         } else {
             double xd[4];
             ld4(a.x64 + idx, xd);
             for (int k = 0; k < 4; ++k) xd[k] += (double)af * (double)pv[k];
-            st4_nt(a.x64 + idx, xd);
+            st4(a.x64 + idx, xd);
         }
         for (int k = 0; k < 4; ++k) {
             const T dm = diagm<T>(fw >> (8 * k));
@@ -1143,7 +1128,7 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_update(PcgArgs a, int L) {
             rz += (double)rv * (double)zv[k];
             rr += (double)rv * (double)rv;
         }
-        st4_nt(zp + idx, zv);
+        st4(zp + idx, zv);
     }
     put_partial2(a.part, P_RZ, P_RR, L, P.b, rz, rr, s_red);
     e.alpha = alpha;
