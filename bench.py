@@ -43,29 +43,36 @@ BYTES = {
     "pcg_iter": lambda np_, nf, na: 41 * nf,
     # its replacement: x64, x32, b, p, flags in -> u, s~ out; then u, flags in -> w~ out
     "pcg_replace_cg1": lambda np_, nf, na: 38 * nf,
+    # one PCG iteration as a stage, at the figure of the minimal fused FP32
+    # iteration (SpMV p, flags -> Ap: 9; x, r read+write, p, Ap read: 24;
+    # r, flags, p read+write: 13), whatever kernels implement it
+    "pcg_iteration": lambda np_, nf, na: 46 * nf,
 }
 
 
 # Measured HBM traffic per dispatch (rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled
 # by a plain-copy calibration) for this workload; the file states its commands.
 TRAFFIC_FILE = ROOT / "profiles" / "pmc_traffic_substep_pool_sphere_256.txt"
-TRAFFIC_KERNEL = {"pcg_dirspmv": "k_pcg_dirspmv<float,true>", "pcg_update": "k_pcg_update<float,true>",
-                  "pcg_replace": "k_pcg_replace<true>", "pcg_iter": "k_cg1_iter<float,true>",
-                  "p2g": "k_p2g_tile<false,false>", "g2p": "k_g2p_tile<false>"}
+TRAFFIC_KERNEL = {"pcg_dirspmv": ["k_pcg_dirspmv<float,true>"], "pcg_update": ["k_pcg_update<float,true>"],
+                  "pcg_replace": ["k_pcg_replace<true>"], "pcg_iter": ["k_cg1_iter<float,true>"],
+                  "p2g": ["k_p2g_tile<false,false>"], "g2p": ["k_g2p_tile<false>"]}
 
 
 def measured_traffic(stage):
-    """HBM bytes per launch of a stage's kernel from the committed PMC summary, or None."""
+    """HBM bytes per launch of a stage's kernels from the committed PMC summary, or None."""
     try:
         lines = TRAFFIC_FILE.read_text().splitlines()
     except OSError:
         return None
-    want = TRAFFIC_KERNEL.get(stage)
+    found = {}
     for line in lines:
         f = line.split()
-        if len(f) == 5 and f[0] == want:
-            return int(f[4])
-    return None
+        if len(f) == 5:
+            found[f[0]] = int(f[4])
+    want = TRAFFIC_KERNEL.get(stage, [])
+    if not want or any(k not in found for k in want):
+        return None
+    return sum(found[k] for k in want)
 
 
 def log(msg):
@@ -233,25 +240,43 @@ def main():
         stages[k] = {"avg_ms": round(avg_ms, 5), "launches": int(calls),
                      "alg_bytes": int(b), "GBps": round(b / (avg_ms * 1e-3) / 1e9, 1),
                      "frac_hbm": round(b / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
-    # whole PCG (iterations at their sampled averages, plus the replacements)
+    # The PCG iteration as one stage: its kernels' sampled average durations
+    # added, at the minimal fused iteration's 46 bytes per fluid cell.
     it_keys = [k for k in pcg_kern if k in ("pcg_dirspmv", "pcg_update", "pcg_iter")]
     rep_key = "pcg_replace" if two_kernel else "pcg_replace_cg1"
-    pcg_ms = sum(stages[k]["avg_ms"] for k in it_keys if k in stages) * n_it + float(kms[3])
-    pcg_bytes = sum(BYTES[k](npart, nf, na) for k in it_keys) * n_it + BYTES[rep_key](npart, nf, na) * n_rep
-    dom = max(("pcg_dirspmv", "pcg_update") if two_kernel else ("pcg_iter",),
-              key=lambda k: kern[k][0])
+    it_ms = sum(stages[k]["avg_ms"] for k in it_keys if k in stages)
+    if it_ms > 0:
+        b = BYTES["pcg_iteration"](npart, nf, na)
+        stages["pcg_iteration"] = {"avg_ms": round(it_ms, 5), "launches": n_sampled,
+                                   "kernels": it_keys, "alg_bytes": int(b),
+                                   "GBps": round(b / (it_ms * 1e-3) / 1e9, 1),
+                                   "frac_hbm": round(b / (it_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    # whole PCG: iterations at their sampled average plus the replacements, in
+    # the kernels' own compulsory bytes (34 per fluid cell per iteration for the
+    # two-kernel form, fewer than the 46 of the stage figure)
+    pcg_ms = it_ms * n_it + float(kms[3])
+    own_bytes = (sum(BYTES[k](npart, nf, na) for k in it_keys) * n_it
+                 + BYTES[rep_key](npart, nf, na) * n_rep)
+    dom = "pcg_iteration"
     d = stages[dom]
+    c3_mixed = args.precision == "mixed" and args.config == "c3"
+    traffic = [measured_traffic(k) for k in it_keys] if c3_mixed else [None]
     roofline = {"bound": "hbm", "achieved": d["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": d["frac_hbm"],
-                "traffic": measured_traffic(dom) if args.precision == "mixed" and args.config == "c3"
-                else None,
+                "traffic": sum(traffic) if None not in traffic else None,
                 "traffic_source": TRAFFIC_FILE.name, "kernel": dom,
+                "launch_kernels": [TRAFFIC_KERNEL[k][0] for k in it_keys],
                 "alg_bytes_per_launch": d["alg_bytes"], "avg_launch_ms": d["avg_ms"],
-                "timed_launches": d["launches"],
-                "timing": "HIP events on the kernel's dispatch packets (hipExtLaunchKernel), "
+                "timed_launches": n_sampled,
+                "timing": "HIP events on the kernels' dispatch packets (hipExtLaunchKernel), "
                           "one iteration per PCG chunk sampled across the timed steps",
-                "pcg_total_GBps": round(pcg_bytes / (pcg_ms * 1e-3) / 1e9, 1),
-                "pcg_total_frac": round(pcg_bytes / (pcg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+                "kernel_own_bytes": {k: {"alg_bytes": stages[k]["alg_bytes"],
+                                         "avg_ms": stages[k]["avg_ms"],
+                                         "frac_hbm": stages[k]["frac_hbm"],
+                                         "traffic": measured_traffic(k) if c3_mixed else None}
+                                     for k in it_keys if k in stages},
+                "pcg_total_own_GBps": round(own_bytes / (pcg_ms * 1e-3) / 1e9, 1),
+                "pcg_total_own_frac": round(own_bytes / (pcg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 
     pcie = pcie_inclusive(s, npart, ms_step)
     log(f"pcie-inclusive: {pcie}")
