@@ -94,6 +94,7 @@ SIGNATURES = {
     "gf_scene_p2g": [SP, I, F, F, F, V],
     "gf_scene_p2g_raw": [SP, I, V, V, V, V],
     "gf_scene_jacobi": [SP, I, I, V],
+    "gf_scene_gsrb": [SP, I, I, V],
     "gf_scene_pcg_sync": [SP, I, I, D, I, I, I, PI, PD, PI, PD, V],
     "gf_scene_pcg_stats": [SP, PI, PD, PI, V],
     "gf_scene_pcg_enqueue": [SP, I, I, D, I, I, I, I, I, V],
@@ -107,6 +108,7 @@ SIGNATURES = {
     "gf_k3_enforce_solid_bc": [V, V, V, V, V, V, V, I, I, I, V],
     "gf_k3_compute_divergence": [V, V, V, V, V, I, I, I, F, F, F, V],
     "gf_k3_jacobi_pressure": [V, V, V, V, I, I, I, V],
+    "gf_k3_gauss_seidel_rb": [V, V, V, I, I, I, I, V],
     "gf_k3_apply_A": [V, V, V, V, I, I, I, V],
     "gf_k3_compute_diag": [V, V, I, I, I, V],
     "gf_k3_apply_invM": [V, V, V, V, V, I, I, I, V],

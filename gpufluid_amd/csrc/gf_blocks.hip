@@ -157,6 +157,20 @@ __global__ void k_jacobi_dense(const float* p_in, float* p_out, const float* div
     p_out[c] = (s - div[c]) / d;
 }
 
+// One red-black Gauss-Seidel half sweep in place (solver3d.py:1204-1267):
+// cells with (i+j+k) % 2 == color.
+__global__ void k_gsrb_dense(float* p, const float* div, const int32_t* marker, int nx, int ny,
+                             int nz, int color) {
+    int i, j, k;
+    if (!idx3(nx, ny, nz, i, j, k)) return;
+    if (((i + j + k) & 1) != color) return;
+    size_t c = ((size_t)i * ny + j) * nz + k;
+    if (marker[c] != 1) { p[c] = 0.0f; return; }
+    float s, d;
+    stencil(p, marker, i, j, k, nx, ny, nz, s, d);
+    p[c] = d < 0.5f ? 0.0f : (s - div[c]) / d;
+}
+
 __global__ void k_apply_A_dense(const float* x, float* y, const int32_t* marker,
                                 const int32_t* done, int nx, int ny, int nz) {
     if (done && done[0] != 0) return;
@@ -505,6 +519,17 @@ extern "C" int gf_k3_jacobi_pressure(const float* p_in, float* p_out, const floa
     hipLaunchKernelGGL(k_jacobi_dense, grid_for((long long)nx * ny * nz), dim3(GF_WG), 0,
                        (hipStream_t)stream, p_in, p_out, div, marker, nx, ny, nz);
     GF_LAUNCH_CHECK("k_jacobi_dense");
+    return 0;
+}
+
+extern "C" int gf_k3_gauss_seidel_rb(float* p, const float* div, const int32_t* marker,
+                                     int nx, int ny, int nz, int color, void* stream) {
+    GRID_ARGS_OK("gf_k3_gauss_seidel_rb");
+    GF_REQUIRE(p && div && marker, "gf_k3_gauss_seidel_rb: null buffer");
+    GF_REQUIRE(color == 0 || color == 1, "gf_k3_gauss_seidel_rb: color %d", color);
+    hipLaunchKernelGGL(k_gsrb_dense, grid_for((long long)nx * ny * nz), dim3(GF_WG), 0,
+                       (hipStream_t)stream, p, div, marker, nx, ny, nz, color);
+    GF_LAUNCH_CHECK("k_gsrb_dense");
     return 0;
 }
 

@@ -1253,6 +1253,39 @@ __global__ __launch_bounds__(GF_WG) void k_jacobi(const uint8_t* __restrict__ fl
     st4(pout + idx, out);
 }
 
+// One red-black Gauss-Seidel half sweep (solver3d.py:1204-1267) in place on
+// compact tiles: cells whose global (i+j+k) is even (color 0) or odd
+// (color 1) get p = (sum of fluid-neighbour p - div) / diag, in the
+// reference's neighbour order; non-fluid cells and diag 0 give 0.  Tiles
+// start at multiples of 8, so the colour of quad element e is
+// (li + lj + e) % 2.  A cell's six neighbours all have the other colour and
+// are only read in this launch, so the in-place update has no race; the
+// quad is written back whole (its other-colour cells unchanged).
+__global__ __launch_bounds__(GF_WG) void k_gsrb(const uint8_t* __restrict__ flags,
+                                                const int32_t* __restrict__ nbr,
+                                                const float* __restrict__ b, float* p, int nact,
+                                                int color) {
+    const int t = __builtin_amdgcn_readfirstlane(2 * gf::xcd_remap(blockIdx.x, gridDim.x) +
+                                                 (threadIdx.x >> 7));
+    if (t >= nact) return;
+    const Quad Q = quad_of();
+    const size_t base = (size_t)t * GF_TILE_CELLS;
+    const size_t idx = base + Q.l0;
+    const uint32_t fw = ld_flags(flags, idx);
+    float c[4], bv[4], out[4];
+    ld4(p + idx, c);
+    ld4(b + idx, bv);
+    Nb4<float> o;
+    load_nb4(p, nb_index(nbr, t, base, Q), o);
+    for (int k = 0; k < 4; ++k) {
+        const int d = fl_diag(fw, k);
+        const float v = (nb_sum4(o, c, k) - bv[k]) / (float)(d > 0 ? d : 1);
+        const float upd = keep_if(v, fl_fluid(fw, k) & (d > 0));
+        out[k] = ((Q.li + Q.lj + k) & 1) == color ? upd : c[k];
+    }
+    st4(p + idx, out);
+}
+
 __global__ void k_fill_f32(float* a, size_t n, float v) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) a[i] = v;
@@ -1319,6 +1352,27 @@ extern "C" int gf_scene_jacobi(const gf_scene_t* s, int n_active, int iters, voi
                            s->nbr, s->b, in, out, n_active);
     }
     GF_LAUNCH_CHECK("k_jacobi");
+    return 0;
+}
+
+extern "C" int gf_scene_gsrb(const gf_scene_t* s, int n_active, int iters, void* stream) {
+    int rc = gf::scene_check(s, "gf_scene_gsrb");
+    if (rc) return rc;
+    const int ntiles = s->tnx * s->tny * s->tnz;
+    GF_REQUIRE(n_active >= 0 && n_active <= ntiles, "gf_scene_gsrb: n_active %d", n_active);
+    GF_REQUIRE(iters >= 0, "gf_scene_gsrb: iters %d", iters);
+    GF_REQUIRE(s->flags && s->nbr && s->b && s->pc, "gf_scene_gsrb: buffers not set");
+    if (n_active == 0) return 0;
+    hipStream_t st = (hipStream_t)stream;
+    const size_t n = (size_t)n_active * GF_TILE_CELLS;
+    hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)((n + GF_WG - 1) / GF_WG)), dim3(GF_WG), 0, st,
+                       s->pc, n, 0.0f);
+    for (int k = 0; k < iters; ++k) {
+        for (int color = 0; color < 2; ++color)
+            hipLaunchKernelGGL(k_gsrb, dim3((n_active + 1) / 2), dim3(GF_WG), 0, st, s->flags,
+                               s->nbr, s->b, s->pc, n_active, color);
+    }
+    GF_LAUNCH_CHECK("k_gsrb");
     return 0;
 }
 

@@ -162,6 +162,14 @@ def k3_jacobi_pressure(p_in, p_out, div, marker, off_x=0, off_y=0, off_z=0):
               ny, nz, _st(p_in))
 
 
+@block("S2.6.2", "Gauss-Seidel red-black sweep (in-place)")
+def k3_gauss_seidel_rb(p, div, marker, color, off_x=0, off_y=0, off_z=0):
+    _offsets("S2.6.2", off_x, off_y, off_z)
+    nx, ny, nz = _cells(marker)
+    _lib.call("S2.6.2", "gf_k3_gauss_seidel_rb", _p(p), _p(div), _p(marker), nx, ny, nz,
+              int(color), _st(p))
+
+
 @block("S2.6.3", "PCG: apply Laplacian operator A to x")
 def k3_apply_A(x, y, marker, done, off_x=0, off_y=0, off_z=0):
     _offsets("S2.6.3", off_x, off_y, off_z)

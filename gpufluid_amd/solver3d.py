@@ -10,7 +10,8 @@ HIP kernels (libgpufluid_hip.so):
   clear       gf_scene_sort_sync   bin + stable counting sort of particles by
                                    cell, marker, active 8^3 tiles
   p2g         gf_scene_p2g         P2G + normalize + gravity + BC + divergence
-  pressure    gf_scene_jacobi | gf_scene_pcg_sync   on compacted active tiles
+  pressure    gf_scene_jacobi | gf_scene_gsrb | gf_scene_pcg_sync   on compacted
+                                   active tiles
   g2p_advect  gf_scene_g2p         gradient subtract + BC + G2P + advection
 
 Differences from the reference that callers can observe (all deliberate):
@@ -326,13 +327,12 @@ class FlipSolver3D:
     def step(self, dt: float, pressure_iters: int = 80, pressure_solver: str = "jacobi",
              pressure_block_sparse: bool = False, *, pressure_tol: float | None = None):
         """Advance one substep of length ``dt`` (solver3d.py:3422-3667).
-        ``pressure_iters`` is the Jacobi sweep count or the PCG iteration cap.
-        ``pressure_block_sparse`` is accepted for compatibility: the pressure
-        solve always runs on the active 8^3 tiles only."""
+        ``pressure_iters`` is the Jacobi / GS-RB sweep count or the PCG
+        iteration cap.  ``pressure_block_sparse`` is accepted for
+        compatibility: the pressure solve always runs on the active 8^3 tiles
+        only, which is what the reference's per-tile variants compute."""
         if pressure_solver not in ("jacobi", "pcg", "gsrb"):
             raise ValueError(f"unknown pressure_solver: {pressure_solver!r}")
-        if pressure_solver == "gsrb":
-            raise NotImplementedError("GS-RB (S2.6.2) is not part of the MI355X substep build")
         tol = self.pcg_tol if pressure_tol is None else float(pressure_tol)
         self._step_impl(float(dt), int(pressure_iters), pressure_solver, tol)
 
@@ -375,6 +375,11 @@ class FlipSolver3D:
         with prof.section("pressure"):
             if pressure_solver == "jacobi":
                 _lib.call("S2.6.1", "gf_scene_jacobi", ctypes.byref(s), na[0],
+                          pressure_iters, stream)
+                self.last_pressure_iters = pressure_iters
+                self.last_pressure_residual = None
+            elif pressure_solver == "gsrb":
+                _lib.call("S2.6.2", "gf_scene_gsrb", ctypes.byref(s), na[0],
                           pressure_iters, stream)
                 self.last_pressure_iters = pressure_iters
                 self.last_pressure_residual = None

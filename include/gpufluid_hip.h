@@ -172,6 +172,13 @@ int gf_scene_p2g_raw(const gf_scene_t* s, int transfer_mode, float* uw, float* v
  * driver (solver3d.py:3570-3600). */
 int gf_scene_jacobi(const gf_scene_t* s, int n_active, int iters, void* stream);
 
+/* `iters` red-black Gauss-Seidel sweeps (red = (i+j+k) even, then black) of
+ * A p = -div in place on the compact tiles from p = 0 (the pressure_solver
+ * "gsrb" branch of solver3d.py:3603-3630, sweep k3_gauss_seidel_rb
+ * 1204-1267; tile-compact like the reference's per-tile variant
+ * k3_gauss_seidel_rb_per_tile).  Result in s->pc. */
+int gf_scene_gsrb(const gf_scene_t* s, int n_active, int iters, void* stream);
+
 /* Diagonal-preconditioned CG for A p = rhs_sign * div on the compact tiles
  * (A = the 7-point operator of k3_apply_A, solver3d.py:1743-1797; Jacobi
  * preconditioner of k3_compute_diag, 1803-1823; driver 2917-3043).
@@ -261,6 +268,10 @@ int gf_k3_compute_divergence(const float* u, const float* v, const float* w, flo
 /* k3_jacobi_pressure, solver3d.py:878-938 */
 int gf_k3_jacobi_pressure(const float* p_in, float* p_out, const float* div,
                           const int32_t* marker, int nx, int ny, int nz, void* stream);
+/* k3_gauss_seidel_rb, solver3d.py:1204-1267: one in-place half sweep over
+ * the cells with (i+j+k) % 2 == color (0 red, 1 black) */
+int gf_k3_gauss_seidel_rb(float* p, const float* div, const int32_t* marker, int nx, int ny,
+                          int nz, int color, void* stream);
 /* k3_apply_A, solver3d.py:1743-1797 (done: device int[1], may be NULL) */
 int gf_k3_apply_A(const float* x, float* y, const int32_t* marker, const int32_t* done,
                   int nx, int ny, int nz, void* stream);

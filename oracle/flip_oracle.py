@@ -288,6 +288,28 @@ def jacobi(div, marker, iters, dtype=np.float32):
     return p
 
 
+def gsrb_half_sweep(p, div, marker, color):
+    """One in-place red-black Gauss-Seidel half sweep (solver3d.py:1204-1267):
+    cells with (i+j+k) % 2 == color get (Σ_fluid-nb p − div)/diag on fluid
+    cells with diag >= 0.5, else 0; other cells keep their value.  A cell's
+    neighbours all have the other colour, so updating the whole colour at
+    once equals the sequential in-place loop."""
+    ii, jj, kk = np.indices(marker.shape)
+    mine = ((ii + jj + kk) % 2) == color
+    new = jacobi_sweep(p, div.astype(p.dtype), marker)
+    return np.where(mine, new, p).astype(p.dtype)
+
+
+def gsrb(div, marker, iters, dtype=np.float32):
+    """Driver of the "gsrb" pressure branch (solver3d.py:3603-3630): p = 0,
+    then `iters` red + black half sweeps."""
+    p = np.zeros(marker.shape, dtype=dtype)
+    for _ in range(iters):
+        p = gsrb_half_sweep(p, div, marker, 0)
+        p = gsrb_half_sweep(p, div, marker, 1)
+    return p
+
+
 def apply_A(x, marker):
     """A x = diag·x − Σ_fluid-nb x on fluid cells, 0 elsewhere (solver3d.py:1743-1797)."""
     diag = neighbour_count(marker).astype(x.dtype)
@@ -520,7 +542,7 @@ def substep(state: dict, dt: float, pressure="pcg", iters=80, tol=1e-4, dtype=np
     """One substep in the stage order of _step_impl (solver3d.py:3475-3667):
     clear → P2G → normalize → gravity → BC → divergence → pressure → grad → BC
     → G2P + advect.  ``state`` holds pos, vel, marker, dx and (APIC) affine.
-    pressure: "jacobi" (fixed iterations), "pcg" (reference restatement;
+    pressure: "jacobi" / "gsrb" (fixed iterations), "pcg" (reference restatement;
     pcg_sign "physical" negates its result, "reference" keeps it) or
     "exact" (spsolve).  Returns a new state plus the grid fields."""
     pos, vel, marker, dx = state["pos"], state["vel"], state["marker"], state["dx"]
@@ -540,6 +562,8 @@ def substep(state: dict, dt: float, pressure="pcg", iters=80, tol=1e-4, dtype=np
     info = {}
     if pressure == "jacobi":
         p = jacobi(div, m, iters, dtype=dtype)
+    elif pressure == "gsrb":
+        p = gsrb(div, m, iters, dtype=dtype)
     elif pressure == "pcg":
         x, it = pcg_reference(div, m, iters, tol=tol, dtype=dtype)
         p = -x if pcg_sign == "physical" else x

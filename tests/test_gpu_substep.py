@@ -117,6 +117,40 @@ def test_jacobi_bit_exact_against_fp32_oracle():
     np.testing.assert_array_equal(got, want)
 
 
+def test_gsrb_bit_exact_against_fp32_oracle():
+    """Red-black Gauss-Seidel on the compact tiles equals the FP32 oracle bit
+    for bit (in-place sweeps, reference sum order)."""
+    from gpufluid_amd import _lib
+    s = _prepared(shape=(20, 24, 28), seed=6)
+    div = s.div.numpy()
+    m = s.marker.numpy()
+    d = s._descriptor()
+    _lib.call("S2.6.2", "gf_scene_gsrb", ctypes.byref(d), s.n_active_tiles, 17, s._stream())
+    s._cache.clear()
+    want = fo.gsrb(div, m, 17, dtype=np.float32)
+    np.testing.assert_array_equal(s.p.numpy(), want)
+
+
+def test_gsrb_block_kernel_bit_exact():
+    """The S2.6.2 block callable (dense, in place) equals the oracle half sweep."""
+    import torch
+    from gpufluid_amd import kernels
+    rng = np.random.default_rng(13)
+    m = fo.wall_marker(10, 12, 9)
+    inner = m[1:-1, 1:-1, 1:-1]
+    inner[...] = np.where(rng.random(inner.shape) < 0.7, fo.FLUID, fo.AIR)
+    div = np.where(m == fo.FLUID, rng.normal(size=m.shape), 0).astype(np.float32)
+    p0 = np.where(m == fo.FLUID, rng.normal(size=m.shape), 0).astype(np.float32)
+    p = torch.tensor(p0, device="cuda:0")
+    dt = torch.tensor(div, device="cuda:0")
+    mt = torch.tensor(m, device="cuda:0")
+    want = p0
+    for color in (0, 1, 0, 1):
+        kernels.k3_gauss_seidel_rb(p, dt, mt, color)
+        want = fo.gsrb_half_sweep(want, div, m, color)
+    np.testing.assert_array_equal(p.cpu().numpy(), want)
+
+
 @pytest.fixture(params=[1, 2], ids=["two_kernel", "one_kernel"])
 def pcg_variant(request):
     """Both PCG iteration forms (1: direction+SpMV and update kernels,
@@ -165,15 +199,16 @@ def test_pcg_reference_sign_is_negated(pcg_variant):
 
 
 @pytest.mark.parametrize("transfer,solver", [("flip", "jacobi"), ("flip", "pcg"),
-                                             ("apic", "pcg"), ("pic", "jacobi")])
+                                             ("apic", "pcg"), ("pic", "jacobi"),
+                                             ("flip", "gsrb")])
 def test_full_substep_matches_oracle(transfer, solver):
     shape, dx, pos, vel = gu.dam_scene(shape=(20, 24, 16), vel_sigma=0.4, seed=11)
     fb = 0.0 if transfer == "pic" else 0.95
     s = gu.make_solver(shape, dx, pos, vel, transfer_mode=transfer, flip_blend=fb)
     dt = 1.0 / 240
-    if solver == "jacobi":
-        s.step(dt, pressure_iters=30, pressure_solver="jacobi")
-        st, grid = fo.substep(gu.oracle_state(shape, dx, pos, vel), dt, pressure="jacobi",
+    if solver in ("jacobi", "gsrb"):
+        s.step(dt, pressure_iters=30, pressure_solver=solver)
+        st, grid = fo.substep(gu.oracle_state(shape, dx, pos, vel), dt, pressure=solver,
                               iters=30, transfer=transfer, flip_blend=fb)
     else:
         s.step(dt, pressure_iters=400, pressure_solver="pcg", pressure_tol=1e-8)

@@ -261,3 +261,51 @@ def test_apic_g2p_affine_reconstruction_closed_form():
     np.testing.assert_allclose(vel[:, 0], a * pos[:, 0].astype(np.float64), rtol=1e-6)
     np.testing.assert_allclose(C[:, 0, 0], 3 * a * sx * (1 - sx), rtol=1e-5)
     np.testing.assert_allclose(C[:, 0, 1:], 0, atol=1e-9)
+
+
+# ------------------------------------------------------------------ GS-RB
+def test_gsrb_half_sweep_matches_sequential_in_place_loop():
+    """The colour-at-once GS-RB half sweep equals the reference's in-place
+    loop over the cells of one colour (solver3d.py:1204-1267), visiting cells
+    in any order, in FP32 with the -x,+x,-y,+y,-z,+z sum order."""
+    m = _blob_marker((6, 7, 5), 4)
+    rng = np.random.default_rng(11)
+    div = np.where(m == fo.FLUID, rng.normal(size=m.shape), 0).astype(np.float32)
+    p0 = np.where(m == fo.FLUID, rng.normal(size=m.shape), 0).astype(np.float32)
+    n = m.shape
+    for color in (0, 1):
+        got = fo.gsrb_half_sweep(p0, div, m, color)
+        want = p0.copy()
+        cells = [tuple(c) for c in np.argwhere(np.ones(n, bool))]
+        rng.shuffle(cells)                          # in-place, any visiting order
+        for i, j, k in cells:
+            if (i + j + k) % 2 != color:
+                continue
+            if m[i, j, k] != fo.FLUID:
+                want[i, j, k] = 0
+                continue
+            s = np.float32(0)
+            d = np.float32(0)
+            for di, dj, dk in ((-1, 0, 0), (1, 0, 0), (0, -1, 0), (0, 1, 0), (0, 0, -1), (0, 0, 1)):
+                a, b, c = i + di, j + dj, k + dk
+                if 0 <= a < n[0] and 0 <= b < n[1] and 0 <= c < n[2] and m[a, b, c] != fo.SOLID:
+                    d = np.float32(d + 1)
+                    if m[a, b, c] == fo.FLUID:
+                        s = np.float32(s + want[a, b, c])
+            want[i, j, k] = 0 if d < 0.5 else np.float32(np.float32(s - div[i, j, k]) / d)
+        np.testing.assert_array_equal(got, want)
+
+
+def test_gsrb_converges_to_the_direct_solve_faster_than_jacobi():
+    m = _blob_marker((10, 9, 8), 6)
+    rng = np.random.default_rng(12)
+    div = np.where(m == fo.FLUID, rng.normal(size=m.shape), 0.0)
+    exact = fo.solve_exact(div, m)
+    scale = np.abs(exact).max()
+    assert np.abs(fo.gsrb(div, m, 300, dtype=np.float64) - exact).max() < 1e-9 * scale
+    err_gs = np.abs(fo.gsrb(div, m, 20, dtype=np.float64) - exact).max() / scale
+    err_j = np.abs(fo.jacobi(div, m, 20, dtype=np.float64) - exact).max() / scale
+    assert err_gs < 0.5 * err_j
+    # one red + black sweep from the exact solution leaves it unchanged
+    p = fo.gsrb_half_sweep(exact, div, m, 0)
+    np.testing.assert_allclose(fo.gsrb_half_sweep(p, div, m, 1), exact, atol=1e-12)
