@@ -144,6 +144,8 @@ def main():
                     help="skip the differentiable-rollout leg (64^3, 40 substeps)")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched-scenes leg (256 scenes, 20 substeps)")
+    ap.add_argument("--no-attributes", action="store_true",
+                    help="skip the colour/temperature transfer leg (3 substeps on the scene)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (3 launches) so that profiler "
                          "byte counters can be scaled against a kernel of known traffic")
@@ -286,6 +288,11 @@ def main():
         fp64 = compare_fp64(s, tol, max_iter, args.check_every)
         log(f"pcg mixed vs fp64: {fp64}")
 
+    attrs = None
+    if not args.no_attributes:
+        attrs = attribute_leg(s, one_step)
+        log(f"attributes: {attrs}")
+
     adj = None
     if not args.no_adjoint:
         del s
@@ -327,10 +334,40 @@ def main():
         "stages": stages,
         "pcg_vs_fp64": fp64,
         "pcie_inclusive_value": pcie,
+        "attributes": attrs,
         "adjoint": adj,
         "batched": bat,
     }
     print(json.dumps(out), flush=True)
+
+
+def attribute_leg(s, one_step, steps=3):
+    """Per-particle colour (3 channels) and temperature (1 channel) on every
+    particle of the scene: substep time with both transfers, and each
+    transfer's own time (HIP events around its launches)."""
+    import numpy as np
+    import torch
+    n = s.n_particles
+    rng = np.random.default_rng(0)
+    s.attr_color = rng.random((n, 3), dtype=np.float32)
+    s.attr_temperature = (300.0 + 50.0 * rng.random(n)).astype(np.float32)
+    one_step()
+    s._prof.reset()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        one_step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    sec = s._prof.summarize()
+    col, tmp = sec.get("color", 0.0) / steps, sec.get("temperature", 0.0) / steps
+    s.attr_color = None
+    s.attr_temperature = None
+    return {"particles": n, "steps": steps, "ms_per_step": round(ms, 3),
+            "color_ms": round(col, 4), "temperature_ms": round(tmp, 4),
+            "color_Mparticles_per_s": round(n / (col * 1e-3) / 1e6, 1) if col else None,
+            "temperature_Mparticles_per_s": round(n / (tmp * 1e-3) / 1e6, 1) if tmp else None,
+            "timing": "wall clock around the substeps; transfers by HIP events (step profiler)"}
 
 
 def adjoint_leg():

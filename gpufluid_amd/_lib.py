@@ -18,6 +18,8 @@ GF_TRANSFER_FLIP = 0
 GF_TRANSFER_APIC = 1
 GF_PCG_MIXED = 0
 GF_PCG_FP64 = 1
+GF_ATTR_COLOR = 0
+GF_ATTR_SCALAR = 1
 GF_EINVAL, GF_ERANGE, GF_ESTATE = -1, -2, -3  # negative return codes of the C-ABI
 TILE = 8
 TILE_CELLS = 512
@@ -99,6 +101,7 @@ SIGNATURES = {
     "gf_scene_pcg_stats": [SP, PI, PD, PI, V],
     "gf_scene_pcg_enqueue": [SP, I, I, D, I, I, I, I, I, V],
     "gf_scene_g2p": [SP, V, I, F, F, F, F, V],
+    "gf_scene_attr_transfer": [SP, V, I, I, V, V, V, V],
     "gf_scene_compact_to_dense": [SP, V, V, V],
     "gf_scene_unsort": [SP, V, I, V, V],
     "gf_k3_clear_grid": [V, V, V, V, V, V, V, I, I, I, V],

@@ -438,6 +438,191 @@ __global__ __launch_bounds__(GF_WG) void k_vmax_sq(const float* vel, int n, floa
     }
 }
 
+// ---- per-particle attribute transfer (S2.15 colour, S2.18 scalar) --------
+// P2G of `nc` channels per particle onto the cell-centred grid at the current
+// positions (solver3d.py:562-605, 670-707), normalise (607-631, 710-727) and
+// trilinear gather back (633-664, 729-755).  The scatter sums fixed-point
+// integers with integer atomics, so the result does not depend on the order
+// of arrival: each contribution is the reference's FP32 product value*w (and
+// w), scaled by a power of two chosen from the largest |value| (values by
+// 2^(48-E) with max |value| < 2^E, weights by 2^48), then rounded to an
+// integer.  A cell sum stays below 2^63 while the particles of the 2x2x2
+// cells around it number fewer than 32768; the rounding error is at most
+// 2^-49 of max |value| per contribution, far below FP32 resolution.
+// The cell sums are converted back to FP32 before the reference's FP32
+// normalisation.  Attributes are stored in particle-id order; slot p of the
+// cell-sorted arrays holds particle pid[p].
+
+// Zero the fixed-point sums and the max-|value| word.
+__global__ void k_attr_clear(int64_t* a, long long n, uint32_t* scratch) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0;
+    if (i == 0) scratch[0] = 0;
+}
+
+// scratch[0] = bits of max |attr| (non-negative floats order as unsigned).
+__global__ void k_attr_absmax(const float* attr, long long n, uint32_t* scratch) {
+    uint32_t m = 0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const uint32_t b = __float_as_uint(attr[i]) & 0x7fffffffu;
+        m = b > m && b < 0x7f800000u ? b : m;  // NaN/inf ignored for the scale
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint32_t o = __shfl_xor(m, off, 64);
+        m = o > m ? o : m;
+    }
+    if ((threadIdx.x & 63) == 0) atomicMax(scratch, m);
+}
+
+// Fixed-point exponents: values get 2^(48 - E) with |value| < 2^E, weights 2^48.
+__device__ __forceinline__ int attr_exp(const uint32_t* scratch) {
+    int e;
+    (void)frexpf(__uint_as_float(scratch[0]), &e);
+    return 48 - e;
+}
+
+__device__ __forceinline__ void attr_cell_coords(const float* pos, int p, float dx, float& fx,
+                                                 float& fy, float& fz) {
+    fx = pos[3 * p + 0] / dx - 0.5f;
+    fy = pos[3 * p + 1] / dx - 0.5f;
+    fz = pos[3 * p + 2] / dx - 0.5f;
+}
+
+// A workgroup's 256 consecutive slots hold particles that were cell-sorted at
+// the start of the substep and have moved by about a cell since, so their
+// base cells fill a small box.  The group sums its contributions in LDS over
+// that box (integer sums, so the order of the LDS atomics does not matter)
+// and adds the non-zero cells to the grid sums; a group whose box exceeds the
+// LDS budget adds each contribution to the grid directly.
+constexpr int ATTR_LDS_BYTES = 48 * 1024;
+
+template <int NC>
+__global__ __launch_bounds__(GF_WG) void k_attr_scatter(const float* pos, const int32_t* pid,
+                                                        const float* attr, int n, int nx, int ny,
+                                                        int nz, float dx,
+                                                        const uint32_t* scratch, int64_t* acc) {
+    constexpr int CH = NC + 1;
+    constexpr int CAP = ATTR_LDS_BYTES / (8 * CH);  // box cells
+    __shared__ unsigned long long box[CAP * CH];
+    __shared__ int red[6][GF_WG / 64];
+    const int p = blockIdx.x * GF_WG + threadIdx.x;
+    const double sv = ldexp(1.0, attr_exp(scratch)), sw = ldexp(1.0, 48);
+    const long long cells = (long long)nx * ny * nz;
+    bool ok = false;
+    int i0 = 0, j0 = 0, k0 = 0;
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    float v[NC];
+    for (int c = 0; c < NC; ++c) v[c] = 0.0f;
+    if (p < n) {
+        float fx, fy, fz;
+        attr_cell_coords(pos, p, dx, fx, fy, fz);
+        // No corner is in range outside (-2, n+1) per axis; this also drops
+        // NaN positions before the float->int conversion.
+        ok = fx > -2.0f && fx < (float)(nx + 1) && fy > -2.0f && fy < (float)(ny + 1) &&
+             fz > -2.0f && fz < (float)(nz + 1);
+        if (ok) {
+            i0 = (int)floorf(fx); j0 = (int)floorf(fy); k0 = (int)floorf(fz);
+            sx = fx - (float)i0; sy = fy - (float)j0; sz = fz - (float)k0;
+            const int id = pid[p];
+            for (int c = 0; c < NC; ++c) v[c] = attr[(size_t)id * NC + c];
+        }
+    }
+    // bounding box of the group's base cells
+    int b[6] = {ok ? i0 : INT_MAX, ok ? j0 : INT_MAX, ok ? k0 : INT_MAX,
+                ok ? -i0 : INT_MAX, ok ? -j0 : INT_MAX, ok ? -k0 : INT_MAX};
+    for (int q = 0; q < 6; ++q)
+        for (int off = 32; off > 0; off >>= 1) b[q] = min(b[q], __shfl_xor(b[q], off, 64));
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int q = 0; q < 6; ++q) red[q][wave] = b[q];
+    __syncthreads();
+    for (int q = 0; q < 6; ++q) {
+        int m = red[q][0];
+        for (int w = 1; w < GF_WG / 64; ++w) m = min(m, red[q][w]);
+        b[q] = m;
+    }
+    if (b[0] == INT_MAX) return;  // no particle of the group is near the grid
+    const int lo_i = b[0], lo_j = b[1], lo_k = b[2];
+    const int by = -b[4] - lo_j + 2, bz = -b[5] - lo_k + 2;
+    const long long nbox = (long long)(-b[3] - lo_i + 2) * by * bz;
+    const bool in_lds = nbox <= CAP;  // uniform over the group
+    const int nb = in_lds ? (int)nbox : 0;
+    if (in_lds) {
+        for (int e = threadIdx.x; e < nb * CH; e += GF_WG) box[e] = 0;
+        __syncthreads();
+    }
+    if (ok) {
+        for (int di = 0; di < 2; ++di)
+            for (int dj = 0; dj < 2; ++dj)
+                for (int dk = 0; dk < 2; ++dk) {
+                    const int ii = i0 + di, jj = j0 + dj, kk = k0 + dk;
+                    if (ii < 0 || ii >= nx || jj < 0 || jj >= ny || kk < 0 || kk >= nz) continue;
+                    const float wx = di ? sx : 1.0f - sx;
+                    const float wy = dj ? sy : 1.0f - sy;
+                    const float wz = dk ? sz : 1.0f - sz;
+                    const float w = wx * wy * wz;
+                    unsigned long long q[CH];
+                    for (int c = 0; c < NC; ++c)
+                        q[c] = (unsigned long long)llrint((double)(v[c] * w) * sv);
+                    q[NC] = (unsigned long long)llrint((double)w * sw);
+                    if (in_lds) {
+                        const int e = ((ii - lo_i) * by + (jj - lo_j)) * bz + (kk - lo_k);
+                        for (int c = 0; c < CH; ++c) atomicAdd(&box[c * nb + e], q[c]);
+                    } else {
+                        const long long cell = ((long long)ii * ny + jj) * nz + kk;
+                        for (int c = 0; c < CH; ++c)
+                            atomicAdd((unsigned long long*)(acc + c * cells + cell), q[c]);
+                    }
+                }
+    }
+    if (!in_lds) return;
+    __syncthreads();
+    // A cell with no weight has no value either: |value*w| * 2^(48-E) is
+    // below w * 2^48, so both round to 0 together.
+    const int byz = by * bz;
+    for (int e = threadIdx.x; e < nb; e += GF_WG) {
+        if (box[NC * nb + e] == 0) continue;
+        const int ei = e / byz, r = e - ei * byz, ej = r / bz, ek = r - ej * bz;
+        const long long cell = ((long long)(lo_i + ei) * ny + (lo_j + ej)) * nz + (lo_k + ek);
+        for (int c = 0; c < CH; ++c) {
+            const unsigned long long x = box[c * nb + e];
+            if (x) atomicAdd((unsigned long long*)(acc + c * cells + cell), x);
+        }
+    }
+}
+
+// grid[c] = normalised channel c, grid[nc] = weight sum (FP32).  mode 0: the
+// colour rule value * (1/w); mode 1: the scalar rule value / w; 0 where
+// w <= 1e-8.
+__global__ void k_attr_normalize(const int64_t* acc, int nc, long long cells,
+                                 const uint32_t* scratch, int mode, float* grid) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cells) return;
+    const double iv = ldexp(1.0, -attr_exp(scratch)), iw = ldexp(1.0, -48);
+    const float w = (float)((double)acc[nc * cells + i] * iw);
+    const float inv = 1.0f / w;
+    for (int c = 0; c < nc; ++c) {
+        const float v = (float)((double)acc[c * cells + i] * iv);
+        grid[c * cells + i] = w > 1.0e-8f ? (mode == 0 ? v * inv : v / w) : 0.0f;
+    }
+    grid[nc * cells + i] = w;
+}
+
+__global__ void k_attr_gather(const float* pos, const int32_t* pid, float* attr, int nc, int n,
+                              int nx, int ny, int nz, float dx, const float* grid) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const long long cells = (long long)nx * ny * nz;
+    float fx, fy, fz;
+    attr_cell_coords(pos, p, dx, fx, fy, fz);
+    const float w = sample3(grid + nc * cells, fx, fy, fz, nx, ny, nz);
+    if (!(w > 1.0e-6f)) return;
+    const int id = pid[p];
+    for (int c = 0; c < nc; ++c)
+        attr[(size_t)id * nc + c] = sample3(grid + c * cells, fx, fy, fz, nx, ny, nz);
+}
+
 }  // namespace
 
 #define GRID_ARGS_OK(who)                                                         \
@@ -648,3 +833,35 @@ extern "C" int gf_k3_vmax_sq_reduce(const float* vel, int n, float* out, void* s
     GF_LAUNCH_CHECK("k_vmax_sq");
     return 0;
 }
+
+extern "C" int gf_scene_attr_transfer(const gf_scene_t* s, float* attr, int channels, int mode,
+                                      int64_t* acc, float* grid, uint32_t* scratch,
+                                      void* stream) {
+    GF_REQUIRE(s, "gf_scene_attr_transfer: null scene");
+    GF_REQUIRE(channels >= 1 && channels <= 3, "gf_scene_attr_transfer: channels %d", channels);
+    GF_REQUIRE(mode == GF_ATTR_COLOR || mode == GF_ATTR_SCALAR, "gf_scene_attr_transfer: mode %d",
+               mode);
+    GF_REQUIRE(s->nx > 0 && s->ny > 0 && s->nz > 0 && s->dx > 0.0f,
+               "gf_scene_attr_transfer: grid");
+    if (s->n == 0) return 0;
+    GF_REQUIRE(attr && acc && grid && scratch && s->pos && s->pid,
+               "gf_scene_attr_transfer: null buffer");
+    hipStream_t st = (hipStream_t)stream;
+    const long long cells = (long long)s->nx * s->ny * s->nz;
+    const long long nacc = (long long)(channels + 1) * cells;
+    hipLaunchKernelGGL(k_attr_clear, grid_for(nacc), dim3(GF_WG), 0, st, acc, nacc, scratch);
+    const long long nv = (long long)s->n * channels;
+    const unsigned gb = (unsigned)std::min<long long>((nv + GF_WG - 1) / GF_WG, 2048);
+    hipLaunchKernelGGL(k_attr_absmax, dim3(gb), dim3(GF_WG), 0, st, attr, nv, scratch);
+    auto scatter = channels == 1 ? k_attr_scatter<1>
+                   : channels == 2 ? k_attr_scatter<2> : k_attr_scatter<3>;
+    hipLaunchKernelGGL(scatter, grid_for(s->n), dim3(GF_WG), 0, st, s->pos, s->pid, attr, s->n,
+                       s->nx, s->ny, s->nz, s->dx, scratch, acc);
+    hipLaunchKernelGGL(k_attr_normalize, grid_for(cells), dim3(GF_WG), 0, st, acc, channels, cells,
+                       scratch, mode, grid);
+    hipLaunchKernelGGL(k_attr_gather, grid_for(s->n), dim3(GF_WG), 0, st, s->pos, s->pid, attr,
+                       channels, s->n, s->nx, s->ny, s->nz, s->dx, grid);
+    GF_LAUNCH_CHECK("attribute transfer");
+    return 0;
+}
+

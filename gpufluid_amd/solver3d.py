@@ -165,6 +165,9 @@ class FlipSolver3D:
         self._pcg_done = DeviceArray(torch.zeros(1, dtype=torch.int32, device=dev))
         self._pcg_tol_sq = DeviceArray(torch.zeros(1, dtype=torch.float32, device=dev))
         self._static_obstacle_sdfs: list = []
+        # per-particle attributes in particle-id (caller) order, None = absent
+        self._attr = {"color": None, "temperature": None}
+        self._attr_ws = None
 
     # ------------------------------------------------------------ attributes
     def _arr(self, name):
@@ -254,13 +257,43 @@ class FlipSolver3D:
         t = self._unsorted("affine", self._affine, 9)
         return None if t is None else DeviceArray(t.view(-1, 3, 3), vec=2)
 
+    # ------------------------------------------------- per-particle attributes
+    def _get_attr(self, key):
+        t = self._attr[key]
+        return None if t is None else DeviceArray(t, vec=1 if t.dim() == 2 else 0)
+
+    def _set_attr(self, key, value, n=None):
+        if value is None:
+            self._attr[key] = None
+            return
+        n = self.n_particles if n is None else n
+        t = as_tensor(value, torch.float32, self.device)
+        shape = (n, 3) if key == "color" else (n,)
+        if t.numel() != math.prod(shape):
+            raise ValueError(f"attr_{key} needs shape {shape}, got {tuple(t.shape)}")
+        t = t.reshape(shape)
+        self._attr[key] = t.clone().contiguous()
+
+    attr_color = property(lambda self: self._get_attr("color"),
+                          lambda self, v: self._set_attr("color", v),
+                          doc="Per-particle linear RGB (n,3), advected by the S2.15 transfer "
+                              "at the end of each substep; None when absent.")
+    attr_temperature = property(lambda self: self._get_attr("temperature"),
+                                lambda self, v: self._set_attr("temperature", v),
+                                doc="Per-particle scalar (n,), advected by the S2.18 transfer "
+                                    "at the end of each substep; None when absent.")
+
     # ------------------------------------------------------------- particles
-    def set_particles(self, pos, vel=None, affine=None) -> None:
+    def set_particles(self, pos, vel=None, affine=None, *, color=None, temperature=None) -> None:
         """Replace the particle set (positions (n,3), velocities (n,3) and,
-        for APIC, affine matrices (n,3,3); missing ones are zero)."""
+        for APIC, affine matrices (n,3,3); missing ones are zero) together
+        with its per-particle attributes (colour (n,3), temperature (n,);
+        a missing one is absent afterwards)."""
         dev = self.device
         p = as_tensor(pos, torch.float32, dev).reshape(-1, 3)
         n = p.shape[0]
+        self._set_attr("color", color, n)
+        self._set_attr("temperature", temperature, n)
         sc = self._sc
         apic = self.transfer_mode == "apic" or affine is not None
         sc.ensure_particles(n, apic)
@@ -283,13 +316,44 @@ class FlipSolver3D:
 
     @block("D4.5.1", "Box seeder (uniform jittered, ppc particles per cell)")
     def seed_box(self, lo, hi, ppc: int = 8, color=None, temperature=None):
-        """Seed [lo, hi] as FlipSolver3D.seed_box (solver3d.py:2826-2910);
-        a second call replaces the particle set (per-particle colour and
-        temperature attributes are not part of this build)."""
-        if color is not None or temperature is not None:
-            raise NotImplementedError("per-particle colour/temperature (S2.15/S2.18) are not "
-                                      "part of the MI355X substep build")
-        self.set_particles(seed_box_positions(lo, hi, ppc, self.dx))
+        """Seed [lo, hi] as FlipSolver3D.seed_box (solver3d.py:2827-2910).
+
+        Append-or-replace as there: when this call or the particles already
+        present carry a colour or temperature, the new particles are appended
+        (previous particles keep position and velocity; every affine matrix
+        is reset to zero); otherwise the particle set is replaced.  New
+        particles take ``color`` (an RGB triple) / ``temperature``; without
+        one they are padded with white / 0.  Earlier particles without the
+        attribute are padded the same way (the reference leaves that mix
+        unsupported)."""
+        new = seed_box_positions(lo, hi, ppc, self.dx)
+        n_prev = self.n_particles
+        have_attrs = (color is not None or temperature is not None
+                      or self._attr["color"] is not None or self._attr["temperature"] is not None)
+        if not (have_attrs and n_prev > 0):
+            n_prev = 0
+        if n_prev:
+            pos = np.concatenate([self.pos.numpy(), new], axis=0)
+            vel = np.concatenate([self.vel.numpy(), np.zeros_like(new)], axis=0)
+        else:
+            pos, vel = new, np.zeros_like(new)
+        n_new = len(new)
+
+        def merged(key, value, pad, width):
+            prev = self._attr[key]
+            if value is None and prev is None:
+                return None
+            fill = np.float32(pad) if value is None else np.asarray(value, np.float32)
+            add = np.broadcast_to(fill, (n_new,) + width).astype(np.float32)
+            if not n_prev:
+                return add
+            old = (prev.cpu().numpy() if prev is not None
+                   else np.full((n_prev,) + width, pad, np.float32))
+            return np.concatenate([old, add], axis=0)
+
+        col = merged("color", color, 1.0, (3,))
+        temp = merged("temperature", temperature, 0.0, ())
+        self.set_particles(pos, vel, color=col, temperature=temp)
 
     def get_particles(self):
         if self._pos is None:
@@ -302,7 +366,7 @@ class FlipSolver3D:
         """Write the solver state in the reference's .npz layout
         (solver3d.py:4127-4158): particles in caller order, the marker grid
         (walls and obstacles included), the grid and physics parameters, and
-        empty per-particle colour/temperature arrays (no attributes here)."""
+        the per-particle colour/temperature (empty arrays when absent)."""
         from pathlib import Path
         Path(path).parent.mkdir(parents=True, exist_ok=True)
         if self.n_particles:
@@ -316,8 +380,11 @@ class FlipSolver3D:
             str(path),
             pos=np.asarray(pos, np.float32), vel=np.asarray(vel, np.float32),
             affine_C=np.asarray(affine, np.float32), marker=self.marker.numpy().astype(np.int32),
-            attr_color=np.zeros((0, 3), dtype=np.float32),
-            attr_temperature=np.zeros((0,), dtype=np.float32),
+            attr_color=(self.attr_color.numpy() if self._attr["color"] is not None
+                        else np.zeros((0, 3), dtype=np.float32)),
+            attr_temperature=(self.attr_temperature.numpy()
+                              if self._attr["temperature"] is not None
+                              else np.zeros((0,), dtype=np.float32)),
             nx=self.nx, ny=self.ny, nz=self.nz, dx=self.dx,
             gravity=self.gravity, flip_blend=self.flip_blend, rho=self.rho,
             viscosity=self.viscosity, transfer_mode=str(self.transfer_mode),
@@ -329,9 +396,9 @@ class FlipSolver3D:
         """Restore a checkpoint written by either implementation
         (solver3d.py:4160-4222): the grid shape and dx must match (else
         RuntimeError); drift of gravity/rho/viscosity/flip_blend/
-        surface_tension is logged and the live values are kept.  Non-empty
-        colour/temperature arrays raise NotImplementedError (no per-particle
-        attributes in this build)."""
+        surface_tension is logged and the live values are kept.  Empty
+        colour/temperature arrays mean the attribute was absent; a length
+        that does not match the particle count raises RuntimeError."""
         import logging
         data = np.load(str(path), allow_pickle=False)
         nx, ny, nz = int(data["nx"]), int(data["ny"]), int(data["nz"])
@@ -352,13 +419,19 @@ class FlipSolver3D:
                     logging.getLogger("gpufluid.checkpoint").warning(
                         "resume: %s drift saved=%s live=%s — running with LIVE value "
                         "(resume is no longer bit-exact).", key, saved, live)
-        for key in ("attr_color", "attr_temperature"):
-            if key in data.files and len(data[key]):
-                raise NotImplementedError(f"checkpoint carries {key}; per-particle colour/"
-                                          "temperature (S2.15/S2.18) are not part of this build")
         pos, vel, affine = data["pos"], data["vel"], data["affine_C"]
+        attrs = {}
+        for key in ("attr_color", "attr_temperature"):
+            a = data[key] if key in data.files else None
+            if a is not None and len(a) == 0:
+                a = None
+            if a is not None and len(a) != len(pos):
+                raise RuntimeError(f"checkpoint {key} holds {len(a)} entries for {len(pos)} "
+                                   "particles")
+            attrs[key] = a
         marker = data["marker"].astype(np.int32)
-        self.set_particles(pos, vel, affine if len(affine) else None)
+        self.set_particles(pos, vel, affine if len(affine) else None,
+                           color=attrs["attr_color"], temperature=attrs["attr_temperature"])
         self._marker_host = marker.copy()
         self._f["marker"].copy_(torch.as_tensor(marker, device=self.device))
         self._cache.clear()
@@ -390,6 +463,23 @@ class FlipSolver3D:
 
     def _descriptor(self):
         return self._sc.descriptor(self._f, self.n_particles, self.transfer_mode == "apic")
+
+    def _attr_transfer(self, s, attr, channels, mode, stream):
+        """S2.15 / S2.18: scatter, normalise and gather one attribute at the
+        advected positions (solver3d.py:3304-3359)."""
+        if attr.shape[0] != self.n_particles:
+            raise RuntimeError(f"attribute holds {attr.shape[0]} particles, the solver "
+                               f"{self.n_particles}")
+        if self._attr_ws is None:
+            cells = self.nx * self.ny * self.nz
+            dev = self.device
+            self._attr_ws = (torch.empty(4 * cells, dtype=torch.int64, device=dev),
+                             torch.empty(4 * cells, dtype=torch.float32, device=dev),
+                             torch.empty(4, dtype=torch.int32, device=dev))
+        acc, grid, scratch = self._attr_ws
+        _lib.call("S2.15" if mode == _lib.GF_ATTR_COLOR else "S2.18", "gf_scene_attr_transfer",
+                  ctypes.byref(s), _lib.ptr(attr), channels, mode, _lib.ptr(acc),
+                  _lib.ptr(grid), _lib.ptr(scratch), stream)
 
     @block("F3.3", "Per-step pipeline: sort, fused P2G, pressure, fused G2P")
     def step(self, dt: float, pressure_iters: int = 80, pressure_solver: str = "jacobi",
@@ -474,7 +564,12 @@ class FlipSolver3D:
             _lib.call("S2.8", "gf_scene_g2p", ctypes.byref(s), _lib.ptr(sc.pc), mode,
                       self.gravity, dt, self.rho, flip_blend, stream)
         prof.fused("grad_subtract_bc")
-        prof.fused("color")
+        if n > 0 and self._attr["color"] is not None:
+            with prof.section("color"):
+                self._attr_transfer(s, self._attr["color"], 3, _lib.GF_ATTR_COLOR, stream)
+        if n > 0 and self._attr["temperature"] is not None:
+            with prof.section("temperature"):
+                self._attr_transfer(s, self._attr["temperature"], 1, _lib.GF_ATTR_SCALAR, stream)
         if hook is not None:
             hook("done")
 

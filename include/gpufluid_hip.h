@@ -77,6 +77,10 @@ int gf_tune_get(const char* key, int* value);
 #define GF_TRANSFER_FLIP 0   /* FLIP/PIC blend by flip_blend, solver3d.py:1414-1463 */
 #define GF_TRANSFER_APIC 1   /* APIC, solver3d.py:1475-1732 */
 
+/* Normalisation rule of gf_scene_attr_transfer. */
+#define GF_ATTR_COLOR 0      /* value * (1/w), solver3d.py:607-631 */
+#define GF_ATTR_SCALAR 1     /* value / w, solver3d.py:710-727 */
+
 /* Pressure precision of gf_scene_pcg_sync. */
 #define GF_PCG_MIXED 0       /* FP32 vectors, FP64 dots, FP64 residual replacement */
 #define GF_PCG_FP64 1        /* plain FP64 PCG */
@@ -229,6 +233,24 @@ int gf_scene_pcg_stats(const gf_scene_t* s, int* iters, double* rel_residual, in
  * fluid cell) and the sorted particle arrays in place. */
 int gf_scene_g2p(const gf_scene_t* s, const float* pressure, int transfer_mode,
                  float gravity, float dt, float rho, float flip_blend, void* stream);
+
+/* Per-particle attribute transfer at the end of a substep (S2.15 colour,
+ * solver3d.py:3333-3359 with k3_p2g_color / k3_normalize_color /
+ * k3_g2p_color 562-664; S2.18 scalar, 3304-3330 with k3_p2g_scalar /
+ * k3_normalize_scalar / k3_g2p_scalar 670-755): trilinear scatter of
+ * `channels` (1..3) values per particle onto the cell-centred grid at the
+ * current (sorted) positions, normalisation by the deposited weight (mode
+ * GF_ATTR_COLOR or GF_ATTR_SCALAR), and trilinear gather back where the
+ * gathered weight exceeds 1e-6.  `attr` is (n, channels) FP32 in particle-id
+ * order (slot p of the scene holds particle pid[p]) and is updated in place.
+ * Workspace: acc (channels+1)*C int64 fixed-point sums, grid (channels+1)*C
+ * FP32 (normalised channels, then the weight), scratch 4 uint32; C = the
+ * cell count.  Sums are formed in fixed point with integer atomics, so the
+ * result is independent of the order of arrival (the reference's FP32
+ * atomics are not); exact while each 2x2x2 cell block holds fewer than
+ * 32768 particles. */
+int gf_scene_attr_transfer(const gf_scene_t* s, float* attr, int channels, int mode,
+                           int64_t* acc, float* grid, uint32_t* scratch, void* stream);
 
 /* Scatter a compact cell vector (e.g. s->pc, s->b) into a dense (nx,ny,nz)
  * array; cells of inactive tiles get 0. No reference counterpart. */

@@ -22,6 +22,9 @@ How each part is pinned (tests/test_oracle_*.py):
   * sample3 / G2P: trilinear interpolation reproduces affine fields exactly;
     PIC == FLIP with flip_blend 0; clamp box;
   * APIC: affine velocity field is transferred exactly (P2G->G2P round trip).
+  * attribute transfer: hand-derived closed form on a two-particle line,
+    identity for one particle at each cell centre, constants preserved,
+    weight sum = particle count, pure-Python per-corner loop.
 Functions without such a check say "parity unpinned" in their docstring.
 """
 from __future__ import annotations
@@ -537,11 +540,127 @@ def g2p_apic_advect(pos, u, v, w, dx, dt, dom):
 
 
 # -------------------------------------------------------------------- substep
+# ------------------------------------------- per-particle attribute transfer
+ATTR_COLOR, ATTR_SCALAR = 0, 1
+
+
+def attr_p2g(pos, attr, dx, shape):
+    """Trilinear scatter of per-particle values onto the cell-centred grid
+    (k3_p2g_color solver3d.py:562-605, k3_p2g_scalar 670-707): fx = p/dx-0.5,
+    corners in range only, w = (wx*wy)*wz and value*w in FP32.  The cell sums
+    are formed in FP64 from those FP32 products (the order-free sum; the
+    reference's FP32 atomics give it up to rounding in the order of arrival).
+    attr (n,) or (n,c).  Returns (vsum (c,nx,ny,nz), wsum (nx,ny,nz)), FP64."""
+    nx, ny, nz = shape
+    a = np.asarray(attr, dtype=np.float32)
+    a2 = a.reshape(len(a), -1)
+    q = np.asarray(pos, dtype=np.float32) / np.float32(dx) - np.float32(0.5)
+    i0 = np.floor(q).astype(np.int64)
+    s = (q - i0.astype(np.float32)).astype(np.float32)
+    one = np.float32(1)
+    vsum = np.zeros((a2.shape[1], nx, ny, nz))
+    wsum = np.zeros((nx, ny, nz))
+    for di in (0, 1):
+        for dj in (0, 1):
+            for dk in (0, 1):
+                ii, jj, kk = i0[:, 0] + di, i0[:, 1] + dj, i0[:, 2] + dk
+                ok = (ii >= 0) & (ii < nx) & (jj >= 0) & (jj < ny) & (kk >= 0) & (kk < nz)
+                wx = s[:, 0] if di else one - s[:, 0]
+                wy = s[:, 1] if dj else one - s[:, 1]
+                wz = s[:, 2] if dk else one - s[:, 2]
+                w = ((wx * wy).astype(np.float32) * wz).astype(np.float32)
+                idx = (ii[ok], jj[ok], kk[ok])
+                np.add.at(wsum, idx, w[ok].astype(np.float64))
+                for c in range(a2.shape[1]):
+                    np.add.at(vsum[c], idx, (a2[ok, c] * w[ok]).astype(np.float32))
+    return vsum, wsum
+
+
+def attr_normalize(vsum, wsum, mode):
+    """k3_normalize_color (solver3d.py:607-631: value * (1/w)) or
+    k3_normalize_scalar (710-727: value / w), FP32, 0 where w <= 1e-8.
+    Returns (grid (c,nx,ny,nz) FP32, w FP32)."""
+    w = np.asarray(wsum).astype(np.float32)
+    v = np.asarray(vsum).astype(np.float32)
+    ok = w > np.float32(1e-8)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        if mode == ATTR_COLOR:
+            g = v * (np.float32(1) / w)
+        else:
+            g = v / w
+    return np.where(ok, g, np.float32(0)).astype(np.float32), w
+
+
+def attr_g2p(pos, attr, grid, wgrid, dx):
+    """k3_g2p_color (solver3d.py:633-664) / k3_g2p_scalar (729-755): FP32
+    sample3 of each channel and of the weight at p/dx-0.5; a particle is
+    updated only where the sampled weight exceeds 1e-6."""
+    a = np.array(attr, dtype=np.float32)
+    a2 = a.reshape(len(a), -1)
+    q = np.asarray(pos, dtype=np.float32) / np.float32(dx) - np.float32(0.5)
+    fx, fy, fz = q[:, 0], q[:, 1], q[:, 2]
+    w = sample3(wgrid, fx, fy, fz)
+    upd = w > np.float32(1e-6)
+    for c in range(a2.shape[1]):
+        a2[upd, c] = sample3(grid[c], fx, fy, fz)[upd]
+    return a2.reshape(a.shape)
+
+
+def attr_transfer(pos, attr, dx, shape, mode):
+    """P2G -> normalize -> G2P of one attribute (_apply_color_transfer
+    solver3d.py:3333-3359, _apply_scalar_transfer 3304-3330)."""
+    vsum, wsum = attr_p2g(pos, attr, dx, shape)
+    grid, w = attr_normalize(vsum, wsum, mode)
+    return attr_g2p(pos, attr, grid, w, dx)
+
+
+def attr_transfer_loop(pos, attr, dx, shape, mode):
+    """Per-particle Python loop of the same transfer for tiny cases, written
+    from the scalar kernels with explicit per-corner bounds checks."""
+    nx, ny, nz = shape
+    a = np.array(attr, dtype=np.float64).reshape(len(attr), -1)
+    nc = a.shape[1]
+    acc = {}
+    for p in range(len(pos)):
+        f = [float(np.float32(pos[p][d]) / np.float32(dx)) - 0.5 for d in range(3)]
+        b = [int(np.floor(x)) for x in f]
+        t = [f[d] - b[d] for d in range(3)]
+        for corner in range(8):
+            o = ((corner >> 2) & 1, (corner >> 1) & 1, corner & 1)
+            c = tuple(b[d] + o[d] for d in range(3))
+            if not all(0 <= c[d] < n for d, n in enumerate((nx, ny, nz))):
+                continue
+            w = 1.0
+            for d in range(3):
+                w *= t[d] if o[d] else 1.0 - t[d]
+            e = acc.setdefault(c, [0.0] * (nc + 1))
+            for ch in range(nc):
+                e[ch] += a[p, ch] * w
+            e[nc] += w
+    grid = np.zeros((nc, nx, ny, nz))
+    wg = np.zeros((nx, ny, nz))
+    for c, e in acc.items():
+        wg[c] = e[nc]
+        if e[nc] > 1e-8:
+            for ch in range(nc):
+                grid[(ch,) + c] = e[ch] / e[nc]
+    out = a.copy()
+    for p in range(len(pos)):
+        f = [float(np.float32(pos[p][d]) / np.float32(dx)) - 0.5 for d in range(3)]
+        wv = float(sample3(wg, np.array([f[0]]), np.array([f[1]]), np.array([f[2]]))[0])
+        if wv > 1e-6:
+            for ch in range(nc):
+                out[p, ch] = float(sample3(grid[ch], np.array([f[0]]), np.array([f[1]]),
+                                           np.array([f[2]]))[0])
+    return out.reshape(np.shape(attr))
+
+
 def substep(state: dict, dt: float, pressure="pcg", iters=80, tol=1e-4, dtype=np.float64,
             gravity=-9.81, flip_blend=0.95, rho=1.0, transfer="flip", pcg_sign="physical"):
     """One substep in the stage order of _step_impl (solver3d.py:3475-3667):
     clear → P2G → normalize → gravity → BC → divergence → pressure → grad → BC
-    → G2P + advect.  ``state`` holds pos, vel, marker, dx and (APIC) affine.
+    → G2P + advect → colour / temperature transfer.  ``state`` holds pos,
+    vel, marker, dx, (APIC) affine and optionally color (n,3) / temperature (n,).
     pressure: "jacobi" / "gsrb" (fixed iterations), "pcg" (reference restatement;
     pcg_sign "physical" negates its result, "reference" keeps it) or
     "exact" (spsolve).  Returns a new state plus the grid fields."""
@@ -582,6 +701,11 @@ def substep(state: dict, dt: float, pressure="pcg", iters=80, tol=1e-4, dtype=np
     else:
         npos, nvel = g2p_advect(pos, vel, u2, v2, w2, us, vs, ws, dx, dt, flip_blend, dom)
     out["pos"], out["vel"] = npos, nvel
+    p32 = np.asarray(npos, dtype=np.float32)
+    if state.get("color") is not None:
+        out["color"] = attr_transfer(p32, state["color"], dx, m.shape, ATTR_COLOR)
+    if state.get("temperature") is not None:
+        out["temperature"] = attr_transfer(p32, state["temperature"], dx, m.shape, ATTR_SCALAR)
     grid = dict(u=u2, v=v2, w=w2, us=us, vs=vs, ws=ws, div=div, p=p, marker=m, info=info)
     return out, grid
 
