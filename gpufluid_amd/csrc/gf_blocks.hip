@@ -438,6 +438,34 @@ __global__ __launch_bounds__(GF_WG) void k_vmax_sq(const float* vel, int n, floa
     }
 }
 
+// k3_jacobi_visc (solver3d.py:178-197): one Jacobi sweep of
+// (I - r Lap) u_new = u_old on one MAC component of shape (sx, sy, sz).
+// Neighbours in range only, summed in the order -x, +x, -y, +y, -z, +z;
+// u_out = (u_old + r*sum) / (1 + r*count) with every operation rounded on
+// its own.  Threads: 64 along k x 4 along j per workgroup, one i per grid z.
+__global__ __launch_bounds__(GF_WG) void k_jacobi_visc(const float* __restrict__ u_old,
+                                                       const float* __restrict__ u_in,
+                                                       float* __restrict__ u_out, int sx,
+                                                       int sy, int sz, float r) {
+#pragma clang fp contract(off)
+    const int k = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int i = blockIdx.z;
+    if (k >= sz || j >= sy) return;
+    const size_t di = (size_t)sy * sz, dj = (size_t)sz;
+    const size_t c = (size_t)i * di + (size_t)j * dj + k;
+    float sum = 0.0f, cnt = 0.0f;
+    if (i > 0) { sum += u_in[c - di]; cnt += 1.0f; }
+    if (i < sx - 1) { sum += u_in[c + di]; cnt += 1.0f; }
+    if (j > 0) { sum += u_in[c - dj]; cnt += 1.0f; }
+    if (j < sy - 1) { sum += u_in[c + dj]; cnt += 1.0f; }
+    if (k > 0) { sum += u_in[c - 1]; cnt += 1.0f; }
+    if (k < sz - 1) { sum += u_in[c + 1]; cnt += 1.0f; }
+    const float num = u_old[c] + r * sum;
+    const float den = 1.0f + r * cnt;
+    u_out[c] = num / den;
+}
+
 // ---- per-particle attribute transfer (S2.15 colour, S2.18 scalar) --------
 // P2G of `nc` channels per particle onto the cell-centred grid at the current
 // positions (solver3d.py:562-605, 670-707), normalise (607-631, 710-727) and
@@ -715,6 +743,20 @@ extern "C" int gf_k3_gauss_seidel_rb(float* p, const float* div, const int32_t* 
     hipLaunchKernelGGL(k_gsrb_dense, grid_for((long long)nx * ny * nz), dim3(GF_WG), 0,
                        (hipStream_t)stream, p, div, marker, nx, ny, nz, color);
     GF_LAUNCH_CHECK("k_gsrb_dense");
+    return 0;
+}
+
+extern "C" int gf_k3_jacobi_visc(const float* u_old, const float* u_in, float* u_out, int sx,
+                                 int sy, int sz, float r, void* stream) {
+    GF_REQUIRE(sx > 0 && sy > 0 && sz > 0, "gf_k3_jacobi_visc: shape %d x %d x %d", sx, sy, sz);
+    GF_REQUIRE(sy <= 4 * 65535 && sx <= 65535, "gf_k3_jacobi_visc: shape %d x %d x %d", sx, sy,
+               sz);
+    GF_REQUIRE(u_old && u_in && u_out && u_in != u_out && u_old != u_out,
+               "gf_k3_jacobi_visc: bad buffers");
+    const dim3 grid((unsigned)((sz + 63) / 64), (unsigned)((sy + 3) / 4), (unsigned)sx);
+    hipLaunchKernelGGL(k_jacobi_visc, grid, dim3(GF_WG), 0, (hipStream_t)stream, u_old, u_in,
+                       u_out, sx, sy, sz, r);
+    GF_LAUNCH_CHECK("k_jacobi_visc");
     return 0;
 }
 

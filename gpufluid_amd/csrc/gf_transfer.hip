@@ -465,6 +465,8 @@ struct G2PParams {
     const float* us; const float* vs; const float* ws;
     const float* su; const float* sv; const float* sw;
     const float* pc;
+    // Pre-projection faces; NULL: rebuilt from us/vs/ws as P2G produced them.
+    const float* pu; const float* pv; const float* pw;
     const int32_t* tile_scene; const float* scene_par;  // packed scenes (NULL: one scene)
     int nx, ny, nz, tnx, tny, tnz;
     float dx, dt, rho, flip_blend, gdt;
@@ -634,17 +636,19 @@ __global__ __launch_bounds__(GF_WG) void k_g2p_tile(G2PParams a) {
         float* gfld = c == 0 ? a.u : (c == 1 ? a.v : a.w);
         const float* gsav = c == 0 ? a.us : (c == 1 ? a.vs : a.ws);
         const float* solid = c == 0 ? a.su : (c == 1 ? a.sv : a.sw);
+        const float* gpre = c == 0 ? a.pu : (c == 1 ? a.pv : a.pw);
         for (int idx = threadIdx.x; idx < G2P_REGION; idx += GF_WG) {
             const int lk = idx % DK, lj = (idx / DK) % DJ, li = idx / (DK * DJ);
             const int ii = oi + li, jj = oj + lj, kk = ok + lk;
             float val = 0.0f, sav = 0.0f;
             if (ii >= 0 && ii < fni && jj >= 0 && jj < fnj && kk >= 0 && kk < fnk) {
                 const size_t f = ((size_t)ii * fnj + jj) * fnk + kk;
-                // Pre-projection face value rebuilt from the saved copy exactly
-                // as P2G produced it (us [+ g*dt on v], then the BC below), so
-                // the halo never reads u while owners overwrite it.
+                // Pre-projection face value: rebuilt from the saved copy exactly
+                // as P2G produced it (us [+ g*dt on v], then the BC below), or
+                // read from a separate pre-projection array, so the halo never
+                // reads u while owners overwrite it.
                 sav = gsav[f];
-                val = c == 1 ? sav + gdt : sav;
+                val = gpre ? gpre[f] : (c == 1 ? sav + gdt : sav);
                 // Cell coords of the face's two sides in the staged cell region.
                 const int ca = ii - (i0 - 1), cb = jj - (j0 - 1), cc = kk - (k0 - 1);
                 int la, lb, lc;  // low-side cell
@@ -761,6 +765,84 @@ __global__ __launch_bounds__(GF_WG) void k_unsort(const float* __restrict__ in,
     for (int c = 0; c < width; ++c) out[(size_t)d * width + c] = in[(size_t)i * width + c];
 }
 
+// Divergence (solver3d.py:846-871) of the dense u,v,w into the compact rhs of
+// the active tiles, the same arithmetic as the fused P2G's; for stages that
+// change the velocities between P2G and the pressure solve.  One workgroup
+// per tile, two cells per thread.
+__global__ __launch_bounds__(GF_WG) void k_scene_divergence(
+    const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
+    const int32_t* __restrict__ marker, const int32_t* __restrict__ tile_map,
+    float* __restrict__ b, int nx, int ny, int nz, int tny, int tnz, float dx, float dt,
+    float rho) {
+    const int t = blockIdx.x;
+    const int slot = tile_map[t];
+    if (slot < 0) return;
+    const int tk = t % tnz, tj = (t / tnz) % tny, ti = t / (tnz * tny);
+    const float scale = rho * dx / dt;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int l = threadIdx.x + h * GF_WG;
+        const int i = ti * 8 + (l >> 6), j = tj * 8 + ((l >> 3) & 7), k = tk * 8 + (l & 7);
+        float dv = 0.0f;
+        if (i < nx && j < ny && k < nz) {
+            const size_t c = ((size_t)i * ny + j) * nz + k;
+            if (marker[c] == 1) {
+                const float u0 = u[c], u1 = u[c + (size_t)ny * nz];
+                const size_t cv = ((size_t)i * (ny + 1) + j) * nz + k;
+                const float v0 = v[cv], v1 = v[cv + nz];
+                const size_t cw = ((size_t)i * ny + j) * (nz + 1) + k;
+                const float w0 = w[cw], w1 = w[cw + 1];
+                const float d = u1 - u0 + v1 - v0 + w1 - w0;
+                dv = d * scale;
+            }
+        }
+        b[(size_t)slot * GF_TILE_CELLS + l] = dv;
+    }
+}
+
+G2PParams g2p_params(const gf_scene_t* s, const float* pressure, float dt, float rho,
+                     float flip_blend, float gdt) {
+    G2PParams a;
+    a.pos = s->pos; a.vel = s->vel; a.C = s->affine_c;
+    a.cell_start = s->cell_start; a.marker = s->marker; a.tile_map = s->tile_map;
+    a.u = s->u; a.v = s->v; a.w = s->w; a.us = s->us; a.vs = s->vs; a.ws = s->ws;
+    a.su = s->solid_u; a.sv = s->solid_v; a.sw = s->solid_w;
+    a.pc = pressure;
+    a.pu = nullptr; a.pv = nullptr; a.pw = nullptr;
+    a.nx = s->nx; a.ny = s->ny; a.nz = s->nz; a.tnx = s->tnx; a.tny = s->tny; a.tnz = s->tnz;
+    a.dx = s->dx; a.dt = dt; a.rho = rho; a.flip_blend = flip_blend; a.gdt = gdt;
+    a.domx = s->dom[0]; a.domy = s->dom[1]; a.domz = s->dom[2];
+    const bool packed = s->n_scenes > 1;
+    a.tile_scene = packed ? s->tile_scene : nullptr;
+    a.scene_par = packed ? s->scene_par : nullptr;
+    return a;
+}
+
+int g2p_check(const gf_scene_t* s, const float* pressure, int transfer_mode, float rho,
+              const char* who) {
+    GF_REQUIRE(transfer_mode == GF_TRANSFER_FLIP || transfer_mode == GF_TRANSFER_APIC,
+               "%s: transfer_mode %d", who, transfer_mode);
+    GF_REQUIRE(s->u && s->v && s->w && s->marker && s->cell_start && s->tile_map && pressure,
+               "%s: grid buffers not set", who);
+    GF_REQUIRE(rho != 0.0f, "%s: rho must be nonzero", who);
+    GF_REQUIRE(s->us && s->vs && s->ws, "%s: us/vs/ws not set", who);
+    if (s->n > 0) GF_REQUIRE(s->pos && s->vel, "%s: particles not set", who);
+    if (transfer_mode == GF_TRANSFER_APIC && s->n > 0)
+        GF_REQUIRE(s->affine_c, "%s: APIC needs affine_c", who);
+    return 0;
+}
+
+int g2p_launch(const gf_scene_t* s, const G2PParams& a, int transfer_mode, void* stream) {
+    const int ntiles = s->tnx * s->tny * s->tnz;
+    hipStream_t st = (hipStream_t)stream;
+    if (transfer_mode == GF_TRANSFER_APIC)
+        hipLaunchKernelGGL(k_g2p_tile<true>, dim3(ntiles), dim3(GF_WG), 0, st, a);
+    else
+        hipLaunchKernelGGL(k_g2p_tile<false>, dim3(ntiles), dim3(GF_WG), 0, st, a);
+    GF_LAUNCH_CHECK("k_g2p_tile");
+    return 0;
+}
+
 P2GParams p2g_params(const gf_scene_t* s, float g, float dt, float rho) {
     P2GParams a;
     a.pos = s->pos; a.vel = s->vel; a.C = s->affine_c;
@@ -834,34 +916,38 @@ extern "C" int gf_scene_g2p(const gf_scene_t* s, const float* pressure, int tran
                             float gravity, float dt, float rho, float flip_blend, void* stream) {
     int rc = gf::scene_check(s, "gf_scene_g2p");
     if (rc) return rc;
-    GF_REQUIRE(transfer_mode == GF_TRANSFER_FLIP || transfer_mode == GF_TRANSFER_APIC,
-               "gf_scene_g2p: transfer_mode %d", transfer_mode);
-    GF_REQUIRE(s->u && s->v && s->w && s->marker && s->cell_start && s->tile_map && pressure,
-               "gf_scene_g2p: grid buffers not set");
-    GF_REQUIRE(rho != 0.0f, "gf_scene_g2p: rho must be nonzero");
-    GF_REQUIRE(s->us && s->vs && s->ws, "gf_scene_g2p: us/vs/ws not set");
-    if (s->n > 0) GF_REQUIRE(s->pos && s->vel, "gf_scene_g2p: particles not set");
-    if (transfer_mode == GF_TRANSFER_APIC && s->n > 0)
-        GF_REQUIRE(s->affine_c, "gf_scene_g2p: APIC needs affine_c");
-    G2PParams a;
-    a.pos = s->pos; a.vel = s->vel; a.C = s->affine_c;
-    a.cell_start = s->cell_start; a.marker = s->marker; a.tile_map = s->tile_map;
-    a.u = s->u; a.v = s->v; a.w = s->w; a.us = s->us; a.vs = s->vs; a.ws = s->ws;
-    a.su = s->solid_u; a.sv = s->solid_v; a.sw = s->solid_w;
-    a.pc = pressure;
-    a.nx = s->nx; a.ny = s->ny; a.nz = s->nz; a.tnx = s->tnx; a.tny = s->tny; a.tnz = s->tnz;
-    a.dx = s->dx; a.dt = dt; a.rho = rho; a.flip_blend = flip_blend; a.gdt = gravity * dt;
-    a.domx = s->dom[0]; a.domy = s->dom[1]; a.domz = s->dom[2];
-    const bool packed = s->n_scenes > 1;
-    a.tile_scene = packed ? s->tile_scene : nullptr;
-    a.scene_par = packed ? s->scene_par : nullptr;
+    rc = g2p_check(s, pressure, transfer_mode, rho, "gf_scene_g2p");
+    if (rc) return rc;
+    return g2p_launch(s, g2p_params(s, pressure, dt, rho, flip_blend, gravity * dt),
+                      transfer_mode, stream);
+}
+
+extern "C" int gf_scene_g2p_pre(const gf_scene_t* s, const float* pressure, const float* pre_u,
+                                const float* pre_v, const float* pre_w, int transfer_mode,
+                                float dt, float rho, float flip_blend, void* stream) {
+    int rc = gf::scene_check(s, "gf_scene_g2p_pre");
+    if (rc) return rc;
+    rc = g2p_check(s, pressure, transfer_mode, rho, "gf_scene_g2p_pre");
+    if (rc) return rc;
+    GF_REQUIRE(pre_u && pre_v && pre_w, "gf_scene_g2p_pre: pre-projection faces not set");
+    GF_REQUIRE(pre_u != s->u && pre_v != s->v && pre_w != s->w,
+               "gf_scene_g2p_pre: pre-projection faces must not alias u, v, w");
+    G2PParams a = g2p_params(s, pressure, dt, rho, flip_blend, 0.0f);
+    a.pu = pre_u; a.pv = pre_v; a.pw = pre_w;
+    return g2p_launch(s, a, transfer_mode, stream);
+}
+
+extern "C" int gf_scene_divergence(const gf_scene_t* s, float dt, float rho, void* stream) {
+    int rc = gf::scene_check(s, "gf_scene_divergence");
+    if (rc) return rc;
+    GF_REQUIRE(s->u && s->v && s->w && s->marker && s->tile_map && s->b,
+               "gf_scene_divergence: grid buffers not set");
+    GF_REQUIRE(dt != 0.0f, "gf_scene_divergence: dt must be nonzero");
     const int ntiles = s->tnx * s->tny * s->tnz;
-    hipStream_t st = (hipStream_t)stream;
-    if (transfer_mode == GF_TRANSFER_APIC)
-        hipLaunchKernelGGL(k_g2p_tile<true>, dim3(ntiles), dim3(GF_WG), 0, st, a);
-    else
-        hipLaunchKernelGGL(k_g2p_tile<false>, dim3(ntiles), dim3(GF_WG), 0, st, a);
-    GF_LAUNCH_CHECK("k_g2p_tile");
+    hipLaunchKernelGGL(k_scene_divergence, dim3(ntiles), dim3(GF_WG), 0, (hipStream_t)stream,
+                       s->u, s->v, s->w, s->marker, s->tile_map, s->b, s->nx, s->ny, s->nz,
+                       s->tny, s->tnz, s->dx, dt, rho);
+    GF_LAUNCH_CHECK("k_scene_divergence");
     return 0;
 }
 

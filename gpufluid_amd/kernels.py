@@ -162,6 +162,19 @@ def k3_jacobi_pressure(p_in, p_out, div, marker, off_x=0, off_y=0, off_z=0):
               ny, nz, _st(p_in))
 
 
+@block("S2.13", "Implicit viscosity Jacobi sweep on one MAC component")
+def k3_jacobi_visc(u_old, u_in, u_out, r):
+    """k3_jacobi_visc (solver3d.py:178-197): u_out = (u_old + r*sum_nb(u_in)) /
+    (1 + r*count_nb) over the in-range neighbours of each entry of one
+    component array."""
+    t = _t(u_in)
+    if t.dim() != 3 or _t(u_old).shape != t.shape or _t(u_out).shape != t.shape:
+        raise ValueError("k3_jacobi_visc: u_old, u_in, u_out must share one 3-D shape")
+    sx, sy, sz = (int(x) for x in t.shape)
+    _lib.call("S2.13", "gf_k3_jacobi_visc", _p(u_old), _p(u_in), _p(u_out), sx, sy, sz,
+              float(r), _st(u_in))
+
+
 @block("S2.6.2", "Gauss-Seidel red-black sweep (in-place)")
 def k3_gauss_seidel_rb(p, div, marker, color, off_x=0, off_y=0, off_z=0):
     _offsets("S2.6.2", off_x, off_y, off_z)

@@ -93,8 +93,6 @@ class FlipSolver3D:
                  sub_rebuild_every: int = 8, sub_dilation: int = 4, *,
                  pcg_precision: str = "mixed", pcg_sign: str = "physical",
                  pcg_tol: float = 1e-4, pcg_check_every: int = 8):
-        if viscosity > 0.0:
-            raise NotImplementedError("viscosity (S2.13) is not part of the MI355X substep build")
         if surface_tension > 0.0:
             raise NotImplementedError("surface tension (S2.14) is not part of the MI355X substep build")
         if enable_sub_dense:
@@ -464,6 +462,33 @@ class FlipSolver3D:
     def _descriptor(self):
         return self._sc.descriptor(self._f, self.n_particles, self.transfer_mode == "apic")
 
+    def _viscosity(self, dt):
+        """S2.13 as _step_impl runs it (solver3d.py:3524-3544): the post-BC
+        faces are copied into the weight arrays as the right-hand side, then
+        `viscosity_iters` Jacobi sweeps per component ping-pong between u and
+        us (so us ends holding the last-but-one iterate, which FLIP then
+        samples as the old velocity, as in the reference), then the solid BC.
+        Returns the pre-projection faces for G2P: a copy of the result in the
+        weight arrays."""
+        f = self._f
+        r = float(dt * self.viscosity / (self.dx * self.dx))
+        rhs = self._weights()
+        for t, key in zip(rhs, ("u", "v", "w")):
+            t.copy_(f[key])
+        for _ in range(int(self.viscosity_iters)):
+            for t, key, sav in zip(rhs, ("u", "v", "w"), ("us", "vs", "ws")):
+                a, b = f[key], f[sav]
+                sx, sy, sz = (int(x) for x in a.shape)
+                _lib.call("S2.13", "gf_k3_jacobi_visc", _lib.ptr(t), _lib.ptr(a), _lib.ptr(b),
+                          sx, sy, sz, r, self._stream())
+                f[key], f[sav] = b, a
+        _lib.call("S2.4", "gf_k3_enforce_solid_bc", *(_lib.ptr(f[k]) for k in
+                  ("u", "v", "w", "marker", "solid_u", "solid_v", "solid_w")),
+                  self.nx, self.ny, self.nz, self._stream())
+        for t, key in zip(rhs, ("u", "v", "w")):
+            t.copy_(f[key])
+        return rhs
+
     def _attr_transfer(self, s, attr, channels, mode, stream):
         """S2.15 / S2.18: scatter, normalise and gather one attribute at the
         advected positions (solver3d.py:3304-3359)."""
@@ -529,7 +554,15 @@ class FlipSolver3D:
                       stream)
         prof.fused("normalize")
         prof.fused("gravity_bc")
-        prof.fused("divergence")
+        pre = None
+        if self.viscosity > 0.0:
+            with prof.section("viscosity"):
+                pre = self._viscosity(dt)
+            s = self._descriptor()
+            with prof.section("divergence"):
+                _lib.call("S2.5", "gf_scene_divergence", ctypes.byref(s), dt, self.rho, stream)
+        else:
+            prof.fused("divergence")
         with prof.section("pressure"):
             if pressure_solver == "jacobi":
                 _lib.call("S2.6.1", "gf_scene_jacobi", ctypes.byref(s), na[0],
@@ -561,8 +594,12 @@ class FlipSolver3D:
                 self.last_pressure_replacements = nrep.value
                 self._pcg_tol_sq.t.fill_(float(np.float32(tol) ** 2))
         with prof.section("g2p_advect"):
-            _lib.call("S2.8", "gf_scene_g2p", ctypes.byref(s), _lib.ptr(sc.pc), mode,
-                      self.gravity, dt, self.rho, flip_blend, stream)
+            if pre is None:
+                _lib.call("S2.8", "gf_scene_g2p", ctypes.byref(s), _lib.ptr(sc.pc), mode,
+                          self.gravity, dt, self.rho, flip_blend, stream)
+            else:
+                _lib.call("S2.8", "gf_scene_g2p_pre", ctypes.byref(s), _lib.ptr(sc.pc),
+                          *(_lib.ptr(t) for t in pre), mode, dt, self.rho, flip_blend, stream)
         prof.fused("grad_subtract_bc")
         if n > 0 and self._attr["color"] is not None:
             with prof.section("color"):

@@ -234,6 +234,21 @@ int gf_scene_pcg_stats(const gf_scene_t* s, int* iters, double* rel_residual, in
 int gf_scene_g2p(const gf_scene_t* s, const float* pressure, int transfer_mode,
                  float gravity, float dt, float rho, float flip_blend, void* stream);
 
+/* gf_scene_g2p for pipelines that change the velocities between P2G and the
+ * pressure solve (viscosity S2.13, solver3d.py:3524-3544): the pre-projection
+ * faces are read from pre_u/pre_v/pre_w (distinct from u,v,w) instead of
+ * being rebuilt from us,vs,ws; FLIP's "old" velocity is read from us,vs,ws as
+ * they stand.  Gravity is not added (it is in pre_*). */
+int gf_scene_g2p_pre(const gf_scene_t* s, const float* pressure, const float* pre_u,
+                     const float* pre_v, const float* pre_w, int transfer_mode, float dt,
+                     float rho, float flip_blend, void* stream);
+
+/* k3_compute_divergence (solver3d.py:846-871) of the dense u,v,w into the
+ * compact rhs s->b of the active tiles (0 outside fluid cells), with the
+ * arithmetic of the fused P2G; for stages between P2G and the pressure
+ * solve. */
+int gf_scene_divergence(const gf_scene_t* s, float dt, float rho, void* stream);
+
 /* Per-particle attribute transfer at the end of a substep (S2.15 colour,
  * solver3d.py:3333-3359 with k3_p2g_color / k3_normalize_color /
  * k3_g2p_color 562-664; S2.18 scalar, 3304-3330 with k3_p2g_scalar /
@@ -294,6 +309,11 @@ int gf_k3_jacobi_pressure(const float* p_in, float* p_out, const float* div,
  * the cells with (i+j+k) % 2 == color (0 red, 1 black) */
 int gf_k3_gauss_seidel_rb(float* p, const float* div, const int32_t* marker, int nx, int ny,
                           int nz, int color, void* stream);
+/* k3_jacobi_visc, solver3d.py:178-197: one Jacobi sweep of
+ * (I - r Lap) u_out = u_old on one MAC component of shape (sx, sy, sz), reading
+ * u_in (distinct from u_out); sx <= 65535, sy <= 262140 */
+int gf_k3_jacobi_visc(const float* u_old, const float* u_in, float* u_out, int sx, int sy,
+                      int sz, float r, void* stream);
 /* k3_apply_A, solver3d.py:1743-1797 (done: device int[1], may be NULL) */
 int gf_k3_apply_A(const float* x, float* y, const int32_t* marker, const int32_t* done,
                   int nx, int ny, int nz, void* stream);

@@ -22,6 +22,8 @@ How each part is pinned (tests/test_oracle_*.py):
   * sample3 / G2P: trilinear interpolation reproduces affine fields exactly;
     PIC == FLIP with flip_blend 0; clamp box;
   * APIC: affine velocity field is transferred exactly (P2G->G2P round trip).
+  * viscosity sweep: constant fields fixed, r = 0 identity, fixed point =
+    numpy.linalg.solve of the assembled (I - r L) system, per-entry loop;
   * attribute transfer: hand-derived closed form on a two-particle line,
     identity for one particle at each cell centre, constants preserved,
     weight sum = particle count, pure-Python per-corner loop.
@@ -540,6 +542,45 @@ def g2p_apic_advect(pos, u, v, w, dx, dt, dom):
 
 
 # -------------------------------------------------------------------- substep
+# ------------------------------------------------------------------ viscosity
+def visc_sweep(u_old, u_in, r):
+    """k3_jacobi_visc (solver3d.py:178-197): one Jacobi sweep of
+    (I - r Lap) u = u_old on one component array; in-range neighbours summed
+    in the order -x, +x, -y, +y, -z, +z; (u_old + r*sum) / (1 + r*count) in
+    the arrays' precision, each operation rounded on its own."""
+    t = u_in.dtype.type
+    r = t(np.float32(r))
+    s = np.zeros_like(u_in)
+    cnt = np.zeros_like(u_in)
+    one = t(1)
+    for ax in range(3):
+        for d in (-1, 1):
+            nb = _shift(u_in, ax, d, 0)
+            ok = _shift(np.ones_like(u_in), ax, d, 0)
+            s = (s + nb).astype(u_in.dtype)
+            cnt = (cnt + ok).astype(u_in.dtype)
+    num = (np.asarray(u_old, u_in.dtype) + (r * s).astype(u_in.dtype)).astype(u_in.dtype)
+    den = (one + (r * cnt).astype(u_in.dtype)).astype(u_in.dtype)
+    return (num / den).astype(u_in.dtype)
+
+
+def viscosity_stage(u, v, w, us, vs, ws, marker, r, iters):
+    """The S2.13 stage of _step_impl (solver3d.py:3524-3544): right-hand side
+    = the post-BC faces; each sweep's result becomes the component and the
+    component before it moves into us/vs/ws (the reference swaps the arrays),
+    so us ends as the last-but-one iterate (or unchanged for 0 sweeps); then
+    the solid BC.  Returns (u, v, w, us, vs, ws)."""
+    rhs = (u.copy(), v.copy(), w.copy())
+    cur = [u, v, w]
+    sav = [us, vs, ws]
+    for _ in range(iters):
+        for c in range(3):
+            new = visc_sweep(rhs[c], cur[c], r)
+            sav[c], cur[c] = cur[c], new
+    cu, cv, cw = enforce_solid_bc(cur[0], cur[1], cur[2], marker)
+    return cu, cv, cw, sav[0], sav[1], sav[2]
+
+
 # ------------------------------------------- per-particle attribute transfer
 ATTR_COLOR, ATTR_SCALAR = 0, 1
 
@@ -656,10 +697,11 @@ def attr_transfer_loop(pos, attr, dx, shape, mode):
 
 
 def substep(state: dict, dt: float, pressure="pcg", iters=80, tol=1e-4, dtype=np.float64,
-            gravity=-9.81, flip_blend=0.95, rho=1.0, transfer="flip", pcg_sign="physical"):
+            gravity=-9.81, flip_blend=0.95, rho=1.0, transfer="flip", pcg_sign="physical",
+            viscosity=0.0, viscosity_iters=12):
     """One substep in the stage order of _step_impl (solver3d.py:3475-3667):
-    clear → P2G → normalize → gravity → BC → divergence → pressure → grad → BC
-    → G2P + advect → colour / temperature transfer.  ``state`` holds pos,
+    clear → P2G → normalize → gravity → BC → [viscosity → BC] → divergence →
+    pressure → grad → BC → G2P + advect → colour / temperature transfer.  ``state`` holds pos,
     vel, marker, dx, (APIC) affine and optionally color (n,3) / temperature (n,).
     pressure: "jacobi" / "gsrb" (fixed iterations), "pcg" (reference restatement;
     pcg_sign "physical" negates its result, "reference" keeps it) or
@@ -677,6 +719,9 @@ def substep(state: dict, dt: float, pressure="pcg", iters=80, tol=1e-4, dtype=np
     w, ws = normalize(w, ww)
     v = add_gravity(v, gravity, dt)
     u, v, w = enforce_solid_bc(u, v, w, m)
+    if viscosity > 0.0:
+        r = np.float32(dt * viscosity / (dx * dx))
+        u, v, w, us, vs, ws = viscosity_stage(u, v, w, us, vs, ws, m, r, viscosity_iters)
     div = divergence(u, v, w, m, dx, dt, rho)
     info = {}
     if pressure == "jacobi":
