@@ -146,6 +146,8 @@ def main():
                     help="skip the batched-scenes leg (256 scenes, 20 substeps)")
     ap.add_argument("--no-attributes", action="store_true",
                     help="skip the colour/temperature transfer leg (3 substeps on the scene)")
+    ap.add_argument("--no-viscosity", action="store_true",
+                    help="skip the viscosity leg (3 viscous substeps on the scene)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (3 launches) so that profiler "
                          "byte counters can be scaled against a kernel of known traffic")
@@ -293,6 +295,11 @@ def main():
         attrs = attribute_leg(s, one_step)
         log(f"attributes: {attrs}")
 
+    visc = None
+    if not args.no_viscosity:
+        visc = viscosity_leg(s, one_step)
+        log(f"viscosity: {visc}")
+
     adj = None
     if not args.no_adjoint:
         del s
@@ -335,6 +342,7 @@ def main():
         "pcg_vs_fp64": fp64,
         "pcie_inclusive_value": pcie,
         "attributes": attrs,
+        "viscosity": visc,
         "adjoint": adj,
         "batched": bat,
     }
@@ -368,6 +376,33 @@ def attribute_leg(s, one_step, steps=3):
             "color_Mparticles_per_s": round(n / (col * 1e-3) / 1e6, 1) if col else None,
             "temperature_Mparticles_per_s": round(n / (tmp * 1e-3) / 1e6, 1) if tmp else None,
             "timing": "wall clock around the substeps; transfers by HIP events (step profiler)"}
+
+
+def viscosity_leg(s, one_step, steps=3, viscosity=1e-3, iters=12):
+    """The scene with viscosity on (S2.13, `iters` Jacobi sweeps per component
+    over the dense face arrays): substep time and the viscosity stage's own
+    time (HIP events), plus its bytes (12 per face per sweep: u_in, u_old,
+    u_out; 16 per face for the two copies of the faces besides)."""
+    import torch
+    s.viscosity, s.viscosity_iters = viscosity, iters
+    one_step()
+    s._prof.reset()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        one_step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    sec = s._prof.summarize()
+    s.viscosity = 0.0
+    vms = sec.get("viscosity", 0.0) / steps
+    faces = sum(int(s._f[k].numel()) for k in ("u", "v", "w"))
+    b = faces * (12 * iters + 16)
+    return {"viscosity": viscosity, "viscosity_iters": iters, "steps": steps,
+            "ms_per_step": round(ms, 3), "viscosity_ms": round(vms, 4),
+            "divergence_ms": round(sec.get("divergence", 0.0) / steps, 4),
+            "stage_bytes": b, "GBps": round(b / (vms * 1e-3) / 1e9, 1) if vms else None,
+            "timing": "wall clock around the substeps; stages by HIP events (step profiler)"}
 
 
 def adjoint_leg():

@@ -104,6 +104,8 @@ SIGNATURES = {
     "gf_scene_attr_transfer": [SP, V, I, I, V, V, V, V],
     "gf_scene_g2p_pre": [SP, V, V, V, V, I, F, F, F, V],
     "gf_scene_divergence": [SP, F, F, V],
+    "gf_csf_workspace": [I, I, I, V, V],
+    "gf_csf_apply": [V, V, V, V, I, I, I, F, F, I, V, V, V],
     "gf_scene_compact_to_dense": [SP, V, V, V],
     "gf_scene_unsort": [SP, V, I, V, V],
     "gf_k3_clear_grid": [V, V, V, V, V, V, V, I, I, I, V],
@@ -115,6 +117,7 @@ SIGNATURES = {
     "gf_k3_jacobi_pressure": [V, V, V, V, I, I, I, V],
     "gf_k3_gauss_seidel_rb": [V, V, V, I, I, I, I, V],
     "gf_k3_jacobi_visc": [V, V, V, I, I, I, F, V],
+    "gf_k3_jacobi_visc3": [V, V, V, V, V, V, V, V, V, I, I, I, F, V],
     "gf_k3_apply_A": [V, V, V, V, I, I, I, V],
     "gf_k3_compute_diag": [V, V, I, I, I, V],
     "gf_k3_apply_invM": [V, V, V, V, V, I, I, I, V],

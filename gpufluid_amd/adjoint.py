@@ -35,8 +35,9 @@ class FlipRollout:
                  pressure_iters: int = 500, pressure_tol: float = 1e-8):
         if solver.transfer_mode not in ("flip", "pic"):
             raise NotImplementedError("the substep adjoint covers FLIP/PIC transfers")
-        if solver.viscosity > 0.0:
-            raise NotImplementedError("the substep adjoint covers the inviscid substep")
+        if solver.viscosity > 0.0 or solver.surface_tension > 0.0:
+            raise NotImplementedError("the substep adjoint covers the substep without viscosity "
+                                      "or surface tension")
         if substeps < 1 or checkpoint_every < 1:
             raise ValueError("substeps and checkpoint_every must be >= 1")
         self.s = solver

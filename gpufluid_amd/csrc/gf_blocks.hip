@@ -466,6 +466,56 @@ __global__ __launch_bounds__(GF_WG) void k_jacobi_visc(const float* __restrict__
     u_out[c] = num / den;
 }
 
+// The same sweep on all three MAC components in one launch, for the substep's
+// viscosity stage.  A thread owns one (j, k) column of one component and
+// marches along i through a chunk of VISC_CHUNK planes, keeping the i-1, i,
+// i+1 values of u_in in registers, so u_in is read about once per sweep.
+constexpr int VISC_CHUNK = 32;
+
+struct ViscArgs {
+    const float* old0; const float* old1; const float* old2;
+    const float* in0; const float* in1; const float* in2;
+    float* out0; float* out1; float* out2;
+    int nx, ny, nz, nchunk;
+    float r;
+};
+
+__global__ __launch_bounds__(GF_WG) void k_jacobi_visc3(ViscArgs a) {
+#pragma clang fp contract(off)
+    const int c = blockIdx.z % 3, chunk = blockIdx.z / 3;
+    const int sx = a.nx + (c == 0), sy = a.ny + (c == 1), sz = a.nz + (c == 2);
+    const int k = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int i0 = chunk * VISC_CHUNK;
+    if (k >= sz || j >= sy || i0 >= sx) return;
+    const int i1 = min(i0 + VISC_CHUNK, sx);
+    const float* u_old = c == 0 ? a.old0 : (c == 1 ? a.old1 : a.old2);
+    const float* u_in = c == 0 ? a.in0 : (c == 1 ? a.in1 : a.in2);
+    float* u_out = c == 0 ? a.out0 : (c == 1 ? a.out1 : a.out2);
+    const size_t di = (size_t)sy * sz;
+    size_t q = (size_t)i0 * di + (size_t)j * sz + k;
+    float prev = i0 > 0 ? u_in[q - di] : 0.0f;
+    float cur = u_in[q];
+    float next = i0 + 1 < sx ? u_in[q + di] : 0.0f;
+    const float r = a.r;
+    for (int i = i0; i < i1; ++i) {
+        float sum = 0.0f, cnt = 0.0f;
+        if (i > 0) { sum += prev; cnt += 1.0f; }
+        if (i < sx - 1) { sum += next; cnt += 1.0f; }
+        if (j > 0) { sum += u_in[q - sz]; cnt += 1.0f; }
+        if (j < sy - 1) { sum += u_in[q + sz]; cnt += 1.0f; }
+        if (k > 0) { sum += u_in[q - 1]; cnt += 1.0f; }
+        if (k < sz - 1) { sum += u_in[q + 1]; cnt += 1.0f; }
+        const float num = u_old[q] + r * sum;
+        const float den = 1.0f + r * cnt;
+        u_out[q] = num / den;
+        prev = cur;
+        cur = next;
+        if (i + 2 < sx) next = u_in[q + 2 * di];
+        q += di;
+    }
+}
+
 // ---- per-particle attribute transfer (S2.15 colour, S2.18 scalar) --------
 // P2G of `nc` channels per particle onto the cell-centred grid at the current
 // positions (solver3d.py:562-605, 670-707), normalise (607-631, 710-727) and
@@ -757,6 +807,31 @@ extern "C" int gf_k3_jacobi_visc(const float* u_old, const float* u_in, float* u
     hipLaunchKernelGGL(k_jacobi_visc, grid, dim3(GF_WG), 0, (hipStream_t)stream, u_old, u_in,
                        u_out, sx, sy, sz, r);
     GF_LAUNCH_CHECK("k_jacobi_visc");
+    return 0;
+}
+
+extern "C" int gf_k3_jacobi_visc3(const float* old_u, const float* old_v, const float* old_w,
+                                  const float* in_u, const float* in_v, const float* in_w,
+                                  float* out_u, float* out_v, float* out_w, int nx, int ny,
+                                  int nz, float r, void* stream) {
+    GRID_ARGS_OK("gf_k3_jacobi_visc3");
+    GF_REQUIRE(ny + 1 <= 4 * 65535, "gf_k3_jacobi_visc3: ny %d", ny);
+    GF_REQUIRE(old_u && old_v && old_w && in_u && in_v && in_w && out_u && out_v && out_w,
+               "gf_k3_jacobi_visc3: null buffer");
+    GF_REQUIRE(out_u != in_u && out_v != in_v && out_w != in_w && out_u != old_u &&
+                   out_v != old_v && out_w != old_w,
+               "gf_k3_jacobi_visc3: out must not alias in or old");
+    ViscArgs a;
+    a.old0 = old_u; a.old1 = old_v; a.old2 = old_w;
+    a.in0 = in_u; a.in1 = in_v; a.in2 = in_w;
+    a.out0 = out_u; a.out1 = out_v; a.out2 = out_w;
+    a.nx = nx; a.ny = ny; a.nz = nz; a.r = r;
+    a.nchunk = (nx + 1 + VISC_CHUNK - 1) / VISC_CHUNK;
+    GF_REQUIRE(3LL * a.nchunk <= 65535, "gf_k3_jacobi_visc3: nx %d", nx);
+    const dim3 grid((unsigned)((nz + 1 + 63) / 64), (unsigned)((ny + 1 + 3) / 4),
+                    (unsigned)(3 * a.nchunk));
+    hipLaunchKernelGGL(k_jacobi_visc3, grid, dim3(GF_WG), 0, (hipStream_t)stream, a);
+    GF_LAUNCH_CHECK("k_jacobi_visc3");
     return 0;
 }
 

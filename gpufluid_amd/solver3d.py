@@ -77,9 +77,11 @@ _TRANSFER = {"flip": _lib.GF_TRANSFER_FLIP, "pic": _lib.GF_TRANSFER_FLIP,
 class FlipSolver3D:
     """3D FLIP/PIC/APIC solver holding all state on the HIP device.
 
-    Parameters follow solver3d.py:2358-2371.  viscosity, surface tension and
-    sub-dense storage are not part of this build and raise
-    NotImplementedError when requested.  Extra keyword-only parameters:
+    Parameters follow solver3d.py:2358-2371; viscosity (S2.13) and surface
+    tension (S2.14) run between P2G and the pressure solve as in the
+    reference; sub-dense storage raises NotImplementedError (the pressure
+    storage is compacted to active tiles instead).  Extra keyword-only
+    parameters:
     pcg_precision ("mixed" | "fp64"), pcg_sign ("physical" | "reference"),
     pcg_tol (reference default 1e-4), pcg_check_every (host readback cadence).
     """
@@ -93,8 +95,6 @@ class FlipSolver3D:
                  sub_rebuild_every: int = 8, sub_dilation: int = 4, *,
                  pcg_precision: str = "mixed", pcg_sign: str = "physical",
                  pcg_tol: float = 1e-4, pcg_check_every: int = 8):
-        if surface_tension > 0.0:
-            raise NotImplementedError("surface tension (S2.14) is not part of the MI355X substep build")
         if enable_sub_dense:
             raise NotImplementedError("sub-dense storage (F3.7) is not part of this build; "
                                       "pressure storage is already compacted to active tiles")
@@ -166,6 +166,7 @@ class FlipSolver3D:
         # per-particle attributes in particle-id (caller) order, None = absent
         self._attr = {"color": None, "temperature": None}
         self._attr_ws = None
+        self._csf_ws = None
 
     # ------------------------------------------------------------ attributes
     def _arr(self, name):
@@ -467,27 +468,43 @@ class FlipSolver3D:
         faces are copied into the weight arrays as the right-hand side, then
         `viscosity_iters` Jacobi sweeps per component ping-pong between u and
         us (so us ends holding the last-but-one iterate, which FLIP then
-        samples as the old velocity, as in the reference), then the solid BC.
-        Returns the pre-projection faces for G2P: a copy of the result in the
-        weight arrays."""
+        samples as the old velocity, as in the reference), then the solid
+        BC."""
         f = self._f
         r = float(dt * self.viscosity / (self.dx * self.dx))
         rhs = self._weights()
         for t, key in zip(rhs, ("u", "v", "w")):
             t.copy_(f[key])
         for _ in range(int(self.viscosity_iters)):
-            for t, key, sav in zip(rhs, ("u", "v", "w"), ("us", "vs", "ws")):
-                a, b = f[key], f[sav]
-                sx, sy, sz = (int(x) for x in a.shape)
-                _lib.call("S2.13", "gf_k3_jacobi_visc", _lib.ptr(t), _lib.ptr(a), _lib.ptr(b),
-                          sx, sy, sz, r, self._stream())
-                f[key], f[sav] = b, a
+            _lib.call("S2.13", "gf_k3_jacobi_visc3", *(_lib.ptr(t) for t in rhs),
+                      *(_lib.ptr(f[k]) for k in ("u", "v", "w", "us", "vs", "ws")),
+                      self.nx, self.ny, self.nz, r, self._stream())
+            for key, sav in (("u", "us"), ("v", "vs"), ("w", "ws")):
+                f[key], f[sav] = f[sav], f[key]
+        self._solid_bc()
+
+    def _solid_bc(self):
+        f = self._f
         _lib.call("S2.4", "gf_k3_enforce_solid_bc", *(_lib.ptr(f[k]) for k in
                   ("u", "v", "w", "marker", "solid_u", "solid_v", "solid_w")),
                   self.nx, self.ny, self.nz, self._stream())
-        for t, key in zip(rhs, ("u", "v", "w")):
-            t.copy_(f[key])
-        return rhs
+
+    def _surface_tension(self, dt):
+        """S2.14 as _step_impl runs it (solver3d.py:3548-3555, 3230-3300): the
+        CSF impulse with force balance on u, v, w, then the solid BC."""
+        if self._csf_ws is None:
+            wf, rd = ctypes.c_int64(), ctypes.c_int64()
+            _lib.call("S2.14", "gf_csf_workspace", self.nx, self.ny, self.nz, ctypes.byref(wf),
+                      ctypes.byref(rd))
+            self._csf_ws = (torch.empty(wf.value, dtype=torch.float32, device=self.device),
+                            torch.empty(rd.value, dtype=torch.float64, device=self.device))
+        work, red = self._csf_ws
+        coeff = float(self.surface_tension * dt / max(self.rho, 1.0e-8))
+        f = self._f
+        _lib.call("S2.14", "gf_csf_apply", *(_lib.ptr(f[k]) for k in ("u", "v", "w", "marker")),
+                  self.nx, self.ny, self.nz, float(self.dx), coeff,
+                  int(self.csf_smoothing_passes), _lib.ptr(work), _lib.ptr(red), self._stream())
+        self._solid_bc()
 
     def _attr_transfer(self, s, attr, channels, mode, stream):
         """S2.15 / S2.18: scatter, normalise and gather one attribute at the
@@ -555,9 +572,16 @@ class FlipSolver3D:
         prof.fused("normalize")
         prof.fused("gravity_bc")
         pre = None
-        if self.viscosity > 0.0:
-            with prof.section("viscosity"):
-                pre = self._viscosity(dt)
+        if self.viscosity > 0.0 or self.surface_tension > 0.0:
+            if self.viscosity > 0.0:
+                with prof.section("viscosity"):
+                    self._viscosity(dt)
+            if self.surface_tension > 0.0:
+                with prof.section("surface_tension"):
+                    self._surface_tension(dt)
+            pre = self._weights()
+            for t, key in zip(pre, ("u", "v", "w")):
+                t.copy_(self._f[key])
             s = self._descriptor()
             with prof.section("divergence"):
                 _lib.call("S2.5", "gf_scene_divergence", ctypes.byref(s), dt, self.rho, stream)

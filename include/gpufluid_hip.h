@@ -249,6 +249,22 @@ int gf_scene_g2p_pre(const gf_scene_t* s, const float* pressure, const float* pr
  * solve. */
 int gf_scene_divergence(const gf_scene_t* s, float dt, float rho, void* stream);
 
+/* Surface tension (S2.14, _apply_surface_tension solver3d.py:3230-3300 with
+ * k3_indicator_from_marker, k3_box_blur_centred, k3_csf_normal,
+ * k3_csf_curvature, k3_csf_apply_u/v/w, k3_csf_sum_* and
+ * k3_csf_subtract_bias_*_dev, 203-549) on dense u (nx+1,ny,nz),
+ * v (nx,ny+1,nz), w (nx,ny,nz+1), in place: chi = fluid indicator blurred
+ * `smoothing_passes` times, n = grad chi/(|grad chi|+1e-8), kappa = -div n,
+ * face += coeff*kappa_face*grad chi on faces with no solid side (coeff =
+ * sigma*dt/max(rho,1e-8)), then each component's mean change over the faces
+ * next to fluid is subtracted there.  The caller applies the solid BC after.
+ * Workspace sizes from gf_csf_workspace: work (6*C floats: chi, blur
+ * ping-pong, normal, curvature), red (doubles). */
+int gf_csf_workspace(int nx, int ny, int nz, int64_t* work_floats, int64_t* red_doubles);
+int gf_csf_apply(float* u, float* v, float* w, const int32_t* marker, int nx, int ny, int nz,
+                 float dx, float coeff, int smoothing_passes, float* work, double* red,
+                 void* stream);
+
 /* Per-particle attribute transfer at the end of a substep (S2.15 colour,
  * solver3d.py:3333-3359 with k3_p2g_color / k3_normalize_color /
  * k3_g2p_color 562-664; S2.18 scalar, 3304-3330 with k3_p2g_scalar /
@@ -314,6 +330,13 @@ int gf_k3_gauss_seidel_rb(float* p, const float* div, const int32_t* marker, int
  * u_in (distinct from u_out); sx <= 65535, sy <= 262140 */
 int gf_k3_jacobi_visc(const float* u_old, const float* u_in, float* u_out, int sx, int sy,
                       int sz, float r, void* stream);
+/* k3_jacobi_visc on u (nx+1,ny,nz), v (nx,ny+1,nz) and w (nx,ny,nz+1) in one
+ * launch, the substep's S2.13 sweep (solver3d.py:3535-3541); same arithmetic
+ * as gf_k3_jacobi_visc.  out_* must not alias in_* or old_*. */
+int gf_k3_jacobi_visc3(const float* old_u, const float* old_v, const float* old_w,
+                       const float* in_u, const float* in_v, const float* in_w, float* out_u,
+                       float* out_v, float* out_w, int nx, int ny, int nz, float r,
+                       void* stream);
 /* k3_apply_A, solver3d.py:1743-1797 (done: device int[1], may be NULL) */
 int gf_k3_apply_A(const float* x, float* y, const int32_t* marker, const int32_t* done,
                   int nx, int ny, int nz, void* stream);

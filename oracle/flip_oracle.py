@@ -24,6 +24,9 @@ How each part is pinned (tests/test_oracle_*.py):
   * APIC: affine velocity field is transferred exactly (P2G->G2P round trip).
   * viscosity sweep: constant fields fixed, r = 0 identity, fixed point =
     numpy.linalg.solve of the assembled (I - r L) system, per-entry loop;
+  * surface tension: blur = brute-force 27-point mean; normal and curvature
+    of a blurred sphere point inward/≈2/R; force balance leaves zero mean
+    change; per-face loop of the impulse;
   * attribute transfer: hand-derived closed form on a two-particle line,
     identity for one particle at each cell centre, constants preserved,
     weight sum = particle count, pure-Python per-corner loop.
@@ -581,6 +584,120 @@ def viscosity_stage(u, v, w, us, vs, ws, marker, r, iters):
     return cu, cv, cw, sav[0], sav[1], sav[2]
 
 
+# ------------------------------------------------------------ surface tension
+def csf_indicator(marker, dtype=np.float32):
+    """k3_indicator_from_marker (solver3d.py:212-229)."""
+    return (np.asarray(marker) == FLUID).astype(dtype)
+
+
+def box_blur(src):
+    """k3_box_blur_centred (solver3d.py:235-256): mean over the in-range cells
+    of the 3x3x3 block, summed with di outermost and dk innermost."""
+    t = src.dtype
+    s = np.zeros_like(src)
+    n = np.zeros_like(src)
+    ones = np.ones_like(src)
+    for di in (-1, 0, 1):
+        for dj in (-1, 0, 1):
+            for dk in (-1, 0, 1):
+                v, ok = src, ones
+                for ax, d in ((0, di), (1, dj), (2, dk)):
+                    if d:
+                        v = _shift(v, ax, d, 0)
+                        ok = _shift(ok, ax, d, 0)
+                s = (s + v).astype(t)
+                n = (n + ok).astype(t)
+    return (s / n).astype(t)
+
+
+def _interior(shape):
+    m = np.zeros(shape, dtype=bool)
+    m[1:-1, 1:-1, 1:-1] = True
+    return m
+
+
+def csf_normal(chi, dx):
+    """k3_csf_normal (solver3d.py:262-286): grad chi / (|grad chi| + 1e-8) by
+    central differences over 2dx; zero on the outer cell layer."""
+    t = chi.dtype.type
+    h = t(2) * t(np.float32(dx))
+    g = [((_shift(chi, ax, 1, 0) - _shift(chi, ax, -1, 0)) / h).astype(chi.dtype)
+         for ax in range(3)]
+    sq = ((g[0] * g[0]).astype(chi.dtype) + (g[1] * g[1]).astype(chi.dtype)).astype(chi.dtype)
+    sq = (sq + (g[2] * g[2]).astype(chi.dtype)).astype(chi.dtype)
+    mag = (np.sqrt(sq) + t(1e-8)).astype(chi.dtype)
+    inside = _interior(chi.shape)
+    return [np.where(inside, (gc / mag).astype(chi.dtype), t(0)) for gc in g]
+
+
+def csf_curvature(n, dx):
+    """k3_csf_curvature (solver3d.py:292-313): kappa = -div n by central
+    differences; zero on the outer cell layer."""
+    t = n[0].dtype.type
+    h = t(2) * t(np.float32(dx))
+    d = [((_shift(n[ax], ax, 1, 0) - _shift(n[ax], ax, -1, 0)) / h).astype(n[0].dtype)
+         for ax in range(3)]
+    k = -(((d[0] + d[1]).astype(n[0].dtype) + d[2]).astype(n[0].dtype))
+    return np.where(_interior(n[0].shape), k, t(0)).astype(n[0].dtype)
+
+
+def _face_masks(marker, comp):
+    """Faces of component comp with both cells in range: (active = neither
+    side solid, fluid = active and one side fluid), as face-shaped masks."""
+    nx, ny, nz = marker.shape
+    shape = [nx, ny, nz]
+    shape[comp] += 1
+    active = np.zeros(shape, dtype=bool)
+    fluid = np.zeros(shape, dtype=bool)
+    sl = [slice(None)] * 3
+    sl[comp] = slice(1, shape[comp] - 1)
+    lo = [slice(None)] * 3
+    lo[comp] = slice(0, -1)
+    hi = [slice(None)] * 3
+    hi[comp] = slice(1, None)
+    mlo, mhi = marker[tuple(lo)], marker[tuple(hi)]
+    act = (mlo != SOLID) & (mhi != SOLID)
+    active[tuple(sl)] = act
+    fluid[tuple(sl)] = act & ((mlo == FLUID) | (mhi == FLUID))
+    return active, fluid, tuple(lo), tuple(hi), tuple(sl)
+
+
+def csf_apply(u, v, w, marker, dx, coeff, passes):
+    """_apply_surface_tension (solver3d.py:3230-3300): indicator, `passes`
+    blurs, normal, curvature; face += coeff*kappa_face*grad chi on faces with
+    no solid side (k3_csf_apply_*, 316-385); then each component's mean change
+    over faces next to fluid, summed exactly and rounded to the fields'
+    precision, is subtracted there (k3_csf_sum_* / subtract_bias_*_dev,
+    400-547).  Returns new (u, v, w)."""
+    dt_ = u.dtype
+    t = dt_.type
+    chi = csf_indicator(marker, dt_)
+    for _ in range(passes):
+        chi = box_blur(chi)
+    kappa = csf_curvature(csf_normal(chi, dx), dx)
+    out = []
+    for comp, f in enumerate((u, v, w)):
+        active, fluid, lo, hi, sl = _face_masks(marker, comp)
+        grad = ((chi[hi] - chi[lo]) / t(np.float32(dx))).astype(dt_)
+        kf = (t(0.5) * (kappa[hi] + kappa[lo]).astype(dt_)).astype(dt_)
+        inc = ((t(np.float32(coeff)) * kf).astype(dt_) * grad).astype(dt_)
+        g = f.copy()
+        inner = g[sl]
+        a = active[sl]
+        inner[a] = (inner[a] + inc[a]).astype(dt_)
+        g[sl] = inner
+        delta = (g - f).astype(dt_)[fluid]
+        cnt = int(fluid.sum())
+        if cnt > 0:
+            total = delta.astype(np.float64).sum()
+            if dt_ == np.float32:
+                total = np.float32(total)
+            mean = t(total) / t(cnt)
+            g[fluid] = (g[fluid] - mean).astype(dt_)
+        out.append(g)
+    return tuple(out)
+
+
 # ------------------------------------------- per-particle attribute transfer
 ATTR_COLOR, ATTR_SCALAR = 0, 1
 
@@ -698,9 +815,10 @@ def attr_transfer_loop(pos, attr, dx, shape, mode):
 
 def substep(state: dict, dt: float, pressure="pcg", iters=80, tol=1e-4, dtype=np.float64,
             gravity=-9.81, flip_blend=0.95, rho=1.0, transfer="flip", pcg_sign="physical",
-            viscosity=0.0, viscosity_iters=12):
+            viscosity=0.0, viscosity_iters=12, surface_tension=0.0, csf_passes=2):
     """One substep in the stage order of _step_impl (solver3d.py:3475-3667):
-    clear → P2G → normalize → gravity → BC → [viscosity → BC] → divergence →
+    clear → P2G → normalize → gravity → BC → [viscosity → BC] → [surface
+    tension → BC] → divergence →
     pressure → grad → BC → G2P + advect → colour / temperature transfer.  ``state`` holds pos,
     vel, marker, dx, (APIC) affine and optionally color (n,3) / temperature (n,).
     pressure: "jacobi" / "gsrb" (fixed iterations), "pcg" (reference restatement;
@@ -722,6 +840,10 @@ def substep(state: dict, dt: float, pressure="pcg", iters=80, tol=1e-4, dtype=np
     if viscosity > 0.0:
         r = np.float32(dt * viscosity / (dx * dx))
         u, v, w, us, vs, ws = viscosity_stage(u, v, w, us, vs, ws, m, r, viscosity_iters)
+    if surface_tension > 0.0:
+        coeff = np.float32(surface_tension * dt / max(rho, 1.0e-8))
+        u, v, w = csf_apply(u, v, w, m, dx, coeff, max(1, int(csf_passes)))
+        u, v, w = enforce_solid_bc(u, v, w, m)
     div = divergence(u, v, w, m, dx, dt, rho)
     info = {}
     if pressure == "jacobi":

@@ -29,6 +29,26 @@ def test_visc_block_kernel_bit_exact(shape):
     np.testing.assert_array_equal(t_out.cpu().numpy(), fo.visc_sweep(u_old, u_in, r))
 
 
+@pytest.mark.parametrize("shape", [(20, 24, 16), (70, 5, 9)])
+def test_visc3_kernel_bit_exact(shape):
+    """The three-component marching sweep of the substep, bit for bit."""
+    from gpufluid_amd import _lib
+    nx, ny, nz = shape
+    rng = np.random.default_rng(nx)
+    shp = [(nx + 1, ny, nz), (nx, ny + 1, nz), (nx, ny, nz + 1)]
+    old = [rng.normal(size=x).astype(np.float32) for x in shp]
+    inp = [rng.normal(size=x).astype(np.float32) for x in shp]
+    r = 0.29
+    print(f"[visc] three-component sweep {shape}", flush=True)
+    t_old = [torch.tensor(a, device="cuda:0") for a in old]
+    t_in = [torch.tensor(a, device="cuda:0") for a in inp]
+    t_out = [torch.empty_like(t) for t in t_in]
+    _lib.call("S2.13", "gf_k3_jacobi_visc3", *(_lib.ptr(t) for t in t_old + t_in + t_out),
+              nx, ny, nz, r, _lib.stream_handle(torch.device("cuda:0")))
+    for c in range(3):
+        np.testing.assert_array_equal(t_out[c].cpu().numpy(), fo.visc_sweep(old[c], inp[c], r))
+
+
 @pytest.mark.parametrize("transfer,solver,iters", [("flip", "jacobi", 12), ("flip", "pcg", 5),
                                                    ("apic", "pcg", 12), ("pic", "jacobi", 0)])
 def test_viscous_substep_matches_oracle(transfer, solver, iters):
