@@ -148,6 +148,8 @@ def main():
                     help="skip the colour/temperature transfer leg (3 substeps on the scene)")
     ap.add_argument("--no-viscosity", action="store_true",
                     help="skip the viscosity leg (3 viscous substeps on the scene)")
+    ap.add_argument("--no-surface-tension", action="store_true",
+                    help="skip the surface-tension leg (3 substeps with CSF on the scene)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (3 launches) so that profiler "
                          "byte counters can be scaled against a kernel of known traffic")
@@ -300,6 +302,11 @@ def main():
         visc = viscosity_leg(s, one_step)
         log(f"viscosity: {visc}")
 
+    csf = None
+    if not args.no_surface_tension:
+        csf = surface_tension_leg(s, one_step)
+        log(f"surface tension: {csf}")
+
     adj = None
     if not args.no_adjoint:
         del s
@@ -343,6 +350,7 @@ def main():
         "pcie_inclusive_value": pcie,
         "attributes": attrs,
         "viscosity": visc,
+        "surface_tension": csf,
         "adjoint": adj,
         "batched": bat,
     }
@@ -402,6 +410,34 @@ def viscosity_leg(s, one_step, steps=3, viscosity=1e-3, iters=12):
             "ms_per_step": round(ms, 3), "viscosity_ms": round(vms, 4),
             "divergence_ms": round(sec.get("divergence", 0.0) / steps, 4),
             "stage_bytes": b, "GBps": round(b / (vms * 1e-3) / 1e9, 1) if vms else None,
+            "timing": "wall clock around the substeps; stages by HIP events (step profiler)"}
+
+
+def surface_tension_leg(s, one_step, steps=3, sigma=0.07, passes=2):
+    """The scene with surface tension on (S2.14: indicator, `passes` blurs,
+    normal, curvature over the dense cells; impulse and force balance over
+    the faces): substep time and the stage's own time (HIP events)."""
+    import torch
+    s.surface_tension, s.csf_smoothing_passes = sigma, passes
+    one_step()
+    s._prof.reset()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        one_step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    sec = s._prof.summarize()
+    s.surface_tension = 0.0
+    cms = sec.get("surface_tension", 0.0) / steps
+    cells = s.nx * s.ny * s.nz
+    faces = sum(int(s._f[k].numel()) for k in ("u", "v", "w"))
+    # cells: indicator 8, blur 8 per pass, normal 16, curvature 16; faces:
+    # apply 8 (+ chi/kappa reads, cached) and balance 8, then the solid BC
+    b = cells * (8 + 8 * passes + 16 + 16) + faces * 16
+    return {"surface_tension": sigma, "smoothing_passes": passes, "steps": steps,
+            "ms_per_step": round(ms, 3), "surface_tension_ms": round(cms, 4),
+            "stage_bytes": b, "GBps": round(b / (cms * 1e-3) / 1e9, 1) if cms else None,
             "timing": "wall clock around the substeps; stages by HIP events (step profiler)"}
 
 

@@ -86,8 +86,11 @@ __global__ void k_add_gravity(float* v, long long n, float g, float dt) {
 __global__ void k_enforce_solid_bc(float* u, float* v, float* w, const int32_t* marker,
                                    const float* su, const float* sv, const float* sw, int nx,
                                    int ny, int nz) {
-    int i, j, k;
-    if (!idx3(nx + 1, ny + 1, nz + 1, i, j, k)) return;
+    // 64 lanes along k x 4 along j per workgroup, one i per grid z
+    const int k = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int i = blockIdx.z;
+    if (k > nz || j > ny || i > nx) return;
     if (i <= nx && j < ny && k < nz) {
         bool left = (i == 0) || (i > 0 && marker[((size_t)(i - 1) * ny + j) * nz + k] == 2);
         bool right = (i == nx) || (i < nx && marker[((size_t)i * ny + j) * nz + k] == 2);
@@ -468,9 +471,11 @@ __global__ __launch_bounds__(GF_WG) void k_jacobi_visc(const float* __restrict__
 
 // The same sweep on all three MAC components in one launch, for the substep's
 // viscosity stage.  A thread owns one (j, k) column of one component and
-// marches along i through a chunk of VISC_CHUNK planes, keeping the i-1, i,
-// i+1 values of u_in in registers, so u_in is read about once per sweep.
+// marches along i through a chunk of VISC_CHUNK planes, VISC_BATCH planes at
+// a time: the batch's loads are all issued before its stores, and the i-1,
+// i, i+1 values of u_in stay in registers, so u_in is read about once.
 constexpr int VISC_CHUNK = 32;
+constexpr int VISC_BATCH = 4;
 
 struct ViscArgs {
     const float* old0; const float* old1; const float* old2;
@@ -480,8 +485,52 @@ struct ViscArgs {
     float r;
 };
 
-__global__ __launch_bounds__(GF_WG) void k_jacobi_visc3(ViscArgs a) {
+__device__ __forceinline__ void visc_column(const float* __restrict__ u_old,
+                                            const float* __restrict__ u_in,
+                                            float* __restrict__ u_out, int sx, int sy, int sz,
+                                            int i0, int i1, int j, int k, float r) {
 #pragma clang fp contract(off)
+    const size_t di = (size_t)sy * sz;
+    const size_t base = (size_t)j * sz + k;
+    // col[m] = u_in at plane i + m - 1 of the current batch (m = 0..VISC_BATCH+1)
+    float col[VISC_BATCH + 2];
+    col[0] = i0 > 0 ? u_in[(size_t)(i0 - 1) * di + base] : 0.0f;
+    col[1] = u_in[(size_t)i0 * di + base];
+    for (int i = i0; i < i1; i += VISC_BATCH) {
+        float ym[VISC_BATCH], yp[VISC_BATCH], zm[VISC_BATCH], zp[VISC_BATCH], uo[VISC_BATCH];
+#pragma unroll
+        for (int b = 0; b < VISC_BATCH; ++b) {
+            const int ii = i + b;
+            const size_t q = (size_t)ii * di + base;
+            col[b + 2] = ii + 1 < sx ? u_in[q + di] : 0.0f;
+            const bool live = ii < i1;
+            ym[b] = live && j > 0 ? u_in[q - sz] : 0.0f;
+            yp[b] = live && j < sy - 1 ? u_in[q + sz] : 0.0f;
+            zm[b] = live && k > 0 ? u_in[q - 1] : 0.0f;
+            zp[b] = live && k < sz - 1 ? u_in[q + 1] : 0.0f;
+            uo[b] = live ? u_old[q] : 0.0f;
+        }
+#pragma unroll
+        for (int b = 0; b < VISC_BATCH; ++b) {
+            const int ii = i + b;
+            if (ii >= i1) break;
+            float sum = 0.0f, cnt = 0.0f;
+            if (ii > 0) { sum += col[b]; cnt += 1.0f; }
+            if (ii < sx - 1) { sum += col[b + 2]; cnt += 1.0f; }
+            if (j > 0) { sum += ym[b]; cnt += 1.0f; }
+            if (j < sy - 1) { sum += yp[b]; cnt += 1.0f; }
+            if (k > 0) { sum += zm[b]; cnt += 1.0f; }
+            if (k < sz - 1) { sum += zp[b]; cnt += 1.0f; }
+            const float num = uo[b] + r * sum;
+            const float den = 1.0f + r * cnt;
+            u_out[(size_t)ii * di + base] = num / den;
+        }
+        col[0] = col[VISC_BATCH];
+        col[1] = col[VISC_BATCH + 1];
+    }
+}
+
+__global__ __launch_bounds__(GF_WG) void k_jacobi_visc3(ViscArgs a) {
     const int c = blockIdx.z % 3, chunk = blockIdx.z / 3;
     const int sx = a.nx + (c == 0), sy = a.ny + (c == 1), sz = a.nz + (c == 2);
     const int k = blockIdx.x * 64 + (threadIdx.x & 63);
@@ -489,31 +538,12 @@ __global__ __launch_bounds__(GF_WG) void k_jacobi_visc3(ViscArgs a) {
     const int i0 = chunk * VISC_CHUNK;
     if (k >= sz || j >= sy || i0 >= sx) return;
     const int i1 = min(i0 + VISC_CHUNK, sx);
-    const float* u_old = c == 0 ? a.old0 : (c == 1 ? a.old1 : a.old2);
-    const float* u_in = c == 0 ? a.in0 : (c == 1 ? a.in1 : a.in2);
-    float* u_out = c == 0 ? a.out0 : (c == 1 ? a.out1 : a.out2);
-    const size_t di = (size_t)sy * sz;
-    size_t q = (size_t)i0 * di + (size_t)j * sz + k;
-    float prev = i0 > 0 ? u_in[q - di] : 0.0f;
-    float cur = u_in[q];
-    float next = i0 + 1 < sx ? u_in[q + di] : 0.0f;
-    const float r = a.r;
-    for (int i = i0; i < i1; ++i) {
-        float sum = 0.0f, cnt = 0.0f;
-        if (i > 0) { sum += prev; cnt += 1.0f; }
-        if (i < sx - 1) { sum += next; cnt += 1.0f; }
-        if (j > 0) { sum += u_in[q - sz]; cnt += 1.0f; }
-        if (j < sy - 1) { sum += u_in[q + sz]; cnt += 1.0f; }
-        if (k > 0) { sum += u_in[q - 1]; cnt += 1.0f; }
-        if (k < sz - 1) { sum += u_in[q + 1]; cnt += 1.0f; }
-        const float num = u_old[q] + r * sum;
-        const float den = 1.0f + r * cnt;
-        u_out[q] = num / den;
-        prev = cur;
-        cur = next;
-        if (i + 2 < sx) next = u_in[q + 2 * di];
-        q += di;
-    }
+    if (c == 0)
+        visc_column(a.old0, a.in0, a.out0, sx, sy, sz, i0, i1, j, k, a.r);
+    else if (c == 1)
+        visc_column(a.old1, a.in1, a.out1, sx, sy, sz, i0, i1, j, k, a.r);
+    else
+        visc_column(a.old2, a.in2, a.out2, sx, sy, sz, i0, i1, j, k, a.r);
 }
 
 // ---- per-particle attribute transfer (S2.15 colour, S2.18 scalar) --------
@@ -755,9 +785,12 @@ extern "C" int gf_k3_enforce_solid_bc(float* u, float* v, float* w, const int32_
                                       void* stream) {
     GRID_ARGS_OK("gf_k3_enforce_solid_bc");
     GF_REQUIRE(u && v && w && marker, "gf_k3_enforce_solid_bc: null buffer");
-    hipLaunchKernelGGL(k_enforce_solid_bc, grid_for((long long)(nx + 1) * (ny + 1) * (nz + 1)),
-                       dim3(GF_WG), 0, (hipStream_t)stream, u, v, w, marker, solid_u, solid_v,
-                       solid_w, nx, ny, nz);
+    GF_REQUIRE(nx + 1 <= 65535 && ny + 1 <= 4 * 65535, "gf_k3_enforce_solid_bc: grid %d x %d",
+               nx, ny);
+    const dim3 grid((unsigned)((nz + 1 + 63) / 64), (unsigned)((ny + 1 + 3) / 4),
+                    (unsigned)(nx + 1));
+    hipLaunchKernelGGL(k_enforce_solid_bc, grid, dim3(GF_WG), 0, (hipStream_t)stream, u, v, w,
+                       marker, solid_u, solid_v, solid_w, nx, ny, nz);
     GF_LAUNCH_CHECK("k_enforce_solid_bc");
     return 0;
 }
