@@ -102,6 +102,12 @@ SIGNATURES = {
     "gf_scene_pcg_enqueue": [SP, I, I, D, I, I, I, I, I, V],
     "gf_scene_g2p": [SP, V, I, F, F, F, F, V],
     "gf_scene_attr_transfer": [SP, V, I, I, V, V, V, V],
+    "gf_reseed_workspace": [I, I, I, I, V],
+    "gf_reseed_plan_sync": [V, I, V, I, I, I, F, I, I, I, V, PI, V],
+    "gf_reseed_compact": [V, I, I, I, I, I, V, V, V],
+    "gf_reseed_emit": [I, I, I, I, F, V, V, V, V],
+    "gf_outflow_mark_sync": [V, I, V, I, V, V, V, PI, V],
+    "gf_compact_rows": [V, I, I, V, V, V, V],
     "gf_scene_g2p_pre": [SP, V, V, V, V, I, F, F, F, V],
     "gf_scene_divergence": [SP, F, F, V],
     "gf_csf_workspace": [I, I, I, V, V],

@@ -17,6 +17,10 @@ int check_launch(const char* what);
 enum { GF_TUNE_PCG_WORKGROUPS = 0, GF_TUNE_PCG_VARIANT, GF_TUNE_COUNT };
 int tunable(int id);
 
+// Exclusive scan of n int32 (gf_sort.hip): out[0..n] with out[n] = total;
+// part holds ceil(n/4096)+1 entries.
+int scan_exclusive_i32(const int32_t* in, int n, int32_t* out, int32_t* part, hipStream_t st);
+
 __device__ __forceinline__ int clampi(int v, int lo, int hi) {
     return v < lo ? lo : (v > hi ? hi : v);
 }

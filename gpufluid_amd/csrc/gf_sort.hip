@@ -279,6 +279,10 @@ int scan_exclusive(const int32_t* in, int n, int32_t* out, int32_t* part, hipStr
 }  // namespace
 
 namespace gf {
+int scan_exclusive_i32(const int32_t* in, int n, int32_t* out, int32_t* part, hipStream_t st) {
+    return scan_exclusive(in, n, out, part, st);
+}
+
 int scene_check(const gf_scene_t* s, const char* who) {
     if (!s) return set_error(GF_EINVAL, "%s: null scene", who);
     if (s->nx < 1 || s->ny < 1 || s->nz < 1)

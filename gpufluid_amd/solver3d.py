@@ -40,6 +40,7 @@ from .blocks import BlockError, block
 from .device import DeviceArray, as_tensor, default_device
 from .profiling import StepProfiler
 from .scene import SceneBuffers
+from .regions import FrameEventQueue
 from .seeding import seed_box_positions
 
 
@@ -67,6 +68,12 @@ def cfl_substep_count(vel_np: np.ndarray, dx: float, target_dt: float, cfl: floa
         return 1
     n = max(1, int(np.ceil(target_dt / (cfl * dx / vmax))))
     return min(n, max_substeps)
+
+
+def _rows(value, width, device):
+    """(n, width) float32 contiguous device tensor of an array-like."""
+    t = value.t if isinstance(value, DeviceArray) else value
+    return as_tensor(t, torch.float32, device).reshape(-1, width).contiguous()
 
 
 _TRANSFER = {"flip": _lib.GF_TRANSFER_FLIP, "pic": _lib.GF_TRANSFER_FLIP,
@@ -167,6 +174,11 @@ class FlipSolver3D:
         self._attr = {"color": None, "temperature": None}
         self._attr_ws = None
         self._csf_ws = None
+        # inflow/outflow regions drained by prepare_frame (D4.7 / F3.6)
+        self.inflows: list = []
+        self.outflows: list = []
+        self._rng = np.random.default_rng(0)
+        self._frame_events = FrameEventQueue()
 
     # ------------------------------------------------------------ attributes
     def _arr(self, name):
@@ -246,15 +258,59 @@ class FlipSolver3D:
         t = self._unsorted("pos", self._pos, 3)
         return None if t is None else DeviceArray(t, vec=1)
 
+    @pos.setter
+    def pos(self, value):
+        """Replace the particle positions, as the reference's callers assign
+        ``solver.pos`` (cli/commands.py:646-655): velocities become zero and
+        affine matrices zero until assigned; attributes are kept when their
+        length still matches, else dropped until assigned."""
+        t = None if value is None else _rows(value, 3, self.device)
+        n = 0 if t is None else t.shape[0]
+        keep = {k: (a if a is not None and a.shape[0] == n else None)
+                for k, a in self._attr.items()}
+        self.set_particles(t if t is not None else torch.zeros((0, 3), device=self.device),
+                           color=keep["color"], temperature=keep["temperature"])
+
     @property
     def vel(self):
         t = self._unsorted("vel", self._vel, 3)
         return None if t is None else DeviceArray(t, vec=1)
 
+    @vel.setter
+    def vel(self, value):
+        """Replace the velocities (caller order, same particle count)."""
+        self._set_rows("vel", value, 3)
+
     @property
     def affine_C(self):
         t = self._unsorted("affine", self._affine, 9)
         return None if t is None else DeviceArray(t.view(-1, 3, 3), vec=2)
+
+    @affine_C.setter
+    def affine_C(self, value):
+        """Replace the APIC matrices (caller order); None zeroes them, as the
+        reference's step rebuilds a dropped affine_C as zeros."""
+        if self.n_particles == 0:
+            return
+        if value is None:
+            a = self._sc.live(self.n_particles)["affine"]
+            if a is not None:
+                a.zero_()
+            self._cache.clear()
+            return
+        self._sc.ensure_particles(self.n_particles, True)
+        self._set_rows("affine", value, 9)
+
+    def _set_rows(self, key, value, width):
+        n = self.n_particles
+        t = _rows(value, width, self.device)
+        if t.shape[0] != n:
+            raise ValueError(f"{key} needs {n} rows, got {t.shape[0]}")
+        if n == 0:
+            return
+        live = self._sc.live(n)
+        live[key].copy_(t[live["pid"].long()])
+        self._cache.clear()
 
     # ------------------------------------------------- per-particle attributes
     def _get_attr(self, key):
@@ -435,6 +491,109 @@ class FlipSolver3D:
         self._f["marker"].copy_(torch.as_tensor(marker, device=self.device))
         self._cache.clear()
         self._cuda_graph = None
+
+    # ------------------------------------------------- frame hook (F3.6, D4.7)
+    @block("D4.7", "Register an inflow region (continuous particle emission)")
+    def add_inflow(self, region):
+        self.inflows.append(region)
+
+    @block("D4.7", "Register an outflow region (particle removal)")
+    def add_outflow(self, region):
+        self.outflows.append(region)
+
+    @block("F3.6", "Per-frame hook: emit inflow particles, drop outflow particles")
+    def prepare_frame(self, frame_idx: int, frame_dt: float):
+        """The inflow/outflow part of prepare_frame (solver3d.py:3911-4072):
+        regions publish this frame's events; outflow boxes are compacted away
+        on the device first, then the emitted particles (minus those landing
+        in a solid cell of the host marker) are appended with their event's
+        colour (else 0) and temperature (else 20) when the attribute is
+        tracked.  Animated obstacles and sub-dense storage are not part of
+        this build."""
+        q = self._frame_events
+        q.clear()
+        for inflow in self.inflows:
+            inflow.publish_for_frame(q, frame_idx, frame_dt, self._rng)
+        for outflow in self.outflows:
+            outflow.publish_for_frame(q, frame_idx)
+        out_ev = q.drain_outflows()
+        if out_ev:
+            self._apply_outflows_gpu(out_ev)
+        emits = q.drain_emits()
+        if not emits:
+            return
+        epos = np.concatenate([e.positions for e in emits]).astype(np.float32)
+        evel = np.concatenate([e.velocities for e in emits]).astype(np.float32)
+        col = temp = None
+        if self._attr["color"] is not None:
+            col = np.concatenate([
+                np.broadcast_to(np.asarray(e.color if e.color is not None else (0.0, 0.0, 0.0),
+                                           np.float32), (len(e.positions), 3))
+                for e in emits])
+        if self._attr["temperature"] is not None:
+            temp = np.concatenate([
+                np.full(len(e.positions), 20.0 if e.temperature is None else e.temperature,
+                        np.float32) for e in emits])
+        if len(epos):
+            mh = self._marker_host
+            idx = [np.clip((epos[:, a] / self.dx).astype(np.int32), 0, mh.shape[a] - 1)
+                   for a in range(3)]
+            keep = mh[idx[0], idx[1], idx[2]] != 2
+            if not keep.all():
+                epos, evel = epos[keep], evel[keep]
+                col = None if col is None else col[keep]
+                temp = None if temp is None else temp[keep]
+        if not len(epos):
+            return
+        dev = self.device
+        cur_p = self.pos.t if self.n_particles else torch.zeros((0, 3), device=dev)
+        cur_v = self.vel.t if self.n_particles else torch.zeros((0, 3), device=dev)
+        cat = lambda a, b: torch.cat([a, torch.as_tensor(np.ascontiguousarray(b),  # noqa: E731
+                                                         device=dev)])
+        new_col = None if col is None else cat(self._attr["color"], col)
+        new_temp = None if temp is None else cat(self._attr["temperature"], temp)
+        self.set_particles(cat(cur_p, epos), cat(cur_v, evel), color=new_col,
+                           temperature=new_temp)
+
+    @block("D4.7.GPU", "GPU outflow compaction (mark + scan + scatter)")
+    def _apply_outflows_gpu(self, outflow_events) -> int:
+        """Remove the particles inside any active outflow box, keeping the
+        survivors' order (solver3d.py:2659-2730): mark, scan and scatter on
+        the device, attributes in step; the APIC matrices restart at zero
+        when particles were removed, as the reference drops them."""
+        n = self.n_particles
+        if n == 0:
+            return 0
+        active = [ev for ev in outflow_events if ev.inside]
+        if not active:
+            return n
+        dev = self.device
+        boxes = torch.tensor([list(ev.lo) + list(ev.hi) for ev in active], dtype=torch.float32,
+                             device=dev)
+        pos = self.pos.t
+        alive = torch.empty(n, dtype=torch.int32, device=dev)
+        prefix = torch.empty(n + 1, dtype=torch.int32, device=dev)
+        part = torch.empty((n + 4095) // 4096 + 1, dtype=torch.int32, device=dev)
+        n_alive = ctypes.c_int()
+        st = self._stream()
+        _lib.call("D4.7.GPU", "gf_outflow_mark_sync", _lib.ptr(pos), n, _lib.ptr(boxes),
+                  len(active), _lib.ptr(alive), _lib.ptr(prefix), _lib.ptr(part),
+                  ctypes.byref(n_alive), st)
+        m = n_alive.value
+        if m == n:
+            return n
+
+        def compact(t, width):
+            out = torch.empty((m, width) if width > 1 else (m,), dtype=torch.float32, device=dev)
+            _lib.call("D4.7.GPU", "gf_compact_rows", _lib.ptr(t.contiguous()), width, n,
+                      _lib.ptr(alive), _lib.ptr(prefix), _lib.ptr(out), st)
+            return out
+
+        vel = self.vel.t
+        col = None if self._attr["color"] is None else compact(self._attr["color"], 3)
+        temp = None if self._attr["temperature"] is None else compact(self._attr["temperature"], 1)
+        self.set_particles(compact(pos, 3), compact(vel, 3), color=col, temperature=temp)
+        return m
 
     def has_diverged(self) -> bool:
         """NaN probe on particle positions (solver3d.py:2642-2656)."""

@@ -265,6 +265,39 @@ int gf_csf_apply(float* u, float* v, float* w, const int32_t* marker, int nx, in
                  float dx, float coeff, int smoothing_passes, float* work, double* red,
                  void* stream);
 
+/* ---- Particle-count-changing passes between frames (caller order) ------
+ * GPU reseed (S2.11.GPU, sim/reseed.py:161-396): gf_reseed_plan_sync bins
+ * the n positions (cell = clamp(int(p/dx))), counts per cell, keeps the
+ * first max_per_cell particles of each non-solid cell in index order (those
+ * in solid cells are dropped when drop_solid, else kept), and computes each
+ * cell's emission count max(min_per_cell - count, 0) over cells that are
+ * fluid in the marker or hold particles and are not solid.  counts_host[0]
+ * = particles kept, counts_host[1] = particles to emit (sync).
+ * gf_reseed_compact moves the kept rows of a (n, width) array to the front
+ * in index order; gf_reseed_emit writes the emitted positions, cells in flat
+ * order each repeated by its count, at (cell + 0.5 + jitter) * dx with
+ * jitter (n_emit, 3) from the caller.  ws: gf_reseed_workspace int32 entries,
+ * shared by the three calls. */
+int gf_reseed_workspace(int n, int nx, int ny, int nz, int64_t* ws_ints);
+int gf_reseed_plan_sync(const float* pos, int n, const int32_t* marker, int nx, int ny, int nz,
+                        float dx, int min_per_cell, int max_per_cell, int drop_solid,
+                        int32_t* ws, int* counts_host, void* stream);
+int gf_reseed_compact(const float* in, int width, int n, int nx, int ny, int nz,
+                      const int32_t* ws, float* out, void* stream);
+int gf_reseed_emit(int n, int nx, int ny, int nz, float dx, const int32_t* ws,
+                   const float* jitter, float* out, void* stream);
+
+/* Outflow compaction (D4.7.GPU, k3_mark_alive_outflows / k3_scatter_alive,
+ * solver3d.py:1946-1982, driver 2659-2730): alive[i] = 0 for particles inside
+ * any of the n_box boxes (lo x,y,z, hi x,y,z; bounds inclusive), prefix =
+ * exclusive scan of alive (n+1 entries), part = ceil(n/4096)+1 scratch;
+ * *n_alive_host = survivors (sync).  gf_compact_rows then moves the alive
+ * rows of a (n, width) array to prefix[i]. */
+int gf_outflow_mark_sync(const float* pos, int n, const float* boxes, int n_box, int32_t* alive,
+                         int32_t* prefix, int32_t* part, int* n_alive_host, void* stream);
+int gf_compact_rows(const float* in, int width, int n, const int32_t* alive,
+                    const int32_t* prefix, float* out, void* stream);
+
 /* Per-particle attribute transfer at the end of a substep (S2.15 colour,
  * solver3d.py:3333-3359 with k3_p2g_color / k3_normalize_color /
  * k3_g2p_color 562-664; S2.18 scalar, 3304-3330 with k3_p2g_scalar /
