@@ -150,6 +150,8 @@ def main():
                     help="skip the viscosity leg (3 viscous substeps on the scene)")
     ap.add_argument("--no-surface-tension", action="store_true",
                     help="skip the surface-tension leg (3 substeps with CSF on the scene)")
+    ap.add_argument("--no-reseed", action="store_true",
+                    help="skip the GPU reseed leg (one reseed of the scene's particles)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (3 launches) so that profiler "
                          "byte counters can be scaled against a kernel of known traffic")
@@ -302,6 +304,11 @@ def main():
         visc = viscosity_leg(s, one_step)
         log(f"viscosity: {visc}")
 
+    rsd = None
+    if not args.no_reseed:
+        rsd = reseed_leg(s)
+        log(f"reseed: {rsd}")
+
     csf = None
     if not args.no_surface_tension:
         csf = surface_tension_leg(s, one_step)
@@ -351,6 +358,7 @@ def main():
         "attributes": attrs,
         "viscosity": visc,
         "surface_tension": csf,
+        "reseed": rsd,
         "adjoint": adj,
         "batched": bat,
     }
@@ -439,6 +447,30 @@ def surface_tension_leg(s, one_step, steps=3, sigma=0.07, passes=2):
             "ms_per_step": round(ms, 3), "surface_tension_ms": round(cms, 4),
             "stage_bytes": b, "GBps": round(b / (cms * 1e-3) / 1e9, 1) if cms else None,
             "timing": "wall clock around the substeps; stages by HIP events (step profiler)"}
+
+
+def reseed_leg(s, repeats=3):
+    """One GPU reseed (S2.11.GPU: count, rank-cull, compact, emit; default
+    bounds 4..16 per cell) of the scene's particles after the timed steps,
+    wall clock around the call (it synchronises once to size the output)."""
+    import numpy as np
+    import torch
+    from gpufluid_amd.reseed import ReseedConfig, reseed_particles_gpu
+    pos, vel, marker = s.pos, s.vel, s.marker
+    cfg = ReseedConfig()
+    out = reseed_particles_gpu(pos, vel, marker, s.dx, cfg, np.random.default_rng(0))
+    times = []
+    for r in range(repeats):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = reseed_particles_gpu(pos, vel, marker, s.dx, cfg, np.random.default_rng(r))
+        torch.cuda.synchronize()
+        times.append((time.perf_counter() - t0) * 1e3)
+    ms = float(np.median(times))
+    n = s.n_particles
+    return {"particles": n, "emitted": out[4], "culled": out[5], "ms": round(ms, 3),
+            "Mparticles_per_s": round(n / (ms * 1e-3) / 1e6, 1),
+            "timing": f"wall clock, median of {repeats}"}
 
 
 def adjoint_leg():

@@ -86,9 +86,9 @@ class FlipSolver3D:
 
     Parameters follow solver3d.py:2358-2371; viscosity (S2.13) and surface
     tension (S2.14) run between P2G and the pressure solve as in the
-    reference; sub-dense storage raises NotImplementedError (the pressure
-    storage is compacted to active tiles instead).  Extra keyword-only
-    parameters:
+    reference; enable_sub_dense is accepted and keeps the dense layout (the
+    pressure storage is compacted to active tiles instead).  Extra
+    keyword-only parameters:
     pcg_precision ("mixed" | "fp64"), pcg_sign ("physical" | "reference"),
     pcg_tol (reference default 1e-4), pcg_check_every (host readback cadence).
     """
@@ -102,9 +102,6 @@ class FlipSolver3D:
                  sub_rebuild_every: int = 8, sub_dilation: int = 4, *,
                  pcg_precision: str = "mixed", pcg_sign: str = "physical",
                  pcg_tol: float = 1e-4, pcg_check_every: int = 8):
-        if enable_sub_dense:
-            raise NotImplementedError("sub-dense storage (F3.7) is not part of this build; "
-                                      "pressure storage is already compacted to active tiles")
         if transfer_mode not in _TRANSFER:
             raise ValueError(f"unknown transfer_mode: {transfer_mode!r}")
         if pcg_precision not in ("mixed", "fp64"):
@@ -126,6 +123,15 @@ class FlipSolver3D:
         self.surface_tension = surface_tension
         self.csf_smoothing_passes = max(1, int(csf_smoothing_passes))
         self.transfer_mode = transfer_mode
+        # Sub-dense storage (F3.7) shrinks the reference's dense arrays to the
+        # fluid's dilated bounding box; here the grid fields stay dense (HBM is
+        # not the constraint) and the pressure solve already runs on the
+        # compacted active tiles, so the flag is accepted and changes nothing.
+        self._enable_sub_dense = bool(enable_sub_dense)
+        self.sub_rebuild_every = int(sub_rebuild_every)
+        self.sub_dilation = int(sub_dilation)
+        self._sub_offset = (0, 0, 0)
+        self._sub_shape = (int(nx), int(ny), int(nz))
         self.pcg_precision = pcg_precision
         self.pcg_sign = pcg_sign
         self.pcg_tol = float(pcg_tol)
