@@ -102,6 +102,7 @@ SIGNATURES = {
     "gf_scene_pcg_enqueue": [SP, I, I, D, I, I, I, I, I, V],
     "gf_scene_g2p": [SP, V, I, F, F, F, F, V],
     "gf_scene_attr_transfer": [SP, V, I, I, V, V, V, V],
+    "gf_anim_obstacles": [V, I, V, V, V, V, V, I, I, I, D, V],
     "gf_reseed_workspace": [I, I, I, I, V],
     "gf_reseed_plan_sync": [V, I, V, I, I, I, F, I, I, I, V, PI, V],
     "gf_reseed_compact": [V, I, I, I, I, I, V, V, V],

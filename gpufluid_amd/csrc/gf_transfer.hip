@@ -39,6 +39,9 @@ template <> struct P2GDims<1> { static constexpr int DI = 8, DJ = 9, DK = 8; };
 template <> struct P2GDims<2> { static constexpr int DI = 8, DJ = 8, DK = 9; };
 
 constexpr int P2G_REGION = 576;
+#ifndef P2G_BATCH
+#define P2G_BATCH 4
+#endif
 
 // Trilinear scatter of one velocity component into the tile's LDS region
 // with LDS atomics (gridmath.py:71-95; out-of-grid corners dropped; corners
@@ -261,30 +264,35 @@ __device__ __forceinline__ void p2g_component(const P2GParams& a, float* s_val, 
         (&A.val[0][0][0])[q] = 0.0f;
         (&A.wt[0][0][0])[q] = 0.0f;
     }
-    // the next particle's loads are issued before the current one's math
-    float npx = 0.f, npy = 0.f, npz = 0.f, nv = 0.f;
-    if (p0 < p1) {
-        npx = a.pos[3 * p0 + 0]; npy = a.pos[3 * p0 + 1]; npz = a.pos[3 * p0 + 2];
-        nv = a.vel[3 * p0 + COMP];
-    }
-    for (int p = p0; p < p1; ++p) {
-        const float px = npx, py = npy, pz = npz, vp = nv;
-        if (p + 1 < p1) {
-            npx = a.pos[3 * p + 3]; npy = a.pos[3 * p + 4]; npz = a.pos[3 * p + 5];
-            nv = a.vel[3 * p + 3 + COMP];
+    // The cell's particles go through in batches of P2G_BATCH: a batch's
+    // loads are all issued before its math, so several are in flight.
+    for (int pb = p0; pb < p1; pb += P2G_BATCH) {
+        float bx[P2G_BATCH], by[P2G_BATCH], bz[P2G_BATCH], bv[P2G_BATCH];
+        float bc[APIC ? P2G_BATCH : 1][3];
+#pragma unroll
+        for (int b = 0; b < P2G_BATCH; ++b) {
+            const int p = pb + b < p1 ? pb + b : p1 - 1;
+            bx[b] = a.pos[3 * p + 0]; by[b] = a.pos[3 * p + 1]; bz[b] = a.pos[3 * p + 2];
+            bv[b] = a.vel[3 * p + COMP];
+            if (APIC) {
+                bc[APIC ? b : 0][0] = a.C[9 * p + 3 * COMP + 0];
+                bc[APIC ? b : 0][1] = a.C[9 * p + 3 * COMP + 1];
+                bc[APIC ? b : 0][2] = a.C[9 * p + 3 * COMP + 2];
+            }
         }
-        float crow[3];
-        if (APIC) {
-            crow[0] = a.C[9 * p + 3 * COMP + 0];
-            crow[1] = a.C[9 * p + 3 * COMP + 1];
-            crow[2] = a.C[9 * p + 3 * COMP + 2];
+#pragma unroll
+        for (int b = 0; b < P2G_BATCH; ++b) {
+            if (pb + b >= p1) break;
+            const float px = bx[b], py = by[b], pz = bz[b], vp = bv[b];
+            const float* crow = bc[APIC ? b : 0];
+            const float qx = px / dx, qy = py / dx, qz = pz / dx;
+            // corners outside the cell stencil (particles outside the domain)
+            // are scattered with LDS atomics
+            if (!accumulate<COMP, APIC>(A, ci, cj, ck, px, py, pz, qx, qy, qz, vp, crow, dx))
+                scatter_comp<COMP, APIC>(s_val, s_wt, px, py, pz, qx, qy, qz, vp, crow, i0, j0,
+                                         k0, nx + (COMP == 0), ny + (COMP == 1), nz + (COMP == 2),
+                                         dx);
         }
-        const float qx = px / dx, qy = py / dx, qz = pz / dx;
-        // corners outside the cell stencil (particles outside the domain) are
-        // scattered with LDS atomics
-        if (!accumulate<COMP, APIC>(A, ci, cj, ck, px, py, pz, qx, qy, qz, vp, crow, dx))
-            scatter_comp<COMP, APIC>(s_val, s_wt, px, py, pz, qx, qy, qz, vp, crow, i0, j0, k0,
-                                     nx + (COMP == 0), ny + (COMP == 1), nz + (COMP == 2), dx);
     }
     for (int ph = r_lo; ph <= r_hi; ++ph) {
         __syncthreads();

@@ -185,6 +185,7 @@ class FlipSolver3D:
         self.outflows: list = []
         self._rng = np.random.default_rng(0)
         self._frame_events = FrameEventQueue()
+        self._anim_specs: list = []
 
     # ------------------------------------------------------------ attributes
     def _arr(self, name):
@@ -507,15 +508,76 @@ class FlipSolver3D:
     def add_outflow(self, region):
         self.outflows.append(region)
 
-    @block("F3.6", "Per-frame hook: emit inflow particles, drop outflow particles")
+    @block("D4.6", "Register an analytic obstacle that moves per frame")
+    def add_animated_obstacle(self, kind: str, base_center, motion, **params):
+        """kind: 'sphere' (radius) | 'box' (half_size) | 'cylinder_y' (radius,
+        half_height), moving by a LinearMotion / KeyframeMotion
+        (solver3d.py:2516-2519).  Mesh obstacles are not part of this build."""
+        if kind not in ("sphere", "box", "cylinder_y"):
+            if kind == "mesh":
+                raise NotImplementedError("animated mesh obstacles are not part of this build")
+            raise ValueError(f"unsupported animated obstacle kind: {kind!r}")
+        self._anim_specs.append({"kind": kind, "base_center": tuple(base_center),
+                                 "motion": motion, "params": dict(params)})
+
+    def _obstacle_velocity_at(self, spec, frame_idx: int):
+        """Linear motion: its velocity; keyframes: centred difference of the
+        centre over one frame at 24 fps (solver3d.py:2521-2535)."""
+        from .animation import evaluate_center
+        m = spec.get("motion")
+        if m is None:
+            return (0.0, 0.0, 0.0)
+        if getattr(m, "kind", None) == "linear":
+            return tuple(float(c) for c in m.velocity)
+        if getattr(m, "kind", None) == "keyframes":
+            c1 = evaluate_center(spec["base_center"], m, frame_idx + 1)
+            c0 = evaluate_center(spec["base_center"], m, frame_idx - 1)
+            fps = m.fps if hasattr(m, "fps") else 24
+            return tuple(float(c) for c in (c1 - c0) * fps / 2.0)
+        return (0.0, 0.0, 0.0)
+
+    def _stamp_animated_obstacles(self, frame_idx: int):
+        """Marker = walls + static obstacles + this frame's animated
+        obstacles, and their velocities on the solid faces, on the device
+        (solver3d.py:3918-3940)."""
+        from .animation import evaluate_center
+        base = self._wall_marker.copy()
+        for sdf in self._static_obstacle_sdfs:
+            base[sdf <= 0.0] = 2
+        rows = []
+        kinds = {"sphere": 0, "box": 1, "cylinder_y": 2}
+        for spec in self._anim_specs:
+            c = evaluate_center(spec["base_center"], spec["motion"], frame_idx)
+            p = spec["params"]
+            if spec["kind"] == "sphere":
+                par = (p["radius"], 0.0, 0.0)
+            elif spec["kind"] == "box":
+                par = tuple(np.broadcast_to(np.asarray(p["half_size"], np.float32), (3,)))
+            else:
+                par = (p["radius"], p["half_height"], 0.0)
+            vel = self._obstacle_velocity_at(spec, frame_idx)
+            rows.append([kinds[spec["kind"]], *[float(x) for x in c], *map(float, par), *vel])
+        dev = self.device
+        obs = torch.tensor(np.asarray(rows, np.float32).reshape(-1, 10), device=dev)
+        base_t = torch.as_tensor(base, device=dev)
+        f = self._f
+        _lib.call("D4.6", "gf_anim_obstacles", _lib.ptr(obs), len(rows), _lib.ptr(base_t),
+                  _lib.ptr(f["marker"]), _lib.ptr(f["solid_u"]), _lib.ptr(f["solid_v"]),
+                  _lib.ptr(f["solid_w"]), self.nx, self.ny, self.nz, float(self.dx),
+                  self._stream())
+        self._marker_host = f["marker"].cpu().numpy()
+
+    @block("F3.6", "Per-frame hook: rebuild marker for anim obstacles, emit inflow particles, "
+                   "drop outflow particles")
     def prepare_frame(self, frame_idx: int, frame_dt: float):
-        """The inflow/outflow part of prepare_frame (solver3d.py:3911-4072):
-        regions publish this frame's events; outflow boxes are compacted away
-        on the device first, then the emitted particles (minus those landing
-        in a solid cell of the host marker) are appended with their event's
-        colour (else 0) and temperature (else 20) when the attribute is
-        tracked.  Animated obstacles and sub-dense storage are not part of
-        this build."""
+        """prepare_frame (solver3d.py:3911-4072): animated obstacles are
+        stamped into the marker and the solid face velocities; then regions
+        publish this frame's events; outflow boxes are compacted away on the
+        device first, then the emitted particles (minus those landing in a
+        solid cell of the host marker) are appended with their event's colour
+        (else 0) and temperature (else 20) when the attribute is tracked."""
+        if self._anim_specs:
+            self._stamp_animated_obstacles(frame_idx)
         q = self._frame_events
         q.clear()
         for inflow in self.inflows:

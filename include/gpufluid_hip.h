@@ -298,6 +298,19 @@ int gf_outflow_mark_sync(const float* pos, int n, const float* boxes, int n_box,
 int gf_compact_rows(const float* in, int width, int n, const int32_t* alive,
                     const int32_t* prefix, float* out, void* stream);
 
+/* Analytic animated obstacles for the per-frame hook (D4.6 in prepare_frame,
+ * solver3d.py:3918-3940 with _build_anim_sdf_per_obstacle /
+ * _stamp_solid_vel_from_sdf 2537-2596, SDFs primitives/sdf.py:40-66).
+ * obs: n_obs (<= 64) rows of 10 floats: kind (0 sphere, 1 box, 2 cylinder
+ * along y), centre x,y,z, params (sphere: radius; box: half sizes x,y,z;
+ * cylinder: radius, half height), velocity x,y,z.  marker = base_marker with
+ * every cell whose SDF at the FP32 cell centre is <= 0 set solid; every face
+ * of solid_u/v/w gets the velocity of the last obstacle (list order) that
+ * contains one of its two cells, else 0.  dx is the exact cell size. */
+int gf_anim_obstacles(const float* obs, int n_obs, const int32_t* base_marker, int32_t* marker,
+                      float* solid_u, float* solid_v, float* solid_w, int nx, int ny, int nz,
+                      double dx, void* stream);
+
 /* Per-particle attribute transfer at the end of a substep (S2.15 colour,
  * solver3d.py:3333-3359 with k3_p2g_color / k3_normalize_color /
  * k3_g2p_color 562-664; S2.18 scalar, 3304-3330 with k3_p2g_scalar /

@@ -8,11 +8,17 @@ the HIP path fixes it: an over-full cell keeps its first ``max_per_cell``
 particles in index order (the reference keeps its atomics' first arrivals on
 the GPU and ``rng.choice`` picks on the host).
 
+Also the analytic animated obstacles of prepare_frame (solver3d.py:3918-3940,
+2537-2596, SDFs primitives/sdf.py:30-66): marker stamping and the per-
+obstacle face-velocity writes of k3_write_solid_face_vel (solver3d.py:818-836).
+
 How each part is pinned (tests/test_oracle_frame.py): a hand-worked 3-cell
 case; invariants (every fluid cell ends with at least min_per_cell
 particles, survivors keep their order, emitted particles lie in their cell,
 counts add up); a per-particle loop for the kept set; the outflow mask
-against a per-particle loop with boundary points.
+against a per-particle loop with boundary points; obstacle cells against an
+FP64 per-cell distance loop, face velocities against a per-face rule
+(last obstacle containing either cell).
 """
 from __future__ import annotations
 
@@ -98,3 +104,50 @@ def outflow_alive(pos, boxes):
         hi = np.asarray(hi, np.float32)
         alive &= ~np.all((p >= lo) & (p <= hi), axis=1)
     return alive
+
+
+def cell_centers(shape, dx):
+    """FP32 cell centres (primitives/sdf.py:30-37)."""
+    axes = [((np.arange(n) + 0.5) * dx) for n in shape]
+    X, Y, Z = np.meshgrid(*axes, indexing="ij")
+    return np.stack([X, Y, Z], axis=-1).astype(np.float32)
+
+
+def obstacle_sdf(grid, kind, center, params):
+    """sdf_sphere / sdf_box / sdf_cylinder_y (primitives/sdf.py:40-66)."""
+    c = np.asarray(center, dtype=np.float32)
+    if kind == "sphere":
+        return np.linalg.norm(grid - c, axis=-1) - float(params["radius"])
+    if kind == "box":
+        h = np.asarray(params["half_size"], dtype=np.float32)
+        d = np.abs(grid - c) - h
+        return np.linalg.norm(np.maximum(d, 0.0), axis=-1) + np.minimum(np.max(d, axis=-1), 0.0)
+    local = grid - c
+    dr = np.sqrt(local[..., 0] ** 2 + local[..., 2] ** 2) - float(params["radius"])
+    da = np.abs(local[..., 1]) - float(params["half_height"])
+    return (np.sqrt(np.maximum(dr, 0.0) ** 2 + np.maximum(da, 0.0) ** 2)
+            + np.minimum(np.maximum(dr, da), 0.0))
+
+
+def stamp_obstacles(base_marker, dx, obstacles):
+    """obstacles: [(kind, centre, params, velocity)] in list order.  Returns
+    (marker, solid_u, solid_v, solid_w): cells with sdf <= 0 solid, faces
+    cleared then written obstacle by obstacle around their inside cells
+    (k3_clear_solid_vel / k3_write_solid_face_vel, solver3d.py:803-836)."""
+    marker = np.array(base_marker, copy=True)
+    nx, ny, nz = marker.shape
+    su = np.zeros((nx + 1, ny, nz), np.float32)
+    sv = np.zeros((nx, ny + 1, nz), np.float32)
+    sw = np.zeros((nx, ny, nz + 1), np.float32)
+    grid = cell_centers(marker.shape, dx)
+    for kind, center, params, vel in obstacles:
+        inside = obstacle_sdf(grid, kind, center, params) <= 0.0
+        marker[inside] = 2
+        v = np.asarray(vel, np.float32)
+        su[:-1][inside] = v[0]
+        su[1:][inside] = v[0]
+        sv[:, :-1][inside] = v[1]
+        sv[:, 1:][inside] = v[1]
+        sw[:, :, :-1][inside] = v[2]
+        sw[:, :, 1:][inside] = v[2]
+    return marker, su, sv, sw

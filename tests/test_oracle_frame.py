@@ -96,3 +96,36 @@ def test_outflow_mask_matches_loop():
                      for lo, hi in boxes)
         assert alive[i] == (not inside)
     assert not alive[0] and not alive[1]
+
+
+def test_obstacle_cells_against_fp64_distance_loop():
+    shape, dx = (12, 10, 11), 0.1
+    base = np.zeros(shape, np.int32)
+    obs = [("sphere", (0.52, 0.47, 0.55), {"radius": 0.27}, (1.0, 0.0, 0.0)),
+           ("box", (0.3, 0.2, 0.3), {"half_size": (0.12, 0.07, 0.21)}, (0.0, 2.0, 0.0)),
+           ("cylinder_y", (0.8, 0.5, 0.5), {"radius": 0.16, "half_height": 0.31}, (0, 0, -1.5))]
+    m, su, sv, sw = fr.stamp_obstacles(base, dx, obs)
+    for i in range(shape[0]):
+        for j in range(shape[1]):
+            for k in range(shape[2]):
+                x, y, z = (i + 0.5) * dx, (j + 0.5) * dx, (k + 0.5) * dx
+                ins = []
+                (cx, cy, cz) = obs[0][1]
+                ins.append((x - cx) ** 2 + (y - cy) ** 2 + (z - cz) ** 2 <= 0.27 ** 2)
+                (cx, cy, cz), h = obs[1][1], obs[1][2]["half_size"]
+                ins.append(abs(x - cx) <= h[0] and abs(y - cy) <= h[1] and abs(z - cz) <= h[2])
+                (cx, cy, cz) = obs[2][1]
+                ins.append((x - cx) ** 2 + (z - cz) ** 2 <= 0.16 ** 2 and abs(y - cy) <= 0.31)
+                assert (m[i, j, k] == 2) == any(ins), (i, j, k)
+    # a face takes the velocity of the last obstacle containing one of its cells
+    cells = [fr.obstacle_sdf(fr.cell_centers(shape, dx), kd, c, p) <= 0 for kd, c, p, _ in obs]
+    for i in range(shape[0] + 1):
+        for j in range(shape[1]):
+            for k in range(shape[2]):
+                want = 0.0
+                for b, (_, _, _, v) in enumerate(obs):
+                    lo = i > 0 and cells[b][i - 1, j, k]
+                    hi = i < shape[0] and cells[b][i, j, k]
+                    if lo or hi:
+                        want = v[0]
+                assert su[i, j, k] == np.float32(want)
