@@ -94,6 +94,8 @@ SIGNATURES = {
     "gf_scene_sizes": [I, I, I, POINTER(GfSizes)],
     "gf_scene_sort_sync": [SP, I, PI, V],
     "gf_scene_p2g": [SP, I, F, F, F, V],
+    "gf_scene_p2g_stage_size": [I, I, I, V, V],
+    "gf_scene_p2g_staged": [SP, I, F, F, F, V, V, V],
     "gf_scene_p2g_raw": [SP, I, V, V, V, V],
     "gf_scene_jacobi": [SP, I, I, V],
     "gf_scene_gsrb": [SP, I, I, V],

@@ -183,7 +183,12 @@ class BatchedScenes:
         _lib.call("X2.3", "gf_scene_sort_sync", ctypes.byref(d), 0, None, st)
         sc.swap()
         d = s._descriptor()
-        _lib.call("X2.3", "gf_scene_p2g", ctypes.byref(d), 0, s.gravity, self.dt, s.rho, st)
+        if s.p2g_staged:
+            stage, has = s._p2g_stage()
+            _lib.call("X2.3", "gf_scene_p2g_staged", ctypes.byref(d), 0, s.gravity, self.dt,
+                      s.rho, _lib.ptr(stage), _lib.ptr(has), st)
+        else:
+            _lib.call("X2.3", "gf_scene_p2g", ctypes.byref(d), 0, s.gravity, self.dt, s.rho, st)
         prec = _lib.GF_PCG_MIXED if s.pcg_precision == "mixed" else _lib.GF_PCG_FP64
         _lib.call("X2.3", "gf_scene_pcg_enqueue", ctypes.byref(d), 0, prec, pressure_tol,
                   pressure_iters, s.pcg_check_every, -1, 0, budget, st)

@@ -39,6 +39,16 @@ template <> struct P2GDims<1> { static constexpr int DI = 8, DJ = 9, DK = 8; };
 template <> struct P2GDims<2> { static constexpr int DI = 8, DJ = 8, DK = 9; };
 
 constexpr int P2G_REGION = 576;
+
+// Face region the tile's own particles reach (staged P2G): 9 entries along
+// the component's own axis from the tile origin, 10 along the others from
+// one below it.
+template <int COMP> struct OwnDims;
+template <> struct OwnDims<0> { static constexpr int DI = 9, DJ = 10, DK = 10; };
+template <> struct OwnDims<1> { static constexpr int DI = 10, DJ = 9, DK = 10; };
+template <> struct OwnDims<2> { static constexpr int DI = 10, DJ = 10, DK = 9; };
+constexpr int OWN_REGION = 900;
+constexpr int STAGE_PER_TILE = 3 * 2 * OWN_REGION;  // val and wt per component
 #ifndef P2G_BATCH
 #define P2G_BATCH 4
 #endif
@@ -48,13 +58,12 @@ constexpr int P2G_REGION = 576;
 // outside the tile's region are finalised by another tile and skipped).
 // Used only for particles whose corners fall outside their cell's stencil
 // (particles outside the domain).
-template <int COMP, bool APIC>
+template <int COMP, bool APIC, class D>
 __device__ __forceinline__ void scatter_comp(float* s_val, float* s_wt, float px, float py,
                                              float pz, float qx, float qy, float qz,
                                              float vp, const float* crow, int i0t,
                                              int j0t, int k0t, int nfi, int nfj,
                                              int nfk, float dx) {
-    using D = P2GDims<COMP>;
     float cx = qx, cy = qy, cz = qz;
     if (COMP != 0) cx = qx - 0.5f;
     if (COMP != 1) cy = qy - 0.5f;
@@ -164,12 +173,11 @@ __device__ __forceinline__ bool accumulate(CellAcc<COMP>& A, int ci, int cj, int
 
 // Adds a cell's stencil into the tile's LDS face region (plain read-modify-
 // write; the caller guarantees no concurrent writer of the same entries).
-template <int COMP>
+template <int COMP, class D>
 __device__ __forceinline__ void flush(const CellAcc<COMP>& A, float* s_val, float* s_wt, int ci,
                                       int cj, int ck, int i0t, int j0t, int k0t, int nx, int ny,
                                       int nz) {
     using C = CellAcc<COMP>;
-    using D = P2GDims<COMP>;
     const int nfi = nx + (COMP == 0), nfj = ny + (COMP == 1), nfk = nz + (COMP == 2);
     const int fi0 = COMP == 0 ? ci : ci - 1, fj0 = COMP == 1 ? cj : cj - 1,
               fk0 = COMP == 2 ? ck : ck - 1;
@@ -252,7 +260,7 @@ __device__ __forceinline__ RowSet build_rows(const int32_t* __restrict__ cell_st
 // One component of the lane-per-cell P2G: accumulate the cell's stencil
 // over particles [p0, p1) in registers, then add it to the LDS region in the
 // flush phases r_lo..r_hi (phase = cell class; cls < 0 for empty cells).
-template <int COMP, bool APIC>
+template <int COMP, bool APIC, class D = P2GDims<COMP>>
 __device__ __forceinline__ void p2g_component(const P2GParams& a, float* s_val, float* s_wt,
                                               int ci, int cj, int ck, int p0, int p1, int cls,
                                               int r_lo, int r_hi, int i0, int j0, int k0) {
@@ -289,14 +297,14 @@ __device__ __forceinline__ void p2g_component(const P2GParams& a, float* s_val, 
             // corners outside the cell stencil (particles outside the domain)
             // are scattered with LDS atomics
             if (!accumulate<COMP, APIC>(A, ci, cj, ck, px, py, pz, qx, qy, qz, vp, crow, dx))
-                scatter_comp<COMP, APIC>(s_val, s_wt, px, py, pz, qx, qy, qz, vp, crow, i0, j0,
-                                         k0, nx + (COMP == 0), ny + (COMP == 1), nz + (COMP == 2),
-                                         dx);
+                scatter_comp<COMP, APIC, D>(s_val, s_wt, px, py, pz, qx, qy, qz, vp, crow, i0,
+                                            j0, k0, nx + (COMP == 0), ny + (COMP == 1),
+                                            nz + (COMP == 2), dx);
         }
     }
     for (int ph = r_lo; ph <= r_hi; ++ph) {
         __syncthreads();
-        if (cls == ph) flush<COMP>(A, s_val, s_wt, ci, cj, ck, i0, j0, k0, nx, ny, nz);
+        if (cls == ph) flush<COMP, D>(A, s_val, s_wt, ci, cj, ck, i0, j0, k0, nx, ny, nz);
     }
     __syncthreads();
 }
@@ -441,6 +449,212 @@ __global__ __launch_bounds__(GF_WG) void k_p2g_tile(P2GParams a) {
     __syncthreads();
 
     // Divergence of the tile's cells (846-871) into the compact rhs.
+    const int slot = a.tile_map[t];
+    if (slot < 0) return;
+    const float scale = a.rho * a.dx / a.dt;
+    const int lj = (threadIdx.x >> 3) & 7, lk = threadIdx.x & 7;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int li = (threadIdx.x >> 6) + 4 * h;
+        const int l = (li * 8 + lj) * 8 + lk;
+        float dv = 0.0f;
+        if (i0 + li < nx && j0 + lj < ny && k0 + lk < nz && s_mk[li + 1][lj + 1][lk + 1] == 1) {
+            const float u0 = s_val[0][(li * 8 + lj) * 8 + lk];
+            const float u1 = s_val[0][((li + 1) * 8 + lj) * 8 + lk];
+            const float v0 = s_val[1][(li * 9 + lj) * 8 + lk];
+            const float v1 = s_val[1][(li * 9 + lj + 1) * 8 + lk];
+            const float w0 = s_val[2][(li * 8 + lj) * 9 + lk];
+            const float w1 = s_val[2][(li * 8 + lj) * 9 + lk + 1];
+            const float d = u1 - u0 + v1 - v0 + w1 - w0;
+            dv = d * scale;
+        }
+        a.b[(size_t)slot * GF_TILE_CELLS + l] = dv;
+    }
+}
+
+// ------------------------------------------------ staged P2G (own cells) ----
+// The same arithmetic in two launches.  k_p2g_own: each tile scatters only
+// the particles of its own 512 cells (each particle is visited once, not by
+// every tile whose halo holds it) into the LDS region those particles reach,
+// and stores that region in `stage`.  k_p2g_finish: each tile sums, for every
+// face it finalises, the staged partials of the tiles whose regions hold the
+// face (at most two per axis) in a fixed order, then normalises, adds gravity,
+// applies the BC and forms the divergence as k_p2g_tile does.
+template <bool APIC>
+__global__ __launch_bounds__(GF_WG) void k_p2g_own(P2GParams a, float* __restrict__ stage,
+                                                   int32_t* __restrict__ has) {
+    __shared__ float s_val[3][OWN_REGION];
+    __shared__ float s_wt[3][OWN_REGION];
+    __shared__ int s_cls[28];
+    __shared__ int s_start[64], s_pref[64];
+    __shared__ int s_wave[GF_WG / GF_WAVE + 1];
+
+    const int ntiles = a.tnx * a.tny * a.tnz;
+    const int t = gf::xcd_remap(blockIdx.x, ntiles);
+    const int tk = t % a.tnz, tj = (t / a.tnz) % a.tny, ti = t / (a.tnz * a.tny);
+    const int i0 = ti * 8, j0 = tj * 8, k0 = tk * 8;
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int ni = min(i0 + 8, nx) - i0, nj = min(j0 + 8, ny) - j0, nk = min(k0 + 8, nz) - k0;
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int r = 0; r < 27; ++r) {
+            const int ri = r / 9, rj = (r / 3) % 3, rk = r % 3;
+            s_cls[r] = acc;
+            acc += max(0, (ni - ri + 2) / 3) * max(0, (nj - rj + 2) / 3) * max(0, (nk - rk + 2) / 3);
+        }
+        s_cls[27] = acc;
+    }
+    const RowSet rs = build_rows(a.cell_start, ny, nz, i0, i0 + ni, j0, j0 + nj, k0, k0 + nk,
+                                 s_start, s_pref, s_wave);
+    if (threadIdx.x == 0) has[t] = rs.total > 0 ? 1 : 0;
+    if (rs.total == 0) return;
+    for (int i = threadIdx.x; i < 3 * OWN_REGION; i += GF_WG) {
+        (&s_val[0][0])[i] = 0.0f;
+        (&s_wt[0][0])[i] = 0.0f;
+    }
+    __syncthreads();
+    const int ncell = ni * nj * nk;
+    const int nrounds = (ncell + GF_WG - 1) / GF_WG;
+    for (int round = 0; round < nrounds; ++round) {
+        const int c = round * GF_WG + threadIdx.x;
+        int ci = 0, cj = 0, ck = 0, cls = -1, p0 = 0, p1 = 0;
+        if (c < ncell) {
+            int r = 0;
+            while (r < 26 && s_cls[r + 1] <= c) ++r;
+            const int ri = r / 9, rj = (r / 3) % 3, rk = r % 3;
+            const int mj = max(0, (nj - rj + 2) / 3), mk = max(0, (nk - rk + 2) / 3);
+            const int loc = c - s_cls[r];
+            ck = k0 + rk + 3 * (loc % mk);
+            cj = j0 + rj + 3 * ((loc / mk) % mj);
+            ci = i0 + ri + 3 * (loc / (mk * mj));
+            const int key = (ci * ny + cj) * nz + ck;
+            p0 = a.cell_start[key];
+            p1 = a.cell_start[key + 1];
+            if (p0 < p1) cls = r;
+        }
+        const int last = min(ncell, (round + 1) * GF_WG) - 1;
+        int r_lo = 0, r_hi = 0;
+        while (r_lo < 26 && s_cls[r_lo + 1] <= round * GF_WG) ++r_lo;
+        while (r_hi < 26 && s_cls[r_hi + 1] <= last) ++r_hi;
+        p2g_component<0, APIC, OwnDims<0>>(a, s_val[0], s_wt[0], ci, cj, ck, p0, p1, cls, r_lo,
+                                           r_hi, i0, j0 - 1, k0 - 1);
+        p2g_component<1, APIC, OwnDims<1>>(a, s_val[1], s_wt[1], ci, cj, ck, p0, p1, cls, r_lo,
+                                           r_hi, i0 - 1, j0, k0 - 1);
+        p2g_component<2, APIC, OwnDims<2>>(a, s_val[2], s_wt[2], ci, cj, ck, p0, p1, cls, r_lo,
+                                           r_hi, i0 - 1, j0 - 1, k0);
+    }
+    float* st = stage + (size_t)t * STAGE_PER_TILE;
+    for (int e = threadIdx.x; e < 3 * OWN_REGION; e += GF_WG) {
+        const int c = e / OWN_REGION, idx = e - c * OWN_REGION;
+        st[(size_t)c * 2 * OWN_REGION + idx] = s_val[c][idx];
+        st[(size_t)c * 2 * OWN_REGION + OWN_REGION + idx] = s_wt[c][idx];
+    }
+}
+
+// Whether tile offset d along an axis holds the face at local coordinate l
+// of this tile's finalised region (own axis: l in 0..8; others: 0..7).
+__device__ __forceinline__ bool holds(int d, int l, bool own) {
+    return d == 0 || (d < 0 ? l == 0 : l == (own ? 8 : 7));
+}
+
+__global__ __launch_bounds__(GF_WG) void k_p2g_finish(P2GParams a,
+                                                      const float* __restrict__ stage,
+                                                      const int32_t* __restrict__ has) {
+    __shared__ float s_val[3][P2G_REGION];
+    __shared__ int8_t s_mk[10][10][10];
+    __shared__ int s_has[27];
+
+    const int ntiles = a.tnx * a.tny * a.tnz;
+    const int t = gf::xcd_remap(blockIdx.x, ntiles);
+    const int tk = t % a.tnz, tj = (t / a.tnz) % a.tny, ti = t / (a.tnz * a.tny);
+    const int i0 = ti * 8, j0 = tj * 8, k0 = tk * 8;
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    for (int i = threadIdx.x; i < 1000; i += GF_WG) {
+        const int ca = i / 100, cb = (i / 10) % 10, cc = i % 10;
+        const int ci = i0 - 1 + ca, cj = j0 - 1 + cb, ck = k0 - 1 + cc;
+        int8_t m = 0;
+        if (ci >= 0 && ci < nx && cj >= 0 && cj < ny && ck >= 0 && ck < nz)
+            m = (int8_t)a.marker[(ci * ny + cj) * nz + ck];
+        s_mk[ca][cb][cc] = m;
+    }
+    if (threadIdx.x < 27) {
+        const int di = (int)threadIdx.x / 9 - 1, dj = ((int)threadIdx.x / 3) % 3 - 1,
+                  dk = (int)threadIdx.x % 3 - 1;
+        const int a_ = ti + di, b_ = tj + dj, c_ = tk + dk;
+        s_has[threadIdx.x] = (a_ >= 0 && a_ < a.tnx && b_ >= 0 && b_ < a.tny && c_ >= 0 &&
+                              c_ < a.tnz)
+                                 ? has[(a_ * a.tny + b_) * a.tnz + c_]
+                                 : 0;
+    }
+    __syncthreads();
+
+    const float gdt = a.tile_scene ? a.scene_par[8 * max(a.tile_scene[t], 0)] : a.g * a.dt;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const int DI = c == 0 ? 9 : 8, DJ = c == 1 ? 9 : 8, DK = c == 2 ? 9 : 8;
+        const int RJ = c == 1 ? 9 : 10, RK = c == 2 ? 9 : 10;  // staged region dims
+        const int fni = nx + (c == 0), fnj = ny + (c == 1), fnk = nz + (c == 2);
+        float* gf_ = c == 0 ? a.u : (c == 1 ? a.v : a.w);
+        for (int idx = threadIdx.x; idx < P2G_REGION; idx += GF_WG) {
+            const int lk = idx % DK, lj = (idx / DK) % DJ, li = idx / (DK * DJ);
+            const int ii = i0 + li, jj = j0 + lj, kk = k0 + lk;
+            if (ii >= fni || jj >= fnj || kk >= fnk) continue;
+            const bool owned = (c == 0 ? (li < 8 || ii == nx) : true) &&
+                               (c == 1 ? (lj < 8 || jj == ny) : true) &&
+                               (c == 2 ? (lk < 8 || kk == nz) : true);
+            const size_t f = ((size_t)ii * fnj + jj) * fnk + kk;
+            // partial sums of the tiles whose staged regions hold this face
+            float sv = 0.0f, sw = 0.0f;
+            for (int di = -1; di <= 1; ++di) {
+                if (!holds(di, li, c == 0)) continue;
+                for (int dj = -1; dj <= 1; ++dj) {
+                    if (!holds(dj, lj, c == 1)) continue;
+                    for (int dk = -1; dk <= 1; ++dk) {
+                        if (!holds(dk, lk, c == 2)) continue;
+                        if (!s_has[(di + 1) * 9 + (dj + 1) * 3 + (dk + 1)]) continue;
+                        const int tt = ((ti + di) * a.tny + (tj + dj)) * a.tnz + (tk + dk);
+                        // local coordinates in that tile's region (origin one
+                        // below its corner on the non-own axes)
+                        const int ri = ii - 8 * (ti + di) + (c != 0);
+                        const int rj = jj - 8 * (tj + dj) + (c != 1);
+                        const int rk = kk - 8 * (tk + dk) + (c != 2);
+                        const float* st = stage + (size_t)tt * STAGE_PER_TILE +
+                                          (size_t)c * 2 * OWN_REGION + (ri * RJ + rj) * RK + rk;
+                        sv += st[0];
+                        sw += st[OWN_REGION];
+                    }
+                }
+            }
+            float val = sw > 1.0e-8f ? sv / sw : 0.0f;
+            const float saved = val;
+            if (c == 1) val = val + gdt;
+            const int ca = li + 1, cb = lj + 1, cc = lk + 1;
+            bool lo_side, hi_side;
+            const float* solid;
+            if (c == 0) {
+                lo_side = ii == 0 || s_mk[ca - 1][cb][cc] == 2;
+                hi_side = ii == nx || s_mk[ca][cb][cc] == 2;
+                solid = a.su;
+            } else if (c == 1) {
+                lo_side = jj == 0 || s_mk[ca][cb - 1][cc] == 2;
+                hi_side = jj == ny || s_mk[ca][cb][cc] == 2;
+                solid = a.sv;
+            } else {
+                lo_side = kk == 0 || s_mk[ca][cb][cc - 1] == 2;
+                hi_side = kk == nz || s_mk[ca][cb][cc] == 2;
+                solid = a.sw;
+            }
+            if (lo_side || hi_side) val = solid ? solid[f] : 0.0f;
+            s_val[c][idx] = val;
+            if (owned) {
+                float* gs = c == 0 ? a.us : (c == 1 ? a.vs : a.ws);
+                gf_[f] = val;
+                gs[f] = saved;
+            }
+        }
+    }
+    __syncthreads();
+
     const int slot = a.tile_map[t];
     if (slot < 0) return;
     const float scale = a.rho * a.dx / a.dt;
@@ -894,6 +1108,42 @@ extern "C" int gf_scene_p2g(const gf_scene_t* s, int transfer_mode, float gravit
     else
         hipLaunchKernelGGL((k_p2g_tile<false, false>), dim3(ntiles), dim3(GF_WG), 0, st, a);
     GF_LAUNCH_CHECK("k_p2g_tile");
+    return 0;
+}
+
+extern "C" int gf_scene_p2g_stage_size(int nx, int ny, int nz, int64_t* stage_floats,
+                                       int64_t* flag_ints) {
+    GF_REQUIRE(nx > 0 && ny > 0 && nz > 0 && stage_floats && flag_ints,
+               "gf_scene_p2g_stage_size: bad arguments");
+    const int64_t ntiles = (int64_t)((nx + 7) / 8) * ((ny + 7) / 8) * ((nz + 7) / 8);
+    *stage_floats = ntiles * STAGE_PER_TILE;
+    *flag_ints = ntiles;
+    return 0;
+}
+
+extern "C" int gf_scene_p2g_staged(const gf_scene_t* s, int transfer_mode, float gravity,
+                                   float dt, float rho, float* stage, int32_t* has,
+                                   void* stream) {
+    int rc = gf::scene_check(s, "gf_scene_p2g_staged");
+    if (rc) return rc;
+    GF_REQUIRE(transfer_mode == GF_TRANSFER_FLIP || transfer_mode == GF_TRANSFER_APIC,
+               "gf_scene_p2g_staged: transfer_mode %d", transfer_mode);
+    GF_REQUIRE(s->u && s->v && s->w && s->us && s->vs && s->ws && s->b && s->marker &&
+                   s->cell_start && s->tile_map && stage && has,
+               "gf_scene_p2g_staged: buffers not set");
+    GF_REQUIRE(dt != 0.0f, "gf_scene_p2g_staged: dt must be nonzero");
+    if (s->n > 0) GF_REQUIRE(s->pos && s->vel, "gf_scene_p2g_staged: particles not set");
+    if (transfer_mode == GF_TRANSFER_APIC && s->n > 0)
+        GF_REQUIRE(s->affine_c, "gf_scene_p2g_staged: APIC needs affine_c");
+    P2GParams a = p2g_params(s, gravity, dt, rho);
+    const int ntiles = s->tnx * s->tny * s->tnz;
+    hipStream_t st = (hipStream_t)stream;
+    if (transfer_mode == GF_TRANSFER_APIC)
+        hipLaunchKernelGGL(k_p2g_own<true>, dim3(ntiles), dim3(GF_WG), 0, st, a, stage, has);
+    else
+        hipLaunchKernelGGL(k_p2g_own<false>, dim3(ntiles), dim3(GF_WG), 0, st, a, stage, has);
+    hipLaunchKernelGGL(k_p2g_finish, dim3(ntiles), dim3(GF_WG), 0, st, a, stage, has);
+    GF_LAUNCH_CHECK("k_p2g_own/k_p2g_finish");
     return 0;
 }
 

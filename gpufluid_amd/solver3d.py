@@ -180,6 +180,11 @@ class FlipSolver3D:
         self._attr = {"color": None, "temperature": None}
         self._attr_ws = None
         self._csf_ws = None
+        # P2G form: True = each tile scatters its own particles and the tiles'
+        # partial sums are merged in a second pass (gf_scene_p2g_staged);
+        # False = one pass in which every tile also reads its halo's particles
+        self.p2g_staged = True
+        self._p2g_ws = None
         # inflow/outflow regions drained by prepare_frame (D4.7 / F3.6)
         self.inflows: list = []
         self.outflows: list = []
@@ -710,6 +715,15 @@ class FlipSolver3D:
                 f[key], f[sav] = f[sav], f[key]
         self._solid_bc()
 
+    def _p2g_stage(self):
+        if self._p2g_ws is None:
+            nf, nt = ctypes.c_int64(), ctypes.c_int64()
+            _lib.call("S2.1", "gf_scene_p2g_stage_size", self.nx, self.ny, self.nz,
+                      ctypes.byref(nf), ctypes.byref(nt))
+            self._p2g_ws = (torch.empty(nf.value, dtype=torch.float32, device=self.device),
+                            torch.empty(nt.value, dtype=torch.int32, device=self.device))
+        return self._p2g_ws
+
     def _solid_bc(self):
         f = self._f
         _lib.call("S2.4", "gf_k3_enforce_solid_bc", *(_lib.ptr(f[k]) for k in
@@ -794,8 +808,13 @@ class FlipSolver3D:
         na[0] = sc.n_active_host
         s = self._descriptor()
         with prof.section("p2g"):
-            _lib.call("S2.1", "gf_scene_p2g", ctypes.byref(s), mode, self.gravity, dt, self.rho,
-                      stream)
+            if self.p2g_staged:
+                stage, has = self._p2g_stage()
+                _lib.call("S2.1", "gf_scene_p2g_staged", ctypes.byref(s), mode, self.gravity, dt,
+                          self.rho, _lib.ptr(stage), _lib.ptr(has), stream)
+            else:
+                _lib.call("S2.1", "gf_scene_p2g", ctypes.byref(s), mode, self.gravity, dt,
+                          self.rho, stream)
         prof.fused("normalize")
         prof.fused("gravity_bc")
         pre = None

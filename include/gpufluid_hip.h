@@ -165,6 +165,15 @@ int gf_scene_sort_sync(gf_scene_t* s, int with_affine, int* n_active_host, void*
 int gf_scene_p2g(const gf_scene_t* s, int transfer_mode, float gravity, float dt,
                  float rho, void* stream);
 
+/* gf_scene_p2g in two launches: each tile scatters only its own cells'
+ * particles (each particle once) into the region they reach and stores it in
+ * `stage`; then each tile sums the staged partials of the tiles holding each
+ * of its faces (at most two per axis, fixed order) and finalises as
+ * gf_scene_p2g.  stage / has: gf_scene_p2g_stage_size floats / int32. */
+int gf_scene_p2g_stage_size(int nx, int ny, int nz, int64_t* stage_floats, int64_t* flag_ints);
+int gf_scene_p2g_staged(const gf_scene_t* s, int transfer_mode, float gravity, float dt,
+                        float rho, float* stage, int32_t* has, void* stream);
+
 /* Raw P2G accumulation into u,v,w (+=) and uw,vw,ww (+=) from the sorted
  * particles: the reference block k3_p2g / k3_p2g_apic (solver3d.py:72-114,
  * 1475-1588) for callers that launch P2G alone. */
