@@ -156,8 +156,18 @@ __global__ __launch_bounds__(GF_WG) void k_rank(
     if (p >= n) return;
     int key = keys[p];
     int s = cell_start[key], e = cell_start[key + 1];
+    // A run of equal keys in this wave that holds every member of the cell
+    // (the common case: particles stay in their cell between substeps) is the
+    // cell's members in index order already; otherwise count the members with
+    // a smaller index.
+    int lead, len;
+    run_of(key, lead, len);
+    const int run_len = __shfl(len, lead, 64);
     int rank = 0;
-    for (int q = s; q < e; ++q) rank += (tmp[q] < p);
+    if (run_len == e - s)
+        rank = (int)(threadIdx.x & 63) - lead;
+    else
+        for (int q = s; q < e; ++q) rank += (tmp[q] < p);
     int d = s + rank;
     pos_o[3 * d + 0] = pos[3 * p + 0];
     pos_o[3 * d + 1] = pos[3 * p + 1];

@@ -551,12 +551,6 @@ __global__ __launch_bounds__(GF_WG) void k_p2g_own(P2GParams a, float* __restric
     }
 }
 
-// Whether tile offset d along an axis holds the face at local coordinate l
-// of this tile's finalised region (own axis: l in 0..8; others: 0..7).
-__device__ __forceinline__ bool holds(int d, int l, bool own) {
-    return d == 0 || (d < 0 ? l == 0 : l == (own ? 8 : 7));
-}
-
 __global__ __launch_bounds__(GF_WG) void k_p2g_finish(P2GParams a,
                                                       const float* __restrict__ stage,
                                                       const int32_t* __restrict__ has) {
@@ -587,6 +581,8 @@ __global__ __launch_bounds__(GF_WG) void k_p2g_finish(P2GParams a,
                                  : 0;
     }
     __syncthreads();
+    bool any_has = false;
+    for (int q = 0; q < 27; ++q) any_has |= s_has[q] != 0;
 
     const float gdt = a.tile_scene ? a.scene_par[8 * max(a.tile_scene[t], 0)] : a.g * a.dt;
 #pragma unroll
@@ -595,6 +591,9 @@ __global__ __launch_bounds__(GF_WG) void k_p2g_finish(P2GParams a,
         const int RJ = c == 1 ? 9 : 10, RK = c == 2 ? 9 : 10;  // staged region dims
         const int fni = nx + (c == 0), fnj = ny + (c == 1), fnk = nz + (c == 2);
         float* gf_ = c == 0 ? a.u : (c == 1 ? a.v : a.w);
+        // staged-region strides: one tile along x, y, z (in floats)
+        const ptrdiff_t sk = STAGE_PER_TILE, sj = sk * a.tnz, si = sj * a.tny;
+        const float* stage_c = stage + (size_t)t * STAGE_PER_TILE + (size_t)c * 2 * OWN_REGION;
         for (int idx = threadIdx.x; idx < P2G_REGION; idx += GF_WG) {
             const int lk = idx % DK, lj = (idx / DK) % DJ, li = idx / (DK * DJ);
             const int ii = i0 + li, jj = j0 + lj, kk = k0 + lk;
@@ -603,26 +602,35 @@ __global__ __launch_bounds__(GF_WG) void k_p2g_finish(P2GParams a,
                                (c == 1 ? (lj < 8 || jj == ny) : true) &&
                                (c == 2 ? (lk < 8 || kk == nz) : true);
             const size_t f = ((size_t)ii * fnj + jj) * fnk + kk;
-            // partial sums of the tiles whose staged regions hold this face
+            // partial sums of the tiles whose staged regions hold this face:
+            // offsets d in [dlo, dhi] per axis, ascending (-1 only at local
+            // coordinate 0, +1 only at the last one)
             float sv = 0.0f, sw = 0.0f;
-            for (int di = -1; di <= 1; ++di) {
-                if (!holds(di, li, c == 0)) continue;
-                for (int dj = -1; dj <= 1; ++dj) {
-                    if (!holds(dj, lj, c == 1)) continue;
-                    for (int dk = -1; dk <= 1; ++dk) {
-                        if (!holds(dk, lk, c == 2)) continue;
-                        if (!s_has[(di + 1) * 9 + (dj + 1) * 3 + (dk + 1)]) continue;
-                        const int tt = ((ti + di) * a.tny + (tj + dj)) * a.tnz + (tk + dk);
-                        // local coordinates in that tile's region (origin one
-                        // below its corner on the non-own axes)
-                        const int ri = ii - 8 * (ti + di) + (c != 0);
-                        const int rj = jj - 8 * (tj + dj) + (c != 1);
-                        const int rk = kk - 8 * (tk + dk) + (c != 2);
-                        const float* st = stage + (size_t)tt * STAGE_PER_TILE +
-                                          (size_t)c * 2 * OWN_REGION + (ri * RJ + rj) * RK + rk;
-                        sv += st[0];
-                        sw += st[OWN_REGION];
+            const int ilo = li == 0 ? -1 : 0, ihi = li == (c == 0 ? 8 : 7) ? 1 : 0;
+            const int jlo = lj == 0 ? -1 : 0, jhi = lj == (c == 1 ? 8 : 7) ? 1 : 0;
+            const int klo = lk == 0 ? -1 : 0, khi = lk == (c == 2 ? 8 : 7) ? 1 : 0;
+            // this tile's entry (region origin one below its corner on the
+            // non-own axes); a neighbour (di,dj,dk) holds the face at the
+            // same entry shifted by one tile and by -8 entries per axis
+            const float* own = stage_c + ((li + (c != 0)) * RJ + lj + (c != 1)) * RK + lk +
+                               (c != 2);
+            if (any_has) {
+                if ((ilo | ihi | jlo | jhi | klo | khi) == 0) {
+                    if (s_has[13]) {
+                        sv = own[0];
+                        sw = own[OWN_REGION];
                     }
+                } else {
+                    for (int di = ilo; di <= ihi; ++di)
+                        for (int dj = jlo; dj <= jhi; ++dj)
+                            for (int dk = klo; dk <= khi; ++dk) {
+                                if (!s_has[(di + 1) * 9 + (dj + 1) * 3 + (dk + 1)]) continue;
+                                const float* st =
+                                    own + (ptrdiff_t)di * (si - 8 * RJ * RK) +
+                                    (ptrdiff_t)dj * (sj - 8 * RK) + (ptrdiff_t)dk * (sk - 8);
+                                sv += st[0];
+                                sw += st[OWN_REGION];
+                            }
                 }
             }
             float val = sw > 1.0e-8f ? sv / sw : 0.0f;
