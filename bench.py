@@ -195,7 +195,10 @@ def main():
     for i in range(args.warmup):
         one_step()
         log(f"warmup {i}: PCG {s.last_pressure_iters} it, residual {s.last_pressure_residual:.2e}")
-    s._prof.reset()
+    # The timed steps run uninstrumented (no stage events, no PCG kernel
+    # events); the stage breakdown comes from instrumented steps afterwards.
+    s._prof.enabled = False
+    s.pcg_kernel_timing = False
     stats = []
     torch.cuda.synchronize()
     start = torch.cuda.Event(enable_timing=True)
@@ -205,8 +208,8 @@ def main():
     for i in range(args.steps):
         one_step()
         stats.append(dict(iters=s.last_pressure_iters, res=s.last_pressure_residual,
-                          nrep=s.last_pressure_replacements, kms=s.last_pcg_kernel_ms,
-                          na=s.n_active_tiles, nf=s._sc_fluid_cells))
+                          nrep=s.last_pressure_replacements, na=s.n_active_tiles,
+                          nf=s._sc_fluid_cells))
     end.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - w0
@@ -217,6 +220,16 @@ def main():
     log(f"timed {args.steps} steps: {ms_step:.3f} ms/step (wall {wall * 1e3 / args.steps:.3f}), "
         f"{value:.1f} Mparticle-substeps/s")
 
+    s._prof.enabled = True
+    s.pcg_kernel_timing = True
+    s._prof.reset()
+    n_instr = max(1, min(args.steps, 3))
+    istats = []
+    for i in range(n_instr):
+        one_step()
+        istats.append(dict(iters=s.last_pressure_iters, nrep=s.last_pressure_replacements,
+                           kms=s.last_pcg_kernel_ms))
+    log(f"instrumented {n_instr} steps for the stage breakdown")
     sec = s._prof.summarize()
     nf = float(np.mean([st["nf"] for st in stats]))
     na = float(np.mean([st["na"] for st in stats])) * 512
@@ -224,10 +237,10 @@ def main():
     # iteration kernel (two-kernel form: direction+SpMV), update (two-kernel
     # form only): ms summed over the sampled iterations (one per PCG chunk);
     # number of sampled iterations; replacements: ms summed over all
-    kms = np.sum([st["kms"] for st in stats], axis=0)
-    n_it = sum(iters)
+    kms = np.sum([st["kms"] for st in istats], axis=0)
+    n_it = sum(st["iters"] for st in istats)
     n_sampled = int(kms[2])
-    n_rep = sum(st["nrep"] for st in stats)
+    n_rep = sum(st["nrep"] for st in istats)
     two_kernel = kms[1] > 0
     if two_kernel:
         pcg_kern = {"pcg_dirspmv": (kms[0], n_sampled), "pcg_update": (kms[1], n_sampled),
@@ -236,8 +249,8 @@ def main():
         pcg_kern = {"pcg_iter": (kms[0], n_sampled), "pcg_replace_cg1": (kms[3], n_rep)}
     kern = {
         **pcg_kern,
-        "sort": (sec.get("clear", 0.0), args.steps), "p2g": (sec.get("p2g", 0.0), args.steps),
-        "g2p": (sec.get("g2p_advect", 0.0), args.steps),
+        "sort": (sec.get("clear", 0.0), n_instr), "p2g": (sec.get("p2g", 0.0), n_instr),
+        "g2p": (sec.get("g2p_advect", 0.0), n_instr),
     }
     stages = {}
     for k, (ms, calls) in kern.items():
@@ -348,7 +361,7 @@ def main():
         "config": {"workload": scene.name, "grid": list(scene.shape), "particles": npart,
                    "transfer": p.get("transfer", "flip"), "dt": dt, "pcg_tol": tol,
                    "pcg_max_iter": max_iter, "pcg_precision": args.precision,
-                   "pcg_iters": iters, "pcg_replacements": n_rep,
+                   "pcg_iters": iters, "pcg_replacements": sum(st["nrep"] for st in stats),
                    "fluid_cells": int(nf), "active_tiles": int(na // 512)},
         "roofline": roofline,
         "cpu_baseline": cpu,

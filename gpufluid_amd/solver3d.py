@@ -185,6 +185,7 @@ class FlipSolver3D:
         # False = one pass in which every tile also reads its halo's particles
         self.p2g_staged = True
         self._p2g_ws = None
+        self._na_pinned = None
         # inflow/outflow regions drained by prepare_frame (D4.7 / F3.6)
         self.inflows: list = []
         self.outflows: list = []
@@ -790,22 +791,32 @@ class FlipSolver3D:
         stream = self._stream()
         self._cache.clear()
         sc.ensure_particles(n, apic)
+        # Without a hook the tile counts are read back after P2G is enqueued
+        # (an async copy into pinned memory and an event), so the device is
+        # not idle while the host reads them.
+        deferred = hook is None
         with prof.section("clear"):
             s = self._descriptor()
             na = (ctypes.c_int * 2)()
             try:
                 _lib.call("F3.3", "gf_scene_sort_sync", ctypes.byref(s), int(apic),
-                          na, stream)
+                          None if deferred else na, stream)
             except BlockError:
                 sc.reset_counts()
                 raise
+            if deferred:
+                if self._na_pinned is None:
+                    self._na_pinned = torch.empty(2, dtype=torch.int32, pin_memory=True)
+                self._na_pinned.copy_(sc.n_active[:2], non_blocking=True)
+                na_ready = torch.cuda.Event()
+                na_ready.record()
             if n > 0:
                 sc.swap()
-            sc.n_active_host = na[0]
-            self._sc_fluid_cells = na[1]
+            if not deferred:
+                sc.n_active_host = na[0]
+                self._sc_fluid_cells = na[1]
         if hook is not None:
             hook("sorted")       # may substitute the masks and tile tables (frozen structure)
-        na[0] = sc.n_active_host
         s = self._descriptor()
         with prof.section("p2g"):
             if self.p2g_staged:
@@ -815,6 +826,11 @@ class FlipSolver3D:
             else:
                 _lib.call("S2.1", "gf_scene_p2g", ctypes.byref(s), mode, self.gravity, dt,
                           self.rho, stream)
+        if deferred:
+            na_ready.synchronize()
+            sc.n_active_host = int(self._na_pinned[0])
+            self._sc_fluid_cells = int(self._na_pinned[1])
+        na[0] = sc.n_active_host
         prof.fused("normalize")
         prof.fused("gravity_bc")
         pre = None
