@@ -53,12 +53,18 @@ def run_sort(s):
     return na[0]
 
 
-def run_p2g(s, dt, gravity=None):
+def run_p2g(s, dt, gravity=None, staged=False):
+    """The fused P2G alone: one-pass (gf_scene_p2g) or staged (the substep's)."""
     from gpufluid_amd import _lib
     d = s._descriptor()
     mode = _lib.GF_TRANSFER_APIC if s.transfer_mode == "apic" else _lib.GF_TRANSFER_FLIP
-    _lib.call("S2.1", "gf_scene_p2g", ctypes.byref(d), mode,
-              s.gravity if gravity is None else gravity, dt, s.rho, s._stream())
+    g = s.gravity if gravity is None else gravity
+    if staged:
+        stage, has = s._p2g_stage()
+        _lib.call("S2.1", "gf_scene_p2g_staged", ctypes.byref(d), mode, g, dt, s.rho,
+                  _lib.ptr(stage), _lib.ptr(has), s._stream())
+    else:
+        _lib.call("S2.1", "gf_scene_p2g", ctypes.byref(d), mode, g, dt, s.rho, s._stream())
     s._cache.clear()
 
 

@@ -70,16 +70,30 @@ def test_sort_bit_exact(shape, seed):
 
 @pytest.mark.parametrize("shape", [(16, 16, 16), (20, 13, 28)])
 @pytest.mark.parametrize("transfer", ["flip", "apic"])
-def test_p2g_fused_matches_oracle(shape, transfer):
+@pytest.mark.parametrize("staged", [False, True], ids=["one_pass", "staged"])
+def test_p2g_fused_matches_oracle(shape, transfer, staged):
     shape, dx, pos, vel = gu.dam_scene(shape=shape, vel_sigma=0.7, seed=5)
     rng = np.random.default_rng(9)
+    # a few particles on and beyond the domain boundary: clamped into boundary
+    # cells, their in-grid corners still land on faces.  (Offsets are whole
+    # fractions of a cell: a particle within rounding of a face gives a
+    # weight near the 1e-8 normalisation guard, where FP32 and the FP64
+    # oracle may fall on different sides.)
+    dom = np.array(shape) * dx
+    extra = np.array([[-0.3 * dx, 0.5, 0.5], [dom[0] + 0.4 * dx, 0.3, 0.2],
+                      [0.4, -1.7 * dx, 0.6], [0.2, 0.3, dom[2] + 0.1 * dx],
+                      [0.0, 0.0, 0.0],
+                      [dom[0] + 0.25 * dx, dom[1] + 0.35 * dx, dom[2] + 0.15 * dx]], np.float32)
+    pos = np.concatenate([pos, extra])
+    vel = np.concatenate([vel, rng.normal(0, 0.7, extra.shape).astype(np.float32)])
     aff = rng.normal(0, 3.0, (len(pos), 3, 3)).astype(np.float32)
+    print(f"[p2g] {shape} {transfer} staged={staged}", flush=True)
     s = gu.make_solver(shape, dx, pos, vel, transfer_mode=transfer)
     if transfer == "apic":
         s.set_particles(pos, vel, aff)
     dt = 1.0 / 240
     gu.run_sort(s)
-    gu.run_p2g(s, dt)
+    gu.run_p2g(s, dt, staged=staged)
     m = fo.update_marker(fo.wall_marker(*shape), pos, dx)
     u, v, w, uw, vw, ww = fo.p2g(pos, vel, m, dx, affine=aff if transfer == "apic" else None)
     u, us = fo.normalize(u, uw)
