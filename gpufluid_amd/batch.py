@@ -108,7 +108,7 @@ class BatchedScenes:
 
     def __init__(self, scenes, gravity: float = -9.81, flip_blend: float = 0.95,
                  rho: float = 1.0, dt: float = 1.0 / 240.0, device=None,
-                 pcg_precision: str = "mixed", pcg_check_every: int = 8):
+                 pcg_precision: str = "mixed", pcg_check_every: int = 10):
         from .solver3d import FlipSolver3D
         self.edges = [int(sc[0]) for sc in scenes]
         self.pack = pack_scenes(self.edges)

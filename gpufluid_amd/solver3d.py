@@ -101,7 +101,7 @@ class FlipSolver3D:
                  enable_cuda_graphs: bool = False, enable_sub_dense: bool = False,
                  sub_rebuild_every: int = 8, sub_dilation: int = 4, *,
                  pcg_precision: str = "mixed", pcg_sign: str = "physical",
-                 pcg_tol: float = 1e-4, pcg_check_every: int = 8):
+                 pcg_tol: float = 1e-4, pcg_check_every: int = 10):
         if transfer_mode not in _TRANSFER:
             raise ValueError(f"unknown transfer_mode: {transfer_mode!r}")
         if pcg_precision not in ("mixed", "fp64"):
