@@ -152,6 +152,8 @@ def main():
                     help="skip the surface-tension leg (3 substeps with CSF on the scene)")
     ap.add_argument("--no-reseed", action="store_true",
                     help="skip the GPU reseed leg (one reseed of the scene's particles)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="skip the small-grid graph-replay leg (64^3, direct vs hipGraph substeps)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (3 launches) so that profiler "
                          "byte counters can be scaled against a kernel of known traffic")
@@ -340,6 +342,12 @@ def main():
         bat = batch_leg()
         log(f"batch: {bat}")
 
+    gr = None
+    if not args.no_graph:
+        torch.cuda.empty_cache()
+        gr = graph_leg()
+        log(f"graph replay: {gr}")
+
     cpu = None
     if not args.no_cpu_baseline:
         c = cpu_baseline(scene)
@@ -374,6 +382,7 @@ def main():
         "reseed": rsd,
         "adjoint": adj,
         "batched": bat,
+        "graph_replay": gr,
     }
     print(json.dumps(out), flush=True)
 
@@ -609,6 +618,41 @@ def batch_leg(substeps=20, alone_substeps=2, tol=1e-8, max_iter=500):
             "autotune_seconds": {str(k): round(v, 5) for k, v in tune_times.items()},
             "graph_pcg_budget_iters": budget, "max_pcg_iters_per_substep": iters,
             "worst_scene_rel_residual": worst_res}
+
+
+def graph_leg(n=64, steps=30, warmup=3):
+    """Small-grid launch overhead: the reference's graph workload (64^3,
+    seed_box (0.3,0.3,0.3)-(0.6,0.6,0.6) at 4 particles per cell, dt 0.005)
+    stepped directly and replayed from captured hipGraphs, with 20 Jacobi
+    sweeps and with PCG (tolerance 1e-4, cap 80).  Wall clock around `steps`
+    substeps after `warmup` (the first graph substeps capture)."""
+    import torch
+    from gpufluid_amd import FlipSolver3D
+    out = {"grid": [n, n, n], "steps": steps, "dt": 0.005,
+           "timing": "wall clock around the substeps, synchronised once at each end"}
+    for solver, iters in (("jacobi", 20), ("pcg", 80)):
+        row = {}
+        for mode in ("direct", "graph"):
+            s = FlipSolver3D(n, n, n, dx=1.0 / n, device="cuda:0",
+                             enable_cuda_graphs=mode == "graph")
+            s.seed_box((0.30, 0.30, 0.30), (0.60, 0.60, 0.60), ppc=4)
+            for _ in range(warmup):
+                s.step(0.005, pressure_iters=iters, pressure_solver=solver)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                s.step(0.005, pressure_iters=iters, pressure_solver=solver)
+            torch.cuda.synchronize()
+            row[f"{mode}_ms"] = round((time.perf_counter() - t0) * 1e3 / steps, 4)
+            row["particles"] = s.n_particles
+            if mode == "graph":
+                row["captures"] = s._cuda_graph_misses
+                row["replays"] = s._cuda_graph_hits
+            if solver == "pcg":
+                row[f"{mode}_pcg_iters"] = s.last_pressure_iters
+        row["speedup"] = round(row["direct_ms"] / row["graph_ms"], 3)
+        out[solver] = row
+    return out
 
 
 def pcie_inclusive(s, npart, ms_step):

@@ -20,6 +20,7 @@ GF_PCG_MIXED = 0
 GF_PCG_FP64 = 1
 GF_ATTR_COLOR = 0
 GF_ATTR_SCALAR = 1
+GF_NACT_DEVICE = -1  # n_active argument: read the active tile count on the device
 GF_EINVAL, GF_ERANGE, GF_ESTATE = -1, -2, -3  # negative return codes of the C-ABI
 TILE = 8
 TILE_CELLS = 512

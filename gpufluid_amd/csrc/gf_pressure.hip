@@ -85,7 +85,8 @@ struct PcgArgs {
     PcgLog* hout;        // when set: the log pairs are also written here (pinned host memory)
     int32_t* part_info;  // [8*PCG_GMAX] per workgroup: scene, wb, we, t0, t1; then [S+1] wg offsets
     const int32_t* scene_slot;  // [S+1] compact tile offsets per scene (NULL: one scene, nact tiles)
-    int nact;            // active tiles (one scene)
+    const int32_t* nact_dev;    // one scene, count on the device ([0] active tiles) or NULL
+    int nact;            // active tiles (one scene, nact_dev NULL)
     int S;               // scenes
     int variant;         // PCG_STD (two kernels per iteration) or PCG_CG1 (one)
     int G;               // workgroups; a scene's workgroups split its tiles evenly
@@ -332,7 +333,7 @@ struct Part {
 __device__ __forceinline__ Part part_of(const PcgArgs& a) {
     Part P;
     P.b = gf::xcd_remap(blockIdx.x, a.G);
-    if (a.S == 1) {  // one scene: the partition is b*nact/G, no table read
+    if (a.S == 1 && !a.nact_dev) {  // one scene: the partition is b*nact/G, no table read
         P.s = 0;
         P.wb = 0;
         P.we = a.G;
@@ -575,14 +576,18 @@ __device__ __forceinline__ bool pass_through(const PcgArgs& a, const Part& P, co
 // Workgroup partition of a solve (one workgroup): scene s gets
 // 1 + floor((G - NE) * tiles_s / tiles) workgroups when it has active tiles
 // (NE = scenes with tiles), none otherwise; each workgroup of a scene takes an
-// even share of its tiles.  Also marks every scene's log as awaiting the setup.
+// even share of its tiles.  A single scene gets at most one workgroup per
+// two tiles, which is the grid the host-count form launches, so both forms
+// split the tiles (and order the partial sums) identically.  Also marks every
+// scene's log as awaiting the setup.
 __global__ __launch_bounds__(GF_WG) void k_pcg_partition(PcgArgs a) {
     __shared__ int s_wave[GF_WG / GF_WAVE + 1];
     __shared__ double s_red[2 * GF_WG / GF_WAVE];
     int32_t* wg_off = a.part_info + 8 * PCG_GMAX;
+    const int n_one = a.nact_dev ? a.nact_dev[0] : a.nact;  // tiles of a single scene
     long long tiles = 0, ne = 0;
     for (int sc = threadIdx.x; sc < a.S; sc += GF_WG) {
-        const int n = a.scene_slot ? a.scene_slot[sc + 1] - a.scene_slot[sc] : a.nact;
+        const int n = a.scene_slot ? a.scene_slot[sc + 1] - a.scene_slot[sc] : n_one;
         tiles += n;
         ne += n > 0;
     }
@@ -595,8 +600,9 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_partition(PcgArgs a) {
         int g = 0, n = 0, slot0 = 0;
         if (sc < a.S) {
             slot0 = a.scene_slot ? a.scene_slot[sc] : 0;
-            n = a.scene_slot ? a.scene_slot[sc + 1] - slot0 : a.nact;
+            n = a.scene_slot ? a.scene_slot[sc + 1] - slot0 : n_one;
             g = n > 0 ? 1 + (int)((spare * n) / (tiles > 0 ? tiles : 1)) : 0;
+            if (!a.scene_slot && g > (n + 1) / 2) g = (n + 1) / 2;
         }
         int total;
         const int off = carry + gf::wg_exclusive_scan(g, s_wave, &total);
@@ -626,7 +632,7 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_partition(PcgArgs a) {
         }
         const int sc = lo, off = wg_off[sc], g = wg_off[sc + 1] - off, q = b - off;
         const int slot0 = a.scene_slot ? a.scene_slot[sc] : 0;
-        const int n = a.scene_slot ? a.scene_slot[sc + 1] - slot0 : a.nact;
+        const int n = a.scene_slot ? a.scene_slot[sc + 1] - slot0 : n_one;
         w[0] = sc; w[1] = off; w[2] = off + g;
         w[3] = slot0 + (int)(((long long)q * n) / g);
         w[4] = slot0 + (int)(((long long)(q + 1) * n) / g);
@@ -1232,10 +1238,11 @@ __global__ __launch_bounds__(GF_WG) void k_jacobi(const uint8_t* __restrict__ fl
                                                   const int32_t* __restrict__ nbr,
                                                   const float* __restrict__ b,
                                                   const float* __restrict__ pin,
-                                                  float* __restrict__ pout, int nact) {
+                                                  float* __restrict__ pout, int nact,
+                                                  const int32_t* __restrict__ nact_dev) {
     const int t = __builtin_amdgcn_readfirstlane(2 * gf::xcd_remap(blockIdx.x, gridDim.x) +
                                                  (threadIdx.x >> 7));
-    if (t >= nact) return;
+    if (t >= (nact_dev ? nact_dev[0] : nact)) return;
     const Quad Q = quad_of();
     const size_t base = (size_t)t * GF_TILE_CELLS;
     const size_t idx = base + Q.l0;
@@ -1264,10 +1271,10 @@ __global__ __launch_bounds__(GF_WG) void k_jacobi(const uint8_t* __restrict__ fl
 __global__ __launch_bounds__(GF_WG) void k_gsrb(const uint8_t* __restrict__ flags,
                                                 const int32_t* __restrict__ nbr,
                                                 const float* __restrict__ b, float* p, int nact,
-                                                int color) {
+                                                const int32_t* __restrict__ nact_dev, int color) {
     const int t = __builtin_amdgcn_readfirstlane(2 * gf::xcd_remap(blockIdx.x, gridDim.x) +
                                                  (threadIdx.x >> 7));
-    if (t >= nact) return;
+    if (t >= (nact_dev ? nact_dev[0] : nact)) return;
     const Quad Q = quad_of();
     const size_t base = (size_t)t * GF_TILE_CELLS;
     const size_t idx = base + Q.l0;
@@ -1335,12 +1342,19 @@ extern "C" int gf_scene_jacobi(const gf_scene_t* s, int n_active, int iters, voi
     int rc = gf::scene_check(s, "gf_scene_jacobi");
     if (rc) return rc;
     const int ntiles = s->tnx * s->tny * s->tnz;
-    GF_REQUIRE(n_active >= 0 && n_active <= ntiles, "gf_scene_jacobi: n_active %d", n_active);
+    const bool dev_count = n_active == GF_NACT_DEVICE;
+    GF_REQUIRE(dev_count || (n_active >= 0 && n_active <= ntiles), "gf_scene_jacobi: n_active %d",
+               n_active);
     GF_REQUIRE(iters >= 0, "gf_scene_jacobi: iters %d", iters);
-    GF_REQUIRE(s->flags && s->nbr && s->b && s->pc && s->ptmp, "gf_scene_jacobi: buffers not set");
+    GF_REQUIRE(s->flags && s->nbr && s->b && s->pc && s->ptmp && (s->n_active || !dev_count),
+               "gf_scene_jacobi: buffers not set");
     if (n_active == 0) return 0;
     hipStream_t st = (hipStream_t)stream;
-    const size_t n = (size_t)n_active * GF_TILE_CELLS;
+    // device count: launches sized for every tile, the kernels stop at n_active[0]
+    const int span = dev_count ? ntiles : n_active;
+    const int nact = dev_count ? 0 : n_active;
+    const int32_t* nact_dev = dev_count ? s->n_active : nullptr;
+    const size_t n = (size_t)span * GF_TILE_CELLS;
     const int fb = (int)((n + GF_WG - 1) / GF_WG);
     hipLaunchKernelGGL(k_fill_f32, dim3(fb), dim3(GF_WG), 0, st, s->pc, n, 0.0f);
     hipLaunchKernelGGL(k_fill_f32, dim3(fb), dim3(GF_WG), 0, st, s->ptmp, n, 0.0f);
@@ -1348,8 +1362,8 @@ extern "C" int gf_scene_jacobi(const gf_scene_t* s, int n_active, int iters, voi
         const bool to_pc = ((iters - 1 - k) % 2) == 0;
         float* out = to_pc ? s->pc : s->ptmp;
         const float* in = to_pc ? s->ptmp : s->pc;
-        hipLaunchKernelGGL(k_jacobi, dim3((n_active + 1) / 2), dim3(GF_WG), 0, st, s->flags,
-                           s->nbr, s->b, in, out, n_active);
+        hipLaunchKernelGGL(k_jacobi, dim3((span + 1) / 2), dim3(GF_WG), 0, st, s->flags,
+                           s->nbr, s->b, in, out, nact, nact_dev);
     }
     GF_LAUNCH_CHECK("k_jacobi");
     return 0;
@@ -1359,18 +1373,24 @@ extern "C" int gf_scene_gsrb(const gf_scene_t* s, int n_active, int iters, void*
     int rc = gf::scene_check(s, "gf_scene_gsrb");
     if (rc) return rc;
     const int ntiles = s->tnx * s->tny * s->tnz;
-    GF_REQUIRE(n_active >= 0 && n_active <= ntiles, "gf_scene_gsrb: n_active %d", n_active);
+    const bool dev_count = n_active == GF_NACT_DEVICE;
+    GF_REQUIRE(dev_count || (n_active >= 0 && n_active <= ntiles), "gf_scene_gsrb: n_active %d",
+               n_active);
     GF_REQUIRE(iters >= 0, "gf_scene_gsrb: iters %d", iters);
-    GF_REQUIRE(s->flags && s->nbr && s->b && s->pc, "gf_scene_gsrb: buffers not set");
+    GF_REQUIRE(s->flags && s->nbr && s->b && s->pc && (s->n_active || !dev_count),
+               "gf_scene_gsrb: buffers not set");
     if (n_active == 0) return 0;
     hipStream_t st = (hipStream_t)stream;
-    const size_t n = (size_t)n_active * GF_TILE_CELLS;
+    const int span = dev_count ? ntiles : n_active;
+    const int nact = dev_count ? 0 : n_active;
+    const int32_t* nact_dev = dev_count ? s->n_active : nullptr;
+    const size_t n = (size_t)span * GF_TILE_CELLS;
     hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)((n + GF_WG - 1) / GF_WG)), dim3(GF_WG), 0, st,
                        s->pc, n, 0.0f);
     for (int k = 0; k < iters; ++k) {
         for (int color = 0; color < 2; ++color)
-            hipLaunchKernelGGL(k_gsrb, dim3((n_active + 1) / 2), dim3(GF_WG), 0, st, s->flags,
-                               s->nbr, s->b, s->pc, n_active, color);
+            hipLaunchKernelGGL(k_gsrb, dim3((span + 1) / 2), dim3(GF_WG), 0, st, s->flags,
+                               s->nbr, s->b, s->pc, nact, nact_dev, color);
     }
     GF_LAUNCH_CHECK("k_gsrb");
     return 0;
@@ -1386,7 +1406,14 @@ int pcg_prepare(const gf_scene_t* s, int n_active, int precision, double tol, in
     const int ntiles = s->tnx * s->tny * s->tnz;
     const bool packed = s->n_scenes > 1;
     const int S = packed ? s->n_scenes : 1;
-    GF_REQUIRE(n_active >= 0 && n_active <= ntiles, "%s: n_active %d", who, n_active);
+    // GF_NACT_DEVICE: the active tile count is read from n_active[0] on the
+    // device, so the launches do not depend on it (capture-safe); grid and
+    // form are then sized for every tile of the grid
+    const bool dev_count = n_active == GF_NACT_DEVICE && !packed;
+    GF_REQUIRE(dev_count || (n_active >= 0 && n_active <= ntiles), "%s: n_active %d", who,
+               n_active);
+    GF_REQUIRE(!dev_count || s->n_active, "%s: n_active buffer not set", who);
+    const int sized = dev_count ? ntiles : n_active;
     GF_REQUIRE(S <= PCG_GMAX, "%s: %d scenes exceed %d", who, S, PCG_GMAX);
     GF_REQUIRE(precision == GF_PCG_MIXED || precision == GF_PCG_FP64, "%s: precision %d", who,
                precision);
@@ -1414,7 +1441,8 @@ int pcg_prepare(const gf_scene_t* s, int n_active, int precision, double tol, in
     a.pc = s->pc; a.part = s->partials; a.log = (PcgLog*)s->scalars; a.hout = nullptr;
     a.part_info = s->pcg_part;
     a.scene_slot = packed ? s->scene_slot : nullptr;
-    a.nact = n_active;
+    a.nact = dev_count ? 0 : n_active;
+    a.nact_dev = dev_count ? s->n_active : nullptr;
     a.S = S;
     // 1024 workgroups (4 per CU) measured best on MI355X at 256^3 (5.4 M
     // active cells); the autotuner may choose another value up to PCG_GMAX.
@@ -1423,14 +1451,14 @@ int pcg_prepare(const gf_scene_t* s, int n_active, int precision, double tol, in
     if (packed) {
         a.G = gmax > S ? gmax : S;
     } else {
-        const int pairs = (n_active + 1) / 2;
+        const int pairs = (sized + 1) / 2;
         a.G = pairs < gmax ? pairs : gmax;
     }
     const int vt = gf::tunable(gf::GF_TUNE_PCG_VARIANT);
     if (vt == PCG_STD || vt == PCG_CG1)
         a.variant = vt;
     else
-        a.variant = n_active < PCG_CG1_MAX_TILES ? PCG_CG1 : PCG_STD;
+        a.variant = sized < PCG_CG1_MAX_TILES ? PCG_CG1 : PCG_STD;
     a.rhs_sign = (float)rhs_sign;
     a.tol2 = tol * tol;
     a.delta2 = 0.01;
@@ -1553,6 +1581,7 @@ extern "C" int gf_scene_pcg_sync(const gf_scene_t* s, int n_active, int precisio
     if (replacements) *replacements = 0;
     if (kernel_ms) for (int i = 0; i < 4; ++i) kernel_ms[i] = 0.0;
     PcgArgs a = {};
+    GF_REQUIRE(n_active != GF_NACT_DEVICE, "gf_scene_pcg_sync: needs the host tile count");
     int rc = pcg_prepare(s, n_active, precision, tol, max_iter, rhs_sign, "gf_scene_pcg_sync", a);
     if (rc) return rc;
     if (check_every < 1) check_every = 1;

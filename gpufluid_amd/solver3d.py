@@ -137,11 +137,15 @@ class FlipSolver3D:
         self.pcg_tol = float(pcg_tol)
         self.pcg_check_every = int(pcg_check_every)
         self._prof = StepProfiler(enabled=enable_timing, device=self.device)
+        # B5.2 graph cache: captured substeps keyed by everything their
+        # launches bake in (see _graph_key); ping-ponged buffers need two
         self._enable_cuda_graphs = bool(enable_cuda_graphs)
+        self._graphs = {}
         self._cuda_graph = None
         self._cuda_graph_key = None
         self._cuda_graph_hits = 0
         self._cuda_graph_misses = 0
+        self._graph_stats = None      # (solver, iters) of a replay whose stats are unread
 
         dev = self.device
         f32 = dict(dtype=torch.float32, device=dev)
@@ -164,10 +168,8 @@ class FlipSolver3D:
         self._uvw_weights = None
         self._sc = SceneBuffers(self.nx, self.ny, self.nz, self.dx, self.dom, dev)
         self.n_particles = 0
-        self._sc_fluid_cells = 0
-        self.last_pressure_iters = 0
-        self.last_pressure_residual = None
-        self.last_pressure_replacements = 0
+        self._fluid_cells = 0
+        self._pressure_stats = (0, None, 0)   # iterations, residual, replacements
         self.pcg_kernel_timing = False      # per-kernel HIP-event timing of PCG launches
         # (dirspmv, update: ms summed over sampled iterations; sampled iterations;
         #  replacements: ms summed), see gf_scene_pcg_sync
@@ -364,6 +366,7 @@ class FlipSolver3D:
         self._set_attr("temperature", temperature, n)
         sc = self._sc
         apic = self.transfer_mode == "apic" or affine is not None
+        self._cuda_graph_invalidate()
         sc.ensure_particles(n, apic)
         self.n_particles = n
         self._cache.clear()
@@ -503,7 +506,7 @@ class FlipSolver3D:
         self._marker_host = marker.copy()
         self._f["marker"].copy_(torch.as_tensor(marker, device=self.device))
         self._cache.clear()
-        self._cuda_graph = None
+        self._cuda_graph_invalidate()
 
     # ------------------------------------------------- frame hook (F3.6, D4.7)
     @block("D4.7", "Register an inflow region (continuous particle emission)")
@@ -734,13 +737,7 @@ class FlipSolver3D:
     def _surface_tension(self, dt):
         """S2.14 as _step_impl runs it (solver3d.py:3548-3555, 3230-3300): the
         CSF impulse with force balance on u, v, w, then the solid BC."""
-        if self._csf_ws is None:
-            wf, rd = ctypes.c_int64(), ctypes.c_int64()
-            _lib.call("S2.14", "gf_csf_workspace", self.nx, self.ny, self.nz, ctypes.byref(wf),
-                      ctypes.byref(rd))
-            self._csf_ws = (torch.empty(wf.value, dtype=torch.float32, device=self.device),
-                            torch.empty(rd.value, dtype=torch.float64, device=self.device))
-        work, red = self._csf_ws
+        work, red = self._csf_workspace()
         coeff = float(self.surface_tension * dt / max(self.rho, 1.0e-8))
         f = self._f
         _lib.call("S2.14", "gf_csf_apply", *(_lib.ptr(f[k]) for k in ("u", "v", "w", "marker")),
@@ -748,19 +745,31 @@ class FlipSolver3D:
                   int(self.csf_smoothing_passes), _lib.ptr(work), _lib.ptr(red), self._stream())
         self._solid_bc()
 
-    def _attr_transfer(self, s, attr, channels, mode, stream):
-        """S2.15 / S2.18: scatter, normalise and gather one attribute at the
-        advected positions (solver3d.py:3304-3359)."""
-        if attr.shape[0] != self.n_particles:
-            raise RuntimeError(f"attribute holds {attr.shape[0]} particles, the solver "
-                               f"{self.n_particles}")
+    def _csf_workspace(self):
+        if self._csf_ws is None:
+            wf, rd = ctypes.c_int64(), ctypes.c_int64()
+            _lib.call("S2.14", "gf_csf_workspace", self.nx, self.ny, self.nz, ctypes.byref(wf),
+                      ctypes.byref(rd))
+            self._csf_ws = (torch.empty(wf.value, dtype=torch.float32, device=self.device),
+                            torch.empty(rd.value, dtype=torch.float64, device=self.device))
+        return self._csf_ws
+
+    def _attr_workspace(self):
         if self._attr_ws is None:
             cells = self.nx * self.ny * self.nz
             dev = self.device
             self._attr_ws = (torch.empty(4 * cells, dtype=torch.int64, device=dev),
                              torch.empty(4 * cells, dtype=torch.float32, device=dev),
                              torch.empty(4, dtype=torch.int32, device=dev))
-        acc, grid, scratch = self._attr_ws
+        return self._attr_ws
+
+    def _attr_transfer(self, s, attr, channels, mode, stream):
+        """S2.15 / S2.18: scatter, normalise and gather one attribute at the
+        advected positions (solver3d.py:3304-3359)."""
+        if attr.shape[0] != self.n_particles:
+            raise RuntimeError(f"attribute holds {attr.shape[0]} particles, the solver "
+                               f"{self.n_particles}")
+        acc, grid, scratch = self._attr_workspace()
         _lib.call("S2.15" if mode == _lib.GF_ATTR_COLOR else "S2.18", "gf_scene_attr_transfer",
                   ctypes.byref(s), _lib.ptr(attr), channels, mode, _lib.ptr(acc),
                   _lib.ptr(grid), _lib.ptr(scratch), stream)
@@ -776,12 +785,149 @@ class FlipSolver3D:
         if pressure_solver not in ("jacobi", "pcg", "gsrb"):
             raise ValueError(f"unknown pressure_solver: {pressure_solver!r}")
         tol = self.pcg_tol if pressure_tol is None else float(pressure_tol)
-        self._step_impl(float(dt), int(pressure_iters), pressure_solver, tol)
+        if self._cuda_graph_eligible(pressure_solver, pressure_block_sparse):
+            self._step_graphed(float(dt), int(pressure_iters), pressure_solver, tol)
+        else:
+            self._step_impl(float(dt), int(pressure_iters), pressure_solver, tol)
 
-    def _step_impl(self, dt, pressure_iters, pressure_solver, tol, hook=None):
+    # ------------------------------------------------------- B5.2 graph cache
+    _GRAPH_CACHE = 4   # captured substeps kept (two per ping-pong state)
+
+    def _cuda_graph_eligible(self, pressure_solver: str, pressure_block_sparse: bool = False) -> bool:
+        """Substeps replayed from a hipGraph (solver3d.py:3362-3391): with
+        ``enable_cuda_graphs``, every solver and stage combination once
+        particles exist, except while per-kernel PCG timing or the stage
+        profiler need host events."""
+        return (self._enable_cuda_graphs and self.n_particles > 0
+                and pressure_solver in ("jacobi", "gsrb", "pcg")
+                and not self.pcg_kernel_timing and not self._prof.enabled)
+
+    def _cuda_graph_invalidate(self):
+        """Drop every captured substep (solver3d.py:3414-3419)."""
+        self._graphs.clear()
+        self._cuda_graph = None
+        self._cuda_graph_key = None
+
+    def _ensure_workspaces(self, pressure_solver):
+        """Allocate every buffer a substep may use, so that none is
+        allocated while a substep is being captured."""
+        self._sc.ensure_particles(self.n_particles, self.transfer_mode == "apic")
+        if self.p2g_staged:
+            self._p2g_stage()
+        if self.viscosity > 0.0 or self.surface_tension > 0.0:
+            self._weights()
+        if self.surface_tension > 0.0:
+            self._csf_workspace()
+        if self._attr["color"] is not None or self._attr["temperature"] is not None:
+            self._attr_workspace()
+        if pressure_solver == "pcg" and self.pcg_precision == "fp64":
+            self._sc.ensure_fp64()
+
+    def _graph_key(self, dt, pressure_iters, pressure_solver, tol):
+        """Everything a captured substep bakes into its launches: the
+        descriptor (buffer addresses, sizes, particle count), the addresses
+        of the buffers held here, the stage switches and constants, and the
+        PCG tunables.  Equal keys give identical launch sequences (the
+        reference keys on a subset, solver3d.py:3393-3412)."""
+        from . import autotune
+        d = self._descriptor()
+        raw = ctypes.string_at(ctypes.addressof(d), ctypes.sizeof(d))
+        held = [self._f[k] for k in sorted(self._f)]
+        held += list(self._uvw_weights or ())
+        held += [self._attr["color"], self._attr["temperature"]]
+        for ws in (self._p2g_ws, self._attr_ws, self._csf_ws):
+            held += list(ws or ())
+        ptrs = tuple(0 if t is None else t.data_ptr() for t in held)
+        return (raw, ptrs, autotune.get_pcg_workgroups(), autotune.get_pcg_variant(),
+                float(dt), int(pressure_iters), pressure_solver, float(tol), self.transfer_mode,
+                self.pcg_precision, self.pcg_sign, int(self.pcg_check_every),
+                float(self.gravity), float(self.rho), float(self.flip_blend),
+                float(self.viscosity), int(self.viscosity_iters), float(self.surface_tension),
+                int(self.csf_smoothing_passes), bool(self.p2g_staged))
+
+    def _step_graphed(self, dt, pressure_iters, pressure_solver, tol):
+        """Replay the captured substep for this key, capturing it first on a
+        miss (solver3d.py:3436-3469).  Unlike the reference, the call that
+        captures also replays, so every call advances the simulation.  The
+        pressure statistics and tile counts are read when first asked for."""
+        sc = self._sc
+        self._ensure_workspaces(pressure_solver)
+        key = self._graph_key(dt, pressure_iters, pressure_solver, tol)
+        entry = self._graphs.get(key)
+        if entry is None:
+            entry = self._capture_substep(dt, pressure_iters, pressure_solver, tol)
+            if len(self._graphs) >= self._GRAPH_CACHE:
+                self._graphs.pop(next(iter(self._graphs)))
+            self._graphs[key] = entry
+            self._cuda_graph_misses += 1
+        else:
+            self._cuda_graph_hits += 1
+        graph, fields, cur = entry
+        self._cache.clear()
+        graph.replay()
+        # the host-side state the substep leaves: ping-ponged particle and
+        # viscosity buffers
+        self._f.update(fields)
+        sc.cur = cur
+        self._cuda_graph, self._cuda_graph_key = graph, key
+        self._graph_stats = (pressure_solver, int(pressure_iters))
+
+    def _capture_substep(self, dt, pressure_iters, pressure_solver, tol):
+        """Record one substep into a hipGraph (nothing runs); returns the
+        graph and the host-side buffer state it ends in, and restores the
+        state it started from."""
+        dev = self.device
+        start = (dict(self._f), self._sc.cur)
+        g = torch.cuda.CUDAGraph()
+        stream = torch.cuda.Stream(device=dev)
+        stream.wait_stream(torch.cuda.current_stream(dev))
+        try:
+            with torch.cuda.stream(stream):
+                with torch.cuda.graph(g, stream=stream):
+                    self._step_impl(dt, pressure_iters, pressure_solver, tol, capture=True)
+            end = (dict(self._f), self._sc.cur)
+        finally:
+            torch.cuda.current_stream(dev).wait_stream(stream)
+            self._f.update(start[0])
+            self._sc.cur = start[1]
+        return g, end[0], end[1]
+
+    def _stats(self):
+        if self._graph_stats is not None:
+            solver, iters = self._graph_stats
+            self._graph_stats = None
+            if solver == "pcg":
+                it, res, nrep = (ctypes.c_int * 1)(), (ctypes.c_double * 1)(), (ctypes.c_int * 1)()
+                d = self._descriptor()
+                _lib.call("F3.3", "gf_scene_pcg_stats", ctypes.byref(d), it, res, nrep,
+                          self._stream())
+                self._pressure_stats = (int(it[0]), float(res[0]), int(nrep[0]))
+            else:
+                self._pressure_stats = (iters, None, 0)
+            na = self._sc.n_active[:2].cpu()
+            self._sc.n_active_host = int(na[0])
+            self._fluid_cells = int(na[1])
+        return self._pressure_stats
+
+    last_pressure_iters = property(lambda self: self._stats()[0],
+                                   doc="PCG iterations (Jacobi / GS-RB: sweeps) of the last substep")
+    last_pressure_residual = property(lambda self: self._stats()[1],
+                                      doc="PCG true relative residual of the last substep")
+    last_pressure_replacements = property(lambda self: self._stats()[2],
+                                          doc="FP64 residual replacements of the last PCG solve")
+
+    @property
+    def _sc_fluid_cells(self) -> int:
+        self._stats()
+        return self._fluid_cells
+
+    def _step_impl(self, dt, pressure_iters, pressure_solver, tol, hook=None, capture=False):
         """The substep pipeline.  ``hook(stage)``, if given, is called after
         the sort ("sorted": particles in this substep's cell order) and at the
-        end ("done"); differentiable rollouts record forward state there."""
+        end ("done"); differentiable rollouts record forward state there.
+        ``capture``: enqueue every launch with no host synchronisation (the
+        pressure solvers read the active tile count on the device), for
+        recording into a hipGraph; the statistics are then read afterwards."""
         prof = self._prof
         sc = self._sc
         apic = self.transfer_mode == "apic"
@@ -790,17 +936,18 @@ class FlipSolver3D:
         n = self.n_particles
         stream = self._stream()
         self._cache.clear()
+        self._graph_stats = None
         sc.ensure_particles(n, apic)
         # Without a hook the tile counts are read back after P2G is enqueued
         # (an async copy into pinned memory and an event), so the device is
         # not idle while the host reads them.
-        deferred = hook is None
+        deferred = hook is None and not capture
         with prof.section("clear"):
             s = self._descriptor()
             na = (ctypes.c_int * 2)()
             try:
                 _lib.call("F3.3", "gf_scene_sort_sync", ctypes.byref(s), int(apic),
-                          None if deferred else na, stream)
+                          None if deferred or capture else na, stream)
             except BlockError:
                 sc.reset_counts()
                 raise
@@ -812,9 +959,9 @@ class FlipSolver3D:
                 na_ready.record()
             if n > 0:
                 sc.swap()
-            if not deferred:
+            if not deferred and not capture:
                 sc.n_active_host = na[0]
-                self._sc_fluid_cells = na[1]
+                self._fluid_cells = na[1]
         if hook is not None:
             hook("sorted")       # may substitute the masks and tile tables (frozen structure)
         s = self._descriptor()
@@ -829,8 +976,8 @@ class FlipSolver3D:
         if deferred:
             na_ready.synchronize()
             sc.n_active_host = int(self._na_pinned[0])
-            self._sc_fluid_cells = int(self._na_pinned[1])
-        na[0] = sc.n_active_host
+            self._fluid_cells = int(self._na_pinned[1])
+        nact = _lib.GF_NACT_DEVICE if capture else sc.n_active_host
         prof.fused("normalize")
         prof.fused("gravity_bc")
         pre = None
@@ -851,33 +998,36 @@ class FlipSolver3D:
             prof.fused("divergence")
         with prof.section("pressure"):
             if pressure_solver == "jacobi":
-                _lib.call("S2.6.1", "gf_scene_jacobi", ctypes.byref(s), na[0],
+                _lib.call("S2.6.1", "gf_scene_jacobi", ctypes.byref(s), nact,
                           pressure_iters, stream)
-                self.last_pressure_iters = pressure_iters
-                self.last_pressure_residual = None
+                self._pressure_stats = (pressure_iters, None, 0)
             elif pressure_solver == "gsrb":
-                _lib.call("S2.6.2", "gf_scene_gsrb", ctypes.byref(s), na[0],
+                _lib.call("S2.6.2", "gf_scene_gsrb", ctypes.byref(s), nact,
                           pressure_iters, stream)
-                self.last_pressure_iters = pressure_iters
-                self.last_pressure_residual = None
+                self._pressure_stats = (pressure_iters, None, 0)
             else:
                 prec = _lib.GF_PCG_MIXED if self.pcg_precision == "mixed" else _lib.GF_PCG_FP64
                 if prec == _lib.GF_PCG_FP64:
                     sc.ensure_fp64()
                     s = self._descriptor()
-                it = ctypes.c_int(0)
-                res = ctypes.c_double(0.0)
-                nrep = ctypes.c_int(0)
                 sign = -1 if self.pcg_sign == "physical" else 1
-                kms = (ctypes.c_double * 4)() if self.pcg_kernel_timing else None
-                _lib.call("S2.6.3", "gf_scene_pcg_sync", ctypes.byref(s), na[0], prec, tol,
-                          pressure_iters, self.pcg_check_every, sign, ctypes.byref(it),
-                          ctypes.byref(res), ctypes.byref(nrep), kms, stream)
-                if kms is not None:
-                    self.last_pcg_kernel_ms = tuple(kms)
-                self.last_pressure_iters = it.value
-                self.last_pressure_residual = res.value
-                self.last_pressure_replacements = nrep.value
+                ce = self.pcg_check_every
+                if capture:
+                    # the whole iteration cap is enqueued; the chunks after
+                    # the stop pass through (on-device stop flag)
+                    _lib.call("S2.6.3", "gf_scene_pcg_enqueue", ctypes.byref(s), nact, prec, tol,
+                              pressure_iters, ce, sign, 0, -(-pressure_iters // ce), stream)
+                else:
+                    it = ctypes.c_int(0)
+                    res = ctypes.c_double(0.0)
+                    nrep = ctypes.c_int(0)
+                    kms = (ctypes.c_double * 4)() if self.pcg_kernel_timing else None
+                    _lib.call("S2.6.3", "gf_scene_pcg_sync", ctypes.byref(s), nact, prec, tol,
+                              pressure_iters, ce, sign, ctypes.byref(it),
+                              ctypes.byref(res), ctypes.byref(nrep), kms, stream)
+                    if kms is not None:
+                        self.last_pcg_kernel_ms = tuple(kms)
+                    self._pressure_stats = (it.value, res.value, nrep.value)
                 self._pcg_tol_sq.t.fill_(float(np.float32(tol) ** 2))
         with prof.section("g2p_advect"):
             if pre is None:
@@ -909,6 +1059,7 @@ class FlipSolver3D:
     # -------------------------------------------------------------- queries
     @property
     def n_active_tiles(self) -> int:
+        self._stats()
         return self._sc.n_active_host
 
     def fluid_cell_count(self) -> int:

@@ -81,6 +81,14 @@ int gf_tune_get(const char* key, int* value);
 #define GF_ATTR_COLOR 0      /* value * (1/w), solver3d.py:607-631 */
 #define GF_ATTR_SCALAR 1     /* value / w, solver3d.py:710-727 */
 
+/* n_active argument of gf_scene_jacobi, gf_scene_gsrb and gf_scene_pcg_enqueue
+ * (one scene): take the active tile count from s->n_active[0] on the device,
+ * as written by gf_scene_sort_sync, instead of a host value.  The launches
+ * are then sized for every tile of the grid and do not depend on the count,
+ * so a whole substep can be captured in a hipGraph and replayed while the
+ * fluid moves (the reference's device n_active, solver3d.py:3369-3385). */
+#define GF_NACT_DEVICE (-1)
+
 /* Pressure precision of gf_scene_pcg_sync. */
 #define GF_PCG_MIXED 0       /* FP32 vectors, FP64 dots, FP64 residual replacement */
 #define GF_PCG_FP64 1        /* plain FP64 PCG */
@@ -182,14 +190,15 @@ int gf_scene_p2g_raw(const gf_scene_t* s, int transfer_mode, float* uw, float* v
 
 /* Jacobi pressure iterations on the compact active tiles, p = 0 initially;
  * result in s->pc. Follows k3_jacobi_pressure (solver3d.py:878-938) and its
- * driver (solver3d.py:3570-3600). */
+ * driver (solver3d.py:3570-3600).  n_active may be GF_NACT_DEVICE. */
 int gf_scene_jacobi(const gf_scene_t* s, int n_active, int iters, void* stream);
 
 /* `iters` red-black Gauss-Seidel sweeps (red = (i+j+k) even, then black) of
  * A p = -div in place on the compact tiles from p = 0 (the pressure_solver
  * "gsrb" branch of solver3d.py:3603-3630, sweep k3_gauss_seidel_rb
  * 1204-1267; tile-compact like the reference's per-tile variant
- * k3_gauss_seidel_rb_per_tile).  Result in s->pc. */
+ * k3_gauss_seidel_rb_per_tile).  Result in s->pc.  n_active may be
+ * GF_NACT_DEVICE. */
 int gf_scene_gsrb(const gf_scene_t* s, int n_active, int iters, void* stream);
 
 /* Diagonal-preconditioned CG for A p = rhs_sign * div on the compact tiles
@@ -217,9 +226,13 @@ int gf_scene_pcg_sync(const gf_scene_t* s, int n_active, int precision, double t
  * of check_every iterations (each with its chunk-end decision), then the
  * result into s->pc.  Scenes that stop early pass through the remaining
  * launches.  A later call with first_chunk = the chunks already enqueued
- * resumes the same solve.  Results: gf_scene_pcg_stats.  No reference
- * counterpart (the reference's graph path fixes the iteration count,
- * solver3d.py:3361-3419). */
+ * resumes the same solve.  Results: gf_scene_pcg_stats.  For one scene,
+ * n_active may be GF_NACT_DEVICE: tiles are split over workgroups as with the
+ * host count, and results are bit-identical to it when the grid has fewer than
+ * 3072 tiles or the PCG form is pinned (gf_tune_set "pcg_variant"); otherwise
+ * the form is chosen from the grid's tile count rather than the active one.
+ * The reference's graph path (solver3d.py:3361-3419) runs its PCG kernels
+ * under an on-device stop flag in the same way. */
 int gf_scene_pcg_enqueue(const gf_scene_t* s, int n_active, int precision, double tol,
                          int max_iter, int check_every, int rhs_sign, int first_chunk,
                          int n_chunks, void* stream);
