@@ -106,6 +106,9 @@ SIGNATURES = {
     "gf_scene_g2p": [SP, V, I, F, F, F, F, V],
     "gf_scene_attr_transfer": [SP, V, I, I, V, V, V, V],
     "gf_anim_obstacles": [V, I, V, V, V, V, V, I, I, I, D, V],
+    "gf_mesh_workspace": [I, I, I, V],
+    "gf_mesh_bin_sync": [V, I, I, I, I, D, V, V, V],
+    "gf_mesh_mark": [V, I, I, I, I, D, V, V, c_int64, V, V, V, V, F, F, F, V],
     "gf_reseed_workspace": [I, I, I, I, V],
     "gf_reseed_plan_sync": [V, I, V, I, I, I, F, I, I, I, V, PI, V],
     "gf_reseed_compact": [V, I, I, I, I, I, V, V, V],

@@ -520,14 +520,27 @@ class FlipSolver3D:
     @block("D4.6", "Register an analytic obstacle that moves per frame")
     def add_animated_obstacle(self, kind: str, base_center, motion, **params):
         """kind: 'sphere' (radius) | 'box' (half_size) | 'cylinder_y' (radius,
-        half_height), moving by a LinearMotion / KeyframeMotion
-        (solver3d.py:2516-2519).  Mesh obstacles are not part of this build."""
-        if kind not in ("sphere", "box", "cylinder_y"):
-            if kind == "mesh":
-                raise NotImplementedError("animated mesh obstacles are not part of this build")
+        half_height) | 'mesh' (mesh_path to an OBJ or STL file, optional scale
+        and translate; or the triangles themselves as ``triangles`` (n,3,3)),
+        moving by a LinearMotion / KeyframeMotion (solver3d.py:2516-2519).  A
+        mesh moves by its centre's displacement from ``base_center``
+        (solver3d.py:2604-2612); its file is read once, here."""
+        if kind not in ("sphere", "box", "cylinder_y", "mesh"):
             raise ValueError(f"unsupported animated obstacle kind: {kind!r}")
-        self._anim_specs.append({"kind": kind, "base_center": tuple(base_center),
-                                 "motion": motion, "params": dict(params)})
+        spec = {"kind": kind, "base_center": tuple(base_center), "motion": motion,
+                "params": dict(params)}
+        if kind == "mesh":
+            from .meshes import read_mesh
+            if "triangles" in params:
+                tri = np.asarray(params["triangles"], dtype=np.float64).reshape(-1, 3, 3)
+            elif "mesh_path" in params:
+                tri = read_mesh(params["mesh_path"])
+            else:
+                raise ValueError("a mesh obstacle needs mesh_path or triangles")
+            # FP64 vertices times scale; the frame's translation is added in
+            # stamping, then the triangles are rounded to FP32
+            spec["mesh_base"] = tri * float(params.get("scale", 1.0))
+        self._anim_specs.append(spec)
 
     def _obstacle_velocity_at(self, spec, frame_idx: int):
         """Linear motion: its velocity; keyframes: centred difference of the
@@ -554,10 +567,17 @@ class FlipSolver3D:
         for sdf in self._static_obstacle_sdfs:
             base[sdf <= 0.0] = 2
         rows = []
+        meshes = []
         kinds = {"sphere": 0, "box": 1, "cylinder_y": 2}
         for spec in self._anim_specs:
             c = evaluate_center(spec["base_center"], spec["motion"], frame_idx)
             p = spec["params"]
+            if spec["kind"] == "mesh":
+                # translate + (centre - base centre), in FP32 (solver3d.py:2609-2612)
+                delta = np.asarray(c, np.float32) - np.asarray(spec["base_center"], np.float32)
+                t_total = np.asarray(p.get("translate", (0.0, 0.0, 0.0)), np.float32) + delta
+                meshes.append((spec, t_total, self._obstacle_velocity_at(spec, frame_idx)))
+                continue
             if spec["kind"] == "sphere":
                 par = (p["radius"], 0.0, 0.0)
             elif spec["kind"] == "box":
@@ -574,6 +594,14 @@ class FlipSolver3D:
                   _lib.ptr(f["marker"]), _lib.ptr(f["solid_u"]), _lib.ptr(f["solid_v"]),
                   _lib.ptr(f["solid_w"]), self.nx, self.ny, self.nz, float(self.dx),
                   self._stream())
+        if meshes:
+            # after the analytic ones, in list order: each mesh's inside cells
+            # not yet solid, and its velocity on their faces (2598-2631)
+            from .meshes import mark_mesh, place
+            faces = (f["solid_u"], f["solid_v"], f["solid_w"])
+            for spec, t_total, vel in meshes:
+                mark_mesh(f["marker"], place(spec["mesh_base"], t_total), self.dx, faces, vel,
+                          self._stream())
         self._marker_host = f["marker"].cpu().numpy()
 
     @block("F3.6", "Per-frame hook: rebuild marker for anim obstacles, emit inflow particles, "

@@ -333,6 +333,28 @@ int gf_anim_obstacles(const float* obs, int n_obs, const int32_t* base_marker, i
                       float* solid_u, float* solid_v, float* solid_w, int nx, int ny, int nz,
                       double dx, void* stream);
 
+/* Solid cells inside a closed triangle mesh (D4.3.GPU, schemes/mesh_marker.py:
+ * 39-80, 197-236), as prepare_frame applies animated mesh obstacles
+ * (solver3d.py:2598-2631).  tris: n_tris triangles of 9 floats (v0, v1, v2).
+ * A cell is inside when the ray from its FP32 centre ((i + 0.5) * dx rounded
+ * once) along +x crosses an odd number of triangles (the reference's
+ * Moller-Trumbore test with direction (1,0,0), t > 1e-6, |det| > 1e-12).
+ * gf_mesh_workspace: int32 entries of the workspace for the grid.
+ * gf_mesh_bin_sync: bins the triangles into the grid's (j, k) columns and
+ * returns the length of the column lists (sync: one 8-byte readback).
+ * gf_mesh_mark: with the same triangles and workspace and a list of that
+ * length, sets marker = 2 on every inside cell that is not yet 2 and, when
+ * the three face arrays are given (else all NULL), writes (vx, vy, vz) into
+ * the six faces of each such cell (k3_write_solid_face_vel, solver3d.py:
+ * 816-836).  The binning changes no result: a crossing count's parity does
+ * not depend on the order of the triangles. */
+int gf_mesh_workspace(int nx, int ny, int nz, int64_t* ws_ints);
+int gf_mesh_bin_sync(const float* tris, int n_tris, int nx, int ny, int nz, double dx, int32_t* ws,
+                     int64_t* n_entries, void* stream);
+int gf_mesh_mark(const float* tris, int n_tris, int nx, int ny, int nz, double dx, int32_t* ws,
+                 int32_t* list, int64_t n_entries, int32_t* marker, float* solid_u, float* solid_v,
+                 float* solid_w, float vx, float vy, float vz, void* stream);
+
 /* Per-particle attribute transfer at the end of a substep (S2.15 colour,
  * solver3d.py:3333-3359 with k3_p2g_color / k3_normalize_color /
  * k3_g2p_color 562-664; S2.18 scalar, 3304-3330 with k3_p2g_scalar /

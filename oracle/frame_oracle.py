@@ -10,7 +10,10 @@ the GPU and ``rng.choice`` picks on the host).
 
 Also the analytic animated obstacles of prepare_frame (solver3d.py:3918-3940,
 2537-2596, SDFs primitives/sdf.py:30-66): marker stamping and the per-
-obstacle face-velocity writes of k3_write_solid_face_vel (solver3d.py:818-836).
+obstacle face-velocity writes of k3_write_solid_face_vel (solver3d.py:818-836);
+and the mesh obstacles after them (solver3d.py:2598-2631): the +x ray-parity
+inside test of k3_mesh_to_solid (schemes/mesh_marker.py:39-80), in FP32 with
+every operation rounded on its own, by brute force over all triangles.
 
 How each part is pinned (tests/test_oracle_frame.py): a hand-worked 3-cell
 case; invariants (every fluid cell ends with at least min_per_cell
@@ -18,7 +21,11 @@ particles, survivors keep their order, emitted particles lie in their cell,
 counts add up); a per-particle loop for the kept set; the outflow mask
 against a per-particle loop with boundary points; obstacle cells against an
 FP64 per-cell distance loop, face velocities against a per-face rule
-(last obstacle containing either cell).
+(last obstacle containing either cell).  The mesh test
+(tests/test_oracle_mesh.py): a closed cube mesh gives the analytic box, a
+polyhedral sphere and torus (non-convex: rays cross up to four times) the
+analytic shapes away from their chord error, reversed winding changes
+nothing, and a per-cell per-triangle scalar loop agrees cell for cell.
 """
 from __future__ import annotations
 
@@ -129,11 +136,61 @@ def obstacle_sdf(grid, kind, center, params):
             + np.minimum(np.maximum(dr, da), 0.0))
 
 
-def stamp_obstacles(base_marker, dx, obstacles):
-    """obstacles: [(kind, centre, params, velocity)] in list order.  Returns
-    (marker, solid_u, solid_v, solid_w): cells with sdf <= 0 solid, faces
-    cleared then written obstacle by obstacle around their inside cells
-    (k3_clear_solid_vel / k3_write_solid_face_vel, solver3d.py:803-836)."""
+def _ray_hits(v, px, py, pz):
+    """Moller-Trumbore with direction (1,0,0) for one triangle v (9 FP32
+    values) and FP32 ray origins (mesh_marker.py:51-76): crossing mask."""
+    f32 = np.float32
+    v = np.asarray(v, np.float32).reshape(9)
+    e1x, e1y, e1z = v[3] - v[0], v[4] - v[1], v[5] - v[2]
+    e2x, e2y, e2z = v[6] - v[0], v[7] - v[1], v[8] - v[2]
+    hx, hy, hz = f32(0.0), -e2z, e2y
+    det = e1x * hx + e1y * hy + e1z * hz
+    if not abs(det) > f32(1.0e-12):
+        return np.zeros(np.shape(px), dtype=bool)
+    f = f32(1.0) / det
+    sx, sy, sz = px - v[0], py - v[1], pz - v[2]
+    u = f * (sx * hx + sy * hy + sz * hz)
+    qx = sy * e1z - sz * e1y
+    w = f * qx
+    qy = sz * e1x - sx * e1z
+    qz = sx * e1y - sy * e1x
+    t = f * (e2x * qx + e2y * qy + e2z * qz)
+    return (u >= 0) & (u <= 1) & (w >= 0) & (u + w <= 1) & (t > f32(1.0e-6))
+
+
+def mesh_inside(shape, dx, tris):
+    """Cells whose FP32 centre's +x ray crosses an odd number of triangles
+    (k3_mesh_to_solid, mesh_marker.py:39-80); tris (n, 3, 3)."""
+    g = cell_centers(shape, dx)
+    px, py, pz = g[..., 0], g[..., 1], g[..., 2]
+    count = np.zeros(shape, dtype=np.int64)
+    with np.errstate(all="ignore"):
+        for v in np.asarray(tris, np.float32).reshape(-1, 9):
+            count += _ray_hits(v, px, py, pz)
+    return (count & 1) == 1
+
+
+def mesh_inside_loop(shape, dx, tris):
+    """The same test one cell and one triangle at a time (tiny grids)."""
+    g = cell_centers(shape, dx)
+    out = np.zeros(shape, dtype=bool)
+    tv = np.asarray(tris, np.float32).reshape(-1, 9)
+    with np.errstate(all="ignore"):
+        for idx in np.ndindex(*shape):
+            p = g[idx]
+            n = sum(bool(_ray_hits(v, p[0], p[1], p[2])) for v in tv)
+            out[idx] = n % 2 == 1
+    return out
+
+
+def stamp_obstacles(base_marker, dx, obstacles, meshes=()):
+    """obstacles: [(kind, centre, params, velocity)] in list order; meshes:
+    [(triangles, velocity)] applied after them.  Returns (marker, solid_u,
+    solid_v, solid_w): cells with sdf <= 0 solid, faces cleared then written
+    obstacle by obstacle around their inside cells (k3_clear_solid_vel /
+    k3_write_solid_face_vel, solver3d.py:803-836); then each mesh's inside
+    cells that were not yet solid become solid and write the mesh's velocity
+    into their six faces (solver3d.py:2598-2631)."""
     marker = np.array(base_marker, copy=True)
     nx, ny, nz = marker.shape
     su = np.zeros((nx + 1, ny, nz), np.float32)
@@ -150,4 +207,14 @@ def stamp_obstacles(base_marker, dx, obstacles):
         sv[:, 1:][inside] = v[1]
         sw[:, :, :-1][inside] = v[2]
         sw[:, :, 1:][inside] = v[2]
+    for tris, vel in meshes:
+        new = mesh_inside(marker.shape, dx, tris) & (marker != 2)
+        marker[new] = 2
+        v = np.asarray(vel, np.float32)
+        su[:-1][new] = v[0]
+        su[1:][new] = v[0]
+        sv[:, :-1][new] = v[1]
+        sv[:, 1:][new] = v[1]
+        sw[:, :, :-1][new] = v[2]
+        sw[:, :, 1:][new] = v[2]
     return marker, su, sv, sw
