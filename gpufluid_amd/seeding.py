@@ -99,10 +99,12 @@ def config_c1() -> Scene:
 
 
 def config_c2() -> Scene:
-    """128^3 APIC dam break, seed 1, dt 1/480, FP32 PCG + FP64 refinement to 1e-8."""
+    """128^3 APIC dam break, seed 1, dt 1/480, FP32 PCG + FP64 refinement to 1e-8
+    (no iteration cap in the recipe: 2000 is a safety bound, never reached)."""
     s = dam_break(128, seed=1)
     s.name = "c2_apic_dam_break_128"
-    s.params = dict(transfer="apic", dt=1.0 / 480.0, tol=1e-8, substeps=50, precision="mixed")
+    s.params = dict(transfer="apic", dt=1.0 / 480.0, tol=1e-8, max_iter=2000, substeps=50,
+                    precision="mixed")
     return s
 
 
