@@ -106,6 +106,14 @@ SIGNATURES = {
     "gf_scene_g2p": [SP, V, I, F, F, F, F, V],
     "gf_scene_attr_transfer": [SP, V, I, I, V, V, V, V],
     "gf_anim_obstacles": [V, I, V, V, V, V, V, I, I, I, D, V],
+    "gf_k3_mark_active_blocks": [V, V, I, I, I, I, I, I, I, V],
+    "gf_k3_jacobi_pressure_blocksparse": [V, V, V, V, V, I, I, I, I, I, I, I, V],
+    "gf_k3_jacobi_pressure_per_tile": [V, V, V, V, V, I, I, V, I, I, I, V],
+    "gf_k3_gauss_seidel_rb_per_tile": [V, V, V, V, I, I, I, V, I, I, I, V],
+    "gf_k3_apply_A_per_tile": [V, V, V, V, I, I, V, V, I, I, I, V],
+    "gf_k3_apply_invM_per_tile": [V, V, V, V, V, I, I, V, V, I, I, I, V],
+    "gf_k3_dot_fluid_per_tile": [V, V, V, V, V, I, I, V, V, I, I, I, V],
+    "gf_k3_axpy_devscalar_per_tile": [V, F, V, V, V, V, V, I, I, V, V, I, I, I, V],
     "gf_mesh_workspace": [I, I, I, V],
     "gf_mesh_bin_sync": [V, I, I, I, I, D, V, V, V],
     "gf_mesh_mark": [V, I, I, I, I, D, V, V, c_int64, V, V, V, V, F, F, F, V],

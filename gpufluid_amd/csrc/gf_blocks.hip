@@ -18,6 +18,38 @@ __device__ __forceinline__ bool idx3(int I, int J, int K, int& i, int& j, int& k
 
 inline dim3 grid_for(long long n) { return dim3((unsigned)((n + GF_WG - 1) / GF_WG)); }
 
+// Cells a pressure block kernel visits: every cell (coords NULL), or the
+// per-tile forms' cells (solver3d.py:2020-2035, 1120-1160): thread t covers
+// cell (di, dj, dk) = t mod bs^3 (dk fastest) of listed block t / bs^3, for
+// the first n_active[0] blocks of coords (block coordinates, 3 ints each),
+// skipping cells outside the grid.
+struct Dom {
+    const int32_t* coords;
+    const int32_t* n_active;
+    int bs;
+};
+
+__device__ __forceinline__ bool dom_cell(const Dom& d, int nx, int ny, int nz, int& i, int& j,
+                                         int& k) {
+    if (!d.coords) return idx3(nx, ny, nz, i, j, k);
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long cpb = (long long)d.bs * d.bs * d.bs;
+    const long long blk = t / cpb;
+    if (blk >= (long long)d.n_active[0]) return false;
+    const int rem = (int)(t - blk * cpb);
+    i = d.coords[3 * blk + 0] * d.bs + rem / (d.bs * d.bs);
+    j = d.coords[3 * blk + 1] * d.bs + (rem / d.bs) % d.bs;
+    k = d.coords[3 * blk + 2] * d.bs + rem % d.bs;
+    return i >= 0 && j >= 0 && k >= 0 && i < nx && j < ny && k < nz;
+}
+
+// Block-sparse skip of k3_jacobi_pressure_blocksparse (solver3d.py:975-1041):
+// cells of blocks with active[bi, bj, bk] == 0 get p_out = 0.
+struct BlkMask {
+    const int32_t* active;  // (nbx, nby, nbz) or NULL
+    int bs, nby, nbz;
+};
+
 // solver3d.py:41-65
 __global__ void k_clear_grid(float* u, float* v, float* w, float* uw, float* vw, float* ww,
                              int32_t* marker, int nx, int ny, int nz) {
@@ -149,10 +181,16 @@ __device__ __forceinline__ void stencil(const float* x, const int32_t* marker, i
 }
 
 __global__ void k_jacobi_dense(const float* p_in, float* p_out, const float* div,
-                               const int32_t* marker, int nx, int ny, int nz) {
+                               const int32_t* marker, int nx, int ny, int nz, Dom dom,
+                               BlkMask bm) {
     int i, j, k;
-    if (!idx3(nx, ny, nz, i, j, k)) return;
+    if (!dom_cell(dom, nx, ny, nz, i, j, k)) return;
     size_t c = ((size_t)i * ny + j) * nz + k;
+    if (bm.active &&
+        bm.active[((size_t)(i / bm.bs) * bm.nby + j / bm.bs) * bm.nbz + k / bm.bs] == 0) {
+        p_out[c] = 0.0f;
+        return;
+    }
     if (marker[c] != 1) { p_out[c] = 0.0f; return; }
     float s, d;
     stencil(p_in, marker, i, j, k, nx, ny, nz, s, d);
@@ -163,9 +201,9 @@ __global__ void k_jacobi_dense(const float* p_in, float* p_out, const float* div
 // One red-black Gauss-Seidel half sweep in place (solver3d.py:1204-1267):
 // cells with (i+j+k) % 2 == color.
 __global__ void k_gsrb_dense(float* p, const float* div, const int32_t* marker, int nx, int ny,
-                             int nz, int color) {
+                             int nz, int color, Dom dom) {
     int i, j, k;
-    if (!idx3(nx, ny, nz, i, j, k)) return;
+    if (!dom_cell(dom, nx, ny, nz, i, j, k)) return;
     if (((i + j + k) & 1) != color) return;
     size_t c = ((size_t)i * ny + j) * nz + k;
     if (marker[c] != 1) { p[c] = 0.0f; return; }
@@ -174,11 +212,14 @@ __global__ void k_gsrb_dense(float* p, const float* div, const int32_t* marker, 
     p[c] = d < 0.5f ? 0.0f : (s - div[c]) / d;
 }
 
+// Products and sums rounded one by one (no fused multiply-add), as the FP32
+// oracle computes them.
 __global__ void k_apply_A_dense(const float* x, float* y, const int32_t* marker,
-                                const int32_t* done, int nx, int ny, int nz) {
+                                const int32_t* done, int nx, int ny, int nz, Dom dom) {
+#pragma clang fp contract(off)
     if (done && done[0] != 0) return;
     int i, j, k;
-    if (!idx3(nx, ny, nz, i, j, k)) return;
+    if (!dom_cell(dom, nx, ny, nz, i, j, k)) return;
     size_t c = ((size_t)i * ny + j) * nz + k;
     if (marker[c] != 1) { y[c] = 0.0f; return; }
     float s, d;
@@ -203,21 +244,26 @@ __global__ void k_compute_diag_dense(const int32_t* marker, float* diag, int nx,
 }
 
 __global__ void k_apply_invM_dense(const float* r, const float* diag, float* z,
-                                   const int32_t* marker, const int32_t* done, long long n) {
+                                   const int32_t* marker, const int32_t* done, int nx, int ny,
+                                   int nz, Dom dom) {
     if (done && done[0] != 0) return;
-    long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n) return;
+    int i, j, k;
+    if (!dom_cell(dom, nx, ny, nz, i, j, k)) return;
+    const size_t c = ((size_t)i * ny + j) * nz + k;
     z[c] = marker[c] != 1 ? 0.0f : r[c] / diag[c];
 }
 
 __global__ __launch_bounds__(GF_WG) void k_dot_dense(const float* a, const float* b, float* out,
                                                      const int32_t* marker, const int32_t* done,
-                                                     long long n) {
+                                                     int nx, int ny, int nz, Dom dom) {
     __shared__ double s_w[GF_WG / GF_WAVE];
     if (done && done[0] != 0) return;
-    long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int i, j, k;
     double v = 0.0;
-    if (c < n && marker[c] == 1) v = (double)(a[c] * b[c]);
+    if (dom_cell(dom, nx, ny, nz, i, j, k)) {
+        const size_t c = ((size_t)i * ny + j) * nz + k;
+        if (marker[c] == 1) v = (double)(a[c] * b[c]);
+    }
     v = gf::wave_sum(v);
     if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = v;
     __syncthreads();
@@ -230,13 +276,25 @@ __global__ __launch_bounds__(GF_WG) void k_dot_dense(const float* a, const float
 
 __global__ void k_axpy_devscalar_dense(const float* a_dev, float sign, const float* x,
                                        const float* y_in, float* y_out, const int32_t* marker,
-                                       const int32_t* done, long long n) {
+                                       const int32_t* done, int nx, int ny, int nz, Dom dom) {
+#pragma clang fp contract(off)
     if (done && done[0] != 0) return;
-    long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n) return;
+    int i, j, k;
+    if (!dom_cell(dom, nx, ny, nz, i, j, k)) return;
+    const size_t c = ((size_t)i * ny + j) * nz + k;
     if (marker[c] != 1) { y_out[c] = 0.0f; return; }
     float a = a_dev[0] * sign;
     y_out[c] = y_in[c] + a * x[c];
+}
+
+// k_mark_active_blocks (solver3d.py:952-969): a fluid cell sets its block's
+// flag to 1 (every writer stores the same value); other flags are untouched.
+__global__ void k_mark_blocks(const int32_t* marker, int32_t* active, int nx, int ny, int nz,
+                              int bs, int nby, int nbz) {
+    int i, j, k;
+    if (!idx3(nx, ny, nz, i, j, k)) return;
+    if (marker[((size_t)i * ny + j) * nz + k] != 1) return;
+    active[((size_t)(i / bs) * nby + j / bs) * nbz + k / bs] = 1;
 }
 
 __global__ void k_scalar_op(int op, float* a, float* b, float* c, float tol, int32_t* done) {
@@ -813,7 +871,8 @@ extern "C" int gf_k3_jacobi_pressure(const float* p_in, float* p_out, const floa
     GF_REQUIRE(p_in && p_out && div && marker && p_in != p_out,
                "gf_k3_jacobi_pressure: bad buffers");
     hipLaunchKernelGGL(k_jacobi_dense, grid_for((long long)nx * ny * nz), dim3(GF_WG), 0,
-                       (hipStream_t)stream, p_in, p_out, div, marker, nx, ny, nz);
+                       (hipStream_t)stream, p_in, p_out, div, marker, nx, ny, nz, Dom{},
+                       BlkMask{});
     GF_LAUNCH_CHECK("k_jacobi_dense");
     return 0;
 }
@@ -824,7 +883,7 @@ extern "C" int gf_k3_gauss_seidel_rb(float* p, const float* div, const int32_t* 
     GF_REQUIRE(p && div && marker, "gf_k3_gauss_seidel_rb: null buffer");
     GF_REQUIRE(color == 0 || color == 1, "gf_k3_gauss_seidel_rb: color %d", color);
     hipLaunchKernelGGL(k_gsrb_dense, grid_for((long long)nx * ny * nz), dim3(GF_WG), 0,
-                       (hipStream_t)stream, p, div, marker, nx, ny, nz, color);
+                       (hipStream_t)stream, p, div, marker, nx, ny, nz, color, Dom{});
     GF_LAUNCH_CHECK("k_gsrb_dense");
     return 0;
 }
@@ -873,7 +932,7 @@ extern "C" int gf_k3_apply_A(const float* x, float* y, const int32_t* marker,
     GRID_ARGS_OK("gf_k3_apply_A");
     GF_REQUIRE(x && y && marker && x != y, "gf_k3_apply_A: bad buffers");
     hipLaunchKernelGGL(k_apply_A_dense, grid_for((long long)nx * ny * nz), dim3(GF_WG), 0,
-                       (hipStream_t)stream, x, y, marker, done, nx, ny, nz);
+                       (hipStream_t)stream, x, y, marker, done, nx, ny, nz, Dom{});
     GF_LAUNCH_CHECK("k_apply_A_dense");
     return 0;
 }
@@ -895,7 +954,7 @@ extern "C" int gf_k3_apply_invM(const float* r, const float* diag, float* z,
     GF_REQUIRE(r && diag && z && marker, "gf_k3_apply_invM: null buffer");
     long long n = (long long)nx * ny * nz;
     hipLaunchKernelGGL(k_apply_invM_dense, grid_for(n), dim3(GF_WG), 0, (hipStream_t)stream, r,
-                       diag, z, marker, done, n);
+                       diag, z, marker, done, nx, ny, nz, Dom{});
     GF_LAUNCH_CHECK("k_apply_invM_dense");
     return 0;
 }
@@ -906,7 +965,7 @@ extern "C" int gf_k3_dot_fluid(const float* a, const float* b, float* out, const
     GF_REQUIRE(a && b && out && marker, "gf_k3_dot_fluid: null buffer");
     long long n = (long long)nx * ny * nz;
     hipLaunchKernelGGL(k_dot_dense, grid_for(n), dim3(GF_WG), 0, (hipStream_t)stream, a, b, out,
-                       marker, done, n);
+                       marker, done, nx, ny, nz, Dom{});
     GF_LAUNCH_CHECK("k_dot_dense");
     return 0;
 }
@@ -918,8 +977,157 @@ extern "C" int gf_k3_axpy_devscalar(const float* a_dev, float sign, const float*
     GF_REQUIRE(a_dev && x && y_in && y_out && marker, "gf_k3_axpy_devscalar: null buffer");
     long long n = (long long)nx * ny * nz;
     hipLaunchKernelGGL(k_axpy_devscalar_dense, grid_for(n), dim3(GF_WG), 0, (hipStream_t)stream,
-                       a_dev, sign, x, y_in, y_out, marker, done, n);
+                       a_dev, sign, x, y_in, y_out, marker, done, nx, ny, nz, Dom{});
     GF_LAUNCH_CHECK("k_axpy_devscalar_dense");
+    return 0;
+}
+
+// ---- block-sparse and per-tile forms (S2.16, S2.6.4-S2.6.6) ----------------
+
+#define BLOCKS_COVER_OK(who)                                                           \
+    GF_REQUIRE(bs >= 1 && bs <= 64 && (long long)nbx * bs >= nx &&                     \
+                   (long long)nby * bs >= ny && (long long)nbz * bs >= nz,              \
+               who ": %d^3 blocks %dx%dx%d do not cover the grid", bs, nbx, nby, nbz)
+
+#define TILES_OK(who)                                                                  \
+    GF_REQUIRE(coords && n_active && bs >= 1 && bs <= 64 && max_blocks >= 0 &&         \
+                   (long long)max_blocks * bs * bs * bs <= (long long)INT32_MAX,                  \
+               who ": tile list (%d blocks of %d^3)", max_blocks, bs)
+
+namespace {
+inline Dom tiles(const int32_t* coords, const int32_t* n_active, int bs) {
+    Dom d;
+    d.coords = coords; d.n_active = n_active; d.bs = bs;
+    return d;
+}
+inline dim3 tile_grid(int max_blocks, int bs) {
+    return grid_for((long long)max_blocks * bs * bs * bs);
+}
+}  // namespace
+
+extern "C" int gf_k3_mark_active_blocks(const int32_t* marker, int32_t* block_active, int nx,
+                                        int ny, int nz, int bs, int nbx, int nby, int nbz,
+                                        void* stream) {
+    GRID_ARGS_OK("gf_k3_mark_active_blocks");
+    BLOCKS_COVER_OK("gf_k3_mark_active_blocks");
+    GF_REQUIRE(marker && block_active, "gf_k3_mark_active_blocks: null buffer");
+    hipLaunchKernelGGL(k_mark_blocks, grid_for((long long)nx * ny * nz), dim3(GF_WG), 0,
+                       (hipStream_t)stream, marker, block_active, nx, ny, nz, bs, nby, nbz);
+    GF_LAUNCH_CHECK("k_mark_blocks");
+    return 0;
+}
+
+extern "C" int gf_k3_jacobi_pressure_blocksparse(const float* p_in, float* p_out, const float* div,
+                                                 const int32_t* marker,
+                                                 const int32_t* block_active, int nx, int ny,
+                                                 int nz, int bs, int nbx, int nby, int nbz,
+                                                 void* stream) {
+    GRID_ARGS_OK("gf_k3_jacobi_pressure_blocksparse");
+    BLOCKS_COVER_OK("gf_k3_jacobi_pressure_blocksparse");
+    GF_REQUIRE(p_in && p_out && div && marker && block_active && p_in != p_out,
+               "gf_k3_jacobi_pressure_blocksparse: bad buffers");
+    BlkMask bm;
+    bm.active = block_active; bm.bs = bs; bm.nby = nby; bm.nbz = nbz;
+    hipLaunchKernelGGL(k_jacobi_dense, grid_for((long long)nx * ny * nz), dim3(GF_WG), 0,
+                       (hipStream_t)stream, p_in, p_out, div, marker, nx, ny, nz, Dom{}, bm);
+    GF_LAUNCH_CHECK("k_jacobi_dense (block-sparse)");
+    return 0;
+}
+
+extern "C" int gf_k3_jacobi_pressure_per_tile(const float* p_in, float* p_out, const float* div,
+                                              const int32_t* marker, const int32_t* coords,
+                                              int max_blocks, int bs, const int32_t* n_active,
+                                              int nx, int ny, int nz, void* stream) {
+    GRID_ARGS_OK("gf_k3_jacobi_pressure_per_tile");
+    TILES_OK("gf_k3_jacobi_pressure_per_tile");
+    GF_REQUIRE(p_in && p_out && div && marker && p_in != p_out,
+               "gf_k3_jacobi_pressure_per_tile: bad buffers");
+    if (max_blocks == 0) return 0;
+    hipLaunchKernelGGL(k_jacobi_dense, tile_grid(max_blocks, bs), dim3(GF_WG), 0,
+                       (hipStream_t)stream, p_in, p_out, div, marker, nx, ny, nz,
+                       tiles(coords, n_active, bs), BlkMask{});
+    GF_LAUNCH_CHECK("k_jacobi_dense (per tile)");
+    return 0;
+}
+
+extern "C" int gf_k3_gauss_seidel_rb_per_tile(float* p, const float* div, const int32_t* marker,
+                                              const int32_t* coords, int max_blocks, int bs,
+                                              int color, const int32_t* n_active, int nx, int ny,
+                                              int nz, void* stream) {
+    GRID_ARGS_OK("gf_k3_gauss_seidel_rb_per_tile");
+    TILES_OK("gf_k3_gauss_seidel_rb_per_tile");
+    GF_REQUIRE(p && div && marker, "gf_k3_gauss_seidel_rb_per_tile: null buffer");
+    GF_REQUIRE(color == 0 || color == 1, "gf_k3_gauss_seidel_rb_per_tile: color %d", color);
+    if (max_blocks == 0) return 0;
+    hipLaunchKernelGGL(k_gsrb_dense, tile_grid(max_blocks, bs), dim3(GF_WG), 0,
+                       (hipStream_t)stream, p, div, marker, nx, ny, nz, color,
+                       tiles(coords, n_active, bs));
+    GF_LAUNCH_CHECK("k_gsrb_dense (per tile)");
+    return 0;
+}
+
+extern "C" int gf_k3_apply_A_per_tile(const float* x, float* y, const int32_t* marker,
+                                      const int32_t* coords, int max_blocks, int bs,
+                                      const int32_t* n_active, const int32_t* done, int nx,
+                                      int ny, int nz, void* stream) {
+    GRID_ARGS_OK("gf_k3_apply_A_per_tile");
+    TILES_OK("gf_k3_apply_A_per_tile");
+    GF_REQUIRE(x && y && marker && x != y, "gf_k3_apply_A_per_tile: bad buffers");
+    if (max_blocks == 0) return 0;
+    hipLaunchKernelGGL(k_apply_A_dense, tile_grid(max_blocks, bs), dim3(GF_WG), 0,
+                       (hipStream_t)stream, x, y, marker, done, nx, ny, nz,
+                       tiles(coords, n_active, bs));
+    GF_LAUNCH_CHECK("k_apply_A_dense (per tile)");
+    return 0;
+}
+
+extern "C" int gf_k3_apply_invM_per_tile(const float* r, const float* diag, float* z,
+                                         const int32_t* marker, const int32_t* coords,
+                                         int max_blocks, int bs, const int32_t* n_active,
+                                         const int32_t* done, int nx, int ny, int nz,
+                                         void* stream) {
+    GRID_ARGS_OK("gf_k3_apply_invM_per_tile");
+    TILES_OK("gf_k3_apply_invM_per_tile");
+    GF_REQUIRE(r && diag && z && marker, "gf_k3_apply_invM_per_tile: null buffer");
+    if (max_blocks == 0) return 0;
+    hipLaunchKernelGGL(k_apply_invM_dense, tile_grid(max_blocks, bs), dim3(GF_WG), 0,
+                       (hipStream_t)stream, r, diag, z, marker, done, nx, ny, nz,
+                       tiles(coords, n_active, bs));
+    GF_LAUNCH_CHECK("k_apply_invM_dense (per tile)");
+    return 0;
+}
+
+extern "C" int gf_k3_dot_fluid_per_tile(const float* a, const float* b, float* out,
+                                        const int32_t* marker, const int32_t* coords,
+                                        int max_blocks, int bs, const int32_t* n_active,
+                                        const int32_t* done, int nx, int ny, int nz,
+                                        void* stream) {
+    GRID_ARGS_OK("gf_k3_dot_fluid_per_tile");
+    TILES_OK("gf_k3_dot_fluid_per_tile");
+    GF_REQUIRE(a && b && out && marker, "gf_k3_dot_fluid_per_tile: null buffer");
+    if (max_blocks == 0) return 0;
+    hipLaunchKernelGGL(k_dot_dense, tile_grid(max_blocks, bs), dim3(GF_WG), 0,
+                       (hipStream_t)stream, a, b, out, marker, done, nx, ny, nz,
+                       tiles(coords, n_active, bs));
+    GF_LAUNCH_CHECK("k_dot_dense (per tile)");
+    return 0;
+}
+
+extern "C" int gf_k3_axpy_devscalar_per_tile(const float* a_dev, float sign, const float* x,
+                                             const float* y_in, float* y_out,
+                                             const int32_t* marker, const int32_t* coords,
+                                             int max_blocks, int bs, const int32_t* n_active,
+                                             const int32_t* done, int nx, int ny, int nz,
+                                             void* stream) {
+    GRID_ARGS_OK("gf_k3_axpy_devscalar_per_tile");
+    TILES_OK("gf_k3_axpy_devscalar_per_tile");
+    GF_REQUIRE(a_dev && x && y_in && y_out && marker,
+               "gf_k3_axpy_devscalar_per_tile: null buffer");
+    if (max_blocks == 0) return 0;
+    hipLaunchKernelGGL(k_axpy_devscalar_dense, tile_grid(max_blocks, bs), dim3(GF_WG), 0,
+                       (hipStream_t)stream, a_dev, sign, x, y_in, y_out, marker, done, nx, ny,
+                       nz, tiles(coords, n_active, bs));
+    GF_LAUNCH_CHECK("k_axpy_devscalar_dense (per tile)");
     return 0;
 }
 

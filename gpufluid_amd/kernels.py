@@ -287,3 +287,107 @@ def k3_g2p_apic_advect(pos, vel, affine_C, u, v, w, dx, dt, nx, ny, nz, dom,
 def k3_vmax_sq_reduce(vel, out_max_sq):
     n = int(_t(vel).reshape(-1, 3).shape[0])
     _lib.call("S2.10.GPU", "gf_k3_vmax_sq_reduce", _p(vel), n, _p(out_max_sq), _st(vel))
+
+
+# ---------------------------------------------- block-sparse and per-tile forms
+def _blocks3(block_id, block_active, bs):
+    t = _t(block_active)
+    if t.dim() != 3 or t.dtype != torch.int32:
+        raise BlockError(block_id, "block_active must be an int32 (nbx, nby, nbz) array")
+    return (int(bs), *(int(x) for x in t.shape))
+
+
+def _tiles(block_id, active_coords, n_active_dev, bs):
+    c = _t(active_coords)
+    if c.dtype != torch.int32 or c.numel() % 3:
+        raise BlockError(block_id, "active_coords must be int32 block coordinates, 3 per block")
+    if _t(n_active_dev).dtype != torch.int32:
+        raise BlockError(block_id, "n_active_dev must be an int32 device array")
+    return _p(active_coords), c.numel() // 3, int(bs), _p(n_active_dev)
+
+
+@block("S2.16", "Active-block bitmask builder (8^3 tiles with any fluid cell)")
+def k_mark_active_blocks(marker, block_active, block_size):
+    """k_mark_active_blocks (solver3d.py:952-969): set block_active[b] = 1 for
+    every block holding a fluid cell (others keep their value)."""
+    nx, ny, nz = _cells(marker)
+    bs, nbx, nby, nbz = _blocks3("S2.16", block_active, block_size)
+    _lib.call("S2.16", "gf_k3_mark_active_blocks", _p(marker), _p(block_active), nx, ny, nz, bs,
+              nbx, nby, nbz, _st(marker))
+
+
+@block("S2.6.4", "Block-sparse Jacobi pressure (skip inactive 8^3 blocks)")
+def k3_jacobi_pressure_blocksparse(p_in, p_out, div, marker, block_active, block_size):
+    """k3_jacobi_pressure_blocksparse (solver3d.py:975-1041)."""
+    nx, ny, nz = _cells(marker)
+    bs, nbx, nby, nbz = _blocks3("S2.6.4", block_active, block_size)
+    _lib.call("S2.6.4", "gf_k3_jacobi_pressure_blocksparse", _p(p_in), _p(p_out), _p(div),
+              _p(marker), _p(block_active), nx, ny, nz, bs, nbx, nby, nbz, _st(p_in))
+
+
+@block("S2.6.4", "Per-tile Jacobi pressure: launch only on active 8^3 blocks")
+def k3_jacobi_pressure_per_tile(p_in, p_out, div, marker, active_coords, block_size,
+                                n_active_dev, off_x=0, off_y=0, off_z=0):
+    """k3_jacobi_pressure_per_tile (solver3d.py:1116-1197): the Jacobi sweep on
+    the cells of the first n_active_dev[0] listed blocks."""
+    _offsets("S2.6.4", off_x, off_y, off_z)
+    nx, ny, nz = _cells(marker)
+    co, nb, bs, na = _tiles("S2.6.4", active_coords, n_active_dev, block_size)
+    _lib.call("S2.6.4", "gf_k3_jacobi_pressure_per_tile", _p(p_in), _p(p_out), _p(div),
+              _p(marker), co, nb, bs, na, nx, ny, nz, _st(p_in))
+
+
+@block("S2.6.5", "Per-tile GS-RB: red-black sweep on active 8^3 blocks only")
+def k3_gauss_seidel_rb_per_tile(p, div, marker, active_coords, block_size, color, n_active_dev,
+                                off_x=0, off_y=0, off_z=0):
+    """k3_gauss_seidel_rb_per_tile (solver3d.py:1270-1357)."""
+    _offsets("S2.6.5", off_x, off_y, off_z)
+    nx, ny, nz = _cells(marker)
+    co, nb, bs, na = _tiles("S2.6.5", active_coords, n_active_dev, block_size)
+    _lib.call("S2.6.5", "gf_k3_gauss_seidel_rb_per_tile", _p(p), _p(div), _p(marker), co, nb,
+              bs, int(color), na, nx, ny, nz, _st(p))
+
+
+@block("S2.6.6", "Per-tile PCG: apply A on active blocks")
+def k3_apply_A_per_tile(x, y, marker, active_coords, block_size, n_active_dev, done,
+                        off_x=0, off_y=0, off_z=0):
+    """k3_apply_A_per_tile (solver3d.py:2038-2105)."""
+    _offsets("S2.6.6", off_x, off_y, off_z)
+    nx, ny, nz = _cells(marker)
+    co, nb, bs, na = _tiles("S2.6.6", active_coords, n_active_dev, block_size)
+    _lib.call("S2.6.6", "gf_k3_apply_A_per_tile", _p(x), _p(y), _p(marker), co, nb, bs, na,
+              _p(done), nx, ny, nz, _st(x))
+
+
+@block("S2.6.6", "Per-tile PCG: apply M^-1 on active blocks")
+def k3_apply_invM_per_tile(r, diag, z, marker, active_coords, block_size, n_active_dev, done,
+                           off_x=0, off_y=0, off_z=0):
+    """k3_apply_invM_per_tile (solver3d.py:2108-2141)."""
+    _offsets("S2.6.6", off_x, off_y, off_z)
+    nx, ny, nz = _cells(marker)
+    co, nb, bs, na = _tiles("S2.6.6", active_coords, n_active_dev, block_size)
+    _lib.call("S2.6.6", "gf_k3_apply_invM_per_tile", _p(r), _p(diag), _p(z), _p(marker), co, nb,
+              bs, na, _p(done), nx, ny, nz, _st(r))
+
+
+@block("S2.6.6", "Per-tile PCG: fluid-cell dot product on active blocks")
+def k3_dot_fluid_per_tile(a, b, out, marker, active_coords, block_size, n_active_dev, done,
+                          off_x=0, off_y=0, off_z=0):
+    """k3_dot_fluid_per_tile (solver3d.py:2144-2176): adds the fluid cells'
+    a*b of the listed blocks into out[0]."""
+    _offsets("S2.6.6", off_x, off_y, off_z)
+    nx, ny, nz = _cells(marker)
+    co, nb, bs, na = _tiles("S2.6.6", active_coords, n_active_dev, block_size)
+    _lib.call("S2.6.6", "gf_k3_dot_fluid_per_tile", _p(a), _p(b), _p(out), _p(marker), co, nb,
+              bs, na, _p(done), nx, ny, nz, _st(a))
+
+
+@block("S2.6.6", "Per-tile PCG: AXPY with device-side scalar on active blocks")
+def k3_axpy_devscalar_per_tile(a_dev, sign, x, y_in, y_out, marker, active_coords, block_size,
+                               n_active_dev, done, off_x=0, off_y=0, off_z=0):
+    """k3_axpy_devscalar_per_tile (solver3d.py:2179-2212)."""
+    _offsets("S2.6.6", off_x, off_y, off_z)
+    nx, ny, nz = _cells(marker)
+    co, nb, bs, na = _tiles("S2.6.6", active_coords, n_active_dev, block_size)
+    _lib.call("S2.6.6", "gf_k3_axpy_devscalar_per_tile", _p(a_dev), float(sign), _p(x),
+              _p(y_in), _p(y_out), _p(marker), co, nb, bs, na, _p(done), nx, ny, nz, _st(x))

@@ -444,6 +444,52 @@ int gf_k3_dot_fluid(const float* a, const float* b, float* out, const int32_t* m
 int gf_k3_axpy_devscalar(const float* a_dev, float sign, const float* x,
                          const float* y_in, float* y_out, const int32_t* marker,
                          const int32_t* done, int nx, int ny, int nz, void* stream);
+/* Active-block bitmask (S2.16, k_mark_active_blocks solver3d.py:952-969):
+ * block_active (nbx, nby, nbz) of bs^3 blocks covering the grid; a block with
+ * a fluid cell is set to 1, others are left as they are. */
+int gf_k3_mark_active_blocks(const int32_t* marker, int32_t* block_active, int nx, int ny, int nz,
+                             int bs, int nbx, int nby, int nbz, void* stream);
+
+/* Block-sparse Jacobi sweep (S2.6.4, k3_jacobi_pressure_blocksparse
+ * solver3d.py:975-1041): gf_k3_jacobi_pressure with p_out = 0 in blocks whose
+ * flag is 0. */
+int gf_k3_jacobi_pressure_blocksparse(const float* p_in, float* p_out, const float* div,
+                                      const int32_t* marker, const int32_t* block_active, int nx,
+                                      int ny, int nz, int bs, int nbx, int nby, int nbz,
+                                      void* stream);
+
+/* Per-tile forms (S2.6.4 k3_jacobi_pressure_per_tile solver3d.py:1116-1197,
+ * S2.6.5 k3_gauss_seidel_rb_per_tile 1270-1357, S2.6.6 k3_apply_A_per_tile /
+ * k3_apply_invM_per_tile / k3_dot_fluid_per_tile / k3_axpy_devscalar_per_tile
+ * 2020-2212): the dense kernel of the same name restricted to the cells of
+ * the first n_active[0] (a device value) of the max_blocks blocks listed in
+ * coords (3 ints per block: block coordinates of bs^3 cells); cells outside
+ * those blocks are not touched.  Launches are sized by max_blocks, so the
+ * count may change between graph replays. */
+int gf_k3_jacobi_pressure_per_tile(const float* p_in, float* p_out, const float* div,
+                                   const int32_t* marker, const int32_t* coords, int max_blocks,
+                                   int bs, const int32_t* n_active, int nx, int ny, int nz,
+                                   void* stream);
+int gf_k3_gauss_seidel_rb_per_tile(float* p, const float* div, const int32_t* marker,
+                                   const int32_t* coords, int max_blocks, int bs, int color,
+                                   const int32_t* n_active, int nx, int ny, int nz, void* stream);
+int gf_k3_apply_A_per_tile(const float* x, float* y, const int32_t* marker, const int32_t* coords,
+                           int max_blocks, int bs, const int32_t* n_active, const int32_t* done,
+                           int nx, int ny, int nz, void* stream);
+int gf_k3_apply_invM_per_tile(const float* r, const float* diag, float* z, const int32_t* marker,
+                              const int32_t* coords, int max_blocks, int bs,
+                              const int32_t* n_active, const int32_t* done, int nx, int ny,
+                              int nz, void* stream);
+int gf_k3_dot_fluid_per_tile(const float* a, const float* b, float* out, const int32_t* marker,
+                             const int32_t* coords, int max_blocks, int bs,
+                             const int32_t* n_active, const int32_t* done, int nx, int ny, int nz,
+                             void* stream);
+int gf_k3_axpy_devscalar_per_tile(const float* a_dev, float sign, const float* x,
+                                  const float* y_in, float* y_out, const int32_t* marker,
+                                  const int32_t* coords, int max_blocks, int bs,
+                                  const int32_t* n_active, const int32_t* done, int nx, int ny,
+                                  int nz, void* stream);
+
 /* one-thread scalar kernels, solver3d.py:1874-1939:
  * op 0 k3_zero_scalar(s=a), 1 k3_div_scalar(out=c, numer=a, denom=b),
  * 2 k3_copy_scalar(src=a, dst=c), 3 k3_set_tol_sq(r0=a, tol, out=c),
