@@ -106,6 +106,14 @@ SIGNATURES = {
     "gf_scene_g2p": [SP, V, I, F, F, F, F, V],
     "gf_scene_attr_transfer": [SP, V, I, I, V, V, V, V],
     "gf_anim_obstacles": [V, I, V, V, V, V, V, I, I, I, D, V],
+    "gf_k3_indicator_from_marker": [V, V, I, I, I, V],
+    "gf_k3_box_blur_centred": [V, V, I, I, I, V],
+    "gf_k3_csf_normal": [V, V, V, V, I, I, I, F, V],
+    "gf_k3_csf_curvature": [V, V, V, V, I, I, I, F, V],
+    "gf_k3_csf_apply": [I, V, V, V, V, F, F, I, I, I, V],
+    "gf_k3_csf_sum_workspace": [I, I, I, V],
+    "gf_k3_csf_sum": [I, V, V, V, V, V, I, I, I, V, V],
+    "gf_k3_csf_subtract_bias": [I, V, V, V, V, I, I, I, V],
     "gf_k3_mark_active_blocks": [V, V, I, I, I, I, I, I, I, V],
     "gf_k3_jacobi_pressure_blocksparse": [V, V, V, V, V, I, I, I, I, I, I, I, V],
     "gf_k3_jacobi_pressure_per_tile": [V, V, V, V, V, I, I, V, I, I, I, V],

@@ -113,10 +113,22 @@ constexpr int CSF_CHUNK = 32;
 struct FaceArgs {
     float* u; float* v; float* w;
     const float* chi; const float* kappa; const int32_t* marker;
-    double* red;          // per workgroup: sum, count; then 3 means
+    double* red;          // per workgroup: sum, count; then 3 means (NULL: not kept)
     int nx, ny, nz, gx, gy;
     float coeff, dx;
+    int comp;             // >= 0: that component only (grid z = chunks), else all three
 };
+
+// Component and i-chunk of this workgroup.
+__device__ __forceinline__ void face_part(const FaceArgs& a, int& c, int& chunk) {
+    if (a.comp >= 0) {
+        c = a.comp;
+        chunk = blockIdx.z;
+    } else {
+        c = blockIdx.z % 3;
+        chunk = blockIdx.z / 3;
+    }
+}
 
 __host__ __device__ __forceinline__ int face_slots(const FaceArgs& a) {
     const int nchunk = (a.nx + 1 + CSF_CHUNK - 1) / CSF_CHUNK;
@@ -170,7 +182,8 @@ __device__ __forceinline__ void block_sum2(double& s, double& n, double* sh) {
 __global__ __launch_bounds__(GF_WG) void k_csf_apply(FaceArgs a) {
 #pragma clang fp contract(off)
     __shared__ double sh[2 * GF_WG / 64];
-    const int c = blockIdx.z % 3, chunk = blockIdx.z / 3;
+    int c, chunk;
+    face_part(a, c, chunk);
     const int k = blockIdx.x * 64 + (threadIdx.x & 63);
     const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
     const int fx = a.nx + (c == 0), fy = a.ny + (c == 1), fz = a.nz + (c == 2);
@@ -192,11 +205,77 @@ __global__ __launch_bounds__(GF_WG) void k_csf_apply(FaceArgs a) {
             }
         }
     }
+    if (!a.red) return;
     block_sum2(s, n, sh);
     if (threadIdx.x == 0) {
         const size_t slot = blockIdx.x + (size_t)a.gx * (blockIdx.y + (size_t)a.gy * blockIdx.z);
         a.red[2 * slot] = s;
         a.red[2 * slot + 1] = n;
+    }
+}
+
+// k3_csf_sum_u/v/w (solver3d.py:400-464) for one component: per-workgroup
+// FP64 partials of (after - before) and of the face count over the faces
+// next to fluid; k_csf_sum_fin adds their fixed-order total, rounded to FP32,
+// to sum[0] and count[0].
+__global__ __launch_bounds__(GF_WG) void k_csf_sum(FaceArgs a, const float* __restrict__ before,
+                                                   const float* __restrict__ after) {
+#pragma clang fp contract(off)
+    __shared__ double sh[2 * GF_WG / 64];
+    const int c = a.comp, chunk = blockIdx.z;
+    const int k = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int fx = a.nx + (c == 0), fy = a.ny + (c == 1), fz = a.nz + (c == 2);
+    double s = 0.0, n = 0.0;
+    if (k < fz && j < fy) {
+        const int i1 = min((chunk + 1) * CSF_CHUNK, fx);
+        for (int i = chunk * CSF_CHUNK; i < i1; ++i) {
+            const Face q = face_at(a, c, i, j, k);
+            if (!q.fluid) continue;
+            s += (double)(after[q.f] - before[q.f]);
+            n += 1.0;
+        }
+    }
+    block_sum2(s, n, sh);
+    if (threadIdx.x == 0) {
+        const size_t slot = blockIdx.x + (size_t)a.gx * (blockIdx.y + (size_t)a.gy * blockIdx.z);
+        a.red[2 * slot] = s;
+        a.red[2 * slot + 1] = n;
+    }
+}
+
+__global__ __launch_bounds__(GF_WG) void k_csf_sum_fin(const double* red, int slots, float* sum,
+                                                       float* count) {
+    __shared__ double sh[2 * GF_WG / 64];
+    double s = 0.0, n = 0.0;
+    for (int e = threadIdx.x; e < slots; e += GF_WG) {
+        s += red[2 * (size_t)e];
+        n += red[2 * (size_t)e + 1];
+    }
+    block_sum2(s, n, sh);
+    if (threadIdx.x == 0) {
+        sum[0] = sum[0] + (float)s;
+        count[0] = count[0] + (float)n;
+    }
+}
+
+// k3_csf_subtract_bias_*_dev (solver3d.py:478-547) for one component:
+// faces next to fluid lose sum[0] / count[0] when count[0] > 0.5.
+__global__ __launch_bounds__(GF_WG) void k_csf_bias(FaceArgs a, const float* __restrict__ sum,
+                                                    const float* __restrict__ count) {
+    const int c = a.comp, chunk = blockIdx.z;
+    const int k = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int fx = a.nx + (c == 0), fy = a.ny + (c == 1), fz = a.nz + (c == 2);
+    if (k >= fz || j >= fy) return;
+    const float cnt = count[0];
+    if (!(cnt > 0.5f)) return;
+    const float mean = sum[0] / cnt;
+    float* vel = c == 0 ? a.u : (c == 1 ? a.v : a.w);
+    const int i1 = min((chunk + 1) * CSF_CHUNK, fx);
+    for (int i = chunk * CSF_CHUNK; i < i1; ++i) {
+        const Face q = face_at(a, c, i, j, k);
+        if (q.fluid) vel[q.f] = vel[q.f] - mean;
     }
 }
 
@@ -223,7 +302,8 @@ __global__ __launch_bounds__(GF_WG) void k_csf_mean(double* red, int slots, int 
 
 // k3_csf_subtract_bias_*_dev (solver3d.py:472-547)
 __global__ __launch_bounds__(GF_WG) void k_csf_balance(FaceArgs a) {
-    const int c = blockIdx.z % 3, chunk = blockIdx.z / 3;
+    int c, chunk;
+    face_part(a, c, chunk);
     const int k = blockIdx.x * 64 + (threadIdx.x & 63);
     const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
     const int fx = a.nx + (c == 0), fy = a.ny + (c == 1), fz = a.nz + (c == 2);
@@ -285,11 +365,137 @@ extern "C" int gf_csf_apply(float* u, float* v, float* w, const int32_t* marker,
     a.gx = (nz + 1 + 63) / 64;
     a.gy = (ny + 1 + 3) / 4;
     a.coeff = coeff; a.dx = dx;
+    a.comp = -1;
     const int nchunk = (nx + 1 + CSF_CHUNK - 1) / CSF_CHUNK;
     const dim3 fg((unsigned)a.gx, (unsigned)a.gy, (unsigned)(3 * nchunk));
     hipLaunchKernelGGL(k_csf_apply, fg, dim3(GF_WG), 0, st, a);
     hipLaunchKernelGGL(k_csf_mean, dim3(3), dim3(GF_WG), 0, st, red, face_slots(a), nchunk);
     hipLaunchKernelGGL(k_csf_balance, fg, dim3(GF_WG), 0, st, a);
     GF_LAUNCH_CHECK("surface tension");
+    return 0;
+}
+
+// ---- the S2.14 sub-blocks one at a time (dense arrays, reference meaning) --
+
+namespace {
+
+int csf_grid_ok(int nx, int ny, int nz, const char* who) {
+    GF_REQUIRE(nx > 0 && ny > 0 && nz > 0, "%s: grid %d x %d x %d", who, nx, ny, nz);
+    GF_REQUIRE(nx + 1 <= 65535 && ny + 1 <= 4 * 65535, "%s: grid %d x %d x %d", who, nx, ny, nz);
+    return 0;
+}
+
+FaceArgs one_comp(int comp, float* vel, const int32_t* marker, int nx, int ny, int nz) {
+    FaceArgs a = {};
+    a.u = comp == 0 ? vel : nullptr;
+    a.v = comp == 1 ? vel : nullptr;
+    a.w = comp == 2 ? vel : nullptr;
+    a.marker = marker;
+    a.nx = nx; a.ny = ny; a.nz = nz;
+    a.gx = (nz + 1 + 63) / 64;
+    a.gy = (ny + 1 + 3) / 4;
+    a.comp = comp;
+    return a;
+}
+
+dim3 comp_grid(const FaceArgs& a) {
+    return dim3((unsigned)a.gx, (unsigned)a.gy, (unsigned)((a.nx + 1 + CSF_CHUNK - 1) / CSF_CHUNK));
+}
+
+}  // namespace
+
+extern "C" int gf_k3_indicator_from_marker(const int32_t* marker, float* chi, int nx, int ny,
+                                           int nz, void* stream) {
+    int rc = csf_grid_ok(nx, ny, nz, "gf_k3_indicator_from_marker");
+    if (rc) return rc;
+    GF_REQUIRE(marker && chi, "gf_k3_indicator_from_marker: null buffer");
+    hipLaunchKernelGGL(k_csf_indicator, cell_grid(nx, ny, nz), dim3(GF_WG), 0,
+                       (hipStream_t)stream, marker, chi, nx, ny, nz);
+    GF_LAUNCH_CHECK("k_csf_indicator");
+    return 0;
+}
+
+extern "C" int gf_k3_box_blur_centred(const float* src, float* dst, int nx, int ny, int nz,
+                                      void* stream) {
+    int rc = csf_grid_ok(nx, ny, nz, "gf_k3_box_blur_centred");
+    if (rc) return rc;
+    GF_REQUIRE(src && dst && src != dst, "gf_k3_box_blur_centred: bad buffers");
+    hipLaunchKernelGGL(k_csf_blur, cell_grid(nx, ny, nz), dim3(GF_WG), 0, (hipStream_t)stream,
+                       src, dst, nx, ny, nz);
+    GF_LAUNCH_CHECK("k_csf_blur");
+    return 0;
+}
+
+extern "C" int gf_k3_csf_normal(const float* chi, float* n_x, float* n_y, float* n_z, int nx,
+                                int ny, int nz, float dx, void* stream) {
+    int rc = csf_grid_ok(nx, ny, nz, "gf_k3_csf_normal");
+    if (rc) return rc;
+    GF_REQUIRE(chi && n_x && n_y && n_z, "gf_k3_csf_normal: null buffer");
+    hipLaunchKernelGGL(k_csf_normal, cell_grid(nx, ny, nz), dim3(GF_WG), 0, (hipStream_t)stream,
+                       chi, n_x, n_y, n_z, nx, ny, nz, dx);
+    GF_LAUNCH_CHECK("k_csf_normal");
+    return 0;
+}
+
+extern "C" int gf_k3_csf_curvature(const float* n_x, const float* n_y, const float* n_z,
+                                   float* kappa, int nx, int ny, int nz, float dx, void* stream) {
+    int rc = csf_grid_ok(nx, ny, nz, "gf_k3_csf_curvature");
+    if (rc) return rc;
+    GF_REQUIRE(n_x && n_y && n_z && kappa, "gf_k3_csf_curvature: null buffer");
+    hipLaunchKernelGGL(k_csf_curvature, cell_grid(nx, ny, nz), dim3(GF_WG), 0,
+                       (hipStream_t)stream, n_x, n_y, n_z, kappa, nx, ny, nz, dx);
+    GF_LAUNCH_CHECK("k_csf_curvature");
+    return 0;
+}
+
+extern "C" int gf_k3_csf_apply(int comp, float* vel, const float* chi, const float* kappa,
+                               const int32_t* marker, float coeff, float dx, int nx, int ny,
+                               int nz, void* stream) {
+    int rc = csf_grid_ok(nx, ny, nz, "gf_k3_csf_apply");
+    if (rc) return rc;
+    GF_REQUIRE(comp >= 0 && comp <= 2, "gf_k3_csf_apply: component %d", comp);
+    GF_REQUIRE(vel && chi && kappa && marker, "gf_k3_csf_apply: null buffer");
+    FaceArgs a = one_comp(comp, vel, marker, nx, ny, nz);
+    a.chi = chi; a.kappa = kappa; a.coeff = coeff; a.dx = dx;
+    hipLaunchKernelGGL(k_csf_apply, comp_grid(a), dim3(GF_WG), 0, (hipStream_t)stream, a);
+    GF_LAUNCH_CHECK("k_csf_apply");
+    return 0;
+}
+
+extern "C" int gf_k3_csf_sum_workspace(int nx, int ny, int nz, int64_t* red_doubles) {
+    GF_REQUIRE(nx > 0 && ny > 0 && nz > 0 && red_doubles, "gf_k3_csf_sum_workspace: bad arguments");
+    const int64_t gx = (nz + 1 + 63) / 64, gy = (ny + 1 + 3) / 4;
+    *red_doubles = 2 * gx * gy * ((nx + 1 + CSF_CHUNK - 1) / CSF_CHUNK);
+    return 0;
+}
+
+extern "C" int gf_k3_csf_sum(int comp, const float* before, const float* after,
+                             const int32_t* marker, float* sum, float* count, int nx, int ny,
+                             int nz, double* red, void* stream) {
+    int rc = csf_grid_ok(nx, ny, nz, "gf_k3_csf_sum");
+    if (rc) return rc;
+    GF_REQUIRE(comp >= 0 && comp <= 2, "gf_k3_csf_sum: component %d", comp);
+    GF_REQUIRE(before && after && marker && sum && count && red, "gf_k3_csf_sum: null buffer");
+    FaceArgs a = one_comp(comp, nullptr, marker, nx, ny, nz);
+    a.red = red;
+    const dim3 g = comp_grid(a);
+    hipLaunchKernelGGL(k_csf_sum, g, dim3(GF_WG), 0, (hipStream_t)stream, a, before, after);
+    hipLaunchKernelGGL(k_csf_sum_fin, dim3(1), dim3(GF_WG), 0, (hipStream_t)stream, red,
+                       (int)(g.x * g.y * g.z), sum, count);
+    GF_LAUNCH_CHECK("k_csf_sum");
+    return 0;
+}
+
+extern "C" int gf_k3_csf_subtract_bias(int comp, float* vel, const int32_t* marker,
+                                       const float* sum, const float* count, int nx, int ny,
+                                       int nz, void* stream) {
+    int rc = csf_grid_ok(nx, ny, nz, "gf_k3_csf_subtract_bias");
+    if (rc) return rc;
+    GF_REQUIRE(comp >= 0 && comp <= 2, "gf_k3_csf_subtract_bias: component %d", comp);
+    GF_REQUIRE(vel && marker && sum && count, "gf_k3_csf_subtract_bias: null buffer");
+    FaceArgs a = one_comp(comp, vel, marker, nx, ny, nz);
+    hipLaunchKernelGGL(k_csf_bias, comp_grid(a), dim3(GF_WG), 0, (hipStream_t)stream, a, sum,
+                       count);
+    GF_LAUNCH_CHECK("k_csf_bias");
     return 0;
 }

@@ -391,3 +391,152 @@ def k3_axpy_devscalar_per_tile(a_dev, sign, x, y_in, y_out, marker, active_coord
     co, nb, bs, na = _tiles("S2.6.6", active_coords, n_active_dev, block_size)
     _lib.call("S2.6.6", "gf_k3_axpy_devscalar_per_tile", _p(a_dev), float(sign), _p(x),
               _p(y_in), _p(y_out), _p(marker), co, nb, bs, na, _p(done), nx, ny, nz, _st(x))
+
+
+# ------------------------------------------------ surface tension sub-blocks
+def _grid3(a):
+    t = _t(a)
+    if t.dim() != 3:
+        raise ValueError("expected a 3-D cell array")
+    return tuple(int(x) for x in t.shape)
+
+
+@block("S2.14.1", "Build raw fluid indicator field from marker (pre-blur)")
+def k3_indicator_from_marker(marker, chi, off_x=0, off_y=0, off_z=0):
+    """k3_indicator_from_marker (solver3d.py:212-229): chi = 1 on fluid, else 0."""
+    _offsets("S2.14.1", off_x, off_y, off_z)
+    nx, ny, nz = _cells(marker)
+    _lib.call("S2.14.1", "gf_k3_indicator_from_marker", _p(marker), _p(chi), nx, ny, nz,
+              _st(marker))
+
+
+@block("S2.14.1", "Box-blur smoothing pass for chi -> chi~")
+def k3_box_blur_centred(src, dst):
+    """k3_box_blur_centred (solver3d.py:235-256): mean of the in-range 3x3x3
+    neighbourhood."""
+    nx, ny, nz = _grid3(src)
+    _lib.call("S2.14.1", "gf_k3_box_blur_centred", _p(src), _p(dst), nx, ny, nz, _st(src))
+
+
+@block("S2.14.2", "Unit-normal field n = grad chi~ / |grad chi~| on cell centres")
+def k3_csf_normal(chi, n_x, n_y, n_z, dx):
+    """k3_csf_normal (solver3d.py:262-286)."""
+    nx, ny, nz = _grid3(chi)
+    _lib.call("S2.14.2", "gf_k3_csf_normal", _p(chi), _p(n_x), _p(n_y), _p(n_z), nx, ny, nz,
+              float(dx), _st(chi))
+
+
+@block("S2.14.3", "Curvature kappa = -div n")
+def k3_csf_curvature(n_x, n_y, n_z, kappa, dx):
+    """k3_csf_curvature (solver3d.py:292-313)."""
+    nx, ny, nz = _grid3(kappa)
+    _lib.call("S2.14.3", "gf_k3_csf_curvature", _p(n_x), _p(n_y), _p(n_z), _p(kappa), nx, ny,
+              nz, float(dx), _st(kappa))
+
+
+def _csf_apply(comp, vel, chi, kappa, marker, sigma_dt_over_rho, dx, nx, ny, nz, offs):
+    _offsets("S2.14.4", *offs)
+    shape = _cells(marker)
+    if (int(nx), int(ny), int(nz)) != shape:
+        raise BlockError("S2.14.4", f"grid ({nx},{ny},{nz}) does not match marker {shape}")
+    _lib.call("S2.14.4", "gf_k3_csf_apply", comp, _p(vel), _p(chi), _p(kappa), _p(marker),
+              float(sigma_dt_over_rho), float(dx), *shape, _st(vel))
+
+
+@block("S2.14.4", "Apply CSF impulse (sigma*kappa*grad chi~ * dt/rho) to MAC u faces")
+def k3_csf_apply_u(u, chi, kappa, marker, sigma_dt_over_rho, dx, nx, ny, nz,
+                   off_x=0, off_y=0, off_z=0):
+    """k3_csf_apply_u (solver3d.py:316-345)."""
+    _csf_apply(0, u, chi, kappa, marker, sigma_dt_over_rho, dx, nx, ny, nz, (off_x, off_y, off_z))
+
+
+@block("S2.14.4", "Apply CSF impulse to MAC v faces")
+def k3_csf_apply_v(v, chi, kappa, marker, sigma_dt_over_rho, dx, nx, ny, nz,
+                   off_x=0, off_y=0, off_z=0):
+    """k3_csf_apply_v (solver3d.py:348-365)."""
+    _csf_apply(1, v, chi, kappa, marker, sigma_dt_over_rho, dx, nx, ny, nz, (off_x, off_y, off_z))
+
+
+@block("S2.14.4", "Apply CSF impulse to MAC w faces")
+def k3_csf_apply_w(w_, chi, kappa, marker, sigma_dt_over_rho, dx, nx, ny, nz,
+                   off_x=0, off_y=0, off_z=0):
+    """k3_csf_apply_w (solver3d.py:368-385)."""
+    _csf_apply(2, w_, chi, kappa, marker, sigma_dt_over_rho, dx, nx, ny, nz, (off_x, off_y, off_z))
+
+
+@block("S2.14.5", "Host helper: capillary-wave dt_max for explicit CSF")
+def csf_max_stable_dt(rho: float, dx: float, surface_tension: float, safety: float = 0.9) -> float:
+    """csf_max_stable_dt (solver3d.py:2284-2307): safety * sqrt(rho dx^3 /
+    (2 pi sigma)), +inf without surface tension."""
+    import math
+    if surface_tension <= 0.0:
+        return float("inf")
+    return float(safety) * math.sqrt(max(rho, 1e-12) * (dx ** 3) / (2.0 * math.pi * surface_tension))
+
+
+_CSF_RED: dict = {}
+
+
+def _csf_sum(comp, before, after, marker, total, count, nx, ny, nz, offs):
+    _offsets("S2.14.6", *offs)
+    shape = _cells(marker)
+    if (int(nx), int(ny), int(nz)) != shape:
+        raise BlockError("S2.14.6", f"grid ({nx},{ny},{nz}) does not match marker {shape}")
+    dev = _t(marker).device
+    red = _CSF_RED.get((shape, dev))
+    if red is None:
+        n = ctypes.c_int64()
+        _lib.call("S2.14.6", "gf_k3_csf_sum_workspace", *shape, ctypes.byref(n))
+        red = torch.empty(n.value, dtype=torch.float64, device=dev)
+        _CSF_RED[(shape, dev)] = red
+    _lib.call("S2.14.6", "gf_k3_csf_sum", comp, _p(before), _p(after), _p(marker), _p(total),
+              _p(count), *shape, _lib.ptr(red), _st(marker))
+
+
+@block("S2.14.6", "Force-balance CSF: sum the u impulse over faces next to fluid")
+def k3_csf_sum_u(u_before, u_after, marker, sum_du, count, nx, ny, nz, off_x=0, off_y=0, off_z=0):
+    """k3_csf_sum_u (solver3d.py:400-420): adds sum(after - before) and the
+    face count over u faces next to fluid to sum_du[0] and count[0]."""
+    _csf_sum(0, u_before, u_after, marker, sum_du, count, nx, ny, nz, (off_x, off_y, off_z))
+
+
+@block("S2.14.6", "Force-balance CSF: sum the v impulse")
+def k3_csf_sum_v(v_before, v_after, marker, sum_dv, count, nx, ny, nz, off_x=0, off_y=0, off_z=0):
+    """k3_csf_sum_v (solver3d.py:423-443)."""
+    _csf_sum(1, v_before, v_after, marker, sum_dv, count, nx, ny, nz, (off_x, off_y, off_z))
+
+
+@block("S2.14.6", "Force-balance CSF: sum the w impulse")
+def k3_csf_sum_w(w_before, w_after, marker, sum_dw, count, nx, ny, nz, off_x=0, off_y=0, off_z=0):
+    """k3_csf_sum_w (solver3d.py:446-464)."""
+    _csf_sum(2, w_before, w_after, marker, sum_dw, count, nx, ny, nz, (off_x, off_y, off_z))
+
+
+def _csf_bias(comp, vel, marker, total, count, nx, ny, nz, offs):
+    _offsets("S2.14.6", *offs)
+    shape = _cells(marker)
+    if (int(nx), int(ny), int(nz)) != shape:
+        raise BlockError("S2.14.6", f"grid ({nx},{ny},{nz}) does not match marker {shape}")
+    _lib.call("S2.14.6", "gf_k3_csf_subtract_bias", comp, _p(vel), _p(marker), _p(total),
+              _p(count), *shape, _st(vel))
+
+
+@block("S2.14.6", "CSF force-balance on u (device-side, no host sync)")
+def k3_csf_subtract_bias_u_dev(u, marker, sum_dev, count_dev, nx, ny, nz,
+                               off_x=0, off_y=0, off_z=0):
+    """k3_csf_subtract_bias_u_dev (solver3d.py:478-500)."""
+    _csf_bias(0, u, marker, sum_dev, count_dev, nx, ny, nz, (off_x, off_y, off_z))
+
+
+@block("S2.14.6", "CSF force-balance on v (device-side)")
+def k3_csf_subtract_bias_v_dev(v, marker, sum_dev, count_dev, nx, ny, nz,
+                               off_x=0, off_y=0, off_z=0):
+    """k3_csf_subtract_bias_v_dev (solver3d.py:503-524)."""
+    _csf_bias(1, v, marker, sum_dev, count_dev, nx, ny, nz, (off_x, off_y, off_z))
+
+
+@block("S2.14.6", "CSF force-balance on w (device-side)")
+def k3_csf_subtract_bias_w_dev(w_, marker, sum_dev, count_dev, nx, ny, nz,
+                               off_x=0, off_y=0, off_z=0):
+    """k3_csf_subtract_bias_w_dev (solver3d.py:527-547)."""
+    _csf_bias(2, w_, marker, sum_dev, count_dev, nx, ny, nz, (off_x, off_y, off_z))

@@ -444,6 +444,32 @@ int gf_k3_dot_fluid(const float* a, const float* b, float* out, const int32_t* m
 int gf_k3_axpy_devscalar(const float* a_dev, float sign, const float* x,
                          const float* y_in, float* y_out, const int32_t* marker,
                          const int32_t* done, int nx, int ny, int nz, void* stream);
+/* The S2.14 sub-blocks one launch at a time, on dense arrays with the
+ * reference's meaning (solver3d.py:203-549): k3_indicator_from_marker,
+ * k3_box_blur_centred (src != dst), k3_csf_normal, k3_csf_curvature,
+ * k3_csf_apply_u/v/w (comp 0/1/2), k3_csf_sum_u/v/w and
+ * k3_csf_subtract_bias_u/v/w_dev.  gf_k3_csf_sum adds the change (after -
+ * before) over the component's faces next to fluid, summed in FP64 in a fixed
+ * order and rounded to FP32, to sum[0], and the face count to count[0]
+ * (device scalars; the reference adds FP32 atomics); `red` is a workspace of
+ * gf_k3_csf_sum_workspace doubles.  gf_k3_csf_subtract_bias subtracts
+ * sum[0]/count[0] on those faces when count[0] > 0.5. */
+int gf_k3_indicator_from_marker(const int32_t* marker, float* chi, int nx, int ny, int nz,
+                                void* stream);
+int gf_k3_box_blur_centred(const float* src, float* dst, int nx, int ny, int nz, void* stream);
+int gf_k3_csf_normal(const float* chi, float* n_x, float* n_y, float* n_z, int nx, int ny, int nz,
+                     float dx, void* stream);
+int gf_k3_csf_curvature(const float* n_x, const float* n_y, const float* n_z, float* kappa, int nx,
+                        int ny, int nz, float dx, void* stream);
+int gf_k3_csf_apply(int comp, float* vel, const float* chi, const float* kappa,
+                    const int32_t* marker, float coeff, float dx, int nx, int ny, int nz,
+                    void* stream);
+int gf_k3_csf_sum_workspace(int nx, int ny, int nz, int64_t* red_doubles);
+int gf_k3_csf_sum(int comp, const float* before, const float* after, const int32_t* marker,
+                  float* sum, float* count, int nx, int ny, int nz, double* red, void* stream);
+int gf_k3_csf_subtract_bias(int comp, float* vel, const int32_t* marker, const float* sum,
+                            const float* count, int nx, int ny, int nz, void* stream);
+
 /* Active-block bitmask (S2.16, k_mark_active_blocks solver3d.py:952-969):
  * block_active (nbx, nby, nbz) of bs^3 blocks covering the grid; a block with
  * a fluid cell is set to 1, others are left as they are. */

@@ -670,32 +670,58 @@ def csf_apply(u, v, w, marker, dx, coeff, passes):
     precision, is subtracted there (k3_csf_sum_* / subtract_bias_*_dev,
     400-547).  Returns new (u, v, w)."""
     dt_ = u.dtype
-    t = dt_.type
     chi = csf_indicator(marker, dt_)
     for _ in range(passes):
         chi = box_blur(chi)
     kappa = csf_curvature(csf_normal(chi, dx), dx)
     out = []
     for comp, f in enumerate((u, v, w)):
-        active, fluid, lo, hi, sl = _face_masks(marker, comp)
-        grad = ((chi[hi] - chi[lo]) / t(np.float32(dx))).astype(dt_)
-        kf = (t(0.5) * (kappa[hi] + kappa[lo]).astype(dt_)).astype(dt_)
-        inc = ((t(np.float32(coeff)) * kf).astype(dt_) * grad).astype(dt_)
-        g = f.copy()
-        inner = g[sl]
-        a = active[sl]
-        inner[a] = (inner[a] + inc[a]).astype(dt_)
-        g[sl] = inner
-        delta = (g - f).astype(dt_)[fluid]
-        cnt = int(fluid.sum())
-        if cnt > 0:
-            total = delta.astype(np.float64).sum()
-            if dt_ == np.float32:
-                total = np.float32(total)
-            mean = t(total) / t(cnt)
-            g[fluid] = (g[fluid] - mean).astype(dt_)
-        out.append(g)
+        g = csf_face_apply(f, chi, kappa, marker, comp, dx, coeff)
+        total, cnt = csf_face_sum(f, g, marker, comp)
+        out.append(csf_face_subtract(g, marker, comp, total, cnt))
     return tuple(out)
+
+
+def csf_face_apply(f, chi, kappa, marker, comp, dx, coeff):
+    """k3_csf_apply_u/v/w (solver3d.py:316-385) on component comp: face +=
+    coeff * kappa_face * grad(chi) on faces with no solid side."""
+    dt_ = f.dtype
+    t = dt_.type
+    active, _, lo, hi, sl = _face_masks(marker, comp)
+    grad = ((chi[hi] - chi[lo]) / t(np.float32(dx))).astype(dt_)
+    kf = (t(0.5) * (kappa[hi] + kappa[lo]).astype(dt_)).astype(dt_)
+    inc = ((t(np.float32(coeff)) * kf).astype(dt_) * grad).astype(dt_)
+    g = f.copy()
+    inner = g[sl]
+    a = active[sl]
+    inner[a] = (inner[a] + inc[a]).astype(dt_)
+    g[sl] = inner
+    return g
+
+
+def csf_face_sum(before, after, marker, comp):
+    """k3_csf_sum_* (solver3d.py:400-464): the change after - before summed
+    exactly over the faces next to fluid (rounded to the fields' precision)
+    and the number of those faces."""
+    _, fluid, _, _, _ = _face_masks(marker, comp)
+    delta = (after - before).astype(after.dtype)[fluid]
+    total = delta.astype(np.float64).sum()
+    if after.dtype == np.float32:
+        total = np.float32(total)
+    return total, int(fluid.sum())
+
+
+def csf_face_subtract(g, marker, comp, total, cnt):
+    """k3_csf_subtract_bias_*_dev (solver3d.py:478-547): faces next to fluid
+    lose total / cnt when cnt > 0."""
+    dt_ = g.dtype
+    t = dt_.type
+    g = g.copy()
+    if cnt > 0:
+        _, fluid, _, _, _ = _face_masks(marker, comp)
+        mean = t(total) / t(cnt)
+        g[fluid] = (g[fluid] - mean).astype(dt_)
+    return g
 
 
 # ------------------------------------------- per-particle attribute transfer

@@ -128,3 +128,16 @@ def test_surface_tension_substep_pulls_a_cube_toward_a_ball():
     corner = (np.abs(rel) > 0.2).all(1)
     radial = (o["vel"] * rel).sum(1)
     assert corner.any() and (radial[corner] < 0).mean() > 0.9
+
+
+def test_csf_max_stable_dt_closed_form():
+    """S2.14.5 host helper (solver3d.py:2284-2307): safety*sqrt(rho dx^3 /
+    (2 pi sigma)); no bound without surface tension."""
+    import math
+    from gpufluid_amd.kernels import csf_max_stable_dt
+    assert csf_max_stable_dt(1.0, 0.1, 0.0) == float("inf")
+    got = csf_max_stable_dt(1000.0, 1.0 / 64, 0.0728, safety=0.5)
+    assert abs(got - 0.5 * math.sqrt(1000.0 / 64 ** 3 / (2 * math.pi * 0.0728))) < 1e-15
+    # doubling sigma shrinks the bound by sqrt(2); dx^(3/2) scaling
+    assert abs(csf_max_stable_dt(1.0, 0.02, 0.2) * math.sqrt(2) - csf_max_stable_dt(1.0, 0.02, 0.1)) < 1e-15
+    assert abs(csf_max_stable_dt(1.0, 0.04, 0.1) / csf_max_stable_dt(1.0, 0.01, 0.1) - 8.0) < 1e-12
