@@ -106,6 +106,9 @@ SIGNATURES = {
     "gf_scene_g2p": [SP, V, I, F, F, F, F, V],
     "gf_scene_attr_transfer": [SP, V, I, I, V, V, V, V],
     "gf_anim_obstacles": [V, I, V, V, V, V, V, I, I, I, D, V],
+    "gf_k3_p2g_attr": [V, V, I, I, V, V, V, V, I, I, I, F, V, V, V],
+    "gf_k3_normalize_attr": [I, I, V, V, V, V, I, I, I, V],
+    "gf_k3_g2p_attr": [V, V, I, I, V, V, V, V, I, I, I, F, V],
     "gf_k3_indicator_from_marker": [V, V, I, I, I, V],
     "gf_k3_box_blur_centred": [V, V, I, I, I, V],
     "gf_k3_csf_normal": [V, V, V, V, I, I, I, F, V],

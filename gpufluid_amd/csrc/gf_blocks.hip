@@ -365,6 +365,7 @@ __global__ void k_subtract_grad(float* u, float* v, float* w, const float* p,
 
 // gridmath.py:30-65 on a dense face array of shape (nmx, nmy, nmz).
 __device__ float sample3(const float* f, float fx, float fy, float fz, int nmx, int nmy, int nmz) {
+#pragma clang fp contract(off)
     fx = fminf(fmaxf(fx, 0.0f), (float)nmx);
     fy = fminf(fmaxf(fy, 0.0f), (float)nmy);
     fz = fminf(fmaxf(fz, 0.0f), (float)nmz);
@@ -690,7 +691,7 @@ __global__ __launch_bounds__(GF_WG) void k_attr_scatter(const float* pos, const 
         if (ok) {
             i0 = (int)floorf(fx); j0 = (int)floorf(fy); k0 = (int)floorf(fz);
             sx = fx - (float)i0; sy = fy - (float)j0; sz = fz - (float)k0;
-            const int id = pid[p];
+            const int id = pid ? pid[p] : p;
             for (int c = 0; c < NC; ++c) v[c] = attr[(size_t)id * NC + c];
         }
     }
@@ -773,6 +774,50 @@ __global__ void k_attr_normalize(const int64_t* acc, int nc, long long cells,
         grid[c * cells + i] = w > 1.0e-8f ? (mode == 0 ? v * inv : v / w) : 0.0f;
     }
     grid[nc * cells + i] = w;
+}
+
+// The block forms' grids (S2.15.1 / S2.18.1 semantics): each channel's exact
+// sum, rounded to FP32, is added to the caller's FP32 grid g[c] (g[nc] = the
+// weight grid), as the reference's atomics add into it.
+struct AttrGrids {
+    float* g[4];
+};
+
+__global__ void k_attr_accumulate(const int64_t* acc, int nc, long long cells,
+                                  const uint32_t* scratch, AttrGrids out) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cells) return;
+    const double iv = ldexp(1.0, -attr_exp(scratch)), iw = ldexp(1.0, -48);
+    for (int c = 0; c <= nc; ++c) {
+        const int64_t a = acc[c * cells + i];
+        if (a == 0) continue;
+        out.g[c][i] = out.g[c][i] + (float)((double)a * (c == nc ? iw : iv));
+    }
+}
+
+// k3_normalize_color (solver3d.py:607-631) / k3_normalize_scalar (710-727).
+__global__ void k_attr_normalize_grids(int nc, long long cells, int mode, AttrGrids g) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cells) return;
+    const float w = g.g[nc][i];
+    if (w > 1.0e-8f) {
+        const float inv = 1.0f / w;
+        for (int c = 0; c < nc; ++c) g.g[c][i] = mode == 0 ? g.g[c][i] * inv : g.g[c][i] / w;
+    } else {
+        for (int c = 0; c < nc; ++c) g.g[c][i] = 0.0f;
+    }
+}
+
+// k3_g2p_color (solver3d.py:633-664) / k3_g2p_scalar (729-755) on the grids.
+__global__ void k_attr_gather_grids(const float* pos, float* attr, int nc, int n, int nx, int ny,
+                                    int nz, float dx, AttrGrids g) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    float fx, fy, fz;
+    attr_cell_coords(pos, p, dx, fx, fy, fz);
+    const float w = sample3(g.g[nc], fx, fy, fz, nx, ny, nz);
+    if (!(w > 1.0e-6f)) return;
+    for (int c = 0; c < nc; ++c) attr[(size_t)p * nc + c] = sample3(g.g[c], fx, fy, fz, nx, ny, nz);
 }
 
 __global__ void k_attr_gather(const float* pos, const int32_t* pid, float* attr, int nc, int n,
@@ -1223,3 +1268,75 @@ extern "C" int gf_scene_attr_transfer(const gf_scene_t* s, float* attr, int chan
     return 0;
 }
 
+
+// ---- colour / scalar transfer one block at a time (S2.15.1-3, S2.18.1-3) ----
+
+namespace {
+int attr_block_ok(int channels, const float* g0, const float* g1, const float* g2,
+                  const float* gw, int nx, int ny, int nz, const char* who) {
+    GF_REQUIRE(channels == 1 || channels == 3, "%s: channels %d", who, channels);
+    GF_REQUIRE(nx > 0 && ny > 0 && nz > 0, "%s: grid", who);
+    GF_REQUIRE(g0 && gw && (channels == 1 || (g1 && g2)), "%s: null grid", who);
+    return 0;
+}
+AttrGrids grids_of(int channels, float* g0, float* g1, float* g2, float* gw) {
+    AttrGrids g;
+    g.g[0] = g0; g.g[1] = g1; g.g[2] = g2; g.g[3] = nullptr;
+    g.g[channels] = gw;
+    return g;
+}
+}  // namespace
+
+extern "C" int gf_k3_p2g_attr(const float* pos, const float* attr, int channels, int n,
+                              float* g0, float* g1, float* g2, float* gw, int nx, int ny, int nz,
+                              float dx, int64_t* acc, uint32_t* scratch, void* stream) {
+    int rc = attr_block_ok(channels, g0, g1, g2, gw, nx, ny, nz, "gf_k3_p2g_attr");
+    if (rc) return rc;
+    GF_REQUIRE(n >= 0 && dx > 0.0f, "gf_k3_p2g_attr: n %d, dx", n);
+    if (n == 0) return 0;
+    GF_REQUIRE(pos && attr && acc && scratch, "gf_k3_p2g_attr: null buffer");
+    hipStream_t st = (hipStream_t)stream;
+    const long long cells = (long long)nx * ny * nz;
+    const long long nacc = (long long)(channels + 1) * cells;
+    hipLaunchKernelGGL(k_attr_clear, grid_for(nacc), dim3(GF_WG), 0, st, acc, nacc, scratch);
+    const long long nv = (long long)n * channels;
+    const unsigned gb = (unsigned)std::min<long long>((nv + GF_WG - 1) / GF_WG, 2048);
+    hipLaunchKernelGGL(k_attr_absmax, dim3(gb), dim3(GF_WG), 0, st, attr, nv, scratch);
+    auto scatter = channels == 1 ? k_attr_scatter<1> : k_attr_scatter<3>;
+    hipLaunchKernelGGL(scatter, grid_for(n), dim3(GF_WG), 0, st, pos, nullptr, attr, n, nx, ny,
+                       nz, dx, scratch, acc);
+    hipLaunchKernelGGL(k_attr_accumulate, grid_for(cells), dim3(GF_WG), 0, st, acc, channels,
+                       cells, scratch, grids_of(channels, g0, g1, g2, gw));
+    GF_LAUNCH_CHECK("gf_k3_p2g_attr");
+    return 0;
+}
+
+extern "C" int gf_k3_normalize_attr(int mode, int channels, float* g0, float* g1, float* g2,
+                                    float* gw, int nx, int ny, int nz, void* stream) {
+    int rc = attr_block_ok(channels, g0, g1, g2, gw, nx, ny, nz, "gf_k3_normalize_attr");
+    if (rc) return rc;
+    GF_REQUIRE(mode == GF_ATTR_COLOR || mode == GF_ATTR_SCALAR, "gf_k3_normalize_attr: mode %d",
+               mode);
+    const long long cells = (long long)nx * ny * nz;
+    hipLaunchKernelGGL(k_attr_normalize_grids, grid_for(cells), dim3(GF_WG), 0,
+                       (hipStream_t)stream, channels, cells, mode,
+                       grids_of(channels, g0, g1, g2, gw));
+    GF_LAUNCH_CHECK("k_attr_normalize_grids");
+    return 0;
+}
+
+extern "C" int gf_k3_g2p_attr(const float* pos, float* attr, int channels, int n, const float* g0,
+                              const float* g1, const float* g2, const float* gw, int nx, int ny,
+                              int nz, float dx, void* stream) {
+    int rc = attr_block_ok(channels, g0, g1, g2, gw, nx, ny, nz, "gf_k3_g2p_attr");
+    if (rc) return rc;
+    GF_REQUIRE(n >= 0 && dx > 0.0f, "gf_k3_g2p_attr: n %d, dx", n);
+    if (n == 0) return 0;
+    GF_REQUIRE(pos && attr, "gf_k3_g2p_attr: null buffer");
+    hipLaunchKernelGGL(k_attr_gather_grids, grid_for(n), dim3(GF_WG), 0, (hipStream_t)stream, pos,
+                       attr, channels, n, nx, ny, nz, dx,
+                       grids_of(channels, const_cast<float*>(g0), const_cast<float*>(g1),
+                                const_cast<float*>(g2), const_cast<float*>(gw)));
+    GF_LAUNCH_CHECK("k_attr_gather_grids");
+    return 0;
+}

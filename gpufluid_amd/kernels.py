@@ -540,3 +540,96 @@ def k3_csf_subtract_bias_w_dev(w_, marker, sum_dev, count_dev, nx, ny, nz,
                                off_x=0, off_y=0, off_z=0):
     """k3_csf_subtract_bias_w_dev (solver3d.py:527-547)."""
     _csf_bias(2, w_, marker, sum_dev, count_dev, nx, ny, nz, (off_x, off_y, off_z))
+
+
+# ------------------------------------------ colour / scalar transfer blocks
+_ATTR_WS: dict = {}
+
+
+def _attr_ws(channels, shape, dev):
+    key = (channels, shape, dev)
+    ws = _ATTR_WS.get(key)
+    if ws is None:
+        cells = shape[0] * shape[1] * shape[2]
+        ws = (torch.empty((channels + 1) * cells, dtype=torch.int64, device=dev),
+              torch.empty(4, dtype=torch.int32, device=dev))
+        _ATTR_WS[key] = ws
+    return ws
+
+
+def _attr_grid_shape(block_id, grids, nx, ny, nz, offs):
+    _offsets(block_id, *offs)
+    shape = _grid3(grids[0])
+    if (int(nx), int(ny), int(nz)) != shape or any(_grid3(g) != shape for g in grids):
+        raise BlockError(block_id, f"grids do not all have the shape ({nx},{ny},{nz})")
+    return shape
+
+
+def _p2g_attr(block_id, pos, attr, channels, grids, dx, nx, ny, nz, offs):
+    shape = _attr_grid_shape(block_id, grids, nx, ny, nz, offs)
+    n = int(_t(pos).reshape(-1, 3).shape[0])
+    if _t(attr).numel() != n * channels:
+        raise BlockError(block_id, f"attribute holds {_t(attr).numel()} values for {n} particles")
+    acc, scratch = _attr_ws(channels, shape, _t(pos).device)
+    g = [_p(x) for x in grids]
+    if channels == 1:
+        g = [g[0], None, None, g[1]]
+    _lib.call(block_id, "gf_k3_p2g_attr", _p(pos), _p(attr), channels, n, *g, *shape, float(dx),
+              _lib.ptr(acc), _lib.ptr(scratch), _st(pos))
+
+
+def _g2p_attr(block_id, pos, attr, channels, grids, dx, nx, ny, nz, offs):
+    shape = _attr_grid_shape(block_id, grids, nx, ny, nz, offs)
+    n = int(_t(pos).reshape(-1, 3).shape[0])
+    if _t(attr).numel() != n * channels:
+        raise BlockError(block_id, f"attribute holds {_t(attr).numel()} values for {n} particles")
+    g = [_p(x) for x in grids]
+    if channels == 1:
+        g = [g[0], None, None, g[1]]
+    _lib.call(block_id, "gf_k3_g2p_attr", _p(pos), _p(attr), channels, n, *g, *shape, float(dx),
+              _st(pos))
+
+
+@block("S2.15.1", "P2G scatter of per-particle RGB color to cell-centred grid")
+def k3_p2g_color(pos, color, cgrid_r, cgrid_g, cgrid_b, cgrid_w, dx, nx, ny, nz,
+                 off_x=0, off_y=0, off_z=0):
+    """k3_p2g_color (solver3d.py:562-600): adds the trilinear sums of
+    colour*w and w at the cell centres to the four grids."""
+    _p2g_attr("S2.15.1", pos, color, 3, (cgrid_r, cgrid_g, cgrid_b, cgrid_w), dx, nx, ny, nz,
+              (off_x, off_y, off_z))
+
+
+@block("S2.15.2", "Normalize grid color by deposited weight")
+def k3_normalize_color(cgrid_r, cgrid_g, cgrid_b, cgrid_w):
+    """k3_normalize_color (solver3d.py:607-631)."""
+    shape = _grid3(cgrid_w)
+    _lib.call("S2.15.2", "gf_k3_normalize_attr", _lib.GF_ATTR_COLOR, 3, _p(cgrid_r),
+              _p(cgrid_g), _p(cgrid_b), _p(cgrid_w), *shape, _st(cgrid_w))
+
+
+@block("S2.15.3", "G2P gather grid color back to particle")
+def k3_g2p_color(pos, color, cgrid_r, cgrid_g, cgrid_b, cgrid_w, dx, nx, ny, nz,
+                 off_x=0, off_y=0, off_z=0):
+    """k3_g2p_color (solver3d.py:633-664)."""
+    _g2p_attr("S2.15.3", pos, color, 3, (cgrid_r, cgrid_g, cgrid_b, cgrid_w), dx, nx, ny, nz,
+              (off_x, off_y, off_z))
+
+
+@block("S2.18.1", "P2G scatter of per-particle scalar onto grid")
+def k3_p2g_scalar(pos, attr, sgrid_v, sgrid_w, dx, nx, ny, nz, off_x=0, off_y=0, off_z=0):
+    """k3_p2g_scalar (solver3d.py:670-707)."""
+    _p2g_attr("S2.18.1", pos, attr, 1, (sgrid_v, sgrid_w), dx, nx, ny, nz, (off_x, off_y, off_z))
+
+
+@block("S2.18.2", "Normalize grid scalar by deposited weight")
+def k3_normalize_scalar(sgrid_v, sgrid_w):
+    """k3_normalize_scalar (solver3d.py:710-727)."""
+    shape = _grid3(sgrid_w)
+    _lib.call("S2.18.2", "gf_k3_normalize_attr", _lib.GF_ATTR_SCALAR, 1, _p(sgrid_v), None, None,
+              _p(sgrid_w), *shape, _st(sgrid_w))
+
+
+@block("S2.18.3", "G2P gather grid scalar back to particle")
+def k3_g2p_scalar(pos, attr, sgrid_v, sgrid_w, dx, nx, ny, nz, off_x=0, off_y=0, off_z=0):
+    """k3_g2p_scalar (solver3d.py:729-755)."""
+    _g2p_attr("S2.18.3", pos, attr, 1, (sgrid_v, sgrid_w), dx, nx, ny, nz, (off_x, off_y, off_z))

@@ -470,6 +470,26 @@ int gf_k3_csf_sum(int comp, const float* before, const float* after, const int32
 int gf_k3_csf_subtract_bias(int comp, float* vel, const int32_t* marker, const float* sum,
                             const float* count, int nx, int ny, int nz, void* stream);
 
+/* Colour / scalar transfer one block at a time on the reference's separate
+ * FP32 cell grids (S2.15.1-3 k3_p2g_color / k3_normalize_color /
+ * k3_g2p_color, solver3d.py:562-664; S2.18.1-3 the scalar forms, 670-755):
+ * channels 3 (g0, g1, g2 = r, g, b) or 1 (g0 = value), gw the weight grid;
+ * particles in caller order.  gf_k3_p2g_attr adds each cell's exact sum of
+ * value*w (and of w), rounded to FP32, to the grids (the reference adds FP32
+ * atomics); acc is (channels+1)*nx*ny*nz int64 and scratch 4 words of
+ * workspace.  gf_k3_normalize_attr: mode GF_ATTR_COLOR (v * (1/w)) or
+ * GF_ATTR_SCALAR (v / w) where w > 1e-8, else 0.  gf_k3_g2p_attr samples the
+ * grids at p/dx - 0.5 and updates a particle where the sampled weight
+ * exceeds 1e-6. */
+int gf_k3_p2g_attr(const float* pos, const float* attr, int channels, int n, float* g0, float* g1,
+                   float* g2, float* gw, int nx, int ny, int nz, float dx, int64_t* acc,
+                   uint32_t* scratch, void* stream);
+int gf_k3_normalize_attr(int mode, int channels, float* g0, float* g1, float* g2, float* gw, int nx,
+                         int ny, int nz, void* stream);
+int gf_k3_g2p_attr(const float* pos, float* attr, int channels, int n, const float* g0,
+                   const float* g1, const float* g2, const float* gw, int nx, int ny, int nz,
+                   float dx, void* stream);
+
 /* Active-block bitmask (S2.16, k_mark_active_blocks solver3d.py:952-969):
  * block_active (nbx, nby, nbz) of bs^3 blocks covering the grid; a block with
  * a fluid cell is set to 1, others are left as they are. */
