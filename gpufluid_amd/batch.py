@@ -153,6 +153,7 @@ class BatchedScenes:
         self.last_residual = None
         self._graph = None
         self._graph_budget = 0
+        self._budget_floor = 0      # chunks; raised when a replay needed a larger budget
 
     @property
     def n_scenes(self) -> int:
@@ -202,13 +203,19 @@ class BatchedScenes:
         """Two substeps from one hipGraph replay.  The PCG of each substep gets
         ``budget_iters`` iterations inside the graph (default: the cap).  If a
         scene has not stopped within the budget in either substep, the pair is
-        rerun from a copy of the particle state with the budget doubled (the
-        graph is recaptured), so results never depend on the budget."""
+        rerun from a copy of the particle state with the budget raised by half (the
+        graph is recaptured), so results never depend on the budget.  A budget
+        that had to grow stays as the floor for later calls, and a pair that
+        comes within 10% of its budget raises that floor by a quarter, so a
+        solve whose iteration count drifts upward is recaptured ahead of time
+        instead of rerun."""
         s = self.s
         ce = s.pcg_check_every
         cap_chunks = math.ceil(pressure_iters / ce)
         budget = max(1, min(math.ceil((budget_iters or pressure_iters) / ce), cap_chunks))
+        budget = max(budget, min(self._budget_floor, cap_chunks))
         s._cache.clear()
+        grew = False
         while True:
             if self._graph is None or self._graph_budget != budget:
                 self._capture(pressure_iters, pressure_tol, budget)
@@ -221,11 +228,21 @@ class BatchedScenes:
             if (done_a.all() and done_b.all()) or budget >= cap_chunks:
                 self._read_stats()
                 s._cache.clear()
+                if grew:
+                    self._budget_floor = max(self._budget_floor, budget)
+                # iteration counts drift as the scenes evolve: a pair that came
+                # within 10% of the budget raises the floor ahead of time (a
+                # recapture costs far less than rerunning a pair)
+                need = int(max(it_a.max(), it_b.max()))
+                if 10 * need > 9 * budget * ce:
+                    self._budget_floor = max(self._budget_floor,
+                                             min(cap_chunks, math.ceil(1.25 * need / ce)))
                 return
+            grew = True
             live = s._sc.live(s.n_particles)
             for k, v in snap.items():
                 live[k].copy_(v)
-            budget = min(cap_chunks, budget * 2)
+            budget = min(cap_chunks, budget + (budget + 1) // 2)
 
     def _res_b(self):
         S = self.n_scenes
