@@ -483,14 +483,18 @@ __global__ __launch_bounds__(GF_WG) void k_p2g_tile(P2GParams a) {
 template <bool APIC>
 __global__ __launch_bounds__(GF_WG) void k_p2g_own(P2GParams a, float* __restrict__ stage,
                                                    int32_t* __restrict__ has) {
-    __shared__ float s_val[3][OWN_REGION];
-    __shared__ float s_wt[3][OWN_REGION];
+    // one velocity component per workgroup: three workgroups share a tile,
+    // each with a third of the LDS and of the work, consecutive in the
+    // XCD-aware order so that they read the tile's particles through one L2
+    __shared__ float s_val[OWN_REGION];
+    __shared__ float s_wt[OWN_REGION];
     __shared__ int s_cls[28];
     __shared__ int s_start[64], s_pref[64];
     __shared__ int s_wave[GF_WG / GF_WAVE + 1];
 
     const int ntiles = a.tnx * a.tny * a.tnz;
-    const int t = gf::xcd_remap(blockIdx.x, ntiles);
+    const int lb = gf::xcd_remap(blockIdx.x, 3 * ntiles);
+    const int t = lb / 3, comp = lb - 3 * t;
     const int tk = t % a.tnz, tj = (t / a.tnz) % a.tny, ti = t / (a.tnz * a.tny);
     const int i0 = ti * 8, j0 = tj * 8, k0 = tk * 8;
     const int nx = a.nx, ny = a.ny, nz = a.nz;
@@ -506,11 +510,11 @@ __global__ __launch_bounds__(GF_WG) void k_p2g_own(P2GParams a, float* __restric
     }
     const RowSet rs = build_rows(a.cell_start, ny, nz, i0, i0 + ni, j0, j0 + nj, k0, k0 + nk,
                                  s_start, s_pref, s_wave);
-    if (threadIdx.x == 0) has[t] = rs.total > 0 ? 1 : 0;
+    if (threadIdx.x == 0 && comp == 0) has[t] = rs.total > 0 ? 1 : 0;
     if (rs.total == 0) return;
-    for (int i = threadIdx.x; i < 3 * OWN_REGION; i += GF_WG) {
-        (&s_val[0][0])[i] = 0.0f;
-        (&s_wt[0][0])[i] = 0.0f;
+    for (int i = threadIdx.x; i < OWN_REGION; i += GF_WG) {
+        s_val[i] = 0.0f;
+        s_wt[i] = 0.0f;
     }
     __syncthreads();
     const int ncell = ni * nj * nk;
@@ -536,18 +540,20 @@ __global__ __launch_bounds__(GF_WG) void k_p2g_own(P2GParams a, float* __restric
         int r_lo = 0, r_hi = 0;
         while (r_lo < 26 && s_cls[r_lo + 1] <= round * GF_WG) ++r_lo;
         while (r_hi < 26 && s_cls[r_hi + 1] <= last) ++r_hi;
-        p2g_component<0, APIC, OwnDims<0>>(a, s_val[0], s_wt[0], ci, cj, ck, p0, p1, cls, r_lo,
-                                           r_hi, i0, j0 - 1, k0 - 1);
-        p2g_component<1, APIC, OwnDims<1>>(a, s_val[1], s_wt[1], ci, cj, ck, p0, p1, cls, r_lo,
-                                           r_hi, i0 - 1, j0, k0 - 1);
-        p2g_component<2, APIC, OwnDims<2>>(a, s_val[2], s_wt[2], ci, cj, ck, p0, p1, cls, r_lo,
-                                           r_hi, i0 - 1, j0 - 1, k0);
+        if (comp == 0)
+            p2g_component<0, APIC, OwnDims<0>>(a, s_val, s_wt, ci, cj, ck, p0, p1, cls, r_lo,
+                                               r_hi, i0, j0 - 1, k0 - 1);
+        else if (comp == 1)
+            p2g_component<1, APIC, OwnDims<1>>(a, s_val, s_wt, ci, cj, ck, p0, p1, cls, r_lo,
+                                               r_hi, i0 - 1, j0, k0 - 1);
+        else
+            p2g_component<2, APIC, OwnDims<2>>(a, s_val, s_wt, ci, cj, ck, p0, p1, cls, r_lo,
+                                               r_hi, i0 - 1, j0 - 1, k0);
     }
-    float* st = stage + (size_t)t * STAGE_PER_TILE;
-    for (int e = threadIdx.x; e < 3 * OWN_REGION; e += GF_WG) {
-        const int c = e / OWN_REGION, idx = e - c * OWN_REGION;
-        st[(size_t)c * 2 * OWN_REGION + idx] = s_val[c][idx];
-        st[(size_t)c * 2 * OWN_REGION + OWN_REGION + idx] = s_wt[c][idx];
+    float* st = stage + (size_t)t * STAGE_PER_TILE + (size_t)comp * 2 * OWN_REGION;
+    for (int e = threadIdx.x; e < OWN_REGION; e += GF_WG) {
+        st[e] = s_val[e];
+        st[OWN_REGION + e] = s_wt[e];
     }
 }
 
@@ -1147,9 +1153,9 @@ extern "C" int gf_scene_p2g_staged(const gf_scene_t* s, int transfer_mode, float
     const int ntiles = s->tnx * s->tny * s->tnz;
     hipStream_t st = (hipStream_t)stream;
     if (transfer_mode == GF_TRANSFER_APIC)
-        hipLaunchKernelGGL(k_p2g_own<true>, dim3(ntiles), dim3(GF_WG), 0, st, a, stage, has);
+        hipLaunchKernelGGL(k_p2g_own<true>, dim3(3 * ntiles), dim3(GF_WG), 0, st, a, stage, has);
     else
-        hipLaunchKernelGGL(k_p2g_own<false>, dim3(ntiles), dim3(GF_WG), 0, st, a, stage, has);
+        hipLaunchKernelGGL(k_p2g_own<false>, dim3(3 * ntiles), dim3(GF_WG), 0, st, a, stage, has);
     hipLaunchKernelGGL(k_p2g_finish, dim3(ntiles), dim3(GF_WG), 0, st, a, stage, has);
     GF_LAUNCH_CHECK("k_p2g_own/k_p2g_finish");
     return 0;
