@@ -159,9 +159,8 @@ def main():
                          "byte counters can be scaled against a kernel of known traffic")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: every box here has one MI355X and the "
-              "substep path is single-device (see DESIGN.md for how it would shard).",
-              file=sys.stderr)
+        print("bench.py measures one GPU: the substep path runs on a single device "
+              "(--gpus 1); several GPUs are not supported.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
