@@ -613,6 +613,9 @@ class FlipSolver3D:
         device first, then the emitted particles (minus those landing in a
         solid cell of the host marker) are appended with their event's colour
         (else 0) and temperature (else 20) when the attribute is tracked."""
+        # the frame may change the particle set and the solids: drop the
+        # captured substeps, as the reference does (solver3d.py:3912-3915)
+        self._cuda_graph_invalidate()
         if self._anim_specs:
             self._stamp_animated_obstacles(frame_idx)
         q = self._frame_events
@@ -819,7 +822,7 @@ class FlipSolver3D:
             self._step_impl(float(dt), int(pressure_iters), pressure_solver, tol)
 
     # ------------------------------------------------------- B5.2 graph cache
-    _GRAPH_CACHE = 4   # captured substeps kept (two per ping-pong state)
+    _GRAPH_CACHE = 4   # captured substeps kept (a configuration takes one per ping-pong state)
 
     def _cuda_graph_eligible(self, pressure_solver: str, pressure_block_sparse: bool = False) -> bool:
         """Substeps replayed from a hipGraph (solver3d.py:3362-3391): with
@@ -884,9 +887,12 @@ class FlipSolver3D:
         entry = self._graphs.get(key)
         if entry is None:
             entry = self._capture_substep(dt, pressure_iters, pressure_solver, tol)
-            if len(self._graphs) >= self._GRAPH_CACHE:
-                self._graphs.pop(next(iter(self._graphs)))
-            self._graphs[key] = entry
+            self._store_graph(key, entry)
+            # The particle buffers (and, with an odd sweep count, the face
+            # buffers) ping-pong, so the next substep starts from the other
+            # state: capture it now as well, so that one configuration is one
+            # miss, as in the reference's single-graph cache.
+            self._capture_follower(entry, dt, pressure_iters, pressure_solver, tol)
             self._cuda_graph_misses += 1
         else:
             self._cuda_graph_hits += 1
@@ -899,6 +905,25 @@ class FlipSolver3D:
         sc.cur = cur
         self._cuda_graph, self._cuda_graph_key = graph, key
         self._graph_stats = (pressure_solver, int(pressure_iters))
+
+    def _store_graph(self, key, entry):
+        if len(self._graphs) >= self._GRAPH_CACHE:
+            self._graphs.pop(next(iter(self._graphs)))
+        self._graphs[key] = entry
+
+    def _capture_follower(self, entry, dt, pressure_iters, pressure_solver, tol):
+        """Capture the substep that starts from the state `entry` leaves."""
+        start = (dict(self._f), self._sc.cur)
+        self._f.update(entry[1])
+        self._sc.cur = entry[2]
+        try:
+            key = self._graph_key(dt, pressure_iters, pressure_solver, tol)
+            if key not in self._graphs:
+                self._store_graph(key, self._capture_substep(dt, pressure_iters,
+                                                             pressure_solver, tol))
+        finally:
+            self._f.update(start[0])
+            self._sc.cur = start[1]
 
     def _capture_substep(self, dt, pressure_iters, pressure_solver, tol):
         """Record one substep into a hipGraph (nothing runs); returns the

@@ -56,8 +56,9 @@ def test_replay_is_bit_identical_to_direct(solver, transfer):
         assert g.last_pressure_replacements == a.last_pressure_replacements
         _assert_same(a, g)
     assert len(set(tiles)) > 1, tiles          # the active tiles changed on the way
-    # the two ping-pong states are captured once each, then replayed
-    assert (g._cuda_graph_misses, g._cuda_graph_hits) == (2, 4)
+    # one configuration: both ping-pong states captured at the first substep
+    assert (g._cuda_graph_misses, g._cuda_graph_hits) == (1, 5)
+    assert len(g._graphs) == 2
     assert a._cuda_graph_misses == 0 and a._cuda_graph is None
     if solver == "pcg":
         assert 0 < a.last_pressure_iters < 200
@@ -83,7 +84,7 @@ def test_replay_with_every_stage():
         np.testing.assert_array_equal(g.attr_color.numpy(), a.attr_color.numpy())
         np.testing.assert_array_equal(g.attr_temperature.numpy(), a.attr_temperature.numpy())
         np.testing.assert_array_equal(g.us.numpy(), a.us.numpy())
-    assert (g._cuda_graph_misses, g._cuda_graph_hits) == (2, 2)
+    assert (g._cuda_graph_misses, g._cuda_graph_hits) == (1, 3)
 
 
 def test_key_changes_recapture():
@@ -94,7 +95,7 @@ def test_key_changes_recapture():
     a, g = _pair(shape, dx, pos, vel)
     extra = (np.array([0.7, 0.7, 0.5], np.float32)
              + 0.05 * np.random.default_rng(33).random((300, 3)).astype(np.float32))
-    # particle buffers ping-pong, so each (key, buffer state) pair captures once
+    # each configuration is one miss (both ping-pong states captured together)
     plan = ([(DT, 100)] * 3 + [(DT / 2, 100)] * 2 + ["grow"] + [(DT / 2, 100)] * 3
             + [(DT / 2, 60)])
     print(f"[graph] key changes {shape} n={len(pos)}", flush=True)
@@ -111,7 +112,7 @@ def test_key_changes_recapture():
             s.step(dt, pressure_iters=iters, pressure_solver="pcg", pressure_tol=1e-6)
         assert g.last_pressure_iters == a.last_pressure_iters
         _assert_same(a, g)
-    assert (g._cuda_graph_misses, g._cuda_graph_hits) == (7, 2)
+    assert (g._cuda_graph_misses, g._cuda_graph_hits) == (4, 5)
     assert len(g._graphs) <= g._GRAPH_CACHE
 
 
@@ -124,3 +125,28 @@ def test_ineligible_configurations_run_direct():
     g.pcg_kernel_timing = False
     g.step(DT, pressure_iters=100, pressure_solver="pcg")
     assert g._cuda_graph_misses == 1
+
+
+def test_cache_counters_follow_the_reference():
+    """tests/test_b5_2_graph_cache.py:33-74 of the reference: off by default;
+    the first substep misses and the next four hit; prepare_frame drops the
+    cache; changing the iteration cap recaptures once."""
+    from gpufluid_amd import FlipSolver3D
+    s = FlipSolver3D(16, 16, 16, dx=1 / 16, device="cuda:0")
+    s.seed_box((0.2, 0.2, 0.2), (0.6, 0.6, 0.6), ppc=8)
+    s.step(0.005, pressure_iters=20)
+    assert s._cuda_graph is None and (s._cuda_graph_hits, s._cuda_graph_misses) == (0, 0)
+    s = FlipSolver3D(16, 16, 16, dx=1 / 16, device="cuda:0", enable_cuda_graphs=True)
+    s.seed_box((0.2, 0.2, 0.2), (0.6, 0.6, 0.6), ppc=8)
+    s.step(0.005, pressure_iters=20)
+    assert s._cuda_graph is not None and (s._cuda_graph_misses, s._cuda_graph_hits) == (1, 0)
+    for _ in range(4):
+        s.step(0.005, pressure_iters=20)
+    assert (s._cuda_graph_misses, s._cuda_graph_hits) == (1, 4)
+    s.prepare_frame(0, 1 / 24)
+    assert s._cuda_graph is None
+    s = FlipSolver3D(16, 16, 16, dx=1 / 16, device="cuda:0", enable_cuda_graphs=True)
+    s.seed_box((0.2, 0.2, 0.2), (0.6, 0.6, 0.6), ppc=8)
+    for iters in (10, 20, 20):
+        s.step(0.005, pressure_iters=iters)
+    assert (s._cuda_graph_misses, s._cuda_graph_hits) == (2, 1)
