@@ -76,3 +76,51 @@ def test_pcg_finishes_inside_its_budget():
     for _ in range(2):
         s.step(0.005, pressure_iters=40, pressure_solver="pcg")
     assert s.last_pressure_iters < 40, s.last_pressure_iters
+
+
+@pytest.mark.parametrize("n,scale,seed", [(1, 1.0, 0), (1000, 0.5, 1), (50_000, 5.0, 2),
+                                          (200_000, 12.0, 3)])
+def test_gpu_cfl_count_equals_host_count(n, scale, seed):
+    """tests/test_s2_10_gpu_cfl.py:24-40 of the reference."""
+    import torch
+    from gpufluid_amd.solver3d import cfl_substep_count, cfl_substep_count_gpu
+    vel = (np.random.default_rng(seed).standard_normal((n, 3)) * scale).astype(np.float32)
+    got = cfl_substep_count_gpu(torch.tensor(vel, device="cuda:0"), dx=0.0125, target_dt=0.04)
+    assert got == cfl_substep_count(vel, dx=0.0125, target_dt=0.04)
+
+
+def test_gpu_cfl_empty_and_still():
+    import torch
+    from gpufluid_amd.solver3d import cfl_substep_count_gpu
+    assert cfl_substep_count_gpu(torch.zeros((0, 3), device="cuda:0"), dx=0.0125,
+                                 target_dt=0.04) == 1
+    assert cfl_substep_count_gpu(torch.zeros((100, 3), device="cuda:0"), dx=0.0125,
+                                 target_dt=0.04) == 1
+
+
+def _drop_on_sphere(mode, frames):
+    """A fluid column dropped on a solid sphere (tests/test_s2_12_apic_obstacle.py:20-37)."""
+    from gpufluid_amd import FlipSolver3D
+    from gpufluid_amd.animation import sdf_sphere
+    s = FlipSolver3D(nx=48, ny=48, nz=48, dx=1.0 / 48.0, gravity=-9.81, transfer_mode=mode,
+                     flip_blend=0.95, device="cuda:0")
+    s.add_solid_from_sdf(sdf_sphere(s.cell_centers_np(), center=(0.5, 0.35, 0.5), radius=0.10))
+    s.seed_box((0.40, 0.65, 0.40), (0.60, 0.90, 0.60))
+    ke, vmax = [], []
+    for _ in range(frames):
+        s.step(0.005, pressure_iters=40)
+        sp = np.linalg.norm(s.vel.numpy(), axis=1)
+        vmax.append(float(sp.max()))
+        ke.append(float(0.5 * (sp ** 2).sum()))
+    return np.asarray(ke), np.asarray(vmax), s.vel.numpy()
+
+
+def test_apic_near_an_obstacle_is_not_more_energetic_than_flip():
+    """tests/test_s2_12_apic_obstacle.py:39-70 of the reference: no NaN,
+    APIC kinetic energy below 1.2x FLIP's over the impact window, max speed
+    below 2x FLIP's."""
+    ke_f, vm_f, _ = _drop_on_sphere("flip", 40)
+    ke_a, vm_a, vel = _drop_on_sphere("apic", 40)
+    assert np.isfinite(vel).all()
+    assert (ke_a[20:40] / np.maximum(ke_f[20:40], 1.0)).max() < 1.2
+    assert (vm_a / np.maximum(vm_f, 1e-3)).max() < 2.0
