@@ -124,3 +124,51 @@ def test_apic_near_an_obstacle_is_not_more_energetic_than_flip():
     assert np.isfinite(vel).all()
     assert (ke_a[20:40] / np.maximum(ke_f[20:40], 1.0)).max() < 1.2
     assert (vm_a / np.maximum(vm_f, 1e-3)).max() < 2.0
+
+
+def _reseed_input(n=80_000, N=32, seed=0):
+    rng = np.random.default_rng(seed)
+    dx = 1.0 / N
+    pos = (rng.random((n, 3)) * (N * dx * 0.8) + N * dx * 0.1).astype(np.float32)
+    vel = (rng.standard_normal((n, 3)) * 0.1).astype(np.float32)
+    marker = np.zeros((N, N, N), np.int32)
+    for a in range(3):
+        idx = [slice(None)] * 3
+        idx[a] = [0, -1]
+        marker[tuple(idx)] = 2
+    return pos, vel, marker, dx
+
+
+def test_device_and_host_reseed_agree_on_counts():
+    """tests/test_s2_11_gpu_reseed.py:41-56 of the reference: same emitted and
+    culled counts from both forms; no cell keeps more than max_per_cell."""
+    import torch
+    from gpufluid_amd.reseed import ReseedConfig, reseed_particles, reseed_particles_gpu
+    pos, vel, marker, dx = _reseed_input()
+    cfg = ReseedConfig(min_per_cell=4, max_per_cell=16)
+    dev = "cuda:0"
+    p_g, _, _, _, ne_g, nc_g = reseed_particles_gpu(torch.tensor(pos, device=dev),
+                                                    torch.tensor(vel, device=dev),
+                                                    torch.tensor(marker, device=dev), dx, cfg,
+                                                    np.random.default_rng(0))
+    _, _, ne_c, nc_c = reseed_particles(pos, vel, marker, dx, cfg, np.random.default_rng(0))
+    assert (ne_g, nc_g) == (ne_c, nc_c)
+    assert ne_g > 0
+
+
+def test_device_reseed_bounds_the_cell_counts():
+    """tests/test_s2_11_gpu_reseed.py:59-78 of the reference."""
+    import torch
+    from gpufluid_amd.reseed import ReseedConfig, reseed_particles_gpu
+    pos, vel, marker, dx = _reseed_input(n=120_000)
+    cfg = ReseedConfig(min_per_cell=4, max_per_cell=12)
+    dev = "cuda:0"
+    p_g, _, _, _, _, n_cull = reseed_particles_gpu(torch.tensor(pos, device=dev),
+                                                   torch.tensor(vel, device=dev),
+                                                   torch.tensor(marker, device=dev), dx, cfg,
+                                                   np.random.default_rng(0))
+    q = np.clip((p_g.numpy() / dx).astype(np.int32), 0, 31)
+    counts = np.bincount((q[:, 0] * 32 + q[:, 1]) * 32 + q[:, 2],
+                         minlength=32 ** 3).reshape(32, 32, 32)
+    assert n_cull > 0
+    assert int(counts[marker != 2].max()) <= cfg.max_per_cell
