@@ -1774,3 +1774,30 @@ extern "C" int gf_scene_pcg_stats(const gf_scene_t* s, int* iters, double* rel_r
     }
     return 0;
 }
+
+// The reference's on-device stop flag and threshold of its last PCG solve
+// (k3_zero_int_scalar / k3_set_tol_sq / k3_check_converged,
+// solver3d.py:1916-1942, 2949-2983): done = 1 when the solve met its
+// tolerance, tol_sq = tol^2 * ||r0||^2 in FP32 (r0 = b).  A zero right-hand
+// side leaves done = 0, as the reference's early return does.  One thread
+// reads scene 0's result log.
+__global__ void k_pcg_flags(const PcgLog* __restrict__ res, float tol, int32_t* done,
+                            float* tol_sq) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (done) done[0] = (res->converged && res->bb > 0.0) ? 1 : 0;
+    if (tol_sq) tol_sq[0] = tol * tol * (float)res->bb;
+}
+
+extern "C" int gf_scene_pcg_flags(const gf_scene_t* s, float tol, int32_t* done, float* tol_sq,
+                                  void* stream) {
+    int rc = gf::scene_check(s, "gf_scene_pcg_flags");
+    if (rc) return rc;
+    GF_REQUIRE(s->scalars, "gf_scene_pcg_flags: scalars not set");
+    GF_REQUIRE(done || tol_sq, "gf_scene_pcg_flags: no output");
+    const int S = s->n_scenes > 1 ? s->n_scenes : 1;
+    const PcgLog* res = reinterpret_cast<const PcgLog*>(s->scalars) + 2 * S;
+    hipLaunchKernelGGL(k_pcg_flags, dim3(1), dim3(64), 0, (hipStream_t)stream, res, tol, done,
+                       tol_sq);
+    GF_LAUNCH_CHECK("gf_scene_pcg_flags");
+    return 0;
+}

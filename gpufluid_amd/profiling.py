@@ -44,8 +44,12 @@ class StepProfiler:
             self._dict.setdefault(name, []).append(float(start.elapsed_time(end)))
 
     def fused(self, name: str) -> None:
-        """Record a stage that ran inside another section's kernel."""
+        """Record a stage that ran inside another section's kernel (nothing
+        while a stream is being captured, like a section)."""
         if self.enabled:
+            import torch
+            if torch.cuda.is_current_stream_capturing():
+                return
             self._dict.setdefault(name, []).append(0.0)
 
     def summarize(self) -> dict[str, float]:

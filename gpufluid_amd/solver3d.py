@@ -493,8 +493,15 @@ class FlipSolver3D:
         pos, vel, affine = data["pos"], data["vel"], data["affine_C"]
         attrs = {}
         for key in ("attr_color", "attr_temperature"):
-            a = data[key] if key in data.files else None
-            if a is not None and len(a) == 0:
+            if key not in data.files:
+                # an older checkpoint without the key: the reference keeps the
+                # live attribute (solver3d.py:4200-4214); here only while it
+                # still has one entry per restored particle
+                live = self._attr[key[5:]]
+                attrs[key] = live if live is not None and len(live) == len(pos) else None
+                continue
+            a = data[key]
+            if len(a) == 0:
                 a = None
             if a is not None and len(a) != len(pos):
                 raise RuntimeError(f"checkpoint {key} holds {len(a)} entries for {len(pos)} "
@@ -827,11 +834,13 @@ class FlipSolver3D:
     def _cuda_graph_eligible(self, pressure_solver: str, pressure_block_sparse: bool = False) -> bool:
         """Substeps replayed from a hipGraph (solver3d.py:3362-3391): with
         ``enable_cuda_graphs``, every solver and stage combination once
-        particles exist, except while per-kernel PCG timing or the stage
-        profiler need host events."""
+        particles exist, except while per-kernel PCG timing needs host
+        events.  The stage profiler's sections record nothing while a
+        substep is captured or replayed, as the reference's do
+        (profiling.py:51-56)."""
         return (self._enable_cuda_graphs and self.n_particles > 0
                 and pressure_solver in ("jacobi", "gsrb", "pcg")
-                and not self.pcg_kernel_timing and not self._prof.enabled)
+                and not self.pcg_kernel_timing)
 
     def _cuda_graph_invalidate(self):
         """Drop every captured substep (solver3d.py:3414-3419)."""
@@ -1081,7 +1090,12 @@ class FlipSolver3D:
                     if kms is not None:
                         self.last_pcg_kernel_ms = tuple(kms)
                     self._pressure_stats = (it.value, res.value, nrep.value)
-                self._pcg_tol_sq.t.fill_(float(np.float32(tol) ** 2))
+                if capture or sc.n_active_host > 0:
+                    _lib.call("S2.6.3", "gf_scene_pcg_flags", ctypes.byref(s), tol,
+                              _lib.ptr(self._pcg_done.t), _lib.ptr(self._pcg_tol_sq.t), stream)
+                else:   # nothing to solve: the reference's early return
+                    self._pcg_done.t.zero_()
+                    self._pcg_tol_sq.t.zero_()
         with prof.section("g2p_advect"):
             if pre is None:
                 _lib.call("S2.8", "gf_scene_g2p", ctypes.byref(s), _lib.ptr(sc.pc), mode,
@@ -1090,8 +1104,10 @@ class FlipSolver3D:
                 _lib.call("S2.8", "gf_scene_g2p_pre", ctypes.byref(s), _lib.ptr(sc.pc),
                           *(_lib.ptr(t) for t in pre), mode, dt, self.rho, flip_blend, stream)
         prof.fused("grad_subtract_bc")
-        if n > 0 and self._attr["color"] is not None:
-            with prof.section("color"):
+        # the colour section opens every step, as the reference's does
+        # (solver3d.py:3656-3658); the transfer runs when the attribute exists
+        with prof.section("color"):
+            if n > 0 and self._attr["color"] is not None:
                 self._attr_transfer(s, self._attr["color"], 3, _lib.GF_ATTR_COLOR, stream)
         if n > 0 and self._attr["temperature"] is not None:
             with prof.section("temperature"):

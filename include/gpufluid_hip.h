@@ -245,6 +245,16 @@ int gf_scene_pcg_enqueue(const gf_scene_t* s, int n_active, int precision, doubl
 int gf_scene_pcg_stats(const gf_scene_t* s, int* iters, double* rel_residual, int* replacements,
                        void* stream);
 
+/* The reference's PCG stop flag and threshold after the last solve on this
+ * descriptor (scene 0), written on the device (enqueue only, capturable):
+ * done[0] = 1 if the solve met its tolerance (0 for a zero right-hand side),
+ * tol_sq[0] = FP32 tol^2 * ||b||^2.  Replaces the values that
+ * k3_zero_int_scalar / k3_set_tol_sq / k3_check_converged leave in
+ * FlipSolver3D._pcg_done / _pcg_tol_sq (solver3d.py:1916-1942, 2949-2983).
+ * Either output may be NULL, not both. */
+int gf_scene_pcg_flags(const gf_scene_t* s, float tol, int32_t* done, float* tol_sq,
+                       void* stream);
+
 /* Fused pressure-gradient subtract + solid BC + G2P + advection over 8^3
  * tiles: k3_subtract_pressure_grad (solver3d.py:1364-1407), k3_enforce_solid_bc
  * (758-799), k3_g2p_and_advect (1414-1463) or k3_g2p_apic_advect (1591-1732).

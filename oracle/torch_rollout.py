@@ -116,11 +116,13 @@ def _sample3(f, fx, fy, fz):
     return c0 * (1 - sz) + c1 * sz
 
 
-def substep(x, v, walls, dx, dt, gravity=-9.81, rho=1.0, flip_blend=0.95):
+def substep(x, v, walls, dx, dt, gravity=-9.81, rho=1.0, flip_blend=0.95, marker=None):
     """One FLIP substep (FP64 torch): returns (x', v').  ``walls`` is the
-    static marker (solids); the fluid mask comes from the FP32 positions."""
+    static marker (solids); the fluid mask comes from the FP32 positions, or
+    is ``marker`` when given (the mask of another trajectory, held fixed)."""
     shape = walls.shape
-    marker = fo.update_marker(walls, x.detach().numpy().astype(np.float32), dx)
+    if marker is None:
+        marker = fo.update_marker(walls, x.detach().numpy().astype(np.float32), dx)
     fluid = torch.from_numpy((marker == fo.FLUID).astype(np.float64))
     dxf = float(np.float32(dx))
     (qu, mu), (qv, mv), (qw, mw) = _p2g(x, v, shape, dxf)
@@ -177,16 +179,20 @@ def substep(x, v, walls, dx, dt, gravity=-9.81, rho=1.0, flip_blend=0.95):
     return nx_, nv
 
 
-def rollout(x0, v0, walls, dx, dt, substeps, checkpoint=True, **kw):
+def rollout(x0, v0, walls, dx, dt, substeps, checkpoint=True, markers=None, **kw):
     """``substeps`` substeps from (x0, v0) (FP64 tensors); with ``checkpoint``
-    every substep is recomputed in the backward (torch.utils.checkpoint)."""
+    every substep is recomputed in the backward (torch.utils.checkpoint).
+    ``markers``: one fluid mask per substep to hold fixed (e.g. those of the
+    GPU's FP32 trajectory), so both runs take the same discrete branch."""
     x, v = x0, v0
-    for _ in range(substeps):
+    for k in range(substeps):
+        mk = None if markers is None else markers[k]
         if checkpoint and (x.requires_grad or v.requires_grad):
             x, v = torch.utils.checkpoint.checkpoint(
-                lambda a, b: substep(a, b, walls, dx, dt, **kw), x, v, use_reentrant=False)
+                lambda a, b, m=mk: substep(a, b, walls, dx, dt, marker=m, **kw), x, v,
+                use_reentrant=False)
         else:
-            x, v = substep(x, v, walls, dx, dt, **kw)
+            x, v = substep(x, v, walls, dx, dt, marker=mk, **kw)
     return x, v
 
 

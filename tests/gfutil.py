@@ -71,3 +71,19 @@ def run_p2g(s, dt, gravity=None, staged=False):
 def oracle_state(shape, dx, pos, vel, marker=None):
     m = fo.wall_marker(*shape) if marker is None else marker.copy()
     return {"pos": pos.copy(), "vel": vel.copy(), "marker": m, "dx": dx}
+
+
+def compact_to_dense_host(s, vec):
+    """A compact per-tile vector (512 entries per active slot, cell
+    (li*8+lj)*8+lk) as a dense (nx,ny,nz) numpy array, on the host, from the
+    solver's tile map; cells of inactive tiles are 0."""
+    vec = vec.detach().cpu().numpy() if hasattr(vec, "detach") else np.asarray(vec)
+    nx, ny, nz = s.nx, s.ny, s.nz
+    tnx, tny, tnz = -(-nx // 8), -(-ny // 8), -(-nz // 8)
+    tmap = s._sc.tile_map.cpu().numpy().reshape(tnx, tny, tnz)
+    full = np.zeros((tnx * 8, tny * 8, tnz * 8), dtype=vec.dtype)
+    ti, tj, tk = np.nonzero(tmap >= 0)
+    blocks = vec[: s._sc.compact].reshape(-1, 8, 8, 8)[tmap[ti, tj, tk]]
+    for t in range(len(ti)):
+        full[ti[t] * 8:ti[t] * 8 + 8, tj[t] * 8:tj[t] * 8 + 8, tk[t] * 8:tk[t] * 8 + 8] = blocks[t]
+    return full[:nx, :ny, :nz]

@@ -309,3 +309,59 @@ def test_gsrb_converges_to_the_direct_solve_faster_than_jacobi():
     # one red + black sweep from the exact solution leaves it unchanged
     p = fo.gsrb_half_sweep(exact, div, m, 0)
     np.testing.assert_allclose(fo.gsrb_half_sweep(p, div, m, 1), exact, atol=1e-12)
+
+
+def _linear_face_fields(coef, shape, dx):
+    """Face arrays of the linear functions f_c(x) = coef[c,0] + coef[c,1:]·x
+    sampled at the MAC face positions: u at (i, j+1/2, k+1/2)·dx, v at
+    (i+1/2, j, k+1/2)·dx, w at (i+1/2, j+1/2, k)·dx."""
+    nx, ny, nz = shape
+    out = []
+    for c, (sh, off) in enumerate((((nx + 1, ny, nz), (0.0, 0.5, 0.5)),
+                                   ((nx, ny + 1, nz), (0.5, 0.0, 0.5)),
+                                   ((nx, ny, nz + 1), (0.5, 0.5, 0.0)))):
+        i, j, k = np.meshgrid(*(np.arange(s, dtype=np.float64) for s in sh), indexing="ij")
+        x = np.stack([(i + off[0]) * dx, (j + off[1]) * dx, (k + off[2]) * dx])
+        out.append(coef[c, 0] + np.tensordot(coef[c, 1:], x, axes=1))
+    return out
+
+
+@pytest.mark.parametrize("fb", [0.5, 0.95, 1.0])
+def test_flip_blend_closed_form(fb):
+    """The FLIP branch of g2p_advect against the reference formula written
+    out by hand (solver3d.py:1447-1453): with grid fields u (post-projection)
+    and us (pre-projection), v_new = fb*(v_p + u(x_p) - us(x_p)) +
+    (1-fb)*u(x_p), x_new = x_p + v_new*dt.  Constant fields u = a, us = a - c
+    give v_new = fb*(v_p + c) + (1-fb)*a; linear fields pin the half-cell
+    offsets of each component's sampling lattice.  A swapped u/us, a sign
+    error in (pic - old), a transposed component or a wrong stagger fails."""
+    rng = np.random.default_rng(21)
+    shape, dx, dt = (10, 12, 9), 1 / 12, 1e-3
+    dom = [fo.domain(n, dx) for n in shape]
+    n = 300
+    # particles at least one cell inside the domain, so no sample is clamped
+    lo, hi = 1.5 * dx, (np.array(shape) - 1.5) * dx
+    pos = (lo + rng.random((n, 3)) * (hi - lo)).astype(np.float32)
+    vel = rng.normal(size=(n, 3)).astype(np.float32)
+    x = pos.astype(np.float64)
+    # constant fields, one value per component
+    a = np.array([0.7, -1.3, 2.1])
+    c = np.array([-0.4, 0.9, 0.25])
+    u, v, w = (np.full(s, a[i]) for i, s in enumerate(((11, 12, 9), (10, 13, 9), (10, 12, 10))))
+    us, vs, ws = u - c[0], v - c[1], w - c[2]
+    p1, v1 = fo.g2p_advect(pos, vel, u, v, w, us, vs, ws, dx, dt, fb, dom)
+    want = fb * (vel.astype(np.float64) + c) + (1 - fb) * a
+    np.testing.assert_allclose(v1, want, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(p1, x + want * dt, rtol=1e-12, atol=1e-12)
+    # linear fields: pic and old are the functions at the particle position
+    cu = rng.normal(size=(3, 4))
+    cs = rng.normal(size=(3, 4))
+    dx32 = float(np.float32(dx))       # the lattice spacing the FP32 position divides by
+    u, v, w = _linear_face_fields(cu, shape, dx32)
+    us, vs, ws = _linear_face_fields(cs, shape, dx32)
+    p2, v2 = fo.g2p_advect(pos, vel, u, v, w, us, vs, ws, dx, dt, fb, dom)
+    pic = cu[:, 0] + x @ cu[:, 1:].T
+    old = cs[:, 0] + x @ cs[:, 1:].T
+    want = fb * (vel.astype(np.float64) + pic - old) + (1 - fb) * pic
+    np.testing.assert_allclose(v2, want, rtol=1e-9, atol=1e-9)
+    np.testing.assert_allclose(p2, x + want * dt, rtol=1e-9, atol=1e-9)
