@@ -93,11 +93,21 @@ def _flags(s):
     return int(s._pcg_done.numpy()[0]), float(s._pcg_tol_sq.numpy()[0])
 
 
+def _dam(**kw):
+    """A block resting on the floor wall: its first substep has a nonzero
+    divergence (a box in free fall has none and the solve does nothing)."""
+    s = _solver(32, gravity=-9.81, **kw)
+    s.seed_box((0.05, 0.05, 0.05), (0.5, 0.6, 0.5), ppc=4)
+    return s
+
+
 def test_pcg_done_flag_surface_and_meaning():
     s = _seeded()
     s.step(0.005, pressure_iters=5, pressure_solver="pcg")
     assert s._pcg_done.shape == (1,) and s._pcg_done.dtype == np.int32
     assert s._pcg_tol_sq.shape == (1,) and s._pcg_tol_sq.dtype == np.float32
+    s = _dam()
+    s.step(0.005, pressure_iters=5, pressure_solver="pcg")
     # 5 iterations do not reach the default 1e-4: the flag stays 0
     done, tol_sq = _flags(s)
     assert s.last_pressure_iters == 5 and done == 0
@@ -108,7 +118,7 @@ def test_pcg_done_flag_surface_and_meaning():
     assert tol_sq == pytest.approx(np.float32(1e-4) ** 2 * bb, rel=1e-5)
     # a solve that meets its tolerance raises the flag, in both paths
     for graphs in (False, True):
-        s = _seeded(enable_cuda_graphs=graphs)
+        s = _dam(enable_cuda_graphs=graphs)
         for _ in range(2):
             s.step(0.005, pressure_iters=200, pressure_solver="pcg")
             done, tol_sq = _flags(s)
@@ -117,7 +127,7 @@ def test_pcg_done_flag_surface_and_meaning():
 
 
 def test_pcg_done_is_zero_without_fluid_to_solve():
-    s = _seeded()
+    s = _dam()
     s.step(0.005, pressure_iters=200, pressure_solver="pcg")
     assert _flags(s)[0] == 1
     s.set_particles(np.zeros((0, 3), np.float32))
@@ -145,9 +155,10 @@ def test_checkpoint_without_attribute_keys_keeps_live_attributes(tmp_path):
     old = tmp_path / "old.npz"
     np.savez(old, **data)
     live_col = s.attr_color.numpy().copy()
+    live_t = s.attr_temperature.numpy().copy()
     s.load_checkpoint(old)              # same particle count: kept
     np.testing.assert_array_equal(s.attr_color.numpy(), live_col)
-    np.testing.assert_array_equal(s.attr_temperature.numpy(), np.full(n, 25.0, np.float32))
+    np.testing.assert_array_equal(s.attr_temperature.numpy(), live_t)
     data["pos"], data["vel"] = data["pos"][:10], data["vel"][:10]
     fewer = tmp_path / "fewer.npz"
     np.savez(fewer, **data)
