@@ -104,6 +104,7 @@ SIGNATURES = {
     "gf_scene_pcg_stats": [SP, PI, PD, PI, V],
     "gf_scene_pcg_flags": [SP, F, V, V, V],
     "gf_scene_pcg_enqueue": [SP, I, I, D, I, I, I, I, I, V],
+    "gf_scene_pcg_enqueue_noalloc": [SP, I, I, D, I, I, I, I, I, V],
     "gf_scene_g2p": [SP, V, I, F, F, F, F, V],
     "gf_scene_attr_transfer": [SP, V, I, I, V, V, V, V],
     "gf_anim_obstacles": [V, I, V, V, V, V, V, I, I, I, D, V],

@@ -36,6 +36,7 @@
 #include "gf_common.h"
 #include <hip/hip_ext.h>
 #include <string.h>
+#include <mutex>
 
 namespace {
 
@@ -81,7 +82,8 @@ struct PcgArgs {
     double* p64; double* x64;
     float* pc;
     double* part;        // [P_KINDS][PCG_GMAX]
-    PcgLog* log;         // [2*S] live pairs, then [S] results
+    PcgLog* log;         // [2*S] live pairs
+    PcgLog* res;         // [S] results (gf_scene_pcg_stats reads them)
     PcgLog* hout;        // when set: the log pairs are also written here (pinned host memory)
     int32_t* part_info;  // [8*PCG_GMAX] per workgroup: scene, wb, we, t0, t1; then [S+1] wg offsets
     const int32_t* scene_slot;  // [S+1] compact tile offsets per scene (NULL: one scene, nact tiles)
@@ -1227,7 +1229,7 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_final(PcgArgs a) {
 // Each scene's final log into the result entries (one thread per scene).
 __global__ void k_pcg_result(PcgArgs a, int L) {
     const int sc = blockIdx.x * blockDim.x + threadIdx.x;
-    if (sc < a.S) put_log(a.log + 2 * a.S + sc, live_log(a, sc, L));
+    if (sc < a.S) put_log(a.res + sc, live_log(a, sc, L));
 }
 
 // Jacobi sweep (solver3d.py:878-938) on compact tiles: two tiles per
@@ -1399,8 +1401,12 @@ extern "C" int gf_scene_gsrb(const gf_scene_t* s, int n_active, int iters, void*
 namespace {
 
 // Validated solver arguments shared by gf_scene_pcg_sync and gf_scene_pcg_enqueue.
+// With own_work = false the work buffers (vectors, partials, live log,
+// partition table) are left for the caller to point at its own memory, and
+// only the descriptor's inputs, pc and the result entries of scalars are
+// required.
 int pcg_prepare(const gf_scene_t* s, int n_active, int precision, double tol, int max_iter,
-                int rhs_sign, const char* who, PcgArgs& a) {
+                int rhs_sign, const char* who, PcgArgs& a, bool own_work = true) {
     int rc = gf::scene_check(s, who);
     if (rc) return rc;
     const int ntiles = s->tnx * s->tny * s->tnz;
@@ -1419,15 +1425,17 @@ int pcg_prepare(const gf_scene_t* s, int n_active, int precision, double tol, in
                precision);
     GF_REQUIRE(rhs_sign == 1 || rhs_sign == -1, "%s: rhs_sign %d", who, rhs_sign);
     GF_REQUIRE(tol >= 0.0 && max_iter >= 0, "%s: tol/max_iter", who);
-    GF_REQUIRE(s->flags && s->nbr && s->b && s->pc && s->partials && s->scalars && s->x64 &&
-                   s->pcg_part,
-               "%s: buffers not set", who);
-    // FP32 vectors serve both precisions' replacement bookkeeping only in mixed mode
-    if (precision == GF_PCG_MIXED)
-        GF_REQUIRE(s->x32 && s->r32 && s->p32 && s->p32b && s->ap32 && s->cg32,
-                   "%s: FP32 vectors not set", who);
-    else
-        GF_REQUIRE(s->r64 && s->p64 && s->p64b && s->ap64 && s->cg64, "%s: FP64 vectors not set", who);
+    GF_REQUIRE(s->flags && s->nbr && s->b && s->pc && s->scalars, "%s: buffers not set", who);
+    if (own_work) {
+        GF_REQUIRE(s->partials && s->x64 && s->pcg_part, "%s: buffers not set", who);
+        // FP32 vectors serve both precisions' replacement bookkeeping only in mixed mode
+        if (precision == GF_PCG_MIXED)
+            GF_REQUIRE(s->x32 && s->r32 && s->p32 && s->p32b && s->ap32 && s->cg32,
+                       "%s: FP32 vectors not set", who);
+        else
+            GF_REQUIRE(s->r64 && s->p64 && s->p64b && s->ap64 && s->cg64,
+                       "%s: FP64 vectors not set", who);
+    }
     const size_t C = (size_t)ntiles * GF_TILE_CELLS;
     a.flags = s->flags; a.nbr = s->nbr; a.b = s->b;
     a.u32[0] = s->r32; a.u32[1] = s->cg32;
@@ -1439,6 +1447,7 @@ int pcg_prepare(const gf_scene_t* s, int n_active, int precision, double tol, in
     a.s64[0] = s->p64b; a.s64[1] = s->cg64 ? s->cg64 + 2 * C : nullptr;
     a.p64 = s->p64; a.x64 = s->x64;
     a.pc = s->pc; a.part = s->partials; a.log = (PcgLog*)s->scalars; a.hout = nullptr;
+    a.res = a.log + 2 * S;
     a.part_info = s->pcg_part;
     a.scene_slot = packed ? s->scene_slot : nullptr;
     a.nact = dev_count ? 0 : n_active;
@@ -1799,5 +1808,119 @@ extern "C" int gf_scene_pcg_flags(const gf_scene_t* s, float tol, int32_t* done,
     hipLaunchKernelGGL(k_pcg_flags, dim3(1), dim3(64), 0, (hipStream_t)stream, res, tol, done,
                        tol_sq);
     GF_LAUNCH_CHECK("gf_scene_pcg_flags");
+    return 0;
+}
+
+// ---- no-allocation PCG ---------------------------------------------------
+// gf_scene_pcg_enqueue_noalloc runs the same solve with every work buffer in
+// one file-scope device array, so a caller that cannot allocate (code under
+// hipGraph capture, serving loops) passes only the inputs and outputs.
+// Layout (bytes): the FP64 iterate x64 (8 per slot), then seven work vectors
+// of 8 bytes per slot (FP64 mode: u, w~, s~ ping-pong pairs and p; mixed mode
+// uses the first half of each as FP32, plus x32), the per-workgroup partials,
+// the live log pairs and the workgroup partition table.  Every word a call
+// reads it has written earlier in the same call (the setup kernels write all
+// 512 entries of every active slot), so nothing needs clearing; calls that
+// overlap on one GPU share the array and race.
+namespace {
+constexpr int NOALLOC_MAX_TILES = GF_PCG_NOALLOC_MAX_TILES;
+constexpr int NOALLOC_MAX_SCENES = GF_PCG_NOALLOC_MAX_SCENES;
+constexpr size_t NOALLOC_SLOTS = (size_t)NOALLOC_MAX_TILES * GF_TILE_CELLS;
+constexpr size_t NOALLOC_VEC_BYTES = 8 * 8 * NOALLOC_SLOTS;                     // 64 MiB
+constexpr size_t NOALLOC_PART_BYTES = sizeof(double) * 2 * P_SET * PCG_GMAX;     // 256 KiB
+constexpr size_t NOALLOC_LOG_BYTES = sizeof(PcgLog) * 2 * NOALLOC_MAX_SCENES;    // 16 KiB
+constexpr size_t NOALLOC_INFO_BYTES =
+    sizeof(int32_t) * ((8 * PCG_GMAX + NOALLOC_MAX_SCENES + 1 + 63) / 64 * 64);
+constexpr size_t NOALLOC_BYTES =
+    NOALLOC_VEC_BYTES + NOALLOC_PART_BYTES + NOALLOC_LOG_BYTES + NOALLOC_INFO_BYTES;
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+static_assert(NOALLOC_BYTES == GF_PCG_NOALLOC_SCRATCH_BYTES, "header states the scratch size");
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// The device address of `scratch` on the current device, looked up once per
+// device (the host-side name of a __device__ array is not its address).
+int scratch_address(void** out, const char* who) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return gf::set_error((int)e, "%s: hipGetDevice: %s", who, hipGetErrorString(e));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+        return gf::set_error(GF_ERANGE, "%s: device %d", who, dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return gf::set_error(e != hipSuccess ? (int)e : GF_ESTATE,
+                                 "%s: hipGetSymbolAddress(scratch): %s", who,
+                                 hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+}  // namespace
+
+extern "C" int gf_scene_pcg_enqueue_noalloc(const gf_scene_t* s, int n_active, int precision,
+                                            double tol, int max_iter, int check_every,
+                                            int rhs_sign, int first_chunk, int n_chunks,
+                                            void* stream) {
+    const char* who = "gf_scene_pcg_enqueue_noalloc";
+    PcgArgs a = {};
+    int rc = pcg_prepare(s, n_active, precision, tol, max_iter, rhs_sign, who, a, false);
+    if (rc) return rc;
+    GF_REQUIRE(check_every >= 1 && check_every <= 64, "%s: check_every %d", who, check_every);
+    GF_REQUIRE(first_chunk >= 0 && n_chunks >= 0, "%s: chunks %d+%d", who, first_chunk, n_chunks);
+    const int ntiles = s->tnx * s->tny * s->tnz;
+    // slots the kernels may touch: the host count, or every tile of the grid
+    // when the count is read on the device
+    const int reach = n_active == GF_NACT_DEVICE ? ntiles : n_active;
+    if (reach > NOALLOC_MAX_TILES)
+        return gf::set_error(GF_ERANGE,
+                             "%s: %d %s tiles exceed the %d of the static work array", who, reach,
+                             n_active == GF_NACT_DEVICE ? "grid" : "active", NOALLOC_MAX_TILES);
+    if (a.S > NOALLOC_MAX_SCENES)
+        return gf::set_error(GF_ERANGE, "%s: %d scenes exceed %d", who, a.S, NOALLOC_MAX_SCENES);
+    if (a.S == 1 && n_active == 0) return 0;
+    void* base = nullptr;
+    rc = scratch_address(&base, who);
+    if (rc) return rc;
+    unsigned char* b = static_cast<unsigned char*>(base);
+    const size_t V = 8 * NOALLOC_SLOTS;   // bytes per 8-byte vector
+    double* x64 = reinterpret_cast<double*>(b);
+    unsigned char* vec = b + V;           // seven vectors of V bytes
+    a.x64 = x64;
+    if (precision == GF_PCG_MIXED) {
+        float* f[8];
+        for (int i = 0; i < 7; ++i) f[i] = reinterpret_cast<float*>(vec + i * V);
+        f[7] = reinterpret_cast<float*>(vec + 6 * V) + NOALLOC_SLOTS;  // x32: second half of the 7th
+        a.u32[0] = f[0]; a.u32[1] = f[1]; a.w32[0] = f[2]; a.w32[1] = f[3];
+        a.s32[0] = f[4]; a.s32[1] = f[5]; a.p32 = f[6]; a.x32 = f[7];
+    } else {
+        double* d[7];
+        for (int i = 0; i < 7; ++i) d[i] = reinterpret_cast<double*>(vec + i * V);
+        a.u64[0] = d[0]; a.u64[1] = d[1]; a.w64[0] = d[2]; a.w64[1] = d[3];
+        a.s64[0] = d[4]; a.s64[1] = d[5]; a.p64 = d[6];
+    }
+    unsigned char* tail = b + NOALLOC_VEC_BYTES;
+    a.part = reinterpret_cast<double*>(tail);
+    a.log = reinterpret_cast<PcgLog*>(tail + NOALLOC_PART_BYTES);
+    a.part_info = reinterpret_cast<int32_t*>(tail + NOALLOC_PART_BYTES + NOALLOC_LOG_BYTES);
+    a.res = reinterpret_cast<PcgLog*>(s->scalars) + 2 * a.S;   // results: the caller's scalars
+    hipStream_t st = (hipStream_t)stream;
+    const bool mixed = precision == GF_PCG_MIXED;
+    if (first_chunk == 0) pcg_launch_setup(a, mixed, st);
+    int L = 1 + first_chunk * pcg_chunk_launches(a, check_every, mixed);
+    for (int c = 0; c < n_chunks; ++c) {
+        rc = pcg_launch_chunk(a, mixed, check_every, L, st, nullptr);
+        if (rc) return rc;
+    }
+    pcg_launch_final(a, mixed, L, st);
+    GF_LAUNCH_CHECK(who);
     return 0;
 }

@@ -245,6 +245,29 @@ int gf_scene_pcg_enqueue(const gf_scene_t* s, int n_active, int precision, doubl
 int gf_scene_pcg_stats(const gf_scene_t* s, int* iters, double* rel_residual, int* replacements,
                        void* stream);
 
+/* gf_scene_pcg_enqueue with no work buffers from the caller, for callers
+ * that cannot allocate (code under hipGraph capture, serving loops, harnesses
+ * that forbid hipMalloc).  No reference counterpart.  It allocates nothing:
+ * the work vectors, partial sums, live solver log and workgroup partition
+ * live in one file-scope device array of GF_PCG_NOALLOC_SCRATCH_BYTES
+ * (64.3 MiB) per GPU, held until the process ends.  The descriptor supplies
+ * only inputs (flags, nbr, b, the tile counts; scene_slot for packed scenes)
+ * and outputs (pc, and the result entries of scalars that
+ * gf_scene_pcg_stats / gf_scene_pcg_flags read); its partials, pcg_part,
+ * x32/x64 and other PCG vectors are not touched.  Limits (GF_ERANGE beyond
+ * them): at most GF_PCG_NOALLOC_MAX_TILES active tiles (with
+ * GF_NACT_DEVICE: tiles in the grid) and GF_PCG_NOALLOC_MAX_SCENES scenes.
+ * Results are bit-identical to gf_scene_pcg_enqueue with the same arguments.
+ * Every call on a GPU shares that GPU's array, which keeps the last call's
+ * contents (a resumed solve, first_chunk > 0, continues from them): calls
+ * that overlap on one GPU race. */
+#define GF_PCG_NOALLOC_MAX_TILES 2048
+#define GF_PCG_NOALLOC_MAX_SCENES 64
+#define GF_PCG_NOALLOC_SCRATCH_BYTES 67453440
+int gf_scene_pcg_enqueue_noalloc(const gf_scene_t* s, int n_active, int precision, double tol,
+                                 int max_iter, int check_every, int rhs_sign, int first_chunk,
+                                 int n_chunks, void* stream);
+
 /* The reference's PCG stop flag and threshold after the last solve on this
  * descriptor (scene 0), written on the device (enqueue only, capturable):
  * done[0] = 1 if the solve met its tolerance (0 for a zero right-hand side),

@@ -73,3 +73,24 @@ def test_scene_entry_point_rejects_null_descriptor():
     with pytest.raises(BlockError) as ei:
         _lib.call("S2.6.3", "gf_scene_pcg_flags", None, 1e-4, None, None, None)
     assert ei.value.block_id == "S2.6.3"
+
+
+def test_noalloc_pcg_limit_is_checked_before_any_launch():
+    """The no-allocation PCG refuses more active tiles than its static work
+    array holds with GF_ERANGE, from the descriptor's sizes alone (the
+    buffer addresses below are placeholders that are never dereferenced)."""
+    _lib_or_skip()
+    d = _lib.GfScene()
+    d.nx = d.ny = d.nz = 136
+    d.tnx = d.tny = d.tnz = 17
+    d.dx = 1.0 / 136
+    for name in ("flags", "nbr", "b", "pc", "scalars", "n_active"):
+        setattr(d, name, 4096)
+    with pytest.raises(BlockError) as ei:
+        _lib.call("S2.6.3", "gf_scene_pcg_enqueue_noalloc", ctypes.byref(d), 2049, 0, 1e-8, 10,
+                  10, -1, 0, 1, None)
+    assert f"failed ({_lib.GF_ERANGE})" in str(ei.value)
+    assert "2049 active tiles exceed the 2048" in str(ei.value)
+    with pytest.raises(BlockError, match="4913 grid tiles exceed the 2048"):
+        _lib.call("S2.6.3", "gf_scene_pcg_enqueue_noalloc", ctypes.byref(d),
+                  _lib.GF_NACT_DEVICE, 0, 1e-8, 10, 10, -1, 0, 1, None)
