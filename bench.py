@@ -75,6 +75,17 @@ def measured_traffic(stage):
     return sum(found[k] for k in want)
 
 
+def _spread(iters, residuals, cap):
+    """min / median / max of the timed solves' iteration counts and FP64 true
+    relative residuals, and how many stopped at the cap."""
+    import numpy as np
+    r = np.asarray(residuals, dtype=np.float64)
+    return {"iters_min": int(min(iters)), "iters_median": float(np.median(iters)),
+            "iters_max": int(max(iters)), "at_cap": int(sum(i >= cap for i in iters)),
+            "rel_residual_min": float(r.min()), "rel_residual_median": float(np.median(r)),
+            "rel_residual_max": float(r.max()), "rel_residuals": [float(x) for x in r]}
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -94,16 +105,60 @@ def _cpu_model():
     return "unknown"
 
 
+def _cpu_topology():
+    """lscpu's view of this host: model, threads per core, and the CPUs this
+    process may run on (its affinity mask, which a container or a scheduler's
+    CPU share sets)."""
+    info = {"model": _cpu_model(), "cpus_allowed": len(os.sched_getaffinity(0))}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "Thread(s) per core":
+                info["threads_per_core"] = int(v)
+            elif k == "Core(s) per socket":
+                info["cores_per_socket"] = int(v)
+            elif k == "Socket(s)":
+                info["sockets"] = int(v)
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    tpc = info.get("threads_per_core", 1)
+    info["physical_cores_allowed"] = max(1, info["cpus_allowed"] // max(1, tpc))
+    return info
+
+
+def _run_cpu(exe, td, scene, substeps, threads, max_iter=None):
+    import numpy as np
+    p = scene.params
+    pfile = Path(td) / f"{scene.name}.f32"
+    if not pfile.exists():
+        np.concatenate([scene.pos, scene.vel], axis=1).astype(np.float32).tofile(pfile)
+    cmd = [str(exe), str(scene.shape[0]), repr(scene.dx), repr(p.get("dt", 1 / 240)),
+           repr(p.get("tol", 1e-8)), str(max_iter or p.get("max_iter", 500)), str(substeps),
+           str(pfile), str(len(scene.pos))]
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    r = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=600)
+    if r.returncode != 0:
+        log(f"cpu baseline failed: {r.stderr[-400:]}")
+        return None
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    out["threads"] = threads
+    return out
+
+
 def cpu_baseline(scene):
-    """Time the C restatement of the reference algorithm (oracle/flip_cpu.c)
-    on this host's cores for one substep of the same scene, with the same PCG
-    tolerance and cap.  It is compiled for the host CPU in a temporary
+    """Time the C restatement of the reference algorithm (oracle/flip_cpu.c,
+    checked against the numpy oracle by tests/test_cpu_restatement.py) on this
+    host's cores: one substep of the benchmark scene on every CPU this process
+    may use, and the 32^3 dam break (C1) for its full 100 substeps on all of
+    them and on one thread.  Compiled for the host CPU in a temporary
     directory (the prebuilt oracle/_build/flip_cpu is the fallback)."""
     import tempfile
-    import numpy as np
+    from gpufluid_amd import seeding
     src = ROOT / "oracle" / "flip_cpu.c"
-    threads = min(16, len(os.sched_getaffinity(0)))
-    p = scene.params
+    topo = _cpu_topology()
+    threads = topo["cpus_allowed"]
     with tempfile.TemporaryDirectory() as td:
         exe = Path(td) / "flip_cpu"
         r = subprocess.run(["gcc", "-O3", "-march=native", "-fopenmp", "-std=c11", str(src), "-o",
@@ -113,21 +168,17 @@ def cpu_baseline(scene):
             if not exe.exists():
                 log("cpu baseline: no compiler and no prebuilt binary")
                 return None
-        pfile = Path(td) / "particles.f32"
-        np.concatenate([scene.pos, scene.vel], axis=1).astype(np.float32).tofile(pfile)
-        n = scene.shape[0]
-        cmd = [str(exe), str(n), repr(scene.dx), repr(p.get("dt", 1 / 240)), repr(p.get("tol", 1e-8)),
-               str(p.get("max_iter", 500)), "1", str(pfile), str(len(scene.pos))]
-        env = dict(os.environ, OMP_NUM_THREADS=str(threads))
-        log(f"cpu baseline: {threads} threads on {_cpu_model()}")
-        r = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=600)
-    if r.returncode != 0:
-        log(f"cpu baseline failed: {r.stderr[-400:]}")
-        return None
-    out = json.loads(r.stdout.strip().splitlines()[-1])
-    out["cores"] = threads
-    out["cpu"] = _cpu_model()
-    return out
+        log(f"cpu baseline: {threads} threads on {topo}")
+        main = _run_cpu(exe, td, scene, 1, threads)
+        if main is None:
+            return None
+        c1 = seeding.config_c1()
+        c1_all = _run_cpu(exe, td, c1, c1.params["substeps"], threads, max_iter=1000)
+        c1_one = _run_cpu(exe, td, c1, c1.params["substeps"], 1, max_iter=1000)
+    main["cores"] = threads
+    main["topology"] = topo
+    main["c1_100_substeps"] = {"all_threads": c1_all, "one_thread": c1_one}
+    return main
 
 
 def main():
@@ -305,7 +356,8 @@ def main():
 
     fp64 = None
     if not args.no_fp64_compare:
-        fp64 = compare_fp64(s, tol, max_iter, args.check_every)
+        fp64 = compare_fp64(s, tol, max_iter, args.check_every,
+                            (s.last_pressure_iters, s.last_pressure_residual))
         log(f"pcg mixed vs fp64: {fp64}")
 
     attrs = None
@@ -353,8 +405,8 @@ def main():
         if c is not None:
             cpu = {"value": round(c["mps"], 4), "unit": "Mparticle-substeps/s",
                    "cores": c["cores"], "kind": "port", "sample": c.get("sample"),
-                   "cpu": c.get("cpu"), "pcg_iters": c.get("iters"),
-                   "seconds": c.get("seconds"),
+                   "topology": c.get("topology"), "pcg_iters": c.get("iters"),
+                   "seconds": c.get("seconds"), "c1_100_substeps": c.get("c1_100_substeps"),
                    "label": "reference-algorithm CPU restatement (oracle/flip_cpu.c, FP32 "
                             "Jacobi-PCG checked every 10 iterations, dense grid)"}
             log(f"cpu baseline: {cpu}")
@@ -369,6 +421,8 @@ def main():
                    "transfer": p.get("transfer", "flip"), "dt": dt, "pcg_tol": tol,
                    "pcg_max_iter": max_iter, "pcg_precision": args.precision,
                    "pcg_iters": iters, "pcg_replacements": sum(st["nrep"] for st in stats),
+                   "pcg_timed_solves": _spread([st["iters"] for st in stats],
+                                               [st["res"] for st in stats], max_iter),
                    "fluid_cells": int(nf), "active_tiles": int(na // 512)},
         "roofline": roofline,
         "cpu_baseline": cpu,
@@ -679,29 +733,54 @@ def pcie_inclusive(s, npart, ms_step):
             "h2d_ms": round(h2d, 3), "d2h_ms": round(d2h, 3), "bytes_each_way": npart * 24}
 
 
-def compare_fp64(s, tol, max_iter, check_every):
-    """Solve the last substep's pressure system again with the mixed and the
-    plain FP64 PCG on the same right-hand side; time both with HIP events."""
+def _solve_timed(s, prec, tol, max_iter, check_every):
+    """One PCG solve of the system the solver's last substep built (same
+    right-hand side, flags and tiles), timed with HIP events on its stream."""
     import torch
     from gpufluid_amd import _lib
     sc = s._sc
-    sc.ensure_fp64()
+    d = s._descriptor()
+    it, rr, nrep = ctypes.c_int(), ctypes.c_double(), ctypes.c_int()
+    stream = torch.cuda.current_stream()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    a.record(stream)
+    _lib.call("S2.6.3", "gf_scene_pcg_sync", ctypes.byref(d), sc.n_active_host, prec, tol,
+              max_iter, check_every, -1, ctypes.byref(it), ctypes.byref(rr), ctypes.byref(nrep),
+              None, s._stream())
+    b.record(stream)
+    torch.cuda.synchronize()
+    return {"ms": round(a.elapsed_time(b), 3), "iters": it.value, "rel_residual": rr.value,
+            "replacements": nrep.value}
+
+
+def compare_fp64(s, tol, max_iter, check_every, own):
+    """The last substep's pressure system solved again:
+    * with the mixed PCG under the substep's own tolerance and cap: the solver
+      is deterministic, so this reproduces the substep's own solve (``own``:
+      its iterations and residual) exactly — the check that the re-solve sees
+      the same system;
+    * with the mixed and the plain FP64 PCG under the same cap (equal
+      iteration budget: the per-iteration cost ratio);
+    * with both run to the same relative residual with no practical cap (time
+      to an equal residual): 1e-6, and the substep's own tolerance."""
+    from gpufluid_amd import _lib
+    s._sc.ensure_fp64()
     res = {}
-    for name, prec in (("mixed", _lib.GF_PCG_MIXED), ("fp64", _lib.GF_PCG_FP64)):
-        d = s._descriptor()
-        it, rr, nrep = ctypes.c_int(), ctypes.c_double(), ctypes.c_int()
-        a = torch.cuda.Event(enable_timing=True)
-        b = torch.cuda.Event(enable_timing=True)
-        torch.cuda.synchronize()
-        a.record()
-        _lib.call("S2.6.3", "gf_scene_pcg_sync", ctypes.byref(d), sc.n_active_host, prec, tol,
-                  max_iter, check_every, -1, ctypes.byref(it), ctypes.byref(rr),
-                  ctypes.byref(nrep), None, s._stream())
-        b.record()
-        torch.cuda.synchronize()
-        res[name] = {"ms": round(a.elapsed_time(b), 3), "iters": it.value,
-                     "rel_residual": rr.value}
-    res["speedup"] = round(res["fp64"]["ms"] / res["mixed"]["ms"], 3)
+    mixed = _solve_timed(s, _lib.GF_PCG_MIXED, tol, max_iter, check_every)
+    res["same_system_resolve"] = {"mixed": mixed, "substep_iters": own[0],
+                                  "substep_rel_residual": own[1],
+                                  "identical": (mixed["iters"], mixed["rel_residual"]) == own}
+    fp64 = _solve_timed(s, _lib.GF_PCG_FP64, tol, max_iter, check_every)
+    res["equal_iteration_cap"] = {"cap": max_iter, "mixed": mixed, "fp64": fp64,
+                                  "speedup": round(fp64["ms"] / mixed["ms"], 3)}
+    eq = {}
+    for t in sorted({1e-6, tol}, reverse=True):
+        m = _solve_timed(s, _lib.GF_PCG_MIXED, t, 20000, check_every)
+        f = _solve_timed(s, _lib.GF_PCG_FP64, t, 20000, check_every)
+        eq[f"{t:g}"] = {"mixed": m, "fp64": f, "speedup": round(f["ms"] / m["ms"], 3)}
+    res["equal_residual"] = eq
     return res
 
 
