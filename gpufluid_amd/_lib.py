@@ -24,6 +24,7 @@ GF_NACT_DEVICE = -1  # n_active argument: read the active tile count on the devi
 GF_EINVAL, GF_ERANGE, GF_ESTATE = -1, -2, -3  # negative return codes of the C-ABI
 TILE = 8
 TILE_CELLS = 512
+SCENE_PAR = 16  # GF_SCENE_PAR: floats per scene in gf_scene_t.scene_par
 
 _lock = threading.Lock()
 _lib = None
@@ -55,6 +56,7 @@ class GfScene(ctypes.Structure):
         ("p32b", c_void_p), ("p64b", c_void_p), ("cg32", c_void_p), ("cg64", c_void_p),
         ("n_scenes", c_int32), ("tile_scene", c_void_p), ("tile_order", c_void_p),
         ("scene_par", c_void_p), ("scene_slot", c_void_p), ("pcg_part", c_void_p),
+        ("scene_pstart", c_void_p),
     ]
 
 

@@ -1,14 +1,14 @@
 """Many independent FLIP scenes of uneven grid sizes in one set of launches.
 
-The scenes are packed into one grid, in grid units (dx = 1): scene s with n_s
-cells per axis and cell size dx_s sits at a tile-aligned offset o_s, inside its
-own one-cell solid shell (the reference walls, solver3d.py:2475-2482), so no
-stencil, scatter or gather ever couples two scenes.  A position p and velocity
-v of scene s become o_s + p/dx_s and v/dx_s.  The projection is the same in
-these units (divergence scale rho*dx/dt and gradient scale dt/(rho*dx) with
-dx = 1 cancel as they do with dx_s); the two dx-dependent quantities travel in
-a per-scene table read through a per-tile scene id: gravity as g*dt/dx_s and
-the clamp box [o_s + 1.001, o_s + n_s - 1.001].
+The scenes are packed into one grid: scene s with n_s cells per axis and cell
+size dx_s sits at a tile-aligned cell offset o_s, inside its own one-cell
+solid shell (the reference walls, solver3d.py:2475-2482), so no stencil,
+scatter or gather ever couples two scenes.  Particles keep their scene's world
+units (its particle ids form one contiguous range), and the kernels take each
+scene's dx, offset, size, g*dt and clamp box from a per-scene table (through
+the tile's scene id, or the particle's id range when binning): a particle's
+cell, its P2G weights, its G2P samples and its clamp are computed exactly as
+for the scene run alone, and only integer indices are shifted by o_s.
 
 Every stage is then one launch for all scenes: the sort, P2G and G2P kernels
 cover all tiles, and the PCG runs segmented — compact tiles are numbered
@@ -122,16 +122,24 @@ class BatchedScenes:
         marker = np.full((nx, ny, nz), 2, dtype=np.int32)
         tnx, tny, tnz = nx // TILE, ny // TILE, nz // TILE
         tile_scene = np.full((tnx, tny, tnz), -1, dtype=np.int32)
-        par = np.zeros((S, 8), dtype=np.float32)
+        par = np.zeros((S, _lib.SCENE_PAR), dtype=np.float32)
         self.dx = np.array([1.0 / n for n in self.edges])
+        f32 = np.float32
         for i, (n, o) in enumerate(zip(self.edges, self.pack.offsets)):
             marker[o[0] + 1:o[0] + n - 1, o[1] + 1:o[1] + n - 1, o[2] + 1:o[2] + n - 1] = 0
             t0, t1 = o // TILE, (o + _ceil8(n)) // TILE
             tile_scene[t0[0]:t1[0], t0[1]:t1[1], t0[2]:t1[2]] = i
-            eps = np.float32(1.0) * np.float32(1.001)
-            par[i, 0] = np.float32(gravity * dt * n)       # g*dt in grid units
-            par[i, 1:4] = (o + eps).astype(np.float32)
-            par[i, 4:7] = (o + n - eps).astype(np.float32)
+            # the FP32 values FlipSolver3D(n, n, n, dx=1/n) hands its kernels:
+            # g*dt, the clamp box [1.001 dx, n dx - 1.001 dx], dx
+            dx32 = f32(1.0 / n)
+            eps = f32(dx32 * f32(1.001))
+            dom = f32(n * (1.0 / n))
+            par[i, 0] = f32(f32(gravity) * f32(dt))
+            par[i, 1:4] = eps
+            par[i, 4:7] = f32(dom - eps)
+            par[i, 7] = dx32
+            par[i, 8:11] = o
+            par[i, 11] = n
         flat = tile_scene.reshape(-1)
         order = np.concatenate([np.nonzero(flat == i)[0] for i in range(S)]
                                + [np.nonzero(flat < 0)[0]]).astype(np.int32)
@@ -139,15 +147,14 @@ class BatchedScenes:
         s._marker_host = marker
         s._wall_marker = marker.copy()
         s._f["marker"].copy_(torch.as_tensor(marker, device=s.device))
-        s._sc.set_scenes(flat, order, par)
-        self.scene_par = par
         counts = [len(sc[1]) for sc in scenes]
         self.start = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
-        pos = np.concatenate([np.asarray(sc[1], dtype=np.float64) * n + o
-                              for sc, n, o in zip(scenes, self.edges, self.pack.offsets)])
-        vel = np.concatenate([np.asarray(sc[2], dtype=np.float64) * n
-                              for sc, n in zip(scenes, self.edges)])
-        s.set_particles(pos.astype(np.float32), vel.astype(np.float32))
+        s._sc.set_scenes(flat, order, par, self.start)
+        self.scene_par = par
+        # particles stay in their scene's world units (scene i: ids start[i]..)
+        pos = np.concatenate([np.asarray(sc[1], dtype=np.float32).reshape(-1, 3) for sc in scenes])
+        vel = np.concatenate([np.asarray(sc[2], dtype=np.float32).reshape(-1, 3) for sc in scenes])
+        s.set_particles(pos, vel)
         self.n_particles = s.n_particles
         self.last_iters = None
         self.last_residual = None
@@ -269,11 +276,15 @@ class BatchedScenes:
 
     # -------------------------------------------------------------- results
     def scene_particles(self, i: int):
-        """World-unit (pos, vel) of scene i as numpy arrays."""
+        """(pos, vel) of scene i in its world units as FP64 numpy arrays."""
         s = self.s
         s._cache.clear()
         pos = s.pos.t[self.start[i]:self.start[i + 1]].double().cpu().numpy()
         vel = s.vel.t[self.start[i]:self.start[i + 1]].double().cpu().numpy()
-        n = self.edges[i]
-        o = self.pack.offsets[i]
-        return (pos - o) / n, vel / n
+        return pos, vel
+
+    def scene_marker(self, i: int):
+        """Scene i's block of the packed marker (its n^3 cells), numpy int32."""
+        n, o = self.edges[i], self.pack.offsets[i]
+        m = self.s._f["marker"][o[0]:o[0] + n, o[1]:o[1] + n, o[2]:o[2] + n]
+        return m.cpu().numpy()

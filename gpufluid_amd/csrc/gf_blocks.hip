@@ -1241,6 +1241,7 @@ extern "C" int gf_scene_attr_transfer(const gf_scene_t* s, float* attr, int chan
                                       int64_t* acc, float* grid, uint32_t* scratch,
                                       void* stream) {
     GF_REQUIRE(s, "gf_scene_attr_transfer: null scene");
+    GF_REQUIRE(s->n_scenes <= 1, "gf_scene_attr_transfer: not for packed scenes");
     GF_REQUIRE(channels >= 1 && channels <= 3, "gf_scene_attr_transfer: channels %d", channels);
     GF_REQUIRE(mode == GF_ATTR_COLOR || mode == GF_ATTR_SCALAR, "gf_scene_attr_transfer: mode %d",
                mode);

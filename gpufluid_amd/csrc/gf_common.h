@@ -48,6 +48,39 @@ __device__ __forceinline__ int bin_axis(float p, float dx, int n) {
     return clampi(c, 0, n - 1);
 }
 
+// Packed scenes: per-scene table of GF_SCENE_PAR floats (include/gpufluid_hip.h,
+// gf_scene_t.scene_par): [0] g*dt, [1..3] clamp lo, [4..6] clamp hi (world
+// units), [7] the scene's dx, [8..10] its cell offset in the packed grid,
+// [11] its cells per axis.
+constexpr int SCENE_PAR = GF_SCENE_PAR;
+
+// The grid a tile's particles live in: their own scene's dx, the scene's
+// cell offset in the packed grid and its size (one scene: the whole grid).
+// Positions stay in the scene's world units, so every per-particle quantity
+// (cell, weights, samples) is computed exactly as for the scene run alone
+// and only integer indices are shifted by the offset.
+struct Frame {
+    float dx;
+    int ox, oy, oz, nx, ny, nz;
+};
+
+__device__ __forceinline__ Frame tile_frame(const int32_t* tile_scene, const float* scene_par,
+                                            int t, float dx, int nx, int ny, int nz) {
+    Frame F;
+    if (!tile_scene) {
+        F.dx = dx;
+        F.ox = F.oy = F.oz = 0;
+        F.nx = nx; F.ny = ny; F.nz = nz;
+        return F;
+    }
+    const int sc = tile_scene[t];
+    const float* sp = scene_par + SCENE_PAR * (sc > 0 ? sc : 0);
+    F.dx = sp[7];
+    F.ox = (int)sp[8]; F.oy = (int)sp[9]; F.oz = (int)sp[10];
+    F.nx = F.ny = F.nz = (int)sp[11];
+    return F;
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll

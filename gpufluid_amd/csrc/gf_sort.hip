@@ -40,19 +40,42 @@ __device__ __forceinline__ void run_of(int key, int& lead, int& len) {
     len = first ? end - lane : 0;
 }
 
+// Packed scenes: a particle is binned in its own scene's grid (its dx and
+// size, from the id range it falls in) and the cell shifted by the scene's
+// offset, so the cell is the one the scene alone gives it.
 __global__ __launch_bounds__(GF_WG) void k_bin(int n, const float* __restrict__ pos,
                                                float dx, int nx, int ny, int nz,
                                                const int32_t* __restrict__ marker,
                                                int32_t* __restrict__ keys,
                                                int32_t* __restrict__ counts,
                                                int32_t* __restrict__ tile_occ, int tny,
-                                               int tnz) {
+                                               int tnz, const int32_t* __restrict__ pid,
+                                               const int32_t* __restrict__ scene_pstart,
+                                               const float* __restrict__ scene_par,
+                                               int n_scenes) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     float x = pos[3 * p + 0], y = pos[3 * p + 1], z = pos[3 * p + 2];
-    int ci = gf::bin_axis(x, dx, nx);
-    int cj = gf::bin_axis(y, dx, ny);
-    int ck = gf::bin_axis(z, dx, nz);
+    int ci, cj, ck;
+    if (scene_pstart) {
+        const int id = pid[p];
+        int lo = 0, hi = n_scenes - 1;  // the last scene whose first id is <= id
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (scene_pstart[mid] <= id) lo = mid;
+            else hi = mid - 1;
+        }
+        const float* sp = scene_par + gf::SCENE_PAR * lo;
+        const float sdx = sp[7];
+        const int ns = (int)sp[11];
+        ci = (int)sp[8] + gf::bin_axis(x, sdx, ns);
+        cj = (int)sp[9] + gf::bin_axis(y, sdx, ns);
+        ck = (int)sp[10] + gf::bin_axis(z, sdx, ns);
+    } else {
+        ci = gf::bin_axis(x, dx, nx);
+        cj = gf::bin_axis(y, dx, ny);
+        ck = gf::bin_axis(z, dx, nz);
+    }
     int key = (ci * ny + cj) * nz + ck;
     keys[p] = key;
     int lead, len;
@@ -303,10 +326,10 @@ int scene_check(const gf_scene_t* s, const char* who) {
         return set_error(GF_ESTATE, "%s: n=%d outside [0, cap=%d]", who, s->n, s->cap);
     if (!(s->dx > 0.0f)) return set_error(GF_EINVAL, "%s: dx must be > 0", who);
     if (s->n_scenes < 0) return set_error(GF_EINVAL, "%s: n_scenes %d", who, s->n_scenes);
-    if (s->n_scenes > 1 &&
-        (!s->tile_scene || !s->tile_order || !s->scene_par || !s->scene_slot || !s->pcg_part))
+    if (s->n_scenes > 1 && (!s->tile_scene || !s->tile_order || !s->scene_par ||
+                            !s->scene_slot || !s->pcg_part || !s->scene_pstart))
         return set_error(GF_EINVAL, "%s: packed scenes need tile_scene, tile_order, scene_par, "
-                                    "scene_slot and pcg_part", who);
+                                    "scene_slot, pcg_part and scene_pstart", who);
     return 0;
 }
 }  // namespace gf
@@ -334,7 +357,8 @@ extern "C" int gf_scene_sort_sync(gf_scene_t* s, int with_affine, int* n_active_
     if (s->n > 0)
         hipLaunchKernelGGL(k_bin, dim3(npb), dim3(GF_WG), 0, st, s->n, s->pos, s->dx, s->nx,
                            s->ny, s->nz, s->marker, s->keys, s->counts, s->tile_occ, s->tny,
-                           s->tnz);
+                           s->tnz, s->pid, s->n_scenes > 1 ? s->scene_pstart : nullptr,
+                           s->scene_par, s->n_scenes);
     GF_LAUNCH_CHECK("k_bin");
     rc = scan_exclusive(s->counts, C, s->cell_start, s->scan_part, st);
     if (rc) return rc;

@@ -260,12 +260,17 @@ __device__ __forceinline__ RowSet build_rows(const int32_t* __restrict__ cell_st
 // One component of the lane-per-cell P2G: accumulate the cell's stencil
 // over particles [p0, p1) in registers, then add it to the LDS region in the
 // flush phases r_lo..r_hi (phase = cell class; cls < 0 for empty cells).
+// Cells and the region origin come in packed-grid indices; the arithmetic
+// runs in the tile's scene frame F (its dx, its own cell indices and size).
 template <int COMP, bool APIC, class D = P2GDims<COMP>>
-__device__ __forceinline__ void p2g_component(const P2GParams& a, float* s_val, float* s_wt,
-                                              int ci, int cj, int ck, int p0, int p1, int cls,
-                                              int r_lo, int r_hi, int i0, int j0, int k0) {
-    const int nx = a.nx, ny = a.ny, nz = a.nz;
-    const float dx = a.dx;
+__device__ __forceinline__ void p2g_component(const P2GParams& a, const gf::Frame& F,
+                                              float* s_val, float* s_wt, int ci, int cj, int ck,
+                                              int p0, int p1, int cls, int r_lo, int r_hi,
+                                              int i0, int j0, int k0) {
+    const int nx = F.nx, ny = F.ny, nz = F.nz;
+    const float dx = F.dx;
+    ci -= F.ox; cj -= F.oy; ck -= F.oz;
+    i0 -= F.ox; j0 -= F.oy; k0 -= F.oz;
     CellAcc<COMP> A;
 #pragma unroll
     for (int q = 0; q < 18; ++q) {
@@ -386,14 +391,14 @@ __global__ __launch_bounds__(GF_WG) void k_p2g_tile(P2GParams a) {
         int r_lo = 0, r_hi = 0;
         while (r_lo < 26 && s_cls[r_lo + 1] <= round * GF_WG) ++r_lo;
         while (r_hi < 26 && s_cls[r_hi + 1] <= last) ++r_hi;
-        p2g_component<0, APIC>(a, s_val[0], s_wt[0], ci, cj, ck, p0, p1, cls, r_lo, r_hi, i0, j0, k0);
-        p2g_component<1, APIC>(a, s_val[1], s_wt[1], ci, cj, ck, p0, p1, cls, r_lo, r_hi, i0, j0, k0);
-        p2g_component<2, APIC>(a, s_val[2], s_wt[2], ci, cj, ck, p0, p1, cls, r_lo, r_hi, i0, j0, k0);
+        const gf::Frame F = gf::tile_frame(nullptr, nullptr, t, a.dx, nx, ny, nz);
+        p2g_component<0, APIC>(a, F, s_val[0], s_wt[0], ci, cj, ck, p0, p1, cls, r_lo, r_hi, i0, j0, k0);
+        p2g_component<1, APIC>(a, F, s_val[1], s_wt[1], ci, cj, ck, p0, p1, cls, r_lo, r_hi, i0, j0, k0);
+        p2g_component<2, APIC>(a, F, s_val[2], s_wt[2], ci, cj, ck, p0, p1, cls, r_lo, r_hi, i0, j0, k0);
     }
 
     // Finalise the region: normalize (121-142), gravity (149-156), BC (758-799).
-    // Packed scenes carry g*dt per scene (grid units differ by scene).
-    const float gdt = a.tile_scene ? a.scene_par[8 * max(a.tile_scene[t], 0)] : a.g * a.dt;
+    const float gdt = a.g * a.dt;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const int DI = c == 0 ? 9 : 8, DJ = c == 1 ? 9 : 8, DK = c == 2 ? 9 : 8;
@@ -511,6 +516,7 @@ __global__ __launch_bounds__(GF_WG) void k_p2g_own(P2GParams a, float* __restric
     const RowSet rs = build_rows(a.cell_start, ny, nz, i0, i0 + ni, j0, j0 + nj, k0, k0 + nk,
                                  s_start, s_pref, s_wave);
     if (threadIdx.x == 0 && comp == 0) has[t] = rs.total > 0 ? 1 : 0;
+    const gf::Frame F = gf::tile_frame(a.tile_scene, a.scene_par, t, a.dx, nx, ny, nz);
     if (rs.total == 0) return;
     for (int i = threadIdx.x; i < OWN_REGION; i += GF_WG) {
         s_val[i] = 0.0f;
@@ -541,13 +547,13 @@ __global__ __launch_bounds__(GF_WG) void k_p2g_own(P2GParams a, float* __restric
         while (r_lo < 26 && s_cls[r_lo + 1] <= round * GF_WG) ++r_lo;
         while (r_hi < 26 && s_cls[r_hi + 1] <= last) ++r_hi;
         if (comp == 0)
-            p2g_component<0, APIC, OwnDims<0>>(a, s_val, s_wt, ci, cj, ck, p0, p1, cls, r_lo,
+            p2g_component<0, APIC, OwnDims<0>>(a, F, s_val, s_wt, ci, cj, ck, p0, p1, cls, r_lo,
                                                r_hi, i0, j0 - 1, k0 - 1);
         else if (comp == 1)
-            p2g_component<1, APIC, OwnDims<1>>(a, s_val, s_wt, ci, cj, ck, p0, p1, cls, r_lo,
+            p2g_component<1, APIC, OwnDims<1>>(a, F, s_val, s_wt, ci, cj, ck, p0, p1, cls, r_lo,
                                                r_hi, i0 - 1, j0, k0 - 1);
         else
-            p2g_component<2, APIC, OwnDims<2>>(a, s_val, s_wt, ci, cj, ck, p0, p1, cls, r_lo,
+            p2g_component<2, APIC, OwnDims<2>>(a, F, s_val, s_wt, ci, cj, ck, p0, p1, cls, r_lo,
                                                r_hi, i0 - 1, j0 - 1, k0);
     }
     float* st = stage + (size_t)t * STAGE_PER_TILE + (size_t)comp * 2 * OWN_REGION;
@@ -590,7 +596,9 @@ __global__ __launch_bounds__(GF_WG) void k_p2g_finish(P2GParams a,
     bool any_has = false;
     for (int q = 0; q < 27; ++q) any_has |= s_has[q] != 0;
 
-    const float gdt = a.tile_scene ? a.scene_par[8 * max(a.tile_scene[t], 0)] : a.g * a.dt;
+    const float gdt =
+        a.tile_scene ? a.scene_par[gf::SCENE_PAR * max(a.tile_scene[t], 0)] : a.g * a.dt;
+    const gf::Frame F = gf::tile_frame(a.tile_scene, a.scene_par, t, a.dx, nx, ny, nz);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const int DI = c == 0 ? 9 : 8, DJ = c == 1 ? 9 : 8, DK = c == 2 ? 9 : 8;
@@ -671,7 +679,7 @@ __global__ __launch_bounds__(GF_WG) void k_p2g_finish(P2GParams a,
 
     const int slot = a.tile_map[t];
     if (slot < 0) return;
-    const float scale = a.rho * a.dx / a.dt;
+    const float scale = a.rho * F.dx / a.dt;
     const int lj = (threadIdx.x >> 3) & 7, lk = threadIdx.x & 7;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
@@ -861,8 +869,11 @@ __global__ __launch_bounds__(GF_WG) void k_g2p_tile(G2PParams a) {
     }
     __syncthreads();
 
-    const float scale = a.dt / (a.rho * a.dx);
-    const float gdt = a.tile_scene ? a.scene_par[8 * max(a.tile_scene[t], 0)] : a.gdt;
+    // the tile's scene frame: dx, cell offset and size (one scene: the grid)
+    const gf::Frame F = gf::tile_frame(a.tile_scene, a.scene_par, t, a.dx, nx, ny, nz);
+    const float scale = a.dt / (a.rho * F.dx);
+    const float gdt =
+        a.tile_scene ? a.scene_par[gf::SCENE_PAR * max(a.tile_scene[t], 0)] : a.gdt;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const int DI = c == 0 ? 9 : 10, DJ = c == 1 ? 9 : 10, DK = c == 2 ? 9 : 10;
@@ -927,13 +938,17 @@ __global__ __launch_bounds__(GF_WG) void k_g2p_tile(G2PParams a) {
     }
     __syncthreads();
 
-    const float dx = a.dx, dt = a.dt;
+    const float dx = F.dx, dt = a.dt;
     const float eps = dx * 1.001f;
+    // samples in the scene's own cell indices: the region origin and the
+    // face-array extents of the scene
+    const int li0 = i0 - F.ox, lj0 = j0 - F.oy, lk0 = k0 - F.oz;
+    const int snx = F.nx, sny = F.ny, snz = F.nz;
     // clamp box: [1.001 dx, dom - 1.001 dx], or the tile's scene box when packed
     float lox = eps, loy = eps, loz = eps;
     float hix = a.domx - eps, hiy = a.domy - eps, hiz = a.domz - eps;
     if (a.tile_scene) {
-        const float* sp = a.scene_par + 8 * max(a.tile_scene[t], 0);
+        const float* sp = a.scene_par + gf::SCENE_PAR * max(a.tile_scene[t], 0);
         lox = sp[1]; loy = sp[2]; loz = sp[3];
         hix = sp[4]; hiy = sp[5]; hiz = sp[6];
     }
@@ -944,12 +959,12 @@ __global__ __launch_bounds__(GF_WG) void k_g2p_tile(G2PParams a) {
         const float lx = px / dx, ly = py / dx, lz = pz / dx;
         float nu, nv, nw;
         if (!APIC) {
-            const float pic_u = sample_region<0>(s_fin[0], lx, ly - 0.5f, lz - 0.5f, nx + 1, ny, nz, i0, j0, k0);
-            const float old_u = sample_region<0>(s_sav[0], lx, ly - 0.5f, lz - 0.5f, nx + 1, ny, nz, i0, j0, k0);
-            const float pic_v = sample_region<1>(s_fin[1], lx - 0.5f, ly, lz - 0.5f, nx, ny + 1, nz, i0, j0, k0);
-            const float old_v = sample_region<1>(s_sav[1], lx - 0.5f, ly, lz - 0.5f, nx, ny + 1, nz, i0, j0, k0);
-            const float pic_w = sample_region<2>(s_fin[2], lx - 0.5f, ly - 0.5f, lz, nx, ny, nz + 1, i0, j0, k0);
-            const float old_w = sample_region<2>(s_sav[2], lx - 0.5f, ly - 0.5f, lz, nx, ny, nz + 1, i0, j0, k0);
+            const float pic_u = sample_region<0>(s_fin[0], lx, ly - 0.5f, lz - 0.5f, snx + 1, sny, snz, li0, lj0, lk0);
+            const float old_u = sample_region<0>(s_sav[0], lx, ly - 0.5f, lz - 0.5f, snx + 1, sny, snz, li0, lj0, lk0);
+            const float pic_v = sample_region<1>(s_fin[1], lx - 0.5f, ly, lz - 0.5f, snx, sny + 1, snz, li0, lj0, lk0);
+            const float old_v = sample_region<1>(s_sav[1], lx - 0.5f, ly, lz - 0.5f, snx, sny + 1, snz, li0, lj0, lk0);
+            const float pic_w = sample_region<2>(s_fin[2], lx - 0.5f, ly - 0.5f, lz, snx, sny, snz + 1, li0, lj0, lk0);
+            const float old_w = sample_region<2>(s_sav[2], lx - 0.5f, ly - 0.5f, lz, snx, sny, snz + 1, li0, lj0, lk0);
             const float ovx = a.vel[3 * pidx + 0], ovy = a.vel[3 * pidx + 1], ovz = a.vel[3 * pidx + 2];
             const float flip_u = ovx + (pic_u - old_u);
             const float flip_v = ovy + (pic_v - old_v);
@@ -961,9 +976,9 @@ __global__ __launch_bounds__(GF_WG) void k_g2p_tile(G2PParams a) {
         } else {
             const float inv_dx2_3 = 3.0f / (dx * dx);
             float cm[9];
-            apic_gather<0>(s_fin[0], px, py, pz, lx, ly, lz, nx, ny, nz, dx, inv_dx2_3, i0, j0, k0, nu, cm + 0);
-            apic_gather<1>(s_fin[1], px, py, pz, lx, ly, lz, nx, ny, nz, dx, inv_dx2_3, i0, j0, k0, nv, cm + 3);
-            apic_gather<2>(s_fin[2], px, py, pz, lx, ly, lz, nx, ny, nz, dx, inv_dx2_3, i0, j0, k0, nw, cm + 6);
+            apic_gather<0>(s_fin[0], px, py, pz, lx, ly, lz, snx, sny, snz, dx, inv_dx2_3, li0, lj0, lk0, nu, cm + 0);
+            apic_gather<1>(s_fin[1], px, py, pz, lx, ly, lz, snx, sny, snz, dx, inv_dx2_3, li0, lj0, lk0, nv, cm + 3);
+            apic_gather<2>(s_fin[2], px, py, pz, lx, ly, lz, snx, sny, snz, dx, inv_dx2_3, li0, lj0, lk0, nw, cm + 6);
 #pragma unroll
             for (int c = 0; c < 9; ++c) a.C[9 * pidx + c] = cm[c];
         }
@@ -1105,6 +1120,8 @@ extern "C" int gf_scene_p2g(const gf_scene_t* s, int transfer_mode, float gravit
                             float rho, void* stream) {
     int rc = gf::scene_check(s, "gf_scene_p2g");
     if (rc) return rc;
+    GF_REQUIRE(s->n_scenes <= 1, "gf_scene_p2g: not for packed scenes (their particles are in "
+                                 "per-scene units; use the staged P2G and gf_scene_g2p)");
     GF_REQUIRE(transfer_mode == GF_TRANSFER_FLIP || transfer_mode == GF_TRANSFER_APIC,
                "gf_scene_p2g: transfer_mode %d", transfer_mode);
     GF_REQUIRE(s->u && s->v && s->w && s->us && s->vs && s->ws && s->b && s->marker &&
@@ -1165,6 +1182,8 @@ extern "C" int gf_scene_p2g_raw(const gf_scene_t* s, int transfer_mode, float* u
                                 float* ww, void* stream) {
     int rc = gf::scene_check(s, "gf_scene_p2g_raw");
     if (rc) return rc;
+    GF_REQUIRE(s->n_scenes <= 1, "gf_scene_p2g_raw: not for packed scenes (their particles are in "
+                                 "per-scene units; use the staged P2G and gf_scene_g2p)");
     GF_REQUIRE(transfer_mode == GF_TRANSFER_FLIP || transfer_mode == GF_TRANSFER_APIC,
                "gf_scene_p2g_raw: transfer_mode %d", transfer_mode);
     GF_REQUIRE(s->u && s->v && s->w && uw && vw && ww && s->cell_start,
@@ -1199,6 +1218,8 @@ extern "C" int gf_scene_g2p_pre(const gf_scene_t* s, const float* pressure, cons
                                 float dt, float rho, float flip_blend, void* stream) {
     int rc = gf::scene_check(s, "gf_scene_g2p_pre");
     if (rc) return rc;
+    GF_REQUIRE(s->n_scenes <= 1, "gf_scene_g2p_pre: not for packed scenes (their particles are in "
+                                 "per-scene units; use the staged P2G and gf_scene_g2p)");
     rc = g2p_check(s, pressure, transfer_mode, rho, "gf_scene_g2p_pre");
     if (rc) return rc;
     GF_REQUIRE(pre_u && pre_v && pre_w, "gf_scene_g2p_pre: pre-projection faces not set");
@@ -1212,6 +1233,8 @@ extern "C" int gf_scene_g2p_pre(const gf_scene_t* s, const float* pressure, cons
 extern "C" int gf_scene_divergence(const gf_scene_t* s, float dt, float rho, void* stream) {
     int rc = gf::scene_check(s, "gf_scene_divergence");
     if (rc) return rc;
+    GF_REQUIRE(s->n_scenes <= 1, "gf_scene_divergence: not for packed scenes (their particles are in "
+                                 "per-scene units; use the staged P2G and gf_scene_g2p)");
     GF_REQUIRE(s->u && s->v && s->w && s->marker && s->tile_map && s->b,
                "gf_scene_divergence: grid buffers not set");
     GF_REQUIRE(dt != 0.0f, "gf_scene_divergence: dt must be nonzero");

@@ -15,6 +15,8 @@ import torch
 
 from . import _lib
 
+SCENE_PAR = _lib.SCENE_PAR
+
 
 class SceneBuffers:
     def __init__(self, nx: int, ny: int, nz: int, dx: float, dom, device: torch.device):
@@ -54,6 +56,7 @@ class SceneBuffers:
         # packed scenes (gpufluid_amd.batch): per-tile scene ids and scene tables
         self.n_scenes = 0
         self.tile_scene = self.tile_order = self.scene_par = self.scene_slot = None
+        self.scene_pstart = None
         self.cap = 0
         self.bufs = [None, None]     # ping-pong particle storage, cap rows each
         self.cur = 0
@@ -147,17 +150,23 @@ class SceneBuffers:
             s.n_scenes = self.n_scenes
             s.tile_scene, s.tile_order = P(self.tile_scene), P(self.tile_order)
             s.scene_par, s.scene_slot = P(self.scene_par), P(self.scene_slot)
+            s.scene_pstart = P(self.scene_pstart)
         return s
 
-    def set_scenes(self, tile_scene, tile_order, scene_par):
-        """Describe packed scenes: per-tile scene id, scene-major tile order
-        and the per-scene table (g*dt, clamp lo x,y,z, clamp hi x,y,z, 0)."""
+    def set_scenes(self, tile_scene, tile_order, scene_par, scene_pstart):
+        """Describe packed scenes: per-tile scene id, scene-major tile order,
+        the per-scene table (SCENE_PAR floats per scene: g*dt, clamp lo
+        x,y,z, clamp hi x,y,z, dx, cell offset x,y,z, cells per axis) and the
+        first particle id of each scene (n_scenes + 1 entries)."""
         dev = self.device
         n = int(scene_par.shape[0])
         self.n_scenes = n
         self.tile_scene = torch.as_tensor(tile_scene, dtype=torch.int32, device=dev).contiguous()
         self.tile_order = torch.as_tensor(tile_order, dtype=torch.int32, device=dev).contiguous()
-        self.scene_par = torch.as_tensor(scene_par, dtype=torch.float32, device=dev).reshape(n, 8).contiguous()
+        self.scene_par = torch.as_tensor(scene_par, dtype=torch.float32,
+                                         device=dev).reshape(n, SCENE_PAR).contiguous()
+        self.scene_pstart = torch.as_tensor(scene_pstart, dtype=torch.int32,
+                                            device=dev).reshape(n + 1).contiguous()
         self.scene_slot = torch.zeros(n + 1, dtype=torch.int32, device=dev)
         self.pcg_part = torch.zeros(self.pcg_part_base + n, dtype=torch.int32, device=dev)
         self.scalars = torch.zeros(48 * n + 64, dtype=torch.float64, device=dev)

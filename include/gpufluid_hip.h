@@ -133,14 +133,26 @@ typedef struct gf_scene {
     float* cg32; double* cg64;   /* 3 more PCG work vectors, 3*(ntiles*512) entries each */
     /* batching: several independent scenes packed into this grid, each inside
      * its own solid shell at a tile-aligned offset (n_scenes <= 1: one scene,
-     * the fields below unused).  Positions are then in grid units (dx = 1). */
+     * the fields below unused).  Particles keep their own scene's world
+     * units: scene s's particles are the ids [scene_pstart[s],
+     * scene_pstart[s+1]), and every per-particle quantity (cell, weights,
+     * samples, clamp) is computed with that scene's dx, so each scene's
+     * binning and transfers are those of the scene run alone, shifted by its
+     * integer cell offset.  Descriptor dx is unused then. */
     int32_t n_scenes;
     const int32_t* tile_scene;   /* (ntiles) scene of each tile, -1 outside every scene */
     const int32_t* tile_order;   /* (ntiles) tile ids in scene-major order (compaction order) */
-    const float* scene_par;      /* (n_scenes*8) per scene: g*dt, clamp lo x,y,z, clamp hi x,y,z, 0 */
+    const float* scene_par;      /* (n_scenes*GF_SCENE_PAR) per scene, see below */
     int32_t* scene_slot;         /* (n_scenes+1) first compact slot of each scene (sort output) */
     int32_t* pcg_part;           /* (gf_scene_sizes.pcg_part + 2*n_scenes) PCG workgroup partition */
+    const int32_t* scene_pstart; /* (n_scenes+1) first particle id of each scene */
 } gf_scene_t;
+
+/* Entries of scene_par per scene: [0] g*dt, [1..3] clamp lo x,y,z, [4..6]
+ * clamp hi x,y,z (world units: 1.001 dx and n dx - 1.001 dx, in FP32 as the
+ * single-scene kernels form them), [7] dx (FP32), [8..10] cell offset of the
+ * scene in the packed grid, [11] cells per axis, [12..15] unused. */
+#define GF_SCENE_PAR 16
 
 typedef struct gf_scene_size_info {
     int64_t cells;        /* C = nx*ny*nz */

@@ -23,7 +23,9 @@
       GPU's own grid fields,
     - particle count and the clamp box.
 * C5 (256 scenes of edge 24..96): a stratified sample of 16 scenes, every edge
-  size including 96, packed against the same scenes run alone.
+  size including 96, packed against the same scenes run alone: the first
+  substep's marker and sort order bit for bit, the particles bit for bit where
+  the first solve did no iteration, then agreement to the PCG's rounding.
 
 Tolerances: integer work bit-exact; FP32 fields 1e-5 relative to the FP64
 oracle (max-abs convention).
@@ -36,6 +38,7 @@ import torch
 
 import gfutil as gu
 from oracle import flip_oracle as fo
+from test_gpu_batch import check_against_alone, packed_first_substep
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-5
@@ -280,7 +283,7 @@ def test_c3_full_size_second_substep():
 # --------------------------------------------------------------------- C5
 @pytest.mark.slow
 def test_c5_stratified_sample_packed_matches_alone():
-    from gpufluid_amd import FlipSolver3D, seeding
+    from gpufluid_amd import seeding
     from gpufluid_amd.batch import BatchedScenes
     scenes = seeding.config_c5()
     edges = np.array([sc.shape[0] for sc in scenes])
@@ -296,27 +299,12 @@ def test_c5_stratified_sample_packed_matches_alone():
     b = BatchedScenes([(sc.shape[0], sc.pos, sc.vel) for sc in sample], dt=dt, device="cuda:0")
     substeps = 3
     its = []
-    for _ in range(substeps):
+    first = None
+    for k in range(substeps):
         b.step(pressure_iters=500, pressure_tol=1e-8)
         its.append(b.last_iters.copy())
         assert (b.last_residual < 1e-8).all()
-    for i, sc in enumerate(sample):
-        n = sc.shape[0]
-        s = FlipSolver3D(n, n, n, dx=sc.dx, device="cuda:0")
-        s.set_particles(sc.pos, sc.vel)
-        ia = []
-        for _ in range(substeps):
-            s.step(dt, pressure_iters=500, pressure_solver="pcg", pressure_tol=1e-8)
-            ia.append(s.last_pressure_iters)
-        pa, va = s.get_particles()
-        pb, vb = b.scene_particles(i)
-        ep = np.abs(pb - pa).max()
-        ev = np.abs(vb - va).max() / max(np.abs(va).max(), 1e-6)
-        _log(f"C5 scene {pick[i]} (n={n}, {sc.n_particles} particles): |dx| {ep:.2e}, "
-             f"|dv|/|v| {ev:.2e}, iters alone {ia} packed {[int(t[i]) for t in its]}")
-        assert ep < 2e-5 and ev < 2e-4
-        # packed positions are held in grid units, so each packed system is
-        # the scene's own up to FP32 rounding: the counts agree to a few
-        # percent, not exactly
-        for k in range(substeps):
-            assert abs(int(its[k][i]) - ia[k]) <= max(8, 0.05 * ia[k])
+        if k == 0:
+            first = packed_first_substep(b)
+    check_against_alone(b, [(sc.shape[0], sc.pos, sc.vel) for sc in sample], its, first,
+                        substeps, "configs C5")
