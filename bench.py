@@ -555,7 +555,7 @@ def adjoint_leg():
     |centre of mass - (0.7,0.3,0.5)|^2.  Reports forward and reverse wall time
     (the reverse includes the segment recomputation).  The gradient itself is
     checked against FP64 references by tests/test_gpu_adjoint.py (16^3) and
-    tests/adjoint_full_check.py (this workload)."""
+    tests/test_gpu_adjoint_c4.py (this workload)."""
     import numpy as np
     import torch
     from gpufluid_amd import FlipSolver3D, seeding
@@ -590,7 +590,7 @@ def adjoint_leg():
             "adjoint_pcg_iters_mean": float(np.mean(roll.adjoint_iters)),
             "grad_norm": float(gvel.double().norm()),
             "gradient_checks": "tests/test_gpu_adjoint.py (16^3, FP64 reference and central "
-                               "differences); tests/adjoint_full_check.py (this workload, FP64)"}
+                               "differences); tests/test_gpu_adjoint_c4.py (this workload, FP64)"}
 
 
 def batch_leg(substeps=20, alone_substeps=2, tol=1e-8, max_iter=500):
