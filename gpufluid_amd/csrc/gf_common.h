@@ -73,11 +73,15 @@ __device__ __forceinline__ Frame tile_frame(const int32_t* tile_scene, const flo
         F.nx = nx; F.ny = ny; F.nz = nz;
         return F;
     }
-    const int sc = tile_scene[t];
+    // t is uniform over the workgroup: readfirstlane keeps the frame in
+    // scalar registers (otherwise it costs the P2G kernels a wave of occupancy)
+    const int sc = __builtin_amdgcn_readfirstlane(tile_scene[t]);
     const float* sp = scene_par + SCENE_PAR * (sc > 0 ? sc : 0);
-    F.dx = sp[7];
-    F.ox = (int)sp[8]; F.oy = (int)sp[9]; F.oz = (int)sp[10];
-    F.nx = F.ny = F.nz = (int)sp[11];
+    F.dx = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(sp[7])));
+    F.ox = __builtin_amdgcn_readfirstlane((int)sp[8]);
+    F.oy = __builtin_amdgcn_readfirstlane((int)sp[9]);
+    F.oz = __builtin_amdgcn_readfirstlane((int)sp[10]);
+    F.nx = F.ny = F.nz = __builtin_amdgcn_readfirstlane((int)sp[11]);
     return F;
 }
 

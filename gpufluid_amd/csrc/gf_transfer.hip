@@ -485,9 +485,12 @@ __global__ __launch_bounds__(GF_WG) void k_p2g_tile(P2GParams a) {
 // face it finalises, the staged partials of the tiles whose regions hold the
 // face (at most two per axis) in a fixed order, then normalises, adds gravity,
 // applies the BC and forms the divergence as k_p2g_tile does.
-template <bool APIC>
-__global__ __launch_bounds__(GF_WG) void k_p2g_own(P2GParams a, float* __restrict__ stage,
-                                                   int32_t* __restrict__ has) {
+// PACKED: tiles read their scene frame from the packed-scene table and the
+// cells go to p2g_component in the scene's own indices; the single-scene
+// instance folds the frame to constants.
+template <bool APIC, bool PACKED>
+__device__ __forceinline__ void p2g_own_body(const P2GParams& a, float* __restrict__ stage,
+                                             int32_t* __restrict__ has) {
     // one velocity component per workgroup: three workgroups share a tile,
     // each with a third of the LDS and of the work, consecutive in the
     // XCD-aware order so that they read the tile's particles through one L2
@@ -516,8 +519,14 @@ __global__ __launch_bounds__(GF_WG) void k_p2g_own(P2GParams a, float* __restric
     const RowSet rs = build_rows(a.cell_start, ny, nz, i0, i0 + ni, j0, j0 + nj, k0, k0 + nk,
                                  s_start, s_pref, s_wave);
     if (threadIdx.x == 0 && comp == 0) has[t] = rs.total > 0 ? 1 : 0;
-    const gf::Frame F = gf::tile_frame(a.tile_scene, a.scene_par, t, a.dx, nx, ny, nz);
+    gf::Frame F =
+        gf::tile_frame(PACKED ? a.tile_scene : nullptr, a.scene_par, t, a.dx, nx, ny, nz);
     if (rs.total == 0) return;
+    // scene-local tile origin; the packed-grid offset only enters the
+    // cell_start key
+    const int ox = F.ox, oy = F.oy, oz = F.oz;
+    const int li0 = i0 - ox, lj0 = j0 - oy, lk0 = k0 - oz;
+    F.ox = F.oy = F.oz = 0;
     for (int i = threadIdx.x; i < OWN_REGION; i += GF_WG) {
         s_val[i] = 0.0f;
         s_wt[i] = 0.0f;
@@ -534,10 +543,10 @@ __global__ __launch_bounds__(GF_WG) void k_p2g_own(P2GParams a, float* __restric
             const int ri = r / 9, rj = (r / 3) % 3, rk = r % 3;
             const int mj = max(0, (nj - rj + 2) / 3), mk = max(0, (nk - rk + 2) / 3);
             const int loc = c - s_cls[r];
-            ck = k0 + rk + 3 * (loc % mk);
-            cj = j0 + rj + 3 * ((loc / mk) % mj);
-            ci = i0 + ri + 3 * (loc / (mk * mj));
-            const int key = (ci * ny + cj) * nz + ck;
+            ck = lk0 + rk + 3 * (loc % mk);
+            cj = lj0 + rj + 3 * ((loc / mk) % mj);
+            ci = li0 + ri + 3 * (loc / (mk * mj));
+            const int key = ((ci + ox) * ny + cj + oy) * nz + ck + oz;
             p0 = a.cell_start[key];
             p1 = a.cell_start[key + 1];
             if (p0 < p1) cls = r;
@@ -548,19 +557,33 @@ __global__ __launch_bounds__(GF_WG) void k_p2g_own(P2GParams a, float* __restric
         while (r_hi < 26 && s_cls[r_hi + 1] <= last) ++r_hi;
         if (comp == 0)
             p2g_component<0, APIC, OwnDims<0>>(a, F, s_val, s_wt, ci, cj, ck, p0, p1, cls, r_lo,
-                                               r_hi, i0, j0 - 1, k0 - 1);
+                                               r_hi, li0, lj0 - 1, lk0 - 1);
         else if (comp == 1)
             p2g_component<1, APIC, OwnDims<1>>(a, F, s_val, s_wt, ci, cj, ck, p0, p1, cls, r_lo,
-                                               r_hi, i0 - 1, j0, k0 - 1);
+                                               r_hi, li0 - 1, lj0, lk0 - 1);
         else
             p2g_component<2, APIC, OwnDims<2>>(a, F, s_val, s_wt, ci, cj, ck, p0, p1, cls, r_lo,
-                                               r_hi, i0 - 1, j0 - 1, k0);
+                                               r_hi, li0 - 1, lj0 - 1, lk0);
     }
     float* st = stage + (size_t)t * STAGE_PER_TILE + (size_t)comp * 2 * OWN_REGION;
     for (int e = threadIdx.x; e < OWN_REGION; e += GF_WG) {
         st[e] = s_val[e];
         st[OWN_REGION + e] = s_wt[e];
     }
+}
+
+// FLIP fits 96 VGPRs (5 waves per SIMD) without spills in both instances;
+// APIC carries the affine rows and runs at 4.
+template <bool PACKED>
+__global__ __launch_bounds__(GF_WG) __attribute__((amdgpu_waves_per_eu(5, 8))) void
+k_p2g_own_flip(P2GParams a, float* __restrict__ stage, int32_t* __restrict__ has) {
+    p2g_own_body<false, PACKED>(a, stage, has);
+}
+
+template <bool PACKED>
+__global__ __launch_bounds__(GF_WG) void k_p2g_own_apic(P2GParams a, float* __restrict__ stage,
+                                                        int32_t* __restrict__ has) {
+    p2g_own_body<true, PACKED>(a, stage, has);
 }
 
 __global__ __launch_bounds__(GF_WG) void k_p2g_finish(P2GParams a,
@@ -1169,10 +1192,10 @@ extern "C" int gf_scene_p2g_staged(const gf_scene_t* s, int transfer_mode, float
     P2GParams a = p2g_params(s, gravity, dt, rho);
     const int ntiles = s->tnx * s->tny * s->tnz;
     hipStream_t st = (hipStream_t)stream;
-    if (transfer_mode == GF_TRANSFER_APIC)
-        hipLaunchKernelGGL(k_p2g_own<true>, dim3(3 * ntiles), dim3(GF_WG), 0, st, a, stage, has);
-    else
-        hipLaunchKernelGGL(k_p2g_own<false>, dim3(3 * ntiles), dim3(GF_WG), 0, st, a, stage, has);
+    const bool apic = transfer_mode == GF_TRANSFER_APIC, packed = a.tile_scene != nullptr;
+    auto own = apic ? (packed ? k_p2g_own_apic<true> : k_p2g_own_apic<false>)
+                    : (packed ? k_p2g_own_flip<true> : k_p2g_own_flip<false>);
+    hipLaunchKernelGGL(own, dim3(3 * ntiles), dim3(GF_WG), 0, st, a, stage, has);
     hipLaunchKernelGGL(k_p2g_finish, dim3(ntiles), dim3(GF_WG), 0, st, a, stage, has);
     GF_LAUNCH_CHECK("k_p2g_own/k_p2g_finish");
     return 0;
