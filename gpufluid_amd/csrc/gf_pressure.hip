@@ -163,10 +163,6 @@ struct NbIdx {
     uint32_t ok;  // bit d set: neighbour d (order -x,+x,-y,+y,-z,+z) exists
 };
 
-// The table row is read through the constant address space: t is
-// wave-uniform and the table is never written by these kernels, so the row
-// comes in with scalar loads (measured 1-4 us less per launch at 13 K tiles
-// than per-lane loads).
 typedef const __attribute__((address_space(4))) int32_t gf_const_i32;
 
 __device__ __forceinline__ NbIdx nb_index(const int32_t* __restrict__ nbr, int t, size_t base,
@@ -225,57 +221,88 @@ __device__ __forceinline__ void load_nb4(const T* __restrict__ v, const NbIdx& n
 // Only the values across a face of the wave's block are loaded: the -x plane
 // below the wave's first plane (previous tile or the other wave's half), the
 // +x plane above its last, the -y row / +y row of the neighbouring tile, and
-// the -z / +z cell of the neighbouring tile.  Every lane issues each face
-// load, at an address that depends on (lj, kh) only for x and on (li, kh)
-// only for y, so a wave-wide load touches a few cache lines instead of the
-// full 1 KB of a quad load, and no load sits behind a branch.
-struct FaceIdx {
-    size_t xm, xp, ym, yp, zm, zp;  // element offsets of the face values
-    uint32_t ok;                    // bit d: face neighbour d exists (-x,+x,-y,+y,-z,+z)
+// the -z / +z cell of the neighbouring tile.  For each direction the tile
+// that holds the face values is the same for the whole wave and the lane's
+// offset inside it does not change from tile to tile, so a face load is a
+// scalar base (per tile) plus a 32-bit lane offset fixed for the launch: no
+// per-lane address arithmetic in the loop.  An absent neighbour reads a tile
+// of zeros.  The kernels are VALU-bound at these sizes (measured: ~1500 VALU
+// instructions per wave against ~45% of wave time waiting on memory), which
+// is what this layout is for.
+struct FaceOff {
+    uint32_t own;                     // the quad's own offset (Q.l0)
+    uint32_t xm, xp, ym, yp, zm, zp;  // element offsets inside the holding tile
 };
 
-__device__ __forceinline__ FaceIdx face_index(const int32_t* __restrict__ nbr, int t, size_t base,
-                                              const Quad& Q) {
-    gf_const_i32* row = (gf_const_i32*)(nbr + (size_t)t * 6);
-    const int nxm = row[0], nxp = row[1], nym = row[2], nyp = row[3], nzm = row[4], nzp = row[5];
-    const int half = Q.li >> 2;                 // wave holds planes 4*half .. 4*half+3
-    const int pos = Q.lj * 8 + Q.kh * 4;        // offset within a plane
-    const size_t own = base + Q.l0;
-    FaceIdx f;
+__device__ __forceinline__ FaceOff face_off(const Quad& Q) {
+    const int half = Q.li >> 2;  // wave holds planes 4*half .. 4*half+3
+    const uint32_t pos = Q.lj * 8 + Q.kh * 4;
+    FaceOff o;
+    o.own = Q.l0;
     // -x of plane 4*half: plane 3 of this tile (half 1) or plane 7 of the -x tile
-    f.xm = half ? base + 3 * 64 + pos : (nxm >= 0 ? (size_t)nxm * 512 + 7 * 64 + pos : own);
+    o.xm = (half ? 3 * 64 : 7 * 64) + pos;
     // +x of plane 4*half+3: plane 4 of this tile (half 0) or plane 0 of the +x tile
-    f.xp = half ? (nxp >= 0 ? (size_t)nxp * 512 + pos : own) : base + 4 * 64 + pos;
-    f.ym = nym >= 0 ? (size_t)nym * 512 + (Q.li * 8 + 7) * 8 + Q.kh * 4 : own;
-    f.yp = nyp >= 0 ? (size_t)nyp * 512 + (Q.li * 8) * 8 + Q.kh * 4 : own;
-    f.zm = nzm >= 0 ? (size_t)nzm * 512 + (Q.li * 8 + Q.lj) * 8 + 7 : own;
-    f.zp = nzp >= 0 ? (size_t)nzp * 512 + (Q.li * 8 + Q.lj) * 8 : own;
-    f.ok = (uint32_t)(half || nxm >= 0) | (uint32_t)(!half || nxp >= 0) << 1 |
-           (uint32_t)(nym >= 0) << 2 | (uint32_t)(nyp >= 0) << 3 | (uint32_t)(nzm >= 0) << 4 |
-           (uint32_t)(nzp >= 0) << 5;
+    o.xp = (half ? 0 : 4 * 64) + pos;
+    o.ym = (Q.li * 8 + 7) * 8 + Q.kh * 4;
+    o.yp = (Q.li * 8) * 8 + Q.kh * 4;
+    o.zm = (Q.li * 8 + Q.lj) * 8 + 7;
+    o.zp = (Q.li * 8 + Q.lj) * 8;
+    return o;
+}
+
+struct FaceTiles {
+    int xm, xp, ym, yp, zm, zp;  // holding tile per direction (wave-uniform; -1: absent)
+};
+
+// The table row is read through the constant address space: t is
+// wave-uniform and the table is never written by these kernels, so the row
+// comes in with scalar loads (measured 1-4 us less per launch at 13 K tiles
+// than per-lane loads).
+__device__ __forceinline__ FaceTiles face_tiles(const int32_t* __restrict__ nbr, int t) {
+    gf_const_i32* row = (gf_const_i32*)(nbr + (size_t)t * 6);
+    const int half = (__builtin_amdgcn_readfirstlane(threadIdx.x) >> 6) & 1;
+    FaceTiles f;
+    f.xm = half ? t : row[0];
+    f.xp = half ? row[1] : t;
+    f.ym = row[2];
+    f.yp = row[3];
+    f.zm = row[4];
+    f.zp = row[5];
     return f;
 }
 
-// Face values of one vector (see FaceIdx).
+// Stands in for the vector entries of an absent neighbour tile (read as FP32
+// or FP64; never written).
+__device__ double g_zero_tile[GF_TILE_CELLS] = {};
+
 template <typename T>
-__device__ __forceinline__ void load_face(const T* __restrict__ v, const FaceIdx& f, Nb4<T>& o) {
-    ld4(v + f.xm, o.xm);
-    ld4(v + f.xp, o.xp);
-    ld4(v + f.ym, o.ym);
-    ld4(v + f.yp, o.yp);
-    o.zm0 = v[f.zm];
-    o.zp3 = v[f.zp];
+__device__ __forceinline__ const T* tile_base(const T* v, int tile) {
+    return tile >= 0 ? v + (size_t)tile * GF_TILE_CELLS : reinterpret_cast<const T*>(g_zero_tile);
 }
 
+// Face values of one vector.
+template <typename T>
+__device__ __forceinline__ void load_face(const T* __restrict__ v, const FaceTiles& f,
+                                          const FaceOff& off, Nb4<T>& o) {
+    ld4(tile_base(v, f.xm) + off.xm, o.xm);
+    ld4(tile_base(v, f.xp) + off.xp, o.xp);
+    ld4(tile_base(v, f.ym) + off.ym, o.ym);
+    ld4(tile_base(v, f.yp) + off.yp, o.yp);
+    o.zm0 = tile_base(v, f.zm)[off.zm];
+    o.zp3 = tile_base(v, f.zp)[off.zp];
+}
+
+// Lanes whose DPP source lies outside the row read 0 (bound_ctrl); they take
+// a face value instead.
 template <int CTRL>
 __device__ __forceinline__ float dpp_mov(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
 }
 template <int CTRL>
 __device__ __forceinline__ double dpp_mov(double v) {
     const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, true);
     return __longlong_as_double(((long long)hi << 32) | (uint32_t)lo);
 }
 __device__ __forceinline__ float lane_read(float v, int byte_addr) {
@@ -291,26 +318,25 @@ __device__ __forceinline__ double lane_read(double v, int byte_addr) {
 enum { DPP_ROW_SHL2 = 0x102, DPP_ROW_SHR2 = 0x112, DPP_SWAP1 = 0xB1 };
 
 // Stencil neighbours of the quad c (this lane's values of the vector) from
-// the other lanes, with the face values fv (zero where ok says the
-// neighbouring tile is absent).  Identical values to load_nb4 on the same
-// vector.
+// the other lanes, with the face values fv of load_face (or values computed
+// from them).  Identical values to load_nb4 on the same vector.
 template <typename T>
-__device__ __forceinline__ void nb_from_lanes(const T (&c)[4], const Nb4<T>& fv,
-                                              const FaceIdx& f, const Quad& Q, Nb4<T>& o) {
+__device__ __forceinline__ void nb_from_lanes(const T (&c)[4], const Nb4<T>& fv, const Quad& Q,
+                                              Nb4<T>& o) {
     const int lane = threadIdx.x & 63;
     const int pl = Q.li & 3;
     const int up = 4 * (lane >= 16 ? lane - 16 : lane), dn = 4 * (lane < 48 ? lane + 16 : lane);
     for (int e = 0; e < 4; ++e) {
         const T xm = lane_read(c[e], up), xp = lane_read(c[e], dn);
         const T ym = dpp_mov<DPP_ROW_SHR2>(c[e]), yp = dpp_mov<DPP_ROW_SHL2>(c[e]);
-        o.xm[e] = pl == 0 ? ((f.ok & 1) ? fv.xm[e] : (T)0) : xm;
-        o.xp[e] = pl == 3 ? ((f.ok & 2) ? fv.xp[e] : (T)0) : xp;
-        o.ym[e] = Q.lj == 0 ? ((f.ok & 4) ? fv.ym[e] : (T)0) : ym;
-        o.yp[e] = Q.lj == 7 ? ((f.ok & 8) ? fv.yp[e] : (T)0) : yp;
+        o.xm[e] = pl == 0 ? fv.xm[e] : xm;
+        o.xp[e] = pl == 3 ? fv.xp[e] : xp;
+        o.ym[e] = Q.lj == 0 ? fv.ym[e] : ym;
+        o.yp[e] = Q.lj == 7 ? fv.yp[e] : yp;
     }
     const T zm = dpp_mov<DPP_SWAP1>(c[3]), zp = dpp_mov<DPP_SWAP1>(c[0]);
-    o.zm0 = Q.kh == 0 ? ((f.ok & 16) ? fv.zm0 : (T)0) : zm;
-    o.zp3 = Q.kh == 1 ? ((f.ok & 32) ? fv.zp3 : (T)0) : zp;
+    o.zm0 = Q.kh == 0 ? fv.zm0 : zm;
+    o.zp3 = Q.kh == 1 ? fv.zp3 : zp;
 }
 
 // Neighbour sum of quad element e in the reference order -x,+x,-y,+y,-z,+z.
@@ -447,6 +473,7 @@ enum { PEND_NONE = 0, PEND_SETUP = 1, PEND_REPLACE = 2 };
 // lets the compiler keep the tile's base and neighbour table in scalar registers.
 #define FOR_QUADS(P)                                                        \
     const Quad Q = quad_of();                                               \
+    const FaceOff FO = face_off(Q);                                         \
     for (int tb_ = P.t0; tb_ < P.t1; tb_ += 2)                              \
         if (const int t = __builtin_amdgcn_readfirstlane(tb_ + (threadIdx.x >> 7)); t < P.t1)
 
@@ -694,8 +721,9 @@ __global__ __launch_bounds__(GF_WG) void k_cg1_setup2(PcgArgs a) {
         const uint32_t fw = ld_flags(a.flags, idx);
         T c[4], y[4], wv[4];
         ld4(u + idx, c);
-        Nb4<T> o;
-        load_nb4(u, nb_index(a.nbr, t, base, Q), o);
+        Nb4<T> fu, o;
+        load_face(u, face_tiles(a.nbr, t), FO, fu);
+        nb_from_lanes(c, fu, Q, o);
         apply_A4(c, o, fw, y);
         for (int k = 0; k < 4; ++k) {
             const T dm = diagm<T>(fw >> (8 * k));
@@ -791,17 +819,17 @@ __global__ __launch_bounds__(GF_WG) void k_cg1_iter(PcgArgs a, int L) {
     FOR_QUADS(P) {
         const size_t base = (size_t)t * GF_TILE_CELLS;
         const size_t idx = base + Q.l0;
-        const NbIdx ni = nb_index(a.nbr, t, base, Q);
-        const uint32_t fw = ld_flags(a.flags, idx);
+        const FaceTiles ft = face_tiles(a.nbr, t);
+        const uint32_t fw = ld_flags(a.flags + base, FO.own);
         T uq[4], wq[4], sq[4], pq[4];
-        ld4(u + idx, uq);
-        ld4(w + idx, wq);
-        ld4(sv + idx, sq);
-        ld4(p + idx, pq);
+        ld4(u + base + FO.own, uq);
+        ld4(w + base + FO.own, wq);
+        ld4(sv + base + FO.own, sq);
+        ld4(p + base + FO.own, pq);
         Nb4<T> nu, nw, ns;
-        load_nb4(u, ni, nu);
-        load_nb4(w, ni, nw);
-        load_nb4(sv, ni, ns);
+        load_face(u, ft, FO, nu);
+        load_face(w, ft, FO, nw);
+        load_face(sv, ft, FO, ns);
         // own cells
         T pn[4], snew[4], c[4];
         for (int k = 0; k < 4; ++k) {
@@ -829,14 +857,16 @@ __global__ __launch_bounds__(GF_WG) void k_cg1_iter(PcgArgs a, int L) {
             for (int k = 0; k < 4; ++k) xd[k] += (double)al * (double)pn[k];
             st4(a.x64 + idx, xd);
         }
-        // neighbours' new u
-        Nb4<T> o;
-        unew4(nu.xm, nw.xm, ns.xm, al, be, o.xm);
-        unew4(nu.xp, nw.xp, ns.xp, al, be, o.xp);
-        unew4(nu.ym, nw.ym, ns.ym, al, be, o.ym);
-        unew4(nu.yp, nw.yp, ns.yp, al, be, o.yp);
-        o.zm0 = fma(-al, fma(be, ns.zm0, nw.zm0), nu.zm0);
-        o.zp3 = fma(-al, fma(be, ns.zp3, nw.zp3), nu.zp3);
+        // face cells' new u (the owner's two fmas, bit for bit); the rest of
+        // the stencil from the lanes' own new u
+        Nb4<T> fc, o;
+        unew4(nu.xm, nw.xm, ns.xm, al, be, fc.xm);
+        unew4(nu.xp, nw.xp, ns.xp, al, be, fc.xp);
+        unew4(nu.ym, nw.ym, ns.ym, al, be, fc.ym);
+        unew4(nu.yp, nw.yp, ns.yp, al, be, fc.yp);
+        fc.zm0 = fma(-al, fma(be, ns.zm0, nw.zm0), nu.zm0);
+        fc.zp3 = fma(-al, fma(be, ns.zp3, nw.zp3), nu.zp3);
+        nb_from_lanes(c, fc, Q, o);
         T y[4], wv[4];
         apply_A4(c, o, fw, y);
         for (int k = 0; k < 4; ++k) {
@@ -885,28 +915,30 @@ __global__ __launch_bounds__(GF_WG) void k_cg1_replace(PcgArgs a, int L) {
         const size_t base = (size_t)t * GF_TILE_CELLS;
         const size_t idx = base + Q.l0;
         const uint32_t fw = ld_flags(a.flags, idx);
-        const NbIdx ni = nb_index(a.nbr, t, base, Q);
+        const FaceTiles ft = face_tiles(a.nbr, t);
         double cd[4];
         float cf[4], bv[4], pq[4];
         ld4(a.x64 + idx, cd);
         ld4(a.x32 + idx, cf);
         ld4(a.b + idx, bv);
         ld4(a.p32 + idx, pq);
-        Nb4<double> od;
-        Nb4<float> of, op;
-        load_nb4(a.x64, ni, od);
-        load_nb4(a.x32, ni, of);
-        load_nb4(a.p32, ni, op);
+        Nb4<double> fd, od;
+        Nb4<float> ff, fp, op;
+        load_face(a.x64, ft, FO, fd);
+        load_face(a.x32, ft, FO, ff);
+        load_face(a.p32, ft, FO, fp);
         double c[4];
         for (int k = 0; k < 4; ++k) {
             c[k] = cd[k] + (double)cf[k];
-            od.xm[k] += (double)of.xm[k];
-            od.xp[k] += (double)of.xp[k];
-            od.ym[k] += (double)of.ym[k];
-            od.yp[k] += (double)of.yp[k];
+            fd.xm[k] += (double)ff.xm[k];
+            fd.xp[k] += (double)ff.xp[k];
+            fd.ym[k] += (double)ff.ym[k];
+            fd.yp[k] += (double)ff.yp[k];
         }
-        od.zm0 += (double)of.zm0;
-        od.zp3 += (double)of.zp3;
+        fd.zm0 += (double)ff.zm0;
+        fd.zp3 += (double)ff.zp3;
+        nb_from_lanes(c, fd, Q, od);
+        nb_from_lanes(pq, fp, Q, op);
         float ap[4], uo[4], so[4];
         apply_A4(pq, op, fw, ap);
         for (int k = 0; k < 4; ++k) {
@@ -946,8 +978,9 @@ __global__ __launch_bounds__(GF_WG) void k_cg1_replace2(PcgArgs a, int L) {
         const uint32_t fw = ld_flags(a.flags, idx);
         float c[4], y[4], wv[4];
         ld4(u + idx, c);
-        Nb4<float> o;
-        load_nb4(u, nb_index(a.nbr, t, base, Q), o);
+        Nb4<float> fu, o;
+        load_face(u, face_tiles(a.nbr, t), FO, fu);
+        nb_from_lanes(c, fu, Q, o);
         apply_A4(c, o, fw, y);
         for (int k = 0; k < 4; ++k) {
             const float dm = diagm<float>(fw >> (8 * k));
@@ -1055,15 +1088,14 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_dirspmv(PcgArgs a, int L) {
     double pap = 0.0;
     FOR_QUADS(P) {
         const size_t base = (size_t)t * GF_TILE_CELLS;
-        const size_t idx = base + Q.l0;
-        const FaceIdx fi = face_index(a.nbr, t, base, Q);
-        const uint32_t fw = ld_flags(a.flags, idx);
+        const FaceTiles ft = face_tiles(a.nbr, t);
+        const uint32_t fw = ld_flags(a.flags + base, FO.own);
         T zq[4], pq[4], c[4], y[4];
-        ld4(z + idx, zq);
-        ld4(po + idx, pq);
+        ld4(z + base + FO.own, zq);
+        ld4(po + base + FO.own, pq);
         Nb4<T> fz, fp, fc, o;
-        load_face(z, fi, fz);
-        load_face(po, fi, fp);
+        load_face(z, ft, FO, fz);
+        load_face(po, ft, FO, fp);
         dir4(zq, pq, bf, c);
         // face cells' new p from their z and old p (the owner's fma, bit for bit)
         dir4(fz.xm, fp.xm, bf, fc.xm);
@@ -1072,10 +1104,10 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_dirspmv(PcgArgs a, int L) {
         dir4(fz.yp, fp.yp, bf, fc.yp);
         fc.zm0 = fma(bf, fp.zm0, fz.zm0);
         fc.zp3 = fma(bf, fp.zp3, fz.zp3);
-        nb_from_lanes(c, fc, fi, Q, o);
+        nb_from_lanes(c, fc, Q, o);
         apply_A4(c, o, fw, y);
         for (int k = 0; k < 4; ++k) pap += (double)c[k] * (double)y[k];
-        st4(pn + idx, c);
+        st4(pn + base + FO.own, c);
     }
     put_partial(a.part, P_PAP, L, P.b, pap, s_red);
     e.beta = beta;
@@ -1099,15 +1131,15 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_update(PcgArgs a, int L) {
     double rz = 0.0, rr = 0.0;
     FOR_QUADS(P) {
         const size_t base = (size_t)t * GF_TILE_CELLS;
-        const size_t idx = base + Q.l0;
-        const uint32_t fw = ld_flags(a.flags, idx);
+        const size_t idx = base + FO.own;
+        const uint32_t fw = ld_flags(a.flags + base, FO.own);
         T pv[4], apv[4], zv[4];
-        ld4(p + idx, pv);
-        ld4(zp + idx, zv);
-        const FaceIdx fi = face_index(a.nbr, t, base, Q);
+        ld4(p + base + FO.own, pv);
+        ld4(zp + base + FO.own, zv);
+        const FaceTiles ft = face_tiles(a.nbr, t);
         Nb4<T> fp, np;
-        load_face(p, fi, fp);
-        nb_from_lanes(pv, fp, fi, Q, np);
+        load_face(p, ft, FO, fp);
+        nb_from_lanes(pv, fp, Q, np);
         apply_A4(pv, np, fw, apv);  // the same operands and order as the direction kernel's
         if (MIXED) {
             float xo[4];
@@ -1167,26 +1199,27 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_replace(PcgArgs a, int L) {
         const size_t base = (size_t)t * GF_TILE_CELLS;
         const size_t idx = base + Q.l0;
         const uint32_t fw = ld_flags(a.flags, idx);
-        const NbIdx ni = nb_index(a.nbr, t, base, Q);
+        const FaceTiles ft = face_tiles(a.nbr, t);
         double cd[4];
         float cf[4], bv[4];
         ld4(a.x64 + idx, cd);
         ld4(a.x32 + idx, cf);
         ld4(a.b + idx, bv);
-        Nb4<double> od;
-        Nb4<float> of;
-        load_nb4(a.x64, ni, od);
-        load_nb4(a.x32, ni, of);
+        Nb4<double> fd, od;
+        Nb4<float> ff;
+        load_face(a.x64, ft, FO, fd);
+        load_face(a.x32, ft, FO, ff);
         double c[4];
         for (int k = 0; k < 4; ++k) {
             c[k] = cd[k] + (double)cf[k];
-            od.xm[k] += (double)of.xm[k];
-            od.xp[k] += (double)of.xp[k];
-            od.ym[k] += (double)of.ym[k];
-            od.yp[k] += (double)of.yp[k];
+            fd.xm[k] += (double)ff.xm[k];
+            fd.xp[k] += (double)ff.xp[k];
+            fd.ym[k] += (double)ff.ym[k];
+            fd.yp[k] += (double)ff.yp[k];
         }
-        od.zm0 += (double)of.zm0;
-        od.zp3 += (double)of.zp3;
+        fd.zm0 += (double)ff.zm0;
+        fd.zp3 += (double)ff.zp3;
+        nb_from_lanes(c, fd, Q, od);
         float z32[4];
         for (int k = 0; k < 4; ++k) {
             const int d = fl_diag(fw, k), fl = fl_fluid(fw, k);
