@@ -1086,27 +1086,42 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_dirspmv(PcgArgs a, int L) {
     T* __restrict__ pn = std_p<T>(a, e.pbuf ^ 1);
     const T* __restrict__ z = Vec<T>::u(a, 0);
     double pap = 0.0;
+    const int lane = threadIdx.x & 63;
+    const int dn = 4 * (lane < 48 ? lane + 16 : lane);  // +x: sixteen lanes up
     FOR_QUADS(P) {
         const size_t base = (size_t)t * GF_TILE_CELLS;
         const FaceTiles ft = face_tiles(a.nbr, t);
         const uint32_t fw = ld_flags(a.flags + base, FO.own);
-        T zq[4], pq[4], c[4], y[4];
+        T zq[4], pq[4], c[4];
         ld4(z + base + FO.own, zq);
         ld4(po + base + FO.own, pq);
-        Nb4<T> fz, fp, fc, o;
-        load_face(z, ft, FO, fz);
-        load_face(po, ft, FO, fp);
+        // z and old p across the +x, +y and +z faces of the wave's block
+        T zx[4], px[4], zy[4], py[4];
+        ld4(tile_base(z, ft.xp) + FO.xp, zx);
+        ld4(tile_base(po, ft.xp) + FO.xp, px);
+        ld4(tile_base(z, ft.yp) + FO.yp, zy);
+        ld4(tile_base(po, ft.yp) + FO.yp, py);
+        const T zz = tile_base(z, ft.zp)[FO.zp], pz = tile_base(po, ft.zp)[FO.zp];
         dir4(zq, pq, bf, c);
         // face cells' new p from their z and old p (the owner's fma, bit for bit)
-        dir4(fz.xm, fp.xm, bf, fc.xm);
-        dir4(fz.xp, fp.xp, bf, fc.xp);
-        dir4(fz.ym, fp.ym, bf, fc.ym);
-        dir4(fz.yp, fp.yp, bf, fc.yp);
-        fc.zm0 = fma(bf, fp.zm0, fz.zm0);
-        fc.zp3 = fma(bf, fp.zp3, fz.zp3);
-        nb_from_lanes(c, fc, Q, o);
-        apply_A4(c, o, fw, y);
-        for (int k = 0; k < 4; ++k) pap += (double)c[k] * (double)y[k];
+        T cx[4], cy[4];
+        dir4(zx, px, bf, cx);
+        dir4(zy, py, bf, cy);
+        const T cz = fma(bf, pz, zz);
+        const T zp = dpp_mov<DPP_SWAP1>(c[0]);
+        // p.Ap as the quadratic form sum_i d_i p_i^2 - 2 sum_{edges} p_i p_j
+        // (A symmetric, p zero off fluid cells): each cell with its +x, +y,
+        // +z neighbours, products exact in FP64
+        for (int k = 0; k < 4; ++k) {
+            // every lane takes part in the exchange (the sources are other lanes)
+            const T lx = lane_read(c[k], dn), ly = dpp_mov<DPP_ROW_SHL2>(c[k]);
+            const T nxp = (Q.li & 3) == 3 ? cx[k] : lx;
+            const T nyp = Q.lj == 7 ? cy[k] : ly;
+            const T nzp = k < 3 ? c[k + 1] : (Q.kh == 1 ? cz : zp);
+            const double cd = (double)c[k];
+            const double nb = (double)nxp + (double)nyp + (double)nzp;
+            pap += cd * ((double)fl_diag(fw, k) * cd - 2.0 * nb);
+        }
         st4(pn + base + FO.own, c);
     }
     put_partial(a.part, P_PAP, L, P.b, pap, s_red);
