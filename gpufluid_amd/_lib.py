@@ -154,6 +154,7 @@ SIGNATURES = {
     "gf_k3_gauss_seidel_rb": [V, V, V, I, I, I, I, V],
     "gf_k3_jacobi_visc": [V, V, V, I, I, I, F, V],
     "gf_k3_jacobi_visc3": [V, V, V, V, V, V, V, V, V, I, I, I, F, V],
+    "gf_k3_jacobi_visc3_box": [V, V, V, V, V, V, V, V, V, I, I, I, I, I, I, I, I, I, F, V],
     "gf_k3_apply_A": [V, V, V, V, I, I, I, V],
     "gf_k3_compute_diag": [V, V, I, I, I, V],
     "gf_k3_apply_invM": [V, V, V, V, V, I, I, I, V],

@@ -536,21 +536,31 @@ __global__ __launch_bounds__(GF_WG) void k_jacobi_visc(const float* __restrict__
 constexpr int VISC_CHUNK = 32;
 constexpr int VISC_BATCH = 4;
 
+// The sweep runs over a box of cells [o, o + b) of the nx*ny*nz grid (the
+// whole grid, or the sub-dense box of solver3d.py:3821-3906, where the
+// reference's k3_jacobi_visc runs on the box-shaped arrays): a component's
+// faces of the box, with the box as the array bounds of the stencil.
 struct ViscArgs {
     const float* old0; const float* old1; const float* old2;
     const float* in0; const float* in1; const float* in2;
     float* out0; float* out1; float* out2;
-    int nx, ny, nz, nchunk;
+    int nx, ny, nz;   // grid cells (strides)
+    int ox, oy, oz;   // box origin (cells)
+    int bx, by, bz;   // box extent (cells)
+    int nchunk;
     float r;
 };
 
+// One (j, k) column of a face array: sx, sy, sz are the stencil bounds (the
+// box's face extents), di / dj the strides of the full array, the pointers
+// already at the box origin.
 __device__ __forceinline__ void visc_column(const float* __restrict__ u_old,
                                             const float* __restrict__ u_in,
                                             float* __restrict__ u_out, int sx, int sy, int sz,
-                                            int i0, int i1, int j, int k, float r) {
+                                            size_t di, size_t dj, int i0, int i1, int j, int k,
+                                            float r) {
 #pragma clang fp contract(off)
-    const size_t di = (size_t)sy * sz;
-    const size_t base = (size_t)j * sz + k;
+    const size_t base = (size_t)j * dj + k;
     // col[m] = u_in at plane i + m - 1 of the current batch (m = 0..VISC_BATCH+1)
     float col[VISC_BATCH + 2];
     col[0] = i0 > 0 ? u_in[(size_t)(i0 - 1) * di + base] : 0.0f;
@@ -563,8 +573,8 @@ __device__ __forceinline__ void visc_column(const float* __restrict__ u_old,
             const size_t q = (size_t)ii * di + base;
             col[b + 2] = ii + 1 < sx ? u_in[q + di] : 0.0f;
             const bool live = ii < i1;
-            ym[b] = live && j > 0 ? u_in[q - sz] : 0.0f;
-            yp[b] = live && j < sy - 1 ? u_in[q + sz] : 0.0f;
+            ym[b] = live && j > 0 ? u_in[q - dj] : 0.0f;
+            yp[b] = live && j < sy - 1 ? u_in[q + dj] : 0.0f;
             zm[b] = live && k > 0 ? u_in[q - 1] : 0.0f;
             zp[b] = live && k < sz - 1 ? u_in[q + 1] : 0.0f;
             uo[b] = live ? u_old[q] : 0.0f;
@@ -591,18 +601,22 @@ __device__ __forceinline__ void visc_column(const float* __restrict__ u_old,
 
 __global__ __launch_bounds__(GF_WG) void k_jacobi_visc3(ViscArgs a) {
     const int c = blockIdx.z % 3, chunk = blockIdx.z / 3;
-    const int sx = a.nx + (c == 0), sy = a.ny + (c == 1), sz = a.nz + (c == 2);
+    // the box's faces of component c, and the full face array's strides
+    const int sx = a.bx + (c == 0), sy = a.by + (c == 1), sz = a.bz + (c == 2);
+    const size_t dj = (size_t)(a.nz + (c == 2));
+    const size_t di = (size_t)(a.ny + (c == 1)) * dj;
+    const size_t org = (size_t)a.ox * di + (size_t)a.oy * dj + (size_t)a.oz;
     const int k = blockIdx.x * 64 + (threadIdx.x & 63);
     const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
     const int i0 = chunk * VISC_CHUNK;
     if (k >= sz || j >= sy || i0 >= sx) return;
     const int i1 = min(i0 + VISC_CHUNK, sx);
     if (c == 0)
-        visc_column(a.old0, a.in0, a.out0, sx, sy, sz, i0, i1, j, k, a.r);
+        visc_column(a.old0 + org, a.in0 + org, a.out0 + org, sx, sy, sz, di, dj, i0, i1, j, k, a.r);
     else if (c == 1)
-        visc_column(a.old1, a.in1, a.out1, sx, sy, sz, i0, i1, j, k, a.r);
+        visc_column(a.old1 + org, a.in1 + org, a.out1 + org, sx, sy, sz, di, dj, i0, i1, j, k, a.r);
     else
-        visc_column(a.old2, a.in2, a.out2, sx, sy, sz, i0, i1, j, k, a.r);
+        visc_column(a.old2 + org, a.in2 + org, a.out2 + org, sx, sy, sz, di, dj, i0, i1, j, k, a.r);
 }
 
 // ---- per-particle attribute transfer (S2.15 colour, S2.18 scalar) --------
@@ -947,29 +961,54 @@ extern "C" int gf_k3_jacobi_visc(const float* u_old, const float* u_in, float* u
     return 0;
 }
 
-extern "C" int gf_k3_jacobi_visc3(const float* old_u, const float* old_v, const float* old_w,
-                                  const float* in_u, const float* in_v, const float* in_w,
-                                  float* out_u, float* out_v, float* out_w, int nx, int ny,
-                                  int nz, float r, void* stream) {
-    GRID_ARGS_OK("gf_k3_jacobi_visc3");
-    GF_REQUIRE(ny + 1 <= 4 * 65535, "gf_k3_jacobi_visc3: ny %d", ny);
+namespace {
+int visc3_launch(const float* old_u, const float* old_v, const float* old_w, const float* in_u,
+                 const float* in_v, const float* in_w, float* out_u, float* out_v, float* out_w,
+                 int nx, int ny, int nz, int ox, int oy, int oz, int bx, int by, int bz, float r,
+                 void* stream, const char* who) {
+    GF_REQUIRE(nx > 0 && ny > 0 && nz > 0, "%s: grid %d x %d x %d", who, nx, ny, nz);
+    GF_REQUIRE(bx > 0 && by > 0 && bz > 0 && ox >= 0 && oy >= 0 && oz >= 0 && ox + bx <= nx &&
+                   oy + by <= ny && oz + bz <= nz,
+               "%s: box origin (%d,%d,%d) extent (%d,%d,%d) not inside the %d x %d x %d grid",
+               who, ox, oy, oz, bx, by, bz, nx, ny, nz);
+    GF_REQUIRE(by + 1 <= 4 * 65535, "%s: box y extent %d", who, by);
     GF_REQUIRE(old_u && old_v && old_w && in_u && in_v && in_w && out_u && out_v && out_w,
-               "gf_k3_jacobi_visc3: null buffer");
+               "%s: null buffer", who);
     GF_REQUIRE(out_u != in_u && out_v != in_v && out_w != in_w && out_u != old_u &&
                    out_v != old_v && out_w != old_w,
-               "gf_k3_jacobi_visc3: out must not alias in or old");
+               "%s: out must not alias in or old", who);
     ViscArgs a;
     a.old0 = old_u; a.old1 = old_v; a.old2 = old_w;
     a.in0 = in_u; a.in1 = in_v; a.in2 = in_w;
     a.out0 = out_u; a.out1 = out_v; a.out2 = out_w;
     a.nx = nx; a.ny = ny; a.nz = nz; a.r = r;
-    a.nchunk = (nx + 1 + VISC_CHUNK - 1) / VISC_CHUNK;
-    GF_REQUIRE(3LL * a.nchunk <= 65535, "gf_k3_jacobi_visc3: nx %d", nx);
-    const dim3 grid((unsigned)((nz + 1 + 63) / 64), (unsigned)((ny + 1 + 3) / 4),
+    a.ox = ox; a.oy = oy; a.oz = oz;
+    a.bx = bx; a.by = by; a.bz = bz;
+    a.nchunk = (bx + 1 + VISC_CHUNK - 1) / VISC_CHUNK;
+    GF_REQUIRE(3LL * a.nchunk <= 65535, "%s: box x extent %d", who, bx);
+    const dim3 grid((unsigned)((bz + 1 + 63) / 64), (unsigned)((by + 1 + 3) / 4),
                     (unsigned)(3 * a.nchunk));
     hipLaunchKernelGGL(k_jacobi_visc3, grid, dim3(GF_WG), 0, (hipStream_t)stream, a);
     GF_LAUNCH_CHECK("k_jacobi_visc3");
     return 0;
+}
+}  // namespace
+
+extern "C" int gf_k3_jacobi_visc3(const float* old_u, const float* old_v, const float* old_w,
+                                  const float* in_u, const float* in_v, const float* in_w,
+                                  float* out_u, float* out_v, float* out_w, int nx, int ny,
+                                  int nz, float r, void* stream) {
+    return visc3_launch(old_u, old_v, old_w, in_u, in_v, in_w, out_u, out_v, out_w, nx, ny, nz,
+                        0, 0, 0, nx, ny, nz, r, stream, "gf_k3_jacobi_visc3");
+}
+
+extern "C" int gf_k3_jacobi_visc3_box(const float* old_u, const float* old_v, const float* old_w,
+                                      const float* in_u, const float* in_v, const float* in_w,
+                                      float* out_u, float* out_v, float* out_w, int nx, int ny,
+                                      int nz, int ox, int oy, int oz, int bx, int by, int bz,
+                                      float r, void* stream) {
+    return visc3_launch(old_u, old_v, old_w, in_u, in_v, in_w, out_u, out_v, out_w, nx, ny, nz,
+                        ox, oy, oz, bx, by, bz, r, stream, "gf_k3_jacobi_visc3_box");
 }
 
 extern "C" int gf_k3_apply_A(const float* x, float* y, const int32_t* marker,

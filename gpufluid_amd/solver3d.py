@@ -30,6 +30,7 @@ Differences from the reference that callers can observe (all deliberate):
 from __future__ import annotations
 
 import ctypes
+import logging
 import math
 
 import numpy as np
@@ -42,6 +43,7 @@ from .profiling import StepProfiler
 from .scene import SceneBuffers
 from .regions import FrameEventQueue
 from .seeding import seed_box_positions
+from .subdense import SubDenseBox
 
 
 class FlipDivergenceError(RuntimeError):
@@ -86,9 +88,11 @@ class FlipSolver3D:
 
     Parameters follow solver3d.py:2358-2371; viscosity (S2.13) and surface
     tension (S2.14) run between P2G and the pressure solve as in the
-    reference; enable_sub_dense is accepted and keeps the dense layout (the
-    pressure storage is compacted to active tiles instead).  Extra
-    keyword-only parameters:
+    reference.  enable_sub_dense (F3.7) keeps a box around the fluid: the grid
+    attributes show the box and the viscosity sweep runs over it, as the
+    reference's box-shaped arrays do (storage stays full-size; the pressure
+    vectors are compacted to active tiles anyway).  Extra keyword-only
+    parameters:
     pcg_precision ("mixed" | "fp64"), pcg_sign ("physical" | "reference"),
     pcg_tol (reference default 1e-4), pcg_check_every (host readback cadence).
     """
@@ -123,15 +127,15 @@ class FlipSolver3D:
         self.surface_tension = surface_tension
         self.csf_smoothing_passes = max(1, int(csf_smoothing_passes))
         self.transfer_mode = transfer_mode
-        # Sub-dense storage (F3.7) shrinks the reference's dense arrays to the
-        # fluid's dilated bounding box; here the grid fields stay dense (HBM is
-        # not the constraint) and the pressure solve already runs on the
-        # compacted active tiles, so the flag is accepted and changes nothing.
-        self._enable_sub_dense = bool(enable_sub_dense)
-        self.sub_rebuild_every = int(sub_rebuild_every)
-        self.sub_dilation = int(sub_dilation)
-        self._sub_offset = (0, 0, 0)
-        self._sub_shape = (int(nx), int(ny), int(nz))
+        # Sub-dense box (F3.7, solver3d.py:2441-2473): the box itself and its
+        # rebuild policy live in SubDenseBox; the fields stay full-size.
+        self._sub = SubDenseBox(self.nx, self.ny, self.nz, enable_sub_dense, sub_rebuild_every,
+                                sub_dilation)
+        if enable_sub_dense and int(sub_dilation) < 1:
+            logging.getLogger("gpufluid.solver").warning(
+                "FlipSolver3D: enable_sub_dense=True with sub_dilation=%d: fluid can reach the "
+                "box edge, where the viscosity stencil is cut off; use it for bookkeeping only",
+                int(sub_dilation))
         self.pcg_precision = pcg_precision
         self.pcg_sign = pcg_sign
         self.pcg_tol = float(pcg_tol)
@@ -199,12 +203,17 @@ class FlipSolver3D:
     def _arr(self, name):
         return DeviceArray(self._f[name])
 
-    u = property(lambda self: self._arr("u"))
-    v = property(lambda self: self._arr("v"))
-    w = property(lambda self: self._arr("w"))
-    us = property(lambda self: self._arr("us"))
-    vs = property(lambda self: self._arr("vs"))
-    ws = property(lambda self: self._arr("ws"))
+    def _boxed(self, t, axis):
+        """t, or its sub-dense box part (a view) while the box is smaller
+        than the grid: the reference's box-shaped arrays (solver3d.py:3858-3879)."""
+        return DeviceArray(t[self._sub.face_slices(axis)] if self._sub.active else t)
+
+    u = property(lambda self: self._boxed(self._f["u"], 0))
+    v = property(lambda self: self._boxed(self._f["v"], 1))
+    w = property(lambda self: self._boxed(self._f["w"], 2))
+    us = property(lambda self: self._boxed(self._f["us"], 0))
+    vs = property(lambda self: self._boxed(self._f["vs"], 1))
+    ws = property(lambda self: self._boxed(self._f["ws"], 2))
     solid_u = property(lambda self: self._arr("solid_u"))
     solid_v = property(lambda self: self._arr("solid_v"))
     solid_w = property(lambda self: self._arr("solid_w"))
@@ -218,9 +227,9 @@ class FlipSolver3D:
             self._uvw_weights = tuple(torch.zeros_like(f[k]) for k in ("u", "v", "w"))
         return self._uvw_weights
 
-    uw = property(lambda self: DeviceArray(self._weights()[0]))
-    vw = property(lambda self: DeviceArray(self._weights()[1]))
-    ww = property(lambda self: DeviceArray(self._weights()[2]))
+    uw = property(lambda self: self._boxed(self._weights()[0], 0))
+    vw = property(lambda self: self._boxed(self._weights()[1], 1))
+    ww = property(lambda self: self._boxed(self._weights()[2], 2))
 
     def _dense(self, key, compact):
         t = self._cache.get(key)
@@ -230,7 +239,7 @@ class FlipSolver3D:
             _lib.call("F3.3", "gf_scene_compact_to_dense", ctypes.byref(s), _lib.ptr(compact),
                       _lib.ptr(t), self._stream())
             self._cache[key] = t
-        return DeviceArray(t)
+        return self._boxed(t, None)
 
     @property
     def p(self):
@@ -619,7 +628,9 @@ class FlipSolver3D:
         publish this frame's events; outflow boxes are compacted away on the
         device first, then the emitted particles (minus those landing in a
         solid cell of the host marker) are appended with their event's colour
-        (else 0) and temperature (else 20) when the attribute is tracked."""
+        (else 0) and temperature (else 20) when the attribute is tracked.
+        Last, with enable_sub_dense, the box is rebuilt when due
+        (solver3d.py:4050-4072)."""
         # the frame may change the particle set and the solids: drop the
         # captured substeps, as the reference does (solver3d.py:3912-3915)
         self._cuda_graph_invalidate()
@@ -635,8 +646,19 @@ class FlipSolver3D:
         if out_ev:
             self._apply_outflows_gpu(out_ev)
         emits = q.drain_emits()
-        if not emits:
-            return
+        if emits:
+            self._emit_particles(emits)
+        if self._sub.enabled and self._should_rebuild_sub_dense(frame_idx):
+            # the marker of the last substep's P2G: before any step it holds
+            # only walls and obstacles and gives no box
+            bbox = self._compute_active_bbox(self._f["marker"].cpu().numpy())
+            if bbox is not None:
+                self._rebuild_sub_dense(*bbox)
+                self._last_sub_rebuild_frame = frame_idx
+                self._check_particles_in_sub_bbox(frame_idx)
+
+    def _emit_particles(self, emits):
+        """Append the frame's emitted particles (see prepare_frame)."""
         epos = np.concatenate([e.positions for e in emits]).astype(np.float32)
         evel = np.concatenate([e.velocities for e in emits]).astype(np.float32)
         col = temp = None
@@ -669,6 +691,62 @@ class FlipSolver3D:
         new_temp = None if temp is None else cat(self._attr["temperature"], temp)
         self.set_particles(cat(cur_p, epos), cat(cur_v, evel), color=new_col,
                            temperature=new_temp)
+
+    # ------------------------------------------------ F3.7 sub-dense box
+    _sub_offset = property(lambda self: self._sub.offset)
+    _sub_shape = property(lambda self: self._sub.shape)
+    _enable_sub_dense = property(lambda self: self._sub.enabled)
+    _sub_dilation = property(lambda self: self._sub.dilation,
+                             lambda self, d: setattr(self._sub, "dilation", max(0, int(d))))
+    _sub_rebuild_every = property(lambda self: self._sub.rebuild_every,
+                                  lambda self, n: setattr(self._sub, "rebuild_every",
+                                                          max(1, int(n))))
+    _last_sub_rebuild_frame = property(
+        lambda self: self._sub.last_rebuild_frame,
+        lambda self, f: setattr(self._sub, "last_rebuild_frame", int(f)))
+    sub_dilation = _sub_dilation
+    sub_rebuild_every = _sub_rebuild_every
+
+    @block("F3.7", "Compute the 8^3-tile bbox of active fluid cells, dilated by sub_dilation "
+                   "and clamped to the domain.")
+    def _compute_active_bbox(self, marker_host: np.ndarray):
+        """((lo), (hi)) cells of the dilated fluid-tile box, or None
+        (solver3d.py:3670-3702)."""
+        return self._sub.bbox(marker_host)
+
+    @block("F3.7", "Map a world-space particle position to local sub-dense cell coords + flag "
+                   "whether it lies inside the bbox.")
+    def _pos_to_sub_cell(self, pos):
+        """((li, lj, lk), inside) of one position (solver3d.py:3704-3726)."""
+        return self._sub.pos_to_cell(pos, self.dx)
+
+    def _check_particles_in_sub_bbox(self, frame_idx: int) -> int:
+        """Count of particles outside the box, with a one-shot warning
+        (solver3d.py:3728-3768)."""
+        if not self._sub.enabled or self.n_particles == 0:
+            return 0
+        return self._sub.count_outside(self.pos.numpy(), self.dx, frame_idx)
+
+    def _should_rebuild_sub_dense(self, frame_idx: int) -> bool:
+        """Periodic / first-call / edge-proximity trigger (solver3d.py:3770-3819).
+        The host marker is refreshed from the device when the device one holds
+        fluid (a marker set by hand on the host is kept otherwise)."""
+        if not self._sub.enabled:
+            return False
+        if self._sub.last_rebuild_frame >= 0 and \
+                frame_idx - self._sub.last_rebuild_frame < self._sub.rebuild_every:
+            dev_marker = self._f["marker"].cpu().numpy()
+            if np.any(dev_marker == 1):
+                self._marker_host = dev_marker
+        return self._sub.should_rebuild(frame_idx, self._marker_host)
+
+    @block("F3.7", "Move the sub-dense box to [lo, hi); grid attributes show the box and the "
+                   "viscosity sweep runs over it.")
+    def _rebuild_sub_dense(self, new_lo, new_hi):
+        """Box bookkeeping of solver3d.py:3821-3906.  Storage is full-size, so
+        nothing is copied: the box's cells keep their global values."""
+        self._sub.rebuild(new_lo, new_hi)
+        self._cuda_graph_invalidate()
 
     @block("D4.7.GPU", "GPU outflow compaction (mark + scan + scatter)")
     def _apply_outflows_gpu(self, outflow_events) -> int:
@@ -749,10 +827,22 @@ class FlipSolver3D:
         rhs = self._weights()
         for t, key in zip(rhs, ("u", "v", "w")):
             t.copy_(f[key])
+        box = self._sub
+        if box.active and int(self.viscosity_iters) > 0:
+            # The sweep covers the box only (its faces, the box as the stencil
+            # bounds, as the reference's k3_jacobi_visc on its box-shaped
+            # arrays); both ping-pong buffers hold u outside it.
+            for key, sav in (("u", "us"), ("v", "vs"), ("w", "ws")):
+                f[sav].copy_(f[key])
         for _ in range(int(self.viscosity_iters)):
-            _lib.call("S2.13", "gf_k3_jacobi_visc3", *(_lib.ptr(t) for t in rhs),
-                      *(_lib.ptr(f[k]) for k in ("u", "v", "w", "us", "vs", "ws")),
-                      self.nx, self.ny, self.nz, r, self._stream())
+            bufs = (*(_lib.ptr(t) for t in rhs),
+                    *(_lib.ptr(f[k]) for k in ("u", "v", "w", "us", "vs", "ws")))
+            if box.active:
+                _lib.call("S2.13", "gf_k3_jacobi_visc3_box", *bufs, self.nx, self.ny, self.nz,
+                          *box.offset, *box.shape, r, self._stream())
+            else:
+                _lib.call("S2.13", "gf_k3_jacobi_visc3", *bufs, self.nx, self.ny, self.nz, r,
+                          self._stream())
             for key, sav in (("u", "us"), ("v", "vs"), ("w", "ws")):
                 f[key], f[sav] = f[sav], f[key]
         self._solid_bc()
@@ -821,6 +911,11 @@ class FlipSolver3D:
         compatibility: the pressure solve always runs on the active 8^3 tiles
         only, which is what the reference's per-tile variants compute."""
         if pressure_solver not in ("jacobi", "pcg", "gsrb"):
+            if self._sub.active:  # the reference's sub-dense guard (solver3d.py:3429-3440)
+                raise NotImplementedError(
+                    "FlipSolver3D.step() with a sub-dense box rejects: "
+                    f"pressure_solver={pressure_solver!r}; disable enable_sub_dense or use "
+                    "'jacobi', 'gsrb' or 'pcg'")
             raise ValueError(f"unknown pressure_solver: {pressure_solver!r}")
         tol = self.pcg_tol if pressure_tol is None else float(pressure_tol)
         if self._cuda_graph_eligible(pressure_solver, pressure_block_sparse):
@@ -883,7 +978,8 @@ class FlipSolver3D:
                 self.pcg_precision, self.pcg_sign, int(self.pcg_check_every),
                 float(self.gravity), float(self.rho), float(self.flip_blend),
                 float(self.viscosity), int(self.viscosity_iters), float(self.surface_tension),
-                int(self.csf_smoothing_passes), bool(self.p2g_staged))
+                int(self.csf_smoothing_passes), bool(self.p2g_staged), self._sub.offset,
+                self._sub.shape)
 
     def _step_graphed(self, dt, pressure_iters, pressure_solver, tol):
         """Replay the captured substep for this key, capturing it first on a

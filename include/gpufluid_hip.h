@@ -472,6 +472,16 @@ int gf_k3_jacobi_visc3(const float* old_u, const float* old_v, const float* old_
                        const float* in_u, const float* in_v, const float* in_w, float* out_u,
                        float* out_v, float* out_w, int nx, int ny, int nz, float r,
                        void* stream);
+/* The same sweep on the faces of the box of cells [o, o+b) of the nx*ny*nz
+ * grid, with the box as the stencil's array bounds: the reference's
+ * k3_jacobi_visc launched on the sub-dense arrays (solver3d.py:178-200,
+ * 3530-3541, box from _rebuild_sub_dense 3821-3906).  Faces outside the box
+ * are neither read nor written. */
+int gf_k3_jacobi_visc3_box(const float* old_u, const float* old_v, const float* old_w,
+                           const float* in_u, const float* in_v, const float* in_w,
+                           float* out_u, float* out_v, float* out_w, int nx, int ny, int nz,
+                           int ox, int oy, int oz, int bx, int by, int bz, float r,
+                           void* stream);
 /* k3_apply_A, solver3d.py:1743-1797 (done: device int[1], may be NULL) */
 int gf_k3_apply_A(const float* x, float* y, const int32_t* marker, const int32_t* done,
                   int nx, int ny, int nz, void* stream);

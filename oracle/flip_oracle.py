@@ -567,12 +567,42 @@ def visc_sweep(u_old, u_in, r):
     return (num / den).astype(u_in.dtype)
 
 
-def viscosity_stage(u, v, w, us, vs, ws, marker, r, iters):
+def box_face_slices(box, c):
+    """Index of a sub-dense box's faces in component c's face array (c None:
+    cells); box = ((ox, oy, oz), (sx, sy, sz))."""
+    lo, shape = box
+    return tuple(slice(lo[a], lo[a] + shape[a] + (1 if a == c else 0)) for a in range(3))
+
+
+def viscosity_stage(u, v, w, us, vs, ws, marker, r, iters, box=None):
     """The S2.13 stage of _step_impl (solver3d.py:3524-3544): right-hand side
     = the post-BC faces; each sweep's result becomes the component and the
     component before it moves into us/vs/ws (the reference swaps the arrays),
     so us ends as the last-but-one iterate (or unchanged for 0 sweeps); then
-    the solid BC.  Returns (u, v, w, us, vs, ws)."""
+    the solid BC.  Returns (u, v, w, us, vs, ws).
+
+    box: the sub-dense box of F3.7 (solver3d.py:3821-3906), on which the
+    reference runs the same sweeps over its box-shaped arrays (the box edge
+    is the stencil's array bound).  The sweeps then run on each component's
+    box slice; outside the box the faces keep their pre-sweep values, in
+    both the component and us/vs/ws (with no sweep at all, as without a
+    box)."""
+    if box is not None and iters > 0:
+        cur = [u.copy(), v.copy(), w.copy()]
+        sav = [u.copy(), v.copy(), w.copy()]
+        sl = [box_face_slices(box, c) for c in range(3)]
+        rhs = [cur[c][sl[c]].copy() for c in range(3)]
+        bc = [cur[c][sl[c]].copy() for c in range(3)]
+        bs = [sav[c][sl[c]].copy() for c in range(3)]
+        for _ in range(iters):
+            for c in range(3):
+                new = visc_sweep(rhs[c], bc[c], r)
+                bs[c], bc[c] = bc[c], new
+        for c in range(3):
+            cur[c][sl[c]] = bc[c]
+            sav[c][sl[c]] = bs[c]
+        cu, cv, cw = enforce_solid_bc(cur[0], cur[1], cur[2], marker)
+        return cu, cv, cw, sav[0], sav[1], sav[2]
     rhs = (u.copy(), v.copy(), w.copy())
     cur = [u, v, w]
     sav = [us, vs, ws]
@@ -841,7 +871,8 @@ def attr_transfer_loop(pos, attr, dx, shape, mode):
 
 def substep(state: dict, dt: float, pressure="pcg", iters=80, tol=1e-4, dtype=np.float64,
             gravity=-9.81, flip_blend=0.95, rho=1.0, transfer="flip", pcg_sign="physical",
-            viscosity=0.0, viscosity_iters=12, surface_tension=0.0, csf_passes=2):
+            viscosity=0.0, viscosity_iters=12, surface_tension=0.0, csf_passes=2,
+            visc_box=None):
     """One substep in the stage order of _step_impl (solver3d.py:3475-3667):
     clear → P2G → normalize → gravity → BC → [viscosity → BC] → [surface
     tension → BC] → divergence →
@@ -849,7 +880,8 @@ def substep(state: dict, dt: float, pressure="pcg", iters=80, tol=1e-4, dtype=np
     vel, marker, dx, (APIC) affine and optionally color (n,3) / temperature (n,).
     pressure: "jacobi" / "gsrb" (fixed iterations), "pcg" (reference restatement;
     pcg_sign "physical" negates its result, "reference" keeps it) or
-    "exact" (spsolve).  Returns a new state plus the grid fields."""
+    "exact" (spsolve).  visc_box: the F3.7 sub-dense box the viscosity runs on
+    (viscosity_stage).  Returns a new state plus the grid fields."""
     pos, vel, marker, dx = state["pos"], state["vel"], state["marker"], state["dx"]
     nx, ny, nz = marker.shape
     dom = [domain(n, dx) for n in (nx, ny, nz)]
@@ -865,7 +897,8 @@ def substep(state: dict, dt: float, pressure="pcg", iters=80, tol=1e-4, dtype=np
     u, v, w = enforce_solid_bc(u, v, w, m)
     if viscosity > 0.0:
         r = np.float32(dt * viscosity / (dx * dx))
-        u, v, w, us, vs, ws = viscosity_stage(u, v, w, us, vs, ws, m, r, viscosity_iters)
+        u, v, w, us, vs, ws = viscosity_stage(u, v, w, us, vs, ws, m, r, viscosity_iters,
+                                              box=visc_box)
     if surface_tension > 0.0:
         coeff = np.float32(surface_tension * dt / max(rho, 1.0e-8))
         u, v, w = csf_apply(u, v, w, m, dx, coeff, max(1, int(csf_passes)))
