@@ -425,11 +425,19 @@ __device__ __forceinline__ const double* pslot(const double* part, int kind, int
 __device__ __forceinline__ double partial_share(const double* part, int kind, int L, int wb,
                                                 int we) {
     const double* src = pslot(part, kind, L);
+    double x[PCG_GMAX / GF_WG];
+    // unconditional loads (index clamped into the slot), so all are issued
+    // back to back; entries past the scene's range are dropped afterwards
+#pragma unroll
+    for (int q = 0; q < PCG_GMAX / GF_WG; ++q) {
+        const int i = wb + threadIdx.x + q * GF_WG;
+        x[q] = src[i < PCG_GMAX ? i : PCG_GMAX - 1];
+    }
     double v = 0.0;
 #pragma unroll
     for (int q = 0; q < PCG_GMAX / GF_WG; ++q) {
         const int i = wb + threadIdx.x + q * GF_WG;
-        v += i < we ? src[i] : 0.0;
+        v += i < we ? x[q] : 0.0;
     }
     return v;
 }
@@ -570,12 +578,9 @@ __device__ __forceinline__ void write_log(const PcgArgs& a, const Part& P, const
 // delta = (Au,u), r.r; counts the finished iteration and sets the stop /
 // replacement flags.  Identical in every workgroup of the scene.
 template <bool MIXED>
-__device__ __forceinline__ void consume(const PcgArgs& a, const Part& P, PcgLog& e, int L,
-                                        double* s_red) {
+__device__ __forceinline__ void consume(const PcgArgs& a, PcgLog& e, double gam, double del,
+                                        double rr, double* s_red) {
     if (!e.fresh) return;
-    double gam = partial_share(a.part, P_RZ, L - 1, P.wb, P.we);
-    double del = partial_share(a.part, P_PAP, L - 1, P.wb, P.we);
-    double rr = partial_share(a.part, P_RR, L - 1, P.wb, P.we);
     wg_sum3(gam, del, rr, s_red);
     e.rz_old = e.rz;
     e.rz = gam;
@@ -593,6 +598,14 @@ __device__ __forceinline__ void consume(const PcgArgs& a, const Part& P, PcgLog&
     } else if (MIXED && rr < a.delta2 * e.rr_upd) {
         e.upd = 1;
     }
+}
+
+// The previous launch's partial sums do not depend on the solver log, but
+// they are only used past the log's branches, where the compiler would sink
+// their loads and pay two memory latencies in a row.  This empty asm uses
+// them together with the log, so both sets of loads are in flight at once.
+__device__ __forceinline__ void issue_together(const PcgLog& e, double x, double y, double z) {
+    asm volatile("" ::"v"(x), "v"(y), "v"(z), "s"(e.done), "s"(e.fresh));
 }
 
 __device__ __forceinline__ bool pass_through(const PcgArgs& a, const Part& P, const PcgLog& e,
@@ -795,8 +808,12 @@ __global__ __launch_bounds__(GF_WG) void k_cg1_iter(PcgArgs a, int L) {
     const Part P = part_of(a);
     if (P.s < 0) return;
     PcgLog e = live_log(a, P.s, L - 1);
+    const double gsh = partial_share(a.part, P_RZ, L - 1, P.wb, P.we);
+    const double dsh = partial_share(a.part, P_PAP, L - 1, P.wb, P.we);
+    const double rrs = partial_share(a.part, P_RR, L - 1, P.wb, P.we);
+    issue_together(e, gsh, dsh, rrs);
     if (pass_through(a, P, e, L)) return;
-    consume<MIXED>(a, P, e, L, s_red);
+    consume<MIXED>(a, e, gsh, dsh, rrs, s_red);
     if (e.done) {
         write_log(a, P, e, L);
         return;
@@ -902,7 +919,11 @@ __global__ __launch_bounds__(GF_WG) void k_cg1_replace(PcgArgs a, int L) {
     const Part P = part_of(a);
     if (P.s < 0) return;
     PcgLog e = live_log(a, P.s, L - 1);
-    consume<MIXED>(a, P, e, L, s_red);
+    const double gsh = partial_share(a.part, P_RZ, L - 1, P.wb, P.we);
+    const double dsh = partial_share(a.part, P_PAP, L - 1, P.wb, P.we);
+    const double rrs = partial_share(a.part, P_RR, L - 1, P.wb, P.we);
+    issue_together(e, gsh, dsh, rrs);
+    consume<MIXED>(a, e, gsh, dsh, rrs, s_red);
     const bool run = MIXED && (e.upd || (e.done && !e.converged));
     if (!run) {
         write_log(a, P, e, L);
@@ -1011,11 +1032,9 @@ __device__ __forceinline__ T* std_p(const PcgArgs& a, int buf) {
 
 // The update kernel's r.z / r.r partials (written by launch L - 1).
 template <bool MIXED>
-__device__ __forceinline__ void consume_std(const PcgArgs& a, const Part& P, PcgLog& e, int L,
+__device__ __forceinline__ void consume_std(const PcgArgs& a, PcgLog& e, double rz_new, double rr,
                                             double* s_red) {
     if (!e.fresh) return;
-    double rz_new = partial_share(a.part, P_RZ, L - 1, P.wb, P.we);
-    double rr = partial_share(a.part, P_RR, L - 1, P.wb, P.we);
     wg_sum2(rz_new, rr, s_red);
     e.rz_old = e.rz;
     e.rz = rz_new;
@@ -1074,8 +1093,11 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_dirspmv(PcgArgs a, int L) {
     const Part P = part_of(a);
     if (P.s < 0) return;
     PcgLog e = live_log(a, P.s, L - 1);
+    const double rzs = partial_share(a.part, P_RZ, L - 1, P.wb, P.we);
+    const double rrs = partial_share(a.part, P_RR, L - 1, P.wb, P.we);
+    issue_together(e, rzs, rrs, 0.0);
     if (pass_through(a, P, e, L)) return;
-    consume_std<MIXED>(a, P, e, L, s_red);
+    consume_std<MIXED>(a, e, rzs, rrs, s_red);
     if (e.done) {
         write_log(a, P, e, L);
         return;
@@ -1136,8 +1158,10 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_update(PcgArgs a, int L) {
     const Part P = part_of(a);
     if (P.s < 0) return;
     PcgLog e = live_log(a, P.s, L - 1);
+    const double paps = partial_share(a.part, P_PAP, L - 1, P.wb, P.we);
+    issue_together(e, paps, 0.0, 0.0);
     if (pass_through(a, P, e, L)) return;
-    const double pap_t = wg_sum(partial_share(a.part, P_PAP, L - 1, P.wb, P.we), s_red);
+    const double pap_t = wg_sum(paps, s_red);
     const double alpha = fabs(pap_t) < 1e-30 ? 0.0 : e.rz / pap_t;
     const T af = (T)alpha;
     const int merge = MIXED ? e.merge : 0;
@@ -1202,7 +1226,10 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_replace(PcgArgs a, int L) {
     const Part P = part_of(a);
     if (P.s < 0) return;
     PcgLog e = live_log(a, P.s, L - 1);
-    consume_std<MIXED>(a, P, e, L, s_red);
+    const double rzs = partial_share(a.part, P_RZ, L - 1, P.wb, P.we);
+    const double rrs = partial_share(a.part, P_RR, L - 1, P.wb, P.we);
+    issue_together(e, rzs, rrs, 0.0);
+    consume_std<MIXED>(a, e, rzs, rrs, s_red);
     const bool run = MIXED && (e.upd || (e.done && !e.converged));
     if (!run) {
         write_log(a, P, e, L);
