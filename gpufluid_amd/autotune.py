@@ -49,6 +49,20 @@ def get_pcg_variant() -> int:
     return v.value
 
 
+def set_pcg_x_every(v: int) -> None:
+    """Two-kernel PCG: 1 updates the iterate x in every update kernel, 2 (or
+    0, the default) in every other one, adding both pending terms in order
+    (bit-identical results, one read and one write of x fewer per two
+    iterations)."""
+    _lib.call("X3.1", "gf_tune_set", b"pcg_x_every", int(v))
+
+
+def get_pcg_x_every() -> int:
+    v = ctypes.c_int(0)
+    _lib.call("X3.1", "gf_tune_get", b"pcg_x_every", ctypes.byref(v))
+    return v.value
+
+
 @block("X3.1", "Autotune the PCG persistent-grid size on the live workload")
 def tune_pcg_workgroups(solve, key, candidates=(512, 768, 1024, 1536, 2048), repeats: int = 2,
                         device=None):

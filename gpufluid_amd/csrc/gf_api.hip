@@ -37,6 +37,7 @@ static int tune_id(const char* key) {
     if (!key) return -1;
     if (!strcmp(key, "pcg_workgroups")) return gf::GF_TUNE_PCG_WORKGROUPS;
     if (!strcmp(key, "pcg_variant")) return gf::GF_TUNE_PCG_VARIANT;
+    if (!strcmp(key, "pcg_x_every")) return gf::GF_TUNE_PCG_X_EVERY;
     return -1;
 }
 

@@ -14,7 +14,7 @@ int set_error(int code, const char* fmt, ...);
 int check_launch(const char* what);
 
 // Launch-shape tunables set through gf_tune_set (0 = built-in choice).
-enum { GF_TUNE_PCG_WORKGROUPS = 0, GF_TUNE_PCG_VARIANT, GF_TUNE_COUNT };
+enum { GF_TUNE_PCG_WORKGROUPS = 0, GF_TUNE_PCG_VARIANT, GF_TUNE_PCG_X_EVERY, GF_TUNE_COUNT };
 int tunable(int id);
 
 // Exclusive scan of n int32 (gf_sort.hip): out[0..n] with out[n] = total;

@@ -37,6 +37,7 @@
 #include <hip/hip_ext.h>
 #include <string.h>
 #include <mutex>
+#include <type_traits>
 
 namespace {
 
@@ -51,7 +52,9 @@ struct PcgLog {
     int iters, done, converged, maxed, upd, merge, nupd, pending;
     int fresh;       // 1: the update kernel's r.z / r.r partials are not consumed yet
     int pbuf;        // direction buffer holding the current p
-    double pad[2];
+    int xpend;       // 1: x still lacks alpha p of the last update (two-kernel form)
+    int pad_i;
+    double pad;
 };
 static_assert(sizeof(PcgLog) == 16 * sizeof(double), "log entry is 16 doubles");
 
@@ -261,9 +264,11 @@ struct FaceTiles {
 __device__ __forceinline__ FaceTiles face_tiles(const int32_t* __restrict__ nbr, int t) {
     gf_const_i32* row = (gf_const_i32*)(nbr + (size_t)t * 6);
     const int half = (__builtin_amdgcn_readfirstlane(threadIdx.x) >> 6) & 1;
+    // the whole row is read (one scalar load, no branch), then selected
+    const int r0 = row[0], r1 = row[1];
     FaceTiles f;
-    f.xm = half ? t : row[0];
-    f.xp = half ? row[1] : t;
+    f.xm = half ? t : r0;
+    f.xp = half ? r1 : t;
     f.ym = row[2];
     f.yp = row[3];
     f.zm = row[4];
@@ -278,6 +283,16 @@ __device__ double g_zero_tile[GF_TILE_CELLS] = {};
 template <typename T>
 __device__ __forceinline__ const T* tile_base(const T* v, int tile) {
     return tile >= 0 ? v + (size_t)tile * GF_TILE_CELLS : reinterpret_cast<const T*>(g_zero_tile);
+}
+
+// Neighbour-table row of no tile (every neighbour absent): the first-pass
+// prefetch of a workgroup with no tiles reads the zero tile through it.
+__device__ int32_t g_no_nbr[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
+
+__device__ __forceinline__ FaceTiles face_tiles_or_none(const int32_t* __restrict__ nbr, int t) {
+    FaceTiles f = face_tiles(t >= 0 ? nbr : g_no_nbr, t >= 0 ? t : 0);
+    if (t < 0) f.xm = f.xp = -1;  // face_tiles names the tile itself for one of them
+    return f;
 }
 
 // Face values of one vector.
@@ -543,14 +558,15 @@ __device__ __forceinline__ void put_partial3(double* part, int ka, int kb, int k
 // and a local leaves part of the local in per-thread LDS.
 #define GF_LOG_FIELDS(X)                                                              \
     X(rz) X(rz_old) X(rr) X(bb) X(alpha) X(beta) X(rr_upd) X(rr_true) X(pap) X(iters) \
-    X(done) X(converged) X(maxed) X(upd) X(merge) X(nupd) X(pending) X(fresh) X(pbuf)
+    X(done) X(converged) X(maxed) X(upd) X(merge) X(nupd) X(pending) X(fresh) X(pbuf) X(xpend)
 
 __device__ __forceinline__ PcgLog copy_log(const PcgLog* src) {
     PcgLog e;
 #define GF_LOG_LD(f) e.f = src->f;
     GF_LOG_FIELDS(GF_LOG_LD)
 #undef GF_LOG_LD
-    e.pad[0] = e.pad[1] = 0.0;
+    e.pad_i = 0;
+    e.pad = 0.0;
     return e;
 }
 __device__ __forceinline__ void put_log(PcgLog* dst, const PcgLog& e) {
@@ -767,6 +783,7 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_fin(PcgArgs a, int L) {
         e.rz = gam; e.pap = del; e.rz_old = 0.0; e.alpha = 0.0;
         e.bb = bb; e.rr = bb; e.rr_upd = bb; e.rr_true = bb;
         e.iters = 0; e.maxed = 0; e.upd = 0; e.merge = 0; e.nupd = 0; e.fresh = 0; e.pbuf = 0;
+        e.xpend = 0;
         const int zero = bb < 1e-30;
         e.converged = zero;
         e.done = zero || a.max_iter <= 0;
@@ -1030,6 +1047,43 @@ __device__ __forceinline__ T* std_p(const PcgArgs& a, int buf) {
     return buf ? Vec<T>::s(a, 0) : Vec<T>::p(a);
 }
 
+// How an update kernel of the two-kernel form treats the iterate x: add
+// alpha p now, leave it pending (the next update adds both terms), or add the
+// pending term and its own.
+enum { X_NOW = 0, X_DEFER = 1, X_PAIR = 2 };
+
+// x += a1 p1 (when two) then x += a2 p2 on a quad, term by term in iteration
+// order, as one update per iteration would add them, so a deferred term
+// changes no bit of x (mixed: x32, first folded into x64 when merge is set).
+template <typename T, bool MIXED>
+__device__ __forceinline__ void x_add(const PcgArgs& a, size_t idx, int merge, bool two, T a1,
+                                      const T (&p1)[4], T a2, const T (&p2)[4]) {
+    if (MIXED) {
+        float xo[4];
+        ld4(a.x32 + idx, xo);
+        if (merge) {
+            double xd[4];
+            ld4(a.x64 + idx, xd);
+            for (int k = 0; k < 4; ++k) {
+                xd[k] += (double)xo[k];
+                xo[k] = 0.0f;
+            }
+            st4(a.x64 + idx, xd);
+        }
+        if (two)
+            for (int k = 0; k < 4; ++k) xo[k] = xo[k] + (float)a1 * (float)p1[k];
+        for (int k = 0; k < 4; ++k) xo[k] = xo[k] + (float)a2 * (float)p2[k];
+        st4(a.x32 + idx, xo);
+    } else {
+        double xd[4];
+        ld4(a.x64 + idx, xd);
+        if (two)
+            for (int k = 0; k < 4; ++k) xd[k] += (double)a1 * (double)p1[k];
+        for (int k = 0; k < 4; ++k) xd[k] += (double)a2 * (double)p2[k];
+        st4(a.x64 + idx, xd);
+    }
+}
+
 // The update kernel's r.z / r.r partials (written by launch L - 1).
 template <bool MIXED>
 __device__ __forceinline__ void consume_std(const PcgArgs& a, PcgLog& e, double rz_new, double rr,
@@ -1087,18 +1141,98 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_setup(PcgArgs a) {
     put_partial3(a.part, P_RZ, P_PAP, P_BB, 0, P.b, rz, 0.0, rr, s_red);  // read by fin at L = 1
 }
 
+// Loads of one tile pass of the direction kernel: this quad's z and old p,
+// and both across the +x, +y and +z faces of the wave's block.
+template <typename T>
+struct DirIn {
+    uint32_t fw;
+    T zq[4], pq[4], zx[4], px[4], zy[4], py[4];
+    T zz, pz;
+};
+
+// t < 0: no tile (reads the zero tile; the values are not used).  ft is the
+// tile's neighbour row (face_tiles_or_none), read one pass ahead.
+template <typename T>
+__device__ __forceinline__ void dir_issue(DirIn<T>& d, const PcgArgs& a, const T* __restrict__ z,
+                                          const T* __restrict__ po, int t, const FaceTiles& ft,
+                                          const FaceOff& FO) {
+    d.fw = ld_flags(tile_base(a.flags, t), FO.own);
+    ld4(tile_base(z, t) + FO.own, d.zq);
+    ld4(tile_base(po, t) + FO.own, d.pq);
+    ld4(tile_base(z, ft.xp) + FO.xp, d.zx);
+    ld4(tile_base(po, ft.xp) + FO.xp, d.px);
+    ld4(tile_base(z, ft.yp) + FO.yp, d.zy);
+    ld4(tile_base(po, ft.yp) + FO.yp, d.py);
+    d.zz = tile_base(z, ft.zp)[FO.zp];
+    d.pz = tile_base(po, ft.zp)[FO.zp];
+}
+
+// p = z + beta p_old for the quad (stored into pn) and p.Ap as the quadratic
+// form sum_i d_i p_i^2 - 2 sum_{edges} p_i p_j (A symmetric, p zero off fluid
+// cells): each cell with its +x, +y, +z neighbours, products exact in FP64.
+// The face cells' new p comes from their z and old p with the owner's fma,
+// bit for bit.
+template <typename T>
+__device__ __forceinline__ void dir_compute(const DirIn<T>& d, T bf, const Quad& Q, int dn,
+                                            T* __restrict__ pn, size_t idx, double& pap) {
+    T c[4], cx[4], cy[4];
+    dir4(d.zq, d.pq, bf, c);
+    dir4(d.zx, d.px, bf, cx);
+    dir4(d.zy, d.py, bf, cy);
+    const T cz = fma(bf, d.pz, d.zz);
+    const T zp = dpp_mov<DPP_SWAP1>(c[0]);
+    for (int k = 0; k < 4; ++k) {
+        // every lane takes part in the exchange (the sources are other lanes)
+        const T lx = lane_read(c[k], dn), ly = dpp_mov<DPP_ROW_SHL2>(c[k]);
+        const T nxp = (Q.li & 3) == 3 ? cx[k] : lx;
+        const T nyp = Q.lj == 7 ? cy[k] : ly;
+        const T nzp = k < 3 ? c[k + 1] : (Q.kh == 1 ? cz : zp);
+        const double cd = (double)c[k];
+        const double nb = (double)nxp + (double)nyp + (double)nzp;
+        pap += cd * ((double)fl_diag(d.fw, k) * cd - 2.0 * nb);
+    }
+    st4(pn + idx, c);
+}
+
+// Every load of a tile pass is issued before any of its values is used (one
+// memory round trip per pass), and the first pass's loads go out before the
+// solver log and the partial sums are waited for.  They read the direction
+// buffer pb that the host expects from the launch sequence; a scene whose log
+// names the other buffer reloads them.
 template <typename T, bool MIXED>
-__global__ __launch_bounds__(GF_WG) void k_pcg_dirspmv(PcgArgs a, int L) {
+__global__ __launch_bounds__(GF_WG) void k_pcg_dirspmv(PcgArgs a, int L, int pb) {
     __shared__ double s_red[3 * GF_WG / GF_WAVE];
     const Part P = part_of(a);
     if (P.s < 0) return;
-    PcgLog e = live_log(a, P.s, L - 1);
+    const Quad Q = quad_of();
+    const FaceOff FO = face_off(Q);
+    const int half = threadIdx.x >> 7;
+    const T* __restrict__ z = Vec<T>::u(a, 0);
+    const bool any = P.t0 < P.t1;
+    const int tf = __builtin_amdgcn_readfirstlane(any ? min(P.t0 + half, P.t1 - 1) : -1);
+    // unconditional, so that the wait counts below see one path
+    const FaceTiles ftf = face_tiles_or_none(a.nbr, tf);
+    DirIn<T> d;
+    dir_issue(d, a, z, std_p<T>(a, pb), tf, ftf, FO);
     const double rzs = partial_share(a.part, P_RZ, L - 1, P.wb, P.we);
     const double rrs = partial_share(a.part, P_RR, L - 1, P.wb, P.we);
+    PcgLog e = live_log(a, P.s, L - 1);
     issue_together(e, rzs, rrs, 0.0);
     if (pass_through(a, P, e, L)) return;
     consume_std<MIXED>(a, e, rzs, rrs, s_red);
     if (e.done) {
+        if (e.xpend) {  // the solve stopped with the last alpha p still pending
+            const T af = (T)e.alpha;
+            const T* __restrict__ p = std_p<T>(a, e.pbuf);
+            FOR_QUADS(P) {
+                const size_t idx = (size_t)t * GF_TILE_CELLS + FO.own;
+                T pv[4];
+                ld4(p + idx, pv);
+                x_add<T, MIXED>(a, idx, MIXED ? e.merge : 0, false, af, pv, af, pv);
+            }
+            e.xpend = 0;
+            e.merge = 0;
+        }
         write_log(a, P, e, L);
         return;
     }
@@ -1106,45 +1240,24 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_dirspmv(PcgArgs a, int L) {
     const T bf = (T)beta;
     const T* __restrict__ po = std_p<T>(a, e.pbuf);
     T* __restrict__ pn = std_p<T>(a, e.pbuf ^ 1);
-    const T* __restrict__ z = Vec<T>::u(a, 0);
+    if (any && e.pbuf != pb) dir_issue(d, a, z, po, tf, ftf, FO);
     double pap = 0.0;
     const int lane = threadIdx.x & 63;
     const int dn = 4 * (lane < 48 ? lane + 16 : lane);  // +x: sixteen lanes up
-    FOR_QUADS(P) {
-        const size_t base = (size_t)t * GF_TILE_CELLS;
-        const FaceTiles ft = face_tiles(a.nbr, t);
-        const uint32_t fw = ld_flags(a.flags + base, FO.own);
-        T zq[4], pq[4], c[4];
-        ld4(z + base + FO.own, zq);
-        ld4(po + base + FO.own, pq);
-        // z and old p across the +x, +y and +z faces of the wave's block
-        T zx[4], px[4], zy[4], py[4];
-        ld4(tile_base(z, ft.xp) + FO.xp, zx);
-        ld4(tile_base(po, ft.xp) + FO.xp, px);
-        ld4(tile_base(z, ft.yp) + FO.yp, zy);
-        ld4(tile_base(po, ft.yp) + FO.yp, py);
-        const T zz = tile_base(z, ft.zp)[FO.zp], pz = tile_base(po, ft.zp)[FO.zp];
-        dir4(zq, pq, bf, c);
-        // face cells' new p from their z and old p (the owner's fma, bit for bit)
-        T cx[4], cy[4];
-        dir4(zx, px, bf, cx);
-        dir4(zy, py, bf, cy);
-        const T cz = fma(bf, pz, zz);
-        const T zp = dpp_mov<DPP_SWAP1>(c[0]);
-        // p.Ap as the quadratic form sum_i d_i p_i^2 - 2 sum_{edges} p_i p_j
-        // (A symmetric, p zero off fluid cells): each cell with its +x, +y,
-        // +z neighbours, products exact in FP64
-        for (int k = 0; k < 4; ++k) {
-            // every lane takes part in the exchange (the sources are other lanes)
-            const T lx = lane_read(c[k], dn), ly = dpp_mov<DPP_ROW_SHL2>(c[k]);
-            const T nxp = (Q.li & 3) == 3 ? cx[k] : lx;
-            const T nyp = Q.lj == 7 ? cy[k] : ly;
-            const T nzp = k < 3 ? c[k + 1] : (Q.kh == 1 ? cz : zp);
-            const double cd = (double)c[k];
-            const double nb = (double)nxp + (double)nyp + (double)nzp;
-            pap += cd * ((double)fl_diag(fw, k) * cd - 2.0 * nb);
+    if (any) {
+        int tn = __builtin_amdgcn_readfirstlane(P.t0 + 2 + half);
+        FaceTiles ftn = face_tiles_or_none(a.nbr, tn < P.t1 ? tn : -1);
+        if (P.t0 + half < P.t1) dir_compute(d, bf, Q, dn, pn, (size_t)tf * GF_TILE_CELLS + FO.own, pap);
+        for (int tb = P.t0 + 2; tb < P.t1; tb += 2) {
+            const int t = tn;
+            if (t >= P.t1) break;
+            const FaceTiles ft = ftn;
+            tn = __builtin_amdgcn_readfirstlane(tb + 2 + half);
+            ftn = face_tiles_or_none(a.nbr, tn < P.t1 ? tn : -1);  // the next pass's row
+            DirIn<T> n;
+            dir_issue(n, a, z, po, t, ft, FO);
+            dir_compute(n, bf, Q, dn, pn, (size_t)t * GF_TILE_CELLS + FO.own, pap);
         }
-        st4(pn + base + FO.own, c);
     }
     put_partial(a.part, P_PAP, L, P.b, pap, s_red);
     e.beta = beta;
@@ -1152,37 +1265,50 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_dirspmv(PcgArgs a, int L) {
     write_log(a, P, e, L);
 }
 
+// Loads of one tile pass of the update kernel: flags, p, z, the iterate x
+// (when this update adds to it), the previous p (when it adds a pending
+// term) and p across the six faces of the wave's block.
 template <typename T, bool MIXED>
-__global__ __launch_bounds__(GF_WG) void k_pcg_update(PcgArgs a, int L) {
-    __shared__ double s_red[3 * GF_WG / GF_WAVE];
-    const Part P = part_of(a);
-    if (P.s < 0) return;
-    PcgLog e = live_log(a, P.s, L - 1);
-    const double paps = partial_share(a.part, P_PAP, L - 1, P.wb, P.we);
-    issue_together(e, paps, 0.0, 0.0);
-    if (pass_through(a, P, e, L)) return;
-    const double pap_t = wg_sum(paps, s_red);
-    const double alpha = fabs(pap_t) < 1e-30 ? 0.0 : e.rz / pap_t;
-    const T af = (T)alpha;
-    const int merge = MIXED ? e.merge : 0;
-    T* __restrict__ zp = Vec<T>::u(a, 0);
-    const T* __restrict__ p = std_p<T>(a, e.pbuf);
-    double rz = 0.0, rr = 0.0;
-    FOR_QUADS(P) {
-        const size_t base = (size_t)t * GF_TILE_CELLS;
-        const size_t idx = base + FO.own;
-        const uint32_t fw = ld_flags(a.flags + base, FO.own);
-        T pv[4], apv[4], zv[4];
-        ld4(p + base + FO.own, pv);
-        ld4(zp + base + FO.own, zv);
-        const FaceTiles ft = face_tiles(a.nbr, t);
-        Nb4<T> fp, np;
-        load_face(p, ft, FO, fp);
-        nb_from_lanes(pv, fp, Q, np);
-        apply_A4(pv, np, fw, apv);  // the same operands and order as the direction kernel's
+struct UpdIn {
+    using XT = typename std::conditional<MIXED, float, double>::type;
+    uint32_t fw;
+    T pv[4], zv[4], pp[4];
+    XT xo[4];
+    Nb4<T> fp;
+};
+
+template <typename T, bool MIXED>
+__device__ __forceinline__ void upd_issue(UpdIn<T, MIXED>& d, const PcgArgs& a,
+                                          const T* __restrict__ p, const T* __restrict__ pprev,
+                                          const T* __restrict__ zp, int t, const FaceTiles& ft,
+                                          const FaceOff& FO, bool need_x, bool need_pp) {
+    // t < 0: no tile (reads the zero tile; the values are not used)
+    const uint32_t o = FO.own;
+    d.fw = ld_flags(tile_base(a.flags, t), o);
+    ld4(tile_base(p, t) + o, d.pv);
+    ld4(tile_base(zp, t) + o, d.zv);
+    load_face(p, ft, FO, d.fp);
+    if (need_x) {
+        if (MIXED) ld4(tile_base(a.x32, t) + o, reinterpret_cast<float(&)[4]>(d.xo));
+        else ld4(tile_base(a.x64, t) + o, reinterpret_cast<double(&)[4]>(d.xo));
+    }
+    if (need_pp) ld4(tile_base(pprev, t) + o, d.pp);
+}
+
+// Ap from p (the direction kernel's operands and order), x += (pending
+// term) + alpha p, r = M z - alpha Ap, z = r / M, and the r.z, r.r partials.
+template <typename T, bool MIXED>
+__device__ __forceinline__ void upd_compute(UpdIn<T, MIXED>& d, const PcgArgs& a, const Quad& Q,
+                                            T af, T aprev, bool defer, bool pair, int merge,
+                                            T* __restrict__ zp, size_t idx, double& rz,
+                                            double& rr) {
+    T apv[4];
+    Nb4<T> np;
+    nb_from_lanes(d.pv, d.fp, Q, np);
+    apply_A4(d.pv, np, d.fw, apv);
+    if (!defer) {
         if (MIXED) {
-            float xo[4];
-            ld4(a.x32 + idx, xo);
+            float(&xo)[4] = reinterpret_cast<float(&)[4]>(d.xo);
             if (merge) {
                 double xd[4];
                 ld4(a.x64 + idx, xd);
@@ -1192,27 +1318,87 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_update(PcgArgs a, int L) {
                 }
                 st4(a.x64 + idx, xd);
             }
-            for (int k = 0; k < 4; ++k) xo[k] = xo[k] + (float)af * (float)pv[k];
+            if (pair)
+                for (int k = 0; k < 4; ++k) xo[k] = xo[k] + (float)aprev * (float)d.pp[k];
+            for (int k = 0; k < 4; ++k) xo[k] = xo[k] + (float)af * (float)d.pv[k];
             st4(a.x32 + idx, xo);
         } else {
-            double xd[4];
-            ld4(a.x64 + idx, xd);
-            for (int k = 0; k < 4; ++k) xd[k] += (double)af * (double)pv[k];
+            double(&xd)[4] = reinterpret_cast<double(&)[4]>(d.xo);
+            if (pair)
+                for (int k = 0; k < 4; ++k) xd[k] += (double)aprev * (double)d.pp[k];
+            for (int k = 0; k < 4; ++k) xd[k] += (double)af * (double)d.pv[k];
             st4(a.x64 + idx, xd);
         }
-        for (int k = 0; k < 4; ++k) {
-            const T dm = diagm<T>(fw >> (8 * k));
-            const T rv = fma(-af, apv[k], dm * zv[k]);
-            zv[k] = rv / dm;
-            rz += (double)rv * (double)zv[k];
-            rr += (double)rv * (double)rv;
+    }
+    T zv[4];
+    for (int k = 0; k < 4; ++k) {
+        const T dm = diagm<T>(d.fw >> (8 * k));
+        const T rv = fma(-af, apv[k], dm * d.zv[k]);
+        zv[k] = rv / dm;
+        rz += (double)rv * (double)zv[k];
+        rr += (double)rv * (double)rv;
+    }
+    st4(zp + idx, zv);
+}
+
+// Loads as in the direction kernel: all of a pass at once, the first pass
+// before the log (current direction buffer pb as the host expects it).
+template <typename T, bool MIXED>
+__global__ __launch_bounds__(GF_WG) void k_pcg_update(PcgArgs a, int L, int xmode, int pb) {
+    __shared__ double s_red[3 * GF_WG / GF_WAVE];
+    const Part P = part_of(a);
+    if (P.s < 0) return;
+    const Quad Q = quad_of();
+    const FaceOff FO = face_off(Q);
+    const int half = threadIdx.x >> 7;
+    T* __restrict__ zp = Vec<T>::u(a, 0);
+    const bool defer = xmode == X_DEFER;
+    const bool want_pp = xmode == X_PAIR;
+    const bool any = P.t0 < P.t1;
+    const int tf = __builtin_amdgcn_readfirstlane(any ? min(P.t0 + half, P.t1 - 1) : -1);
+    // unconditional, so that the wait counts below see one path
+    const FaceTiles ftf = face_tiles_or_none(a.nbr, tf);
+    UpdIn<T, MIXED> d;
+    upd_issue(d, a, std_p<T>(a, pb), std_p<T>(a, pb ^ 1), zp, tf, ftf, FO, !defer, want_pp);
+    const double paps = partial_share(a.part, P_PAP, L - 1, P.wb, P.we);
+    PcgLog e = live_log(a, P.s, L - 1);
+    issue_together(e, paps, 0.0, 0.0);
+    if (pass_through(a, P, e, L)) return;
+    const double pap_t = wg_sum(paps, s_red);
+    const double alpha = fabs(pap_t) < 1e-30 ? 0.0 : e.rz / pap_t;
+    const T af = (T)alpha;
+    const int merge = MIXED ? e.merge : 0;
+    const T* __restrict__ p = std_p<T>(a, e.pbuf);
+    // the previous iteration's p (still in the other direction buffer) and
+    // alpha, when its term is pending
+    const bool pair = want_pp && e.xpend;
+    const T* __restrict__ pprev = std_p<T>(a, e.pbuf ^ 1);
+    const T aprev = (T)e.alpha;
+    if (any && e.pbuf != pb) upd_issue(d, a, p, pprev, zp, tf, ftf, FO, !defer, want_pp);
+    double rz = 0.0, rr = 0.0;
+    if (any) {
+        int tn = __builtin_amdgcn_readfirstlane(P.t0 + 2 + half);
+        FaceTiles ftn = face_tiles_or_none(a.nbr, tn < P.t1 ? tn : -1);
+        if (P.t0 + half < P.t1)
+            upd_compute(d, a, Q, af, aprev, defer, pair, merge, zp,
+                        (size_t)tf * GF_TILE_CELLS + FO.own, rz, rr);
+        for (int tb = P.t0 + 2; tb < P.t1; tb += 2) {
+            const int t = tn;
+            if (t >= P.t1) break;
+            const FaceTiles ft = ftn;
+            tn = __builtin_amdgcn_readfirstlane(tb + 2 + half);
+            ftn = face_tiles_or_none(a.nbr, tn < P.t1 ? tn : -1);  // the next pass's row
+            UpdIn<T, MIXED> n;
+            upd_issue(n, a, p, pprev, zp, t, ft, FO, !defer, want_pp);
+            upd_compute(n, a, Q, af, aprev, defer, pair, merge, zp,
+                        (size_t)t * GF_TILE_CELLS + FO.own, rz, rr);
         }
-        st4(zp + idx, zv);
     }
     put_partial2(a.part, P_RZ, P_RR, L, P.b, rz, rr, s_red);
     e.alpha = alpha;
     e.pap = pap_t;
-    e.merge = 0;
+    if (!defer) e.merge = 0;
+    e.xpend = defer ? 1 : 0;
     e.fresh = 1;
     write_log(a, P, e, L);
 }
@@ -1598,9 +1784,18 @@ int pcg_launch_chunk(const PcgArgs& a, bool mixed, int chunk, int& L, hipStream_
         if (ev) hipExtLaunchKernelGGL(K, grid, block, 0, st, E0, E1, 0, __VA_ARGS__); \
         else hipLaunchKernelGGL(K, grid, block, 0, st, __VA_ARGS__);                \
     } while (0)
+    // two-kernel form: x every other update (the chunk's last update of an
+    // odd chunk adds its own term), unless the tunable asks for every one
+    const bool xpair = gf::tunable(gf::GF_TUNE_PCG_X_EVERY) != 1;
+    const int c0 = (L - 1) / pcg_chunk_launches(a, chunk, mixed);  // chunks before this one
     for (int j = 0; j < chunk; ++j) {
         const int i = 0;
         ev = j == chunk / 2 ? ev_all : nullptr;
+        const int xmode = !xpair ? X_NOW : ((j & 1) ? X_PAIR : (j + 1 < chunk ? X_DEFER : X_NOW));
+        // direction buffer of the k-th direction kernel since the setup
+        // (k = 1, 2, ...): p_old in (k-1)&1, p_new in k&1, for every scene
+        // still iterating (a stopped scene passes through)
+        const int k = c0 * chunk + j + 1;
         if (a.variant == PCG_CG1) {
             ++L;
             if (mixed)
@@ -1610,14 +1805,15 @@ int pcg_launch_chunk(const PcgArgs& a, bool mixed, int chunk, int& L, hipStream_
         } else {
             ++L;
             if (mixed)
-                GF_TLAUNCH((k_pcg_dirspmv<float, true>), e0(i), e1(i), a, L);
+                GF_TLAUNCH((k_pcg_dirspmv<float, true>), e0(i), e1(i), a, L, (k - 1) & 1);
             else
-                GF_TLAUNCH((k_pcg_dirspmv<double, false>), e0(i), e1(i), a, L);
+                GF_TLAUNCH((k_pcg_dirspmv<double, false>), e0(i), e1(i), a, L, (k - 1) & 1);
             ++L;
             if (mixed)
-                GF_TLAUNCH((k_pcg_update<float, true>), e0(i + 1), e1(i + 1), a, L);
+                GF_TLAUNCH((k_pcg_update<float, true>), e0(i + 1), e1(i + 1), a, L, xmode, k & 1);
             else
-                GF_TLAUNCH((k_pcg_update<double, false>), e0(i + 1), e1(i + 1), a, L);
+                GF_TLAUNCH((k_pcg_update<double, false>), e0(i + 1), e1(i + 1), a, L, xmode,
+                           k & 1);
         }
     }
     const int ic = kpi;

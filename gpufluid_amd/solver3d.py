@@ -974,6 +974,7 @@ class FlipSolver3D:
             held += list(ws or ())
         ptrs = tuple(0 if t is None else t.data_ptr() for t in held)
         return (raw, ptrs, autotune.get_pcg_workgroups(), autotune.get_pcg_variant(),
+                autotune.get_pcg_x_every(),
                 float(dt), int(pressure_iters), pressure_solver, float(tol), self.transfer_mode,
                 self.pcg_precision, self.pcg_sign, int(self.pcg_check_every),
                 float(self.gravity), float(self.rho), float(self.flip_blend),

@@ -199,6 +199,49 @@ def test_pcg_reaches_1e8_and_matches_direct_solve(precision, pcg_variant):
     assert fo.residual_norm(got, div, m) < 1e-5
 
 
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_pcg_x_every_other_update_is_bit_identical(precision):
+    """Two-kernel PCG: deferring x += alpha p to the next update kernel (which
+    then adds both terms in iteration order) gives the same bits as adding
+    it in every update: pressure, iteration count, residual and replacement
+    count, for even and odd chunks and for solves that stop (by tolerance or
+    by the iteration cap) right after an update that left its term pending,
+    where the direction kernel adds it."""
+    from gpufluid_amd import _lib, autotune
+    s = _prepared(shape=(32, 32, 32), pcg_precision=precision)
+    if precision == "fp64":
+        s._sc.ensure_fp64()
+    d = s._descriptor()
+    prec = _lib.GF_PCG_MIXED if precision == "mixed" else _lib.GF_PCG_FP64
+    deferred_stops = 0
+    autotune.set_pcg_variant(1)
+    try:
+        for check_every, tol, cap in [(10, 1e-8, 500), (7, 1e-8, 500), (4, 1e-5, 500),
+                                      (10, 1e-6, 500), (3, 1e-7, 500), (10, 1e-8, 37),
+                                      (6, 1e-8, 41)]:
+            got = {}
+            for xe in (1, 2):
+                autotune.set_pcg_x_every(xe)
+                it, res, nrep = ctypes.c_int(), ctypes.c_double(), ctypes.c_int()
+                _lib.call("S2.6.3", "gf_scene_pcg_sync", ctypes.byref(d), s.n_active_tiles, prec,
+                          tol, cap, check_every, -1, ctypes.byref(it), ctypes.byref(res),
+                          ctypes.byref(nrep), None, s._stream())
+                s._cache.clear()
+                got[xe] = (s.p.numpy().copy(), it.value, res.value, nrep.value)
+            print(f"[pcg x] {precision} chunk {check_every} tol {tol} cap {cap}: "
+                  f"{got[1][1:]} vs {got[2][1:]}", flush=True)
+            np.testing.assert_array_equal(got[2][0], got[1][0])
+            assert got[2][1:] == got[1][1:]
+            # the update after which the solve stopped was a deferring one
+            # (even position in its chunk, not the last of an odd chunk)
+            j = (got[1][1] - 1) % check_every
+            deferred_stops += j % 2 == 0 and j + 1 < check_every
+    finally:
+        autotune.set_pcg_x_every(0)
+        autotune.set_pcg_variant(0)
+    assert deferred_stops > 0
+
+
 def test_pcg_reference_sign_is_negated(pcg_variant):
     from gpufluid_amd import _lib
     s = _prepared(shape=(16, 16, 16))
