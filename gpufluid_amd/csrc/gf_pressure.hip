@@ -289,9 +289,26 @@ __device__ __forceinline__ const T* tile_base(const T* v, int tile) {
 // prefetch of a workgroup with no tiles reads the zero tile through it.
 __device__ int32_t g_no_nbr[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
 
-__device__ __forceinline__ FaceTiles face_tiles_or_none(const int32_t* __restrict__ nbr, int t) {
-    FaceTiles f = face_tiles(t >= 0 ? nbr : g_no_nbr, t >= 0 ? t : 0);
-    if (t < 0) f.xm = f.xp = -1;  // face_tiles names the tile itself for one of them
+// The neighbour row of tile t as stored (t < 0: every neighbour absent); the
+// wave's own -x or +x entry is put in by for_half when the row is used, so
+// a row loaded one pass ahead is not waited for until then.
+__device__ __forceinline__ FaceTiles face_row(const int32_t* __restrict__ nbr, int t) {
+    gf_const_i32* row = (gf_const_i32*)(t >= 0 ? nbr + (size_t)t * 6 : g_no_nbr);
+    FaceTiles f;
+    f.xm = row[0];
+    f.xp = row[1];
+    f.ym = row[2];
+    f.yp = row[3];
+    f.zm = row[4];
+    f.zp = row[5];
+    return f;
+}
+
+// face_tiles of tile t from its stored row.
+__device__ __forceinline__ FaceTiles for_half(FaceTiles f, int t) {
+    const int half = (__builtin_amdgcn_readfirstlane(threadIdx.x) >> 6) & 1;
+    f.xm = half ? t : f.xm;
+    f.xp = half ? f.xp : t;
     return f;
 }
 
@@ -1150,12 +1167,13 @@ struct DirIn {
     T zz, pz;
 };
 
-// t < 0: no tile (reads the zero tile; the values are not used).  ft is the
-// tile's neighbour row (face_tiles_or_none), read one pass ahead.
+// t < 0: no tile (reads the zero tile; the values are not used).  row is the
+// tile's stored neighbour row (face_row), read one pass ahead.
 template <typename T>
 __device__ __forceinline__ void dir_issue(DirIn<T>& d, const PcgArgs& a, const T* __restrict__ z,
-                                          const T* __restrict__ po, int t, const FaceTiles& ft,
+                                          const T* __restrict__ po, int t, const FaceTiles& row,
                                           const FaceOff& FO) {
+    const FaceTiles ft = for_half(row, t);
     d.fw = ld_flags(tile_base(a.flags, t), FO.own);
     ld4(tile_base(z, t) + FO.own, d.zq);
     ld4(tile_base(po, t) + FO.own, d.pq);
@@ -1211,7 +1229,7 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_dirspmv(PcgArgs a, int L, int pb)
     const bool any = P.t0 < P.t1;
     const int tf = __builtin_amdgcn_readfirstlane(any ? min(P.t0 + half, P.t1 - 1) : -1);
     // unconditional, so that the wait counts below see one path
-    const FaceTiles ftf = face_tiles_or_none(a.nbr, tf);
+    const FaceTiles ftf = face_row(a.nbr, tf);
     DirIn<T> d;
     dir_issue(d, a, z, std_p<T>(a, pb), tf, ftf, FO);
     const double rzs = partial_share(a.part, P_RZ, L - 1, P.wb, P.we);
@@ -1246,16 +1264,17 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_dirspmv(PcgArgs a, int L, int pb)
     const int dn = 4 * (lane < 48 ? lane + 16 : lane);  // +x: sixteen lanes up
     if (any) {
         int tn = __builtin_amdgcn_readfirstlane(P.t0 + 2 + half);
-        FaceTiles ftn = face_tiles_or_none(a.nbr, tn < P.t1 ? tn : -1);
+        FaceTiles ftn = face_row(a.nbr, tn < P.t1 ? tn : -1);
         if (P.t0 + half < P.t1) dir_compute(d, bf, Q, dn, pn, (size_t)tf * GF_TILE_CELLS + FO.own, pap);
         for (int tb = P.t0 + 2; tb < P.t1; tb += 2) {
             const int t = tn;
             if (t >= P.t1) break;
-            const FaceTiles ft = ftn;
-            tn = __builtin_amdgcn_readfirstlane(tb + 2 + half);
-            ftn = face_tiles_or_none(a.nbr, tn < P.t1 ? tn : -1);  // the next pass's row
             DirIn<T> n;
-            dir_issue(n, a, z, po, t, ft, FO);
+            dir_issue(n, a, z, po, t, ftn, FO);
+            // the next pass's row is read after this pass's vector loads are out
+            __builtin_amdgcn_sched_barrier(0);
+            tn = __builtin_amdgcn_readfirstlane(tb + 2 + half);
+            ftn = face_row(a.nbr, tn < P.t1 ? tn : -1);
             dir_compute(n, bf, Q, dn, pn, (size_t)t * GF_TILE_CELLS + FO.own, pap);
         }
     }
@@ -1280,9 +1299,11 @@ struct UpdIn {
 template <typename T, bool MIXED>
 __device__ __forceinline__ void upd_issue(UpdIn<T, MIXED>& d, const PcgArgs& a,
                                           const T* __restrict__ p, const T* __restrict__ pprev,
-                                          const T* __restrict__ zp, int t, const FaceTiles& ft,
+                                          const T* __restrict__ zp, int t, const FaceTiles& row,
                                           const FaceOff& FO, bool need_x, bool need_pp) {
-    // t < 0: no tile (reads the zero tile; the values are not used)
+    // t < 0: no tile (reads the zero tile; the values are not used); row: the
+    // stored neighbour row (face_row), read one pass ahead
+    const FaceTiles ft = for_half(row, t);
     const uint32_t o = FO.own;
     d.fw = ld_flags(tile_base(a.flags, t), o);
     ld4(tile_base(p, t) + o, d.pv);
@@ -1357,7 +1378,7 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_update(PcgArgs a, int L, int xmod
     const bool any = P.t0 < P.t1;
     const int tf = __builtin_amdgcn_readfirstlane(any ? min(P.t0 + half, P.t1 - 1) : -1);
     // unconditional, so that the wait counts below see one path
-    const FaceTiles ftf = face_tiles_or_none(a.nbr, tf);
+    const FaceTiles ftf = face_row(a.nbr, tf);
     UpdIn<T, MIXED> d;
     upd_issue(d, a, std_p<T>(a, pb), std_p<T>(a, pb ^ 1), zp, tf, ftf, FO, !defer, want_pp);
     const double paps = partial_share(a.part, P_PAP, L - 1, P.wb, P.we);
@@ -1378,18 +1399,19 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_update(PcgArgs a, int L, int xmod
     double rz = 0.0, rr = 0.0;
     if (any) {
         int tn = __builtin_amdgcn_readfirstlane(P.t0 + 2 + half);
-        FaceTiles ftn = face_tiles_or_none(a.nbr, tn < P.t1 ? tn : -1);
+        FaceTiles ftn = face_row(a.nbr, tn < P.t1 ? tn : -1);
         if (P.t0 + half < P.t1)
             upd_compute(d, a, Q, af, aprev, defer, pair, merge, zp,
                         (size_t)tf * GF_TILE_CELLS + FO.own, rz, rr);
         for (int tb = P.t0 + 2; tb < P.t1; tb += 2) {
             const int t = tn;
             if (t >= P.t1) break;
-            const FaceTiles ft = ftn;
-            tn = __builtin_amdgcn_readfirstlane(tb + 2 + half);
-            ftn = face_tiles_or_none(a.nbr, tn < P.t1 ? tn : -1);  // the next pass's row
             UpdIn<T, MIXED> n;
-            upd_issue(n, a, p, pprev, zp, t, ft, FO, !defer, want_pp);
+            upd_issue(n, a, p, pprev, zp, t, ftn, FO, !defer, want_pp);
+            // the next pass's row is read after this pass's vector loads are out
+            __builtin_amdgcn_sched_barrier(0);
+            tn = __builtin_amdgcn_readfirstlane(tb + 2 + half);
+            ftn = face_row(a.nbr, tn < P.t1 ? tn : -1);
             upd_compute(n, a, Q, af, aprev, defer, pair, merge, zp,
                         (size_t)t * GF_TILE_CELLS + FO.own, rz, rr);
         }
