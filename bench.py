@@ -55,7 +55,7 @@ BYTES = {
 TRAFFIC_FILE = ROOT / "profiles" / "pmc_traffic_substep_pool_sphere_256.txt"
 TRAFFIC_KERNEL = {"pcg_dirspmv": ["k_pcg_dirspmv<float,true>"], "pcg_update": ["k_pcg_update<float,true>"],
                   "pcg_replace": ["k_pcg_replace<true>"], "pcg_iter": ["k_cg1_iter<float,true>"],
-                  "p2g": ["k_p2g_own_flip<false>", "k_p2g_finish"], "g2p": ["k_g2p_tile<false>"]}
+                  "p2g": ["k_p2g_cells_flip<false>", "k_p2g_finish"], "g2p": ["k_g2p_tile<false>"]}
 
 
 def measured_traffic(stage):

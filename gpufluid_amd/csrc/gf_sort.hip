@@ -223,7 +223,7 @@ __global__ __launch_bounds__(1024) void k_tile_scan(const int32_t* __restrict__ 
     for (int base = 0; base < ntiles; base += blockDim.x) {
         const int pos = base + threadIdx.x;
         const int t = pos < ntiles ? (order ? order[pos] : pos) : 0;
-        int o = (pos < ntiles && occ[t]) ? 1 : 0;
+        int o = (pos < ntiles && (occ[t] & 1)) ? 1 : 0;
         int total;
         int ex = gf::wg_exclusive_scan(o, s_wave, &total);
         if (pos < ntiles) {
@@ -249,7 +249,7 @@ __global__ __launch_bounds__(1024) void k_tile_scan(const int32_t* __restrict__ 
 __global__ __launch_bounds__(GF_WG) void k_tile_setup(
     int nx, int ny, int nz, int tnx, int tny, int tnz, const int32_t* __restrict__ cell_start,
     int32_t* marker, const int32_t* __restrict__ tile_map, uint8_t* __restrict__ flags,
-    int32_t* __restrict__ nbr, int32_t* __restrict__ n_active) {
+    int32_t* __restrict__ nbr, int32_t* __restrict__ n_active, int32_t* __restrict__ tile_occ) {
     __shared__ int s_wave[GF_WG / GF_WAVE + 1];
     const int ntiles = tnx * tny * tnz;
     const int t = gf::xcd_remap(blockIdx.x, ntiles);
@@ -257,7 +257,7 @@ __global__ __launch_bounds__(GF_WG) void k_tile_setup(
     const int slot = tile_map[t];
     const int i0 = ti * 8, j0 = tj * 8, k0 = tk * 8;
     const int lj = (threadIdx.x >> 3) & 7, lk = threadIdx.x & 7;
-    int nfluid = 0;
+    int nfluid = 0, anyp = 0, anys = 0;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         const int li = (threadIdx.x >> 6) + 4 * h;
@@ -268,6 +268,8 @@ __global__ __launch_bounds__(GF_WG) void k_tile_setup(
             const int c = (i * ny + j) * nz + k;
             const int mold = marker[c];
             const int cnt = cell_start[c + 1] - cell_start[c];
+            anyp |= cnt > 0;
+            anys |= mold == 2;
             const int m = (mold == 2) ? 2 : (cnt > 0 ? 1 : 0);
             if (m != mold) marker[c] = m;
             if (m == 1) {
@@ -287,6 +289,10 @@ __global__ __launch_bounds__(GF_WG) void k_tile_setup(
     int total;
     gf::wg_exclusive_scan(nfluid, s_wave, &total);
     if (threadIdx.x == 0 && total > 0) atomicAdd(&n_active[1], total);
+    // bit 1: the tile holds a particle (in any cell, solid ones included);
+    // the staged P2G scatters only such tiles.  bit 2: it holds a solid cell.
+    const int bits = (__syncthreads_or(anyp) ? 2 : 0) | (__syncthreads_or(anys) ? 4 : 0);
+    if (bits && threadIdx.x == 0) tile_occ[t] |= bits;
     if (slot >= 0 && threadIdx.x < 6) {
         int d = threadIdx.x;
         int a = ti, b = tj, c = tk;
@@ -378,7 +384,7 @@ extern "C" int gf_scene_sort_sync(gf_scene_t* s, int with_affine, int* n_active_
                        s->n_active);
     hipLaunchKernelGGL(k_tile_setup, dim3(ntiles), dim3(GF_WG), 0, st, s->nx, s->ny, s->nz,
                        s->tnx, s->tny, s->tnz, s->cell_start, s->marker, s->tile_map,
-                       s->flags, s->nbr, s->n_active);
+                       s->flags, s->nbr, s->n_active, s->tile_occ);
     GF_LAUNCH_CHECK("k_tile_scan/k_tile_setup");
     if (n_active_host) {
         int32_t na[2] = {0, 0};

@@ -27,6 +27,8 @@ struct P2GParams {
     const float* su; const float* sv; const float* sw;
     float* b;
     const int32_t* tile_scene; const float* scene_par;  // packed scenes (NULL: one scene)
+    const int32_t* tile_occ;  // staged P2G: bit 1 = the tile holds a particle, bit 2 = a solid
+    const uint8_t* flags;     // staged P2G finish: compact flags (bit 3 = fluid)
     int nx, ny, nz, tnx, tny, tnz;
     float dx, g, dt, rho;
 };
@@ -478,114 +480,520 @@ __global__ __launch_bounds__(GF_WG) void k_p2g_tile(P2GParams a) {
 }
 
 // ------------------------------------------------ staged P2G (own cells) ----
-// The same arithmetic in two launches.  k_p2g_own: each tile scatters only
-// the particles of its own 512 cells (each particle is visited once, not by
-// every tile whose halo holds it) into the LDS region those particles reach,
-// and stores that region in `stage`.  k_p2g_finish: each tile sums, for every
-// face it finalises, the staged partials of the tiles whose regions hold the
-// face (at most two per axis) in a fixed order, then normalises, adds gravity,
+// Two launches.  k_p2g_cells: each tile that holds particles scatters only
+// the particles of its own 512 cells (each particle is read once, for all
+// three components) into the face region those particles reach, and stores
+// that region in `stage`.  k_p2g_finish: each tile sums, for every face it
+// finalises, the staged partials of the tiles whose regions hold the face
+// (at most two per axis) in a fixed order, then normalises, adds gravity,
 // applies the BC and forms the divergence as k_p2g_tile does.
-// PACKED: tiles read their scene frame from the packed-scene table and the
-// cells go to p2g_component in the scene's own indices; the single-scene
-// instance folds the frame to constants.
-template <bool APIC, bool PACKED>
-__device__ __forceinline__ void p2g_own_body(const P2GParams& a, float* __restrict__ stage,
-                                             int32_t* __restrict__ has) {
-    // one velocity component per workgroup: three workgroups share a tile,
-    // each with a third of the LDS and of the work, consecutive in the
-    // XCD-aware order so that they read the tile's particles through one L2
-    __shared__ float s_val[OWN_REGION];
-    __shared__ float s_wt[OWN_REGION];
-    __shared__ int s_cls[28];
-    __shared__ int s_start[64], s_pref[64];
-    __shared__ int s_wave[GF_WG / GF_WAVE + 1];
+//
+// k_p2g_cells, one workgroup of P2GC_NW waves per tile: wave w takes the
+// i-planes [w*PPW, (w+1)*PPW) of the tile; in a plane lane (lj, lk) =
+// (lane/8, lane%8) owns one cell and runs through the cell's particles (a
+// contiguous run of the sorted arrays), accumulating every component's face
+// stencil in registers.  A particle's position relative to its cell,
+// t = p/dx - cell, is in [0,1] on each axis, so its stencil slots are fixed:
+// on a component's own axis the faces cell and cell+1 with weights (1-t, t)
+// (gridmath.py:71-95, fx = p/dx), on the other axes the faces cell-1, cell,
+// cell+1 with weights (max(0.5-t,0), 1-|t-0.5|, max(t-0.5,0)), which is the
+// reference's floor(f-0.5) corner pair with the unused slot at weight 0.
+// The plane's stencils then go into the wave's own LDS face region: summed
+// along k across lanes by DPP shifts (a cell's k-1/k+1 slots meet its
+// neighbours' overlapping faces), then added along j in lj%3 phases (lj%2 on
+// v's own axis), so no two lanes touch one entry in a phase and every
+// entry's summation order is fixed.  The adds are plain LDS read-modify-
+// writes: a wave's LDS accesses complete in program order, and LDS float
+// atomics cost the kernel twice its time.  Waves then merge their regions in
+// wave order.
+// Particles outside the domain (t off [0,1] by more than P2GC_TOL, or NaN)
+// skip the stencil and are scattered afterwards, one lane at a time, with
+// the reference's own arithmetic (scatter of corners inside the grid).
+// PACKED: tiles read their scene frame from the packed-scene table (dx,
+// cell offset, size); cells are shifted to the scene's own indices.
+constexpr int P2GC_NW = 2;                  // waves per workgroup
+constexpr int P2GC_PPW = 8 / P2GC_NW;       // i-planes per wave
+// One wave's face region: u i in [p0, p0+PPW], j,k in [-1, 8];
+// v i in [p0-1, p0+PPW], j in [0, 8], k in [-1, 8];
+// w i in [p0-1, p0+PPW], j in [-1, 8], k in [0, 8] (tile-relative faces).
+constexpr int P2GC_NU = (P2GC_PPW + 1) * 100;
+constexpr int P2GC_NV = (P2GC_PPW + 2) * 90;
+constexpr int P2GC_NREG = P2GC_NU + 2 * P2GC_NV;
+constexpr float P2GC_TOL = 1.0e-6f;
 
+
+// Slots of component C's cell stencil: NA x NB x NC along x, y, z.
+template <int C> struct Sten {
+    static constexpr int NA = C == 0 ? 2 : 3, NB = C == 1 ? 2 : 3, NC = C == 2 ? 2 : 3;
+};
+
+template <int C>
+__device__ __forceinline__ int p2gc_reg(int ri, int rj, int rk) {
+    if (C == 0) return (ri * 10 + rj) * 10 + rk;
+    if (C == 1) return P2GC_NU + (ri * 9 + rj) * 10 + rk;
+    return P2GC_NU + P2GC_NV + (ri * 10 + rj) * 9 + rk;
+}
+
+// Lanes of one wave pass partial sums to each other through LDS by plain
+// read-modify-writes in phases.  The hardware performs a wave's LDS accesses
+// in program order; this fence (no instruction at wavefront scope) stops the
+// compiler from moving or merging accesses of different phases, which from
+// one lane's point of view are mutually exclusive branches.
+__device__ __forceinline__ void p2gc_phase_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+}
+
+template <int CTRL>
+__device__ __forceinline__ float p2gc_dpp(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+enum { P2GC_ROW_SHL1 = 0x101, P2GC_ROW_SHR1 = 0x111 };
+
+// Per-particle axis data: position relative to the cell, clamped to [0,1].
+struct AxisT {
+    float t[3];
+    bool out;  // outside the domain: scattered by the exact path instead
+};
+
+__device__ __forceinline__ AxisT p2gc_axes(float x, float y, float z, float ihi, float ilo,
+                                           float fci, float fcj, float fck) {
+    AxisT r;
+    // p/dx - cell with the quotient carried in two parts (1/dx = ihi + ilo),
+    // one rounding at the size of t
+    r.t[0] = fmaf(x, ilo, fmaf(x, ihi, -fci));
+    r.t[1] = fmaf(y, ilo, fmaf(y, ihi, -fcj));
+    r.t[2] = fmaf(z, ilo, fmaf(z, ihi, -fck));
+    r.out = false;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        r.out |= !(fabsf(r.t[d] - 0.5f) <= 0.5f + P2GC_TOL);
+        r.t[d] = fminf(fmaxf(r.t[d], 0.0f), 1.0f);
+    }
+    return r;
+}
+
+// Adds one particle to component C's stencil.  o: own-axis weights (already
+// zero for a particle that must not count), g: staggered-axis weights, val:
+// the value the weights multiply, per slot (FLIP: the velocity component).
+template <int C>
+__device__ __forceinline__ void p2gc_add(float (&S)[2][18], const float (&o)[3][2],
+                                         const float (&g)[3][3], float v) {
+    constexpr int A1 = C == 0 ? 1 : 0, A2 = C == 2 ? 1 : 2;  // the staggered axes
+    float P[3][3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int n = 0; n < 3; ++n) P[m][n] = g[A1][m] * g[A2][n];
+    const float vo[2] = {v * o[C][0], v * o[C][1]};
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+#pragma unroll
+            for (int n = 0; n < 3; ++n) {
+                const int slot = C == 0 ? (s * 3 + m) * 3 + n
+                                        : (C == 1 ? (m * 2 + s) * 3 + n : (m * 3 + n) * 2 + s);
+                S[0][slot] = fmaf(vo[s], P[m][n], S[0][slot]);
+                S[1][slot] = fmaf(o[C][s], P[m][n], S[1][slot]);
+            }
+}
+
+// APIC: the slot value is v + c . (face position - particle position)
+// (solver3d.py:1475-1588); fp: the stencil faces' world positions per axis.
+template <int C>
+__device__ __forceinline__ void p2gc_add_apic(float (&S)[2][18], const float (&o)[3][2],
+                                              const float (&g)[3][3], float v, const float* c,
+                                              float px, float py, float pz,
+                                              const float (&fp)[3][3]) {
+    using T = Sten<C>;
+    float ox[3], oy[3], oz[3];
+#pragma unroll
+    for (int a = 0; a < T::NA; ++a) ox[a] = fp[0][a] - px;
+#pragma unroll
+    for (int b = 0; b < T::NB; ++b) oy[b] = fp[1][b] - py;
+#pragma unroll
+    for (int e = 0; e < T::NC; ++e) oz[e] = fp[2][e] - pz;
+    const float* wa = C == 0 ? o[0] : g[0];
+    const float* wb = C == 1 ? o[1] : g[1];
+    const float* wc = C == 2 ? o[2] : g[2];
+#pragma unroll
+    for (int a = 0; a < T::NA; ++a) {
+        const float va = fmaf(c[0], ox[a], v);
+#pragma unroll
+        for (int b = 0; b < T::NB; ++b) {
+            const float wab = wa[a] * wb[b];
+            const float vab = fmaf(c[1], oy[b], va);
+#pragma unroll
+            for (int e = 0; e < T::NC; ++e) {
+                const int slot = (a * T::NB + b) * T::NC + e;
+                const float w = wab * wc[e];
+                S[0][slot] = fmaf(fmaf(c[2], oz[e], vab), w, S[0][slot]);
+                S[1][slot] += w;
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ void p2gc_weights(const AxisT& r, bool keep, float (&o)[3][2],
+                                             float (&g)[3][3]) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const float t = r.t[d];
+        o[d][0] = keep ? 1.0f - t : 0.0f;
+        o[d][1] = keep ? t : 0.0f;
+        const float h = t - 0.5f;
+        g[d][0] = fmaxf(-h, 0.0f);
+        g[d][1] = 1.0f - fabsf(h);
+        g[d][2] = fmaxf(h, 0.0f);
+    }
+}
+
+// Writes the plane's component-C stencils into the wave's region (see the
+// section comment).  cl: the lane's cell in scene indices, sn: scene size.
+template <int C>
+__device__ __forceinline__ void p2gc_flush(float (&S)[2][18], float* reg, int lp, int lj,
+                                           int lk, const int (&cl)[3], const int (&sn)[3]) {
+    using T = Sten<C>;
+    // entries of faces outside the scene's grid are dropped (staggered axes
+    // at the first and last cell; own-axis faces always lie in the grid)
+#pragma unroll
+    for (int a = 0; a < T::NA; ++a)
+#pragma unroll
+        for (int b = 0; b < T::NB; ++b)
+#pragma unroll
+            for (int e = 0; e < T::NC; ++e) {
+                bool drop = false;
+                if (C != 0) drop |= (a == 0 && cl[0] == 0) || (a == 2 && cl[0] == sn[0] - 1);
+                if (C != 1) drop |= (b == 0 && cl[1] == 0) || (b == 2 && cl[1] == sn[1] - 1);
+                if (C != 2) drop |= (e == 0 && cl[2] == 0) || (e == 2 && cl[2] == sn[2] - 1);
+                const int slot = (a * T::NB + b) * T::NC + e;
+                if (drop) {
+                    S[0][slot] = 0.0f;
+                    S[1][slot] = 0.0f;
+                }
+            }
+    // along k: face k (this lane's cell) takes this cell's middle slot, then
+    // the k-1 cell's upper slot, then the k+1 cell's lower slot; the tile's
+    // edge lanes keep the slots of the faces outside the tile's k range
+    constexpr int NAB = T::NA * T::NB;
+    float R[2][NAB], E[2][NAB];
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int ab = 0; ab < NAB; ++ab) {
+            if (T::NC == 3) {
+                float lo = p2gc_dpp<P2GC_ROW_SHR1>(S[q][ab * 3 + 2]);
+                float hi = p2gc_dpp<P2GC_ROW_SHL1>(S[q][ab * 3 + 0]);
+                lo = lk == 0 ? 0.0f : lo;
+                hi = lk == 7 ? 0.0f : hi;
+                R[q][ab] = (S[q][ab * 3 + 1] + lo) + hi;
+                E[q][ab] = lk == 0 ? S[q][ab * 3 + 0] : S[q][ab * 3 + 2];
+            } else {
+                float lo = p2gc_dpp<P2GC_ROW_SHR1>(S[q][ab * 2 + 1]);
+                lo = lk == 0 ? 0.0f : lo;
+                R[q][ab] = S[q][ab * 2 + 0] + lo;
+                E[q][ab] = S[q][ab * 2 + 1];
+            }
+        }
+    const int rk = T::NC == 3 ? lk + 1 : lk;
+    const bool edge = T::NC == 3 ? (lk == 0 || lk == 7) : lk == 7;
+    const int rke = T::NC == 3 ? (lk == 0 ? 0 : 9) : 8;
+    constexpr int NPH = T::NB == 3 ? 3 : 2;
+#pragma unroll
+    for (int ph = 0; ph < NPH; ++ph) {
+        p2gc_phase_fence();
+        if (lj % NPH != ph) continue;
+#pragma unroll
+        for (int a = 0; a < T::NA; ++a)
+#pragma unroll
+            for (int b = 0; b < T::NB; ++b) {
+                const int idx = p2gc_reg<C>(lp + a, lj + b, rk);
+                reg[idx] += R[0][a * T::NB + b];
+                reg[P2GC_NREG + idx] += R[1][a * T::NB + b];
+            }
+        if (edge) {
+#pragma unroll
+            for (int a = 0; a < T::NA; ++a)
+#pragma unroll
+                for (int b = 0; b < T::NB; ++b) {
+                    const int idx = p2gc_reg<C>(lp + a, lj + b, rke);
+                    reg[idx] += E[0][a * T::NB + b];
+                    reg[P2GC_NREG + idx] += E[1][a * T::NB + b];
+                }
+        }
+    }
+    p2gc_phase_fence();
+}
+
+// The reference's scatter (gridmath.py:71-95) of one particle's component C
+// into the wave's region: f = p/dx (IEEE), corners floor(f - 0.5 off-axis),
+// corners outside the scene's grid dropped.  o: the region's origin in scene
+// face indices.
+template <int C, bool APIC>
+__device__ __forceinline__ void p2gc_exact(float* reg, float px, float py, float pz, float vp,
+                                           const float* crow, float dx, const int (&sn)[3],
+                                           const int (&o)[3]) {
+    float cx = px / dx, cy = py / dx, cz = pz / dx;
+    if (C != 0) cx -= 0.5f;
+    if (C != 1) cy -= 0.5f;
+    if (C != 2) cz -= 0.5f;
+    const int i0 = (int)floorf(cx), j0 = (int)floorf(cy), k0 = (int)floorf(cz);
+    const float sx = cx - (float)i0, sy = cy - (float)j0, sz = cz - (float)k0;
+    const int nfi = sn[0] + (C == 0), nfj = sn[1] + (C == 1), nfk = sn[2] + (C == 2);
+    const int RI = C == 0 ? P2GC_PPW + 1 : P2GC_PPW + 2, RJ = C == 1 ? 9 : 10,
+              RK = C == 2 ? 9 : 10;
+#pragma unroll
+    for (int di = 0; di < 2; ++di)
+#pragma unroll
+        for (int dj = 0; dj < 2; ++dj)
+#pragma unroll
+            for (int dk = 0; dk < 2; ++dk) {
+                const int ii = i0 + di, jj = j0 + dj, kk = k0 + dk;
+                if (ii < 0 || ii >= nfi || jj < 0 || jj >= nfj || kk < 0 || kk >= nfk) continue;
+                const int ri = ii - o[0], rj = jj - o[1], rk = kk - o[2];
+                if ((unsigned)ri >= (unsigned)RI || (unsigned)rj >= (unsigned)RJ ||
+                    (unsigned)rk >= (unsigned)RK)
+                    continue;
+                const float wt = (di ? sx : 1.0f - sx) * (dj ? sy : 1.0f - sy) * (dk ? sz : 1.0f - sz);
+                float val = vp;
+                if (APIC) {
+                    const float ox = (C == 0 ? (float)ii : (float)ii + 0.5f) * dx - px;
+                    const float oy = (C == 1 ? (float)jj : (float)jj + 0.5f) * dx - py;
+                    const float oz = (C == 2 ? (float)kk : (float)kk + 0.5f) * dx - pz;
+                    val = vp + crow[0] * ox + crow[1] * oy + crow[2] * oz;
+                }
+                const int idx = p2gc_reg<C>(ri, rj, rk);
+                reg[idx] += val * wt;
+                reg[P2GC_NREG + idx] += wt;
+            }
+}
+
+// One particle's data for the components [C0, C1).
+struct P2GCPart {
+    float x, y, z, v[3], c[3][3];
+};
+
+template <bool APIC, int C0, int C1>
+__device__ __forceinline__ void p2gc_load(const P2GParams& a, int q, P2GCPart& d) {
+    d.x = a.pos[3 * q];
+    d.y = a.pos[3 * q + 1];
+    d.z = a.pos[3 * q + 2];
+#pragma unroll
+    for (int c = C0; c < C1; ++c) {
+        d.v[c] = a.vel[3 * q + c];
+        if (APIC) {
+#pragma unroll
+            for (int e = 0; e < 3; ++e) d.c[c][e] = a.C[9 * q + 3 * c + e];
+        }
+    }
+}
+
+// Accumulates the lane's cell particles [p0, p1) for the components
+// [C0, C1) into S; cur holds particle p0 already.  The next particle is
+// loaded before the current one is added, so its load overlaps the math.
+template <bool APIC, int C0, int C1>
+__device__ __forceinline__ void p2gc_accumulate(const P2GParams& a, float (&S)[3][2][18],
+                                                P2GCPart cur, int p0, int p1, float ihi,
+                                                float ilo, const int (&cl)[3], float dx,
+                                                bool& anyout) {
+    const float fci = (float)cl[0], fcj = (float)cl[1], fck = (float)cl[2];
+    // APIC: world positions of the stencil faces per axis (own axis of
+    // component C: cell, cell+1; others: cell-1, cell, cell+1, at +0.5)
+    float fp[3][3][3];
+    if (APIC) {
+#pragma unroll
+        for (int c = C0; c < C1; ++c)
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+#pragma unroll
+                for (int m = 0; m < 3; ++m)
+                    fp[c][d][m] = d == c ? (float)(cl[d] + m) * dx
+                                         : ((float)(cl[d] - 1 + m) + 0.5f) * dx;
+    }
+    for (int p = p0; p < p1; ++p) {
+        P2GCPart nxt;
+        p2gc_load<APIC, C0, C1>(a, p + 1 < p1 ? p + 1 : p, nxt);
+        const AxisT r = p2gc_axes(cur.x, cur.y, cur.z, ihi, ilo, fci, fcj, fck);
+        anyout |= r.out;
+        float o[3][2], g[3][3];
+        p2gc_weights(r, !r.out, o, g);
+#pragma unroll
+        for (int c = C0; c < C1; ++c) {
+            if (APIC) {
+                if (c == 0) p2gc_add_apic<0>(S[0], o, g, cur.v[0], cur.c[0], cur.x, cur.y, cur.z, fp[0]);
+                if (c == 1) p2gc_add_apic<1>(S[1], o, g, cur.v[1], cur.c[1], cur.x, cur.y, cur.z, fp[1]);
+                if (c == 2) p2gc_add_apic<2>(S[2], o, g, cur.v[2], cur.c[2], cur.x, cur.y, cur.z, fp[2]);
+            } else {
+                if (c == 0) p2gc_add<0>(S[0], o, g, cur.v[0]);
+                if (c == 1) p2gc_add<1>(S[1], o, g, cur.v[1]);
+                if (c == 2) p2gc_add<2>(S[2], o, g, cur.v[2]);
+            }
+        }
+        cur = nxt;
+    }
+}
+
+// Particles outside the domain among the lane's [p0, p1), one lane at a time
+// in lane order, with the reference's arithmetic.
+template <bool APIC, int C0, int C1>
+__device__ __forceinline__ void p2gc_outside(const P2GParams& a, float* reg, bool anyout, int p0,
+                                             int p1, float ihi, float ilo, const int (&cl)[3],
+                                             const int (&sn)[3], float dx,
+                                             const int (&org)[3]) {
+    const float fci = (float)cl[0], fcj = (float)cl[1], fck = (float)cl[2];
+    unsigned long long m = __ballot(anyout);
+    while (m) {
+        const int L = __ffsll((long long)m) - 1;
+        m &= m - 1;
+        p2gc_phase_fence();
+        if ((int)(threadIdx.x & 63) != L) continue;
+        for (int p = p0; p < p1; ++p) {
+            const float px = a.pos[3 * p], py = a.pos[3 * p + 1], pz = a.pos[3 * p + 2];
+            if (!p2gc_axes(px, py, pz, ihi, ilo, fci, fcj, fck).out) continue;
+#pragma unroll
+            for (int c = C0; c < C1; ++c) {
+                float crow[3] = {0.0f, 0.0f, 0.0f};
+                if (APIC)
+                    for (int e = 0; e < 3; ++e) crow[e] = a.C[9 * p + 3 * c + e];
+                const float vp = a.vel[3 * p + c];
+                // region origin of component c in scene face indices
+                const int oc[3] = {org[0] - (c != 0), org[1] - (c != 1), org[2] - (c != 2)};
+                if (c == 0) p2gc_exact<0, APIC>(reg, px, py, pz, vp, crow, dx, sn, oc);
+                if (c == 1) p2gc_exact<1, APIC>(reg, px, py, pz, vp, crow, dx, sn, oc);
+                if (c == 2) p2gc_exact<2, APIC>(reg, px, py, pz, vp, crow, dx, sn, oc);
+            }
+        }
+    }
+}
+
+template <bool APIC, bool PACKED>
+__device__ __forceinline__ void p2g_cells_body(const P2GParams& a, float* __restrict__ stage,
+                                               int32_t* __restrict__ has) {
+    __shared__ float s_reg[P2GC_NW][2 * P2GC_NREG];
     const int ntiles = a.tnx * a.tny * a.tnz;
-    const int lb = gf::xcd_remap(blockIdx.x, 3 * ntiles);
-    const int t = lb / 3, comp = lb - 3 * t;
+    const int t = gf::xcd_remap(blockIdx.x, ntiles);
+    const int occ = a.tile_occ[t];
+    if (threadIdx.x == 0) has[t] = (occ >> 1) & 1;
+    if (!(occ & 2)) return;
     const int tk = t % a.tnz, tj = (t / a.tnz) % a.tny, ti = t / (a.tnz * a.tny);
     const int i0 = ti * 8, j0 = tj * 8, k0 = tk * 8;
     const int nx = a.nx, ny = a.ny, nz = a.nz;
-    const int ni = min(i0 + 8, nx) - i0, nj = min(j0 + 8, ny) - j0, nk = min(k0 + 8, nz) - k0;
-    if (threadIdx.x == 0) {
-        int acc = 0;
-        for (int r = 0; r < 27; ++r) {
-            const int ri = r / 9, rj = (r / 3) % 3, rk = r % 3;
-            s_cls[r] = acc;
-            acc += max(0, (ni - ri + 2) / 3) * max(0, (nj - rj + 2) / 3) * max(0, (nk - rk + 2) / 3);
-        }
-        s_cls[27] = acc;
-    }
-    const RowSet rs = build_rows(a.cell_start, ny, nz, i0, i0 + ni, j0, j0 + nj, k0, k0 + nk,
-                                 s_start, s_pref, s_wave);
-    if (threadIdx.x == 0 && comp == 0) has[t] = rs.total > 0 ? 1 : 0;
-    gf::Frame F =
+    const gf::Frame F =
         gf::tile_frame(PACKED ? a.tile_scene : nullptr, a.scene_par, t, a.dx, nx, ny, nz);
-    if (rs.total == 0) return;
-    // scene-local tile origin; the packed-grid offset only enters the
-    // cell_start key
-    const int ox = F.ox, oy = F.oy, oz = F.oz;
-    const int li0 = i0 - ox, lj0 = j0 - oy, lk0 = k0 - oz;
-    F.ox = F.oy = F.oz = 0;
-    for (int i = threadIdx.x; i < OWN_REGION; i += GF_WG) {
-        s_val[i] = 0.0f;
-        s_wt[i] = 0.0f;
+    const float dx = F.dx;
+    const float ihi = 1.0f / dx;
+    const float ilo = fmaf(-ihi, dx, 1.0f) * ihi;
+    const int sn[3] = {F.nx, F.ny, F.nz};
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int lj = lane >> 3, lk = lane & 7;
+    float* reg = s_reg[wv];
+    for (int e = lane; e < 2 * P2GC_NREG; e += 64) reg[e] = 0.0f;
+    // region origin (u faces; v/w one lower on their staggered axes) in
+    // scene face indices
+    const int org[3] = {i0 - F.ox + wv * P2GC_PPW, j0 - F.oy, k0 - F.oz};
+    const int cj = j0 + lj, ck = k0 + lk;
+    auto range = [&](int lp, int& q0, int& q1) {
+        const int ci = i0 + wv * P2GC_PPW + lp;
+        q0 = q1 = 0;
+        if (ci < nx && cj < ny && ck < nz) {
+            const int key = (ci * ny + cj) * nz + ck;
+            q0 = a.cell_start[key];
+            q1 = a.cell_start[key + 1];
+        }
+    };
+    int p0, p1;
+    range(0, p0, p1);
+    P2GCPart cur;
+    if (!APIC && p0 < p1) p2gc_load<false, 0, 3>(a, p0, cur);
+    for (int lp = 0; lp < P2GC_PPW; ++lp) {
+        int n0 = 0, n1 = 0;
+        if (lp + 1 < P2GC_PPW) range(lp + 1, n0, n1);
+        const int cl[3] = {i0 + wv * P2GC_PPW + lp - F.ox, cj - F.oy, ck - F.oz};
+        float S[3][2][18];
+        bool anyout = false;
+        if (APIC) {
+            // one component per pass keeps the affine rows in registers
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+#pragma unroll
+                for (int q = 0; q < 18; ++q) S[c][0][q] = S[c][1][q] = 0.0f;
+                P2GCPart first;
+                bool out_c = false;
+                if (c == 0) {
+                    if (p0 < p1) p2gc_load<true, 0, 1>(a, p0, first);
+                    p2gc_accumulate<true, 0, 1>(a, S, first, p0, p1, ihi, ilo, cl, dx, out_c);
+                    p2gc_flush<0>(S[0], reg, lp, lj, lk, cl, sn);
+                    p2gc_outside<true, 0, 1>(a, reg, out_c, p0, p1, ihi, ilo, cl, sn, dx, org);
+                } else if (c == 1) {
+                    if (p0 < p1) p2gc_load<true, 1, 2>(a, p0, first);
+                    p2gc_accumulate<true, 1, 2>(a, S, first, p0, p1, ihi, ilo, cl, dx, out_c);
+                    p2gc_flush<1>(S[1], reg, lp, lj, lk, cl, sn);
+                    p2gc_outside<true, 1, 2>(a, reg, out_c, p0, p1, ihi, ilo, cl, sn, dx, org);
+                } else {
+                    if (p0 < p1) p2gc_load<true, 2, 3>(a, p0, first);
+                    p2gc_accumulate<true, 2, 3>(a, S, first, p0, p1, ihi, ilo, cl, dx, out_c);
+                    p2gc_flush<2>(S[2], reg, lp, lj, lk, cl, sn);
+                    p2gc_outside<true, 2, 3>(a, reg, out_c, p0, p1, ihi, ilo, cl, sn, dx, org);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int q = 0; q < 18; ++q) S[c][0][q] = S[c][1][q] = 0.0f;
+            p2gc_accumulate<false, 0, 3>(a, S, cur, p0, p1, ihi, ilo, cl, dx, anyout);
+            // the next plane's first particle is loaded while this plane's
+            // stencils are written out
+            if (lp + 1 < P2GC_PPW && n0 < n1) p2gc_load<false, 0, 3>(a, n0, cur);
+            p2gc_flush<0>(S[0], reg, lp, lj, lk, cl, sn);
+            p2gc_flush<1>(S[1], reg, lp, lj, lk, cl, sn);
+            p2gc_flush<2>(S[2], reg, lp, lj, lk, cl, sn);
+            p2gc_outside<false, 0, 3>(a, reg, anyout, p0, p1, ihi, ilo, cl, sn, dx, org);
+        }
+        p0 = n0;
+        p1 = n1;
     }
     __syncthreads();
-    const int ncell = ni * nj * nk;
-    const int nrounds = (ncell + GF_WG - 1) / GF_WG;
-    for (int round = 0; round < nrounds; ++round) {
-        const int c = round * GF_WG + threadIdx.x;
-        int ci = 0, cj = 0, ck = 0, cls = -1, p0 = 0, p1 = 0;
-        if (c < ncell) {
-            int r = 0;
-            while (r < 26 && s_cls[r + 1] <= c) ++r;
-            const int ri = r / 9, rj = (r / 3) % 3, rk = r % 3;
-            const int mj = max(0, (nj - rj + 2) / 3), mk = max(0, (nk - rk + 2) / 3);
-            const int loc = c - s_cls[r];
-            ck = lk0 + rk + 3 * (loc % mk);
-            cj = lj0 + rj + 3 * ((loc / mk) % mj);
-            ci = li0 + ri + 3 * (loc / (mk * mj));
-            const int key = ((ci + ox) * ny + cj + oy) * nz + ck + oz;
-            p0 = a.cell_start[key];
-            p1 = a.cell_start[key + 1];
-            if (p0 < p1) cls = r;
+    // merge the waves' regions (in wave order) into the tile's staged region
+    float* st = stage + (size_t)t * STAGE_PER_TILE;
+    for (int e = threadIdx.x; e < 3 * OWN_REGION; e += 64 * P2GC_NW) {
+        const int c = e / OWN_REGION, r = e - c * OWN_REGION;
+        const int plane = c == 0 ? 100 : 90;
+        const int si = r / plane, rest = r - si * plane;
+        const int base = c == 0 ? 0 : (c == 1 ? P2GC_NU : P2GC_NU + P2GC_NV);
+        const int rmax = c == 0 ? P2GC_PPW : P2GC_PPW + 1;
+        float sv = 0.0f, sw = 0.0f;
+#pragma unroll
+        for (int w = 0; w < P2GC_NW; ++w) {
+            const int ri = si - w * P2GC_PPW;
+            if (ri >= 0 && ri <= rmax) {
+                const int idx = base + ri * plane + rest;
+                sv += s_reg[w][idx];
+                sw += s_reg[w][P2GC_NREG + idx];
+            }
         }
-        const int last = min(ncell, (round + 1) * GF_WG) - 1;
-        int r_lo = 0, r_hi = 0;
-        while (r_lo < 26 && s_cls[r_lo + 1] <= round * GF_WG) ++r_lo;
-        while (r_hi < 26 && s_cls[r_hi + 1] <= last) ++r_hi;
-        if (comp == 0)
-            p2g_component<0, APIC, OwnDims<0>>(a, F, s_val, s_wt, ci, cj, ck, p0, p1, cls, r_lo,
-                                               r_hi, li0, lj0 - 1, lk0 - 1);
-        else if (comp == 1)
-            p2g_component<1, APIC, OwnDims<1>>(a, F, s_val, s_wt, ci, cj, ck, p0, p1, cls, r_lo,
-                                               r_hi, li0 - 1, lj0, lk0 - 1);
-        else
-            p2g_component<2, APIC, OwnDims<2>>(a, F, s_val, s_wt, ci, cj, ck, p0, p1, cls, r_lo,
-                                               r_hi, li0 - 1, lj0 - 1, lk0);
-    }
-    float* st = stage + (size_t)t * STAGE_PER_TILE + (size_t)comp * 2 * OWN_REGION;
-    for (int e = threadIdx.x; e < OWN_REGION; e += GF_WG) {
-        st[e] = s_val[e];
-        st[OWN_REGION + e] = s_wt[e];
+        st[c * 2 * OWN_REGION + r] = sv;
+        st[c * 2 * OWN_REGION + OWN_REGION + r] = sw;
     }
 }
 
-// FLIP fits 96 VGPRs (5 waves per SIMD) without spills in both instances;
-// APIC carries the affine rows and runs at 4.
 template <bool PACKED>
-__global__ __launch_bounds__(GF_WG) __attribute__((amdgpu_waves_per_eu(5, 8))) void
-k_p2g_own_flip(P2GParams a, float* __restrict__ stage, int32_t* __restrict__ has) {
-    p2g_own_body<false, PACKED>(a, stage, has);
+__global__ __launch_bounds__(64 * P2GC_NW) void k_p2g_cells_flip(P2GParams a,
+                                                                  float* __restrict__ stage,
+                                                                  int32_t* __restrict__ has) {
+    p2g_cells_body<false, PACKED>(a, stage, has);
 }
 
 template <bool PACKED>
-__global__ __launch_bounds__(GF_WG) void k_p2g_own_apic(P2GParams a, float* __restrict__ stage,
-                                                        int32_t* __restrict__ has) {
-    p2g_own_body<true, PACKED>(a, stage, has);
+__global__ __launch_bounds__(64 * P2GC_NW) void k_p2g_cells_apic(P2GParams a,
+                                                                  float* __restrict__ stage,
+                                                                  int32_t* __restrict__ has) {
+    p2g_cells_body<true, PACKED>(a, stage, has);
 }
 
+// The marker box is loaded only when a neighbouring tile holds a solid cell
+// (tile_occ bit 2); otherwise the BC applies at the grid boundary alone.  The
+// divergence takes the fluid test from the compact flags (bit 3 = marker 1).
 __global__ __launch_bounds__(GF_WG) void k_p2g_finish(P2GParams a,
                                                       const float* __restrict__ stage,
                                                       const int32_t* __restrict__ has) {
@@ -598,24 +1006,31 @@ __global__ __launch_bounds__(GF_WG) void k_p2g_finish(P2GParams a,
     const int tk = t % a.tnz, tj = (t / a.tnz) % a.tny, ti = t / (a.tnz * a.tny);
     const int i0 = ti * 8, j0 = tj * 8, k0 = tk * 8;
     const int nx = a.nx, ny = a.ny, nz = a.nz;
-    for (int i = threadIdx.x; i < 1000; i += GF_WG) {
-        const int ca = i / 100, cb = (i / 10) % 10, cc = i % 10;
-        const int ci = i0 - 1 + ca, cj = j0 - 1 + cb, ck = k0 - 1 + cc;
-        int8_t m = 0;
-        if (ci >= 0 && ci < nx && cj >= 0 && cj < ny && ck >= 0 && ck < nz)
-            m = (int8_t)a.marker[(ci * ny + cj) * nz + ck];
-        s_mk[ca][cb][cc] = m;
-    }
+    int solid_nb = 0;
     if (threadIdx.x < 27) {
         const int di = (int)threadIdx.x / 9 - 1, dj = ((int)threadIdx.x / 3) % 3 - 1,
                   dk = (int)threadIdx.x % 3 - 1;
         const int a_ = ti + di, b_ = tj + dj, c_ = tk + dk;
-        s_has[threadIdx.x] = (a_ >= 0 && a_ < a.tnx && b_ >= 0 && b_ < a.tny && c_ >= 0 &&
-                              c_ < a.tnz)
-                                 ? has[(a_ * a.tny + b_) * a.tnz + c_]
-                                 : 0;
+        int h = 0;
+        if (a_ >= 0 && a_ < a.tnx && b_ >= 0 && b_ < a.tny && c_ >= 0 && c_ < a.tnz) {
+            const int q = (a_ * a.tny + b_) * a.tnz + c_;
+            h = has[q];
+            solid_nb = (a.tile_occ[q] >> 2) & 1;
+        }
+        s_has[threadIdx.x] = h;
     }
-    __syncthreads();
+    const bool any_solid = __syncthreads_or(solid_nb);
+    if (any_solid) {
+        for (int i = threadIdx.x; i < 1000; i += GF_WG) {
+            const int ca = i / 100, cb = (i / 10) % 10, cc = i % 10;
+            const int ci = i0 - 1 + ca, cj = j0 - 1 + cb, ck = k0 - 1 + cc;
+            int8_t m = 0;
+            if (ci >= 0 && ci < nx && cj >= 0 && cj < ny && ck >= 0 && ck < nz)
+                m = (int8_t)a.marker[(ci * ny + cj) * nz + ck];
+            s_mk[ca][cb][cc] = m;
+        }
+        __syncthreads();
+    }
     bool any_has = false;
     for (int q = 0; q < 27; ++q) any_has |= s_has[q] != 0;
 
@@ -677,16 +1092,16 @@ __global__ __launch_bounds__(GF_WG) void k_p2g_finish(P2GParams a,
             bool lo_side, hi_side;
             const float* solid;
             if (c == 0) {
-                lo_side = ii == 0 || s_mk[ca - 1][cb][cc] == 2;
-                hi_side = ii == nx || s_mk[ca][cb][cc] == 2;
+                lo_side = ii == 0 || (any_solid && s_mk[ca - 1][cb][cc] == 2);
+                hi_side = ii == nx || (any_solid && s_mk[ca][cb][cc] == 2);
                 solid = a.su;
             } else if (c == 1) {
-                lo_side = jj == 0 || s_mk[ca][cb - 1][cc] == 2;
-                hi_side = jj == ny || s_mk[ca][cb][cc] == 2;
+                lo_side = jj == 0 || (any_solid && s_mk[ca][cb - 1][cc] == 2);
+                hi_side = jj == ny || (any_solid && s_mk[ca][cb][cc] == 2);
                 solid = a.sv;
             } else {
-                lo_side = kk == 0 || s_mk[ca][cb][cc - 1] == 2;
-                hi_side = kk == nz || s_mk[ca][cb][cc] == 2;
+                lo_side = kk == 0 || (any_solid && s_mk[ca][cb][cc - 1] == 2);
+                hi_side = kk == nz || (any_solid && s_mk[ca][cb][cc] == 2);
                 solid = a.sw;
             }
             if (lo_side || hi_side) val = solid ? solid[f] : 0.0f;
@@ -698,10 +1113,10 @@ __global__ __launch_bounds__(GF_WG) void k_p2g_finish(P2GParams a,
             }
         }
     }
-    __syncthreads();
-
     const int slot = a.tile_map[t];
     if (slot < 0) return;
+    __syncthreads();
+
     const float scale = a.rho * F.dx / a.dt;
     const int lj = (threadIdx.x >> 3) & 7, lk = threadIdx.x & 7;
 #pragma unroll
@@ -709,7 +1124,8 @@ __global__ __launch_bounds__(GF_WG) void k_p2g_finish(P2GParams a,
         const int li = (threadIdx.x >> 6) + 4 * h;
         const int l = (li * 8 + lj) * 8 + lk;
         float dv = 0.0f;
-        if (i0 + li < nx && j0 + lj < ny && k0 + lk < nz && s_mk[li + 1][lj + 1][lk + 1] == 1) {
+        if (i0 + li < nx && j0 + lj < ny && k0 + lk < nz &&
+            (a.flags[(size_t)slot * GF_TILE_CELLS + l] & 8)) {
             const float u0 = s_val[0][(li * 8 + lj) * 8 + lk];
             const float u1 = s_val[0][((li + 1) * 8 + lj) * 8 + lk];
             const float v0 = s_val[1][(li * 9 + lj) * 8 + lk];
@@ -1128,6 +1544,8 @@ P2GParams p2g_params(const gf_scene_t* s, float g, float dt, float rho) {
     const bool packed = s->n_scenes > 1;
     a.tile_scene = packed ? s->tile_scene : nullptr;
     a.scene_par = packed ? s->scene_par : nullptr;
+    a.tile_occ = s->tile_occ;
+    a.flags = s->flags;
     a.nx = s->nx; a.ny = s->ny; a.nz = s->nz; a.tnx = s->tnx; a.tny = s->tny; a.tnz = s->tnz;
     a.dx = s->dx; a.g = g; a.dt = dt; a.rho = rho;
     return a;
@@ -1183,7 +1601,7 @@ extern "C" int gf_scene_p2g_staged(const gf_scene_t* s, int transfer_mode, float
     GF_REQUIRE(transfer_mode == GF_TRANSFER_FLIP || transfer_mode == GF_TRANSFER_APIC,
                "gf_scene_p2g_staged: transfer_mode %d", transfer_mode);
     GF_REQUIRE(s->u && s->v && s->w && s->us && s->vs && s->ws && s->b && s->marker &&
-                   s->cell_start && s->tile_map && stage && has,
+                   s->cell_start && s->tile_map && s->tile_occ && s->flags && stage && has,
                "gf_scene_p2g_staged: buffers not set");
     GF_REQUIRE(dt != 0.0f, "gf_scene_p2g_staged: dt must be nonzero");
     if (s->n > 0) GF_REQUIRE(s->pos && s->vel, "gf_scene_p2g_staged: particles not set");
@@ -1193,11 +1611,11 @@ extern "C" int gf_scene_p2g_staged(const gf_scene_t* s, int transfer_mode, float
     const int ntiles = s->tnx * s->tny * s->tnz;
     hipStream_t st = (hipStream_t)stream;
     const bool apic = transfer_mode == GF_TRANSFER_APIC, packed = a.tile_scene != nullptr;
-    auto own = apic ? (packed ? k_p2g_own_apic<true> : k_p2g_own_apic<false>)
-                    : (packed ? k_p2g_own_flip<true> : k_p2g_own_flip<false>);
-    hipLaunchKernelGGL(own, dim3(3 * ntiles), dim3(GF_WG), 0, st, a, stage, has);
+    auto own = apic ? (packed ? k_p2g_cells_apic<true> : k_p2g_cells_apic<false>)
+                    : (packed ? k_p2g_cells_flip<true> : k_p2g_cells_flip<false>);
+    hipLaunchKernelGGL(own, dim3(ntiles), dim3(64 * P2GC_NW), 0, st, a, stage, has);
     hipLaunchKernelGGL(k_p2g_finish, dim3(ntiles), dim3(GF_WG), 0, st, a, stage, has);
-    GF_LAUNCH_CHECK("k_p2g_own/k_p2g_finish");
+    GF_LAUNCH_CHECK("k_p2g_cells/k_p2g_finish");
     return 0;
 }
 

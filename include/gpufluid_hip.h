@@ -115,7 +115,7 @@ typedef struct gf_scene {
     int32_t* cell_start;         /* (C+1) exclusive prefix of counts */
     int32_t* scan_part;          /* (gf_scene_sizes.scan_parts) */
     /* tiles */
-    int32_t* tile_occ;           /* (ntiles) 1 if the tile holds a fluid cell */
+    int32_t* tile_occ;           /* (ntiles) bit 0: the tile holds a fluid cell; 1: a particle; 2: a solid cell */
     int32_t* tile_map;           /* (ntiles) compact slot or -1 */
     int32_t* active;             /* (ntiles) tile id per compact slot */
     int32_t* n_active;           /* (4) [0] active tiles, [1] fluid cells */
@@ -193,7 +193,8 @@ int gf_scene_p2g(const gf_scene_t* s, int transfer_mode, float gravity, float dt
  * particles (each particle once) into the region they reach and stores it in
  * `stage`; then each tile sums the staged partials of the tiles holding each
  * of its faces (at most two per axis, fixed order) and finalises as
- * gf_scene_p2g.  stage / has: gf_scene_p2g_stage_size floats / int32. */
+ * gf_scene_p2g.  stage / has: gf_scene_p2g_stage_size floats / int32.
+ * Reads s->tile_occ as the last gf_scene_sort_sync left it. */
 int gf_scene_p2g_stage_size(int nx, int ny, int nz, int64_t* stage_floats, int64_t* flag_ints);
 int gf_scene_p2g_staged(const gf_scene_t* s, int transfer_mode, float gravity, float dt,
                         float rho, float* stage, int32_t* has, void* stream);
