@@ -12,6 +12,10 @@
 namespace {
 
 constexpr int SCAN_ITEMS = 16;
+// k_bin / k_slot: each wave takes SORT_SUB consecutive groups of 64
+// particles (one per lane each), with all loads of the groups issued before
+// the first is used.
+constexpr int SORT_SUB = 4;
 constexpr int SCAN_CHUNK = GF_WG * SCAN_ITEMS;  // 4096
 
 __global__ void k_fill_i32(int32_t* a, int n, int32_t v) {
@@ -53,36 +57,49 @@ __global__ __launch_bounds__(GF_WG) void k_bin(int n, const float* __restrict__ 
                                                const int32_t* __restrict__ scene_pstart,
                                                const float* __restrict__ scene_par,
                                                int n_scenes) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    float x = pos[3 * p + 0], y = pos[3 * p + 1], z = pos[3 * p + 2];
-    int ci, cj, ck;
-    if (scene_pstart) {
-        const int id = pid[p];
-        int lo = 0, hi = n_scenes - 1;  // the last scene whose first id is <= id
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (scene_pstart[mid] <= id) lo = mid;
-            else hi = mid - 1;
-        }
-        const float* sp = scene_par + gf::SCENE_PAR * lo;
-        const float sdx = sp[7];
-        const int ns = (int)sp[11];
-        ci = (int)sp[8] + gf::bin_axis(x, sdx, ns);
-        cj = (int)sp[9] + gf::bin_axis(y, sdx, ns);
-        ck = (int)sp[10] + gf::bin_axis(z, sdx, ns);
-    } else {
-        ci = gf::bin_axis(x, dx, nx);
-        cj = gf::bin_axis(y, dx, ny);
-        ck = gf::bin_axis(z, dx, nz);
+    const int lane = threadIdx.x & 63;
+    const int base = (blockIdx.x * (GF_WG / 64) + (threadIdx.x >> 6)) * (64 * SORT_SUB) + lane;
+    float x[SORT_SUB], y[SORT_SUB], z[SORT_SUB];
+    int id[SORT_SUB];
+#pragma unroll
+    for (int j = 0; j < SORT_SUB; ++j) {
+        const int p = min(base + 64 * j, n - 1);
+        x[j] = pos[3 * p + 0];
+        y[j] = pos[3 * p + 1];
+        z[j] = pos[3 * p + 2];
+        if (scene_pstart) id[j] = pid[p];
     }
-    int key = (ci * ny + cj) * nz + ck;
-    keys[p] = key;
-    int lead, len;
-    run_of(key, lead, len);
-    if (len > 0) {
-        atomicAdd(&counts[key], len);
-        if (marker[key] != 2) tile_occ[((ci >> 3) * tny + (cj >> 3)) * tnz + (ck >> 3)] = 1;
+#pragma unroll
+    for (int j = 0; j < SORT_SUB; ++j) {
+        const int p = base + 64 * j;
+        if (p >= n) break;  // lanes past the end stay out of run_of
+        int ci, cj, ck;
+        if (scene_pstart) {
+            int lo = 0, hi = n_scenes - 1;  // the last scene whose first id is <= id
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (scene_pstart[mid] <= id[j]) lo = mid;
+                else hi = mid - 1;
+            }
+            const float* sp = scene_par + gf::SCENE_PAR * lo;
+            const float sdx = sp[7];
+            const int ns = (int)sp[11];
+            ci = (int)sp[8] + gf::bin_axis(x[j], sdx, ns);
+            cj = (int)sp[9] + gf::bin_axis(y[j], sdx, ns);
+            ck = (int)sp[10] + gf::bin_axis(z[j], sdx, ns);
+        } else {
+            ci = gf::bin_axis(x[j], dx, nx);
+            cj = gf::bin_axis(y[j], dx, ny);
+            ck = gf::bin_axis(z[j], dx, nz);
+        }
+        const int key = (ci * ny + cj) * nz + ck;
+        keys[p] = key;
+        int lead, len;
+        run_of(key, lead, len);
+        if (len > 0) {
+            atomicAdd(&counts[key], len);
+            if (marker[key] != 2) tile_occ[((ci >> 3) * tny + (cj >> 3)) * tnz + (ck >> 3)] = 1;
+        }
     }
 }
 
@@ -151,20 +168,41 @@ __global__ __launch_bounds__(GF_WG) void k_scan_apply(const int32_t* __restrict_
     if (blockIdx.x == nb - 1 && threadIdx.x == 0) out[n] = part[nb];
 }
 
+// A cell whose members all sit in one run of one wave (the common case:
+// particles stay in their cell between substeps) needs no slot: k_rank ranks
+// them by lane, and the run's leader returns the cell's count to 0.  Other
+// cells' members take slots with run-aggregated atomicSub (which also returns
+// the count to 0), in no particular order; k_rank orders them.
 __global__ __launch_bounds__(GF_WG) void k_slot(int n, const int32_t* __restrict__ keys,
                                                 int32_t* __restrict__ counts,
                                                 const int32_t* __restrict__ cell_start,
                                                 int32_t* __restrict__ tmp) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    int key = keys[p];
-    int lead, len;
-    run_of(key, lead, len);
-    int old = 0;
-    if (len > 0) old = atomicSub(&counts[key], len);
-    old = __shfl(old, lead, 64);
     const int lane = threadIdx.x & 63;
-    tmp[cell_start[key] + old - 1 - (lane - lead)] = p;
+    const int base = (blockIdx.x * (GF_WG / 64) + (threadIdx.x >> 6)) * (64 * SORT_SUB) + lane;
+    int key[SORT_SUB], s[SORT_SUB], e[SORT_SUB];
+#pragma unroll
+    for (int j = 0; j < SORT_SUB; ++j) key[j] = keys[min(base + 64 * j, n - 1)];
+#pragma unroll
+    for (int j = 0; j < SORT_SUB; ++j) {
+        s[j] = cell_start[key[j]];
+        e[j] = cell_start[key[j] + 1];
+    }
+#pragma unroll
+    for (int j = 0; j < SORT_SUB; ++j) {
+        const int p = base + 64 * j;
+        if (p >= n) break;  // lanes past the end stay out of run_of
+        int lead, len;
+        run_of(key[j], lead, len);
+        const int run_len = __shfl(len, lead, 64);
+        if (run_len == e[j] - s[j]) {
+            if (len > 0) counts[key[j]] = 0;
+            continue;
+        }
+        int old = 0;
+        if (len > 0) old = atomicSub(&counts[key[j]], len);
+        old = __shfl(old, lead, 64);
+        tmp[s[j] + old - 1 - (lane - lead)] = p;
+    }
 }
 
 // Rank each particle among the members of its cell by original index (a stable
@@ -205,10 +243,12 @@ __global__ __launch_bounds__(GF_WG) void k_rank(
     }
 }
 
-// One workgroup: compact slot per occupied tile (in tile-id order).
 // Compact slots of the occupied tiles, in tile order or, for packed scenes,
 // in the scene-major order of `order` (so that each scene's slots are
 // contiguous; scene_slot[s] = its first slot, scene_slot[n_scenes] = total).
+// One workgroup; each thread takes TILE_SCAN_ITEMS consecutive positions of a
+// chunk, so a chunk costs one workgroup scan.
+constexpr int TILE_SCAN_ITEMS = 16;
 __global__ __launch_bounds__(1024) void k_tile_scan(const int32_t* __restrict__ occ,
                                                     int ntiles,
                                                     const int32_t* __restrict__ order,
@@ -219,20 +259,41 @@ __global__ __launch_bounds__(1024) void k_tile_scan(const int32_t* __restrict__ 
                                                     int32_t* __restrict__ active,
                                                     int32_t* __restrict__ n_active) {
     __shared__ int s_wave[1024 / GF_WAVE + 1];
+    constexpr int CHUNK = 1024 * TILE_SCAN_ITEMS;
     int carry = 0;
-    for (int base = 0; base < ntiles; base += blockDim.x) {
-        const int pos = base + threadIdx.x;
-        const int t = pos < ntiles ? (order ? order[pos] : pos) : 0;
-        int o = (pos < ntiles && (occ[t] & 1)) ? 1 : 0;
+    for (int base = 0; base < ntiles; base += CHUNK) {
+        const int p0 = base + threadIdx.x * TILE_SCAN_ITEMS;
+        int tl[TILE_SCAN_ITEMS], o[TILE_SCAN_ITEMS];
+        int cnt = 0;
+        // every load of the thread's items is issued before the first use
+        // (positions past the end read the last tile and count as empty)
+#pragma unroll
+        for (int i = 0; i < TILE_SCAN_ITEMS; ++i) {
+            const int pos = min(p0 + i, ntiles - 1);
+            tl[i] = order ? order[pos] : pos;
+        }
+#pragma unroll
+        for (int i = 0; i < TILE_SCAN_ITEMS; ++i) o[i] = occ[tl[i]];
+#pragma unroll
+        for (int i = 0; i < TILE_SCAN_ITEMS; ++i) {
+            o[i] = (p0 + i < ntiles && (o[i] & 1)) ? 1 : 0;
+            cnt += o[i];
+        }
         int total;
-        int ex = gf::wg_exclusive_scan(o, s_wave, &total);
-        if (pos < ntiles) {
-            tile_map[t] = o ? carry + ex : -1;
-            if (o) active[carry + ex] = t;
-            if (scene_slot) {
-                const int sc = tile_scene[t];
-                const int prev = pos > 0 ? tile_scene[order[pos - 1]] : -1;
-                if (sc >= 0 && sc != prev) scene_slot[sc] = carry + ex;
+        int ex = carry + gf::wg_exclusive_scan(cnt, s_wave, &total);
+#pragma unroll
+        for (int i = 0; i < TILE_SCAN_ITEMS; ++i) {
+            const int pos = p0 + i;
+            if (pos < ntiles) {
+                const int t = tl[i];
+                tile_map[t] = o[i] ? ex : -1;
+                if (o[i]) active[ex] = t;
+                if (scene_slot) {
+                    const int sc = tile_scene[t];
+                    const int prev = pos > 0 ? tile_scene[order[pos - 1]] : -1;
+                    if (sc >= 0 && sc != prev) scene_slot[sc] = ex;
+                }
+                ex += o[i];
             }
         }
         carry += total;
@@ -246,6 +307,8 @@ __global__ __launch_bounds__(1024) void k_tile_scan(const int32_t* __restrict__ 
 
 // Per tile: new marker (solid kept, fluid where a particle sits), compact
 // flags (fluid bit + non-solid neighbour count) and the neighbour slot table.
+// Every load a thread needs (its two cells' old marker, counts and six
+// neighbour markers, and the neighbour tiles' slots) is issued at once.
 __global__ __launch_bounds__(GF_WG) void k_tile_setup(
     int nx, int ny, int nz, int tnx, int tny, int tnz, const int32_t* __restrict__ cell_start,
     int32_t* marker, const int32_t* __restrict__ tile_map, uint8_t* __restrict__ flags,
@@ -257,6 +320,37 @@ __global__ __launch_bounds__(GF_WG) void k_tile_setup(
     const int slot = tile_map[t];
     const int i0 = ti * 8, j0 = tj * 8, k0 = tk * 8;
     const int lj = (threadIdx.x >> 3) & 7, lk = threadIdx.x & 7;
+    const int sx = ny * nz;
+    int mold[2], cnt[2], nb[2][6];
+    bool in[2];
+    int cell[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int li = (threadIdx.x >> 6) + 4 * h;
+        const int i = i0 + li, j = j0 + lj, k = k0 + lk;
+        in[h] = i < nx && j < ny && k < nz;
+        const int c = in[h] ? (i * ny + j) * nz + k : 0;
+        cell[h] = c;
+        mold[h] = marker[c];
+        cnt[h] = cell_start[c + 1] - cell_start[c];
+        // neighbours outside the grid read the cell itself and are ignored
+        nb[h][0] = marker[i > 0 ? c - sx : c];
+        nb[h][1] = marker[i < nx - 1 ? c + sx : c];
+        nb[h][2] = marker[j > 0 ? c - nz : c];
+        nb[h][3] = marker[j < ny - 1 ? c + nz : c];
+        nb[h][4] = marker[k > 0 ? c - 1 : c];
+        nb[h][5] = marker[k < nz - 1 ? c + 1 : c];
+    }
+    int ns = -1;
+    if (slot >= 0 && threadIdx.x < 6) {
+        const int d = threadIdx.x;
+        int a = ti, b = tj, c = tk;
+        if (d == 0) a -= 1; else if (d == 1) a += 1;
+        else if (d == 2) b -= 1; else if (d == 3) b += 1;
+        else if (d == 4) c -= 1; else c += 1;
+        if (a >= 0 && a < tnx && b >= 0 && b < tny && c >= 0 && c < tnz)
+            ns = tile_map[(a * tny + b) * tnz + c];
+    }
     int nfluid = 0, anyp = 0, anys = 0;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
@@ -264,22 +358,19 @@ __global__ __launch_bounds__(GF_WG) void k_tile_setup(
         const int i = i0 + li, j = j0 + lj, k = k0 + lk;
         const int l = (li * 8 + lj) * 8 + lk;
         uint8_t f = 0;
-        if (i < nx && j < ny && k < nz) {
-            const int c = (i * ny + j) * nz + k;
-            const int mold = marker[c];
-            const int cnt = cell_start[c + 1] - cell_start[c];
-            anyp |= cnt > 0;
-            anys |= mold == 2;
-            const int m = (mold == 2) ? 2 : (cnt > 0 ? 1 : 0);
-            if (m != mold) marker[c] = m;
+        if (in[h]) {
+            anyp |= cnt[h] > 0;
+            anys |= mold[h] == 2;
+            const int m = (mold[h] == 2) ? 2 : (cnt[h] > 0 ? 1 : 0);
+            if (m != mold[h]) marker[cell[h]] = m;
             if (m == 1) {
                 int d = 0;
-                if (i > 0 && marker[c - ny * nz] != 2) ++d;
-                if (i < nx - 1 && marker[c + ny * nz] != 2) ++d;
-                if (j > 0 && marker[c - nz] != 2) ++d;
-                if (j < ny - 1 && marker[c + nz] != 2) ++d;
-                if (k > 0 && marker[c - 1] != 2) ++d;
-                if (k < nz - 1 && marker[c + 1] != 2) ++d;
+                if (i > 0 && nb[h][0] != 2) ++d;
+                if (i < nx - 1 && nb[h][1] != 2) ++d;
+                if (j > 0 && nb[h][2] != 2) ++d;
+                if (j < ny - 1 && nb[h][3] != 2) ++d;
+                if (k > 0 && nb[h][4] != 2) ++d;
+                if (k < nz - 1 && nb[h][5] != 2) ++d;
                 f = (uint8_t)(8 | d);
                 ++nfluid;
             }
@@ -293,17 +384,7 @@ __global__ __launch_bounds__(GF_WG) void k_tile_setup(
     // the staged P2G scatters only such tiles.  bit 2: it holds a solid cell.
     const int bits = (__syncthreads_or(anyp) ? 2 : 0) | (__syncthreads_or(anys) ? 4 : 0);
     if (bits && threadIdx.x == 0) tile_occ[t] |= bits;
-    if (slot >= 0 && threadIdx.x < 6) {
-        int d = threadIdx.x;
-        int a = ti, b = tj, c = tk;
-        if (d == 0) a -= 1; else if (d == 1) a += 1;
-        else if (d == 2) b -= 1; else if (d == 3) b += 1;
-        else if (d == 4) c -= 1; else c += 1;
-        int ns = -1;
-        if (a >= 0 && a < tnx && b >= 0 && b < tny && c >= 0 && c < tnz)
-            ns = tile_map[(a * tny + b) * tnz + c];
-        nbr[(size_t)slot * 6 + d] = ns;
-    }
+    if (slot >= 0 && threadIdx.x < 6) nbr[(size_t)slot * 6 + threadIdx.x] = ns;
 }
 
 int scan_exclusive(const int32_t* in, int n, int32_t* out, int32_t* part, hipStream_t st) {
@@ -360,8 +441,9 @@ extern "C" int gf_scene_sort_sync(gf_scene_t* s, int with_affine, int* n_active_
     hipLaunchKernelGGL(k_fill_i32, dim3((ntiles + GF_WG - 1) / GF_WG), dim3(GF_WG), 0, st,
                        s->tile_occ, ntiles, 0);
     const int npb = (s->n + GF_WG - 1) / GF_WG;
+    const int nsb = (s->n + GF_WG * SORT_SUB - 1) / (GF_WG * SORT_SUB);
     if (s->n > 0)
-        hipLaunchKernelGGL(k_bin, dim3(npb), dim3(GF_WG), 0, st, s->n, s->pos, s->dx, s->nx,
+        hipLaunchKernelGGL(k_bin, dim3(nsb), dim3(GF_WG), 0, st, s->n, s->pos, s->dx, s->nx,
                            s->ny, s->nz, s->marker, s->keys, s->counts, s->tile_occ, s->tny,
                            s->tnz, s->pid, s->n_scenes > 1 ? s->scene_pstart : nullptr,
                            s->scene_par, s->n_scenes);
@@ -369,7 +451,7 @@ extern "C" int gf_scene_sort_sync(gf_scene_t* s, int with_affine, int* n_active_
     rc = scan_exclusive(s->counts, C, s->cell_start, s->scan_part, st);
     if (rc) return rc;
     if (s->n > 0) {
-        hipLaunchKernelGGL(k_slot, dim3(npb), dim3(GF_WG), 0, st, s->n, s->keys, s->counts,
+        hipLaunchKernelGGL(k_slot, dim3(nsb), dim3(GF_WG), 0, st, s->n, s->keys, s->counts,
                            s->cell_start, s->tmp);
         hipLaunchKernelGGL(k_rank, dim3(npb), dim3(GF_WG), 0, st, s->n, s->keys,
                            s->cell_start, s->tmp, s->pos, s->vel,
