@@ -1783,13 +1783,15 @@ void pcg_launch_setup(const PcgArgs& a, bool mixed, hipStream_t st) {
 // One chunk: `chunk` iterations, then the chunk-end decision (and residual
 // replacement when due).  L is the launch number before the chunk.  ev, when
 // given, holds 2*k+2 events (k = pcg_kernels_per_iter): the start and stop of
-// each kernel of the chunk's middle iteration (chunk/2), then of the chunk-end
+// each kernel of the chunk's iteration jsample, then of the chunk-end
 // replacement, taken from the kernels' own dispatch packets
 // (hipExtLaunchKernel), so they time the kernels' execution and not the gaps
 // between launches.  One iteration per chunk is a sample that keeps the
-// timing's own cost out of the other launches.
+// timing's own cost out of the other launches; consecutive chunks sample an
+// odd and an even iteration (the update kernel adds x on every other one).
 int pcg_launch_chunk(const PcgArgs& a, bool mixed, int chunk, int& L, hipStream_t st,
-                     hipEvent_t* ev, PcgLog* hout = nullptr) {
+                     hipEvent_t* ev, PcgLog* hout = nullptr, int jsample = -1) {
+    if (jsample < 0) jsample = chunk / 2;
     const dim3 grid(a.G), block(GF_WG);
     const int kpi = pcg_kernels_per_iter(a);
     // the chunk's last kernel also writes the logs to hout (host readback
@@ -1812,7 +1814,7 @@ int pcg_launch_chunk(const PcgArgs& a, bool mixed, int chunk, int& L, hipStream_
     const int c0 = (L - 1) / pcg_chunk_launches(a, chunk, mixed);  // chunks before this one
     for (int j = 0; j < chunk; ++j) {
         const int i = 0;
-        ev = j == chunk / 2 ? ev_all : nullptr;
+        ev = j == jsample ? ev_all : nullptr;
         const int xmode = !xpair ? X_NOW : ((j & 1) ? X_PAIR : (j + 1 < chunk ? X_DEFER : X_NOW));
         // direction buffer of the k-th direction kernel since the setup
         // (k = 1, 2, ...): p_old in (k-1)&1, p_new in k&1, for every scene
@@ -1904,8 +1906,8 @@ extern "C" int gf_scene_pcg_sync(const gf_scene_t* s, int n_active, int precisio
     // Optional per-kernel timing of the iterations that did work: events
     // around every launch of a chunk, read when the chunk's logs are read.
     // kernel_ms: [0] iteration kernel (two-kernel form: direction+SpMV), [1] update
-    // (two-kernel form), summed over the sampled iterations (the middle one of
-    // each chunk, when it did work), [2] the number of sampled iterations, [3]
+    // (two-kernel form), summed over the sampled iterations (one per chunk,
+    // when it did work), [2] the number of sampled iterations, [3]
     // chunk-end replacements (every one that ran).
     PcgLog* dlog = nullptr;  // device-side address of hlog
     PcgLog* hlog = gf::pinned_pcg_logs((size_t)6 * S, &dlog);  // setup + 2 chunk slots, 2S entries each
@@ -1959,11 +1961,14 @@ extern "C" int gf_scene_pcg_sync(const gf_scene_t* s, int n_active, int precisio
     if (e != hipSuccess) rc = gf::set_error((int)e, "gf_scene_pcg_sync: readback: %s", hipGetErrorString(e));
     else hs = aggregate(hlog, L);
 
+    // the timed iteration of a chunk: the middle one in slot 0, the one
+    // before it in slot 1 (slots alternate)
+    auto jsample = [&](int sl) { return chunk / 2 - (chunk >= 2 ? sl : 0); };
     int launched = 0, next_slot = 0, npend = 0, pend[2] = {0, 0}, pend_L[2] = {0, 0};
     auto enqueue_chunk = [&]() -> int {
         const int sl = next_slot;
         int r = pcg_launch_chunk(a, mixed, chunk, L, st, timed ? ev[sl] : nullptr,
-                                 dlog + (size_t)2 * S * (1 + sl));
+                                 dlog + (size_t)2 * S * (1 + sl), jsample(sl));
         if (r) return r;
         hipError_t ce = hipEventRecord(done_ev[sl], st);
         if (ce != hipSuccess)
@@ -1995,7 +2000,7 @@ extern "C" int gf_scene_pcg_sync(const gf_scene_t* s, int n_active, int precisio
         hs = aggregate(hlog + (size_t)2 * S * (1 + sl), Lsl);
         if (timed) {
             const int real = hs.iters - iters_before;
-            if (real > chunk / 2) {  // the sampled iteration did work
+            if (real > jsample(sl)) {  // the sampled iteration did work
                 for (int m = 0; m < kpi; ++m) {
                     float t0 = 0.f;
                     (void)hipEventElapsedTime(&t0, ev[sl][2 * m], ev[sl][2 * m + 1]);
