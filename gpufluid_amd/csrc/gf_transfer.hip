@@ -1391,10 +1391,28 @@ __global__ __launch_bounds__(GF_WG) void k_g2p_tile(G2PParams a) {
         lox = sp[1]; loy = sp[2]; loz = sp[3];
         hix = sp[4]; hiy = sp[5]; hiz = sp[6];
     }
-    for (int q = threadIdx.x; q < rs.total; q += GF_WG) {
+    // The next particle's position and velocity are loaded before this one's
+    // are computed and stored (the stores could alias the loads as far as the
+    // compiler knows, so otherwise every particle waits a full round trip).
+    auto pindex = [&](int q) {
         const int r = row_find(s_pref, rs.nrows, q);
-        const int pidx = s_start[r] + (q - s_pref[r]);
-        const float px = a.pos[3 * pidx + 0], py = a.pos[3 * pidx + 1], pz = a.pos[3 * pidx + 2];
+        return s_start[r] + (q - s_pref[r]);
+    };
+    int nidx = threadIdx.x < rs.total ? pindex(threadIdx.x) : 0;
+    float nx_ = 0.0f, ny_ = 0.0f, nz_ = 0.0f, nvx = 0.0f, nvy = 0.0f, nvz = 0.0f;
+    if (threadIdx.x < rs.total) {
+        nx_ = a.pos[3 * nidx + 0]; ny_ = a.pos[3 * nidx + 1]; nz_ = a.pos[3 * nidx + 2];
+        if (!APIC) { nvx = a.vel[3 * nidx + 0]; nvy = a.vel[3 * nidx + 1]; nvz = a.vel[3 * nidx + 2]; }
+    }
+    for (int q = threadIdx.x; q < rs.total; q += GF_WG) {
+        const int pidx = nidx;
+        const float px = nx_, py = ny_, pz = nz_;
+        const float ovx = nvx, ovy = nvy, ovz = nvz;
+        if (q + GF_WG < rs.total) {
+            nidx = pindex(q + GF_WG);
+            nx_ = a.pos[3 * nidx + 0]; ny_ = a.pos[3 * nidx + 1]; nz_ = a.pos[3 * nidx + 2];
+            if (!APIC) { nvx = a.vel[3 * nidx + 0]; nvy = a.vel[3 * nidx + 1]; nvz = a.vel[3 * nidx + 2]; }
+        }
         const float lx = px / dx, ly = py / dx, lz = pz / dx;
         float nu, nv, nw;
         if (!APIC) {
@@ -1404,7 +1422,6 @@ __global__ __launch_bounds__(GF_WG) void k_g2p_tile(G2PParams a) {
             const float old_v = sample_region<1>(s_sav[1], lx - 0.5f, ly, lz - 0.5f, snx, sny + 1, snz, li0, lj0, lk0);
             const float pic_w = sample_region<2>(s_fin[2], lx - 0.5f, ly - 0.5f, lz, snx, sny, snz + 1, li0, lj0, lk0);
             const float old_w = sample_region<2>(s_sav[2], lx - 0.5f, ly - 0.5f, lz, snx, sny, snz + 1, li0, lj0, lk0);
-            const float ovx = a.vel[3 * pidx + 0], ovy = a.vel[3 * pidx + 1], ovz = a.vel[3 * pidx + 2];
             const float flip_u = ovx + (pic_u - old_u);
             const float flip_v = ovy + (pic_v - old_v);
             const float flip_w = ovz + (pic_w - old_w);
