@@ -84,6 +84,10 @@ struct PcgArgs {
     double* u64[2]; double* w64[2]; double* s64[2];
     double* p64; double* x64;
     float* pc;
+    // two-kernel form: the cells' flags at 4 bits per cell (one 16-bit word per
+    // quad, 128 per tile), written by its setup into the buffer the one-kernel
+    // form uses for w~ (unused by this form); read by every iteration
+    uint16_t* pflags;
     double* part;        // [P_KINDS][PCG_GMAX]
     PcgLog* log;         // [2*S] live pairs
     PcgLog* res;         // [S] results (gf_scene_pcg_stats reads them)
@@ -144,6 +148,15 @@ __device__ __forceinline__ uint32_t ld_flags(const uint8_t* f, size_t idx) {
     return *reinterpret_cast<const uint32_t*>(f + idx);
 }
 __device__ __forceinline__ int fl_fluid(uint32_t fw, int e) { return (fw >> (8 * e + 3)) & 1; }
+
+// A quad's four flag bytes (each < 16: fluid bit 3, diag bits 0-2) as one
+// 16-bit word of nibbles, and back.
+__device__ __forceinline__ uint32_t pack_flags(uint32_t fw) {
+    return (fw & 0xFu) | ((fw >> 4) & 0xF0u) | ((fw >> 8) & 0xF00u) | ((fw >> 12) & 0xF000u);
+}
+__device__ __forceinline__ uint32_t unpack_flags(uint32_t n) {
+    return (n & 0xFu) | ((n & 0xF0u) << 4) | ((n & 0xF00u) << 8) | ((n & 0xF000u) << 12);
+}
 
 // v if keep, else +0, as a bit mask: a select here would let the compiler
 // sink the loads that feed v into a branch and serialise them.
@@ -283,6 +296,13 @@ __device__ double g_zero_tile[GF_TILE_CELLS] = {};
 template <typename T>
 __device__ __forceinline__ const T* tile_base(const T* v, int tile) {
     return tile >= 0 ? v + (size_t)tile * GF_TILE_CELLS : reinterpret_cast<const T*>(g_zero_tile);
+}
+
+// The packed flags of the quad at element offset o of tile t (t < 0: none).
+__device__ __forceinline__ uint32_t ld_pflags(const uint16_t* pf, int t, uint32_t o) {
+    const uint16_t* base = t >= 0 ? pf + (size_t)t * (GF_TILE_CELLS / 4)
+                                  : reinterpret_cast<const uint16_t*>(g_zero_tile);
+    return unpack_flags(base[o >> 2]);
 }
 
 // Neighbour-table row of no tile (every neighbour absent): the first-pass
@@ -1137,6 +1157,7 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_setup(PcgArgs a) {
     FOR_QUADS(P) {
         const size_t idx = (size_t)t * GF_TILE_CELLS + Q.l0;
         const uint32_t fw = ld_flags(a.flags, idx);
+        a.pflags[idx >> 2] = (uint16_t)pack_flags(fw);
         float bv[4];
         ld4(a.b + idx, bv);
         T zv[4], zero[4] = {(T)0, (T)0, (T)0, (T)0};
@@ -1174,7 +1195,7 @@ __device__ __forceinline__ void dir_issue(DirIn<T>& d, const PcgArgs& a, const T
                                           const T* __restrict__ po, int t, const FaceTiles& row,
                                           const FaceOff& FO) {
     const FaceTiles ft = for_half(row, t);
-    d.fw = ld_flags(tile_base(a.flags, t), FO.own);
+    d.fw = ld_pflags(a.pflags, t, FO.own);
     ld4(tile_base(z, t) + FO.own, d.zq);
     ld4(tile_base(po, t) + FO.own, d.pq);
     ld4(tile_base(z, ft.xp) + FO.xp, d.zx);
@@ -1305,7 +1326,7 @@ __device__ __forceinline__ void upd_issue(UpdIn<T, MIXED>& d, const PcgArgs& a,
     // stored neighbour row (face_row), read one pass ahead
     const FaceTiles ft = for_half(row, t);
     const uint32_t o = FO.own;
-    d.fw = ld_flags(tile_base(a.flags, t), o);
+    d.fw = ld_pflags(a.pflags, t, o);
     ld4(tile_base(p, t) + o, d.pv);
     ld4(tile_base(zp, t) + o, d.zv);
     load_face(p, ft, FO, d.fp);
@@ -1688,6 +1709,14 @@ namespace {
 // partition table) are left for the caller to point at its own memory, and
 // only the descriptor's inputs, pc and the result entries of scalars are
 // required.
+// The two-kernel form's packed flags live in its unused w~ buffer.
+void set_pflags(PcgArgs& a, bool mixed) {
+    a.pflags = a.variant == PCG_STD
+                   ? reinterpret_cast<uint16_t*>(mixed ? static_cast<void*>(a.w32[0])
+                                                       : static_cast<void*>(a.w64[0]))
+                   : nullptr;
+}
+
 int pcg_prepare(const gf_scene_t* s, int n_active, int precision, double tol, int max_iter,
                 int rhs_sign, const char* who, PcgArgs& a, bool own_work = true) {
     int rc = gf::scene_check(s, who);
@@ -1755,6 +1784,7 @@ int pcg_prepare(const gf_scene_t* s, int n_active, int precision, double tol, in
     a.tol2 = tol * tol;
     a.delta2 = 0.01;
     a.max_iter = max_iter;
+    set_pflags(a, precision == GF_PCG_MIXED);
     return 0;
 }
 
@@ -2205,6 +2235,7 @@ extern "C" int gf_scene_pcg_enqueue_noalloc(const gf_scene_t* s, int n_active, i
         a.u64[0] = d[0]; a.u64[1] = d[1]; a.w64[0] = d[2]; a.w64[1] = d[3];
         a.s64[0] = d[4]; a.s64[1] = d[5]; a.p64 = d[6];
     }
+    set_pflags(a, precision == GF_PCG_MIXED);
     unsigned char* tail = b + NOALLOC_VEC_BYTES;
     a.part = reinterpret_cast<double*>(tail);
     a.log = reinterpret_cast<PcgLog*>(tail + NOALLOC_PART_BYTES);

@@ -128,7 +128,10 @@ typedef struct gf_scene {
     /* compact pressure storage, (ntiles*512) entries, slot-major */
     uint8_t* flags;              /* bit3 fluid, bits0-2 non-solid neighbour count */
     float* b;                    /* divergence (reference scaling) */
-    float* x32; float* r32; float* p32; float* ap32;  /* FP32 PCG vectors (r32/r64 hold z = r/max(diag,1)) */
+    float* x32; float* r32; float* p32; float* ap32;  /* FP32 PCG vectors (r32/r64 hold z = r/max(diag,1));
+                                                         the two-kernel form keeps the cells' flags at 4 bits
+                                                         per cell in ap32 (ap64 in FP64), which it does not
+                                                         otherwise use */
     double* x64; double* r64; double* p64; double* ap64; /* FP64 vectors */
     float* pc;  float* ptmp;     /* pressure result / Jacobi ping-pong */
     double* partials;            /* (gf_scene_sizes.partials) per-workgroup partial sums */
