@@ -1211,6 +1211,24 @@ __device__ __forceinline__ float sample_region(const float* s, float fx, float f
     return c0 * (1.0f - sz) + c1 * sz;
 }
 
+// sample3's interpolation (x, then y, then z) at a corner whose eight values
+// sit at base + {0, 1} (k) + {0, SJ} (j) + {0, SI} (i) of a staged region: the
+// clamps of sample_region are no-ops for a particle whose corners all lie
+// inside the grid and the region (the caller checks).
+template <int SJ, int SI>
+__device__ __forceinline__ float tri_at(const float* s, int base, float sx, float sy, float sz) {
+    const float* b = s + base;
+    const float c000 = b[0], c100 = b[SI], c010 = b[SJ], c110 = b[SI + SJ];
+    const float c001 = b[1], c101 = b[SI + 1], c011 = b[SJ + 1], c111 = b[SI + SJ + 1];
+    const float c00 = c000 * (1.0f - sx) + c100 * sx;
+    const float c10 = c010 * (1.0f - sx) + c110 * sx;
+    const float c01 = c001 * (1.0f - sx) + c101 * sx;
+    const float c11 = c011 * (1.0f - sx) + c111 * sx;
+    const float c0 = c00 * (1.0f - sy) + c10 * sy;
+    const float c1 = c01 * (1.0f - sy) + c11 * sy;
+    return c0 * (1.0f - sz) + c1 * sz;
+}
+
 // APIC gather of one component (solver3d.py:1620-1716): clamped corner
 // indices, unclamped weights, affine row from world-space face offsets.
 template <int COMP>
@@ -1383,6 +1401,8 @@ __global__ __launch_bounds__(GF_WG) void k_g2p_tile(G2PParams a) {
     // face-array extents of the scene
     const int li0 = i0 - F.ox, lj0 = j0 - F.oy, lk0 = k0 - F.oz;
     const int snx = F.nx, sny = F.ny, snz = F.nz;
+    const float inv_dx = 1.0f / dx;
+    const float fsnx = (float)(snx - 1), fsny = (float)(sny - 1), fsnz = (float)(snz - 1);
     // clamp box: [1.001 dx, dom - 1.001 dx], or the tile's scene box when packed
     float lox = eps, loy = eps, loz = eps;
     float hix = a.domx - eps, hiy = a.domy - eps, hiz = a.domz - eps;
@@ -1413,9 +1433,46 @@ __global__ __launch_bounds__(GF_WG) void k_g2p_tile(G2PParams a) {
             nx_ = a.pos[3 * nidx + 0]; ny_ = a.pos[3 * nidx + 1]; nz_ = a.pos[3 * nidx + 2];
             if (!APIC) { nvx = a.vel[3 * nidx + 0]; nvy = a.vel[3 * nidx + 1]; nvz = a.vel[3 * nidx + 2]; }
         }
-        const float lx = px / dx, ly = py / dx, lz = pz / dx;
         float nu, nv, nw;
+        // Fast path (FLIP): a particle at least one cell inside the scene's
+        // grid whose corners all fall in the staged regions is sampled without
+        // sample3's clamps (they change nothing there), with p/dx as p * (1/dx)
+        // and the corner addresses shared by the new and old values of a
+        // component.  Other particles take the reference's sample3.
+        bool fast = false;
         if (!APIC) {
+            const float qx = px * inv_dx, qy = py * inv_dx, qz = pz * inv_dx;
+            const float fxo = floorf(qx), fyo = floorf(qy), fzo = floorf(qz);
+            const float fxs = floorf(qx - 0.5f), fys = floorf(qy - 0.5f), fzs = floorf(qz - 0.5f);
+            // region-local indices: own axis from the tile corner, staggered
+            // axes from one below it
+            const int ixo = (int)fxo - li0, iyo = (int)fyo - lj0, izo = (int)fzo - lk0;
+            const int ixs = (int)fxs - li0 + 1, iys = (int)fys - lj0 + 1, izs = (int)fzs - lk0 + 1;
+            fast = qx >= 1.0f && qx <= fsnx && qy >= 1.0f && qy <= fsny && qz >= 1.0f &&
+                   qz <= fsnz && (unsigned)ixo <= 7u && (unsigned)iyo <= 7u &&
+                   (unsigned)izo <= 7u && (unsigned)ixs <= 8u && (unsigned)iys <= 8u &&
+                   (unsigned)izs <= 8u;
+            if (fast) {
+                const float sxo = qx - fxo, syo = qy - fyo, szo = qz - fzo;
+                const float sxs = (qx - 0.5f) - fxs, sys = (qy - 0.5f) - fys,
+                            szs = (qz - 0.5f) - fzs;
+                const int bu = (ixo * 10 + iys) * 10 + izs;  // u: 9 x 10 x 10
+                const int bv = (ixs * 9 + iyo) * 10 + izs;   // v: 10 x 9 x 10
+                const int bw = (ixs * 10 + iys) * 9 + izo;   // w: 10 x 10 x 9
+                const float pic_u = tri_at<10, 100>(s_fin[0], bu, sxo, sys, szs);
+                const float old_u = tri_at<10, 100>(s_sav[0], bu, sxo, sys, szs);
+                const float pic_v = tri_at<10, 90>(s_fin[1], bv, sxs, syo, szs);
+                const float old_v = tri_at<10, 90>(s_sav[1], bv, sxs, syo, szs);
+                const float pic_w = tri_at<9, 90>(s_fin[2], bw, sxs, sys, szo);
+                const float old_w = tri_at<9, 90>(s_sav[2], bw, sxs, sys, szo);
+                const float fb = a.flip_blend;
+                nu = fb * (ovx + (pic_u - old_u)) + (1.0f - fb) * pic_u;
+                nv = fb * (ovy + (pic_v - old_v)) + (1.0f - fb) * pic_v;
+                nw = fb * (ovz + (pic_w - old_w)) + (1.0f - fb) * pic_w;
+            }
+        }
+        if (!APIC && !fast) {
+            const float lx = px / dx, ly = py / dx, lz = pz / dx;
             const float pic_u = sample_region<0>(s_fin[0], lx, ly - 0.5f, lz - 0.5f, snx + 1, sny, snz, li0, lj0, lk0);
             const float old_u = sample_region<0>(s_sav[0], lx, ly - 0.5f, lz - 0.5f, snx + 1, sny, snz, li0, lj0, lk0);
             const float pic_v = sample_region<1>(s_fin[1], lx - 0.5f, ly, lz - 0.5f, snx, sny + 1, snz, li0, lj0, lk0);
@@ -1429,7 +1486,8 @@ __global__ __launch_bounds__(GF_WG) void k_g2p_tile(G2PParams a) {
             nu = fb * flip_u + (1.0f - fb) * pic_u;
             nv = fb * flip_v + (1.0f - fb) * pic_v;
             nw = fb * flip_w + (1.0f - fb) * pic_w;
-        } else {
+        } else if (APIC) {
+            const float lx = px / dx, ly = py / dx, lz = pz / dx;
             const float inv_dx2_3 = 3.0f / (dx * dx);
             float cm[9];
             apic_gather<0>(s_fin[0], px, py, pz, lx, ly, lz, snx, sny, snz, dx, inv_dx2_3, li0, lj0, lk0, nu, cm + 0);
