@@ -50,10 +50,11 @@ def get_pcg_variant() -> int:
 
 
 def set_pcg_x_every(v: int) -> None:
-    """Two-kernel PCG: 1 updates the iterate x in every update kernel, 2 (or
-    0, the default) in every other one, adding both pending terms in order
-    (bit-identical results, one read and one write of x fewer per two
-    iterations)."""
+    """Two-kernel PCG: the iterate x takes its terms alpha p on every n-th
+    update kernel (n = 1..4; 0, the default, or above 4 means 4) and on each
+    check chunk's last, the pending terms in iteration order (bit-identical
+    results; n-1 reads and writes of x fewer per n iterations, for n-1 reads
+    of the pending directions)."""
     _lib.call("X3.1", "gf_tune_set", b"pcg_x_every", int(v))
 
 

@@ -35,6 +35,7 @@
 // Sleijpen & van der Vorst 1996).
 #include "gf_common.h"
 #include <hip/hip_ext.h>
+#include <atomic>
 #include <string.h>
 #include <mutex>
 #include <type_traits>
@@ -48,13 +49,14 @@ namespace {
 struct PcgLog {
     double rz;       // r.z of the current residual (used by the next alpha)
     double rz_old;   // r.z used by the last alpha (denominator of the next beta)
-    double rr, bb, alpha, beta, rr_upd, rr_true, pap;
+    double rr, bb, alpha, alpha2, rr_upd, rr_true, pap;
     int iters, done, converged, maxed, upd, merge, nupd, pending;
     int fresh;       // 1: the update kernel's r.z / r.r partials are not consumed yet
     int pbuf;        // direction buffer holding the current p
-    int xpend;       // 1: x still lacks alpha p of the last update (two-kernel form)
+    int xpend;       // two-kernel form: x lacks alpha p of the last xpend (<= 3) updates,
+                     // whose alphas are alpha, alpha2, alpha3 (latest first)
     int pad_i;
-    double pad;
+    double alpha3;
 };
 static_assert(sizeof(PcgLog) == 16 * sizeof(double), "log entry is 16 doubles");
 
@@ -594,8 +596,9 @@ __device__ __forceinline__ void put_partial3(double* part, int ka, int kb, int k
 // Logs are copied field by field: a whole-struct copy between global memory
 // and a local leaves part of the local in per-thread LDS.
 #define GF_LOG_FIELDS(X)                                                              \
-    X(rz) X(rz_old) X(rr) X(bb) X(alpha) X(beta) X(rr_upd) X(rr_true) X(pap) X(iters) \
-    X(done) X(converged) X(maxed) X(upd) X(merge) X(nupd) X(pending) X(fresh) X(pbuf) X(xpend)
+    X(rz) X(rz_old) X(rr) X(bb) X(alpha) X(alpha2) X(rr_upd) X(rr_true) X(pap) X(iters) \
+    X(done) X(converged) X(maxed) X(upd) X(merge) X(nupd) X(pending) X(fresh) X(pbuf) X(xpend) \
+    X(alpha3)
 
 __device__ __forceinline__ PcgLog copy_log(const PcgLog* src) {
     PcgLog e;
@@ -603,7 +606,6 @@ __device__ __forceinline__ PcgLog copy_log(const PcgLog* src) {
     GF_LOG_FIELDS(GF_LOG_LD)
 #undef GF_LOG_LD
     e.pad_i = 0;
-    e.pad = 0.0;
     return e;
 }
 __device__ __forceinline__ void put_log(PcgLog* dst, const PcgLog& e) {
@@ -820,7 +822,7 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_fin(PcgArgs a, int L) {
         e.rz = gam; e.pap = del; e.rz_old = 0.0; e.alpha = 0.0;
         e.bb = bb; e.rr = bb; e.rr_upd = bb; e.rr_true = bb;
         e.iters = 0; e.maxed = 0; e.upd = 0; e.merge = 0; e.nupd = 0; e.fresh = 0; e.pbuf = 0;
-        e.xpend = 0;
+        e.xpend = 0; e.alpha2 = 0.0; e.alpha3 = 0.0;
         const int zero = bb < 1e-30;
         e.converged = zero;
         e.done = zero || a.max_iter <= 0;
@@ -955,7 +957,6 @@ __global__ __launch_bounds__(GF_WG) void k_cg1_iter(PcgArgs a, int L) {
     }
     put_partial3(a.part, P_RZ, P_PAP, P_RR, L, P.b, gam, del, rr, s_red);
     e.alpha = alpha;
-    e.beta = beta;
     e.pbuf = nxt;
     e.merge = 0;
     e.fresh = 1;
@@ -1070,31 +1071,48 @@ __global__ __launch_bounds__(GF_WG) void k_cg1_replace2(PcgArgs a, int L) {
 }
 
 // ---- two-kernel (standard) PCG -----------------------------------------
-//   dirspmv: beta = rz/rz_old, p = z + beta p (ping-pong; the stencil
-//            neighbours' p is recomputed from their z and old p), p.Ap
+//   dirspmv: beta = rz/rz_old, p = z + beta p (into the next slot of a ring
+//            of four direction buffers; the stencil neighbours' p is
+//            recomputed from their z and old p), p.Ap
 //   update:  alpha = rz/pAp, Ap again from the stored p (bit-identical to the
-//            direction kernel's), x += alpha p, r = M z - alpha Ap, z = r/M,
-//            r.z, r.r
+//            direction kernel's), x += alpha p (every few updates, for the
+//            pending terms at once), r = M z - alpha Ap, z = r/M, r.z, r.r
 // Ap is recomputed rather than stored: the stencil re-reads p through the
 // caches, while storing and reloading Ap would cost 8 bytes per cell of HBM
-// traffic per iteration.  Storage: z in u[0], p ping-pong in {p, s[0]}.
+// traffic per iteration.  Deferring x's terms saves the x read and write
+// (8 or 16 bytes per cell) of the deferring updates; the flushing update
+// reads the pending directions instead.  Storage: z in u[0], the direction
+// ring in {p, s[0], u[1], w[1]}, the packed flags in w[0].
 
 template <typename T>
 __device__ __forceinline__ T* std_p(const PcgArgs& a, int buf) {
-    return buf ? Vec<T>::s(a, 0) : Vec<T>::p(a);
+    // a ring of four direction buffers: p, s~[0] and the one-kernel form's
+    // second u and w~ (this form does not use them otherwise)
+    buf &= 3;
+    return buf == 0 ? Vec<T>::p(a) : (buf == 1 ? Vec<T>::s(a, 0) : (buf == 2 ? Vec<T>::u(a, 1) : Vec<T>::w(a, 1)));
 }
 
-// How an update kernel of the two-kernel form treats the iterate x: add
-// alpha p now, leave it pending (the next update adds both terms), or add the
-// pending term and its own.
-enum { X_NOW = 0, X_DEFER = 1, X_PAIR = 2 };
+// The two-kernel form adds alpha p to the iterate x on every XE-th update
+// (and on a chunk's last one); the updates in between leave their terms
+// pending.  xmode argument of the update kernel: X_DEFER, or the number of
+// pending terms the host expects this update to add besides its own.
+constexpr int X_DEFER = -1;
+constexpr int X_MAX_PEND = 3;  // pending terms; the ring holds them and the current p
 
-// x += a1 p1 (when two) then x += a2 p2 on a quad, term by term in iteration
-// order, as one update per iteration would add them, so a deferred term
-// changes no bit of x (mixed: x32, first folded into x64 when merge is set).
+// The alpha of pending term i (0 = the latest) in a log.
+__device__ __forceinline__ double pend_alpha(const PcgLog& e, int i) {
+    return i == 0 ? e.alpha : (i == 1 ? e.alpha2 : e.alpha3);
+}
+
+// x += a[n-1] p[n-1] + ... + a[0] p[0] (when own: then + ao po) on a quad,
+// term by term in iteration order (oldest first), as one update per
+// iteration would add them, so deferring a term changes no bit of x (mixed:
+// x32, first folded into x64 when merge is set).
 template <typename T, bool MIXED>
-__device__ __forceinline__ void x_add(const PcgArgs& a, size_t idx, int merge, bool two, T a1,
-                                      const T (&p1)[4], T a2, const T (&p2)[4]) {
+__device__ __forceinline__ void x_addn(const PcgArgs& a, size_t idx, int merge, int n,
+                                       const T (&al)[X_MAX_PEND],
+                                       const T (&pp)[X_MAX_PEND][4], bool own, T ao,
+                                       const T (&po)[4]) {
     if (MIXED) {
         float xo[4];
         ld4(a.x32 + idx, xo);
@@ -1107,16 +1125,22 @@ __device__ __forceinline__ void x_add(const PcgArgs& a, size_t idx, int merge, b
             }
             st4(a.x64 + idx, xd);
         }
-        if (two)
-            for (int k = 0; k < 4; ++k) xo[k] = xo[k] + (float)a1 * (float)p1[k];
-        for (int k = 0; k < 4; ++k) xo[k] = xo[k] + (float)a2 * (float)p2[k];
+#pragma unroll
+        for (int i = X_MAX_PEND - 1; i >= 0; --i)
+            if (i < n)
+                for (int k = 0; k < 4; ++k) xo[k] = xo[k] + (float)al[i] * (float)pp[i][k];
+        if (own)
+            for (int k = 0; k < 4; ++k) xo[k] = xo[k] + (float)ao * (float)po[k];
         st4(a.x32 + idx, xo);
     } else {
         double xd[4];
         ld4(a.x64 + idx, xd);
-        if (two)
-            for (int k = 0; k < 4; ++k) xd[k] += (double)a1 * (double)p1[k];
-        for (int k = 0; k < 4; ++k) xd[k] += (double)a2 * (double)p2[k];
+#pragma unroll
+        for (int i = X_MAX_PEND - 1; i >= 0; --i)
+            if (i < n)
+                for (int k = 0; k < 4; ++k) xd[k] += (double)al[i] * (double)pp[i][k];
+        if (own)
+            for (int k = 0; k < 4; ++k) xd[k] += (double)ao * (double)po[k];
         st4(a.x64 + idx, xd);
     }
 }
@@ -1260,14 +1284,17 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_dirspmv(PcgArgs a, int L, int pb)
     if (pass_through(a, P, e, L)) return;
     consume_std<MIXED>(a, e, rzs, rrs, s_red);
     if (e.done) {
-        if (e.xpend) {  // the solve stopped with the last alpha p still pending
-            const T af = (T)e.alpha;
-            const T* __restrict__ p = std_p<T>(a, e.pbuf);
+        if (e.xpend) {  // the solve stopped with alpha p of the last updates pending
+            const int n = e.xpend;
+            T al[X_MAX_PEND], pp[X_MAX_PEND][4], none[4] = {};
+#pragma unroll
+            for (int i = 0; i < X_MAX_PEND; ++i) al[i] = (T)pend_alpha(e, i);
             FOR_QUADS(P) {
                 const size_t idx = (size_t)t * GF_TILE_CELLS + FO.own;
-                T pv[4];
-                ld4(p + idx, pv);
-                x_add<T, MIXED>(a, idx, MIXED ? e.merge : 0, false, af, pv, af, pv);
+#pragma unroll
+                for (int i = 0; i < X_MAX_PEND; ++i)
+                    if (i < n) ld4(std_p<T>(a, e.pbuf - i) + idx, pp[i]);
+                x_addn<T, MIXED>(a, idx, MIXED ? e.merge : 0, n, al, pp, false, (T)0, none);
             }
             e.xpend = 0;
             e.merge = 0;
@@ -1278,7 +1305,7 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_dirspmv(PcgArgs a, int L, int pb)
     const double beta = fabs(e.rz_old) < 1e-30 ? 0.0 : e.rz / e.rz_old;
     const T bf = (T)beta;
     const T* __restrict__ po = std_p<T>(a, e.pbuf);
-    T* __restrict__ pn = std_p<T>(a, e.pbuf ^ 1);
+    T* __restrict__ pn = std_p<T>(a, e.pbuf + 1);
     if (any && e.pbuf != pb) dir_issue(d, a, z, po, tf, ftf, FO);
     double pap = 0.0;
     const int lane = threadIdx.x & 63;
@@ -1300,55 +1327,58 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_dirspmv(PcgArgs a, int L, int pb)
         }
     }
     put_partial(a.part, P_PAP, L, P.b, pap, s_red);
-    e.beta = beta;
-    e.pbuf ^= 1;
+    e.pbuf = (e.pbuf + 1) & 3;
     write_log(a, P, e, L);
 }
 
 // Loads of one tile pass of the update kernel: flags, p, z, the iterate x
-// (when this update adds to it), the previous p (when it adds a pending
-// term) and p across the six faces of the wave's block.
+// and the pending terms' directions (when this update adds to x) and p across
+// the six faces of the wave's block.
 template <typename T, bool MIXED>
 struct UpdIn {
     using XT = typename std::conditional<MIXED, float, double>::type;
     uint32_t fw;
-    T pv[4], zv[4], pp[4];
+    T pv[4], zv[4], pp[X_MAX_PEND][4];
     XT xo[4];
     Nb4<T> fp;
 };
 
+// np >= 0: this update adds to x, with np pending terms whose directions sit
+// in the ring buffers before pb; np < 0: it defers.
 template <typename T, bool MIXED>
-__device__ __forceinline__ void upd_issue(UpdIn<T, MIXED>& d, const PcgArgs& a,
-                                          const T* __restrict__ p, const T* __restrict__ pprev,
+__device__ __forceinline__ void upd_issue(UpdIn<T, MIXED>& d, const PcgArgs& a, int pb,
                                           const T* __restrict__ zp, int t, const FaceTiles& row,
-                                          const FaceOff& FO, bool need_x, bool need_pp) {
+                                          const FaceOff& FO, int np) {
     // t < 0: no tile (reads the zero tile; the values are not used); row: the
     // stored neighbour row (face_row), read one pass ahead
     const FaceTiles ft = for_half(row, t);
     const uint32_t o = FO.own;
+    const T* __restrict__ p = std_p<T>(a, pb);
     d.fw = ld_pflags(a.pflags, t, o);
     ld4(tile_base(p, t) + o, d.pv);
     ld4(tile_base(zp, t) + o, d.zv);
     load_face(p, ft, FO, d.fp);
-    if (need_x) {
+    if (np >= 0) {
         if (MIXED) ld4(tile_base(a.x32, t) + o, reinterpret_cast<float(&)[4]>(d.xo));
         else ld4(tile_base(a.x64, t) + o, reinterpret_cast<double(&)[4]>(d.xo));
     }
-    if (need_pp) ld4(tile_base(pprev, t) + o, d.pp);
+#pragma unroll
+    for (int i = 0; i < X_MAX_PEND; ++i)
+        if (i < np) ld4(tile_base(std_p<T>(a, pb - 1 - i), t) + o, d.pp[i]);
 }
 
 // Ap from p (the direction kernel's operands and order), x += (pending
-// term) + alpha p, r = M z - alpha Ap, z = r / M, and the r.z, r.r partials.
+// terms) + alpha p, r = M z - alpha Ap, z = r / M, and the r.z, r.r partials.
 template <typename T, bool MIXED>
 __device__ __forceinline__ void upd_compute(UpdIn<T, MIXED>& d, const PcgArgs& a, const Quad& Q,
-                                            T af, T aprev, bool defer, bool pair, int merge,
+                                            T af, const T (&al)[X_MAX_PEND], int np, int merge,
                                             T* __restrict__ zp, size_t idx, double& rz,
                                             double& rr) {
     T apv[4];
-    Nb4<T> np;
-    nb_from_lanes(d.pv, d.fp, Q, np);
-    apply_A4(d.pv, np, d.fw, apv);
-    if (!defer) {
+    Nb4<T> nq;
+    nb_from_lanes(d.pv, d.fp, Q, nq);
+    apply_A4(d.pv, nq, d.fw, apv);
+    if (np >= 0) {
         if (MIXED) {
             float(&xo)[4] = reinterpret_cast<float(&)[4]>(d.xo);
             if (merge) {
@@ -1360,14 +1390,18 @@ __device__ __forceinline__ void upd_compute(UpdIn<T, MIXED>& d, const PcgArgs& a
                 }
                 st4(a.x64 + idx, xd);
             }
-            if (pair)
-                for (int k = 0; k < 4; ++k) xo[k] = xo[k] + (float)aprev * (float)d.pp[k];
+#pragma unroll
+            for (int i = X_MAX_PEND - 1; i >= 0; --i)
+                if (i < np)
+                    for (int k = 0; k < 4; ++k) xo[k] = xo[k] + (float)al[i] * (float)d.pp[i][k];
             for (int k = 0; k < 4; ++k) xo[k] = xo[k] + (float)af * (float)d.pv[k];
             st4(a.x32 + idx, xo);
         } else {
             double(&xd)[4] = reinterpret_cast<double(&)[4]>(d.xo);
-            if (pair)
-                for (int k = 0; k < 4; ++k) xd[k] += (double)aprev * (double)d.pp[k];
+#pragma unroll
+            for (int i = X_MAX_PEND - 1; i >= 0; --i)
+                if (i < np)
+                    for (int k = 0; k < 4; ++k) xd[k] += (double)al[i] * (double)d.pp[i][k];
             for (int k = 0; k < 4; ++k) xd[k] += (double)af * (double)d.pv[k];
             st4(a.x64 + idx, xd);
         }
@@ -1384,7 +1418,8 @@ __device__ __forceinline__ void upd_compute(UpdIn<T, MIXED>& d, const PcgArgs& a
 }
 
 // Loads as in the direction kernel: all of a pass at once, the first pass
-// before the log (current direction buffer pb as the host expects it).
+// before the log (current direction buffer pb and pending count xmode as the
+// host expects them; the log's values win when they differ).
 template <typename T, bool MIXED>
 __global__ __launch_bounds__(GF_WG) void k_pcg_update(PcgArgs a, int L, int xmode, int pb) {
     __shared__ double s_red[3 * GF_WG / GF_WAVE];
@@ -1395,13 +1430,12 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_update(PcgArgs a, int L, int xmod
     const int half = threadIdx.x >> 7;
     T* __restrict__ zp = Vec<T>::u(a, 0);
     const bool defer = xmode == X_DEFER;
-    const bool want_pp = xmode == X_PAIR;
     const bool any = P.t0 < P.t1;
     const int tf = __builtin_amdgcn_readfirstlane(any ? min(P.t0 + half, P.t1 - 1) : -1);
     // unconditional, so that the wait counts below see one path
     const FaceTiles ftf = face_row(a.nbr, tf);
     UpdIn<T, MIXED> d;
-    upd_issue(d, a, std_p<T>(a, pb), std_p<T>(a, pb ^ 1), zp, tf, ftf, FO, !defer, want_pp);
+    upd_issue(d, a, pb, zp, tf, ftf, FO, defer ? -1 : xmode);
     const double paps = partial_share(a.part, P_PAP, L - 1, P.wb, P.we);
     PcgLog e = live_log(a, P.s, L - 1);
     issue_together(e, paps, 0.0, 0.0);
@@ -1410,38 +1444,45 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_update(PcgArgs a, int L, int xmod
     const double alpha = fabs(pap_t) < 1e-30 ? 0.0 : e.rz / pap_t;
     const T af = (T)alpha;
     const int merge = MIXED ? e.merge : 0;
-    const T* __restrict__ p = std_p<T>(a, e.pbuf);
-    // the previous iteration's p (still in the other direction buffer) and
-    // alpha, when its term is pending
-    const bool pair = want_pp && e.xpend;
-    const T* __restrict__ pprev = std_p<T>(a, e.pbuf ^ 1);
-    const T aprev = (T)e.alpha;
-    if (any && e.pbuf != pb) upd_issue(d, a, p, pprev, zp, tf, ftf, FO, !defer, want_pp);
+    // the pending terms (their directions are the ring buffers before this
+    // update's p) and their alphas, latest first
+    const int np = defer ? -1 : e.xpend;
+    T al[X_MAX_PEND];
+#pragma unroll
+    for (int i = 0; i < X_MAX_PEND; ++i) al[i] = (T)pend_alpha(e, i);
+    if (any && (e.pbuf != pb || (!defer && np != xmode)))
+        upd_issue(d, a, e.pbuf, zp, tf, ftf, FO, np);
     double rz = 0.0, rr = 0.0;
     if (any) {
         int tn = __builtin_amdgcn_readfirstlane(P.t0 + 2 + half);
         FaceTiles ftn = face_row(a.nbr, tn < P.t1 ? tn : -1);
         if (P.t0 + half < P.t1)
-            upd_compute(d, a, Q, af, aprev, defer, pair, merge, zp,
-                        (size_t)tf * GF_TILE_CELLS + FO.own, rz, rr);
+            upd_compute(d, a, Q, af, al, np, merge, zp, (size_t)tf * GF_TILE_CELLS + FO.own, rz,
+                        rr);
         for (int tb = P.t0 + 2; tb < P.t1; tb += 2) {
             const int t = tn;
             if (t >= P.t1) break;
             UpdIn<T, MIXED> n;
-            upd_issue(n, a, p, pprev, zp, t, ftn, FO, !defer, want_pp);
+            upd_issue(n, a, e.pbuf, zp, t, ftn, FO, np);
             // the next pass's row is read after this pass's vector loads are out
             __builtin_amdgcn_sched_barrier(0);
             tn = __builtin_amdgcn_readfirstlane(tb + 2 + half);
             ftn = face_row(a.nbr, tn < P.t1 ? tn : -1);
-            upd_compute(n, a, Q, af, aprev, defer, pair, merge, zp,
-                        (size_t)t * GF_TILE_CELLS + FO.own, rz, rr);
+            upd_compute(n, a, Q, af, al, np, merge, zp, (size_t)t * GF_TILE_CELLS + FO.own, rz,
+                        rr);
         }
     }
     put_partial2(a.part, P_RZ, P_RR, L, P.b, rz, rr, s_red);
-    e.alpha = alpha;
     e.pap = pap_t;
-    if (!defer) e.merge = 0;
-    e.xpend = defer ? 1 : 0;
+    if (defer) {  // this update's term joins the pending ones
+        e.alpha3 = e.alpha2;
+        e.alpha2 = e.alpha;
+        e.xpend += 1;
+    } else {
+        e.merge = 0;
+        e.xpend = 0;
+    }
+    e.alpha = alpha;
     e.fresh = 1;
     write_log(a, P, e, L);
 }
@@ -1817,8 +1858,9 @@ void pcg_launch_setup(const PcgArgs& a, bool mixed, hipStream_t st) {
 // replacement, taken from the kernels' own dispatch packets
 // (hipExtLaunchKernel), so they time the kernels' execution and not the gaps
 // between launches.  One iteration per chunk is a sample that keeps the
-// timing's own cost out of the other launches; consecutive chunks sample an
-// odd and an even iteration (the update kernel adds x on every other one).
+// timing's own cost out of the other launches; the caller varies jsample
+// over chunks (the update kernel adds x's pending terms on some iterations
+// only).
 int pcg_launch_chunk(const PcgArgs& a, bool mixed, int chunk, int& L, hipStream_t st,
                      hipEvent_t* ev, PcgLog* hout = nullptr, int jsample = -1) {
     if (jsample < 0) jsample = chunk / 2;
@@ -1838,17 +1880,19 @@ int pcg_launch_chunk(const PcgArgs& a, bool mixed, int chunk, int& L, hipStream_
         if (ev) hipExtLaunchKernelGGL(K, grid, block, 0, st, E0, E1, 0, __VA_ARGS__); \
         else hipLaunchKernelGGL(K, grid, block, 0, st, __VA_ARGS__);                \
     } while (0)
-    // two-kernel form: x every other update (the chunk's last update of an
-    // odd chunk adds its own term), unless the tunable asks for every one
-    const bool xpair = gf::tunable(gf::GF_TUNE_PCG_X_EVERY) != 1;
+    // two-kernel form: x on every xe-th update and on the chunk's last one
+    // (tunable pcg_x_every, 1..4; default 4)
+    int xe = gf::tunable(gf::GF_TUNE_PCG_X_EVERY);
+    xe = xe >= 1 && xe <= X_MAX_PEND + 1 ? xe : X_MAX_PEND + 1;
     const int c0 = (L - 1) / pcg_chunk_launches(a, chunk, mixed);  // chunks before this one
     for (int j = 0; j < chunk; ++j) {
         const int i = 0;
         ev = j == jsample ? ev_all : nullptr;
-        const int xmode = !xpair ? X_NOW : ((j & 1) ? X_PAIR : (j + 1 < chunk ? X_DEFER : X_NOW));
+        const bool xadd = j % xe == xe - 1 || j + 1 == chunk;
+        const int xmode = xadd ? j % xe : X_DEFER;
         // direction buffer of the k-th direction kernel since the setup
-        // (k = 1, 2, ...): p_old in (k-1)&1, p_new in k&1, for every scene
-        // still iterating (a stopped scene passes through)
+        // (k = 1, 2, ...): p_old in ring slot (k-1)&3, p_new in k&3, for every
+        // scene still iterating (a stopped scene passes through)
         const int k = c0 * chunk + j + 1;
         if (a.variant == PCG_CG1) {
             ++L;
@@ -1859,15 +1903,15 @@ int pcg_launch_chunk(const PcgArgs& a, bool mixed, int chunk, int& L, hipStream_
         } else {
             ++L;
             if (mixed)
-                GF_TLAUNCH((k_pcg_dirspmv<float, true>), e0(i), e1(i), a, L, (k - 1) & 1);
+                GF_TLAUNCH((k_pcg_dirspmv<float, true>), e0(i), e1(i), a, L, (k - 1) & 3);
             else
-                GF_TLAUNCH((k_pcg_dirspmv<double, false>), e0(i), e1(i), a, L, (k - 1) & 1);
+                GF_TLAUNCH((k_pcg_dirspmv<double, false>), e0(i), e1(i), a, L, (k - 1) & 3);
             ++L;
             if (mixed)
-                GF_TLAUNCH((k_pcg_update<float, true>), e0(i + 1), e1(i + 1), a, L, xmode, k & 1);
+                GF_TLAUNCH((k_pcg_update<float, true>), e0(i + 1), e1(i + 1), a, L, xmode, k & 3);
             else
                 GF_TLAUNCH((k_pcg_update<double, false>), e0(i + 1), e1(i + 1), a, L, xmode,
-                           k & 1);
+                           k & 3);
         }
     }
     const int ic = kpi;
@@ -1991,14 +2035,19 @@ extern "C" int gf_scene_pcg_sync(const gf_scene_t* s, int n_active, int precisio
     if (e != hipSuccess) rc = gf::set_error((int)e, "gf_scene_pcg_sync: readback: %s", hipGetErrorString(e));
     else hs = aggregate(hlog, L);
 
-    // the timed iteration of a chunk: the middle one in slot 0, the one
-    // before it in slot 1 (slots alternate)
-    auto jsample = [&](int sl) { return chunk / 2 - (chunk >= 2 ? sl : 0); };
+    // the timed iteration of a chunk: its position cycles through the chunk
+    // over consecutive timed chunks (of all solves in the process), so that
+    // the iterations that add x's pending terms are sampled as often as
+    // they occur
+    static std::atomic<unsigned> sample_rot{0};
+    int js[2] = {0, 0};
     int launched = 0, next_slot = 0, npend = 0, pend[2] = {0, 0}, pend_L[2] = {0, 0};
     auto enqueue_chunk = [&]() -> int {
         const int sl = next_slot;
+        js[sl] = timed ? (int)(sample_rot.fetch_add(1, std::memory_order_relaxed) % (unsigned)chunk)
+                       : 0;
         int r = pcg_launch_chunk(a, mixed, chunk, L, st, timed ? ev[sl] : nullptr,
-                                 dlog + (size_t)2 * S * (1 + sl), jsample(sl));
+                                 dlog + (size_t)2 * S * (1 + sl), js[sl]);
         if (r) return r;
         hipError_t ce = hipEventRecord(done_ev[sl], st);
         if (ce != hipSuccess)
@@ -2030,7 +2079,7 @@ extern "C" int gf_scene_pcg_sync(const gf_scene_t* s, int n_active, int precisio
         hs = aggregate(hlog + (size_t)2 * S * (1 + sl), Lsl);
         if (timed) {
             const int real = hs.iters - iters_before;
-            if (real > jsample(sl)) {  // the sampled iteration did work
+            if (real > js[sl]) {  // the sampled iteration did work
                 for (int m = 0; m < kpi; ++m) {
                     float t0 = 0.f;
                     (void)hipEventElapsedTime(&t0, ev[sl][2 * m], ev[sl][2 * m + 1]);

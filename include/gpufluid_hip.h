@@ -57,10 +57,11 @@ int gf_init(int device, int* cus);
  * 1024, maximum 2048); "pcg_variant" — 1: two kernels per iteration (standard
  * PCG), 2: one kernel per iteration (Chronopoulos-Gear form), 0 (default):
  * the one-kernel form below 3072 active tiles, else the two-kernel form;
- * "pcg_x_every" — two-kernel form: 1: the iterate x is updated by every
- * update kernel, 2 or 0 (default): every other update kernel adds the two
- * pending terms (the same additions in the same order, so the results are
- * bit-identical; it saves one read and one write of x per two iterations).
+ * "pcg_x_every" — two-kernel form: n = 1..4 (0 or above 4: 4, the default):
+ * every n-th update kernel, and a check chunk's last, adds the pending terms
+ * alpha p of the iterate x (the same additions in the same order, so the
+ * results are bit-identical; it saves n-1 reads and writes of x per n
+ * iterations, for n-1 reads of the pending directions).
  * Used by the autotuner (gpufluid_amd/autotune.py).  No reference
  * counterpart (Warp picks its own launch shapes). */
 int gf_tune_set(const char* key, int value);

@@ -200,27 +200,28 @@ def test_pcg_reaches_1e8_and_matches_direct_solve(precision, pcg_variant):
 
 
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
-def test_pcg_x_every_other_update_is_bit_identical(precision):
-    """Two-kernel PCG: deferring x += alpha p to the next update kernel (which
-    then adds both terms in iteration order) gives the same bits as adding
-    it in every update: pressure, iteration count, residual and replacement
-    count, for even and odd chunks and for solves that stop (by tolerance or
-    by the iteration cap) right after an update that left its term pending,
-    where the direction kernel adds it."""
+def test_pcg_x_deferral_is_bit_identical(precision):
+    """Two-kernel PCG: deferring x += alpha p over up to three update kernels
+    (the next flushing one adds the pending terms in iteration order) gives
+    the same bits as adding it in every update: pressure, iteration count,
+    residual and replacement count, for chunks of several lengths and for
+    solves that stop (by tolerance or by the iteration cap) right after an
+    update that left one, two or three terms pending, where the direction
+    kernel adds them."""
     from gpufluid_amd import _lib, autotune
     s = _prepared(shape=(32, 32, 32), pcg_precision=precision)
     if precision == "fp64":
         s._sc.ensure_fp64()
     d = s._descriptor()
     prec = _lib.GF_PCG_MIXED if precision == "mixed" else _lib.GF_PCG_FP64
-    deferred_stops = 0
+    pending_at_stop = set()
     autotune.set_pcg_variant(1)
     try:
         for check_every, tol, cap in [(10, 1e-8, 500), (7, 1e-8, 500), (4, 1e-5, 500),
                                       (10, 1e-6, 500), (3, 1e-7, 500), (10, 1e-8, 37),
-                                      (6, 1e-8, 41)]:
+                                      (6, 1e-8, 41), (10, 1e-8, 22), (9, 1e-8, 30)]:
             got = {}
-            for xe in (1, 2):
+            for xe in (1, 2, 3, 4):
                 autotune.set_pcg_x_every(xe)
                 it, res, nrep = ctypes.c_int(), ctypes.c_double(), ctypes.c_int()
                 _lib.call("S2.6.3", "gf_scene_pcg_sync", ctypes.byref(d), s.n_active_tiles, prec,
@@ -229,17 +230,19 @@ def test_pcg_x_every_other_update_is_bit_identical(precision):
                 s._cache.clear()
                 got[xe] = (s.p.numpy().copy(), it.value, res.value, nrep.value)
             print(f"[pcg x] {precision} chunk {check_every} tol {tol} cap {cap}: "
-                  f"{got[1][1:]} vs {got[2][1:]}", flush=True)
-            np.testing.assert_array_equal(got[2][0], got[1][0])
-            assert got[2][1:] == got[1][1:]
-            # the update after which the solve stopped was a deferring one
-            # (even position in its chunk, not the last of an odd chunk)
+                  + " vs ".join(str(got[xe][1:]) for xe in got), flush=True)
+            for xe in (2, 3, 4):
+                np.testing.assert_array_equal(got[xe][0], got[1][0])
+                assert got[xe][1:] == got[1][1:]
+            # terms pending (with x added every 4th update) after the update at
+            # which the solve stopped: its position j in its chunk
             j = (got[1][1] - 1) % check_every
-            deferred_stops += j % 2 == 0 and j + 1 < check_every
+            if j % 4 != 3 and j + 1 < check_every:
+                pending_at_stop.add(j % 4 + 1)
     finally:
         autotune.set_pcg_x_every(0)
         autotune.set_pcg_variant(0)
-    assert deferred_stops > 0
+    assert pending_at_stop == {1, 2, 3}
 
 
 def test_pcg_reference_sign_is_negated(pcg_variant):
