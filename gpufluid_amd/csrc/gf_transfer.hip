@@ -1036,17 +1036,43 @@ __global__ __launch_bounds__(GF_WG) void k_p2g_finish(P2GParams a,
 
     const float gdt =
         a.tile_scene ? a.scene_par[gf::SCENE_PAR * max(a.tile_scene[t], 0)] : a.g * a.dt;
+    const int slot = a.tile_map[t];
+    // A tile away from the grid boundary with no particle and no solid cell in
+    // its 27-tile neighbourhood (so no fluid cell either) gets what the face
+    // loop below computes for it: no weight, so u = w = 0, v = g dt (gravity
+    // added to 0), and the saved pre-gravity faces 0; no BC applies.
+    if (!any_has && !any_solid && slot < 0 && ti > 0 && ti < a.tnx - 1 && tj > 0 &&
+        tj < a.tny - 1 && tk > 0 && tk < a.tnz - 1) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const int fnj = ny + (c == 1), fnk = nz + (c == 2);
+            float* gf_ = c == 0 ? a.u : (c == 1 ? a.v : a.w);
+            float* gs = c == 0 ? a.us : (c == 1 ? a.vs : a.ws);
+            const float val = c == 1 ? gdt : 0.0f;
+            for (int e = threadIdx.x; e < GF_TILE_CELLS; e += GF_WG) {
+                const int li = e >> 6, lj = (e >> 3) & 7, lk = e & 7;
+                const size_t f = ((size_t)(i0 + li) * fnj + (j0 + lj)) * fnk + (k0 + lk);
+                gf_[f] = val;
+                gs[f] = 0.0f;
+            }
+        }
+        return;
+    }
     const gf::Frame F = gf::tile_frame(a.tile_scene, a.scene_par, t, a.dx, nx, ny, nz);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        const int DI = c == 0 ? 9 : 8, DJ = c == 1 ? 9 : 8, DK = c == 2 ? 9 : 8;
+        const int DJ = c == 1 ? 9 : 8, DK = c == 2 ? 9 : 8;
         const int RJ = c == 1 ? 9 : 10, RK = c == 2 ? 9 : 10;  // staged region dims
         const int fni = nx + (c == 0), fnj = ny + (c == 1), fnk = nz + (c == 2);
         float* gf_ = c == 0 ? a.u : (c == 1 ? a.v : a.w);
         // staged-region strides: one tile along x, y, z (in floats)
         const ptrdiff_t sk = STAGE_PER_TILE, sj = sk * a.tnz, si = sj * a.tny;
         const float* stage_c = stage + (size_t)t * STAGE_PER_TILE + (size_t)c * 2 * OWN_REGION;
-        for (int idx = threadIdx.x; idx < P2G_REGION; idx += GF_WG) {
+        // a thread's (at most three) faces of the component's region, unrolled
+#pragma unroll
+        for (int r = 0; r < (P2G_REGION + GF_WG - 1) / GF_WG; ++r) {
+            const int idx = (int)threadIdx.x + r * GF_WG;
+            if (idx >= P2G_REGION) continue;
             const int lk = idx % DK, lj = (idx / DK) % DJ, li = idx / (DK * DJ);
             const int ii = i0 + li, jj = j0 + lj, kk = k0 + lk;
             if (ii >= fni || jj >= fnj || kk >= fnk) continue;
@@ -1055,7 +1081,7 @@ __global__ __launch_bounds__(GF_WG) void k_p2g_finish(P2GParams a,
                                (c == 2 ? (lk < 8 || kk == nz) : true);
             const size_t f = ((size_t)ii * fnj + jj) * fnk + kk;
             // partial sums of the tiles whose staged regions hold this face:
-            // offsets d in [dlo, dhi] per axis, ascending (-1 only at local
+            // offsets d in [lo, hi] per axis, ascending (-1 only at local
             // coordinate 0, +1 only at the last one)
             float sv = 0.0f, sw = 0.0f;
             const int ilo = li == 0 ? -1 : 0, ihi = li == (c == 0 ? 8 : 7) ? 1 : 0;
@@ -1067,22 +1093,33 @@ __global__ __launch_bounds__(GF_WG) void k_p2g_finish(P2GParams a,
             const float* own = stage_c + ((li + (c != 0)) * RJ + lj + (c != 1)) * RK + lk +
                                (c != 2);
             if (any_has) {
-                if ((ilo | ihi | jlo | jhi | klo | khi) == 0) {
-                    if (s_has[13]) {
-                        sv = own[0];
-                        sw = own[OWN_REGION];
-                    }
-                } else {
-                    for (int di = ilo; di <= ihi; ++di)
-                        for (int dj = jlo; dj <= jhi; ++dj)
-                            for (int dk = klo; dk <= khi; ++dk) {
-                                if (!s_has[(di + 1) * 9 + (dj + 1) * 3 + (dk + 1)]) continue;
-                                const float* st =
-                                    own + (ptrdiff_t)di * (si - 8 * RJ * RK) +
-                                    (ptrdiff_t)dj * (sj - 8 * RK) + (ptrdiff_t)dk * (sk - 8);
-                                sv += st[0];
-                                sw += st[OWN_REGION];
-                            }
+                // the (at most 2 x 2 x 2) offsets in ascending order, per axis
+                // d = lo + b for b < n (n = 2 at a region border).  Every
+                // load is issued (an absent one reads this tile's own entry,
+                // always in the buffer) and masked to +0 afterwards, so all
+                // are in flight together; adding +0 leaves the partial sum
+                // (never -0) unchanged, so the sum is the one over the
+                // present offsets in the same order.
+                const int ni = 1 + ((ilo | ihi) != 0), nj = 1 + ((jlo | jhi) != 0),
+                          nk = 1 + ((klo | khi) != 0);
+                float cv[8], cw[8];
+#pragma unroll
+                for (int m = 0; m < 8; ++m) {
+                    const int bi = m >> 2, bj = (m >> 1) & 1, bk = m & 1;
+                    const int di = ilo + bi, dj = jlo + bj, dk = klo + bk;
+                    const bool ok = bi < ni && bj < nj && bk < nk &&
+                                    s_has[(di + 1) * 9 + (dj + 1) * 3 + (dk + 1)];
+                    const ptrdiff_t off = (ptrdiff_t)di * (si - 8 * RJ * RK) +
+                                          (ptrdiff_t)dj * (sj - 8 * RK) + (ptrdiff_t)dk * (sk - 8);
+                    const float* st = own + (ok ? off : 0);
+                    const uint32_t keep = 0u - (uint32_t)ok;
+                    cv[m] = __uint_as_float(__float_as_uint(st[0]) & keep);
+                    cw[m] = __uint_as_float(__float_as_uint(st[OWN_REGION]) & keep);
+                }
+#pragma unroll
+                for (int m = 0; m < 8; ++m) {
+                    sv += cv[m];
+                    sw += cw[m];
                 }
             }
             float val = sw > 1.0e-8f ? sv / sw : 0.0f;
@@ -1113,7 +1150,6 @@ __global__ __launch_bounds__(GF_WG) void k_p2g_finish(P2GParams a,
             }
         }
     }
-    const int slot = a.tile_map[t];
     if (slot < 0) return;
     __syncthreads();
 
