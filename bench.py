@@ -351,6 +351,18 @@ def main():
                 "pcg_total_own_GBps": round(own_bytes / (pcg_ms * 1e-3) / 1e9, 1),
                 "pcg_total_own_frac": round(own_bytes / (pcg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 
+    # The GPU's own account of the timed window: HIP events on the stream
+    # around the timed steps against the host wall clock, and the kernels'
+    # execution per step (stage events of the instrumented steps, PCG from the
+    # kernels' dispatch-packet events) as a fraction of the event window.
+    kernel_ms = sum(sec.get(k, 0.0) for k in ("clear", "p2g", "g2p_advect")) / n_instr + \
+        pcg_ms / n_instr
+    gpu_window = {"event_ms_per_step": round(ms_step, 4),
+                  "wall_ms_per_step": round(wall * 1e3 / args.steps, 4),
+                  "kernel_ms_per_step": round(float(kernel_ms), 4),
+                  "busy_frac": round(float(kernel_ms) / ms_step, 4)}
+    log(f"gpu window: {gpu_window}")
+
     pcie = pcie_inclusive(s, npart, ms_step)
     log(f"pcie-inclusive: {pcie}")
 
@@ -427,6 +439,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "stages": stages,
+        "gpu_window": gpu_window,
         "pcg_vs_fp64": fp64,
         "pcie_inclusive_value": pcie,
         "attributes": attrs,

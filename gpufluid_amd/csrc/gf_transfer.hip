@@ -1192,6 +1192,9 @@ struct G2PParams {
     float domx, domy, domz;
 };
 
+// Stands in for an absent value in loads that are issued unconditionally.
+__device__ const float g_zero_f = 0.0f;
+
 // Face region a tile samples: component c spans [o, o+9) on its own axis with
 // origin at the tile corner and [o-1, o+9) (10 entries) on the other axes.
 constexpr int G2P_REGION = 900;
@@ -1342,23 +1345,44 @@ __global__ __launch_bounds__(GF_WG) void k_g2p_tile(G2PParams a) {
                       (tk > 0 && a.tile_map[t - 1] >= 0);
     if (rs.total == 0 && !near_fluid) return;
 
-    for (int i = threadIdx.x; i < 1000; i += GF_WG) {
-        const int ca = i / 100, cb = (i / 10) % 10, cc = i % 10;
-        const int ci = i0 - 1 + ca, cj = j0 - 1 + cb, ck = k0 - 1 + cc;
-        int8_t m = 0;
-        float pv = 0.0f;
-        if (ci >= 0 && ci < nx && cj >= 0 && cj < ny && ck >= 0 && ck < nz) {
-            m = (int8_t)a.marker[(ci * ny + cj) * nz + ck];
-            if (m == 1) {
-                const int tt = ((ci >> 3) * a.tny + (cj >> 3)) * a.tnz + (ck >> 3);
-                const int slot = a.tile_map[tt];
-                if (slot >= 0)
-                    pv = a.pc[(size_t)slot * GF_TILE_CELLS +
-                              (((ci & 7) * 8 + (cj & 7)) * 8 + (ck & 7))];
-            }
+    // The cell box around the tile: markers, then the tiles' slots, then the
+    // pressures, each step's loads for all of a thread's cells issued
+    // together (out-of-grid or non-fluid cells read an in-bounds stand-in and
+    // are masked).
+    {
+        constexpr int NB = (1000 + GF_WG - 1) / GF_WG;
+        int mk[NB], tt[NB], off[NB], sl[NB];
+        float pv[NB];
+#pragma unroll
+        for (int r = 0; r < NB; ++r) {
+            const int i = min((int)threadIdx.x + r * GF_WG, 999);
+            const int ca = i / 100, cb = (i / 10) % 10, cc = i % 10;
+            const int ci = i0 - 1 + ca, cj = j0 - 1 + cb, ck = k0 - 1 + cc;
+            const bool in = ci >= 0 && ci < nx && cj >= 0 && cj < ny && ck >= 0 && ck < nz;
+            mk[r] = a.marker[in ? (ci * ny + cj) * nz + ck : 0];
+            mk[r] = in ? mk[r] : 0;
+            tt[r] = in ? ((ci >> 3) * a.tny + (cj >> 3)) * a.tnz + (ck >> 3) : 0;
+            off[r] = ((ci & 7) * 8 + (cj & 7)) * 8 + (ck & 7);
         }
-        s_mk[ca][cb][cc] = m;
-        s_p[ca][cb][cc] = pv;
+#pragma unroll
+        for (int r = 0; r < NB; ++r) {
+            sl[r] = a.tile_map[mk[r] == 1 ? tt[r] : 0];
+            sl[r] = mk[r] == 1 ? sl[r] : -1;
+        }
+#pragma unroll
+        for (int r = 0; r < NB; ++r) {
+            const float* src =
+                sl[r] >= 0 ? a.pc + (size_t)sl[r] * GF_TILE_CELLS + off[r] : &g_zero_f;
+            pv[r] = *src;
+        }
+#pragma unroll
+        for (int r = 0; r < NB; ++r) {
+            const int i = (int)threadIdx.x + r * GF_WG;
+            if (i >= 1000) continue;
+            const int ca = i / 100, cb = (i / 10) % 10, cc = i % 10;
+            s_mk[ca][cb][cc] = (int8_t)mk[r];
+            s_p[ca][cb][cc] = pv[r];
+        }
     }
     __syncthreads();
 
@@ -1367,17 +1391,43 @@ __global__ __launch_bounds__(GF_WG) void k_g2p_tile(G2PParams a) {
     const float scale = a.dt / (a.rho * F.dx);
     const float gdt =
         a.tile_scene ? a.scene_par[gf::SCENE_PAR * max(a.tile_scene[t], 0)] : a.gdt;
+    // The face regions: first the saved (and pre-projection) values of all a
+    // thread's faces, loads only, so they are in flight together; then the
+    // gradient subtract, BC and stores.
+    constexpr int NF = (G2P_REGION + GF_WG - 1) / GF_WG;
+    float fsav[3][NF], fpre[3][NF];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        const int DI = c == 0 ? 9 : 10, DJ = c == 1 ? 9 : 10, DK = c == 2 ? 9 : 10;
+        const int DJ = c == 1 ? 9 : 10, DK = c == 2 ? 9 : 10;
+        const int oi = c == 0 ? i0 : i0 - 1, oj = c == 1 ? j0 : j0 - 1,
+                  ok = c == 2 ? k0 : k0 - 1;
+        const int fni = nx + (c == 0), fnj = ny + (c == 1), fnk = nz + (c == 2);
+        const float* gsav = c == 0 ? a.us : (c == 1 ? a.vs : a.ws);
+        const float* gpre = c == 0 ? a.pu : (c == 1 ? a.pv : a.pw);
+#pragma unroll
+        for (int r = 0; r < NF; ++r) {
+            const int idx = min((int)threadIdx.x + r * GF_WG, G2P_REGION - 1);
+            const int lk = idx % DK, lj = (idx / DK) % DJ, li = idx / (DK * DJ);
+            const int ii = oi + li, jj = oj + lj, kk = ok + lk;
+            const bool in = ii >= 0 && ii < fni && jj >= 0 && jj < fnj && kk >= 0 && kk < fnk;
+            const size_t f = in ? ((size_t)ii * fnj + jj) * fnk + kk : 0;
+            fsav[c][r] = gsav[f];
+            fpre[c][r] = gpre ? gpre[f] : 0.0f;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const int DJ = c == 1 ? 9 : 10, DK = c == 2 ? 9 : 10;
         const int oi = c == 0 ? i0 : i0 - 1, oj = c == 1 ? j0 : j0 - 1,
                   ok = c == 2 ? k0 : k0 - 1;
         const int fni = nx + (c == 0), fnj = ny + (c == 1), fnk = nz + (c == 2);
         float* gfld = c == 0 ? a.u : (c == 1 ? a.v : a.w);
-        const float* gsav = c == 0 ? a.us : (c == 1 ? a.vs : a.ws);
         const float* solid = c == 0 ? a.su : (c == 1 ? a.sv : a.sw);
-        const float* gpre = c == 0 ? a.pu : (c == 1 ? a.pv : a.pw);
-        for (int idx = threadIdx.x; idx < G2P_REGION; idx += GF_WG) {
+        const bool has_pre = (c == 0 ? a.pu : (c == 1 ? a.pv : a.pw)) != nullptr;
+#pragma unroll
+        for (int r = 0; r < NF; ++r) {
+            const int idx = (int)threadIdx.x + r * GF_WG;
+            if (idx >= G2P_REGION) continue;
             const int lk = idx % DK, lj = (idx / DK) % DJ, li = idx / (DK * DJ);
             const int ii = oi + li, jj = oj + lj, kk = ok + lk;
             float val = 0.0f, sav = 0.0f;
@@ -1387,8 +1437,8 @@ __global__ __launch_bounds__(GF_WG) void k_g2p_tile(G2PParams a) {
                 // as P2G produced it (us [+ g*dt on v], then the BC below), or
                 // read from a separate pre-projection array, so the halo never
                 // reads u while owners overwrite it.
-                sav = gsav[f];
-                val = gpre ? gpre[f] : (c == 1 ? sav + gdt : sav);
+                sav = fsav[c][r];
+                val = has_pre ? fpre[c][r] : (c == 1 ? sav + gdt : sav);
                 // Cell coords of the face's two sides in the staged cell region.
                 const int ca = ii - (i0 - 1), cb = jj - (j0 - 1), cc = kk - (k0 - 1);
                 int la, lb, lc;  // low-side cell
