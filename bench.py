@@ -136,7 +136,7 @@ def _run_cpu(exe, td, scene, substeps, threads, max_iter=None):
         np.concatenate([scene.pos, scene.vel], axis=1).astype(np.float32).tofile(pfile)
     cmd = [str(exe), str(scene.shape[0]), repr(scene.dx), repr(p.get("dt", 1 / 240)),
            repr(p.get("tol", 1e-8)), str(max_iter or p.get("max_iter", 500)), str(substeps),
-           str(pfile), str(len(scene.pos))]
+           str(pfile), str(len(scene.pos)), "-", p.get("transfer", "flip")]
     env = dict(os.environ, OMP_NUM_THREADS=str(threads))
     r = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=600)
     if r.returncode != 0:
@@ -151,8 +151,9 @@ def cpu_baseline(scene):
     """Time the C restatement of the reference algorithm (oracle/flip_cpu.c,
     checked against the numpy oracle by tests/test_cpu_restatement.py) on this
     host's cores: one substep of the benchmark scene on every CPU this process
-    may use, and the 32^3 dam break (C1) for its full 100 substeps on all of
-    them and on one thread.  Compiled for the host CPU in a temporary
+    may use, one substep of the 128^3 APIC dam break (C2) on all of them, and
+    the 32^3 dam break (C1) for its full 100 substeps on all of them and on
+    one thread.  Compiled for the host CPU in a temporary
     directory (the prebuilt oracle/_build/flip_cpu is the fallback)."""
     import tempfile
     from gpufluid_amd import seeding
@@ -175,9 +176,12 @@ def cpu_baseline(scene):
         c1 = seeding.config_c1()
         c1_all = _run_cpu(exe, td, c1, c1.params["substeps"], threads, max_iter=1000)
         c1_one = _run_cpu(exe, td, c1, c1.params["substeps"], 1, max_iter=1000)
+        c2 = seeding.config_c2()
+        c2_all = _run_cpu(exe, td, c2, 1, threads)
     main["cores"] = threads
     main["topology"] = topo
     main["c1_100_substeps"] = {"all_threads": c1_all, "one_thread": c1_one}
+    main["c2_one_substep"] = c2_all
     return main
 
 
@@ -419,6 +423,7 @@ def main():
                    "cores": c["cores"], "kind": "port", "sample": c.get("sample"),
                    "topology": c.get("topology"), "pcg_iters": c.get("iters"),
                    "seconds": c.get("seconds"), "c1_100_substeps": c.get("c1_100_substeps"),
+                   "c2_one_substep": c.get("c2_one_substep"),
                    "label": "reference-algorithm CPU restatement (oracle/flip_cpu.c, FP32 "
                             "Jacobi-PCG checked every 10 iterations, dense grid)"}
             log(f"cpu baseline: {cpu}")

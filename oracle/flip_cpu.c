@@ -17,12 +17,17 @@
  *                             is negated to the Jacobi-consistent pressure)
  *   gradient subtract         solver3d.py:1364-1407
  *   G2P + advection (FLIP)    solver3d.py:1414-1463, primitives/gridmath.py:30-65
+ *   APIC P2G / G2P            solver3d.py:1475-1588 / 1591-1732 (transfer "apic":
+ *                             corners receive v + C row . (x_face - x_p); the
+ *                             gather clamps corner indices and rebuilds C)
  * The reference's float atomics are replaced by a race-free schedule: particles
  * are binned into x-slabs of 4 cells and even and odd slabs are scattered in two
  * parallel phases.  Dot products accumulate in double.
  *
- * usage: flip_cpu N DX DT TOL MAX_ITER SUBSTEPS PARTICLES_F32 NP
- *   PARTICLES_F32 holds NP rows of (x, y, z, vx, vy, vz) float32.
+ * usage: flip_cpu N DX DT TOL MAX_ITER SUBSTEPS PARTICLES_F32 NP [OUT_F32 [flip|apic]]
+ *   PARTICLES_F32 holds NP rows of (x, y, z, vx, vy, vz) float32; OUT_F32 ("-":
+ *   none) receives the final positions, velocities and (APIC) the NP x 3 x 3
+ *   affine matrices, which start at zero.
  * Prints one JSON line: seconds, Mparticle-substeps/s, PCG iterations.
  */
 #include <math.h>
@@ -66,6 +71,64 @@ static void scatter_face(float* f, float* wgt, float val, float fx, float fy, fl
                 f[o] += val * w;
                 wgt[o] += w;
             }
+}
+
+/* k3_p2g_apic's corner scatter (solver3d.py:1475-1588): the value at a corner
+ * is v + C[0]*(xf - px) + C[1]*(yf - py) + C[2]*(zf - pz), with the corner's
+ * world position (index + 0 on the component's own axis, + 0.5 on the others)
+ * times dx; corners outside the face array are dropped. */
+static void scatter_face_apic(float* f, float* wgt, float val, const float* crow, float px,
+                              float py, float pz, float dx, int comp, float fx, float fy, float fz,
+                              int NX, int NY, int NZ) {
+    int i0 = (int)floorf(fx), j0 = (int)floorf(fy), k0 = (int)floorf(fz);
+    float sx = fx - (float)i0, sy = fy - (float)j0, sz = fz - (float)k0;
+    const float ox = comp == 0 ? 0.0f : 0.5f, oy = comp == 1 ? 0.0f : 0.5f,
+                oz = comp == 2 ? 0.0f : 0.5f;
+    for (int di = 0; di < 2; ++di)
+        for (int dj = 0; dj < 2; ++dj)
+            for (int dk = 0; dk < 2; ++dk) {
+                int ii = i0 + di, jj = j0 + dj, kk = k0 + dk;
+                if (ii < 0 || ii >= NX || jj < 0 || jj >= NY || kk < 0 || kk >= NZ) continue;
+                float wx = di ? sx : 1.0f - sx, wy = dj ? sy : 1.0f - sy, wz = dk ? sz : 1.0f - sz;
+                float wt = wx * wy * wz;
+                float fxw = ((float)ii + ox) * dx - px;
+                float fyw = ((float)jj + oy) * dx - py;
+                float fzw = ((float)kk + oz) * dx - pz;
+                float v = val + crow[0] * fxw;
+                v = v + crow[1] * fyw;
+                v = v + crow[2] * fzw;
+                size_t q = IDX(ii, jj, kk, NY, NZ);
+                f[q] += v * wt;
+                wgt[q] += wt;
+            }
+}
+
+/* k3_g2p_apic_advect's gather of one component (solver3d.py:1591-1732):
+ * corner indices clamped into the face array, velocity = sum w*u, and the
+ * affine row C[a] += w*u*(3/dx^2) * (x_face[a] - x_p[a]). */
+static float gather_apic(const float* f, float px, float py, float pz, float dx, float inv,
+                         int comp, float fx, float fy, float fz, int NX, int NY, int NZ,
+                         float* crow) {
+    int i0 = (int)floorf(fx), j0 = (int)floorf(fy), k0 = (int)floorf(fz);
+    float sx = fx - (float)i0, sy = fy - (float)j0, sz = fz - (float)k0;
+    const float off[3] = {comp == 0 ? 0.0f : 0.5f, comp == 1 ? 0.0f : 0.5f,
+                          comp == 2 ? 0.0f : 0.5f};
+    const float pp[3] = {px, py, pz};
+    float vel = 0.0f;
+    crow[0] = crow[1] = crow[2] = 0.0f;
+    for (int di = 0; di < 2; ++di)
+        for (int dj = 0; dj < 2; ++dj)
+            for (int dk = 0; dk < 2; ++dk) {
+                int idx[3] = {clampi(i0 + di, 0, NX - 1), clampi(j0 + dj, 0, NY - 1),
+                              clampi(k0 + dk, 0, NZ - 1)};
+                float wx = di ? sx : 1.0f - sx, wy = dj ? sy : 1.0f - sy, wz = dk ? sz : 1.0f - sz;
+                float wt = wx * wy * wz;
+                float val = f[IDX(idx[0], idx[1], idx[2], NY, NZ)];
+                vel += wt * val;
+                float fu = wt * val * inv;
+                for (int a = 0; a < 3; ++a) crow[a] += fu * (((float)idx[a] + off[a]) * dx - pp[a]);
+            }
+    return vel;
 }
 
 static float sample3(const float* f, float fx, float fy, float fz, int nmx, int nmy, int nmz) {
@@ -216,7 +279,7 @@ static void enforce_bc(Grid* G) {
             }
 }
 
-static int substep(Grid* G, float* pos, float* vel, int np, float tol, int max_iter,
+static int substep(Grid* G, float* pos, float* vel, float* cmat, int np, float tol, int max_iter,
                    int32_t* order, int32_t* slab_start, int nslab, int slab_w) {
     const int nx = G->nx, ny = G->ny, nz = G->nz;
     const size_t C = (size_t)nx * ny * nz;
@@ -253,9 +316,19 @@ static int substep(Grid* G, float* pos, float* vel, int np, float tol, int max_i
                 size_t c = IDX(ci, cj, ck, ny, nz);
                 if (G->marker[c] != 2) G->marker[c] = 1;
                 float fx = x / dx, fy = y / dx, fz = z / dx;
-                scatter_face(G->u, G->uw, vel[3 * p], fx, fy - 0.5f, fz - 0.5f, nx + 1, ny, nz);
-                scatter_face(G->v, G->vw, vel[3 * p + 1], fx - 0.5f, fy, fz - 0.5f, nx, ny + 1, nz);
-                scatter_face(G->w, G->ww, vel[3 * p + 2], fx - 0.5f, fy - 0.5f, fz, nx, ny, nz + 1);
+                if (cmat) {
+                    const float* cp = cmat + 9 * (size_t)p;
+                    scatter_face_apic(G->u, G->uw, vel[3 * p], cp, x, y, z, dx, 0, fx, fy - 0.5f,
+                                      fz - 0.5f, nx + 1, ny, nz);
+                    scatter_face_apic(G->v, G->vw, vel[3 * p + 1], cp + 3, x, y, z, dx, 1,
+                                      fx - 0.5f, fy, fz - 0.5f, nx, ny + 1, nz);
+                    scatter_face_apic(G->w, G->ww, vel[3 * p + 2], cp + 6, x, y, z, dx, 2,
+                                      fx - 0.5f, fy - 0.5f, fz, nx, ny, nz + 1);
+                } else {
+                    scatter_face(G->u, G->uw, vel[3 * p], fx, fy - 0.5f, fz - 0.5f, nx + 1, ny, nz);
+                    scatter_face(G->v, G->vw, vel[3 * p + 1], fx - 0.5f, fy, fz - 0.5f, nx, ny + 1, nz);
+                    scatter_face(G->w, G->ww, vel[3 * p + 2], fx - 0.5f, fy - 0.5f, fz, nx, ny, nz + 1);
+                }
             }
         }
     }
@@ -316,8 +389,28 @@ static int substep(Grid* G, float* pos, float* vel, int np, float tol, int max_i
                 }
             }
     enforce_bc(G);
-    /* G2P + advect (1414-1463) */
+    /* G2P + advect (1414-1463; APIC 1591-1732) */
     const float eps = dx * 1.001f;
+    if (cmat) {
+        const float inv = 3.0f / (dx * dx);
+#pragma omp parallel for schedule(static)
+        for (int p = 0; p < np; ++p) {
+            float px = pos[3 * p], py = pos[3 * p + 1], pz = pos[3 * p + 2];
+            float lx = px / dx, ly = py / dx, lz = pz / dx;
+            float* cp = cmat + 9 * (size_t)p;
+            float nu_ = gather_apic(G->u, px, py, pz, dx, inv, 0, lx, ly - 0.5f, lz - 0.5f,
+                                    nx + 1, ny, nz, cp);
+            float nv_ = gather_apic(G->v, px, py, pz, dx, inv, 1, lx - 0.5f, ly, lz - 0.5f,
+                                    nx, ny + 1, nz, cp + 3);
+            float nw_ = gather_apic(G->w, px, py, pz, dx, inv, 2, lx - 0.5f, ly - 0.5f, lz,
+                                    nx, ny, nz + 1, cp + 6);
+            vel[3 * p] = nu_; vel[3 * p + 1] = nv_; vel[3 * p + 2] = nw_;
+            pos[3 * p] = fminf(fmaxf(px + nu_ * dt, eps), G->dom[0] - eps);
+            pos[3 * p + 1] = fminf(fmaxf(py + nv_ * dt, eps), G->dom[1] - eps);
+            pos[3 * p + 2] = fminf(fmaxf(pz + nw_ * dt, eps), G->dom[2] - eps);
+        }
+        return iters;
+    }
 #pragma omp parallel for schedule(static)
     for (int p = 0; p < np; ++p) {
         float lx = pos[3 * p] / dx, ly = pos[3 * p + 1] / dx, lz = pos[3 * p + 2] / dx;
@@ -341,7 +434,13 @@ static int substep(Grid* G, float* pos, float* vel, int np, float tol, int max_i
 
 int main(int argc, char** argv) {
     if (argc < 9) {
-        fprintf(stderr, "usage: %s N DX DT TOL MAX_ITER SUBSTEPS PARTICLES_F32 NP [OUT_F32]\n", argv[0]);
+        fprintf(stderr, "usage: %s N DX DT TOL MAX_ITER SUBSTEPS PARTICLES_F32 NP [OUT_F32 [flip|apic]]\n",
+                argv[0]);
+        return 2;
+    }
+    const int apic = argc > 10 && strcmp(argv[10], "apic") == 0;
+    if (argc > 10 && !apic && strcmp(argv[10], "flip") != 0) {
+        fprintf(stderr, "unknown transfer %s\n", argv[10]);
         return 2;
     }
     Grid G;
@@ -385,25 +484,32 @@ int main(int argc, char** argv) {
             vel[3 * p + a] = buf[6 * p + 3 + a];
         }
     free(buf);
+    float* cmat = NULL;  /* APIC affine matrices, zero at the start */
+    if (apic) {
+        cmat = (float*)xalloc((size_t)np * 36);
+        memset(cmat, 0, (size_t)np * 36);
+    }
     const int slab_w = 4, nslab = (n + slab_w - 1) / slab_w;
     int32_t* order = (int32_t*)xalloc((size_t)np * 4);
     int32_t* slab_start = (int32_t*)xalloc(sizeof(int32_t) * (nslab + 1));
     double t0 = omp_get_wtime();
     int iters = 0;
-    for (int s = 0; s < substeps; ++s) iters = substep(&G, pos, vel, np, tol, max_iter, order, slab_start, nslab, slab_w);
+    for (int s = 0; s < substeps; ++s)
+        iters = substep(&G, pos, vel, cmat, np, tol, max_iter, order, slab_start, nslab, slab_w);
     double t1 = omp_get_wtime();
-    if (argc > 9) {
+    if (argc > 9 && strcmp(argv[9], "-") != 0) {
         FILE* fo = fopen(argv[9], "wb");
         if (fo) {
             fwrite(pos, 4, (size_t)np * 3, fo);
             fwrite(vel, 4, (size_t)np * 3, fo);
+            if (cmat) fwrite(cmat, 4, (size_t)np * 9, fo);
             fclose(fo);
         }
     }
     const double sec = t1 - t0;
     printf("{\"seconds\": %.6f, \"mps\": %.6f, \"iters\": %d, \"threads\": %d, "
-           "\"sample\": \"%d substep(s) of %d^3 with %d particles, Jacobi-PCG cap %d\"}\n",
+           "\"sample\": \"%d substep(s) of %d^3 with %d particles, %s, Jacobi-PCG cap %d\"}\n",
            sec, (double)np * substeps / sec / 1e6, iters, omp_get_max_threads(), substeps, n, np,
-           max_iter);
+           apic ? "APIC" : "FLIP", max_iter);
     return 0;
 }

@@ -32,19 +32,23 @@ def _exe():
     return EXE
 
 
-def _run(tmp_path, n, dx, dt, pos, vel, substeps, tol=1e-6, max_iter=1000, threads=2):
+def _run(tmp_path, n, dx, dt, pos, vel, substeps, tol=1e-6, max_iter=1000, threads=2,
+         transfer="flip"):
     exe = _exe()
     src = tmp_path / "in.f32"
     out = tmp_path / "out.f32"
     np.concatenate([pos, vel], axis=1).astype(np.float32).tofile(src)
     r = subprocess.run([str(exe), str(n), repr(dx), repr(dt), repr(tol), str(max_iter),
-                        str(substeps), str(src), str(len(pos)), str(out)],
+                        str(substeps), str(src), str(len(pos)), str(out), transfer],
                        capture_output=True, text=True, check=True,
                        env={"OMP_NUM_THREADS": str(threads), "PATH": "/usr/bin:/bin"})
     info = json.loads(r.stdout.strip().splitlines()[-1])
     data = np.fromfile(out, dtype=np.float32)
     npart = len(pos)
-    return data[:3 * npart].reshape(npart, 3), data[3 * npart:].reshape(npart, 3), info
+    p, v = data[:3 * npart].reshape(npart, 3), data[3 * npart:6 * npart].reshape(npart, 3)
+    if transfer == "apic":
+        return p, v, info, data[6 * npart:].reshape(npart, 3, 3)
+    return p, v, info
 
 
 @pytest.mark.parametrize("substeps", [1, 3])
@@ -81,3 +85,25 @@ def test_flip_cpu_catches_a_planted_sign_error(tmp_path):
     st, _ = fo.substep(gu.oracle_state(shape, dx, pos, vel), dt, pressure="pcg", iters=1000,
                        tol=1e-10, pcg_sign="reference")
     assert gu.rel_err(cv, st["vel"]) > 1e-2
+
+
+@pytest.mark.parametrize("substeps", [1, 3])
+def test_flip_cpu_apic_matches_the_fp64_oracle(tmp_path, substeps):
+    """The APIC transfer of the C restatement (the CPU baseline of the APIC
+    configuration) against the oracle's APIC substep: positions, velocities
+    and the affine matrices after 1 and 3 substeps from C = 0."""
+    shape, dx, pos, vel = gu.dam_scene(n=16, seed=9, vel_sigma=0.3)
+    dt = 1.0 / 240
+    cp, cv, info, cc = _run(tmp_path, 16, dx, dt, pos, vel, substeps, transfer="apic")
+    assert info["iters"] > 0 and "APIC" in info["sample"]
+    st = gu.oracle_state(shape, dx, pos, vel)
+    for _ in range(substeps):
+        st, _ = fo.substep(st, dt, pressure="exact", transfer="apic")
+    ev, ep = gu.rel_err(cv, st["vel"]), gu.rel_err(cp, st["pos"])
+    ec = gu.rel_err(cc, st["affine"])
+    print(f"flip_cpu apic {substeps} substep(s): rel err vel {ev:.2e} pos {ep:.2e} C {ec:.2e}")
+    assert ev < 1e-5 and ep < 1e-5 and ec < 1e-4
+    # the affine terms matter: the FLIP run of the same scene differs
+    fp, fv, _ = _run(tmp_path, 16, dx, dt, pos, vel, substeps)
+    if substeps > 1:
+        assert gu.rel_err(fv, cv) > 1e-3
