@@ -234,7 +234,8 @@ int gf_scene_gsrb(const gf_scene_t* s, int n_active, int iters, void* stream);
  * number of FP64 residual replacements, and kernel_ms[4]: GPU execution time
  * (HIP events on the launches' dispatch packets on `stream`) of the
  * direction+SpMV (or one-kernel iteration) kernel [0] and the update kernel
- * [1], summed over a sample of one iteration per check_every-chunk, [2] the
+ * [1], summed over a sample of one iteration per check_every-chunk (its
+ * position in the chunk cycling over the calls of the process), [2] the
  * number of sampled iterations, and [3] the summed time of the residual
  * replacements that ran; NULL disables timing. */
 int gf_scene_pcg_sync(const gf_scene_t* s, int n_active, int precision, double tol,
