@@ -1104,15 +1104,15 @@ __device__ __forceinline__ double pend_alpha(const PcgLog& e, int i) {
     return i == 0 ? e.alpha : (i == 1 ? e.alpha2 : e.alpha3);
 }
 
-// x += a[n-1] p[n-1] + ... + a[0] p[0] (when own: then + ao po) on a quad,
-// term by term in iteration order (oldest first), as one update per
-// iteration would add them, so deferring a term changes no bit of x (mixed:
-// x32, first folded into x64 when merge is set).
+// x += a[n-1] p[n-1] + ... + a[0] p[0] on a quad, term by term in iteration
+// order (oldest first), as one update per iteration would add them, so
+// deferring a term changes no bit of x (mixed: x32, first folded into x64 when
+// merge is set).  The direction kernel's flush of a solve that stopped with
+// terms pending; the update kernel adds in the same order (upd_compute).
 template <typename T, bool MIXED>
 __device__ __forceinline__ void x_addn(const PcgArgs& a, size_t idx, int merge, int n,
                                        const T (&al)[X_MAX_PEND],
-                                       const T (&pp)[X_MAX_PEND][4], bool own, T ao,
-                                       const T (&po)[4]) {
+                                       const T (&pp)[X_MAX_PEND][4]) {
     if (MIXED) {
         float xo[4];
         ld4(a.x32 + idx, xo);
@@ -1129,8 +1129,6 @@ __device__ __forceinline__ void x_addn(const PcgArgs& a, size_t idx, int merge, 
         for (int i = X_MAX_PEND - 1; i >= 0; --i)
             if (i < n)
                 for (int k = 0; k < 4; ++k) xo[k] = xo[k] + (float)al[i] * (float)pp[i][k];
-        if (own)
-            for (int k = 0; k < 4; ++k) xo[k] = xo[k] + (float)ao * (float)po[k];
         st4(a.x32 + idx, xo);
     } else {
         double xd[4];
@@ -1139,8 +1137,6 @@ __device__ __forceinline__ void x_addn(const PcgArgs& a, size_t idx, int merge, 
         for (int i = X_MAX_PEND - 1; i >= 0; --i)
             if (i < n)
                 for (int k = 0; k < 4; ++k) xd[k] += (double)al[i] * (double)pp[i][k];
-        if (own)
-            for (int k = 0; k < 4; ++k) xd[k] += (double)ao * (double)po[k];
         st4(a.x64 + idx, xd);
     }
 }
@@ -1286,7 +1282,7 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_dirspmv(PcgArgs a, int L, int pb)
     if (e.done) {
         if (e.xpend) {  // the solve stopped with alpha p of the last updates pending
             const int n = e.xpend;
-            T al[X_MAX_PEND], pp[X_MAX_PEND][4], none[4] = {};
+            T al[X_MAX_PEND], pp[X_MAX_PEND][4];
 #pragma unroll
             for (int i = 0; i < X_MAX_PEND; ++i) al[i] = (T)pend_alpha(e, i);
             FOR_QUADS(P) {
@@ -1294,7 +1290,7 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_dirspmv(PcgArgs a, int L, int pb)
 #pragma unroll
                 for (int i = 0; i < X_MAX_PEND; ++i)
                     if (i < n) ld4(std_p<T>(a, e.pbuf - i) + idx, pp[i]);
-                x_addn<T, MIXED>(a, idx, MIXED ? e.merge : 0, n, al, pp, false, (T)0, none);
+                x_addn<T, MIXED>(a, idx, MIXED ? e.merge : 0, n, al, pp);
             }
             e.xpend = 0;
             e.merge = 0;
