@@ -629,6 +629,67 @@ __device__ __forceinline__ void write_log(const PcgArgs& a, const Part& P, const
     }
 }
 
+// ---- the replacement's last workgroup --------------------------------------
+// The two-kernel form's replacement finishes in the scene's last workgroup
+// to arrive instead of a separate one-workgroup launch.  Its partials are
+// written and read with device-coherent accesses (cache policy sc1, past the
+// caches that are not coherent between XCDs), and each workgroup's stores are
+// acknowledged before it counts itself, so no cache write-back or
+// invalidation (device-scope fences cost about 20 us over 1,024 workgroups on
+// MI355X) is needed.  No workgroup waits for another.  Counters: one per
+// scene in a spare partial kind (set 0), zeroed by the partition kernel and
+// reset by the last workgroup.
+constexpr int P_ARRIVE = P_KINDS;
+static_assert(P_ARRIVE < P_SET, "arrival counters need a spare partial kind");
+__device__ __forceinline__ unsigned* arrive_cnt(double* part) {
+    return reinterpret_cast<unsigned*>(pslot(part, P_ARRIVE, 0));
+}
+__device__ __forceinline__ void st_coh(double* p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double ld_coh(const double* p) {
+    return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// partial_share with device-coherent loads.
+__device__ __forceinline__ double partial_share_coh(const double* part, int kind, int L, int wb,
+                                                    int we) {
+    const double* src = pslot(part, kind, L);
+    double x[PCG_GMAX / GF_WG];
+#pragma unroll
+    for (int q = 0; q < PCG_GMAX / GF_WG; ++q) {
+        const int i = wb + threadIdx.x + q * GF_WG;
+        x[q] = ld_coh(src + (i < PCG_GMAX ? i : PCG_GMAX - 1));
+    }
+    double v = 0.0;
+#pragma unroll
+    for (int q = 0; q < PCG_GMAX / GF_WG; ++q) {
+        const int i = wb + threadIdx.x + q * GF_WG;
+        v += i < we ? x[q] : 0.0;
+    }
+    return v;
+}
+
+// put_partial2 with device-coherent stores, then the arrival count: true (in
+// every thread) in the scene's last workgroup to arrive.
+__device__ __forceinline__ bool put_partial2_last(double* part, int ka, int kb, int L,
+                                                  const Part& P, double va, double vb,
+                                                  double* s_red, int* s_last) {
+    wg_sum2(va, vb, s_red);
+    if (threadIdx.x == 0) {
+        st_coh(pslot(part, ka, L) + P.b, va);
+        st_coh(pslot(part, kb, L) + P.b, vb);
+        // the partials are acknowledged before the workgroup counts itself
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        unsigned* c = arrive_cnt(part) + P.s;
+        const bool last = atomicAdd(c, 1u) + 1u == (unsigned)(P.we - P.wb);
+        if (last) atomicExch(c, 0u);
+        *s_last = last;
+    }
+    __syncthreads();
+    return *s_last != 0;
+}
+
 // Consumes the last iteration's partials (when fresh): gamma = (r,u),
 // delta = (Au,u), r.r; counts the finished iteration and sets the stop /
 // replacement flags.  Identical in every workgroup of the scene.
@@ -708,6 +769,7 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_partition(PcgArgs a) {
             PcgLog e = {};
             e.pending = PEND_SETUP;
             put_log(a.log + 2 * sc, e);
+            arrive_cnt(a.part)[sc] = 0u;
         }
         carry += total;
     }
@@ -1541,9 +1603,24 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_replace(PcgArgs a, int L) {
         }
         st4(z + idx, z32);
     }
-    put_partial2(a.part, P_RRT, P_RZT, L, P.b, rrt, rz, s_red);
-    e.pending = PEND_REPLACE;
-    write_log(a, P, e, L);
+    __shared__ int s_last;
+    if (!put_partial2_last(a.part, P_RRT, P_RZT, L, P, rrt, rz, s_red, &s_last)) return;
+    // the scene's last workgroup: the true residual and the new r.z
+    // (k_pcg_fin's sums, in its order), and the chunk-end decision
+    rrt = partial_share_coh(a.part, P_RRT, L, P.wb, P.we);
+    rz = partial_share_coh(a.part, P_RZT, L, P.wb, P.we);
+    double del = 0.0;
+    wg_sum3(rrt, rz, del, s_red);
+    e.rr_true = rrt; e.rr = rrt; e.rr_upd = rrt; e.rz = rz; e.pap = del;
+    e.upd = 0; e.merge = 1; e.nupd += 1;
+    if (rrt < a.tol2 * e.bb) { e.converged = 1; e.done = 1; }
+    else if (e.iters >= a.max_iter) { e.maxed = 1; e.done = 1; }
+    else e.done = 0;
+    e.pending = PEND_NONE;
+    if (threadIdx.x == 0) {
+        set_live_log(a, P.s, L, e);
+        if (a.hout) put_log(a.hout + 2 * P.s + (L & 1), e);
+    }
 }
 
 // pc = FP32 result (x64 [+ x32]); non-fluid cells hold 0 in both.
@@ -1828,7 +1905,7 @@ int pcg_prepare(const gf_scene_t* s, int n_active, int precision, double tol, in
 // Kernels per iteration and launch numbers used by one chunk of `chunk` iterations.
 int pcg_kernels_per_iter(const PcgArgs& a) { return a.variant == PCG_CG1 ? 1 : 2; }
 int pcg_chunk_launches(const PcgArgs& a, int chunk, bool mixed) {
-    return a.variant == PCG_CG1 ? chunk + 1 + (mixed ? 2 : 0) : 2 * chunk + 1 + (mixed ? 1 : 0);
+    return a.variant == PCG_CG1 ? chunk + 1 + (mixed ? 2 : 0) : 2 * chunk + 1;
 }
 
 void pcg_launch_setup(const PcgArgs& a, bool mixed, hipStream_t st) {
@@ -1922,12 +1999,13 @@ int pcg_launch_chunk(const PcgArgs& a, bool mixed, int chunk, int& L, hipStream_
             GF_TLAUNCH(k_cg1_replace<false>, e0(ic), e1(ic), al, L);
         }
     } else {
+        // the mixed replacement finishes in its last workgroup (no k_pcg_fin)
         if (mixed)
-            GF_TLAUNCH(k_pcg_replace<true>, e0(ic), e1(ic), a, L);
+            GF_TLAUNCH(k_pcg_replace<true>, e0(ic), e1(ic), al, L);
         else
             GF_TLAUNCH(k_pcg_replace<false>, e0(ic), e1(ic), al, L);
     }
-    if (mixed) {
+    if (mixed && a.variant == PCG_CG1) {
         ++L;
         hipLaunchKernelGGL(k_pcg_fin, dim3(a.S), block, 0, st, al, L);
     }
