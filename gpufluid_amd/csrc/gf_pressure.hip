@@ -868,8 +868,8 @@ __global__ __launch_bounds__(GF_WG) void k_cg1_setup2(PcgArgs a) {
     put_partial3(a.part, P_RZ, P_PAP, P_BB, 0, P.b, gam, del, rr, s_red);  // read by fin at L = 1
 }
 
-// Finalisation after the setup or a residual replacement: one workgroup per
-// scene (grid S).
+// Finalisation after the setup or a residual replacement of the one-kernel
+// form: one workgroup per scene (grid S).
 __global__ __launch_bounds__(GF_WG) void k_pcg_fin(PcgArgs a, int L) {
     __shared__ double s_red[3 * GF_WG / GF_WAVE];
     const int sc = blockIdx.x;
@@ -889,8 +889,8 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_fin(PcgArgs a, int L) {
         e.converged = zero;
         e.done = zero || a.max_iter <= 0;
     } else if (e.pending == PEND_REPLACE) {
-        // one-kernel form: replace ran at L - 2, replace2 at L - 1;
-        // two-kernel form: replace at L - 1 (delta unused)
+        // one-kernel form: replace ran at L - 2, replace2 at L - 1 (the
+        // two-kernel form finishes its replacement in the replacement kernel)
         const bool cg1 = a.variant == PCG_CG1;
         double rrt = partial_share(a.part, P_RRT, cg1 ? L - 2 : L - 1, wb, we);
         double gam = partial_share(a.part, P_RZT, L - 1, wb, we);
@@ -1547,7 +1547,8 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_update(PcgArgs a, int L, int xmod
 
 // Chunk end of the two-kernel form: consume and decide; in mixed mode, when
 // due, z = M^-1 (rhs - A (x64 + x32)) from the FP64 true residual, with
-// partials of its r.r and r.z for k_pcg_fin.
+// partials of its r.r and r.z that the scene's last workgroup to arrive sums
+// to finish the log (put_partial2_last).
 template <bool MIXED>
 __global__ __launch_bounds__(GF_WG) void k_pcg_replace(PcgArgs a, int L) {
     __shared__ double s_red[3 * GF_WG / GF_WAVE];
