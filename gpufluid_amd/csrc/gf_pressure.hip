@@ -690,6 +690,20 @@ __device__ __forceinline__ bool put_partial2_last(double* part, int ka, int kb, 
     return *s_last != 0;
 }
 
+// The end of a residual replacement from the scene's sums: the true residual
+// r.r, the new gamma (r.z) and delta (one-kernel form; zero in the two-kernel
+// form); the solve stops when the true residual meets the tolerance or the
+// iterations are spent.
+__device__ __forceinline__ void finish_replace(const PcgArgs& a, PcgLog& e, double rrt,
+                                               double gam, double del) {
+    e.rr_true = rrt; e.rr = rrt; e.rr_upd = rrt; e.rz = gam; e.pap = del;
+    e.upd = 0; e.merge = 1; e.nupd += 1;
+    if (rrt < a.tol2 * e.bb) { e.converged = 1; e.done = 1; }
+    else if (e.iters >= a.max_iter) { e.maxed = 1; e.done = 1; }
+    else e.done = 0;
+    e.pending = PEND_NONE;
+}
+
 // Consumes the last iteration's partials (when fresh): gamma = (r,u),
 // delta = (Au,u), r.r; counts the finished iteration and sets the stop /
 // replacement flags.  Identical in every workgroup of the scene.
@@ -868,8 +882,8 @@ __global__ __launch_bounds__(GF_WG) void k_cg1_setup2(PcgArgs a) {
     put_partial3(a.part, P_RZ, P_PAP, P_BB, 0, P.b, gam, del, rr, s_red);  // read by fin at L = 1
 }
 
-// Finalisation after the setup or a residual replacement of the one-kernel
-// form: one workgroup per scene (grid S).
+// Finalisation after the setup: one workgroup per scene (grid S).  (A
+// residual replacement finishes in its own last workgroup.)
 __global__ __launch_bounds__(GF_WG) void k_pcg_fin(PcgArgs a, int L) {
     __shared__ double s_red[3 * GF_WG / GF_WAVE];
     const int sc = blockIdx.x;
@@ -888,19 +902,6 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_fin(PcgArgs a, int L) {
         const int zero = bb < 1e-30;
         e.converged = zero;
         e.done = zero || a.max_iter <= 0;
-    } else if (e.pending == PEND_REPLACE) {
-        // one-kernel form: replace ran at L - 2, replace2 at L - 1 (the
-        // two-kernel form finishes its replacement in the replacement kernel)
-        const bool cg1 = a.variant == PCG_CG1;
-        double rrt = partial_share(a.part, P_RRT, cg1 ? L - 2 : L - 1, wb, we);
-        double gam = partial_share(a.part, P_RZT, L - 1, wb, we);
-        double del = cg1 ? partial_share(a.part, P_BB, L - 1, wb, we) : 0.0;
-        wg_sum3(rrt, gam, del, s_red);
-        e.rr_true = rrt; e.rr = rrt; e.rr_upd = rrt; e.rz = gam; e.pap = del;
-        e.upd = 0; e.merge = 1; e.nupd += 1;
-        if (rrt < a.tol2 * e.bb) { e.converged = 1; e.done = 1; }
-        else if (e.iters >= a.max_iter) { e.maxed = 1; e.done = 1; }
-        else e.done = 0;
     }
     e.pending = PEND_NONE;
     if (threadIdx.x == 0) {
@@ -1097,7 +1098,8 @@ __global__ __launch_bounds__(GF_WG) void k_cg1_replace(PcgArgs a, int L) {
 }
 
 // Pass 2 of a replacement (mixed mode; passes the log through otherwise):
-// w~ = M^-1 A u from the new u; partials of gamma and delta for k_pcg_fin.
+// w~ = M^-1 A u from the new u; partials of gamma and delta, which the
+// scene's last workgroup to arrive sums with pass 1's r.r to finish the log.
 __global__ __launch_bounds__(GF_WG) void k_cg1_replace2(PcgArgs a, int L) {
     __shared__ double s_red[3 * GF_WG / GF_WAVE];
     const Part P = part_of(a);
@@ -1128,8 +1130,18 @@ __global__ __launch_bounds__(GF_WG) void k_cg1_replace2(PcgArgs a, int L) {
         }
         st4(w + idx, wv);
     }
-    put_partial2(a.part, P_RZT, P_BB, L, P.b, gam, del, s_red);
-    write_log(a, P, e, L);
+    __shared__ int s_last;
+    if (!put_partial2_last(a.part, P_RZT, P_BB, L, P, gam, del, s_red, &s_last)) return;
+    // the scene's last workgroup: pass 1's true residual, gamma and delta
+    double rrt = partial_share_coh(a.part, P_RRT, L - 1, P.wb, P.we);
+    gam = partial_share_coh(a.part, P_RZT, L, P.wb, P.we);
+    del = partial_share_coh(a.part, P_BB, L, P.wb, P.we);
+    wg_sum3(rrt, gam, del, s_red);
+    finish_replace(a, e, rrt, gam, del);
+    if (threadIdx.x == 0) {
+        set_live_log(a, P.s, L, e);
+        if (a.hout) put_log(a.hout + 2 * P.s + (L & 1), e);
+    }
 }
 
 // ---- two-kernel (standard) PCG -----------------------------------------
@@ -1612,12 +1624,7 @@ __global__ __launch_bounds__(GF_WG) void k_pcg_replace(PcgArgs a, int L) {
     rz = partial_share_coh(a.part, P_RZT, L, P.wb, P.we);
     double del = 0.0;
     wg_sum3(rrt, rz, del, s_red);
-    e.rr_true = rrt; e.rr = rrt; e.rr_upd = rrt; e.rz = rz; e.pap = del;
-    e.upd = 0; e.merge = 1; e.nupd += 1;
-    if (rrt < a.tol2 * e.bb) { e.converged = 1; e.done = 1; }
-    else if (e.iters >= a.max_iter) { e.maxed = 1; e.done = 1; }
-    else e.done = 0;
-    e.pending = PEND_NONE;
+    finish_replace(a, e, rrt, rz, del);
     if (threadIdx.x == 0) {
         set_live_log(a, P.s, L, e);
         if (a.hout) put_log(a.hout + 2 * P.s + (L & 1), e);
@@ -1906,7 +1913,7 @@ int pcg_prepare(const gf_scene_t* s, int n_active, int precision, double tol, in
 // Kernels per iteration and launch numbers used by one chunk of `chunk` iterations.
 int pcg_kernels_per_iter(const PcgArgs& a) { return a.variant == PCG_CG1 ? 1 : 2; }
 int pcg_chunk_launches(const PcgArgs& a, int chunk, bool mixed) {
-    return a.variant == PCG_CG1 ? chunk + 1 + (mixed ? 2 : 0) : 2 * chunk + 1;
+    return a.variant == PCG_CG1 ? chunk + 1 + (mixed ? 1 : 0) : 2 * chunk + 1;
 }
 
 void pcg_launch_setup(const PcgArgs& a, bool mixed, hipStream_t st) {
@@ -1995,21 +2002,17 @@ int pcg_launch_chunk(const PcgArgs& a, bool mixed, int chunk, int& L, hipStream_
         if (mixed) {
             GF_TLAUNCH(k_cg1_replace<true>, e0(ic), nullptr, a, L);
             ++L;
-            GF_TLAUNCH(k_cg1_replace2, nullptr, e1(ic), a, L);
+            GF_TLAUNCH(k_cg1_replace2, nullptr, e1(ic), al, L);
         } else {
             GF_TLAUNCH(k_cg1_replace<false>, e0(ic), e1(ic), al, L);
         }
     } else {
-        // the mixed replacement finishes in its last workgroup (no k_pcg_fin)
         if (mixed)
             GF_TLAUNCH(k_pcg_replace<true>, e0(ic), e1(ic), al, L);
         else
             GF_TLAUNCH(k_pcg_replace<false>, e0(ic), e1(ic), al, L);
     }
-    if (mixed && a.variant == PCG_CG1) {
-        ++L;
-        hipLaunchKernelGGL(k_pcg_fin, dim3(a.S), block, 0, st, al, L);
-    }
+    // a mixed replacement finishes in its last workgroup (no k_pcg_fin)
 #undef GF_TLAUNCH
     return gf::check_launch("pcg iteration");
 }
